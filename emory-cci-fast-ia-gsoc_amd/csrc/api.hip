@@ -1,0 +1,395 @@
+// api.hip -- extern "C" entry points of libnscale_hip.so (declared in
+// include/nscale_hip.h).  Argument checks mirror the reference's CV_Assert
+// sites; failures return a status and leave a message in nsx_last_error().
+#include "nsx_internal.h"
+
+#include <climits>
+#include <cstring>
+#include <new>
+
+using namespace nsx;
+
+nsx_status nsx_ctx::ws(int slot, size_t bytes, void** out) {
+    if (slot < 0 || slot >= WS_NSLOTS) return fail(NSX_ERR_INVALID, "bad workspace slot");
+    if ((int)bufs.size() < WS_NSLOTS) bufs.resize(WS_NSLOTS);
+    DevBuf& b = bufs[slot];
+    if (bytes == 0) bytes = 1;
+    if (b.bytes >= bytes) { *out = b.ptr; return NSX_OK; }
+    // grow by 1/8 to avoid reallocating on slowly growing inputs
+    size_t want = bytes + bytes / 8;
+    want = (want + 255) & ~(size_t)255;
+    if (budget && held - b.bytes + want > budget) {
+        want = bytes;
+        if (held - b.bytes + want > budget)
+            return fail(NSX_ERR_BUDGET, "workspace request exceeds the context's HBM budget");
+    }
+    if (b.ptr) {
+        hipError_t e = hipStreamSynchronize(stream);
+        (void)e;
+        hipFree(b.ptr);
+        held -= b.bytes;
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(NSX_ERR_NOMEM, "hipMalloc of workspace failed");
+    }
+    b.ptr = p;
+    b.bytes = want;
+    held += want;
+    *out = p;
+    return NSX_OK;
+}
+
+namespace {
+
+const char kVersion[] = "nscale-hip 0.1 (gfx950)";
+
+hipStream_t pick_stream(nsx_ctx* ctx, void* stream) {
+    return stream ? (hipStream_t)stream : ctx->stream;
+}
+
+nsx_status check_img(nsx_ctx* ctx, const nsx_img* im, const char* name) {
+    if (!im) return ctx->fail(NSX_ERR_INVALID, std::string(name) + " is NULL");
+    if (im->rows < 0 || im->cols < 0) return ctx->fail(NSX_ERR_INVALID, std::string(name) + ": negative size");
+    if (im->dtype != NSX_U8 && im->dtype != NSX_I32 && im->dtype != NSX_F32)
+        return ctx->fail(NSX_ERR_INVALID, std::string(name) + ": unknown dtype");
+    if (im->mem != NSX_MEM_DEVICE && im->mem != NSX_MEM_HOST)
+        return ctx->fail(NSX_ERR_INVALID, std::string(name) + ": unknown memory kind");
+    if (im->rows > 0 && im->cols > 0) {
+        if (!im->data) return ctx->fail(NSX_ERR_INVALID, std::string(name) + ": data is NULL");
+        if (im->pitch_bytes < (int64_t)im->cols * (int64_t)dtype_size(im->dtype))
+            return ctx->fail(NSX_ERR_INVALID, std::string(name) + ": pitch smaller than a row");
+        if ((int64_t)im->rows * im->cols >= (int64_t)INT32_MAX)
+            return ctx->fail(NSX_ERR_INVALID, std::string(name) + ": more than 2^31-1 pixels");
+    }
+    return NSX_OK;
+}
+
+bool same_shape(const nsx_img* a, const nsx_img* b) { return a->rows == b->rows && a->cols == b->cols; }
+
+// Device view of an image: device images are used in place; host images are
+// staged through a workspace slot (copied in when `copy_in`).
+nsx_status to_device(nsx_ctx* ctx, const nsx_img* im, int slot, bool copy_in, hipStream_t s, View* v) {
+    v->rows = im->rows;
+    v->cols = im->cols;
+    v->dtype = im->dtype;
+    if (im->mem == NSX_MEM_DEVICE) {
+        v->data = (char*)im->data;
+        v->pitch = im->pitch_bytes;
+        return NSX_OK;
+    }
+    const size_t rowb = (size_t)im->cols * dtype_size(im->dtype);
+    const size_t pitch = (rowb + 255) & ~(size_t)255;
+    void* p;
+    NSX_TRY(ctx->ws(slot, pitch * (size_t)(im->rows > 0 ? im->rows : 1), &p));
+    v->data = (char*)p;
+    v->pitch = (int64_t)pitch;
+    if (copy_in && im->rows > 0 && rowb > 0)
+        NSX_HIP_TRY(ctx, hipMemcpy2DAsync(p, pitch, im->data, im->pitch_bytes, rowb, im->rows, hipMemcpyHostToDevice, s));
+    return NSX_OK;
+}
+
+nsx_status from_device(nsx_ctx* ctx, const View& v, const nsx_img* im, hipStream_t s) {
+    if (im->mem == NSX_MEM_DEVICE) return NSX_OK;
+    const size_t rowb = (size_t)im->cols * dtype_size(im->dtype);
+    if (im->rows > 0 && rowb > 0)
+        NSX_HIP_TRY(ctx, hipMemcpy2DAsync(im->data, im->pitch_bytes, v.data, v.pitch, rowb, im->rows, hipMemcpyDeviceToHost, s));
+    NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+    return NSX_OK;
+}
+
+bool overlaps(const nsx_img* a, const nsx_img* b) {
+    if (!a || !b || !a->data || !b->data || a->mem != b->mem) return false;
+    const char* a0 = (const char*)a->data;
+    const char* a1 = a0 + (a->rows > 0 ? (a->rows - 1) * a->pitch_bytes + (int64_t)a->cols * dtype_size(a->dtype) : 0);
+    const char* b0 = (const char*)b->data;
+    const char* b1 = b0 + (b->rows > 0 ? (b->rows - 1) * b->pitch_bytes + (int64_t)b->cols * dtype_size(b->dtype) : 0);
+    return a0 < b1 && b0 < a1;
+}
+
+nsx_status read_count(nsx_ctx* ctx, const int32_t* dev, int32_t* host, hipStream_t s) {
+    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 8, dev, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (host) *host = ctx->h_pinned[8];
+    return NSX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nsx_version(void) { return kVersion; }
+
+nsx_status nsx_ctx_create(nsx_ctx** out, int device, size_t hbm_budget_bytes) {
+    if (!out) return NSX_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return NSX_ERR_HIP;
+    }
+    if (device < 0 || device >= ndev) return NSX_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return NSX_ERR_HIP;
+    nsx_ctx* c = new (std::nothrow) nsx_ctx();
+    if (!c) return NSX_ERR_NOMEM;
+    c->device = device;
+    c->budget = hbm_budget_bytes;
+    c->bufs.resize(WS_NSLOTS);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        c->cu_count = cus;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void**)&c->d_stats, 256) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_pinned, 64 * sizeof(int32_t)) != hipSuccess) {
+        nsx_ctx_destroy(c);
+        return NSX_ERR_HIP;
+    }
+    hipMemset(c->d_stats, 0, 256);
+    c->held = 256;
+    *out = c;
+    return NSX_OK;
+}
+
+void nsx_ctx_destroy(nsx_ctx* ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (auto& b : ctx->bufs)
+        if (b.ptr) hipFree(b.ptr);
+    if (ctx->d_stats) hipFree(ctx->d_stats);
+    if (ctx->h_pinned) hipHostFree(ctx->h_pinned);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* nsx_last_error(const nsx_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+nsx_status nsx_ctx_synchronize(nsx_ctx* ctx) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return NSX_OK;
+}
+
+size_t nsx_ctx_workspace_bytes(const nsx_ctx* ctx) { return ctx ? ctx->held : 0; }
+
+nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int64_t stats[3]) {
+    if (!ctx || !stats) return NSX_ERR_INVALID;
+    int64_t h[4];
+    NSX_HIP_TRY(ctx, hipDeviceSynchronize());
+    NSX_HIP_TRY(ctx, hipMemcpy(h, ctx->d_stats, sizeof(h), hipMemcpyDeviceToHost));
+    stats[0] = h[0];
+    stats[1] = h[1];
+    stats[2] = h[2];
+    return NSX_OK;
+}
+
+nsx_status nsx_imreconstruct(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* mask, nsx_img* out, int conn,
+                             void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(check_img(ctx, marker, "marker"));
+    NSX_TRY(check_img(ctx, mask, "mask"));
+    NSX_TRY(check_img(ctx, out, "out"));
+    if (!same_shape(marker, mask) || !same_shape(mask, out))
+        return ctx->fail(NSX_ERR_INVALID, "marker, mask and out must have the same size");
+    if (marker->dtype != mask->dtype || out->dtype != mask->dtype)
+        return ctx->fail(NSX_ERR_INVALID, "marker, mask and out must have the same type");
+    if (overlaps(out, marker) || overlaps(out, mask)) return ctx->fail(NSX_ERR_INVALID, "out aliases an input");
+    hipStream_t s = pick_stream(ctx, stream);
+    View vm, vk, vo;
+    NSX_TRY(to_device(ctx, marker, WS_STAGE_A, true, s, &vm));
+    NSX_TRY(to_device(ctx, mask, WS_STAGE_B, true, s, &vk));
+    NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
+    NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vo, conn == 8 ? 8 : 4, s));
+    return from_device(ctx, vo, out, s);
+}
+
+nsx_status nsx_imreconstruct_binary(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* mask, nsx_img* out,
+                                    int conn, void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(check_img(ctx, marker, "marker"));
+    NSX_TRY(check_img(ctx, mask, "mask"));
+    NSX_TRY(check_img(ctx, out, "out"));
+    if (!same_shape(marker, mask) || !same_shape(mask, out))
+        return ctx->fail(NSX_ERR_INVALID, "marker, mask and out must have the same size");
+    if (marker->dtype != mask->dtype || out->dtype != mask->dtype)
+        return ctx->fail(NSX_ERR_INVALID, "marker, mask and out must have the same type");
+    if (mask->dtype == NSX_F32)
+        return ctx->fail(NSX_ERR_UNSUPPORTED, "imreconstructBinary is instantiated for uchar and int only");
+    if (overlaps(out, marker) || overlaps(out, mask)) return ctx->fail(NSX_ERR_INVALID, "out aliases an input");
+    hipStream_t s = pick_stream(ctx, stream);
+    View vm, vk, vo;
+    NSX_TRY(to_device(ctx, marker, WS_STAGE_A, true, s, &vm));
+    NSX_TRY(to_device(ctx, mask, WS_STAGE_B, true, s, &vk));
+    NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
+    NSX_TRY(recon_run(ctx, RM_BINARY, &vm, vk, vo, conn == 8 ? 8 : 4, s));
+    return from_device(ctx, vo, out, s);
+}
+
+nsx_status nsx_fill_holes(nsx_ctx* ctx, const nsx_img* in, nsx_img* out, int binary, int conn, void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(check_img(ctx, in, "in"));
+    NSX_TRY(check_img(ctx, out, "out"));
+    if (!same_shape(in, out) || in->dtype != out->dtype)
+        return ctx->fail(NSX_ERR_INVALID, "in and out must have the same size and type");
+    if (in->dtype == NSX_F32) return ctx->fail(NSX_ERR_UNSUPPORTED, "imfillHoles is instantiated for uchar and int only");
+    if (overlaps(out, in)) return ctx->fail(NSX_ERR_INVALID, "out aliases in");
+    hipStream_t s = pick_stream(ctx, stream);
+    View vi, vo;
+    NSX_TRY(to_device(ctx, in, WS_STAGE_B, true, s, &vi));
+    NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
+    if (binary && in->dtype == NSX_I32) {
+        // int marker interior is min<int> != 0, so imreconstructBinary keeps it
+        // everywhere and the inverted result is max<int> (MorphologicOperations.cpp:1889-1925).
+        NSX_TRY(fill_const_i32(ctx, vo, INT_MAX, s));
+    } else {
+        NSX_TRY(recon_run(ctx, binary ? RM_FILLBIN : RM_FILL, nullptr, vi, vo, conn == 8 ? 8 : 4, s));
+    }
+    return from_device(ctx, vo, out, s);
+}
+
+nsx_status nsx_bwlabel(nsx_ctx* ctx, const nsx_img* bin, nsx_img* labels, int conn, int relabel, int32_t* ncomp,
+                       void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(check_img(ctx, bin, "bin"));
+    NSX_TRY(check_img(ctx, labels, "labels"));
+    if (bin->dtype != NSX_U8) return ctx->fail(NSX_ERR_INVALID, "bwlabel2 requires CV_8U input");
+    if (labels->dtype != NSX_I32) return ctx->fail(NSX_ERR_INVALID, "labels must be int32");
+    if (!same_shape(bin, labels)) return ctx->fail(NSX_ERR_INVALID, "bin and labels must have the same size");
+    if (overlaps(bin, labels)) return ctx->fail(NSX_ERR_INVALID, "labels aliases bin");
+    hipStream_t s = pick_stream(ctx, stream);
+    View vi, vo;
+    NSX_TRY(to_device(ctx, bin, WS_STAGE_B, true, s, &vi));
+    NSX_TRY(to_device(ctx, labels, WS_STAGE_C, false, s, &vo));
+    CclParams p;
+    p.conn = conn == 8 ? 8 : 4;
+    p.out_mode = relabel ? CO_CANON : CO_ROOT;
+    int32_t* dcount = (int32_t*)ctx->d_stats + 12;
+    NSX_TRY(ccl_run(ctx, vi, vo, p, dcount, s));
+    NSX_TRY(from_device(ctx, vo, labels, s));
+    int32_t n = 0;
+    NSX_TRY(read_count(ctx, dcount, &n, s));
+    if (ncomp) *ncomp = relabel ? n : 0;
+    return NSX_OK;
+}
+
+nsx_status nsx_bwareaopen(nsx_ctx* ctx, const nsx_img* in, int labeled, int flatten, int min_size, int max_size,
+                          int conn, nsx_img* out, int32_t* count, void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(check_img(ctx, in, "in"));
+    NSX_TRY(check_img(ctx, out, "out"));
+    if (!labeled && in->dtype != NSX_U8) return ctx->fail(NSX_ERR_INVALID, "bwareaopen2 requires CV_8U input");
+    if (labeled && in->dtype != NSX_I32) return ctx->fail(NSX_ERR_INVALID, "labeled bwareaopen2 requires CV_32S input");
+    if (out->dtype != (flatten ? NSX_U8 : NSX_I32))
+        return ctx->fail(NSX_ERR_INVALID, flatten ? "flattened output must be u8" : "output must be int32");
+    if (!same_shape(in, out)) return ctx->fail(NSX_ERR_INVALID, "in and out must have the same size");
+    if (overlaps(in, out)) return ctx->fail(NSX_ERR_INVALID, "out aliases in");
+    hipStream_t s = pick_stream(ctx, stream);
+    View vi, vo;
+    NSX_TRY(to_device(ctx, in, WS_STAGE_B, true, s, &vi));
+    NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
+    int32_t* dcount = (int32_t*)ctx->d_stats + 12;
+    if (!labeled) {
+        CclParams p;
+        p.conn = conn == 8 ? 8 : 4;
+        p.out_mode = flatten ? CO_FLAT_U8 : CO_ROOT_FILTERED;
+        p.use_area = true;
+        p.min_size = min_size;
+        p.max_size = max_size;
+        NSX_TRY(ccl_run(ctx, vi, vo, p, dcount, s));
+    } else {
+        NSX_TRY(ccl_area_labeled(ctx, vi, vo, min_size, max_size, flatten != 0, dcount, s));
+    }
+    NSX_TRY(from_device(ctx, vo, out, s));
+    int32_t n = 0;
+    NSX_TRY(read_count(ctx, dcount, &n, s));
+    if (count) *count = n;
+    return NSX_OK;
+}
+
+nsx_status nsx_relabel(nsx_ctx* ctx, nsx_img* labels, int bgval, int32_t* ncomp, void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(check_img(ctx, labels, "labels"));
+    if (labels->dtype != NSX_I32) return ctx->fail(NSX_ERR_INVALID, "labels must be int32");
+    hipStream_t s = pick_stream(ctx, stream);
+    View v;
+    NSX_TRY(to_device(ctx, labels, WS_STAGE_C, true, s, &v));
+    int32_t* dcount = (int32_t*)ctx->d_stats + 12;
+    NSX_TRY(ccl_relabel(ctx, v, bgval, dcount, s));
+    NSX_TRY(from_device(ctx, v, labels, s));
+    int32_t n = 0;
+    NSX_TRY(read_count(ctx, dcount, &n, s));
+    if (ncomp) *ncomp = n;
+    return NSX_OK;
+}
+
+nsx_status nsx_threshold_gt(nsx_ctx* ctx, const nsx_img* in, double k, nsx_img* out, void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(check_img(ctx, in, "in"));
+    NSX_TRY(check_img(ctx, out, "out"));
+    if (out->dtype != NSX_U8) return ctx->fail(NSX_ERR_INVALID, "threshold output must be u8");
+    if (!same_shape(in, out)) return ctx->fail(NSX_ERR_INVALID, "in and out must have the same size");
+    hipStream_t s = pick_stream(ctx, stream);
+    View vi, vo;
+    NSX_TRY(to_device(ctx, in, WS_STAGE_B, true, s, &vi));
+    NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
+    NSX_TRY(threshold_run(ctx, vi, k, vo, s));
+    return from_device(ctx, vo, out, s);
+}
+
+nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* mask, int recon_conn, int fill_conn,
+                             double thresh, int ccl_conn, int min_size, int max_size, nsx_img* recon_out,
+                             nsx_img* fill_out, nsx_img* labels_out, int32_t* ncomp, void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(check_img(ctx, marker, "marker"));
+    NSX_TRY(check_img(ctx, mask, "mask"));
+    NSX_TRY(check_img(ctx, labels_out, "labels_out"));
+    if (!same_shape(marker, mask) || !same_shape(mask, labels_out))
+        return ctx->fail(NSX_ERR_INVALID, "marker, mask and labels must have the same size");
+    if (marker->dtype != mask->dtype) return ctx->fail(NSX_ERR_INVALID, "marker and mask must have the same type");
+    if (mask->dtype == NSX_F32) return ctx->fail(NSX_ERR_UNSUPPORTED, "imfillHoles is instantiated for uchar and int only");
+    if (labels_out->dtype != NSX_I32) return ctx->fail(NSX_ERR_INVALID, "labels must be int32");
+    for (const nsx_img* o : {recon_out, fill_out}) {
+        if (!o) continue;
+        NSX_TRY(check_img(ctx, o, "chain output"));
+        if (!same_shape(o, mask) || o->dtype != mask->dtype)
+            return ctx->fail(NSX_ERR_INVALID, "chain outputs must match the mask's size and type");
+    }
+    hipStream_t s = pick_stream(ctx, stream);
+    View vm, vk, vr, vf, vl;
+    NSX_TRY(to_device(ctx, marker, WS_STAGE_A, true, s, &vm));
+    NSX_TRY(to_device(ctx, mask, WS_STAGE_B, true, s, &vk));
+    const size_t es = dtype_size(mask->dtype);
+    auto inter = [&](nsx_img* o, int slot, View* v) -> nsx_status {
+        if (o && o->mem == NSX_MEM_DEVICE) return to_device(ctx, o, slot, false, s, v);
+        const size_t pitch = (((size_t)mask->cols * es) + 255) & ~(size_t)255;
+        void* p;
+        NSX_TRY(ctx->ws(slot, pitch * (size_t)(mask->rows > 0 ? mask->rows : 1), &p));
+        v->data = (char*)p; v->pitch = (int64_t)pitch; v->rows = mask->rows; v->cols = mask->cols; v->dtype = mask->dtype;
+        return NSX_OK;
+    };
+    NSX_TRY(inter(recon_out, WS_CHAIN_RECON, &vr));
+    NSX_TRY(inter(fill_out, WS_CHAIN_FILL, &vf));
+    NSX_TRY(to_device(ctx, labels_out, WS_STAGE_C, false, s, &vl));
+    NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vr, recon_conn == 8 ? 8 : 4, s));
+    NSX_TRY(recon_run(ctx, RM_FILL, nullptr, vr, vf, fill_conn == 8 ? 8 : 4, s));
+    CclParams p;
+    p.conn = ccl_conn == 8 ? 8 : 4;
+    p.out_mode = CO_CANON;
+    p.threshold = true;
+    p.thr = thresh;
+    p.use_area = true;
+    p.min_size = min_size;
+    p.max_size = max_size;
+    int32_t* dcount = (int32_t*)ctx->d_stats + 12;
+    NSX_TRY(ccl_run(ctx, vf, vl, p, dcount, s));
+    if (recon_out && recon_out->mem == NSX_MEM_HOST) NSX_TRY(from_device(ctx, vr, recon_out, s));
+    if (fill_out && fill_out->mem == NSX_MEM_HOST) NSX_TRY(from_device(ctx, vf, fill_out, s));
+    NSX_TRY(from_device(ctx, vl, labels_out, s));
+    if (ncomp) NSX_TRY(read_count(ctx, dcount, ncomp, s));
+    return NSX_OK;
+}
+
+}  // extern "C"

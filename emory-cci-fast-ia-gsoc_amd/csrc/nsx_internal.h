@@ -1,0 +1,139 @@
+// nsx_internal.h -- shared declarations of the gfx950 morphology / CCL library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+#include <mutex>
+
+#include "../../include/nscale_hip.h"
+
+namespace nsx {
+
+// ---- errors ------------------------------------------------------------------
+struct Error {
+    nsx_status code;
+    std::string msg;
+};
+
+#define NSX_HIP_TRY(ctx, expr)                                                            \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            return (ctx)->fail(NSX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+        }                                                                                \
+    } while (0)
+
+#define NSX_TRY(expr)                    \
+    do {                                 \
+        nsx_status s_ = (expr);          \
+        if (s_ != NSX_OK) return s_;     \
+    } while (0)
+
+// ---- device workspace ----------------------------------------------------------
+// A named, grow-only device buffer held by a context.  Every buffer counts
+// against the context's HBM budget.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace nsx
+
+struct nsx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t budget = 0;       // 0 = unlimited
+    size_t held = 0;         // bytes of workspace held
+    std::string last_error;
+    std::vector<nsx::DevBuf> bufs;   // indexed by workspace slot
+    int64_t* d_stats = nullptr;      // [rounds, tile passes, ntiles, spare]
+    int32_t* h_pinned = nullptr;     // small pinned readback area
+    int cu_count = 256;
+
+    nsx_status fail(nsx_status code, const std::string& msg) {
+        last_error = msg;
+        return code;
+    }
+    // Returns a device buffer of at least `bytes` in workspace slot `slot`.
+    nsx_status ws(int slot, size_t bytes, void** out);
+};
+
+namespace nsx {
+
+// Workspace slots.
+enum WsSlot {
+    WS_RECON_STAMP = 0,  // per-tile queue stamps
+    WS_RECON_LIST,       // 2 tile lists
+    WS_RECON_COUNTS,     // 3 round counters (+ pad)
+    WS_CCL_LABEL,        // int32 per pixel (when the caller's output cannot hold it)
+    WS_CCL_AREA,         // int32 per pixel (roots only are touched)
+    WS_CCL_ROOTS,        // compact local-root table: (global index, count) pairs
+    WS_CCL_BITS,         // kept-root bitmap
+    WS_CCL_SCAN,         // per-word prefix + block sums
+    WS_CCL_MISC,         // counters
+    WS_CHAIN_RECON,      // chain intermediates
+    WS_CHAIN_FILL,
+    WS_STAGE_A,          // host<->device staging
+    WS_STAGE_B,
+    WS_STAGE_C,
+    WS_NSLOTS
+};
+
+// ---- image descriptors ------------------------------------------------------------
+struct View {
+    char* data;
+    int64_t pitch;  // bytes
+    int rows, cols;
+    int dtype;
+};
+
+inline size_t dtype_size(int dt) { return dt == NSX_U8 ? 1 : 4; }
+
+// ---- reconstruction ---------------------------------------------------------------
+enum ReconMode { RM_RECON = 0, RM_BINARY = 1, RM_FILL = 2, RM_FILLBIN = 3 };
+
+// Runs reconstruction in `mode` on device views (marker may be null for the
+// fill modes).  `state` is the output buffer and holds the propagation state.
+nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& state,
+                     int conn, hipStream_t s);
+
+// ---- connected components ---------------------------------------------------------
+enum CclOut {
+    CO_ROOT = 0,          // int32: root raster index, background -1
+    CO_ROOT_FILTERED = 1, // int32: root index if kept by the area filter, else -1
+    CO_CANON = 2,         // int32: 1..N in raster order of kept roots, background 0
+    CO_FLAT_U8 = 3        // u8: 255 if kept, else 0
+};
+
+struct CclParams {
+    int conn = 8;
+    int out_mode = CO_CANON;
+    bool threshold = false;   // input is thresholded on the fly: fg = in > thr
+    double thr = 0.0;
+    int min_size = 0;         // area filter (inclusive) ; filter on when use_area
+    int max_size = 0;         // (exclusive)
+    bool use_area = false;
+};
+
+// Labels the image `in` (u8; or u8/i32/f32 with threshold) into `out`.
+// `ncomp_dev` (may be null) receives the kept component count on the device.
+nsx_status ccl_run(nsx_ctx* ctx, const View& in, const View& out, const CclParams& p,
+                   int32_t* ncomp_dev, hipStream_t s);
+
+// Area filter on int32 root labels (background -1), out int32 or u8.
+nsx_status ccl_area_labeled(nsx_ctx* ctx, const View& in, const View& out, int min_size, int max_size,
+                            bool flatten, int32_t* count_dev, hipStream_t s);
+
+// Canonical relabel of int32 labels (roots are label[i]==i), in place.
+nsx_status ccl_relabel(nsx_ctx* ctx, const View& lab, int bgval, int32_t* ncomp_dev, hipStream_t s);
+
+// Pointwise `in > k` -> u8 255/0.
+nsx_status threshold_run(nsx_ctx* ctx, const View& in, double k, const View& out, hipStream_t s);
+
+// Pointwise i32 fill of a constant.
+nsx_status fill_const_i32(nsx_ctx* ctx, const View& out, int32_t v, hipStream_t s);
+
+}  // namespace nsx

@@ -1,0 +1,423 @@
+// recon.hip -- morphological reconstruction by dilation on gfx950.
+//
+// Replaces the reference's CPU hybrid algorithm (MorphologicOperations.cpp:
+// 665-838 imreconstruct, 1688-1829 imreconstructBinary, 1859-1926 imfillHoles)
+// and its GPU scan + queue variants (imreconstruct_int_kernel.cu:589-712,
+// global_queue.cu:746-857).  The result is the unique fixed point
+//     R = min(max(R, dilate_conn(R)), mask),   R >= min(marker, mask),
+// with a constant frame around the image, so any propagation order that reaches
+// the fixed point is bit-exact with the reference's sequential order.
+//
+// Design (one wave64 per 64x64 register tile):
+//   * The image is cut into 62x62 tiles; each wave loads its tile plus a 1-px
+//     halo (64x64) into VGPRs, lane = column, register = row, as 32-bit order-
+//     preserving keys (u8 zero-extended, i32 sign-flipped, f32 ordered-int).
+//   * Pixels outside the image take the frame value with mask == value, so they
+//     never change: the frame needs no special case in the sweeps.
+//   * Propagation inside the tile runs to the tile's fixed point by repeated
+//     sweeps: a downward and an upward sweep (each row takes the clamped max of
+//     its own value and the previous row's 1- or 3-neighbourhood, the latter
+//     through DPP wave_shr/wave_shl), then the tile is transposed through LDS
+//     and the same two sweeps run along the other axis.  v_med3_u32 does the
+//     clamp: new = med3(neighbour max, cur, mask) since cur <= mask.
+//   * Halo pixels are computed with the tile (their raised values are valid
+//     lower bounds) but never written; when one rises, the tile that owns it is
+//     queued for the next round.  Interior pixels are written back.
+//   * Rounds repeat over a device-side list of queued tiles (atomic append with
+//     a per-tile stamp so a tile is queued at most once per round) until the
+//     list is empty.  The list count lives on the device; the host only reads it
+//     every few rounds to stop launching.
+#include "nsx_internal.h"
+
+#include <climits>
+#include <cfloat>
+#include <algorithm>
+
+namespace nsx {
+namespace {
+
+constexpr int TS = 64;          // register tile side, halo included
+constexpr int TI = TS - 2;      // interior side written back
+constexpr int LDS_PITCH = TS + 1;
+constexpr int MAX_IT = 64;      // sweep iterations per visit before re-queueing itself
+constexpr int MAX_ROUNDS = 1 << 22;
+
+// ---- order-preserving 32-bit keys -------------------------------------------------
+template <typename T> struct Key;
+template <> struct Key<uint8_t> {
+    __device__ static uint32_t to(uint8_t v) { return v; }
+    __device__ static uint8_t from(uint32_t k) { return (uint8_t)k; }
+};
+template <> struct Key<int32_t> {
+    __device__ static uint32_t to(int32_t v) { return (uint32_t)v ^ 0x80000000u; }
+    __device__ static int32_t from(uint32_t k) { return (int32_t)(k ^ 0x80000000u); }
+};
+template <> struct Key<float> {
+    // Compare-select order of finite floats; -0.0 sorts just below +0.0.
+    __device__ static uint32_t to(float v) {
+        uint32_t b = __float_as_uint(v);
+        return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    }
+    __device__ static float from(uint32_t k) {
+        uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+        return __uint_as_float(b);
+    }
+};
+
+// PixelOperations::invert<T> (PixelOperations.cpp:17-39): 255-x, saturating
+// two's-complement negation, -x.
+__device__ inline uint8_t inv(uint8_t v) { return (uint8_t)(255 - v); }
+__device__ inline int32_t inv(int32_t v) { return v == INT_MIN ? INT_MAX : -v; }
+__device__ inline float inv(float v) { return -v; }
+
+template <typename T> __device__ inline T tmax();
+template <> __device__ inline uint8_t tmax<uint8_t>() { return 255; }
+template <> __device__ inline int32_t tmax<int32_t>() { return INT_MAX; }
+template <> __device__ inline float tmax<float>() { return FLT_MAX; }
+// cci::common::type::min<T> (TypeUtils.h:33-41)
+template <typename T> __device__ inline T tmin();
+template <> __device__ inline uint8_t tmin<uint8_t>() { return 0; }
+template <> __device__ inline int32_t tmin<int32_t>() { return INT_MIN; }
+template <> __device__ inline float tmin<float>() { return -FLT_MAX; }
+
+// ---- per-mode codecs ------------------------------------------------------------
+// frame(): key of the constant ring around the image.
+// mkey(a, m): mask key.   first(a, m): initial key.
+// dec(s, a, m): key of the stored state.   enc(k, a): stored value of a key.
+template <typename T, int MODE> struct Codec;
+
+// imreconstruct: state = R; frame 0 (MorphologicOperations.cpp:671-674).
+template <typename T> struct Codec<T, RM_RECON> {
+    static constexpr bool kMarkerFirst = true, kMarkerLater = false;
+    __device__ static uint32_t frame() { return Key<T>::to(T(0)); }
+    __device__ static uint32_t mkey(T, T m) { return Key<T>::to(m); }
+    __device__ static uint32_t first(T a, T m) { return min(Key<T>::to(a), Key<T>::to(m)); }
+    __device__ static uint32_t dec(T s, T, T) { return Key<T>::to(s); }
+    __device__ static T enc(uint32_t k, T) { return Key<T>::from(k); }
+};
+
+// imreconstructBinary: key = reached (0/1) in the set mask != 0, seeded by
+// marker != 0 && mask != 0; nonzero marker pixels keep their value
+// (MorphologicOperations.cpp:1665-1672,1712-1756).
+template <typename T> struct Codec<T, RM_BINARY> {
+    static constexpr bool kMarkerFirst = true, kMarkerLater = true;
+    __device__ static uint32_t frame() { return 0u; }
+    __device__ static uint32_t mkey(T, T m) { return m != T(0) ? 1u : 0u; }
+    __device__ static uint32_t first(T a, T m) { return (a != T(0) && m != T(0)) ? 1u : 0u; }
+    __device__ static uint32_t dec(T s, T a, T m) {
+        return a != T(0) ? (m != T(0) ? 1u : 0u) : (s != T(0) ? 1u : 0u);
+    }
+    __device__ static T enc(uint32_t k, T a) { return a != T(0) ? a : (k ? tmax<T>() : T(0)); }
+};
+
+// imfillHoles, grayscale: R = reconstruction of marker (min<T> inside, max<T>
+// on the frame) under invert(pad(image, min<T>)); output invert(R)
+// (MorphologicOperations.cpp:1889-1925).  The frame ring holds
+// min(max<T>, invert(min<T>)) = max<T>.  State stored = invert(R) = output.
+template <typename T> struct Codec<T, RM_FILL> {
+    static constexpr bool kMarkerFirst = false, kMarkerLater = false;
+    __device__ static uint32_t frame() { return Key<T>::to(tmax<T>()); }
+    __device__ static uint32_t mkey(T, T m) { return Key<T>::to(inv(m)); }
+    __device__ static uint32_t first(T, T) { return Key<T>::to(tmin<T>()); }
+    __device__ static uint32_t dec(T s, T, T) { return Key<T>::to(inv(s)); }
+    __device__ static T enc(uint32_t k, T) { return inv(Key<T>::from(k)); }
+};
+
+// imfillHoles, binary, u8: reached set of invert(image) != 0 (image != 255)
+// from the frame; output invert(reached ? 255 : 0).
+template <typename T> struct Codec<T, RM_FILLBIN> {
+    static constexpr bool kMarkerFirst = false, kMarkerLater = false;
+    __device__ static uint32_t frame() { return 1u; }
+    __device__ static uint32_t mkey(T, T m) { return inv(m) != T(0) ? 1u : 0u; }
+    __device__ static uint32_t first(T, T) { return 0u; }
+    __device__ static uint32_t dec(T s, T, T) { return s == inv(tmax<T>()) ? 1u : 0u; }
+    __device__ static T enc(uint32_t k, T) { return k ? inv(tmax<T>()) : inv(T(0)); }
+};
+
+struct ReconArgs {
+    const char* marker; int64_t pm;
+    const char* mask;   int64_t pk;
+    char* state;        int64_t ps;
+    int rows, cols, ntx, nty, ntiles;
+    uint32_t* stamp;
+    int32_t* list0;
+    int32_t* list1;
+    int32_t* counts;    // [3] rotating round counters
+    int64_t* stats;     // [0] rounds with work, [1] tile visits, [2] tiles
+};
+
+template <typename T>
+__device__ __forceinline__ T ld(const char* base, int64_t pitch, int y, int x) {
+    return *reinterpret_cast<const T*>(base + (int64_t)y * pitch + (int64_t)x * (int64_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st(char* base, int64_t pitch, int y, int x, T v) {
+    *reinterpret_cast<T*>(base + (int64_t)y * pitch + (int64_t)x * (int64_t)sizeof(T)) = v;
+}
+
+// lane i <- lane i-1 (lane 0 <- 0), lane i <- lane i+1 (lane 63 <- 0); 0 is the
+// identity of max over keys.
+__device__ __forceinline__ uint32_t from_left(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, true);  // wave_shr:1
+}
+__device__ __forceinline__ uint32_t from_right(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xF, 0xF, true);  // wave_shl:1
+}
+
+__device__ __forceinline__ uint32_t med3u(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+template <int CONN>
+__device__ __forceinline__ uint32_t nbr(uint32_t p) {
+    if constexpr (CONN == 8) return max(p, max(from_left(p), from_right(p)));
+    else return p;
+}
+
+// Forward sweep over registers 1..TS-1 then backward over TS-2..0.
+template <int CONN>
+__device__ __forceinline__ void sweep_pair(uint32_t (&c)[TS], const uint32_t (&m)[TS]) {
+    uint32_t p = c[0];
+#pragma unroll
+    for (int k = 1; k < TS; ++k) {
+        c[k] = med3u(nbr<CONN>(p), c[k], m[k]);
+        p = c[k];
+    }
+    p = c[TS - 1];
+#pragma unroll
+    for (int k = TS - 2; k >= 0; --k) {
+        c[k] = med3u(nbr<CONN>(p), c[k], m[k]);
+        p = c[k];
+    }
+}
+
+// Register tile transpose through LDS: lane l, register k <-> lane k, register l.
+__device__ __forceinline__ void transpose(uint32_t (&c)[TS], uint32_t* lds) {
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TS; ++k) lds[k * LDS_PITCH + lane] = c[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TS; ++k) c[k] = lds[lane * LDS_PITCH + k];
+    __syncthreads();
+}
+
+template <typename T> struct SumT { typedef uint64_t type; };
+template <> struct SumT<uint8_t> { typedef uint32_t type; };
+
+template <typename S>
+__device__ __forceinline__ S wave_sum(S v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename S>
+__device__ __forceinline__ S tile_sum(const uint32_t (&c)[TS]) {
+    S s = 0;
+#pragma unroll
+    for (int k = 0; k < TS; ++k) s += (S)c[k];
+    return wave_sum<S>(s);
+}
+
+__device__ __forceinline__ void activate(const ReconArgs& A, int ty, int tx, int round) {
+    if (ty < 0 || tx < 0 || ty >= A.nty || tx >= A.ntx) return;
+    const int t = ty * A.ntx + tx;
+    const uint32_t want = (uint32_t)round + 2u;   // "queued for round+1"
+    if (atomicMax(&A.stamp[t], want) < want) {
+        const int idx = atomicAdd(&A.counts[(round + 1) % 3], 1);
+        int32_t* nl = ((round + 1) & 1) ? A.list1 : A.list0;
+        nl[idx] = t;
+    }
+}
+
+template <typename T, int MODE, int CONN, bool FIRST>
+__device__ __forceinline__ void process_tile(const ReconArgs& A, int t, int round, uint32_t* lds) {
+    using C = Codec<T, MODE>;
+    using S = typename SumT<T>::type;
+    const int lane = threadIdx.x;
+    const int ty = t / A.ntx, tx = t - ty * A.ntx;
+    const int x0 = tx * TI - 1, y0 = ty * TI - 1;
+    const uint32_t F = C::frame();
+    const bool needA = FIRST ? C::kMarkerFirst : C::kMarkerLater;
+
+    auto key_at = [&](int gy, int gx, uint32_t& mk) -> uint32_t {
+        if (gx < 0 || gx >= A.cols || gy < 0 || gy >= A.rows) { mk = F; return F; }
+        const T mv = ld<T>(A.mask, A.pk, gy, gx);
+        const T av = needA ? ld<T>(A.marker, A.pm, gy, gx) : T(0);
+        mk = C::mkey(av, mv);
+        if (FIRST) return C::first(av, mv);
+        return C::dec(ld<T>(A.state, A.ps, gy, gx), av, mv);
+    };
+
+    uint32_t c[TS], m[TS];
+    const int gx = x0 + lane;
+#pragma unroll
+    for (int k = 0; k < TS; ++k) c[k] = key_at(y0 + k, gx, m[k]);
+    uint32_t dummy;
+    const uint32_t initL = key_at(y0 + lane, x0, dummy);
+    const uint32_t initR = key_at(y0 + lane, x0 + TS - 1, dummy);
+    const uint32_t initT = c[0], initB = c[TS - 1];
+
+    // transposed mask for the horizontal sweeps
+    uint32_t mt[TS];
+#pragma unroll
+    for (int k = 0; k < TS; ++k) mt[k] = m[k];
+    transpose(mt, lds);
+
+    const S tot_in = tile_sum<S>(c);
+    S tot = tot_in;
+    bool converged = false;
+    for (int it = 0; it < MAX_IT; ++it) {
+        sweep_pair<CONN>(c, m);        // along rows' column direction (y)
+        transpose(c, lds);
+        sweep_pair<CONN>(c, mt);       // along x
+        const S t1 = tile_sum<S>(c);
+        if (t1 == tot) { converged = true; break; }
+        tot = t1;
+        if (it + 1 == MAX_IT) break;
+        transpose(c, lds);
+    }
+    // c is in transposed orientation here: register 0 / TS-1 = left / right halo column.
+    const uint32_t finL = c[0], finR = c[TS - 1];
+    transpose(c, lds);
+
+    if (FIRST || tot != tot_in) {
+        if (lane >= 1 && lane <= TI && gx < A.cols) {
+#pragma unroll
+            for (int k = 1; k <= TI; ++k) {
+                const int gy = y0 + k;
+                if (gy < A.rows) {
+                    const T av = (MODE == RM_BINARY) ? ld<T>(A.marker, A.pm, gy, gx) : T(0);
+                    st<T>(A.state, A.ps, gy, gx, C::enc(c[k], av));
+                }
+            }
+        }
+    }
+
+    const uint64_t topm = __ballot(c[0] != initT);
+    const uint64_t botm = __ballot(c[TS - 1] != initB);
+    const uint64_t lefm = __ballot(finL != initL);
+    const uint64_t rigm = __ballot(finR != initR);
+    if (lane == 0) {
+        const uint64_t mid = 0x7FFFFFFFFFFFFFFEull;
+        if (topm & mid) activate(A, ty - 1, tx, round);
+        if (topm & 1ull) activate(A, ty - 1, tx - 1, round);
+        if (topm >> 63) activate(A, ty - 1, tx + 1, round);
+        if (botm & mid) activate(A, ty + 1, tx, round);
+        if (botm & 1ull) activate(A, ty + 1, tx - 1, round);
+        if (botm >> 63) activate(A, ty + 1, tx + 1, round);
+        if (lefm & mid) activate(A, ty, tx - 1, round);
+        if (rigm & mid) activate(A, ty, tx + 1, round);
+        if (!converged) activate(A, ty, tx, round);
+    }
+}
+
+template <typename T, int MODE, int CONN, bool FIRST>
+__global__ __launch_bounds__(64) void recon_round_kernel(ReconArgs A, int round) {
+    __shared__ uint32_t lds[TS * LDS_PITCH];
+    int n;
+    const int32_t* list = (round & 1) ? A.list1 : A.list0;
+    if (FIRST) n = A.ntiles;
+    else n = __builtin_nontemporal_load(&A.counts[round % 3]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        A.counts[(round + 2) % 3] = 0;
+        if (n > 0) {
+            A.stats[0] += 1;
+            A.stats[1] += n;
+        }
+    }
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int t = FIRST ? i : list[i];
+        process_tile<T, MODE, CONN, FIRST>(A, t, round, lds);
+    }
+}
+
+__global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts, int64_t* stats) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int j = i; j < ntiles; j += gridDim.x * blockDim.x) stamp[j] = 0u;
+    if (i < 3) counts[i] = 0;
+    if (i == 0) { stats[0] = 0; stats[1] = 0; stats[2] = ntiles; }
+}
+
+template <typename T, int MODE, int CONN>
+nsx_status run_typed(nsx_ctx* ctx, ReconArgs A, hipStream_t s) {
+    const int grid_reset = std::min(1024, (A.ntiles + 255) / 256);
+    hipLaunchKernelGGL(recon_reset_kernel, dim3(grid_reset), dim3(256), 0, s, A.stamp, A.ntiles, A.counts, A.stats);
+    hipLaunchKernelGGL((recon_round_kernel<T, MODE, CONN, true>), dim3(A.ntiles), dim3(64), 0, s, A, 0);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    const int grid = std::max(1, std::min(A.ntiles, 8 * ctx->cu_count));
+    int round = 1;
+    int batch = 2;
+    while (true) {
+        for (int j = 0; j < batch; ++j, ++round)
+            hipLaunchKernelGGL((recon_round_kernel<T, MODE, CONN, false>), dim3(grid), dim3(64), 0, s, A, round);
+        NSX_HIP_TRY(ctx, hipGetLastError());
+        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, &A.counts[round % 3], sizeof(int32_t),
+                                        hipMemcpyDeviceToHost, s));
+        NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+        if (ctx->h_pinned[0] == 0) break;
+        if (round > MAX_ROUNDS) return ctx->fail(NSX_ERR_NOT_CONVERGED, "reconstruction did not converge");
+        batch = std::min(batch * 2, 64);
+    }
+    return NSX_OK;
+}
+
+template <typename T, int MODE>
+nsx_status run_conn(nsx_ctx* ctx, const ReconArgs& A, int conn, hipStream_t s) {
+    if (conn == 8) return run_typed<T, MODE, 8>(ctx, A, s);
+    return run_typed<T, MODE, 4>(ctx, A, s);
+}
+
+}  // namespace
+
+nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& state,
+                     int conn, hipStream_t s) {
+    ReconArgs A{};
+    A.marker = marker ? marker->data : nullptr;
+    A.pm = marker ? marker->pitch : 0;
+    A.mask = mask.data;
+    A.pk = mask.pitch;
+    A.state = state.data;
+    A.ps = state.pitch;
+    A.rows = mask.rows;
+    A.cols = mask.cols;
+    A.ntx = (A.cols + TI - 1) / TI;
+    A.nty = (A.rows + TI - 1) / TI;
+    A.ntiles = A.ntx * A.nty;
+    if (A.rows <= 0 || A.cols <= 0) return NSX_OK;
+    void *stamp, *lists, *counts;
+    NSX_TRY(ctx->ws(WS_RECON_STAMP, sizeof(uint32_t) * (size_t)A.ntiles, &stamp));
+    NSX_TRY(ctx->ws(WS_RECON_LIST, 2 * sizeof(int32_t) * (size_t)A.ntiles, &lists));
+    NSX_TRY(ctx->ws(WS_RECON_COUNTS, 64, &counts));
+    A.stamp = (uint32_t*)stamp;
+    A.list0 = (int32_t*)lists;
+    A.list1 = (int32_t*)lists + A.ntiles;
+    A.counts = (int32_t*)counts;
+    A.stats = ctx->d_stats;
+    const int dt = mask.dtype;
+    switch (mode) {
+        case RM_RECON:
+            if (dt == NSX_U8) return run_conn<uint8_t, RM_RECON>(ctx, A, conn, s);
+            if (dt == NSX_I32) return run_conn<int32_t, RM_RECON>(ctx, A, conn, s);
+            return run_conn<float, RM_RECON>(ctx, A, conn, s);
+        case RM_BINARY:
+            if (dt == NSX_U8) return run_conn<uint8_t, RM_BINARY>(ctx, A, conn, s);
+            if (dt == NSX_I32) return run_conn<int32_t, RM_BINARY>(ctx, A, conn, s);
+            break;
+        case RM_FILL:
+            if (dt == NSX_U8) return run_conn<uint8_t, RM_FILL>(ctx, A, conn, s);
+            if (dt == NSX_I32) return run_conn<int32_t, RM_FILL>(ctx, A, conn, s);
+            break;
+        case RM_FILLBIN:
+            if (dt == NSX_U8) return run_conn<uint8_t, RM_FILLBIN>(ctx, A, conn, s);
+            break;
+        default:
+            break;
+    }
+    return ctx->fail(NSX_ERR_UNSUPPORTED, "reconstruction mode/dtype not instantiated by the reference");
+}
+
+}  // namespace nsx

@@ -1,0 +1,309 @@
+"""nscale_hip -- Python mirror of the nscale:: morphology / CCL API over the
+gfx950 C ABI (include/nscale_hip.h).
+
+Function names, argument meaning and error behaviour follow the reference's
+C++ API (src/segment/MorphologicOperations.h):
+
+    imreconstruct(seeds, image, connectivity)          (:21-22)
+    imreconstructBinary(seeds, binaryImage, connectivity)  (:43-44)
+    imfillHoles(image, binary, connectivity)            (:50-51)
+    bwlabel2(binaryImage, connectivity, relab)          (:59)
+    bwareaopen2(image, labeled, flatten, minSize, maxSize, connectivity) -> (out, count)   (:74)
+
+Inputs are numpy arrays (host; copied to the GPU and back, like the
+reference's synchronous CPU calls) or torch tensors on a ROCm device (used in
+place, outputs allocated on the same device).  Failures raise NsxError, the
+counterpart of the reference's cv::Exception from CV_Assert.  There is no CPU
+fallback: every call runs the HIP kernels of libnscale_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+
+import numpy as np
+
+from . import _lib
+from ._lib import NsxImg
+
+__all__ = [
+    "NsxError", "Context", "default_context",
+    "imreconstruct", "imreconstructBinary", "imfillHoles", "fillHoles",
+    "bwlabel2", "bwlabel", "bwareaopen2", "relabel", "threshold_gt", "segment_chain",
+    "gen_disks_u8", "gen_marker_u8", "gen_maze_u8", "scale_i32", "scale_f32",
+]
+
+_DT = {np.dtype(np.uint8): _lib.NSX_U8, np.dtype(np.int32): _lib.NSX_I32, np.dtype(np.float32): _lib.NSX_F32}
+
+
+class NsxError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"nsx status {status}: {msg}")
+        self.status = status
+
+
+def _torch():
+    try:
+        import torch  # noqa: F401
+        return torch
+    except Exception:  # pragma: no cover
+        return None
+
+
+def _is_device_tensor(x) -> bool:
+    t = _torch()
+    return t is not None and isinstance(x, t.Tensor) and x.is_cuda
+
+
+def _torch_dt(x) -> int:
+    t = _torch()
+    m = {t.uint8: _lib.NSX_U8, t.int32: _lib.NSX_I32, t.float32: _lib.NSX_F32}
+    if x.dtype not in m:
+        raise NsxError(_lib.NSX_ERR_INVALID, f"unsupported dtype {x.dtype}")
+    return m[x.dtype]
+
+
+def describe(x) -> NsxImg:
+    """nsx_img of a 2-D numpy array or device tensor (rows, cols, pitch)."""
+    if _is_device_tensor(x):
+        if x.dim() != 2 or x.stride(1) != 1:
+            raise NsxError(_lib.NSX_ERR_INVALID, "device image must be 2-D with unit column stride")
+        return NsxImg(x.data_ptr(), x.stride(0) * x.element_size(), x.shape[0], x.shape[1], _torch_dt(x),
+                      _lib.NSX_MEM_DEVICE)
+    a = x
+    if not isinstance(a, np.ndarray) or a.ndim != 2:
+        raise NsxError(_lib.NSX_ERR_INVALID, "image must be a 2-D numpy array or device tensor (one channel)")
+    if a.dtype not in _DT:
+        raise NsxError(_lib.NSX_ERR_INVALID, f"unsupported dtype {a.dtype}")
+    if a.strides[1] != a.itemsize:
+        raise NsxError(_lib.NSX_ERR_INVALID, "host image must have unit column stride")
+    return NsxImg(a.ctypes.data, a.strides[0], a.shape[0], a.shape[1], _DT[a.dtype], _lib.NSX_MEM_HOST)
+
+
+def _like(x, dtype_np):
+    if _is_device_tensor(x):
+        t = _torch()
+        tdt = {np.uint8: t.uint8, np.int32: t.int32, np.float32: t.float32}[dtype_np]
+        return t.empty(tuple(x.shape), dtype=tdt, device=x.device)
+    return np.empty(x.shape, dtype=dtype_np)
+
+
+def _np_dtype(x):
+    if _is_device_tensor(x):
+        t = _torch()
+        return {t.uint8: np.uint8, t.int32: np.int32, t.float32: np.float32}[x.dtype]
+    return x.dtype.type
+
+
+class Context:
+    """Owns an nsx_ctx (device workspace + HBM budget).  One per thread/stream."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        st = self.lib.nsx_ctx_create(C.byref(h), device, hbm_budget_bytes)
+        if st != _lib.NSX_OK:
+            raise NsxError(st, f"nsx_ctx_create(device={device}) failed")
+        self.h = h
+        self.device = device
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.nsx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, st: int) -> None:
+        if st != _lib.NSX_OK:
+            msg = self.lib.nsx_last_error(self.h)
+            raise NsxError(st, msg.decode() if msg else "")
+
+    def synchronize(self) -> None:
+        self.check(self.lib.nsx_ctx_synchronize(self.h))
+
+    def workspace_bytes(self) -> int:
+        return int(self.lib.nsx_ctx_workspace_bytes(self.h))
+
+    def recon_stats(self) -> dict:
+        s = (C.c_int64 * 3)()
+        self.check(self.lib.nsx_last_recon_stats(self.h, s))
+        return {"rounds": int(s[0]), "tile_visits": int(s[1]), "tiles": int(s[2])}
+
+
+_tls = threading.local()
+
+
+def default_context(device: int = 0) -> Context:
+    ctxs = getattr(_tls, "ctxs", None)
+    if ctxs is None:
+        ctxs = _tls.ctxs = {}
+    if device not in ctxs:
+        ctxs[device] = Context(device)
+    return ctxs[device]
+
+
+def _ctx(x, ctx):
+    if ctx is not None:
+        return ctx
+    dev = x.device.index if _is_device_tensor(x) and x.device.index is not None else 0
+    return default_context(dev)
+
+
+def _stream_arg(stream):
+    if stream is None:
+        return None
+    return C.c_void_p(getattr(stream, "cuda_stream", stream))
+
+
+def _torch_stream(x, stream):
+    """Device tensors run on torch's current stream unless a stream is given."""
+    if stream is not None:
+        return _stream_arg(stream)
+    if _is_device_tensor(x):
+        return C.c_void_p(_torch().cuda.current_stream(x.device).cuda_stream)
+    return None
+
+
+def imreconstruct(seeds, image, connectivity: int = 4, *, out=None, ctx=None, stream=None):
+    """Grayscale reconstruction by dilation (nscale::imreconstruct<T>)."""
+    c = _ctx(image, ctx)
+    out = _like(image, _np_dtype(image)) if out is None else out
+    c.check(c.lib.nsx_imreconstruct(c.h, C.byref(describe(seeds)), C.byref(describe(image)),
+                                    C.byref(describe(out)), connectivity, _torch_stream(image, stream)))
+    return out
+
+
+def imreconstructBinary(seeds, binaryImage, connectivity: int = 4, *, out=None, ctx=None, stream=None):
+    """Binary reconstruction (nscale::imreconstructBinary<T>)."""
+    c = _ctx(binaryImage, ctx)
+    out = _like(binaryImage, _np_dtype(binaryImage)) if out is None else out
+    c.check(c.lib.nsx_imreconstruct_binary(c.h, C.byref(describe(seeds)), C.byref(describe(binaryImage)),
+                                           C.byref(describe(out)), connectivity,
+                                           _torch_stream(binaryImage, stream)))
+    return out
+
+
+def imfillHoles(image, binary: bool = False, connectivity: int = 4, *, out=None, ctx=None, stream=None):
+    """Hole filling (nscale::imfillHoles<T>)."""
+    c = _ctx(image, ctx)
+    out = _like(image, _np_dtype(image)) if out is None else out
+    c.check(c.lib.nsx_fill_holes(c.h, C.byref(describe(image)), C.byref(describe(out)), int(bool(binary)),
+                                 connectivity, _torch_stream(image, stream)))
+    return out
+
+
+fillHoles = imfillHoles
+
+
+def bwlabel2(binaryImage, connectivity: int = 8, relab: bool = True, *, out=None, ctx=None, stream=None,
+             return_count: bool = False):
+    """Union-find labelling (nscale::bwlabel2): int32 labels."""
+    c = _ctx(binaryImage, ctx)
+    out = _like(binaryImage, np.int32) if out is None else out
+    n = C.c_int32(0)
+    c.check(c.lib.nsx_bwlabel(c.h, C.byref(describe(binaryImage)), C.byref(describe(out)), connectivity,
+                              int(bool(relab)), C.byref(n), _torch_stream(binaryImage, stream)))
+    return (out, int(n.value)) if return_count else out
+
+
+def bwlabel(binaryImage, connectivity: int = 8, relab: bool = True, **kw):
+    """Union-find labelling under the name of the reference's contour bwlabel."""
+    return bwlabel2(binaryImage, connectivity, relab, **kw)
+
+
+def bwareaopen2(image, labeled: bool, flatten: bool, minSize: int, maxSize: int, connectivity: int = 8, *,
+                out=None, ctx=None, stream=None):
+    """Area filter (nscale::bwareaopen2), inclusive min / exclusive max.  Returns (out, count)."""
+    c = _ctx(image, ctx)
+    out = _like(image, np.uint8 if flatten else np.int32) if out is None else out
+    n = C.c_int32(0)
+    c.check(c.lib.nsx_bwareaopen(c.h, C.byref(describe(image)), int(bool(labeled)), int(bool(flatten)), minSize,
+                                 maxSize, connectivity, C.byref(describe(out)), C.byref(n),
+                                 _torch_stream(image, stream)))
+    return out, int(n.value)
+
+
+def relabel(labels, bgval: int = -1, *, ctx=None, stream=None):
+    """ConnComponents::relabel in place; returns (labels, count)."""
+    c = _ctx(labels, ctx)
+    n = C.c_int32(0)
+    c.check(c.lib.nsx_relabel(c.h, C.byref(describe(labels)), bgval, C.byref(n), _torch_stream(labels, stream)))
+    return labels, int(n.value)
+
+
+def threshold_gt(image, k: float, *, out=None, ctx=None, stream=None):
+    """`image > k` as u8 255/0."""
+    c = _ctx(image, ctx)
+    out = _like(image, np.uint8) if out is None else out
+    c.check(c.lib.nsx_threshold_gt(c.h, C.byref(describe(image)), float(k), C.byref(describe(out)),
+                                   _torch_stream(image, stream)))
+    return out
+
+
+def segment_chain(marker, mask, recon_conn=8, fill_conn=4, thresh=128.0, ccl_conn=8, min_size=11, max_size=1000,
+                  *, recon_out=None, fill_out=None, labels_out=None, want_intermediates=True, ctx=None,
+                  stream=None):
+    """imreconstruct -> imfillHoles -> (> thresh) -> bwareaopen2 -> relabel, fused.
+
+    Returns (recon, filled, labels, ncomp); recon/filled are None when
+    want_intermediates is False and no output buffer was given."""
+    c = _ctx(mask, ctx)
+    dt = _np_dtype(mask)
+    if want_intermediates:
+        recon_out = _like(mask, dt) if recon_out is None else recon_out
+        fill_out = _like(mask, dt) if fill_out is None else fill_out
+    labels_out = _like(mask, np.int32) if labels_out is None else labels_out
+    n = C.c_int32(0)
+    r = C.byref(describe(recon_out)) if recon_out is not None else None
+    f = C.byref(describe(fill_out)) if fill_out is not None else None
+    c.check(c.lib.nsx_segment_chain(c.h, C.byref(describe(marker)), C.byref(describe(mask)), recon_conn, fill_conn,
+                                    float(thresh), ccl_conn, min_size, max_size, r, f,
+                                    C.byref(describe(labels_out)), C.byref(n), _torch_stream(mask, stream)))
+    return recon_out, fill_out, labels_out, int(n.value)
+
+
+# ---- synthetic inputs ------------------------------------------------------------
+def gen_disks_u8(rows: int, cols: int, ndisks: int, seed: int) -> np.ndarray:
+    lib = _lib.load()
+    a = np.empty((rows, cols), np.uint8)
+    if lib.nsx_gen_disks_u8(a.ctypes.data, rows, cols, cols, ndisks, seed) != 0:
+        raise NsxError(_lib.NSX_ERR_INVALID, "gen_disks_u8")
+    return a
+
+
+def gen_marker_u8(mask: np.ndarray, sub: int = 30) -> np.ndarray:
+    lib = _lib.load()
+    mask = np.ascontiguousarray(mask)
+    out = np.empty_like(mask)
+    lib.nsx_gen_marker_u8(mask.ctypes.data, out.ctypes.data, mask.size, sub)
+    return out
+
+
+def gen_maze_u8(rows: int, cols: int, seed: int):
+    lib = _lib.load()
+    mask = np.empty((rows, cols), np.uint8)
+    marker = np.empty((rows, cols), np.uint8)
+    if lib.nsx_gen_maze_u8(mask.ctypes.data, marker.ctypes.data, rows, cols, seed) != 0:
+        raise NsxError(_lib.NSX_ERR_INVALID, "gen_maze_u8")
+    return mask, marker
+
+
+def scale_i32(a: np.ndarray, scale: int = 257) -> np.ndarray:
+    lib = _lib.load()
+    a = np.ascontiguousarray(a)
+    out = np.empty(a.shape, np.int32)
+    lib.nsx_gen_scale_i32(a.ctypes.data, out.ctypes.data, a.size, scale)
+    return out
+
+
+def scale_f32(a: np.ndarray) -> np.ndarray:
+    lib = _lib.load()
+    a = np.ascontiguousarray(a)
+    out = np.empty(a.shape, np.float32)
+    lib.nsx_gen_scale_f32(a.ctypes.data, out.ctypes.data, a.size)
+    return out

@@ -1,0 +1,176 @@
+/* nscale_hip.h -- C ABI of the MI355X (gfx950) morphology + CCL library.
+ *
+ * Drop-in boundary for the nscale:: morphological-reconstruction and
+ * connected-component path of fastIA (paths below are relative to
+ * fastIA-master/).  Every entry point takes plain pointers and sizes, returns an
+ * nsx_status and never throws; reference CV_Assert failures become
+ * NSX_ERR_INVALID with a message from nsx_last_error().
+ *
+ * Images are described by nsx_img (what cv::Mat exposes: data, step, rows,
+ * cols, type).  Buffers are caller-owned.  NSX_MEM_DEVICE images live in HBM
+ * (hipMalloc'd, on the context's device); NSX_MEM_HOST images are copied in
+ * and out by the call (synchronously, like the reference CPU functions).
+ * Output images must not alias inputs.
+ *
+ * A context owns the device workspace (tile work lists, union-find tables)
+ * and its HBM budget.  Contexts are independent: there is no module-global
+ * device state, so one context per host thread or stream is reentrant.
+ * Entry points are ordered on the stream passed in (a hipStream_t, or NULL for
+ * the context's stream); calls that return a count (ncomp, count) synchronise
+ * that stream before returning.
+ */
+#ifndef NSCALE_HIP_H_
+#define NSCALE_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    NSX_OK = 0,
+    NSX_ERR_INVALID = 1,      /* bad argument (reference: CV_Assert) */
+    NSX_ERR_UNSUPPORTED = 2,  /* dtype/mode the reference does not instantiate */
+    NSX_ERR_HIP = 3,          /* HIP runtime error */
+    NSX_ERR_NOMEM = 4,        /* device allocation failed */
+    NSX_ERR_BUDGET = 5,       /* request exceeds the context's HBM budget */
+    NSX_ERR_NOT_CONVERGED = 6 /* propagation hit its round limit */
+} nsx_status;
+
+typedef enum { NSX_U8 = 0, NSX_I32 = 1, NSX_F32 = 2 } nsx_dtype;
+typedef enum { NSX_MEM_DEVICE = 0, NSX_MEM_HOST = 1 } nsx_mem;
+
+typedef struct {
+    void* data;
+    int64_t pitch_bytes; /* row step in bytes (cv::Mat::step) */
+    int32_t rows;
+    int32_t cols;
+    int32_t dtype;       /* nsx_dtype */
+    int32_t mem;         /* nsx_mem */
+} nsx_img;
+
+typedef struct nsx_ctx nsx_ctx;
+
+/* ---- context -------------------------------------------------------------- */
+/* Creates a context on `device`.  hbm_budget_bytes bounds the workspace the
+ * context may hold (0 = no limit).  No reference counterpart: the reference
+ * keeps its queue state in module globals (src/segment/cuda/global_queue.cu:9-22). */
+nsx_status nsx_ctx_create(nsx_ctx** ctx, int device, size_t hbm_budget_bytes);
+void nsx_ctx_destroy(nsx_ctx* ctx);
+const char* nsx_last_error(const nsx_ctx* ctx);
+nsx_status nsx_ctx_synchronize(nsx_ctx* ctx);
+/* Device bytes currently held by the context's workspace. */
+size_t nsx_ctx_workspace_bytes(const nsx_ctx* ctx);
+/* Reads the statistics of the last reconstruction run on this context:
+ * stats[0] = propagation rounds, stats[1] = tile passes (sum of active tiles
+ * over rounds), stats[2] = tiles in the image.  Synchronises the stream. */
+nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int64_t stats[3]);
+
+/* ---- morphology ----------------------------------------------------------- */
+/* Grayscale reconstruction by dilation of `marker` under `mask`; conn 8 gives
+ * 8-connectivity, anything else 4.  u8, i32, f32.
+ * Replaces nscale::imreconstruct<T>(const Mat& seeds, const Mat& image, int)
+ * (src/segment/MorphologicOperations.h:21-22, .cpp:665-838) and
+ * nscale::gpu::imreconstruct / imreconstructQueueSpeedup (.h:101-130,
+ * MorphologicOperationsGPU.cpp:172-200,286-441). */
+nsx_status nsx_imreconstruct(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* mask,
+                             nsx_img* out, int conn, void* stream);
+
+/* Binary reconstruction: pixels of `mask` != 0 connected to a seed (marker != 0
+ * and mask != 0) become max(T); nonzero marker pixels are kept as given.
+ * u8, i32.  Replaces nscale::imreconstructBinary<T> (MorphologicOperations.h:43-44,
+ * .cpp:1688-1829) and nscale::gpu::imreconstructBinary (MorphologicOperationsGPU.cpp:686-721). */
+nsx_status nsx_imreconstruct_binary(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* mask,
+                                    nsx_img* out, int conn, void* stream);
+
+/* Hole filling.  u8, i32.  Replaces nscale::imfillHoles<T>(const Mat&, bool
+ * binary, int) (MorphologicOperations.h:50-51, .cpp:1859-1926) and
+ * nscale::gpu::imfillHoles (MorphologicOperationsGPU.cpp:751-836). */
+nsx_status nsx_fill_holes(nsx_ctx* ctx, const nsx_img* in, nsx_img* out, int binary, int conn,
+                          void* stream);
+
+/* ---- connected components ------------------------------------------------- */
+/* Union-find labelling of a u8 image: 4/8-neighbours of equal nonzero value are
+ * joined.  relabel=0: label = raster index of the component's first pixel,
+ * background -1.  relabel=1: labels 1..N in raster order of those first pixels,
+ * background 0; *ncomp = N (may be NULL).
+ * Replaces nscale::bwlabel2(const Mat&, int, bool) (MorphologicOperations.h:59,
+ * .cpp:2012-2035; ConnComponents.cpp:52-118) and nscale::gpu::bwlabel
+ * (MorphologicOperationsGPU.cpp:878-900). */
+nsx_status nsx_bwlabel(nsx_ctx* ctx, const nsx_img* bin, nsx_img* labels, int conn, int relabel,
+                       int32_t* ncomp, void* stream);
+
+/* Area filter, inclusive min, exclusive max.  labeled=0: `in` is u8 and is
+ * labelled first (conn); labeled=1: `in` is int32 root labels, background -1.
+ * flatten=0: out is int32 root labels of the kept components, background -1;
+ * flatten=1: out is u8, 255 on kept components, else 0.  *count = number kept.
+ * Replaces nscale::bwareaopen2 (MorphologicOperations.h:74, .cpp:2199-2232;
+ * ConnComponents.cpp:205-239) and nscale::gpu::bwareaopen2
+ * (MorphologicOperationsGPU.cpp:1024-1057). */
+nsx_status nsx_bwareaopen(nsx_ctx* ctx, const nsx_img* in, int labeled, int flatten, int min_size,
+                          int max_size, int conn, nsx_img* out, int32_t* count, void* stream);
+
+/* Canonical relabel of int32 labels in place: pixels whose label equals their
+ * own raster index are roots, numbered 1.. in raster order; background (bgval)
+ * and everything else map to their root's number, background to 0.
+ * Replaces ConnComponents::relabel (ConnComponents.cpp:89-118) and
+ * nscale::gpu::relabel (src/segment/cuda/ccl_uf.cu:285-479). */
+nsx_status nsx_relabel(nsx_ctx* ctx, nsx_img* labels, int bgval, int32_t* ncomp, void* stream);
+
+/* `Mat > k` glue on u8/i32/f32 input: out (u8) = 255 where in > k, else 0.
+ * Replaces the comparisons at HistologicalEntities.cpp:214,261 and
+ * nscale::gpu::PixelOperations::threshold (src/segment/cuda/pixel-ops.cu:348-362). */
+nsx_status nsx_threshold_gt(nsx_ctx* ctx, const nsx_img* in, double k, nsx_img* out, void* stream);
+
+/* ---- fused chain ---------------------------------------------------------- */
+/* The nuclei-candidate chain (HistologicalEntities.cpp:195,219,246,567):
+ *   recon  = imreconstruct(marker, mask, recon_conn)
+ *   filled = imfillHoles(recon, binary=0, fill_conn)
+ *   labels = relabel(bwareaopen2(filled > thresh, min_size, max_size, ccl_conn))
+ * recon_out / fill_out may be NULL when only labels are wanted (then the
+ * context holds the intermediates).  u8 and i32 marker/mask; labels int32 with
+ * background 0.  *ncomp = number of labels (may be NULL). */
+nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* mask,
+                             int recon_conn, int fill_conn, double thresh, int ccl_conn,
+                             int min_size, int max_size, nsx_img* recon_out, nsx_img* fill_out,
+                             nsx_img* labels_out, int32_t* ncomp, void* stream);
+
+/* ---- batched tiles -------------------------------------------------------- */
+/* Many independent device-resident tiles of uneven size run reconstruction +
+ * threshold + labelling together, captured once in a hipGraph and replayed.
+ * Replaces nscale::gpu::imreconstructQueueThroughput (MorphologicOperationsGPU.cpp:447-545,
+ * global_queue.cu:903-968), which gives each image one thread block.
+ * Per tile: out_recon = imreconstruct(marker, mask, conn); labels =
+ * bwlabel(out_recon > thresh, ccl_conn, relabel=1). */
+typedef struct nsx_batch nsx_batch;
+nsx_status nsx_batch_create(nsx_ctx* ctx, nsx_batch** batch);
+nsx_status nsx_batch_add(nsx_batch* batch, const nsx_img* marker, const nsx_img* mask,
+                         nsx_img* recon_out, nsx_img* labels_out, int conn, double thresh,
+                         int ccl_conn);
+/* Builds the work tables and (use_graph=1) captures the graph.  Call once after
+ * the last add. */
+nsx_status nsx_batch_finalize(nsx_batch* batch, int use_graph);
+/* Runs the whole batch on the stream (graph replay when captured). */
+nsx_status nsx_batch_run(nsx_batch* batch, void* stream);
+/* Per-tile component counts after a run (array of n tiles). Synchronises. */
+nsx_status nsx_batch_counts(nsx_batch* batch, int32_t* counts, int n);
+void nsx_batch_destroy(nsx_batch* batch);
+
+/* ---- synthetic inputs (host) ---------------------------------------------- */
+uint64_t nsx_splitmix64_next(uint64_t* state);
+int nsx_gen_disks_u8(uint8_t* mask, int rows, int cols, long pitch, int ndisks, uint64_t seed);
+void nsx_gen_marker_u8(const uint8_t* mask, uint8_t* marker, long n, int sub);
+void nsx_gen_scale_i32(const uint8_t* in, int32_t* out, long n, int scale);
+void nsx_gen_scale_f32(const uint8_t* in, float* out, long n);
+int nsx_gen_maze_u8(uint8_t* mask, uint8_t* marker, int rows, int cols, uint64_t seed);
+
+/* Library version string. */
+const char* nsx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NSCALE_HIP_H_ */

@@ -1,0 +1,516 @@
+// nscale_oracle.cpp -- CPU restatement of the reference's morphology / CCL path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: it is compiled into
+// oracle/liboracle.so and loaded by tests/, by __graft_entry__.smoke() as the
+// checker, and by bench.py's cpu_baseline leg.  The product library never links,
+// loads or calls it, and nothing here is a fallback for the GPU path.
+//
+// Every function restates, loop for loop, the reference CPU code it cites
+// (paths relative to fastIA-master/).  OpenCV's Mat is replaced by plain
+// row-major buffers; copyMakeBorder(BORDER_CONSTANT, v) becomes an explicit
+// (rows+2)x(cols+2) padded copy, and the returned ROI becomes a crop.
+//
+// Pinning (what checks each function, see tests/test_oracle.py):
+//   or_imreconstruct_*      brute-force geodesic dilation to the fixed point on
+//                           tiny images, the 32x32 flood-fill fixture of
+//                           test/segment/cuda/globalQueueTest.cu:446-583,
+//                           order-isomorphism u8 -> i32/f32, closed forms.
+//   or_imreconstruct_binary brute-force BFS flood fill + marker-kept quirk.
+//   or_fill_holes           MATLAB imfill 'holes' closed form (complement of
+//                           the border-reachable set / reconstruction from frame).
+//   or_bwlabel / relabel /  the reference's own ConnComponents.cpp compiled
+//   or_bwareaopen           unmodified into oracle/_ref (exact), scipy
+//                           ndimage.label partitions, and the canonical-numbering
+//                           rule.
+//   i32 invert at INT_MIN   parity unpinned (OpenCV 2.x unavailable): saturating
+//                           negation chosen, see or_invert_i32.
+#include <cstdint>
+#include <cstring>
+#include <climits>
+#include <cfloat>
+#include <limits>
+#include <queue>
+#include <vector>
+#include <algorithm>
+#include <unordered_map>
+
+namespace {
+
+// cci::common::type::min<T> (src/common/TypeUtils.h:33-41): integer min, or
+// -max for floating point.
+template <typename T> inline T type_min() {
+    if constexpr (std::numeric_limits<T>::is_integer) return std::numeric_limits<T>::min();
+    else return -std::numeric_limits<T>::max();
+}
+
+// copyMakeBorder(src, dst, 1,1,1,1, BORDER_CONSTANT, v)
+template <typename T>
+std::vector<T> pad1(const T* src, int rows, int cols, T v) {
+    const int W = cols + 2, H = rows + 2;
+    std::vector<T> dst((size_t)W * H, v);
+    for (int y = 0; y < rows; ++y)
+        std::memcpy(&dst[(size_t)(y + 1) * W + 1], src + (size_t)y * cols, sizeof(T) * cols);
+    return dst;
+}
+
+template <typename T>
+void crop1(const std::vector<T>& src, int rows, int cols, T* dst) {
+    const int W = cols + 2;
+    for (int y = 0; y < rows; ++y)
+        std::memcpy(dst + (size_t)y * cols, &src[(size_t)(y + 1) * W + 1], sizeof(T) * cols);
+}
+
+// propagate<T> (MorphologicOperations.cpp:428-439)
+template <typename T>
+inline void propagate(std::queue<int>& xQ, std::queue<int>& yQ, int x, int y,
+                      const T* iPtr, T* oPtr, const T& pval) {
+    T qval = oPtr[x];
+    T ival = iPtr[x];
+    if ((qval < pval) && (ival != qval)) {
+        oPtr[x] = std::min(pval, ival);
+        xQ.push(x);
+        yQ.push(y);
+    }
+}
+
+// imreconstruct<T> (MorphologicOperations.cpp:665-838): Vincent's hybrid
+// raster / anti-raster scan followed by a FIFO drain, on a 0-framed copy.
+// Operates on padded buffers; returns the padded output.
+template <typename T>
+void imreconstruct_padded(std::vector<T>& output, const std::vector<T>& input,
+                          int prow, int pcol, int connectivity) {
+    const int W = pcol;
+    T pval, preval;
+    int xminus, xplus, yminus, yplus;
+    const int maxx = pcol - 1;
+    const int maxy = prow - 1;
+    std::queue<int> xQ, yQ;
+    T *oPtr, *oPtrMinus, *oPtrPlus;
+    const T *iPtr, *iPtrPlus, *iPtrMinus;
+    auto orow = [&](int y) { return output.data() + (size_t)y * W; };
+    auto irow = [&](int y) { return input.data() + (size_t)y * W; };
+
+    // raster scan (:691-713)
+    for (int y = 1; y < maxy; ++y) {
+        oPtr = orow(y);
+        oPtrMinus = orow(y - 1);
+        iPtr = irow(y);
+        preval = oPtr[0];
+        for (int x = 1; x < maxx; ++x) {
+            xminus = x - 1;
+            xplus = x + 1;
+            pval = oPtr[x];
+            pval = std::max(pval, std::max(preval, oPtrMinus[x]));
+            if (connectivity == 8) pval = std::max(pval, std::max(oPtrMinus[xplus], oPtrMinus[xminus]));
+            preval = std::min(pval, iPtr[x]);
+            oPtr[x] = preval;
+        }
+    }
+
+    // anti-raster scan with seed capture (:715-763)
+    for (int y = maxy - 1; y > 0; --y) {
+        oPtr = orow(y);
+        oPtrPlus = orow(y + 1);
+        oPtrMinus = orow(y - 1);
+        iPtr = irow(y);
+        iPtrPlus = irow(y + 1);
+        preval = oPtr[maxx];
+        for (int x = maxx - 1; x > 0; --x) {
+            xminus = x - 1;
+            xplus = x + 1;
+            pval = oPtr[x];
+            pval = std::max(pval, std::max(preval, oPtrPlus[x]));
+            if (connectivity == 8) pval = std::max(pval, std::max(oPtrPlus[xplus], oPtrPlus[xminus]));
+            preval = std::min(pval, iPtr[x]);
+            oPtr[x] = preval;
+
+            pval = oPtr[x];
+            if ((oPtr[xplus] < std::min(pval, iPtr[xplus])) ||
+                (oPtrPlus[x] < std::min(pval, iPtrPlus[x]))) {
+                xQ.push(x);
+                yQ.push(y);
+                continue;
+            }
+            if (connectivity == 8) {
+                if ((oPtrPlus[xplus] < std::min(pval, iPtrPlus[xplus])) ||
+                    (oPtrPlus[xminus] < std::min(pval, iPtrPlus[xminus]))) {
+                    xQ.push(x);
+                    yQ.push(y);
+                    continue;
+                }
+            }
+        }
+    }
+
+    // FIFO drain (:772-828)
+    int x, y;
+    while (!xQ.empty()) {
+        x = xQ.front();
+        y = yQ.front();
+        xQ.pop();
+        yQ.pop();
+        xminus = x - 1;
+        xplus = x + 1;
+        yminus = y - 1;
+        yplus = y + 1;
+        oPtr = orow(y);
+        oPtrPlus = orow(yplus);
+        oPtrMinus = orow(yminus);
+        iPtr = irow(y);
+        iPtrPlus = irow(yplus);
+        iPtrMinus = irow(yminus);
+        pval = oPtr[x];
+        if (y > 0) propagate<T>(xQ, yQ, x, yminus, iPtrMinus, oPtrMinus, pval);
+        if (y < maxy) propagate<T>(xQ, yQ, x, yplus, iPtrPlus, oPtrPlus, pval);
+        if (x > 0) propagate<T>(xQ, yQ, xminus, y, iPtr, oPtr, pval);
+        if (x < maxx) propagate<T>(xQ, yQ, xplus, y, iPtr, oPtr, pval);
+        if (connectivity == 8) {
+            if (y > 0) {
+                if (x > 0) propagate<T>(xQ, yQ, xminus, yminus, iPtrMinus, oPtrMinus, pval);
+                if (x < maxx) propagate<T>(xQ, yQ, xplus, yminus, iPtrMinus, oPtrMinus, pval);
+            }
+            if (y < maxy) {
+                if (x > 0) propagate<T>(xQ, yQ, xminus, yplus, iPtrPlus, oPtrPlus, pval);
+                if (x < maxx) propagate<T>(xQ, yQ, xplus, yplus, iPtrPlus, oPtrPlus, pval);
+            }
+        }
+    }
+}
+
+template <typename T>
+void imreconstruct(const T* seeds, const T* image, T* out, int rows, int cols, int connectivity) {
+    std::vector<T> output = pad1<T>(seeds, rows, cols, T(0));   // :671-672
+    std::vector<T> input = pad1<T>(image, rows, cols, T(0));    // :673-674
+    imreconstruct_padded<T>(output, input, rows + 2, cols + 2, connectivity);
+    crop1<T>(output, rows, cols, out);                          // :836
+}
+
+// propagateBinary<T> (MorphologicOperations.cpp:1665-1672)
+template <typename T>
+inline void propagateBinary(std::queue<int>& xQ, std::queue<int>& yQ, int x, int y,
+                            const T* iPtr, T* oPtr, const T& foreground) {
+    if ((oPtr[x] == 0) && (iPtr[x] != 0)) {
+        oPtr[x] = foreground;
+        xQ.push(x);
+        yQ.push(y);
+    }
+}
+
+// imreconstructBinary<T> (MorphologicOperations.cpp:1688-1829), padded form.
+template <typename T>
+void imreconstructBinary_padded(std::vector<T>& output, const std::vector<T>& input,
+                                int prow, int pcol, int connectivity) {
+    const int W = pcol;
+    T pval, ival;
+    int xminus, xplus, yminus, yplus;
+    const int maxx = pcol - 1;
+    const int maxy = prow - 1;
+    std::queue<int> xQ, yQ;
+    T *oPtr, *oPtrPlus, *oPtrMinus;
+    const T *iPtr, *iPtrPlus, *iPtrMinus;
+    auto orow = [&](int y) { return output.data() + (size_t)y * W; };
+    auto irow = [&](int y) { return input.data() + (size_t)y * W; };
+
+    // contour pixels (:1712-1756)
+    for (int y = 1; y < maxy; ++y) {
+        oPtr = orow(y);
+        oPtrPlus = orow(y + 1);
+        oPtrMinus = orow(y - 1);
+        iPtr = irow(y);
+        for (int x = 1; x < maxx; ++x) {
+            pval = oPtr[x];
+            ival = iPtr[x];
+            if (pval != 0 && ival != 0) {
+                xminus = x - 1;
+                xplus = x + 1;
+                if ((oPtrMinus[x] == 0) || (oPtrPlus[x] == 0) || (oPtr[xplus] == 0) || (oPtr[xminus] == 0)) {
+                    xQ.push(x);
+                    yQ.push(y);
+                    continue;
+                }
+                if (connectivity == 8) {
+                    if ((oPtrMinus[xminus] == 0) || (oPtrMinus[xplus] == 0) ||
+                        (oPtrPlus[xminus] == 0) || (oPtrPlus[xplus] == 0)) {
+                        xQ.push(x);
+                        yQ.push(y);
+                        continue;
+                    }
+                }
+            }
+        }
+    }
+
+    // BFS (:1763-1823)
+    const T outval = std::numeric_limits<T>::max();
+    int x, y;
+    while (!xQ.empty()) {
+        x = xQ.front();
+        y = yQ.front();
+        xQ.pop();
+        yQ.pop();
+        xminus = x - 1;
+        yminus = y - 1;
+        yplus = y + 1;
+        xplus = x + 1;
+        oPtr = orow(y);
+        oPtrMinus = orow(y - 1);
+        oPtrPlus = orow(y + 1);
+        iPtr = irow(y);
+        iPtrMinus = irow(y - 1);
+        iPtrPlus = irow(y + 1);
+        if (y > 0) propagateBinary<T>(xQ, yQ, x, yminus, iPtrMinus, oPtrMinus, outval);
+        if (y < maxy) propagateBinary<T>(xQ, yQ, x, yplus, iPtrPlus, oPtrPlus, outval);
+        if (x > 0) propagateBinary<T>(xQ, yQ, xminus, y, iPtr, oPtr, outval);
+        if (x < maxx) propagateBinary<T>(xQ, yQ, xplus, y, iPtr, oPtr, outval);
+        if (connectivity == 8) {
+            if (y > 0) {
+                if (x > 0) propagateBinary<T>(xQ, yQ, xminus, yminus, iPtrMinus, oPtrMinus, outval);
+                if (x < maxx) propagateBinary<T>(xQ, yQ, xplus, yminus, iPtrMinus, oPtrMinus, outval);
+            }
+            if (y < maxy) {
+                if (x > 0) propagateBinary<T>(xQ, yQ, xminus, yplus, iPtrPlus, oPtrPlus, outval);
+                if (x < maxx) propagateBinary<T>(xQ, yQ, xplus, yplus, iPtrPlus, oPtrPlus, outval);
+            }
+        }
+    }
+}
+
+template <typename T>
+void imreconstructBinary(const T* seeds, const T* image, T* out, int rows, int cols, int connectivity) {
+    std::vector<T> output = pad1<T>(seeds, rows, cols, T(0));   // :1694
+    std::vector<T> input = pad1<T>(image, rows, cols, T(0));    // :1696
+    imreconstructBinary_padded<T>(output, input, rows + 2, cols + 2, connectivity);
+    crop1<T>(output, rows, cols, out);                          // :1827
+}
+
+// PixelOperations::invert<T> (PixelOperations.cpp:17-39).
+//   unsigned: max - x ;  float: -x ;  signed int: bitwise_not(x) + 1.
+// The signed case goes through OpenCV Mat+Scalar on CV_32S, whose overflow
+// behaviour at x == INT_MIN cannot be checked here (OpenCV is absent).  We take
+// the saturate_cast contract: ~INT_MIN + 1 saturates to INT_MAX.  All other
+// inputs are exact two's-complement negation.  (Parity unpinned at INT_MIN.)
+inline uint8_t invert_one(uint8_t v) { return (uint8_t)(255 - v); }
+inline int32_t invert_one(int32_t v) { return v == INT_MIN ? INT_MAX : -v; }
+inline float invert_one(float v) { return -v; }
+
+template <typename T>
+void invert_vec(std::vector<T>& v) {
+    for (auto& e : v) e = invert_one(e);
+}
+
+// imfillHoles<T> (MorphologicOperations.cpp:1859-1926).
+template <typename T>
+void imfillHoles(const T* image, T* out, int rows, int cols, bool binary, int connectivity) {
+    const T mn = type_min<T>();                                  // :1889
+    const T mx = std::numeric_limits<T>::max();                  // :1890
+    std::vector<T> mask = pad1<T>(image, rows, cols, mn);         // :1894-1895
+    std::vector<T> marker2((size_t)rows * cols, mn);             // :1898
+    std::vector<T> marker = pad1<T>(marker2.data(), rows, cols, mx);  // :1900
+    invert_vec(mask);                                            // :1903
+    // imreconstruct / imreconstructBinary pad again with a 0 frame.
+    const int R = rows + 2, C = cols + 2;
+    std::vector<T> output;
+    output.resize((size_t)R * C);
+    if (binary) imreconstructBinary<T>(marker.data(), mask.data(), output.data(), R, C, connectivity);
+    else imreconstruct<T>(marker.data(), mask.data(), output.data(), R, C, connectivity);
+    invert_vec(output);                                          // :1923
+    crop1<T>(output, rows, cols, out);                           // :1925
+}
+
+// ---- ConnComponents (src/segment/ConnComponents.cpp) ----------------------
+// find/flatten are written iteratively: the recursive originals (:24-32,
+// :137-147) compress every node on the path to the root, which the two-pass
+// loops below reproduce exactly, without deep recursion on very large images.
+inline int cc_find(int* label, int x) {
+    int r = x;
+    while (label[r] != r) r = label[r];
+    while (label[x] != r) { int n = label[x]; label[x] = r; x = n; }
+    return r;
+}
+
+inline void cc_merge(int* label, int x, int y) {   // :34-43
+    x = cc_find(label, x);
+    y = cc_find(label, y);
+    if (label[x] < label[y]) label[y] = x;
+    else if (label[x] > label[y]) label[x] = y;
+}
+
+inline int cc_flatten(int* label, int x, int bgval) {   // :137-147
+    if (label[x] == bgval) return bgval;
+    int r = x;
+    while (label[r] != r) {
+        r = label[r];
+        if (r == bgval) break;   // not reachable for well-formed forests
+    }
+    while (x != r && label[x] != r) { int n = label[x]; label[x] = r; x = n; }
+    return r;
+}
+
+void cc_label(const uint8_t* img, int w, int h, int* label, int bgval, int connectivity) {  // :52-86
+    const long length = (long)w * h;
+    for (long i = 0; i < length; ++i) label[i] = (int)i;
+    long i = -1, imw = 0;
+    uint8_t p;
+    for (int y = 0; y < h; ++y) {
+        for (int x = 0; x < w; ++x) {
+            ++i;
+            p = img[i];
+            if (p == 0) {
+                label[i] = bgval;
+            } else {
+                imw = i - w;
+                if (x > 0 && p == img[i - 1]) cc_merge(label, (int)i, (int)(i - 1));
+                if (y > 0 && p == img[imw]) cc_merge(label, (int)i, (int)imw);
+                if (connectivity == 8) {
+                    if (y > 0 && x > 0 && p == img[imw - 1]) cc_merge(label, (int)i, (int)(imw - 1));
+                    if (y > 0 && x < w - 1 && p == img[imw + 1]) cc_merge(label, (int)i, (int)(imw + 1));
+                }
+            }
+        }
+    }
+    for (long k = 0; k < length; ++k) label[k] = cc_flatten(label, (int)k, bgval);
+}
+
+// relabel (:89-118): roots (label[i]==i) numbered 1.. in raster order, bg -> 0.
+int cc_relabel(int w, int h, int* label, int bgval) {
+    const long length = (long)w * h;
+    std::unordered_map<int, int> labelmap;
+    int j = 1;
+    labelmap[bgval] = 0;
+    for (long i = 0; i < length; ++i) {
+        if (label[i] == i) {
+            labelmap[(int)i] = j;
+            ++j;
+        }
+    }
+    for (long i = 0; i < length; ++i) label[i] = labelmap[label[i]];
+    return j - 1;
+}
+
+// areaThresholdLabeled (:205-239): keep labels whose pixel count is in
+// [lower, upper); everything else becomes bgval.  Returns the kept count.
+int cc_area_threshold_labeled(const int* label, int w, int h, int* n_label, int bgval, int lower, int upper) {
+    const long length = (long)w * h;
+    typedef std::unordered_map<int, int> AreaMap;
+    AreaMap areas, thresholdedareas;
+    for (long i = 0; i < length; ++i) {
+        if (label[i] != bgval) areas[label[i]] += 1;
+        n_label[i] = bgval;
+    }
+    int j = 0;
+    for (AreaMap::iterator it = areas.begin(); it != areas.end(); ++it) {
+        if (it->second >= lower && it->second < upper) {
+            thresholdedareas[it->first] = 1;
+            ++j;
+        } else {
+            thresholdedareas[it->first] = 0;
+        }
+    }
+    areas.clear();
+    for (long i = 0; i < length; ++i) {
+        if (label[i] != bgval && thresholdedareas[label[i]] == 1) n_label[i] = label[i];
+    }
+    return j;
+}
+
+}  // namespace
+
+extern "C" {
+
+int or_imreconstruct_u8(const uint8_t* marker, const uint8_t* mask, uint8_t* out, int rows, int cols, int conn) {
+    imreconstruct<uint8_t>(marker, mask, out, rows, cols, conn);
+    return 0;
+}
+int or_imreconstruct_i32(const int32_t* marker, const int32_t* mask, int32_t* out, int rows, int cols, int conn) {
+    imreconstruct<int32_t>(marker, mask, out, rows, cols, conn);
+    return 0;
+}
+int or_imreconstruct_f32(const float* marker, const float* mask, float* out, int rows, int cols, int conn) {
+    imreconstruct<float>(marker, mask, out, rows, cols, conn);
+    return 0;
+}
+int or_imreconstruct_binary_u8(const uint8_t* marker, const uint8_t* mask, uint8_t* out, int rows, int cols, int conn) {
+    imreconstructBinary<uint8_t>(marker, mask, out, rows, cols, conn);
+    return 0;
+}
+int or_imreconstruct_binary_i32(const int32_t* marker, const int32_t* mask, int32_t* out, int rows, int cols, int conn) {
+    imreconstructBinary<int32_t>(marker, mask, out, rows, cols, conn);
+    return 0;
+}
+int or_fill_holes_u8(const uint8_t* in, uint8_t* out, int rows, int cols, int binary, int conn) {
+    imfillHoles<uint8_t>(in, out, rows, cols, binary != 0, conn);
+    return 0;
+}
+int or_fill_holes_i32(const int32_t* in, int32_t* out, int rows, int cols, int binary, int conn) {
+    imfillHoles<int32_t>(in, out, rows, cols, binary != 0, conn);
+    return 0;
+}
+void or_invert_i32(const int32_t* in, int32_t* out, long n) {
+    for (long i = 0; i < n; ++i) out[i] = invert_one(in[i]);
+}
+
+// bwlabel2 (MorphologicOperations.cpp:2012-2035).  relab=0 returns root-index
+// labels with background -1; relab=1 returns 1..N in raster order of the roots,
+// background 0.  Returns N (relab=1) or 0.
+int or_bwlabel2(const uint8_t* bin, int32_t* out, int rows, int cols, int conn, int relab) {
+    cc_label(bin, cols, rows, out, -1, conn);
+    int j = 0;
+    if (relab) j = cc_relabel(cols, rows, out, -1);
+    return j;
+}
+
+// bwareaopen2 (MorphologicOperations.cpp:2199-2232).  labeled=0: input is u8,
+// labelled with union-find first; labeled=1: input is int32 root labels (bg -1).
+// flatten=0 writes int32 labels (bg -1) to out_i32; flatten=1 writes u8 255/0
+// to out_u8.  Returns count (number of kept components).
+int or_bwareaopen2(const void* in, int labeled, int flatten, int min_size, int max_size, int conn,
+                   int rows, int cols, int32_t* out_i32, uint8_t* out_u8) {
+    const long n = (long)rows * cols;
+    std::vector<int> output(n > 0 ? n : 1);
+    int count;
+    if (!labeled) {
+        std::vector<int> temp(n > 0 ? n : 1);
+        cc_label((const uint8_t*)in, cols, rows, temp.data(), -1, conn);
+        count = cc_area_threshold_labeled(temp.data(), cols, rows, output.data(), -1, min_size, max_size);
+    } else {
+        count = cc_area_threshold_labeled((const int*)in, cols, rows, output.data(), -1, min_size, max_size);
+    }
+    if (flatten) {
+        for (long i = 0; i < n; ++i) out_u8[i] = output[i] > -1 ? 255 : 0;
+    } else {
+        std::memcpy(out_i32, output.data(), sizeof(int) * n);
+    }
+    return count;
+}
+
+// ConnComponents::relabel on an int32 label image with background bgval.
+int or_relabel(int32_t* labels, int rows, int cols, int bgval) {
+    return cc_relabel(cols, rows, labels, bgval);
+}
+
+// Threshold glue: OpenCV `Mat > k` on u8 (255 where v > k, else 0).
+void or_threshold_gt_u8(const uint8_t* in, uint8_t* out, long n, int k) {
+    for (long i = 0; i < n; ++i) out[i] = in[i] > k ? 255 : 0;
+}
+
+// The nuclei-segmentation chain pinned by HistologicalEntities.cpp:195,219,246,567
+// in the order the reconstruct + CCL benchmark runs it:
+//   r = imreconstruct<u8>(marker, mask, recon_conn)
+//   f = imfillHoles<u8>(r, binary=false, fill_conn)
+//   b = f > thresh
+//   l = bwareaopen2(b, labeled=false, flatten=false, min, max, ccl_conn)
+//   l = relabel(l)   (canonical raster order, background 0)
+// Writes r, f and l; returns the component count after the area filter.
+int or_chain_u8(const uint8_t* marker, const uint8_t* mask, int rows, int cols,
+                int recon_conn, int fill_conn, int thresh, int ccl_conn, int min_size, int max_size,
+                uint8_t* recon_out, uint8_t* fill_out, int32_t* labels_out) {
+    const long n = (long)rows * cols;
+    imreconstruct<uint8_t>(marker, mask, recon_out, rows, cols, recon_conn);
+    imfillHoles<uint8_t>(recon_out, fill_out, rows, cols, false, fill_conn);
+    std::vector<uint8_t> b(n > 0 ? n : 1);
+    for (long i = 0; i < n; ++i) b[i] = fill_out[i] > thresh ? 255 : 0;
+    or_bwareaopen2(b.data(), 0, 0, min_size, max_size, ccl_conn, rows, cols, labels_out, nullptr);
+    return cc_relabel(cols, rows, labels_out, -1);
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+"""Python access to the parity oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads oracle/liboracle.so (our C++ restatement of the reference CPU path,
+nscale_oracle.cpp) and, when built, oracle/_ref/libccref.so (the reference's
+own ConnComponents.cpp).  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+REF = HERE / "_ref" / "libccref.so"
+
+_lib = None
+_ref = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"{LIB} missing: run `make -C oracle`")
+        _lib = C.CDLL(str(LIB))
+    return _lib
+
+
+def ref():
+    """The compiled reference ConnComponents (None when not built here)."""
+    global _ref
+    if _ref is None and REF.exists():
+        _ref = C.CDLL(str(REF))
+    return _ref
+
+
+def _c(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def imreconstruct(marker, mask, conn=4):
+    mask = np.ascontiguousarray(mask)
+    marker = _c(marker, mask.dtype)
+    out = np.empty_like(mask)
+    fn = {np.uint8: "or_imreconstruct_u8", np.int32: "or_imreconstruct_i32",
+          np.float32: "or_imreconstruct_f32"}[mask.dtype.type]
+    getattr(lib(), fn)(_p(marker), _p(mask), _p(out), mask.shape[0], mask.shape[1], conn)
+    return out
+
+
+def imreconstruct_binary(marker, mask, conn=4):
+    mask = np.ascontiguousarray(mask)
+    marker = _c(marker, mask.dtype)
+    out = np.empty_like(mask)
+    fn = {np.uint8: "or_imreconstruct_binary_u8", np.int32: "or_imreconstruct_binary_i32"}[mask.dtype.type]
+    getattr(lib(), fn)(_p(marker), _p(mask), _p(out), mask.shape[0], mask.shape[1], conn)
+    return out
+
+
+def fill_holes(img, binary=False, conn=4):
+    img = np.ascontiguousarray(img)
+    out = np.empty_like(img)
+    fn = {np.uint8: "or_fill_holes_u8", np.int32: "or_fill_holes_i32"}[img.dtype.type]
+    getattr(lib(), fn)(_p(img), _p(out), img.shape[0], img.shape[1], int(bool(binary)), conn)
+    return out
+
+
+def bwlabel2(binimg, conn=8, relab=True):
+    b = _c(binimg, np.uint8)
+    out = np.empty(b.shape, np.int32)
+    n = lib().or_bwlabel2(_p(b), _p(out), b.shape[0], b.shape[1], conn, int(bool(relab)))
+    return out, int(n)
+
+
+def bwareaopen2(img, labeled, flatten, min_size, max_size, conn=8):
+    a = np.ascontiguousarray(img)
+    rows, cols = a.shape
+    out_i = np.empty((rows, cols), np.int32)
+    out_u = np.empty((rows, cols), np.uint8)
+    n = lib().or_bwareaopen2(_p(a), int(bool(labeled)), int(bool(flatten)), min_size, max_size, conn, rows, cols,
+                             _p(out_i), _p(out_u))
+    return (out_u if flatten else out_i), int(n)
+
+
+def relabel(labels, bgval=-1):
+    a = np.array(labels, dtype=np.int32, copy=True, order="C")
+    n = lib().or_relabel(_p(a), a.shape[0], a.shape[1], bgval)
+    return a, int(n)
+
+
+def chain_u8(marker, mask, recon_conn=8, fill_conn=4, thresh=128, ccl_conn=8, min_size=11, max_size=1000):
+    mask = _c(mask, np.uint8)
+    marker = _c(marker, np.uint8)
+    r = np.empty_like(mask)
+    f = np.empty_like(mask)
+    lab = np.empty(mask.shape, np.int32)
+    n = lib().or_chain_u8(_p(marker), _p(mask), mask.shape[0], mask.shape[1], recon_conn, fill_conn, int(thresh),
+                          ccl_conn, min_size, max_size, _p(r), _p(f), _p(lab))
+    return r, f, lab, int(n)
+
+
+# ---- the reference's own ConnComponents.cpp -------------------------------------------
+def ref_label(binimg, conn=8, bgval=-1):
+    r = ref()
+    b = _c(binimg, np.uint8).copy()
+    out = np.empty(b.shape, np.int32)
+    r.ref_label(_p(b), b.shape[1], b.shape[0], _p(out), bgval, conn)
+    return out
+
+
+def ref_relabel(labels, bgval=-1):
+    r = ref()
+    a = np.array(labels, dtype=np.int32, copy=True, order="C")
+    n = r.ref_relabel(a.shape[1], a.shape[0], _p(a), bgval)
+    return a, int(n)
+
+
+def ref_area_threshold_labeled(labels, lower, upper, bgval=-1):
+    r = ref()
+    a = _c(labels, np.int32)
+    out = np.empty_like(a)
+    n = r.ref_area_threshold_labeled(_p(a), a.shape[1], a.shape[0], _p(out), bgval, lower, upper)
+    return out, int(n)
+
+
+def ref_area_threshold(binimg, lower, upper, conn=8, bgval=-1):
+    r = ref()
+    b = _c(binimg, np.uint8).copy()
+    out = np.empty(b.shape, np.int32)
+    n = r.ref_area_threshold(_p(b), b.shape[1], b.shape[0], _p(out), bgval, lower, upper, conn)
+    return out, int(n)

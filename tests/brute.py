@@ -1,0 +1,119 @@
+"""Independent restatements used to pin the oracle (test helpers).
+
+Each one computes the same mathematical object as the reference by a
+different, textbook method, so that a mistake in the oracle's line-by-line
+restatement (a dropped neighbour, a wrong frame, an off-by-one in a scan)
+shows up as a mismatch.
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy import ndimage
+
+
+def _dilate(x: np.ndarray, conn: int) -> np.ndarray:
+    """Max over the conn-neighbourhood (self included) of a padded array; the
+    outermost ring of the result is not used."""
+    d = x.copy()
+    d[1:, :] = np.maximum(d[1:, :], x[:-1, :])
+    d[:-1, :] = np.maximum(d[:-1, :], x[1:, :])
+    d[:, 1:] = np.maximum(d[:, 1:], x[:, :-1])
+    d[:, :-1] = np.maximum(d[:, :-1], x[:, 1:])
+    if conn == 8:
+        d[1:, 1:] = np.maximum(d[1:, 1:], x[:-1, :-1])
+        d[1:, :-1] = np.maximum(d[1:, :-1], x[:-1, 1:])
+        d[:-1, 1:] = np.maximum(d[:-1, 1:], x[1:, :-1])
+        d[:-1, :-1] = np.maximum(d[:-1, :-1], x[1:, 1:])
+    return d
+
+
+def recon_fixpoint(marker: np.ndarray, mask: np.ndarray, conn: int, frame=0) -> np.ndarray:
+    """Geodesic dilation iterated to its fixed point with a constant frame:
+    X0 = min(marker, mask); X <- min(dilate(X), mask) until stable."""
+    dt = mask.dtype
+    mp = np.pad(mask, 1, constant_values=frame).astype(dt)
+    xp = np.pad(np.minimum(marker, mask), 1, constant_values=frame).astype(dt)
+    while True:
+        nx = np.minimum(_dilate(xp, conn), mp)
+        nx[0, :] = frame
+        nx[-1, :] = frame
+        nx[:, 0] = frame
+        nx[:, -1] = frame
+        if np.array_equal(nx, xp):
+            return xp[1:-1, 1:-1]
+        xp = nx
+
+
+def structure(conn: int) -> np.ndarray:
+    return np.ones((3, 3), bool) if conn == 8 else ndimage.generate_binary_structure(2, 1)
+
+
+def recon_binary_bfs(marker: np.ndarray, mask: np.ndarray, conn: int) -> np.ndarray:
+    """Binary reconstruction by flood fill: components of mask != 0 that hold a
+    pixel with marker != 0 and mask != 0 become max(T); nonzero marker pixels
+    keep their value."""
+    s = mask != 0
+    lab, _ = ndimage.label(s, structure=structure(conn))
+    seeds = np.unique(lab[(marker != 0) & s])
+    seeds = seeds[seeds > 0]
+    reached = np.isin(lab, seeds) & s
+    tmax = np.iinfo(mask.dtype).max
+    out = np.where(reached, tmax, 0).astype(mask.dtype)
+    return np.where(marker != 0, marker, out).astype(mask.dtype)
+
+
+def fill_holes_binary_scipy(bw: np.ndarray, conn: int) -> np.ndarray:
+    """MATLAB imfill(BW, 'holes') with background connectivity conn."""
+    filled = ndimage.binary_fill_holes(bw != 0, structure=structure(conn))
+    return np.where(filled, 255, 0).astype(np.uint8)
+
+
+def fill_holes_gray_u8(img: np.ndarray, conn: int) -> np.ndarray:
+    """255 - R(marker = 255 on a frame ring, mask = 255 - img) by fixed point."""
+    inv = (255 - img.astype(np.int32)).astype(np.uint8)
+    mk = np.pad(inv, 1, constant_values=255)
+    mr = np.zeros_like(mk)
+    mr[0, :] = mr[-1, :] = mr[:, 0] = mr[:, -1] = 255
+    r = recon_fixpoint(mr, mk, conn, frame=0)
+    return (255 - r[1:-1, 1:-1].astype(np.int32)).astype(np.uint8)
+
+
+def canonical_labels(img: np.ndarray, conn: int) -> np.ndarray:
+    """Union of equal-valued nonzero neighbours, numbered 1.. by the raster
+    order of each component's first pixel (scipy labels per value, then
+    renumbered)."""
+    out = np.zeros(img.shape, np.int64)
+    nxt = 0
+    for v in np.unique(img):
+        if v == 0:
+            continue
+        lab, n = ndimage.label(img == v, structure=structure(conn))
+        out[lab > 0] = lab[lab > 0] + nxt
+        nxt += n
+    flat = out.ravel()
+    order = {}
+    res = np.zeros(flat.shape, np.int32)
+    k = 0
+    for i, l in enumerate(flat):
+        if l == 0:
+            continue
+        if l not in order:
+            k += 1
+            order[l] = k
+        res[i] = order[l]
+    return res.reshape(img.shape)
+
+
+def root_labels_from_canonical(canon: np.ndarray) -> np.ndarray:
+    """Root-index labels (raster index of each component's first pixel, -1 for
+    background) from canonical labels."""
+    flat = canon.ravel()
+    first = {}
+    for i, l in enumerate(flat):
+        if l and l not in first:
+            first[l] = i
+    out = np.full(flat.shape, -1, np.int32)
+    for i, l in enumerate(flat):
+        if l:
+            out[i] = first[l]
+    return out.reshape(canon.shape)

@@ -1,0 +1,198 @@
+"""HIP path vs the oracle, bit-exact, through the C ABI (nscale_hip)."""
+import numpy as np
+import pytest
+
+import brute
+
+pytestmark = pytest.mark.gpu
+
+# tile edges of the reconstruction (62) and labelling (64 x 32) kernels
+SHAPES = [(1, 1), (1, 130), (130, 1), (3, 5), (61, 62), (62, 62), (63, 64), (64, 65), (96, 33), (124, 125),
+          (130, 300), (257, 190)]
+
+
+def rand_case(rng, shape, dtype, hi=12, p=0.1):
+    mask = rng.integers(0, hi, size=shape).astype(dtype)
+    marker = np.where(rng.random(shape) < p, rng.integers(0, hi, size=shape), 0).astype(dtype)
+    return marker, mask
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_recon_u8_random(nh, orc, gpu, shape, conn):
+    rng = np.random.default_rng(hash((shape, conn)) & 0xFFFFF)
+    for _ in range(3):
+        marker, mask = rand_case(rng, shape, np.uint8)
+        got = nh.imreconstruct(marker, mask, conn)
+        assert np.array_equal(got, orc.imreconstruct(marker, mask, conn))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_recon_u8_c0_disks(nh, orc, gpu, conn):
+    """512x512, 200 disks + noise, marker = mask - 30."""
+    mask = nh.gen_disks_u8(512, 512, 200, 0x5EED0000)
+    marker = nh.gen_marker_u8(mask, 30)
+    got = nh.imreconstruct(marker, mask, conn)
+    assert np.array_equal(got, orc.imreconstruct(marker, mask, conn))
+
+
+def test_recon_u8_marker_above_mask(nh, orc, gpu):
+    rng = np.random.default_rng(1)
+    mask = rng.integers(0, 256, size=(200, 150)).astype(np.uint8)
+    marker = rng.integers(0, 256, size=(200, 150)).astype(np.uint8)
+    for conn in (4, 8):
+        assert np.array_equal(nh.imreconstruct(marker, mask, conn), orc.imreconstruct(marker, mask, conn))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_recon_i32_f32(nh, orc, gpu, conn):
+    rng = np.random.default_rng(2 + conn)
+    for shape in [(63, 64), (130, 257)]:
+        m = rng.integers(-3000, 3000, size=shape).astype(np.int32)
+        k = np.where(rng.random(shape) < 0.2, rng.integers(-5000, 3000, size=shape), -5000).astype(np.int32)
+        assert np.array_equal(nh.imreconstruct(k, m, conn), orc.imreconstruct(k, m, conn))
+        mf = (m.astype(np.float32) / np.float32(7.0))
+        kf = (k.astype(np.float32) / np.float32(7.0))
+        mf[mf == 0] = np.float32(0.0)
+        kf[kf == 0] = np.float32(0.0)
+        assert np.array_equal(nh.imreconstruct(kf, mf, conn), orc.imreconstruct(kf, mf, conn))
+
+
+def test_known_answer_32x32(nh, gpu):
+    from test_oracle import fixture_32x32
+    img, seeds = fixture_32x32()
+    out = nh.imreconstruct(seeds, img, 4)
+    expect = img.copy()
+    expect[20:, :] = 0
+    assert np.array_equal(out, expect)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int32])
+@pytest.mark.parametrize("conn", [4, 8])
+def test_recon_binary(nh, orc, gpu, dtype, conn):
+    rng = np.random.default_rng(5 + conn)
+    for shape in [(1, 1), (62, 63), (150, 131)]:
+        mask = ((rng.random(shape) < 0.55) * rng.integers(1, 4, size=shape)).astype(dtype)
+        marker = ((rng.random(shape) < 0.03) * rng.integers(1, 9, size=shape)).astype(dtype)
+        got = nh.imreconstructBinary(marker, mask, conn)
+        assert np.array_equal(got, orc.imreconstruct_binary(marker, mask, conn))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("binary", [False, True])
+def test_fill_holes_u8(nh, orc, gpu, conn, binary):
+    rng = np.random.default_rng(7 + conn + 2 * binary)
+    for shape in [(1, 1), (5, 70), (62, 62), (128, 190)]:
+        bw = np.where(rng.random(shape) < 0.6, 255, 0).astype(np.uint8)
+        assert np.array_equal(nh.imfillHoles(bw, binary, conn), orc.fill_holes(bw, binary, conn))
+        g = rng.integers(0, 256, size=shape).astype(np.uint8)
+        assert np.array_equal(nh.imfillHoles(g, binary, conn), orc.fill_holes(g, binary, conn))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_fill_holes_i32(nh, orc, gpu, conn):
+    rng = np.random.default_rng(17 + conn)
+    g = rng.integers(0, 65536, size=(100, 77)).astype(np.int32)
+    assert np.array_equal(nh.imfillHoles(g, False, conn), orc.fill_holes(g, False, conn))
+    assert np.array_equal(nh.imfillHoles(g, True, conn), orc.fill_holes(g, True, conn))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("shape", [(1, 1), (1, 200), (200, 1), (31, 63), (32, 64), (33, 65), (97, 130), (300, 257)])
+def test_bwlabel(nh, orc, gpu, shape, conn):
+    rng = np.random.default_rng(hash((shape, conn, 1)) & 0xFFFFF)
+    for p in (0.3, 0.6, 0.9):
+        img = np.where(rng.random(shape) < p, rng.integers(1, 3, size=shape), 0).astype(np.uint8)
+        for relab in (False, True):
+            got, n = nh.bwlabel2(img, conn, relab, return_count=True)
+            want, wn = orc.bwlabel2(img, conn, relab)
+            assert np.array_equal(got, want)
+            assert n == wn
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_bwareaopen(nh, orc, gpu, conn):
+    rng = np.random.default_rng(23 + conn)
+    for shape in [(64, 64), (130, 77)]:
+        img = np.where(rng.random(shape) < 0.5, 255, 0).astype(np.uint8)
+        for lo, hi in [(1, 1 << 30), (3, 12), (11, 1000)]:
+            for flatten in (False, True):
+                got, n = nh.bwareaopen2(img, False, flatten, lo, hi, conn)
+                want, wn = orc.bwareaopen2(img, False, flatten, lo, hi, conn)
+                assert np.array_equal(got, want) and n == wn
+            lab, _ = orc.bwlabel2(img, conn, relab=False)
+            got, n = nh.bwareaopen2(lab, True, False, lo, hi, conn)
+            want, wn = orc.bwareaopen2(lab, True, False, lo, hi, conn)
+            assert np.array_equal(got, want) and n == wn
+
+
+def test_relabel(nh, orc, gpu):
+    rng = np.random.default_rng(29)
+    img = np.where(rng.random((90, 140)) < 0.5, 255, 0).astype(np.uint8)
+    lab, _ = orc.bwlabel2(img, 8, relab=False)
+    got, n = nh.relabel(lab.copy(), -1)
+    want, wn = orc.relabel(lab, -1)
+    assert np.array_equal(got, want) and n == wn
+
+
+def test_threshold(nh, gpu):
+    rng = np.random.default_rng(31)
+    a = rng.integers(0, 256, size=(70, 90)).astype(np.uint8)
+    assert np.array_equal(nh.threshold_gt(a, 128), np.where(a > 128, 255, 0).astype(np.uint8))
+
+
+@pytest.mark.parametrize("shape,nd", [((256, 320), 60), ((512, 512), 200), ((700, 530), 300)])
+def test_chain_device(nh, orc, gpu, shape, nd):
+    import torch
+    mask = nh.gen_disks_u8(shape[0], shape[1], nd, 0x5EED0000 + nd)
+    marker = nh.gen_marker_u8(mask, 30)
+    r, f, lab, n = nh.segment_chain(torch.from_numpy(marker).cuda(), torch.from_numpy(mask).cuda())
+    er, ef, elab, en = orc.chain_u8(marker, mask)
+    assert np.array_equal(r.cpu().numpy(), er)
+    assert np.array_equal(f.cpu().numpy(), ef)
+    assert np.array_equal(lab.cpu().numpy(), elab)
+    assert n == en
+
+
+def test_device_tensors_and_pitch(nh, orc, gpu):
+    """Non-continuous device rows (a column slice keeps the parent's pitch)."""
+    import torch
+    mask = nh.gen_disks_u8(300, 400, 80, 99)
+    marker = nh.gen_marker_u8(mask, 30)
+    big_m = torch.zeros((300, 512), dtype=torch.uint8, device="cuda")
+    big_k = torch.zeros((300, 512), dtype=torch.uint8, device="cuda")
+    big_m[:, 50:450] = torch.from_numpy(mask).cuda()
+    big_k[:, 50:450] = torch.from_numpy(marker).cuda()
+    out = torch.zeros((300, 600), dtype=torch.uint8, device="cuda")[:, 100:500]
+    nh.imreconstruct(big_k[:, 50:450], big_m[:, 50:450], 8, out=out)
+    assert np.array_equal(out.cpu().numpy(), orc.imreconstruct(marker, mask, 8))
+
+
+def test_errors(nh, gpu):
+    with pytest.raises(nh.NsxError):
+        nh.imreconstruct(np.zeros((3, 4), np.uint8), np.zeros((4, 3), np.uint8))
+    with pytest.raises(nh.NsxError):
+        nh.bwlabel2(np.zeros((3, 4), np.int32))
+    with pytest.raises(nh.NsxError):
+        nh.imfillHoles(np.zeros((3, 4), np.float32))
+
+
+def test_golden_fixtures(nh, gpu):
+    from pathlib import Path
+    g = np.load(Path(__file__).parent / "golden" / "small_cases.npz")
+    mask, marker = g["disk_mask"], g["disk_marker"]
+    for conn in (4, 8):
+        r = nh.imreconstruct(marker, mask, conn)
+        assert np.array_equal(r, g[f"recon_u8_c{conn}"])
+        assert np.array_equal(nh.imreconstruct(marker.astype(np.int32) * 257, mask.astype(np.int32) * 257, conn),
+                              g[f"recon_i32_c{conn}"])
+        assert np.array_equal(nh.imreconstruct(marker.astype(np.float32) / np.float32(255),
+                                               mask.astype(np.float32) / np.float32(255), conn),
+                              g[f"recon_f32_c{conn}"])
+        assert np.array_equal(nh.imfillHoles(r, False, conn), g[f"fill_gray_c{conn}"])
+        assert np.array_equal(nh.imfillHoles(r, True, conn), g[f"fill_bin_c{conn}"])
+        assert np.array_equal(nh.imreconstructBinary(g["bin_seed"], g["bin_mask"], conn), g[f"recon_bin_c{conn}"])
+        assert np.array_equal(nh.bwlabel2(g["bin_mask"], conn, False), g[f"label_root_c{conn}"])
+        assert np.array_equal(nh.bwlabel2(g["bin_mask"], conn, True), g[f"label_canon_c{conn}"])
+        got, _ = nh.bwareaopen2(g["bin_mask"], False, False, 3, 12, conn)
+        assert np.array_equal(got, g[f"areaopen_c{conn}"])

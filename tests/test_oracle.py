@@ -1,0 +1,229 @@
+"""Pins the oracle (oracle/nscale_oracle.cpp) before anything is checked
+against it: brute-force fixed points, textbook flood fills, scipy labelling,
+the reference's own compiled ConnComponents.cpp, the 32x32 known-answer
+fixture of test/segment/cuda/globalQueueTest.cu:446-583, and closed forms."""
+import numpy as np
+import pytest
+
+import brute
+
+SIZES = [(1, 1), (1, 7), (9, 1), (2, 2), (5, 8), (13, 11), (31, 33), (40, 64)]
+
+
+def rand_pair(rng, shape, dtype, lo=0, hi=12):
+    mask = rng.integers(lo, hi, size=shape).astype(dtype)
+    marker = rng.integers(lo, hi, size=shape).astype(dtype)
+    marker = np.where(rng.random(shape) < 0.15, marker, lo).astype(dtype)
+    return marker, mask
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("shape", SIZES)
+def test_recon_u8_vs_fixpoint(orc, conn, shape):
+    rng = np.random.default_rng(hash((conn,) + shape) & 0xFFFF)
+    for _ in range(6):
+        marker, mask = rand_pair(rng, shape, np.uint8)
+        got = orc.imreconstruct(marker, mask, conn)
+        assert np.array_equal(got, brute.recon_fixpoint(marker, mask, conn))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_recon_marker_above_mask_and_noise(orc, conn):
+    rng = np.random.default_rng(7)
+    mask = rng.integers(0, 256, size=(37, 29)).astype(np.uint8)
+    marker = rng.integers(0, 256, size=(37, 29)).astype(np.uint8)  # often > mask
+    assert np.array_equal(orc.imreconstruct(marker, mask, conn), brute.recon_fixpoint(marker, mask, conn))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_recon_i32_negative_values_see_zero_frame(orc, conn):
+    # negative masks: the constant 0 frame raises border pixels to their mask
+    rng = np.random.default_rng(11)
+    mask = rng.integers(-50, 50, size=(23, 19)).astype(np.int32)
+    marker = rng.integers(-80, 50, size=(23, 19)).astype(np.int32)
+    got = orc.imreconstruct(marker, mask, conn)
+    assert np.array_equal(got, brute.recon_fixpoint(marker, mask, conn, frame=0))
+    # a wrong frame value would give a different answer
+    assert not np.array_equal(got, brute.recon_fixpoint(marker, mask, conn, frame=np.int32(-1000)))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_recon_order_isomorphism(orc, conn):
+    """R(f(m), f(M)) = f(R(m, M)) for strictly increasing f with f(0)=0: the
+    u8 result maps exactly onto i32 (x257) and f32 (/255) results."""
+    mask = np.zeros((64, 80), np.uint8)
+    rng = np.random.default_rng(3)
+    mask[:] = rng.integers(0, 30, size=mask.shape)
+    marker = np.clip(mask.astype(int) - 10, 0, 255).astype(np.uint8)
+    r8 = orc.imreconstruct(marker, mask, conn)
+    r32 = orc.imreconstruct(marker.astype(np.int32) * 257, mask.astype(np.int32) * 257, conn)
+    rf = orc.imreconstruct(marker.astype(np.float32) / np.float32(255), mask.astype(np.float32) / np.float32(255), conn)
+    assert np.array_equal(r32, r8.astype(np.int32) * 257)
+    assert np.array_equal(rf, r8.astype(np.float32) / np.float32(255))
+
+
+def test_recon_closed_forms(orc):
+    rng = np.random.default_rng(5)
+    mask = rng.integers(0, 200, size=(30, 40)).astype(np.uint8)
+    for conn in (4, 8):
+        assert np.array_equal(orc.imreconstruct(mask, mask, conn), mask)          # marker == mask
+        assert np.array_equal(orc.imreconstruct(np.zeros_like(mask), mask, conn), np.zeros_like(mask))
+        r = orc.imreconstruct(np.clip(mask.astype(int) - 20, 0, 255).astype(np.uint8), mask, conn)
+        assert np.array_equal(orc.imreconstruct(r, mask, conn), r)                # idempotent
+
+
+def fixture_32x32():
+    """test/segment/cuda/globalQueueTest.cu:446-583: a 0/1 mask with three blobs
+    and seeds at (8,8) and (4,24); the blob at rows 22-25 has no seed."""
+    img = np.zeros((32, 32), np.uint8)
+    rows = {4: (22, 29), 5: (22, 29), 6: [(7, 10), (22, 24), (28, 29)], 7: [(8, 11), (22, 24), (28, 29)],
+            8: [(7, 10), (22, 24), (27, 29)], 9: [(6, 9), (22, 29)], 10: [(8, 11), (22, 29)], 11: (10, 13),
+            22: (14, 17), 23: (15, 18), 24: (13, 16), 25: (15, 18)}
+    for r, spans in rows.items():
+        if isinstance(spans, tuple):
+            spans = [spans]
+        for a, b in spans:
+            img[r, a:b + 1] = 1
+    seeds = np.zeros((32, 32), np.uint8)
+    seeds[4, 24] = 1
+    seeds[8, 8] = 1
+    return img, seeds
+
+
+def test_known_answer_32x32(orc):
+    img, seeds = fixture_32x32()
+    out = orc.imreconstruct(seeds, img, 4)
+    expect = img.copy()
+    expect[20:, :] = 0          # unseeded blob stays 0
+    assert np.array_equal(out, expect)
+    assert out[22:26].sum() == 0 and out[4:12].sum() == img[4:12].sum()
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int32])
+@pytest.mark.parametrize("conn", [4, 8])
+def test_recon_binary_vs_bfs(orc, dtype, conn):
+    rng = np.random.default_rng(21 + conn)
+    for shape in [(1, 1), (6, 9), (33, 31), (50, 64)]:
+        for _ in range(4):
+            mask = (rng.random(shape) < 0.55).astype(dtype) * rng.integers(1, 4, size=shape).astype(dtype)
+            marker = ((rng.random(shape) < 0.05) * rng.integers(1, 9, size=shape)).astype(dtype)
+            got = orc.imreconstruct_binary(marker, mask, conn)
+            assert np.array_equal(got, brute.recon_binary_bfs(marker, mask, conn))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_fill_holes_binary_images(orc, conn):
+    rng = np.random.default_rng(31 + conn)
+    for shape in [(1, 1), (3, 3), (20, 17), (48, 70)]:
+        for _ in range(4):
+            bw = np.where(rng.random(shape) < 0.6, 255, 0).astype(np.uint8)
+            want = brute.fill_holes_binary_scipy(bw, conn)
+            assert np.array_equal(orc.fill_holes(bw, False, conn), want)
+            assert np.array_equal(orc.fill_holes(bw, True, conn), want)
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_fill_holes_gray_u8(orc, conn):
+    rng = np.random.default_rng(41 + conn)
+    for shape in [(1, 1), (7, 5), (30, 41)]:
+        img = rng.integers(0, 256, size=shape).astype(np.uint8)
+        got = orc.fill_holes(img, False, conn)
+        assert np.array_equal(got, brute.fill_holes_gray_u8(img, conn))
+        assert (got >= img).all()
+        assert np.array_equal(orc.fill_holes(got, False, conn), got)
+
+
+def test_fill_holes_binary_treats_only_255_as_wall(orc):
+    img = np.zeros((7, 7), np.uint8)
+    img[1:6, 1:6] = 255
+    img[2:5, 2:5] = 128          # non-255 inside a 255 ring: a hole
+    img[3, 3] = 0
+    got = orc.fill_holes(img, True, 4)
+    assert (got[1:6, 1:6] == 255).all() and got[0].sum() == 0
+
+
+def test_fill_holes_i32(orc):
+    rng = np.random.default_rng(9)
+    img = rng.integers(0, 65536, size=(25, 31)).astype(np.int32)
+    for conn in (4, 8):
+        got = orc.fill_holes(img, False, conn)
+        # -R(-I) from the frame, by fixed point
+        inv = -img
+        mk = np.pad(inv, 1, constant_values=np.iinfo(np.int32).max)
+        mr = np.full_like(mk, np.iinfo(np.int32).min)
+        mr[0, :] = mr[-1, :] = mr[:, 0] = mr[:, -1] = np.iinfo(np.int32).max
+        r = brute.recon_fixpoint(mr, mk, conn, frame=0)[1:-1, 1:-1]
+        assert np.array_equal(got, -r)
+        # binary int fill: the int marker interior min<int> is nonzero and kept
+        assert (orc.fill_holes(img, True, conn) == np.iinfo(np.int32).max).all()
+
+
+def test_bwlabel_vs_reference_and_scipy(orc):
+    if orc.ref() is None:
+        pytest.skip("reference ConnComponents not built here")
+    rng = np.random.default_rng(51)
+    for shape in [(1, 1), (1, 9), (8, 1), (17, 23), (64, 64), (65, 130)]:
+        for p in (0.3, 0.55, 0.8):
+            for conn in (4, 8):
+                img = np.where(rng.random(shape) < p, rng.integers(1, 3, size=shape), 0).astype(np.uint8)
+                lab, _ = orc.bwlabel2(img, conn, relab=False)
+                assert np.array_equal(lab, orc.ref_label(img, conn))
+                can, n = orc.bwlabel2(img, conn, relab=True)
+                rcan, rn = orc.ref_relabel(orc.ref_label(img, conn))
+                assert np.array_equal(can, rcan) and n == rn
+                assert np.array_equal(can, brute.canonical_labels(img, conn))
+                assert np.array_equal(lab, brute.root_labels_from_canonical(can))
+
+
+def test_bwareaopen_vs_reference(orc):
+    if orc.ref() is None:
+        pytest.skip("reference ConnComponents not built here")
+    rng = np.random.default_rng(61)
+    for shape in [(10, 10), (48, 77)]:
+        for conn in (4, 8):
+            img = np.where(rng.random(shape) < 0.5, 255, 0).astype(np.uint8)
+            for lo, hi in [(1, 1 << 30), (3, 12), (11, 1000), (2, 3)]:
+                out, n = orc.bwareaopen2(img, False, False, lo, hi, conn)
+                ref_out, ref_n = orc.ref_area_threshold_labeled(orc.ref_label(img, conn), lo, hi)
+                assert np.array_equal(out, ref_out) and n == ref_n
+                # areaThreshold (the other reference variant) agrees
+                rr, rn = orc.ref_area_threshold(img, lo, hi, conn)
+                assert np.array_equal(out, rr) and n == rn
+                flat, n2 = orc.bwareaopen2(img, False, True, lo, hi, conn)
+                assert np.array_equal(flat, np.where(out > -1, 255, 0)) and n2 == n
+                lab_in, _ = orc.bwlabel2(img, conn, relab=False)
+                out2, n3 = orc.bwareaopen2(lab_in, True, False, lo, hi, conn)
+                assert np.array_equal(out2, out) and n3 == n
+
+
+def test_chain_composition(orc):
+    mask = np.zeros((60, 70), np.uint8)
+    rng = np.random.default_rng(71)
+    mask[:] = rng.integers(0, 9, size=mask.shape)
+    mask[10:30, 10:30] = 200
+    mask[15:20, 15:20] = 40       # a hole
+    mask[40:45, 50:53] = 180
+    marker = np.clip(mask.astype(int) - 30, 0, 255).astype(np.uint8)
+    r, f, lab, n = orc.chain_u8(marker, mask)
+    assert np.array_equal(r, orc.imreconstruct(marker, mask, 8))
+    assert np.array_equal(f, orc.fill_holes(r, False, 4))
+    b = np.where(f > 128, 255, 0).astype(np.uint8)
+    filt, _ = orc.bwareaopen2(b, False, False, 11, 1000, 8)
+    can, n2 = orc.relabel(filt)
+    assert np.array_equal(lab, can) and n == n2 == 2
+
+
+def test_golden_fixtures(orc):
+    """The committed fixtures are reproduced by the oracle."""
+    from pathlib import Path
+    g = np.load(Path(__file__).parent / "golden" / "small_cases.npz")
+    mask, marker = g["disk_mask"], g["disk_marker"]
+    for conn in (4, 8):
+        r = orc.imreconstruct(marker, mask, conn)
+        assert np.array_equal(r, g[f"recon_u8_c{conn}"])
+        assert np.array_equal(r, brute.recon_fixpoint(marker, mask, conn))
+        assert np.array_equal(orc.fill_holes(r, False, conn), g[f"fill_gray_c{conn}"])
+        assert np.array_equal(orc.fill_holes(r, True, conn), g[f"fill_bin_c{conn}"])
+        assert np.array_equal(orc.imreconstruct_binary(g["bin_seed"], g["bin_mask"], conn), g[f"recon_bin_c{conn}"])
+        assert np.array_equal(orc.bwlabel2(g["bin_mask"], conn, False)[0], g[f"label_root_c{conn}"])
+        assert np.array_equal(orc.bwlabel2(g["bin_mask"], conn, True)[0], g[f"label_canon_c{conn}"])
