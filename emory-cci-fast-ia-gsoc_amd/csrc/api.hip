@@ -43,6 +43,39 @@ nsx_status nsx_ctx::ws(int slot, size_t bytes, void** out) {
     return NSX_OK;
 }
 
+int nsx_ctx::tic(int kind, hipStream_t s) {
+    if (!timing) return -1;
+    auto take = [&]() {
+        hipEvent_t e;
+        if (!ev_pool.empty()) { e = ev_pool.back(); ev_pool.pop_back(); return e; }
+        if (hipEventCreate(&e) != hipSuccess) return (hipEvent_t) nullptr;
+        return e;
+    };
+    EvPair p{take(), take(), kind};
+    if (!p.a || !p.b) return -1;
+    (void)hipEventRecord(p.a, s);
+    ev_pending.push_back(p);
+    return (int)ev_pending.size() - 1;
+}
+
+void nsx_ctx::toc(int handle, hipStream_t s) {
+    if (handle < 0 || handle >= (int)ev_pending.size()) return;
+    (void)hipEventRecord(ev_pending[handle].b, s);
+}
+
+void nsx_ctx::collect_timing() {
+    for (auto& p : ev_pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            t_ms[p.kind] += ms;
+            t_n[p.kind] += 1;
+        }
+        ev_pool.push_back(p.a);
+        ev_pool.push_back(p.b);
+    }
+    ev_pending.clear();
+}
+
 namespace {
 
 const char kVersion[] = "nscale-hip 0.1 (gfx950)";
@@ -110,10 +143,32 @@ bool overlaps(const nsx_img* a, const nsx_img* b) {
     return a0 < b1 && b0 < a1;
 }
 
+// Synchronises s and reports a reconstruction whose device queue gave up
+// (sticky error word d_stats[3], cleared here).
+nsx_status check_converged(nsx_ctx* ctx, hipStream_t s) {
+    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 16, ctx->d_stats + 3, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+    int64_t err;
+    std::memcpy(&err, ctx->h_pinned + 16, sizeof(err));
+    if (err != 0) {
+        NSX_HIP_TRY(ctx, hipMemsetAsync(ctx->d_stats + 3, 0, sizeof(int64_t), s));
+        NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+        return ctx->fail(NSX_ERR_NOT_CONVERGED, "reconstruction work queue stalled (error bits " +
+                                                    std::to_string(err) + ")");
+    }
+    return NSX_OK;
+}
+
 nsx_status read_count(nsx_ctx* ctx, const int32_t* dev, int32_t* host, hipStream_t s) {
     NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 8, dev, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+    NSX_TRY(check_converged(ctx, s));
     if (host) *host = ctx->h_pinned[8];
+    return NSX_OK;
+}
+
+nsx_status finish_recon(nsx_ctx* ctx, const View& v, const nsx_img* out, hipStream_t s) {
+    NSX_TRY(from_device(ctx, v, out, s));
+    if (out->mem == NSX_MEM_HOST) NSX_TRY(check_converged(ctx, s));
     return NSX_OK;
 }
 
@@ -161,6 +216,8 @@ void nsx_ctx_destroy(nsx_ctx* ctx) {
         if (b.ptr) hipFree(b.ptr);
     if (ctx->d_stats) hipFree(ctx->d_stats);
     if (ctx->h_pinned) hipHostFree(ctx->h_pinned);
+    ctx->collect_timing();
+    for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -174,6 +231,26 @@ nsx_status nsx_ctx_synchronize(nsx_ctx* ctx) {
 }
 
 size_t nsx_ctx_workspace_bytes(const nsx_ctx* ctx) { return ctx ? ctx->held : 0; }
+
+nsx_status nsx_ctx_set_timing(nsx_ctx* ctx, int enable) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->collect_timing();
+    for (int k = 0; k < TK_NKINDS; ++k) { ctx->t_ms[k] = 0.0; ctx->t_n[k] = 0; }
+    ctx->timing = enable != 0;
+    return NSX_OK;
+}
+
+nsx_status nsx_ctx_read_timing(nsx_ctx* ctx, double* ms, int64_t* launches, int nkinds) {
+    if (!ctx || nkinds < 0) return NSX_ERR_INVALID;
+    NSX_HIP_TRY(ctx, hipDeviceSynchronize());
+    ctx->collect_timing();
+    for (int k = 0; k < nkinds && k < TK_NKINDS; ++k) {
+        if (ms) ms[k] = ctx->t_ms[k];
+        if (launches) launches[k] = ctx->t_n[k];
+    }
+    return NSX_OK;
+}
 
 nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int64_t stats[3]) {
     if (!ctx || !stats) return NSX_ERR_INVALID;
@@ -203,7 +280,7 @@ nsx_status nsx_imreconstruct(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
     NSX_TRY(to_device(ctx, mask, WS_STAGE_B, true, s, &vk));
     NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
     NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vo, conn == 8 ? 8 : 4, s));
-    return from_device(ctx, vo, out, s);
+    return finish_recon(ctx, vo, out, s);
 }
 
 nsx_status nsx_imreconstruct_binary(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* mask, nsx_img* out,
@@ -225,7 +302,7 @@ nsx_status nsx_imreconstruct_binary(nsx_ctx* ctx, const nsx_img* marker, const n
     NSX_TRY(to_device(ctx, mask, WS_STAGE_B, true, s, &vk));
     NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
     NSX_TRY(recon_run(ctx, RM_BINARY, &vm, vk, vo, conn == 8 ? 8 : 4, s));
-    return from_device(ctx, vo, out, s);
+    return finish_recon(ctx, vo, out, s);
 }
 
 nsx_status nsx_fill_holes(nsx_ctx* ctx, const nsx_img* in, nsx_img* out, int binary, int conn, void* stream) {
@@ -247,7 +324,7 @@ nsx_status nsx_fill_holes(nsx_ctx* ctx, const nsx_img* in, nsx_img* out, int bin
     } else {
         NSX_TRY(recon_run(ctx, binary ? RM_FILLBIN : RM_FILL, nullptr, vi, vo, conn == 8 ? 8 : 4, s));
     }
-    return from_device(ctx, vo, out, s);
+    return finish_recon(ctx, vo, out, s);
 }
 
 nsx_status nsx_bwlabel(nsx_ctx* ctx, const nsx_img* bin, nsx_img* labels, int conn, int relabel, int32_t* ncomp,

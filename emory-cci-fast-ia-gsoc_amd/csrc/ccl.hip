@@ -381,8 +381,13 @@ nsx_status ccl_run_conn(nsx_ctx* ctx, CclArgs A, hipStream_t s) {
     const dim3 tiles((A.cols + CW - 1) / CW, (A.rows + CH - 1) / CH);
     const int stream_grid = 4 * ctx->cu_count;
     hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, s, A.nentries, 2);
+    int h = ctx->tic(TK_CCL_LOCAL, s);
     hipLaunchKernelGGL(ccl_local_kernel<CONN>, tiles, dim3(CTHREADS), 0, s, A);
+    ctx->toc(h, s);
+    h = ctx->tic(TK_CCL_MERGE, s);
     hipLaunchKernelGGL(ccl_merge_kernel<CONN>, tiles, dim3(128), 0, s, A);
+    ctx->toc(h, s);
+    h = ctx->tic(TK_CCL_SMALL, s);
     if (A.use_area) hipLaunchKernelGGL(ccl_area_kernel, dim3(stream_grid), dim3(256), 0, s, A);
     hipLaunchKernelGGL(ccl_keep_kernel, dim3(stream_grid), dim3(256), 0, s, A);
     if (A.out_mode == CO_CANON) {
@@ -390,7 +395,10 @@ nsx_status ccl_run_conn(nsx_ctx* ctx, CclArgs A, hipStream_t s) {
         hipLaunchKernelGGL(scan_words_kernel, dim3(nb), dim3(256), 0, s, A.bits, A.nwords, A.wordpre, A.blockpre);
         hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, s, A.blockpre, nb);
     }
+    ctx->toc(h, s);
+    h = ctx->tic(TK_CCL_FINAL, s);
     hipLaunchKernelGGL(ccl_final_kernel<CONN>, tiles, dim3(CTHREADS), 0, s, A);
+    ctx->toc(h, s);
     NSX_HIP_TRY(ctx, hipGetLastError());
     return NSX_OK;
 }

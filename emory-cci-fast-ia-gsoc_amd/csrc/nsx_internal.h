@@ -42,6 +42,19 @@ struct DevBuf {
 
 }  // namespace nsx
 
+namespace nsx {
+// Kernel kinds timed with HIP events when a context's timing is on.
+enum TimedKind {
+    TK_RECON_FIRST = 0,   // first full pass of a reconstruction (all tiles)
+    TK_RECON_ROUND,       // one later propagation round / drain launch
+    TK_CCL_LOCAL,         // tile-local labelling
+    TK_CCL_MERGE,         // tile-edge union
+    TK_CCL_FINAL,         // relabel + output write
+    TK_CCL_SMALL,         // area / keep / scan kernels (one event pair per call)
+    TK_NKINDS
+};
+}  // namespace nsx
+
 struct nsx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -53,12 +66,26 @@ struct nsx_ctx {
     int32_t* h_pinned = nullptr;     // small pinned readback area
     int cu_count = 256;
 
+    // HIP-event kernel timing (off by default).
+    bool timing = false;
+    struct EvPair { hipEvent_t a, b; int kind; };
+    std::vector<EvPair> ev_pending;
+    std::vector<hipEvent_t> ev_pool;
+    double t_ms[nsx::TK_NKINDS] = {};
+    int64_t t_n[nsx::TK_NKINDS] = {};
+
     nsx_status fail(nsx_status code, const std::string& msg) {
         last_error = msg;
         return code;
     }
     // Returns a device buffer of at least `bytes` in workspace slot `slot`.
     nsx_status ws(int slot, size_t bytes, void** out);
+
+    // Event bracket around launches of `kind` on stream s (no-op when timing off).
+    int tic(int kind, hipStream_t s);
+    void toc(int handle, hipStream_t s);
+    // Folds completed event pairs into t_ms / t_n (call after a stream sync).
+    void collect_timing();
 };
 
 namespace nsx {
@@ -97,8 +124,24 @@ enum ReconMode { RM_RECON = 0, RM_BINARY = 1, RM_FILL = 2, RM_FILLBIN = 3 };
 
 // Runs reconstruction in `mode` on device views (marker may be null for the
 // fill modes).  `state` is the output buffer and holds the propagation state.
+// Enqueues work only (no host synchronisation); errors of the device queue
+// are reported in ctx->d_stats[0] (nonzero = did not converge).
 nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& state,
                      int conn, hipStream_t s);
+
+// Several images of one dtype/mode/connectivity in one launch pair.
+struct ReconBatch {
+    int nimg = 0;
+    int ntiles = 0;                 // total tiles over images
+    const void* host_desc = nullptr;  // the descriptor when nimg == 1
+    const void* dev_desc = nullptr;   // device array of nimg descriptors
+    const int32_t* tile_img = nullptr;  // device: tile -> image index (nimg > 1)
+    int ws_base = 0;                // first of 3 consecutive workspace slots
+};
+int recon_tiles(int rows, int cols);
+size_t recon_desc_bytes();
+void recon_fill_desc(void* out, const View* marker, const View& mask, const View& state, int tile_base);
+nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const ReconBatch& B, hipStream_t s);
 
 // ---- connected components ---------------------------------------------------------
 enum CclOut {

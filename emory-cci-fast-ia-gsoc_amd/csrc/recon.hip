@@ -22,11 +22,20 @@
 //     clamp: new = med3(neighbour max, cur, mask) since cur <= mask.
 //   * Halo pixels are computed with the tile (their raised values are valid
 //     lower bounds) but never written; when one rises, the tile that owns it is
-//     queued for the next round.  Interior pixels are written back.
-//   * Rounds repeat over a device-side list of queued tiles (atomic append with
-//     a per-tile stamp so a tile is queued at most once per round) until the
-//     list is empty.  The list count lives on the device; the host only reads it
-//     every few rounds to stop launching.
+//     activated.  Interior pixels are written back.
+//   * One launch visits every tile once (state = min(marker, mask), then the
+//     local fixed point).  Activated tiles go to a device-side FIFO of tile ids
+//     (atomic append into a power-of-two ring; a per-tile state word keeps a tile
+//     in the ring at most once).  A persistent drain kernel, one wave per
+//     worker, pops tiles until no tile is queued or being visited, so a whole
+//     reconstruction is two launches with no host round trip.  Values only
+//     rise, so reading a neighbour mid-update gives a valid lower bound; the
+//     activation protocol re-visits any tile whose halo rose after it was read.
+//   * Every spin in the queue is bounded; a stuck queue sets an error bit and
+//     all waves exit (reported as NSX_ERR_NOT_CONVERGED), so the kernel cannot
+//     hang the device.
+//   * Several images (the batched form) share one launch: tiles are numbered
+//     across images and a tile -> image table selects each tile's descriptor.
 #include "nsx_internal.h"
 
 #include <climits>
@@ -40,7 +49,6 @@ constexpr int TS = 64;          // register tile side, halo included
 constexpr int TI = TS - 2;      // interior side written back
 constexpr int LDS_PITCH = TS + 1;
 constexpr int MAX_IT = 64;      // sweep iterations per visit before re-queueing itself
-constexpr int MAX_ROUNDS = 1 << 22;
 
 // ---- order-preserving 32-bit keys -------------------------------------------------
 template <typename T> struct Key;
@@ -134,18 +142,6 @@ template <typename T> struct Codec<T, RM_FILLBIN> {
     __device__ static T enc(uint32_t k, T) { return k ? inv(tmax<T>()) : inv(T(0)); }
 };
 
-struct ReconArgs {
-    const char* marker; int64_t pm;
-    const char* mask;   int64_t pk;
-    char* state;        int64_t ps;
-    int rows, cols, ntx, nty, ntiles;
-    uint32_t* stamp;
-    int32_t* list0;
-    int32_t* list1;
-    int32_t* counts;    // [3] rotating round counters
-    int64_t* stats;     // [0] rounds with work, [1] tile visits, [2] tiles
-};
-
 template <typename T>
 __device__ __forceinline__ T ld(const char* base, int64_t pitch, int y, int x) {
     return *reinterpret_cast<const T*>(base + (int64_t)y * pitch + (int64_t)x * (int64_t)sizeof(T));
@@ -222,34 +218,114 @@ __device__ __forceinline__ S tile_sum(const uint32_t (&c)[TS]) {
     return wave_sum<S>(s);
 }
 
-__device__ __forceinline__ void activate(const ReconArgs& A, int ty, int tx, int round) {
-    if (ty < 0 || tx < 0 || ty >= A.nty || tx >= A.ntx) return;
-    const int t = ty * A.ntx + tx;
-    const uint32_t want = (uint32_t)round + 2u;   // "queued for round+1"
-    if (atomicMax(&A.stamp[t], want) < want) {
-        const int idx = atomicAdd(&A.counts[(round + 1) % 3], 1);
-        int32_t* nl = ((round + 1) & 1) ? A.list1 : A.list0;
-        nl[idx] = t;
+// ---- device tile queue ------------------------------------------------------------
+// Per-tile state bits: TQ_DIRTY = a visit is owed, TQ_RUN = a wave is visiting it.
+// All transitions are RMWs, so a release fence before an activation and an
+// acquire fence after the visit's own RMW order the tile data between waves.
+constexpr uint32_t TQ_DIRTY = 1u, TQ_RUN = 2u;
+constexpr int32_t RING_EMPTY = -1;
+constexpr uint32_t SPIN_LIMIT = 1u << 26;   // polls without progress before giving up
+
+struct Queue {
+    uint32_t* tstate;   // per tile
+    int32_t* ring;      // tile ids, capacity a power of two >= tiles
+    uint32_t mask;      // capacity - 1
+    uint32_t* ctl;      // [0] head [1] tail [2] pending [3] error [4] visits
+};
+
+__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ void q_push(const Queue& Q, int32_t t) {
+    atomicAdd(&Q.ctl[2], 1u);
+    const uint32_t pos = atomicAdd(&Q.ctl[1], 1u);
+    int32_t* slot = &Q.ring[pos & Q.mask];
+    for (uint32_t spin = 0; atomicCAS(slot, RING_EMPTY, t) != RING_EMPTY; ++spin) {
+        __builtin_amdgcn_s_sleep(2);
+        if (spin > SPIN_LIMIT) { atomicOr(&Q.ctl[3], 1u); return; }
     }
 }
 
+// Marks tile t as owing a visit; the caller's tile data must be released first.
+__device__ __forceinline__ void q_activate(const Queue& Q, int32_t t) {
+    if (atomicOr(&Q.tstate[t], TQ_DIRTY) == 0u) q_push(Q, t);
+}
+
+// Returns the next queued tile, or -1 when no tile is queued or being visited.
+__device__ int32_t q_pop(const Queue& Q) {
+    uint32_t idle = 0, last_visits = ld_relaxed(&Q.ctl[4]);
+    while (true) {
+        const uint32_t h = ld_relaxed(&Q.ctl[0]);
+        const uint32_t t = ld_relaxed(&Q.ctl[1]);
+        if (h < t) {
+            if (atomicCAS(&Q.ctl[0], h, h + 1u) == h) {
+                int32_t* slot = &Q.ring[h & Q.mask];
+                for (uint32_t spin = 0;; ++spin) {
+                    const int32_t v = atomicExch(slot, RING_EMPTY);
+                    if (v != RING_EMPTY) return v;
+                    __builtin_amdgcn_s_sleep(1);
+                    if (spin > SPIN_LIMIT) { atomicOr(&Q.ctl[3], 2u); return -1; }
+                }
+            }
+            continue;
+        }
+        if (ld_relaxed(&Q.ctl[2]) == 0u || ld_relaxed(&Q.ctl[3]) != 0u) return -1;
+        __builtin_amdgcn_s_sleep(8);
+        const uint32_t v = ld_relaxed(&Q.ctl[4]);
+        if (v != last_visits) { last_visits = v; idle = 0; }
+        else if (++idle > SPIN_LIMIT) { atomicOr(&Q.ctl[3], 4u); return -1; }
+    }
+}
+
+// One image of a reconstruction launch.
+struct ImgDesc {
+    const char* marker; int64_t pm;
+    const char* mask;   int64_t pk;
+    char* state;        int64_t ps;
+    int rows, cols, ntx, nty;
+    int tile_base;      // global id of the image's first tile
+    int pad;
+};
+
+struct ReconArgs {
+    ImgDesc img0;              // used when nimg == 1
+    const ImgDesc* imgs;       // device table when nimg > 1
+    const int32_t* tile_img;   // global tile -> image index when nimg > 1
+    int nimg, ntiles;
+    Queue Q;
+};
+
+__device__ __forceinline__ void tile_of(const ReconArgs& A, int32_t t, ImgDesc& D, int& lt) {
+    if (A.nimg == 1) { D = A.img0; lt = t; return; }
+    D = A.imgs[A.tile_img[t]];
+    lt = t - D.tile_base;
+}
+
+__device__ __forceinline__ void activate(const ReconArgs& A, const ImgDesc& D, int ty, int tx) {
+    if (ty < 0 || tx < 0 || ty >= D.nty || tx >= D.ntx) return;
+    q_activate(A.Q, D.tile_base + ty * D.ntx + tx);
+}
+
+// Loads a tile (+ halo), runs it to its local fixed point, writes the interior
+// back and activates the tiles whose pixels in the halo rose.
 template <typename T, int MODE, int CONN, bool FIRST>
-__device__ __forceinline__ void process_tile(const ReconArgs& A, int t, int round, uint32_t* lds) {
+__device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& D, int lt, uint32_t* lds) {
     using C = Codec<T, MODE>;
     using S = typename SumT<T>::type;
     const int lane = threadIdx.x;
-    const int ty = t / A.ntx, tx = t - ty * A.ntx;
+    const int ty = lt / D.ntx, tx = lt - ty * D.ntx;
     const int x0 = tx * TI - 1, y0 = ty * TI - 1;
     const uint32_t F = C::frame();
     const bool needA = FIRST ? C::kMarkerFirst : C::kMarkerLater;
 
     auto key_at = [&](int gy, int gx, uint32_t& mk) -> uint32_t {
-        if (gx < 0 || gx >= A.cols || gy < 0 || gy >= A.rows) { mk = F; return F; }
-        const T mv = ld<T>(A.mask, A.pk, gy, gx);
-        const T av = needA ? ld<T>(A.marker, A.pm, gy, gx) : T(0);
+        if (gx < 0 || gx >= D.cols || gy < 0 || gy >= D.rows) { mk = F; return F; }
+        const T mv = ld<T>(D.mask, D.pk, gy, gx);
+        const T av = needA ? ld<T>(D.marker, D.pm, gy, gx) : T(0);
         mk = C::mkey(av, mv);
         if (FIRST) return C::first(av, mv);
-        return C::dec(ld<T>(A.state, A.ps, gy, gx), av, mv);
+        return C::dec(ld<T>(D.state, D.ps, gy, gx), av, mv);
     };
 
     uint32_t c[TS], m[TS];
@@ -271,7 +347,7 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, int t, int roun
     S tot = tot_in;
     bool converged = false;
     for (int it = 0; it < MAX_IT; ++it) {
-        sweep_pair<CONN>(c, m);        // along rows' column direction (y)
+        sweep_pair<CONN>(c, m);        // along y
         transpose(c, lds);
         sweep_pair<CONN>(c, mt);       // along x
         const S t1 = tile_sum<S>(c);
@@ -285,13 +361,13 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, int t, int roun
     transpose(c, lds);
 
     if (FIRST || tot != tot_in) {
-        if (lane >= 1 && lane <= TI && gx < A.cols) {
+        if (lane >= 1 && lane <= TI && gx < D.cols) {
 #pragma unroll
             for (int k = 1; k <= TI; ++k) {
                 const int gy = y0 + k;
-                if (gy < A.rows) {
-                    const T av = (MODE == RM_BINARY) ? ld<T>(A.marker, A.pm, gy, gx) : T(0);
-                    st<T>(A.state, A.ps, gy, gx, C::enc(c[k], av));
+                if (gy < D.rows) {
+                    const T av = (MODE == RM_BINARY) ? ld<T>(D.marker, D.pm, gy, gx) : T(0);
+                    st<T>(D.state, D.ps, gy, gx, C::enc(c[k], av));
                 }
             }
         }
@@ -301,123 +377,184 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, int t, int roun
     const uint64_t botm = __ballot(c[TS - 1] != initB);
     const uint64_t lefm = __ballot(finL != initL);
     const uint64_t rigm = __ballot(finR != initR);
-    if (lane == 0) {
-        const uint64_t mid = 0x7FFFFFFFFFFFFFFEull;
-        if (topm & mid) activate(A, ty - 1, tx, round);
-        if (topm & 1ull) activate(A, ty - 1, tx - 1, round);
-        if (topm >> 63) activate(A, ty - 1, tx + 1, round);
-        if (botm & mid) activate(A, ty + 1, tx, round);
-        if (botm & 1ull) activate(A, ty + 1, tx - 1, round);
-        if (botm >> 63) activate(A, ty + 1, tx + 1, round);
-        if (lefm & mid) activate(A, ty, tx - 1, round);
-        if (rigm & mid) activate(A, ty, tx + 1, round);
-        if (!converged) activate(A, ty, tx, round);
-    }
-}
-
-template <typename T, int MODE, int CONN, bool FIRST>
-__global__ __launch_bounds__(64) void recon_round_kernel(ReconArgs A, int round) {
-    __shared__ uint32_t lds[TS * LDS_PITCH];
-    int n;
-    const int32_t* list = (round & 1) ? A.list1 : A.list0;
-    if (FIRST) n = A.ntiles;
-    else n = __builtin_nontemporal_load(&A.counts[round % 3]);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        A.counts[(round + 2) % 3] = 0;
-        if (n > 0) {
-            A.stats[0] += 1;
-            A.stats[1] += n;
+    const uint64_t mid = 0x7FFFFFFFFFFFFFFEull;
+    const bool any = (topm | botm | ((lefm | rigm) & mid)) != 0ull || !converged;
+    if (any) {
+        __threadfence();   // release this tile's writes before anyone is told to read them
+        if (lane == 0) {
+            if (topm & mid) activate(A, D, ty - 1, tx);
+            if (topm & 1ull) activate(A, D, ty - 1, tx - 1);
+            if (topm >> 63) activate(A, D, ty - 1, tx + 1);
+            if (botm & mid) activate(A, D, ty + 1, tx);
+            if (botm & 1ull) activate(A, D, ty + 1, tx - 1);
+            if (botm >> 63) activate(A, D, ty + 1, tx + 1);
+            if (lefm & mid) activate(A, D, ty, tx - 1);
+            if (rigm & mid) activate(A, D, ty, tx + 1);
+            if (!converged) activate(A, D, ty, tx);
         }
     }
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
-        const int t = FIRST ? i : list[i];
-        process_tile<T, MODE, CONN, FIRST>(A, t, round, lds);
+}
+
+// First visit of every tile: state = clamped marker, then the local fixed point.
+template <typename T, int MODE, int CONN>
+__global__ __launch_bounds__(64) void recon_first_kernel(ReconArgs A) {
+    __shared__ uint32_t lds[TS * LDS_PITCH];
+    ImgDesc D;
+    int lt;
+    tile_of(A, (int32_t)blockIdx.x, D, lt);
+    process_tile<T, MODE, CONN, true>(A, D, lt, lds);
+}
+
+// Persistent drain: each wave pops queued tiles until none is queued or running.
+template <typename T, int MODE, int CONN>
+__global__ __launch_bounds__(64) void recon_drain_kernel(ReconArgs A) {
+    __shared__ uint32_t lds[TS * LDS_PITCH];
+    const int lane = threadIdx.x;
+    while (true) {
+        int32_t t = 0;
+        if (lane == 0) {
+            t = q_pop(A.Q);
+            if (t >= 0) atomicExch(&A.Q.tstate[t], TQ_RUN);   // clears TQ_DIRTY
+        }
+        t = __shfl(t, 0, 64);
+        if (t < 0) break;
+        __threadfence();   // acquire the data released by the tile's activators
+        ImgDesc D;
+        int lt;
+        tile_of(A, t, D, lt);
+        process_tile<T, MODE, CONN, false>(A, D, lt, lds);
+        if (lane == 0) {
+            if (atomicCAS(&A.Q.tstate[t], TQ_RUN, 0u) != TQ_RUN) {
+                // activated while running: owe another visit
+                atomicExch(&A.Q.tstate[t], TQ_DIRTY);
+                q_push(A.Q, t);
+            }
+            atomicAdd(&A.Q.ctl[4], 1u);
+            atomicSub(&A.Q.ctl[2], 1u);
+        }
     }
 }
 
-__global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts, int64_t* stats) {
+__global__ void recon_reset_kernel(Queue Q, int ntiles, int cap) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int j = i; j < ntiles; j += gridDim.x * blockDim.x) stamp[j] = 0u;
-    if (i < 3) counts[i] = 0;
-    if (i == 0) { stats[0] = 0; stats[1] = 0; stats[2] = ntiles; }
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = i; j < ntiles; j += stride) Q.tstate[j] = 0u;
+    for (int j = i; j < cap; j += stride) Q.ring[j] = RING_EMPTY;
+    if (i < 8) Q.ctl[i] = 0u;
+}
+
+__global__ void recon_stats_kernel(Queue Q, int ntiles, int64_t* stats) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        stats[0] = Q.ctl[3];            // error bits (0 = converged)
+        stats[1] = (int64_t)Q.ctl[4];   // drain visits
+        stats[2] = ntiles;
+        stats[3] |= (int64_t)Q.ctl[3];  // sticky until the host reads it
+    }
 }
 
 template <typename T, int MODE, int CONN>
-nsx_status run_typed(nsx_ctx* ctx, ReconArgs A, hipStream_t s) {
-    const int grid_reset = std::min(1024, (A.ntiles + 255) / 256);
-    hipLaunchKernelGGL(recon_reset_kernel, dim3(grid_reset), dim3(256), 0, s, A.stamp, A.ntiles, A.counts, A.stats);
-    hipLaunchKernelGGL((recon_round_kernel<T, MODE, CONN, true>), dim3(A.ntiles), dim3(64), 0, s, A, 0);
+nsx_status run_typed(nsx_ctx* ctx, const ReconArgs& A, int cap, int workers, hipStream_t s) {
+    const int grid_reset = std::max(1, std::min(1024, (std::max(A.ntiles, cap) + 255) / 256));
+    hipLaunchKernelGGL(recon_reset_kernel, dim3(grid_reset), dim3(256), 0, s, A.Q, A.ntiles, cap);
+    const int h0 = ctx->tic(TK_RECON_FIRST, s);
+    hipLaunchKernelGGL((recon_first_kernel<T, MODE, CONN>), dim3(A.ntiles), dim3(64), 0, s, A);
+    ctx->toc(h0, s);
+    const int h1 = ctx->tic(TK_RECON_ROUND, s);
+    hipLaunchKernelGGL((recon_drain_kernel<T, MODE, CONN>), dim3(workers), dim3(64), 0, s, A);
+    ctx->toc(h1, s);
+    hipLaunchKernelGGL(recon_stats_kernel, dim3(1), dim3(64), 0, s, A.Q, A.ntiles, ctx->d_stats);
     NSX_HIP_TRY(ctx, hipGetLastError());
-    const int grid = std::max(1, std::min(A.ntiles, 8 * ctx->cu_count));
-    int round = 1;
-    int batch = 2;
-    while (true) {
-        for (int j = 0; j < batch; ++j, ++round)
-            hipLaunchKernelGGL((recon_round_kernel<T, MODE, CONN, false>), dim3(grid), dim3(64), 0, s, A, round);
-        NSX_HIP_TRY(ctx, hipGetLastError());
-        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, &A.counts[round % 3], sizeof(int32_t),
-                                        hipMemcpyDeviceToHost, s));
-        NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
-        if (ctx->h_pinned[0] == 0) break;
-        if (round > MAX_ROUNDS) return ctx->fail(NSX_ERR_NOT_CONVERGED, "reconstruction did not converge");
-        batch = std::min(batch * 2, 64);
-    }
     return NSX_OK;
 }
 
 template <typename T, int MODE>
-nsx_status run_conn(nsx_ctx* ctx, const ReconArgs& A, int conn, hipStream_t s) {
-    if (conn == 8) return run_typed<T, MODE, 8>(ctx, A, s);
-    return run_typed<T, MODE, 4>(ctx, A, s);
+nsx_status run_conn(nsx_ctx* ctx, const ReconArgs& A, int conn, int cap, int workers, hipStream_t s) {
+    if (conn == 8) return run_typed<T, MODE, 8>(ctx, A, cap, workers, s);
+    return run_typed<T, MODE, 4>(ctx, A, cap, workers, s);
+}
+
+ImgDesc make_desc(const View* marker, const View& mask, const View& state, int tile_base) {
+    ImgDesc D{};
+    D.marker = marker ? marker->data : nullptr;
+    D.pm = marker ? marker->pitch : 0;
+    D.mask = mask.data;
+    D.pk = mask.pitch;
+    D.state = state.data;
+    D.ps = state.pitch;
+    D.rows = mask.rows;
+    D.cols = mask.cols;
+    D.ntx = (D.cols + TI - 1) / TI;
+    D.nty = (D.rows + TI - 1) / TI;
+    D.tile_base = tile_base;
+    return D;
 }
 
 }  // namespace
 
-nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& state,
-                     int conn, hipStream_t s) {
+int recon_tiles(int rows, int cols) {
+    if (rows <= 0 || cols <= 0) return 0;
+    return ((cols + TI - 1) / TI) * ((rows + TI - 1) / TI);
+}
+
+nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const ReconBatch& B, hipStream_t s) {
     ReconArgs A{};
-    A.marker = marker ? marker->data : nullptr;
-    A.pm = marker ? marker->pitch : 0;
-    A.mask = mask.data;
-    A.pk = mask.pitch;
-    A.state = state.data;
-    A.ps = state.pitch;
-    A.rows = mask.rows;
-    A.cols = mask.cols;
-    A.ntx = (A.cols + TI - 1) / TI;
-    A.nty = (A.rows + TI - 1) / TI;
-    A.ntiles = A.ntx * A.nty;
-    if (A.rows <= 0 || A.cols <= 0) return NSX_OK;
-    void *stamp, *lists, *counts;
-    NSX_TRY(ctx->ws(WS_RECON_STAMP, sizeof(uint32_t) * (size_t)A.ntiles, &stamp));
-    NSX_TRY(ctx->ws(WS_RECON_LIST, 2 * sizeof(int32_t) * (size_t)A.ntiles, &lists));
-    NSX_TRY(ctx->ws(WS_RECON_COUNTS, 64, &counts));
-    A.stamp = (uint32_t*)stamp;
-    A.list0 = (int32_t*)lists;
-    A.list1 = (int32_t*)lists + A.ntiles;
-    A.counts = (int32_t*)counts;
-    A.stats = ctx->d_stats;
-    const int dt = mask.dtype;
+    A.nimg = B.nimg;
+    A.ntiles = B.ntiles;
+    if (A.ntiles <= 0) return NSX_OK;
+    if (B.nimg == 1) A.img0 = *reinterpret_cast<const ImgDesc*>(B.host_desc);
+    A.imgs = reinterpret_cast<const ImgDesc*>(B.dev_desc);
+    A.tile_img = B.tile_img;
+    int cap = 1;
+    while (cap < A.ntiles) cap <<= 1;
+    void *tstate, *ring, *ctl;
+    NSX_TRY(ctx->ws(B.ws_base + 0, sizeof(uint32_t) * (size_t)A.ntiles, &tstate));
+    NSX_TRY(ctx->ws(B.ws_base + 1, sizeof(int32_t) * (size_t)cap, &ring));
+    NSX_TRY(ctx->ws(B.ws_base + 2, 64, &ctl));
+    A.Q.tstate = (uint32_t*)tstate;
+    A.Q.ring = (int32_t*)ring;
+    A.Q.mask = (uint32_t)(cap - 1);
+    A.Q.ctl = (uint32_t*)ctl;
+    // resident waves: the drain kernel holds one wave per SIMD (>256 VGPR+AGPR)
+    const int per_cu = 4;
+    const int workers = std::max(1, std::min(A.ntiles, per_cu * ctx->cu_count));
     switch (mode) {
         case RM_RECON:
-            if (dt == NSX_U8) return run_conn<uint8_t, RM_RECON>(ctx, A, conn, s);
-            if (dt == NSX_I32) return run_conn<int32_t, RM_RECON>(ctx, A, conn, s);
-            return run_conn<float, RM_RECON>(ctx, A, conn, s);
+            if (dtype == NSX_U8) return run_conn<uint8_t, RM_RECON>(ctx, A, conn, cap, workers, s);
+            if (dtype == NSX_I32) return run_conn<int32_t, RM_RECON>(ctx, A, conn, cap, workers, s);
+            return run_conn<float, RM_RECON>(ctx, A, conn, cap, workers, s);
         case RM_BINARY:
-            if (dt == NSX_U8) return run_conn<uint8_t, RM_BINARY>(ctx, A, conn, s);
-            if (dt == NSX_I32) return run_conn<int32_t, RM_BINARY>(ctx, A, conn, s);
+            if (dtype == NSX_U8) return run_conn<uint8_t, RM_BINARY>(ctx, A, conn, cap, workers, s);
+            if (dtype == NSX_I32) return run_conn<int32_t, RM_BINARY>(ctx, A, conn, cap, workers, s);
             break;
         case RM_FILL:
-            if (dt == NSX_U8) return run_conn<uint8_t, RM_FILL>(ctx, A, conn, s);
-            if (dt == NSX_I32) return run_conn<int32_t, RM_FILL>(ctx, A, conn, s);
+            if (dtype == NSX_U8) return run_conn<uint8_t, RM_FILL>(ctx, A, conn, cap, workers, s);
+            if (dtype == NSX_I32) return run_conn<int32_t, RM_FILL>(ctx, A, conn, cap, workers, s);
             break;
         case RM_FILLBIN:
-            if (dt == NSX_U8) return run_conn<uint8_t, RM_FILLBIN>(ctx, A, conn, s);
+            if (dtype == NSX_U8) return run_conn<uint8_t, RM_FILLBIN>(ctx, A, conn, cap, workers, s);
             break;
         default:
             break;
     }
     return ctx->fail(NSX_ERR_UNSUPPORTED, "reconstruction mode/dtype not instantiated by the reference");
+}
+
+size_t recon_desc_bytes() { return sizeof(ImgDesc); }
+
+void recon_fill_desc(void* out, const View* marker, const View& mask, const View& state, int tile_base) {
+    *reinterpret_cast<ImgDesc*>(out) = make_desc(marker, mask, state, tile_base);
+}
+
+nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& state,
+                     int conn, hipStream_t s) {
+    ImgDesc D = make_desc(marker, mask, state, 0);
+    ReconBatch B{};
+    B.nimg = 1;
+    B.ntiles = D.ntx * D.nty;
+    B.host_desc = &D;
+    B.ws_base = WS_RECON_STAMP;
+    if (mask.rows <= 0 || mask.cols <= 0) return NSX_OK;
+    return recon_launch(ctx, mode, mask.dtype, conn == 8 ? 8 : 4, B, s);
 }
 
 }  // namespace nsx

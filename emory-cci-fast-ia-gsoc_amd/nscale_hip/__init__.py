@@ -129,10 +129,24 @@ class Context:
     def workspace_bytes(self) -> int:
         return int(self.lib.nsx_ctx_workspace_bytes(self.h))
 
+    TIMED_KINDS = ("recon_first", "recon_round", "ccl_local", "ccl_merge", "ccl_final", "ccl_small")
+
+    def set_timing(self, enable: bool) -> None:
+        """Start (and clear) or stop HIP-event timing of the library's kernels."""
+        self.check(self.lib.nsx_ctx_set_timing(self.h, int(bool(enable))))
+
+    def read_timing(self) -> dict:
+        """{kind: (total_ms, launches)} for the kernel kinds in TIMED_KINDS."""
+        k = len(self.TIMED_KINDS)
+        ms = (C.c_double * k)()
+        n = (C.c_int64 * k)()
+        self.check(self.lib.nsx_ctx_read_timing(self.h, ms, n, k))
+        return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(self.TIMED_KINDS)}
+
     def recon_stats(self) -> dict:
         s = (C.c_int64 * 3)()
         self.check(self.lib.nsx_last_recon_stats(self.h, s))
-        return {"rounds": int(s[0]), "tile_visits": int(s[1]), "tiles": int(s[2])}
+        return {"queue_error": int(s[0]), "drain_visits": int(s[1]), "tiles": int(s[2])}
 
 
 _tls = threading.local()

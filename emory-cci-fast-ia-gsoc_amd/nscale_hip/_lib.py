@@ -43,6 +43,8 @@ SIGNATURES = {
     "nsx_ctx_synchronize": (C.c_int, [_CTX]),
     "nsx_ctx_workspace_bytes": (C.c_size_t, [_CTX]),
     "nsx_last_recon_stats": (C.c_int, [_CTX, P(C.c_int64)]),
+    "nsx_ctx_set_timing": (C.c_int, [_CTX, C.c_int]),
+    "nsx_ctx_read_timing": (C.c_int, [_CTX, P(C.c_double), P(C.c_int64), C.c_int]),
     "nsx_imreconstruct": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_imreconstruct_binary": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_fill_holes": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_int, C.c_void_p]),

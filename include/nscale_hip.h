@@ -64,9 +64,21 @@ nsx_status nsx_ctx_synchronize(nsx_ctx* ctx);
 /* Device bytes currently held by the context's workspace. */
 size_t nsx_ctx_workspace_bytes(const nsx_ctx* ctx);
 /* Reads the statistics of the last reconstruction run on this context:
- * stats[0] = propagation rounds, stats[1] = tile passes (sum of active tiles
- * over rounds), stats[2] = tiles in the image.  Synchronises the stream. */
+ * stats[0] = work-queue error bits (0 = converged), stats[1] = tile visits of
+ * the drain after the first full pass, stats[2] = tiles in the image.
+ * Synchronises the device. */
 nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int64_t stats[3]);
+
+/* Kernel timing with HIP events recorded on the stream each kernel runs on.
+ * nsx_ctx_set_timing(ctx, 1) starts recording (and clears the totals);
+ * nsx_ctx_read_timing fills ms[k] (summed kernel time) and launches[k] for the
+ * kernel kinds k = 0 reconstruction first pass, 1 reconstruction propagation
+ * launch, 2 tile-local labelling, 3 tile-edge union, 4 label output, 5 area /
+ * keep / scan kernels (per call).  Synchronises the context.  No reference
+ * counterpart (the reference's CUDA event timing is commented out,
+ * src/segment/cuda/ccl_uf.cu:28-31). */
+nsx_status nsx_ctx_set_timing(nsx_ctx* ctx, int enable);
+nsx_status nsx_ctx_read_timing(nsx_ctx* ctx, double* ms, int64_t* launches, int nkinds);
 
 /* ---- morphology ----------------------------------------------------------- */
 /* Grayscale reconstruction by dilation of `marker` under `mask`; conn 8 gives
