@@ -150,6 +150,19 @@ template <typename T>
 __device__ __forceinline__ void st(char* base, int64_t pitch, int y, int x, T v) {
     *reinterpret_cast<T*>(base + (int64_t)y * pitch + (int64_t)x * (int64_t)sizeof(T)) = v;
 }
+// Device-coherent (agent-scope) access to the state buffer while tiles are
+// visited concurrently from different XCDs: the loads and stores carry sc1 and
+// do not rely on another XCD's L2 being written back or invalidated.
+template <typename T>
+__device__ __forceinline__ T ldc(const char* base, int64_t pitch, int y, int x) {
+    const T* p = reinterpret_cast<const T*>(base + (int64_t)y * pitch + (int64_t)x * (int64_t)sizeof(T));
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void stc(char* base, int64_t pitch, int y, int x, T v) {
+    T* p = reinterpret_cast<T*>(base + (int64_t)y * pitch + (int64_t)x * (int64_t)sizeof(T));
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // lane i <- lane i-1 (lane 0 <- 0), lane i <- lane i+1 (lane 63 <- 0); 0 is the
 // identity of max over keys.
@@ -224,26 +237,32 @@ __device__ __forceinline__ S tile_sum(const uint32_t (&c)[TS]) {
 // acquire fence after the visit's own RMW order the tile data between waves.
 constexpr uint32_t TQ_DIRTY = 1u, TQ_RUN = 2u;
 constexpr int32_t RING_EMPTY = -1;
-constexpr uint32_t SPIN_LIMIT = 1u << 26;   // polls without progress before giving up
+// A wait with no queue progress for this long (s_memrealtime runs at 100 MHz)
+// gives up and sets an error bit, so the kernel always exits.
+constexpr uint64_t STALL_TICKS = 200000000ull;   // 2 s
 
 struct Queue {
     uint32_t* tstate;   // per tile
     int32_t* ring;      // tile ids, capacity a power of two >= tiles
     uint32_t mask;      // capacity - 1
     uint32_t* ctl;      // [0] head [1] tail [2] pending [3] error [4] visits
+    uint32_t max_visits;  // visits after which the drain gives up (livelock guard)
 };
 
 __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ uint64_t now_ticks() { return __builtin_amdgcn_s_memrealtime(); }
+
 __device__ void q_push(const Queue& Q, int32_t t) {
     atomicAdd(&Q.ctl[2], 1u);
     const uint32_t pos = atomicAdd(&Q.ctl[1], 1u);
     int32_t* slot = &Q.ring[pos & Q.mask];
-    for (uint32_t spin = 0; atomicCAS(slot, RING_EMPTY, t) != RING_EMPTY; ++spin) {
+    const uint64_t t0 = now_ticks();
+    while (atomicCAS(slot, RING_EMPTY, t) != RING_EMPTY) {
         __builtin_amdgcn_s_sleep(2);
-        if (spin > SPIN_LIMIT) { atomicOr(&Q.ctl[3], 1u); return; }
+        if (now_ticks() - t0 > STALL_TICKS) { atomicOr(&Q.ctl[3], 1u); return; }
     }
 }
 
@@ -254,18 +273,21 @@ __device__ __forceinline__ void q_activate(const Queue& Q, int32_t t) {
 
 // Returns the next queued tile, or -1 when no tile is queued or being visited.
 __device__ int32_t q_pop(const Queue& Q) {
-    uint32_t idle = 0, last_visits = ld_relaxed(&Q.ctl[4]);
+    uint32_t last_visits = ld_relaxed(&Q.ctl[4]);
+    if (last_visits > Q.max_visits) { atomicOr(&Q.ctl[3], 8u); return -1; }
+    uint64_t t0 = now_ticks();
     while (true) {
         const uint32_t h = ld_relaxed(&Q.ctl[0]);
         const uint32_t t = ld_relaxed(&Q.ctl[1]);
         if (h < t) {
             if (atomicCAS(&Q.ctl[0], h, h + 1u) == h) {
                 int32_t* slot = &Q.ring[h & Q.mask];
-                for (uint32_t spin = 0;; ++spin) {
+                const uint64_t s0 = now_ticks();
+                while (true) {
                     const int32_t v = atomicExch(slot, RING_EMPTY);
                     if (v != RING_EMPTY) return v;
                     __builtin_amdgcn_s_sleep(1);
-                    if (spin > SPIN_LIMIT) { atomicOr(&Q.ctl[3], 2u); return -1; }
+                    if (now_ticks() - s0 > STALL_TICKS) { atomicOr(&Q.ctl[3], 2u); return -1; }
                 }
             }
             continue;
@@ -273,8 +295,8 @@ __device__ int32_t q_pop(const Queue& Q) {
         if (ld_relaxed(&Q.ctl[2]) == 0u || ld_relaxed(&Q.ctl[3]) != 0u) return -1;
         __builtin_amdgcn_s_sleep(8);
         const uint32_t v = ld_relaxed(&Q.ctl[4]);
-        if (v != last_visits) { last_visits = v; idle = 0; }
-        else if (++idle > SPIN_LIMIT) { atomicOr(&Q.ctl[3], 4u); return -1; }
+        if (v != last_visits) { last_visits = v; t0 = now_ticks(); }
+        else if (now_ticks() - t0 > STALL_TICKS) { atomicOr(&Q.ctl[3], 4u); return -1; }
     }
 }
 
@@ -325,7 +347,7 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
         const T av = needA ? ld<T>(D.marker, D.pm, gy, gx) : T(0);
         mk = C::mkey(av, mv);
         if (FIRST) return C::first(av, mv);
-        return C::dec(ld<T>(D.state, D.ps, gy, gx), av, mv);
+        return C::dec(ldc<T>(D.state, D.ps, gy, gx), av, mv);
     };
 
     uint32_t c[TS], m[TS];
@@ -367,16 +389,18 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
                 const int gy = y0 + k;
                 if (gy < D.rows) {
                     const T av = (MODE == RM_BINARY) ? ld<T>(D.marker, D.pm, gy, gx) : T(0);
-                    st<T>(D.state, D.ps, gy, gx, C::enc(c[k], av));
+                    if (FIRST) st<T>(D.state, D.ps, gy, gx, C::enc(c[k], av));
+                    else stc<T>(D.state, D.ps, gy, gx, C::enc(c[k], av));
                 }
             }
         }
     }
 
-    const uint64_t topm = __ballot(c[0] != initT);
-    const uint64_t botm = __ballot(c[TS - 1] != initB);
-    const uint64_t lefm = __ballot(finL != initL);
-    const uint64_t rigm = __ballot(finR != initR);
+    // Halo pixels that rose above what was read: their owners must re-visit.
+    const uint64_t topm = __ballot(c[0] > initT);
+    const uint64_t botm = __ballot(c[TS - 1] > initB);
+    const uint64_t lefm = __ballot(finL > initL);
+    const uint64_t rigm = __ballot(finR > initR);
     const uint64_t mid = 0x7FFFFFFFFFFFFFFEull;
     const bool any = (topm | botm | ((lefm | rigm) & mid)) != 0ull || !converged;
     if (any) {
@@ -514,6 +538,10 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     A.Q.ring = (int32_t*)ring;
     A.Q.mask = (uint32_t)(cap - 1);
     A.Q.ctl = (uint32_t*)ctl;
+    // Every visit raises at least one pixel or ends a tile's chain, so the visit
+    // count is bounded by the value range; this generous cap only catches a
+    // drain that stopped making progress while still visiting.
+    A.Q.max_visits = (uint32_t)std::min<int64_t>(0x7FFFFFFF, std::max<int64_t>(1 << 20, (int64_t)A.ntiles * 512));
     // resident waves: the drain kernel holds one wave per SIMD (>256 VGPR+AGPR)
     const int per_cu = 4;
     const int workers = std::max(1, std::min(A.ntiles, per_cu * ctx->cu_count));
