@@ -126,7 +126,8 @@ def main():
         "config": {"workload": f"{n}x{n} u8 disks({args.disks})+noise: recon8 -> fillholes4 -> >128 -> "
                                "bwlabel8 -> bwareaopen[11,1000) -> relabel",
                    "global_batch": 1, "pixels": px, "components": ncomp,
-                   "fill_drain_visits": stats["drain_visits"], "tiles": stats["tiles"]},
+                   "recon_rounds": stats["rounds"], "recon_tile_visits": stats["tile_visits"],
+                   "tiles": stats["tiles"]},
     }
     print(json.dumps(out))
 

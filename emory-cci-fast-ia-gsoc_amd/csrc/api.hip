@@ -143,33 +143,15 @@ bool overlaps(const nsx_img* a, const nsx_img* b) {
     return a0 < b1 && b0 < a1;
 }
 
-// Synchronises s and reports a reconstruction whose device queue gave up
-// (sticky error word d_stats[3], cleared here).
-nsx_status check_converged(nsx_ctx* ctx, hipStream_t s) {
-    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 16, ctx->d_stats + 3, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
-    int64_t err;
-    std::memcpy(&err, ctx->h_pinned + 16, sizeof(err));
-    if (err != 0) {
-        NSX_HIP_TRY(ctx, hipMemsetAsync(ctx->d_stats + 3, 0, sizeof(int64_t), s));
-        NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
-        return ctx->fail(NSX_ERR_NOT_CONVERGED, "reconstruction work queue stalled (error bits " +
-                                                    std::to_string(err) + ")");
-    }
-    return NSX_OK;
-}
-
 nsx_status read_count(nsx_ctx* ctx, const int32_t* dev, int32_t* host, hipStream_t s) {
     NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 8, dev, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    NSX_TRY(check_converged(ctx, s));
+    NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
     if (host) *host = ctx->h_pinned[8];
     return NSX_OK;
 }
 
 nsx_status finish_recon(nsx_ctx* ctx, const View& v, const nsx_img* out, hipStream_t s) {
-    NSX_TRY(from_device(ctx, v, out, s));
-    if (out->mem == NSX_MEM_HOST) NSX_TRY(check_converged(ctx, s));
-    return NSX_OK;
+    return from_device(ctx, v, out, s);
 }
 
 }  // namespace
@@ -450,8 +432,6 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
     NSX_TRY(inter(recon_out, WS_CHAIN_RECON, &vr));
     NSX_TRY(inter(fill_out, WS_CHAIN_FILL, &vf));
     NSX_TRY(to_device(ctx, labels_out, WS_STAGE_C, false, s, &vl));
-    NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vr, recon_conn == 8 ? 8 : 4, s));
-    NSX_TRY(recon_run(ctx, RM_FILL, nullptr, vr, vf, fill_conn == 8 ? 8 : 4, s));
     CclParams p;
     p.conn = ccl_conn == 8 ? 8 : 4;
     p.out_mode = CO_CANON;
@@ -460,12 +440,54 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
     p.use_area = true;
     p.min_size = min_size;
     p.max_size = max_size;
-    int32_t* dcount = (int32_t*)ctx->d_stats + 12;
-    NSX_TRY(ccl_run(ctx, vf, vl, p, dcount, s));
+    // d_stats words: int32 [12] components, [13] recon tiles still queued,
+    // [14] fill tiles still queued; int64 [8..10] the fill stage's statistics.
+    int32_t* dw = (int32_t*)ctx->d_stats;
+    const int rconn = recon_conn == 8 ? 8 : 4, fconn = fill_conn == 8 ? 8 : 4;
+    std::vector<uint8_t> dr(recon_desc_bytes()), df(recon_desc_bytes());
+    recon_fill_desc(dr.data(), &vm, vk, vr, 0);
+    recon_fill_desc(df.data(), nullptr, vr, vf, 0);
+    ReconBatch br{}, bf{};
+    br.nimg = bf.nimg = 1;
+    br.ntiles = bf.ntiles = recon_tiles(mask->rows, mask->cols);
+    br.host_desc = dr.data();
+    bf.host_desc = df.data();
+    br.ws_base = WS_RECON_STAMP;
+    bf.ws_base = WS_FILL_STAMP;
+    bf.stats = ctx->d_stats + 8;
+    ReconRun rr, rf;
+    // All stages are enqueued back to back; convergence of both propagations
+    // is checked once, together with the component count.
+    NSX_TRY(recon_launch(ctx, RM_RECON, mask->dtype, rconn, br, s, &rr, dw + 13));
+    NSX_TRY(recon_launch(ctx, RM_FILL, mask->dtype, fconn, bf, s, &rf, dw + 14));
+    NSX_TRY(ccl_run(ctx, vf, vl, p, dw + 12, s));
+    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 12, dw + 12, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 16, ctx->d_stats, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 18, ctx->d_stats + 8, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (ctx->h_pinned[13] != 0 || ctx->h_pinned[14] != 0) {
+        // a propagation needed more rounds than predicted: finish it and redo
+        // the stages downstream of it
+        if (ctx->h_pinned[13] != 0) {
+            NSX_TRY(recon_complete(ctx, &rr, s, nullptr));
+            NSX_TRY(recon_launch(ctx, RM_FILL, mask->dtype, fconn, bf, s, &rf, nullptr));
+        }
+        NSX_TRY(recon_complete(ctx, &rf, s, nullptr));
+        NSX_TRY(ccl_run(ctx, vf, vl, p, dw + 12, s));
+        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 12, dw + 12, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+    } else {
+        int64_t wr, wf;   // rounds that had work, for the next call's prediction
+        std::memcpy(&wr, ctx->h_pinned + 16, sizeof(wr));
+        std::memcpy(&wf, ctx->h_pinned + 18, sizeof(wf));
+        recon_note_rounds(ctx, &rr, wr);
+        recon_note_rounds(ctx, &rf, wf);
+    }
+    const int32_t n = ctx->h_pinned[12];
     if (recon_out && recon_out->mem == NSX_MEM_HOST) NSX_TRY(from_device(ctx, vr, recon_out, s));
     if (fill_out && fill_out->mem == NSX_MEM_HOST) NSX_TRY(from_device(ctx, vf, fill_out, s));
     NSX_TRY(from_device(ctx, vl, labels_out, s));
-    if (ncomp) NSX_TRY(read_count(ctx, dcount, ncomp, s));
+    if (ncomp) *ncomp = n;
     return NSX_OK;
 }
 

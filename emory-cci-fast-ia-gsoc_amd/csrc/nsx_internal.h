@@ -7,6 +7,8 @@
 #include <string>
 #include <vector>
 #include <mutex>
+#include <unordered_map>
+#include <cstring>
 
 #include "../../include/nscale_hip.h"
 
@@ -66,6 +68,9 @@ struct nsx_ctx {
     int32_t* h_pinned = nullptr;     // small pinned readback area
     int cu_count = 256;
 
+    // Propagation rounds the last reconstruction of a given size/mode needed.
+    std::unordered_map<uint64_t, int> round_hint;
+
     // HIP-event kernel timing (off by default).
     bool timing = false;
     struct EvPair { hipEvent_t a, b; int kind; };
@@ -103,6 +108,9 @@ enum WsSlot {
     WS_CCL_MISC,         // counters
     WS_CHAIN_RECON,      // chain intermediates
     WS_CHAIN_FILL,
+    WS_FILL_STAMP,       // a second reconstruction in flight (the chain's fill)
+    WS_FILL_LIST,
+    WS_FILL_COUNTS,
     WS_STAGE_A,          // host<->device staging
     WS_STAGE_B,
     WS_STAGE_C,
@@ -123,13 +131,12 @@ inline size_t dtype_size(int dt) { return dt == NSX_U8 ? 1 : 4; }
 enum ReconMode { RM_RECON = 0, RM_BINARY = 1, RM_FILL = 2, RM_FILLBIN = 3 };
 
 // Runs reconstruction in `mode` on device views (marker may be null for the
-// fill modes).  `state` is the output buffer and holds the propagation state.
-// Enqueues work only (no host synchronisation); errors of the device queue
-// are reported in ctx->d_stats[0] (nonzero = did not converge).
+// fill modes) to convergence.  `state` is the output buffer and holds the
+// propagation state.  Synchronises the stream at least once.
 nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& state,
                      int conn, hipStream_t s);
 
-// Several images of one dtype/mode/connectivity in one launch pair.
+// Several images of one dtype/mode/connectivity in one set of launches.
 struct ReconBatch {
     int nimg = 0;
     int ntiles = 0;                 // total tiles over images
@@ -137,11 +144,29 @@ struct ReconBatch {
     const void* dev_desc = nullptr;   // device array of nimg descriptors
     const int32_t* tile_img = nullptr;  // device: tile -> image index (nimg > 1)
     int ws_base = 0;                // first of 3 consecutive workspace slots
+    int64_t* stats = nullptr;       // device stats words (default ctx->d_stats)
+};
+struct ReconRunImpl;
+// A launched reconstruction whose convergence has not been checked yet.
+struct ReconRun {
+    ReconRunImpl* impl = nullptr;
+    ~ReconRun() { release(); }
+    void release();
 };
 int recon_tiles(int rows, int cols);
 size_t recon_desc_bytes();
 void recon_fill_desc(void* out, const View* marker, const View& mask, const View& state, int tile_base);
-nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const ReconBatch& B, hipStream_t s);
+// Enqueues round 0 and the predicted number of further rounds (no host sync).
+// When pending_dev is given, the number of tiles still queued after the last
+// launched round is copied there (0 = converged) for a later batched check.
+nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const ReconBatch& B, hipStream_t s,
+                        ReconRun* run, int32_t* pending_dev);
+// Synchronises, launches further rounds until the work list is empty, and
+// updates the round prediction.  *extended tells whether rounds were added.
+nsx_status recon_complete(nsx_ctx* ctx, ReconRun* run, hipStream_t s, bool* extended);
+// For a run already known to have converged: records how many rounds had work
+// (read by the caller from the run's stats word 0) and releases the run.
+void recon_note_rounds(nsx_ctx* ctx, ReconRun* run, int64_t rounds_with_work);
 
 // ---- connected components ---------------------------------------------------------
 enum CclOut {

@@ -23,17 +23,14 @@
 //   * Halo pixels are computed with the tile (their raised values are valid
 //     lower bounds) but never written; when one rises, the tile that owns it is
 //     activated.  Interior pixels are written back.
-//   * One launch visits every tile once (state = min(marker, mask), then the
-//     local fixed point).  Activated tiles go to a device-side FIFO of tile ids
-//     (atomic append into a power-of-two ring; a per-tile state word keeps a tile
-//     in the ring at most once).  A persistent drain kernel, one wave per
-//     worker, pops tiles until no tile is queued or being visited, so a whole
-//     reconstruction is two launches with no host round trip.  Values only
-//     rise, so reading a neighbour mid-update gives a valid lower bound; the
-//     activation protocol re-visits any tile whose halo rose after it was read.
-//   * Every spin in the queue is bounded; a stuck queue sets an error bit and
-//     all waves exit (reported as NSX_ERR_NOT_CONVERGED), so the kernel cannot
-//     hang the device.
+//   * Round 0 visits every tile (state = min(marker, mask), then the local
+//     fixed point).  Activated tiles are appended to a device-side work list
+//     (atomicAdd on the round's counter, a per-tile stamp keeps a tile in a list
+//     at most once); each later round is one launch over the previous round's
+//     list, and the list length lives only on the device, so rounds are launched
+//     back to back with no host round trip.  The host predicts the round count
+//     from the previous call of the same size and checks convergence once, at
+//     the call's synchronisation point, resuming if the prediction fell short.
 //   * Several images (the batched form) share one launch: tiles are numbered
 //     across images and a tile -> image table selects each tile's descriptor.
 #include "nsx_internal.h"
@@ -150,19 +147,6 @@ template <typename T>
 __device__ __forceinline__ void st(char* base, int64_t pitch, int y, int x, T v) {
     *reinterpret_cast<T*>(base + (int64_t)y * pitch + (int64_t)x * (int64_t)sizeof(T)) = v;
 }
-// Device-coherent (agent-scope) access to the state buffer while tiles are
-// visited concurrently from different XCDs: the loads and stores carry sc1 and
-// do not rely on another XCD's L2 being written back or invalidated.
-template <typename T>
-__device__ __forceinline__ T ldc(const char* base, int64_t pitch, int y, int x) {
-    const T* p = reinterpret_cast<const T*>(base + (int64_t)y * pitch + (int64_t)x * (int64_t)sizeof(T));
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ void stc(char* base, int64_t pitch, int y, int x, T v) {
-    T* p = reinterpret_cast<T*>(base + (int64_t)y * pitch + (int64_t)x * (int64_t)sizeof(T));
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // lane i <- lane i-1 (lane 0 <- 0), lane i <- lane i+1 (lane 63 <- 0); 0 is the
 // identity of max over keys.
@@ -231,74 +215,14 @@ __device__ __forceinline__ S tile_sum(const uint32_t (&c)[TS]) {
     return wave_sum<S>(s);
 }
 
-// ---- device tile queue ------------------------------------------------------------
-// Per-tile state bits: TQ_DIRTY = a visit is owed, TQ_RUN = a wave is visiting it.
-// All transitions are RMWs, so a release fence before an activation and an
-// acquire fence after the visit's own RMW order the tile data between waves.
-constexpr uint32_t TQ_DIRTY = 1u, TQ_RUN = 2u;
-constexpr int32_t RING_EMPTY = -1;
-// A wait with no queue progress for this long (s_memrealtime runs at 100 MHz)
-// gives up and sets an error bit, so the kernel always exits.
-constexpr uint64_t STALL_TICKS = 200000000ull;   // 2 s
-
-struct Queue {
-    uint32_t* tstate;   // per tile
-    int32_t* ring;      // tile ids, capacity a power of two >= tiles
-    uint32_t mask;      // capacity - 1
-    uint32_t* ctl;      // [0] head [1] tail [2] pending [3] error [4] visits
-    uint32_t max_visits;  // visits after which the drain gives up (livelock guard)
-};
-
-__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ uint64_t now_ticks() { return __builtin_amdgcn_s_memrealtime(); }
-
-__device__ void q_push(const Queue& Q, int32_t t) {
-    atomicAdd(&Q.ctl[2], 1u);
-    const uint32_t pos = atomicAdd(&Q.ctl[1], 1u);
-    int32_t* slot = &Q.ring[pos & Q.mask];
-    const uint64_t t0 = now_ticks();
-    while (atomicCAS(slot, RING_EMPTY, t) != RING_EMPTY) {
-        __builtin_amdgcn_s_sleep(2);
-        if (now_ticks() - t0 > STALL_TICKS) { atomicOr(&Q.ctl[3], 1u); return; }
-    }
-}
-
-// Marks tile t as owing a visit; the caller's tile data must be released first.
-__device__ __forceinline__ void q_activate(const Queue& Q, int32_t t) {
-    if (atomicOr(&Q.tstate[t], TQ_DIRTY) == 0u) q_push(Q, t);
-}
-
-// Returns the next queued tile, or -1 when no tile is queued or being visited.
-__device__ int32_t q_pop(const Queue& Q) {
-    uint32_t last_visits = ld_relaxed(&Q.ctl[4]);
-    if (last_visits > Q.max_visits) { atomicOr(&Q.ctl[3], 8u); return -1; }
-    uint64_t t0 = now_ticks();
-    while (true) {
-        const uint32_t h = ld_relaxed(&Q.ctl[0]);
-        const uint32_t t = ld_relaxed(&Q.ctl[1]);
-        if (h < t) {
-            if (atomicCAS(&Q.ctl[0], h, h + 1u) == h) {
-                int32_t* slot = &Q.ring[h & Q.mask];
-                const uint64_t s0 = now_ticks();
-                while (true) {
-                    const int32_t v = atomicExch(slot, RING_EMPTY);
-                    if (v != RING_EMPTY) return v;
-                    __builtin_amdgcn_s_sleep(1);
-                    if (now_ticks() - s0 > STALL_TICKS) { atomicOr(&Q.ctl[3], 2u); return -1; }
-                }
-            }
-            continue;
-        }
-        if (ld_relaxed(&Q.ctl[2]) == 0u || ld_relaxed(&Q.ctl[3]) != 0u) return -1;
-        __builtin_amdgcn_s_sleep(8);
-        const uint32_t v = ld_relaxed(&Q.ctl[4]);
-        if (v != last_visits) { last_visits = v; t0 = now_ticks(); }
-        else if (now_ticks() - t0 > STALL_TICKS) { atomicOr(&Q.ctl[3], 4u); return -1; }
-    }
-}
+// ---- propagation rounds over a device tile list ---------------------------------------
+// Round 0 visits every tile.  A visit that raises a halo pixel appends the
+// owning tile to the next round's list (a per-tile stamp keeps each tile at
+// most once per round).  Round r reads list[r&1] and its count counts[r%3],
+// appends to list[(r+1)&1] / counts[(r+1)%3] and clears counts[(r+2)%3] for
+// the round after.  Kernel boundaries order the rounds, so a tile sees every
+// write of the previous round; visits inside a round only read neighbours,
+// whose values are valid lower bounds whether or not they are mid-update.
 
 // One image of a reconstruction launch.
 struct ImgDesc {
@@ -315,7 +239,11 @@ struct ReconArgs {
     const ImgDesc* imgs;       // device table when nimg > 1
     const int32_t* tile_img;   // global tile -> image index when nimg > 1
     int nimg, ntiles;
-    Queue Q;
+    uint32_t* stamp;           // per tile: last round (+2) it was queued for
+    int32_t* list0;
+    int32_t* list1;
+    int32_t* counts;           // [3] rotating round counters
+    int64_t* stats;            // [0] rounds with work [1] tile visits after round 0 [2] tiles
 };
 
 __device__ __forceinline__ void tile_of(const ReconArgs& A, int32_t t, ImgDesc& D, int& lt) {
@@ -324,15 +252,21 @@ __device__ __forceinline__ void tile_of(const ReconArgs& A, int32_t t, ImgDesc& 
     lt = t - D.tile_base;
 }
 
-__device__ __forceinline__ void activate(const ReconArgs& A, const ImgDesc& D, int ty, int tx) {
+__device__ __forceinline__ void activate(const ReconArgs& A, const ImgDesc& D, int ty, int tx, int round) {
     if (ty < 0 || tx < 0 || ty >= D.nty || tx >= D.ntx) return;
-    q_activate(A.Q, D.tile_base + ty * D.ntx + tx);
+    const int t = D.tile_base + ty * D.ntx + tx;
+    const uint32_t want = (uint32_t)round + 2u;   // "queued for round+1"
+    if (atomicMax(&A.stamp[t], want) < want) {
+        const int idx = atomicAdd(&A.counts[(round + 1) % 3], 1);
+        int32_t* nl = ((round + 1) & 1) ? A.list1 : A.list0;
+        nl[idx] = t;
+    }
 }
 
 // Loads a tile (+ halo), runs it to its local fixed point, writes the interior
-// back and activates the tiles whose pixels in the halo rose.
+// back and queues the tiles whose pixels in the halo rose.
 template <typename T, int MODE, int CONN, bool FIRST>
-__device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& D, int lt, uint32_t* lds) {
+__device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& D, int lt, int round, uint32_t* lds) {
     using C = Codec<T, MODE>;
     using S = typename SumT<T>::type;
     const int lane = threadIdx.x;
@@ -347,7 +281,7 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
         const T av = needA ? ld<T>(D.marker, D.pm, gy, gx) : T(0);
         mk = C::mkey(av, mv);
         if (FIRST) return C::first(av, mv);
-        return C::dec(ldc<T>(D.state, D.ps, gy, gx), av, mv);
+        return C::dec(ld<T>(D.state, D.ps, gy, gx), av, mv);
     };
 
     uint32_t c[TS], m[TS];
@@ -389,8 +323,7 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
                 const int gy = y0 + k;
                 if (gy < D.rows) {
                     const T av = (MODE == RM_BINARY) ? ld<T>(D.marker, D.pm, gy, gx) : T(0);
-                    if (FIRST) st<T>(D.state, D.ps, gy, gx, C::enc(c[k], av));
-                    else stc<T>(D.state, D.ps, gy, gx, C::enc(c[k], av));
+                    st<T>(D.state, D.ps, gy, gx, C::enc(c[k], av));
                 }
             }
         }
@@ -401,100 +334,89 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
     const uint64_t botm = __ballot(c[TS - 1] > initB);
     const uint64_t lefm = __ballot(finL > initL);
     const uint64_t rigm = __ballot(finR > initR);
-    const uint64_t mid = 0x7FFFFFFFFFFFFFFEull;
-    const bool any = (topm | botm | ((lefm | rigm) & mid)) != 0ull || !converged;
-    if (any) {
-        __threadfence();   // release this tile's writes before anyone is told to read them
-        if (lane == 0) {
-            if (topm & mid) activate(A, D, ty - 1, tx);
-            if (topm & 1ull) activate(A, D, ty - 1, tx - 1);
-            if (topm >> 63) activate(A, D, ty - 1, tx + 1);
-            if (botm & mid) activate(A, D, ty + 1, tx);
-            if (botm & 1ull) activate(A, D, ty + 1, tx - 1);
-            if (botm >> 63) activate(A, D, ty + 1, tx + 1);
-            if (lefm & mid) activate(A, D, ty, tx - 1);
-            if (rigm & mid) activate(A, D, ty, tx + 1);
-            if (!converged) activate(A, D, ty, tx);
-        }
+    if (lane == 0) {
+        const uint64_t mid = 0x7FFFFFFFFFFFFFFEull;
+        if (topm & mid) activate(A, D, ty - 1, tx, round);
+        if (topm & 1ull) activate(A, D, ty - 1, tx - 1, round);
+        if (topm >> 63) activate(A, D, ty - 1, tx + 1, round);
+        if (botm & mid) activate(A, D, ty + 1, tx, round);
+        if (botm & 1ull) activate(A, D, ty + 1, tx - 1, round);
+        if (botm >> 63) activate(A, D, ty + 1, tx + 1, round);
+        if (lefm & mid) activate(A, D, ty, tx - 1, round);
+        if (rigm & mid) activate(A, D, ty, tx + 1, round);
+        if (!converged) activate(A, D, ty, tx, round);
     }
 }
 
-// First visit of every tile: state = clamped marker, then the local fixed point.
-template <typename T, int MODE, int CONN>
-__global__ __launch_bounds__(64) void recon_first_kernel(ReconArgs A) {
+template <typename T, int MODE, int CONN, bool FIRST>
+__global__ __launch_bounds__(64) void recon_round_kernel(ReconArgs A, int round) {
     __shared__ uint32_t lds[TS * LDS_PITCH];
-    ImgDesc D;
-    int lt;
-    tile_of(A, (int32_t)blockIdx.x, D, lt);
-    process_tile<T, MODE, CONN, true>(A, D, lt, lds);
-}
-
-// Persistent drain: each wave pops queued tiles until none is queued or running.
-template <typename T, int MODE, int CONN>
-__global__ __launch_bounds__(64) void recon_drain_kernel(ReconArgs A) {
-    __shared__ uint32_t lds[TS * LDS_PITCH];
-    const int lane = threadIdx.x;
-    while (true) {
-        int32_t t = 0;
-        if (lane == 0) {
-            t = q_pop(A.Q);
-            if (t >= 0) atomicExch(&A.Q.tstate[t], TQ_RUN);   // clears TQ_DIRTY
+    int n;
+    const int32_t* list = (round & 1) ? A.list1 : A.list0;
+    if (FIRST) n = A.ntiles;
+    else n = A.counts[round % 3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        A.counts[(round + 2) % 3] = 0;
+        if (!FIRST && n > 0) {
+            A.stats[0] += 1;
+            A.stats[1] += n;
         }
-        t = __shfl(t, 0, 64);
-        if (t < 0) break;
-        __threadfence();   // acquire the data released by the tile's activators
+    }
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int t = FIRST ? i : list[i];
         ImgDesc D;
         int lt;
         tile_of(A, t, D, lt);
-        process_tile<T, MODE, CONN, false>(A, D, lt, lds);
-        if (lane == 0) {
-            if (atomicCAS(&A.Q.tstate[t], TQ_RUN, 0u) != TQ_RUN) {
-                // activated while running: owe another visit
-                atomicExch(&A.Q.tstate[t], TQ_DIRTY);
-                q_push(A.Q, t);
-            }
-            atomicAdd(&A.Q.ctl[4], 1u);
-            atomicSub(&A.Q.ctl[2], 1u);
-        }
+        process_tile<T, MODE, CONN, FIRST>(A, D, lt, round, lds);
     }
 }
 
-__global__ void recon_reset_kernel(Queue Q, int ntiles, int cap) {
+__global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts, int64_t* stats) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int stride = gridDim.x * blockDim.x;
-    for (int j = i; j < ntiles; j += stride) Q.tstate[j] = 0u;
-    for (int j = i; j < cap; j += stride) Q.ring[j] = RING_EMPTY;
-    if (i < 8) Q.ctl[i] = 0u;
+    for (int j = i; j < ntiles; j += gridDim.x * blockDim.x) stamp[j] = 0u;
+    if (i < 3) counts[i] = 0;
+    if (i == 0) { stats[0] = 0; stats[1] = 0; stats[2] = ntiles; }
 }
 
-__global__ void recon_stats_kernel(Queue Q, int ntiles, int64_t* stats) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        stats[0] = Q.ctl[3];            // error bits (0 = converged)
-        stats[1] = (int64_t)Q.ctl[4];   // drain visits
-        stats[2] = ntiles;
-        stats[3] |= (int64_t)Q.ctl[3];  // sticky until the host reads it
-    }
-}
+typedef void (*RoundFn)(const ReconArgs&, int, int, hipStream_t);
 
-template <typename T, int MODE, int CONN>
-nsx_status run_typed(nsx_ctx* ctx, const ReconArgs& A, int cap, int workers, hipStream_t s) {
-    const int grid_reset = std::max(1, std::min(1024, (std::max(A.ntiles, cap) + 255) / 256));
-    hipLaunchKernelGGL(recon_reset_kernel, dim3(grid_reset), dim3(256), 0, s, A.Q, A.ntiles, cap);
-    const int h0 = ctx->tic(TK_RECON_FIRST, s);
-    hipLaunchKernelGGL((recon_first_kernel<T, MODE, CONN>), dim3(A.ntiles), dim3(64), 0, s, A);
-    ctx->toc(h0, s);
-    const int h1 = ctx->tic(TK_RECON_ROUND, s);
-    hipLaunchKernelGGL((recon_drain_kernel<T, MODE, CONN>), dim3(workers), dim3(64), 0, s, A);
-    ctx->toc(h1, s);
-    hipLaunchKernelGGL(recon_stats_kernel, dim3(1), dim3(64), 0, s, A.Q, A.ntiles, ctx->d_stats);
-    NSX_HIP_TRY(ctx, hipGetLastError());
-    return NSX_OK;
+template <typename T, int MODE, int CONN, bool FIRST>
+void launch_round(const ReconArgs& A, int round, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((recon_round_kernel<T, MODE, CONN, FIRST>), dim3(grid), dim3(64), 0, s, A, round);
 }
 
 template <typename T, int MODE>
-nsx_status run_conn(nsx_ctx* ctx, const ReconArgs& A, int conn, int cap, int workers, hipStream_t s) {
-    if (conn == 8) return run_typed<T, MODE, 8>(ctx, A, cap, workers, s);
-    return run_typed<T, MODE, 4>(ctx, A, cap, workers, s);
+bool pick(int conn, RoundFn* first, RoundFn* later) {
+    if (conn == 8) {
+        *first = launch_round<T, MODE, 8, true>;
+        *later = launch_round<T, MODE, 8, false>;
+    } else {
+        *first = launch_round<T, MODE, 4, true>;
+        *later = launch_round<T, MODE, 4, false>;
+    }
+    return true;
+}
+
+bool select_fns(int mode, int dtype, int conn, RoundFn* first, RoundFn* later) {
+    switch (mode) {
+        case RM_RECON:
+            if (dtype == NSX_U8) return pick<uint8_t, RM_RECON>(conn, first, later);
+            if (dtype == NSX_I32) return pick<int32_t, RM_RECON>(conn, first, later);
+            return pick<float, RM_RECON>(conn, first, later);
+        case RM_BINARY:
+            if (dtype == NSX_U8) return pick<uint8_t, RM_BINARY>(conn, first, later);
+            if (dtype == NSX_I32) return pick<int32_t, RM_BINARY>(conn, first, later);
+            return false;
+        case RM_FILL:
+            if (dtype == NSX_U8) return pick<uint8_t, RM_FILL>(conn, first, later);
+            if (dtype == NSX_I32) return pick<int32_t, RM_FILL>(conn, first, later);
+            return false;
+        case RM_FILLBIN:
+            if (dtype == NSX_U8) return pick<uint8_t, RM_FILLBIN>(conn, first, later);
+            return false;
+        default:
+            return false;
+    }
 }
 
 ImgDesc make_desc(const View* marker, const View& mask, const View& state, int tile_base) {
@@ -515,56 +437,18 @@ ImgDesc make_desc(const View* marker, const View& mask, const View& state, int t
 
 }  // namespace
 
+// Host-side state of one reconstruction between its launch and its check.
+struct ReconRunImpl {
+    ReconArgs A;
+    RoundFn later;
+    int grid;
+    int next_round;   // first round not yet launched
+    uint64_t key;     // round-count memory key
+};
+
 int recon_tiles(int rows, int cols) {
     if (rows <= 0 || cols <= 0) return 0;
     return ((cols + TI - 1) / TI) * ((rows + TI - 1) / TI);
-}
-
-nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const ReconBatch& B, hipStream_t s) {
-    ReconArgs A{};
-    A.nimg = B.nimg;
-    A.ntiles = B.ntiles;
-    if (A.ntiles <= 0) return NSX_OK;
-    if (B.nimg == 1) A.img0 = *reinterpret_cast<const ImgDesc*>(B.host_desc);
-    A.imgs = reinterpret_cast<const ImgDesc*>(B.dev_desc);
-    A.tile_img = B.tile_img;
-    int cap = 1;
-    while (cap < A.ntiles) cap <<= 1;
-    void *tstate, *ring, *ctl;
-    NSX_TRY(ctx->ws(B.ws_base + 0, sizeof(uint32_t) * (size_t)A.ntiles, &tstate));
-    NSX_TRY(ctx->ws(B.ws_base + 1, sizeof(int32_t) * (size_t)cap, &ring));
-    NSX_TRY(ctx->ws(B.ws_base + 2, 64, &ctl));
-    A.Q.tstate = (uint32_t*)tstate;
-    A.Q.ring = (int32_t*)ring;
-    A.Q.mask = (uint32_t)(cap - 1);
-    A.Q.ctl = (uint32_t*)ctl;
-    // Every visit raises at least one pixel or ends a tile's chain, so the visit
-    // count is bounded by the value range; this generous cap only catches a
-    // drain that stopped making progress while still visiting.
-    A.Q.max_visits = (uint32_t)std::min<int64_t>(0x7FFFFFFF, std::max<int64_t>(1 << 20, (int64_t)A.ntiles * 512));
-    // resident waves: the drain kernel holds one wave per SIMD (>256 VGPR+AGPR)
-    const int per_cu = 4;
-    const int workers = std::max(1, std::min(A.ntiles, per_cu * ctx->cu_count));
-    switch (mode) {
-        case RM_RECON:
-            if (dtype == NSX_U8) return run_conn<uint8_t, RM_RECON>(ctx, A, conn, cap, workers, s);
-            if (dtype == NSX_I32) return run_conn<int32_t, RM_RECON>(ctx, A, conn, cap, workers, s);
-            return run_conn<float, RM_RECON>(ctx, A, conn, cap, workers, s);
-        case RM_BINARY:
-            if (dtype == NSX_U8) return run_conn<uint8_t, RM_BINARY>(ctx, A, conn, cap, workers, s);
-            if (dtype == NSX_I32) return run_conn<int32_t, RM_BINARY>(ctx, A, conn, cap, workers, s);
-            break;
-        case RM_FILL:
-            if (dtype == NSX_U8) return run_conn<uint8_t, RM_FILL>(ctx, A, conn, cap, workers, s);
-            if (dtype == NSX_I32) return run_conn<int32_t, RM_FILL>(ctx, A, conn, cap, workers, s);
-            break;
-        case RM_FILLBIN:
-            if (dtype == NSX_U8) return run_conn<uint8_t, RM_FILLBIN>(ctx, A, conn, cap, workers, s);
-            break;
-        default:
-            break;
-    }
-    return ctx->fail(NSX_ERR_UNSUPPORTED, "reconstruction mode/dtype not instantiated by the reference");
 }
 
 size_t recon_desc_bytes() { return sizeof(ImgDesc); }
@@ -573,16 +457,113 @@ void recon_fill_desc(void* out, const View* marker, const View& mask, const View
     *reinterpret_cast<ImgDesc*>(out) = make_desc(marker, mask, state, tile_base);
 }
 
+void ReconRun::release() {
+    delete impl;
+    impl = nullptr;
+}
+
+nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const ReconBatch& B, hipStream_t s,
+                        ReconRun* run, int32_t* pending_dev) {
+    run->release();
+    if (B.ntiles <= 0) return NSX_OK;
+    RoundFn first, later;
+    if (!select_fns(mode, dtype, conn, &first, &later))
+        return ctx->fail(NSX_ERR_UNSUPPORTED, "reconstruction mode/dtype not instantiated by the reference");
+    ReconRunImpl* R = new ReconRunImpl();
+    ReconArgs& A = R->A;
+    A = ReconArgs{};
+    A.nimg = B.nimg;
+    A.ntiles = B.ntiles;
+    if (B.nimg == 1) A.img0 = *reinterpret_cast<const ImgDesc*>(B.host_desc);
+    A.imgs = reinterpret_cast<const ImgDesc*>(B.dev_desc);
+    A.tile_img = B.tile_img;
+    void *stamp, *lists, *counts;
+    nsx_status st = ctx->ws(B.ws_base + 0, sizeof(uint32_t) * (size_t)A.ntiles, &stamp);
+    if (st == NSX_OK) st = ctx->ws(B.ws_base + 1, 2 * sizeof(int32_t) * (size_t)A.ntiles, &lists);
+    if (st == NSX_OK) st = ctx->ws(B.ws_base + 2, 64, &counts);
+    if (st != NSX_OK) { delete R; return st; }
+    A.stamp = (uint32_t*)stamp;
+    A.list0 = (int32_t*)lists;
+    A.list1 = (int32_t*)lists + A.ntiles;
+    A.counts = (int32_t*)counts;
+    A.stats = B.stats ? B.stats : ctx->d_stats;
+    R->later = later;
+    R->grid = std::max(1, std::min(A.ntiles, 8 * ctx->cu_count));
+    R->key = ((uint64_t)A.ntiles << 16) ^ ((uint64_t)mode << 8) ^ ((uint64_t)dtype << 4) ^ (uint64_t)conn;
+    run->impl = R;
+
+    const int grid_reset = std::max(1, std::min(1024, (A.ntiles + 255) / 256));
+    hipLaunchKernelGGL(recon_reset_kernel, dim3(grid_reset), dim3(256), 0, s, A.stamp, A.ntiles, A.counts, A.stats);
+    const int h0 = ctx->tic(TK_RECON_FIRST, s);
+    first(A, 0, A.ntiles, s);
+    ctx->toc(h0, s);
+    // Rounds the same-sized problem needed last time, launched without a host check.
+    auto it = ctx->round_hint.find(R->key);
+    const int hint = it != ctx->round_hint.end() ? it->second : 4;
+    for (int r = 1; r <= hint; ++r) {
+        const int h = ctx->tic(TK_RECON_ROUND, s);
+        later(A, r, R->grid, s);
+        ctx->toc(h, s);
+    }
+    R->next_round = hint + 1;
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    if (pending_dev)
+        NSX_HIP_TRY(ctx, hipMemcpyAsync(pending_dev, &A.counts[R->next_round % 3], sizeof(int32_t),
+                                        hipMemcpyDeviceToDevice, s));
+    return NSX_OK;
+}
+
+nsx_status recon_complete(nsx_ctx* ctx, ReconRun* run, hipStream_t s, bool* extended) {
+    if (extended) *extended = false;
+    ReconRunImpl* R = run->impl;
+    if (!R) return NSX_OK;
+    const ReconArgs& A = R->A;
+    int batch = 2;
+    while (true) {
+        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, &A.counts[R->next_round % 3], sizeof(int32_t),
+                                        hipMemcpyDeviceToHost, s));
+        NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+        if (ctx->h_pinned[0] == 0) break;
+        if (extended) *extended = true;
+        if (R->next_round > (1 << 24)) return ctx->fail(NSX_ERR_NOT_CONVERGED, "reconstruction did not converge");
+        for (int j = 0; j < batch; ++j, ++R->next_round) {
+            const int h = ctx->tic(TK_RECON_ROUND, s);
+            R->later(A, R->next_round, R->grid, s);
+            ctx->toc(h, s);
+        }
+        NSX_HIP_TRY(ctx, hipGetLastError());
+        batch = std::min(batch * 2, 64);
+    }
+    // remember how many rounds this size needed (the last launched round had work
+    // or was the first empty one); read the device round counter for precision
+    int64_t st[2];
+    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 32, A.stats, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+    std::memcpy(st, ctx->h_pinned + 32, sizeof(st));
+    ctx->round_hint[R->key] = (int)std::min<int64_t>(4096, st[0] + 1);
+    run->release();
+    return NSX_OK;
+}
+
+void recon_note_rounds(nsx_ctx* ctx, ReconRun* run, int64_t rounds_with_work) {
+    if (run->impl) ctx->round_hint[run->impl->key] = (int)std::min<int64_t>(4096, rounds_with_work + 1);
+    run->release();
+}
+
 nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& state,
                      int conn, hipStream_t s) {
+    if (mask.rows <= 0 || mask.cols <= 0) return NSX_OK;
     ImgDesc D = make_desc(marker, mask, state, 0);
     ReconBatch B{};
     B.nimg = 1;
     B.ntiles = D.ntx * D.nty;
     B.host_desc = &D;
     B.ws_base = WS_RECON_STAMP;
-    if (mask.rows <= 0 || mask.cols <= 0) return NSX_OK;
-    return recon_launch(ctx, mode, mask.dtype, conn == 8 ? 8 : 4, B, s);
+    ReconRun run;
+    NSX_TRY(recon_launch(ctx, mode, mask.dtype, conn == 8 ? 8 : 4, B, s, &run, nullptr));
+    nsx_status st = recon_complete(ctx, &run, s, nullptr);
+    run.release();
+    return st;
 }
 
 }  // namespace nsx

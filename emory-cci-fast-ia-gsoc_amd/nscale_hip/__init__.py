@@ -146,7 +146,7 @@ class Context:
     def recon_stats(self) -> dict:
         s = (C.c_int64 * 3)()
         self.check(self.lib.nsx_last_recon_stats(self.h, s))
-        return {"queue_error": int(s[0]), "drain_visits": int(s[1]), "tiles": int(s[2])}
+        return {"rounds": int(s[0]), "tile_visits": int(s[1]), "tiles": int(s[2])}
 
 
 _tls = threading.local()

@@ -64,8 +64,8 @@ nsx_status nsx_ctx_synchronize(nsx_ctx* ctx);
 /* Device bytes currently held by the context's workspace. */
 size_t nsx_ctx_workspace_bytes(const nsx_ctx* ctx);
 /* Reads the statistics of the last reconstruction run on this context:
- * stats[0] = work-queue error bits (0 = converged), stats[1] = tile visits of
- * the drain after the first full pass, stats[2] = tiles in the image.
+ * stats[0] = propagation rounds that had work after the first full pass,
+ * stats[1] = tile visits in those rounds, stats[2] = tiles in the image.
  * Synchronises the device. */
 nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int64_t stats[3]);
 
