@@ -45,6 +45,54 @@ def parse():
     return p.parse_args()
 
 
+TILE_PX = 62 * 62   # interior pixels of one reconstruction tile visit
+
+
+def roofline(kt: dict, px: int, visits: tuple, steps: int) -> dict:
+    """HBM roofline of the kernel kind with the most time per step.
+
+    Algorithmic bytes per pixel (compulsory I/O of the operation a launch
+    performs): reconstruction 3 B (marker, mask in; state out), fillHoles 2 B
+    (image in; state out), labelling read 1 B, label output 1 B in + 4 B out.
+    A reconstruction first pass processes every pixel of the image; a
+    propagation round processes the tiles it visits (62x62 pixels each).
+    """
+    v_recon, v_fill = visits
+    alg_step = {
+        "recon_first": (3 + 2) * px,
+        "recon_round": (3 * v_recon + 2 * v_fill) * TILE_PX,
+        "ccl_local": 1 * px,
+        "ccl_final": 5 * px,
+        "ccl_merge": 0,
+        "ccl_small": 0,
+    }
+    kind = max(kt, key=lambda k: kt[k][0])
+    ms_step = kt[kind][0] / steps
+    launches_step = kt[kind][1] / steps
+    alg_launch = alg_step[kind] / launches_step if launches_step else 0.0
+    avg_launch_s = ms_step / launches_step / 1e3 if launches_step else float("nan")
+    achieved = alg_launch / avg_launch_s / 1e9 if launches_step else 0.0
+    return {"bound": "hbm", "kernel": kind, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "alg_bytes_per_launch": int(alg_launch), "avg_launch_us": round(avg_launch_s * 1e6, 3),
+            "launches_per_step": round(launches_step, 2),
+            "per_kind_ms_per_step": {k: round(v[0] / steps, 4) for k, v in kt.items()}}
+
+
+def cpu_baseline(marker, mask, runs: int) -> dict:
+    """The oracle's single-threaded restatement of the reference chain, timed
+    on this host on `runs` repetitions of the same 4096x4096 workload."""
+    from oracle import oracle as orc
+    orc.chain_u8(marker, mask)   # page in
+    t0 = time.perf_counter()
+    for _ in range(runs):
+        orc.chain_u8(marker, mask, 8, 4, 128, 8, 11, 1000)
+    dt = time.perf_counter() - t0
+    return {"value": round(runs * mask.size / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": f"{runs} runs of the full {mask.shape[0]}x{mask.shape[1]} chain, single thread, "
+                      f"{dt:.1f} s", "host_cpus": os.cpu_count()}
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -100,9 +148,13 @@ def main():
     elapsed = time.perf_counter() - t0
     kt = ctx.read_timing()
     ctx.set_timing(False)
-    stats = ctx.recon_stats()
+    stats = ctx.recon_stats(0)
+    fstats = ctx.recon_stats(1)
     log("kernel time per step (ms): " + ", ".join(f"{k}={v[0] / args.steps:.4f} ({v[1] / args.steps:.1f} launches)"
                                                   for k, v in kt.items()))
+    log(f"recon stage {stats}; fill stage {fstats}")
+    roof = roofline(kt, n * n, (stats["tile_visits"], fstats["tile_visits"]), args.steps)
+    cpu = None if args.no_cpu else cpu_baseline(marker, mask, args.cpu_runs)
     px = n * n
     value = args.steps * px / elapsed / 1e6
     lat_sorted = sorted(lat)
@@ -127,7 +179,11 @@ def main():
                                "bwlabel8 -> bwareaopen[11,1000) -> relabel",
                    "global_batch": 1, "pixels": px, "components": ncomp,
                    "recon_rounds": stats["rounds"], "recon_tile_visits": stats["tile_visits"],
-                   "tiles": stats["tiles"]},
+                   "fill_rounds": fstats["rounds"], "fill_tile_visits": fstats["tile_visits"],
+                   "tiles": stats["tiles"], "step_alg_bytes": 10 * px,
+                   "step_hbm_frac": round(10 * px / (elapsed / args.steps) / (HBM_PEAK_GBS * 1e9), 5)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

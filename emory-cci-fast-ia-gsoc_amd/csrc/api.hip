@@ -234,14 +234,15 @@ nsx_status nsx_ctx_read_timing(nsx_ctx* ctx, double* ms, int64_t* launches, int 
     return NSX_OK;
 }
 
-nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int64_t stats[3]) {
-    if (!ctx || !stats) return NSX_ERR_INVALID;
+nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int stage, int64_t stats[4]) {
+    if (!ctx || !stats || stage < 0 || stage > 1) return NSX_ERR_INVALID;
     int64_t h[4];
     NSX_HIP_TRY(ctx, hipDeviceSynchronize());
-    NSX_HIP_TRY(ctx, hipMemcpy(h, ctx->d_stats, sizeof(h), hipMemcpyDeviceToHost));
+    NSX_HIP_TRY(ctx, hipMemcpy(h, ctx->d_stats + 8 * stage, sizeof(h), hipMemcpyDeviceToHost));
     stats[0] = h[0];
     stats[1] = h[1];
     stats[2] = h[2];
+    stats[3] = h[3];
     return NSX_OK;
 }
 

@@ -26,6 +26,7 @@
 #include "nsx_internal.h"
 
 #include <algorithm>
+#include <cmath>
 
 namespace nsx {
 namespace {
@@ -39,7 +40,7 @@ constexpr uint32_t BGL = 0xFFFFFFFFu;
 
 struct CclArgs {
     const char* in; int64_t pin; int in_dtype;
-    bool thresh; double thr;
+    bool thresh; int64_t thr_i; float thr_f;
     int rows, cols;
     int64_t npix;
     int32_t* lab;        // sparse: boundary pixels + local roots
@@ -56,14 +57,18 @@ struct CclArgs {
 };
 
 // Pixel value used for connectivity: raw u8, or (in > thr) for the fused
-// threshold (OpenCV `Mat > k`, 255/0 -> here 1/0).
+// threshold (OpenCV `Mat > k`, 255/0 -> here 1/0).  The threshold is
+// pre-converted on the host to the input type's comparison (thr_i for
+// integers: v > k  <=>  v > floor(k); thr_f for float).
+template <typename InT, bool THR>
 __device__ __forceinline__ uint32_t pix(const CclArgs& A, int y, int x) {
-    const char* row = A.in + (int64_t)y * A.pin;
-    if (!A.thresh) return (uint32_t)((const uint8_t*)row)[x];
-    switch (A.in_dtype) {
-        case NSX_U8: return (double)((const uint8_t*)row)[x] > A.thr ? 1u : 0u;
-        case NSX_I32: return (double)((const int32_t*)row)[x] > A.thr ? 1u : 0u;
-        default: return (double)((const float*)row)[x] > A.thr ? 1u : 0u;
+    const InT v = reinterpret_cast<const InT*>(A.in + (int64_t)y * A.pin)[x];
+    if constexpr (!THR) {
+        return (uint32_t)v;
+    } else if constexpr (sizeof(InT) == 4 && !__is_same(InT, int32_t)) {
+        return v > A.thr_f ? 1u : 0u;
+    } else {
+        return (int64_t)v > A.thr_i ? 1u : 0u;
     }
 }
 
@@ -105,19 +110,23 @@ __device__ __forceinline__ void g_union(int32_t* lab, int32_t a, int32_t b) {
 
 // Tile-local labelling shared by K1 and K6.  On return val[] holds the pixel
 // values and par[i] the local root of pixel i (BGL for background).
-template <int CONN>
+template <int CONN, typename InT, bool THR>
 __device__ __forceinline__ void local_label(const CclArgs& A, int x0, int y0, uint32_t* val, uint32_t* par) {
     const int tid = threadIdx.x;
     const int lx = tid & 63, w = tid >> 6;
     const int gx = x0 + lx;
+    // all loads first (independent, in flight together), then the LDS writes
+    uint32_t v[CPT];
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-        const int ly = w + 4 * j;
-        const int gy = y0 + ly;
-        const int i = ly * CW + lx;
-        const uint32_t v = (gx < A.cols && gy < A.rows) ? pix(A, gy, gx) : 0u;
-        val[i] = v;
-        par[i] = v ? (uint32_t)i : BGL;
+        const int gy = y0 + w + 4 * j;
+        v[j] = (gx < A.cols && gy < A.rows) ? pix<InT, THR>(A, gy, gx) : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int i = (w + 4 * j) * CW + lx;
+        val[i] = v[j];
+        par[i] = v[j] ? (uint32_t)i : BGL;
     }
     __syncthreads();
 #pragma unroll
@@ -148,7 +157,7 @@ __device__ __forceinline__ int64_t gidx(const CclArgs& A, int y0, int x0, uint32
     return (int64_t)(y0 + (int)(li >> 6)) * A.cols + x0 + (int)(li & 63);
 }
 
-template <int CONN>
+template <int CONN, typename InT, bool THR>
 __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
     __shared__ uint32_t val[CN];
     __shared__ uint32_t par[CN];
@@ -158,7 +167,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
     const int x0 = blockIdx.x * CW, y0 = blockIdx.y * CH;
     for (int i = tid; i < CN; i += CTHREADS) cnt[i] = 0;
     if (tid == 0) nroots = 0;
-    local_label<CONN>(A, x0, y0, val, par);
+    local_label<CONN, InT, THR>(A, x0, y0, val, par);
     const int lx = tid & 63, w = tid >> 6;
     const int gx = x0 + lx;
     // pixel counts per local root, bitmap words of this tile cleared
@@ -204,7 +213,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
 // threads 0..63 the top row (up, up-left, up-right), 64..95 the left column
 // (left, and up-left below the top row), 96..127 the right column (up-right
 // below the top row).
-template <int CONN>
+template <int CONN, typename InT, bool THR>
 __global__ __launch_bounds__(128) void ccl_merge_kernel(CclArgs A) {
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * CW, y0 = blockIdx.y * CH;
@@ -213,12 +222,12 @@ __global__ __launch_bounds__(128) void ccl_merge_kernel(CclArgs A) {
     else if (tid < 96) { y = y0 + (tid - 64); x = x0; }
     else { y = y0 + 1 + (tid - 96); x = x0 + CW - 1; if (tid == 127) return; }
     if (x >= A.cols || y >= A.rows) return;
-    const uint32_t v = pix(A, y, x);
+    const uint32_t v = pix<InT, THR>(A,y, x);
     if (!v) return;
     const int64_t g = (int64_t)y * A.cols + x;
     auto join = [&](int qy, int qx) {
         if (qy < 0 || qx < 0 || qx >= A.cols) return;
-        if (pix(A, qy, qx) != v) return;
+        if (pix<InT, THR>(A,qy, qx) != v) return;
         const int64_t q = (int64_t)qy * A.cols + qx;
         g_union(A.lab, A.lab[g], A.lab[q]);
     };
@@ -330,14 +339,14 @@ __device__ __forceinline__ int32_t rank1(const CclArgs& A, int32_t r) {
     return (int32_t)(A.blockpre[r >> 15] + A.wordpre[r >> 5] + __popc(w & ((1u << (r & 31)) - 1u))) + 1;
 }
 
-template <int CONN>
+template <int CONN, typename InT, bool THR>
 __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
     __shared__ uint32_t val[CN];
     __shared__ uint32_t par[CN];
     __shared__ int32_t outv[CN];
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * CW, y0 = blockIdx.y * CH;
-    local_label<CONN>(A, x0, y0, val, par);
+    local_label<CONN, InT, THR>(A, x0, y0, val, par);
     const int lx = tid & 63, w = tid >> 6;
     const int gx = x0 + lx;
     // each local root resolves its global root once
@@ -376,16 +385,16 @@ __global__ void zero_i32_kernel(int32_t* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-template <int CONN>
+template <int CONN, typename InT, bool THR>
 nsx_status ccl_run_conn(nsx_ctx* ctx, CclArgs A, hipStream_t s) {
     const dim3 tiles((A.cols + CW - 1) / CW, (A.rows + CH - 1) / CH);
     const int stream_grid = 4 * ctx->cu_count;
     hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, s, A.nentries, 2);
     int h = ctx->tic(TK_CCL_LOCAL, s);
-    hipLaunchKernelGGL(ccl_local_kernel<CONN>, tiles, dim3(CTHREADS), 0, s, A);
+    hipLaunchKernelGGL((ccl_local_kernel<CONN, InT, THR>), tiles, dim3(CTHREADS), 0, s, A);
     ctx->toc(h, s);
     h = ctx->tic(TK_CCL_MERGE, s);
-    hipLaunchKernelGGL(ccl_merge_kernel<CONN>, tiles, dim3(128), 0, s, A);
+    hipLaunchKernelGGL((ccl_merge_kernel<CONN, InT, THR>), tiles, dim3(128), 0, s, A);
     ctx->toc(h, s);
     h = ctx->tic(TK_CCL_SMALL, s);
     if (A.use_area) hipLaunchKernelGGL(ccl_area_kernel, dim3(stream_grid), dim3(256), 0, s, A);
@@ -397,7 +406,7 @@ nsx_status ccl_run_conn(nsx_ctx* ctx, CclArgs A, hipStream_t s) {
     }
     ctx->toc(h, s);
     h = ctx->tic(TK_CCL_FINAL, s);
-    hipLaunchKernelGGL(ccl_final_kernel<CONN>, tiles, dim3(CTHREADS), 0, s, A);
+    hipLaunchKernelGGL((ccl_final_kernel<CONN, InT, THR>), tiles, dim3(CTHREADS), 0, s, A);
     ctx->toc(h, s);
     NSX_HIP_TRY(ctx, hipGetLastError());
     return NSX_OK;
@@ -488,7 +497,11 @@ nsx_status ccl_run(nsx_ctx* ctx, const View& in, const View& out, const CclParam
                    hipStream_t s) {
     CclArgs A{};
     A.in = in.data; A.pin = in.pitch; A.in_dtype = in.dtype;
-    A.thresh = p.threshold; A.thr = p.thr;
+    A.thresh = p.threshold;
+    // v > k for an integer v  <=>  v > floor(k); for float, compare in float
+    // (the reference compares a float Mat with a scalar converted to float).
+    A.thr_i = (int64_t)std::floor(std::max(-9.0e18, std::min(9.0e18, p.thr)));
+    A.thr_f = (float)p.thr;
     A.rows = in.rows; A.cols = in.cols;
     A.npix = (int64_t)in.rows * in.cols;
     if (A.npix == 0) return NSX_OK;
@@ -512,7 +525,17 @@ nsx_status ccl_run(nsx_ctx* ctx, const View& in, const View& out, const CclParam
     A.bits = (uint32_t*)bits;
     A.wordpre = (uint32_t*)scan;
     A.blockpre = scan ? (uint32_t*)scan + A.nwords : nullptr;
-    nsx_status st = p.conn == 8 ? ccl_run_conn<8>(ctx, A, s) : ccl_run_conn<4>(ctx, A, s);
+    nsx_status st;
+    if (!p.threshold) {
+        if (in.dtype != NSX_U8) return ctx->fail(NSX_ERR_INVALID, "labelling needs u8 input or a threshold");
+        st = p.conn == 8 ? ccl_run_conn<8, uint8_t, false>(ctx, A, s) : ccl_run_conn<4, uint8_t, false>(ctx, A, s);
+    } else if (in.dtype == NSX_U8) {
+        st = p.conn == 8 ? ccl_run_conn<8, uint8_t, true>(ctx, A, s) : ccl_run_conn<4, uint8_t, true>(ctx, A, s);
+    } else if (in.dtype == NSX_I32) {
+        st = p.conn == 8 ? ccl_run_conn<8, int32_t, true>(ctx, A, s) : ccl_run_conn<4, int32_t, true>(ctx, A, s);
+    } else {
+        st = p.conn == 8 ? ccl_run_conn<8, float, true>(ctx, A, s) : ccl_run_conn<4, float, true>(ctx, A, s);
+    }
     if (st != NSX_OK) return st;
     if (ncomp_dev) NSX_HIP_TRY(ctx, hipMemcpyAsync(ncomp_dev, A.nentries + 1, sizeof(int32_t), hipMemcpyDeviceToDevice, s));
     return NSX_OK;

@@ -299,22 +299,43 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
     for (int k = 0; k < TS; ++k) mt[k] = m[k];
     transpose(mt, lds);
 
+    // Half-passes alternate between the column sweeps (orientation A: lane =
+    // column) and the row sweeps (after a transpose, orientation B: lane = row).
+    // Values only rise, so an unchanged tile sum means an unchanged tile.
+    // 4-connectivity: a half-pass is an exact 1-D reconstruction of every
+    // column (row) and so idempotent; one quiet half-pass after the other
+    // orientation ran proves the fixed point.  8-connectivity: the diagonal
+    // terms break idempotence, so two consecutive quiet half-passes are needed.
     const S tot_in = tile_sum<S>(c);
     S tot = tot_in;
     bool converged = false;
-    for (int it = 0; it < MAX_IT; ++it) {
-        sweep_pair<CONN>(c, m);        // along y
-        transpose(c, lds);
-        sweep_pair<CONN>(c, mt);       // along x
+    bool orientB = false;
+    int quiet = 0;
+    int half = 0;
+    for (; half < 2 * MAX_IT; ++half) {
+        sweep_pair<CONN>(c, orientB ? mt : m);
         const S t1 = tile_sum<S>(c);
-        if (t1 == tot) { converged = true; break; }
+        quiet = (t1 == tot) ? quiet + 1 : 0;
         tot = t1;
-        if (it + 1 == MAX_IT) break;
+        if (CONN == 4 ? (quiet >= 1 && half >= 1) : (quiet >= 2)) { converged = true; break; }
+        transpose(c, lds);
+        orientB = !orientB;
+    }
+    if (A.stats) {
+        if (lane == 0) atomicAdd((unsigned long long*)&A.stats[3], (unsigned long long)(half + 1));
+    }
+    uint32_t finL, finR;
+    if (orientB) {
+        // register 0 / TS-1 = left / right halo column
+        finL = c[0];
+        finR = c[TS - 1];
+        transpose(c, lds);
+    } else {
+        transpose(c, lds);
+        finL = c[0];
+        finR = c[TS - 1];
         transpose(c, lds);
     }
-    // c is in transposed orientation here: register 0 / TS-1 = left / right halo column.
-    const uint32_t finL = c[0], finR = c[TS - 1];
-    transpose(c, lds);
 
     if (FIRST || tot != tot_in) {
         if (lane >= 1 && lane <= TI && gx < D.cols) {
@@ -375,7 +396,7 @@ __global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts,
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     for (int j = i; j < ntiles; j += gridDim.x * blockDim.x) stamp[j] = 0u;
     if (i < 3) counts[i] = 0;
-    if (i == 0) { stats[0] = 0; stats[1] = 0; stats[2] = ntiles; }
+    if (i == 0) { stats[0] = 0; stats[1] = 0; stats[2] = ntiles; stats[3] = 0; }
 }
 
 typedef void (*RoundFn)(const ReconArgs&, int, int, hipStream_t);

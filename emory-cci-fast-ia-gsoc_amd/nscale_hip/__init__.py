@@ -143,10 +143,11 @@ class Context:
         self.check(self.lib.nsx_ctx_read_timing(self.h, ms, n, k))
         return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(self.TIMED_KINDS)}
 
-    def recon_stats(self) -> dict:
-        s = (C.c_int64 * 3)()
-        self.check(self.lib.nsx_last_recon_stats(self.h, s))
-        return {"rounds": int(s[0]), "tile_visits": int(s[1]), "tiles": int(s[2])}
+    def recon_stats(self, stage: int = 0) -> dict:
+        """Statistics of the last reconstruction (stage 1: the chain's fill)."""
+        s = (C.c_int64 * 4)()
+        self.check(self.lib.nsx_last_recon_stats(self.h, stage, s))
+        return {"rounds": int(s[0]), "tile_visits": int(s[1]), "tiles": int(s[2]), "half_passes": int(s[3])}
 
 
 _tls = threading.local()

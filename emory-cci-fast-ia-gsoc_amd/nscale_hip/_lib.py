@@ -42,7 +42,7 @@ SIGNATURES = {
     "nsx_last_error": (C.c_char_p, [_CTX]),
     "nsx_ctx_synchronize": (C.c_int, [_CTX]),
     "nsx_ctx_workspace_bytes": (C.c_size_t, [_CTX]),
-    "nsx_last_recon_stats": (C.c_int, [_CTX, P(C.c_int64)]),
+    "nsx_last_recon_stats": (C.c_int, [_CTX, C.c_int, P(C.c_int64)]),
     "nsx_ctx_set_timing": (C.c_int, [_CTX, C.c_int]),
     "nsx_ctx_read_timing": (C.c_int, [_CTX, P(C.c_double), P(C.c_int64), C.c_int]),
     "nsx_imreconstruct": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_void_p]),

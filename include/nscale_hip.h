@@ -63,11 +63,12 @@ const char* nsx_last_error(const nsx_ctx* ctx);
 nsx_status nsx_ctx_synchronize(nsx_ctx* ctx);
 /* Device bytes currently held by the context's workspace. */
 size_t nsx_ctx_workspace_bytes(const nsx_ctx* ctx);
-/* Reads the statistics of the last reconstruction run on this context:
- * stats[0] = propagation rounds that had work after the first full pass,
- * stats[1] = tile visits in those rounds, stats[2] = tiles in the image.
- * Synchronises the device. */
-nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int64_t stats[3]);
+/* Reads the statistics of the last reconstruction run on this context;
+ * stage 0 = the call's (first) reconstruction, stage 1 = the fill stage of
+ * nsx_segment_chain.  stats[0] = propagation rounds that had work after the
+ * first full pass, stats[1] = tile visits in those rounds, stats[2] = tiles,
+ * stats[3] = sweep half-passes summed over all visits.  Synchronises the device. */
+nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int stage, int64_t stats[4]);
 
 /* Kernel timing with HIP events recorded on the stream each kernel runs on.
  * nsx_ctx_set_timing(ctx, 1) starts recording (and clears the totals);
