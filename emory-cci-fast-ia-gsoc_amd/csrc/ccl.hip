@@ -72,9 +72,16 @@ __device__ __forceinline__ uint32_t pix(const CclArgs& A, int y, int x) {
     }
 }
 
-__device__ __forceinline__ uint32_t lds_find(const uint32_t* par, uint32_t x) {
+// Find with path halving: each visited node is re-pointed to its grandparent
+// (an ancestor), which concurrent atomicMin unions tolerate.
+__device__ __forceinline__ uint32_t lds_find(uint32_t* par, uint32_t x) {
     uint32_t p = par[x];
-    while (p != x) { x = p; p = par[x]; }
+    while (p != x) {
+        const uint32_t g = par[p];
+        if (g != p) par[x] = g;
+        x = p;
+        p = g;
+    }
     return x;
 }
 
@@ -122,26 +129,40 @@ __device__ __forceinline__ void local_label(const CclArgs& A, int x0, int y0, ui
         const int gy = y0 + w + 4 * j;
         v[j] = (gx < A.cols && gy < A.rows) ? pix<InT, THR>(A, gy, gx) : 0u;
     }
+    // Each wave holds whole tile rows (lane = column): a pixel's initial parent
+    // is the first pixel of its horizontal run of equal values, found with a
+    // ballot of run starts, so the unions below only join runs of adjacent rows.
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int i = (w + 4 * j) * CW + lx;
+        const uint32_t left = __shfl_up(v[j], 1, 64);
+        const bool start = v[j] != 0u && (lx == 0 || left != v[j]);
+        const uint64_t starts = __ballot(start);
+        const uint64_t upto = lx == 63 ? ~0ull : ((2ull << lx) - 1ull);
+        const int rs = 63 - __clzll(starts & upto);
         val[i] = v[j];
-        par[i] = v[j] ? (uint32_t)i : BGL;
+        par[i] = v[j] ? (uint32_t)((w + 4 * j) * CW + rs) : BGL;
     }
     __syncthreads();
+    // Join each pair of overlapping runs once, at the leftmost column where they
+    // touch: a vertical (or diagonal) link that a left neighbour in the same run
+    // already makes through the same run above is skipped.
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int ly = w + 4 * j;
         const int i = ly * CW + lx;
-        const uint32_t v = val[i];
-        if (!v) continue;
-        if (lx > 0 && val[i - 1] == v) lds_union(par, i, i - 1);
-        if (ly > 0) {
-            if (val[i - CW] == v) lds_union(par, i, i - CW);
-            if (CONN == 8) {
-                if (lx > 0 && val[i - CW - 1] == v) lds_union(par, i, i - CW - 1);
-                if (lx < CW - 1 && val[i - CW + 1] == v) lds_union(par, i, i - CW + 1);
-            }
+        const uint32_t vv = val[i];
+        if (!vv || ly == 0) continue;
+        const int u = i - CW;
+        const uint32_t vu = val[u];
+        const uint32_t vl = lx > 0 ? val[i - 1] : 0u;
+        const uint32_t vul = lx > 0 ? val[u - 1] : 0u;
+        const uint32_t vur = lx < CW - 1 ? val[u + 1] : 0u;
+        const uint32_t vr = lx < CW - 1 ? val[i + 1] : 0u;
+        if (vu == vv && !(vl == vv && vul == vv)) lds_union(par, i, u);
+        if (CONN == 8) {
+            if (vul == vv && vu != vv && vl != vv) lds_union(par, i, u - 1);
+            if (vur == vv && vu != vv && vr != vv) lds_union(par, i, u + 1);
         }
     }
     __syncthreads();

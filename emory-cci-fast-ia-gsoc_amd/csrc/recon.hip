@@ -312,18 +312,22 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
     bool orientB = false;
     int quiet = 0;
     int half = 0;
-    for (; half < 2 * MAX_IT; ++half) {
-        sweep_pair<CONN>(c, orientB ? mt : m);
-        const S t1 = tile_sum<S>(c);
+    auto settled = [&](const S t1) {
         quiet = (t1 == tot) ? quiet + 1 : 0;
         tot = t1;
-        if (CONN == 4 ? (quiet >= 1 && half >= 1) : (quiet >= 2)) { converged = true; break; }
+        return CONN == 4 ? (quiet >= 1 && half >= 1) : (quiet >= 2);
+    };
+    for (int it = 0; it < MAX_IT; ++it) {
+        sweep_pair<CONN>(c, m);                       // columns (orientation A)
+        if (settled(tile_sum<S>(c))) { converged = true; break; }
+        ++half;
         transpose(c, lds);
-        orientB = !orientB;
+        sweep_pair<CONN>(c, mt);                      // rows (orientation B)
+        if (settled(tile_sum<S>(c))) { converged = true; orientB = true; break; }
+        ++half;
+        transpose(c, lds);
     }
-    if (A.stats) {
-        if (lane == 0) atomicAdd((unsigned long long*)&A.stats[3], (unsigned long long)(half + 1));
-    }
+    if (lane == 0) atomicAdd((unsigned long long*)&A.stats[3], (unsigned long long)(half + 1));
     uint32_t finL, finR;
     if (orientB) {
         // register 0 / TS-1 = left / right halo column
