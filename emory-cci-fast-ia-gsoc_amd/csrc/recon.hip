@@ -169,28 +169,39 @@ __device__ __forceinline__ uint32_t nbr(uint32_t p) {
     else return p;
 }
 
+// The mask lives in LDS as mb[row * LDS_PITCH + col] (u8 keys for u8 images,
+// u32 otherwise).  Orientation A (lane = column, register = row) reads
+// mb[k * P + lane]; orientation B (lane = row, register = column) reads
+// mb[lane * P + k]; the odd pitch keeps both conflict-free.
+template <bool B, typename LT>
+__device__ __forceinline__ uint32_t mask_at(const LT* mb, int k) {
+    const int lane = threadIdx.x;
+    return (uint32_t)(B ? mb[lane * LDS_PITCH + k] : mb[k * LDS_PITCH + lane]);
+}
+
 // Forward sweep over registers 1..TS-1 then backward over TS-2..0.
-template <int CONN>
-__device__ __forceinline__ void sweep_pair(uint32_t (&c)[TS], const uint32_t (&m)[TS]) {
+template <int CONN, bool B, typename LT>
+__device__ __forceinline__ void sweep_pair(uint32_t (&c)[TS], const LT* mb) {
     uint32_t p = c[0];
 #pragma unroll
     for (int k = 1; k < TS; ++k) {
-        c[k] = med3u(nbr<CONN>(p), c[k], m[k]);
+        c[k] = med3u(nbr<CONN>(p), c[k], mask_at<B>(mb, k));
         p = c[k];
     }
     p = c[TS - 1];
 #pragma unroll
     for (int k = TS - 2; k >= 0; --k) {
-        c[k] = med3u(nbr<CONN>(p), c[k], m[k]);
+        c[k] = med3u(nbr<CONN>(p), c[k], mask_at<B>(mb, k));
         p = c[k];
     }
 }
 
 // Register tile transpose through LDS: lane l, register k <-> lane k, register l.
-__device__ __forceinline__ void transpose(uint32_t (&c)[TS], uint32_t* lds) {
+template <typename LT>
+__device__ __forceinline__ void transpose(uint32_t (&c)[TS], LT* lds) {
     const int lane = threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < TS; ++k) lds[k * LDS_PITCH + lane] = c[k];
+    for (int k = 0; k < TS; ++k) lds[k * LDS_PITCH + lane] = (LT)c[k];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < TS; ++k) c[k] = lds[lane * LDS_PITCH + k];
@@ -199,6 +210,10 @@ __device__ __forceinline__ void transpose(uint32_t (&c)[TS], uint32_t* lds) {
 
 template <typename T> struct SumT { typedef uint64_t type; };
 template <> struct SumT<uint8_t> { typedef uint32_t type; };
+// LDS element of the mask / transpose buffers: every key of a u8 image (value,
+// inverted value, 0/1 reached flag, frame) fits in a byte.
+template <typename T> struct LdsT { typedef uint32_t type; };
+template <> struct LdsT<uint8_t> { typedef uint8_t type; };
 
 template <typename S>
 __device__ __forceinline__ S wave_sum(S v) {
@@ -246,9 +261,13 @@ struct ReconArgs {
     int64_t* stats;            // [0] rounds with work [1] tile visits after round 0 [2] tiles
 };
 
+// The tile id is wave-uniform; readfirstlane makes that visible to the
+// compiler so the descriptor is fetched with scalar loads into SGPRs.
 __device__ __forceinline__ void tile_of(const ReconArgs& A, int32_t t, ImgDesc& D, int& lt) {
+    t = __builtin_amdgcn_readfirstlane(t);
     if (A.nimg == 1) { D = A.img0; lt = t; return; }
-    D = A.imgs[A.tile_img[t]];
+    const int idx = __builtin_amdgcn_readfirstlane(A.tile_img[t]);
+    D = A.imgs[idx];
     lt = t - D.tile_base;
 }
 
@@ -266,7 +285,9 @@ __device__ __forceinline__ void activate(const ReconArgs& A, const ImgDesc& D, i
 // Loads a tile (+ halo), runs it to its local fixed point, writes the interior
 // back and queues the tiles whose pixels in the halo rose.
 template <typename T, int MODE, int CONN, bool FIRST>
-__device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& D, int lt, int round, uint32_t* lds) {
+__device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& D, int lt, int round,
+                                             typename LdsT<T>::type* mb, typename LdsT<T>::type* tb) {
+    using LT = typename LdsT<T>::type;
     using C = Codec<T, MODE>;
     using S = typename SumT<T>::type;
     const int lane = threadIdx.x;
@@ -284,20 +305,19 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
         return C::dec(ld<T>(D.state, D.ps, gy, gx), av, mv);
     };
 
-    uint32_t c[TS], m[TS];
+    uint32_t c[TS];
     const int gx = x0 + lane;
 #pragma unroll
-    for (int k = 0; k < TS; ++k) c[k] = key_at(y0 + k, gx, m[k]);
+    for (int k = 0; k < TS; ++k) {
+        uint32_t mk;
+        c[k] = key_at(y0 + k, gx, mk);
+        mb[k * LDS_PITCH + lane] = (LT)mk;
+    }
     uint32_t dummy;
     const uint32_t initL = key_at(y0 + lane, x0, dummy);
     const uint32_t initR = key_at(y0 + lane, x0 + TS - 1, dummy);
     const uint32_t initT = c[0], initB = c[TS - 1];
-
-    // transposed mask for the horizontal sweeps
-    uint32_t mt[TS];
-#pragma unroll
-    for (int k = 0; k < TS; ++k) mt[k] = m[k];
-    transpose(mt, lds);
+    __syncthreads();
 
     // Half-passes alternate between the column sweeps (orientation A: lane =
     // column) and the row sweeps (after a transpose, orientation B: lane = row).
@@ -318,14 +338,14 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
         return CONN == 4 ? (quiet >= 1 && half >= 1) : (quiet >= 2);
     };
     for (int it = 0; it < MAX_IT; ++it) {
-        sweep_pair<CONN>(c, m);                       // columns (orientation A)
+        sweep_pair<CONN, false>(c, mb);               // columns (orientation A)
         if (settled(tile_sum<S>(c))) { converged = true; break; }
         ++half;
-        transpose(c, lds);
-        sweep_pair<CONN>(c, mt);                      // rows (orientation B)
+        transpose(c, tb);
+        sweep_pair<CONN, true>(c, mb);                // rows (orientation B)
         if (settled(tile_sum<S>(c))) { converged = true; orientB = true; break; }
         ++half;
-        transpose(c, lds);
+        transpose(c, tb);
     }
     if (lane == 0) atomicAdd((unsigned long long*)&A.stats[3], (unsigned long long)(half + 1));
     uint32_t finL, finR;
@@ -333,12 +353,12 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
         // register 0 / TS-1 = left / right halo column
         finL = c[0];
         finR = c[TS - 1];
-        transpose(c, lds);
+        transpose(c, tb);
     } else {
-        transpose(c, lds);
+        transpose(c, tb);
         finL = c[0];
         finR = c[TS - 1];
-        transpose(c, lds);
+        transpose(c, tb);
     }
 
     if (FIRST || tot != tot_in) {
@@ -375,7 +395,9 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
 
 template <typename T, int MODE, int CONN, bool FIRST>
 __global__ __launch_bounds__(64) void recon_round_kernel(ReconArgs A, int round) {
-    __shared__ uint32_t lds[TS * LDS_PITCH];
+    using LT = typename LdsT<T>::type;
+    __shared__ LT mb[TS * LDS_PITCH];   // mask keys of the tile
+    __shared__ LT tb[TS * LDS_PITCH];   // transpose staging
     int n;
     const int32_t* list = (round & 1) ? A.list1 : A.list0;
     if (FIRST) n = A.ntiles;
@@ -392,7 +414,8 @@ __global__ __launch_bounds__(64) void recon_round_kernel(ReconArgs A, int round)
         ImgDesc D;
         int lt;
         tile_of(A, t, D, lt);
-        process_tile<T, MODE, CONN, FIRST>(A, D, lt, round, lds);
+        process_tile<T, MODE, CONN, FIRST>(A, D, lt, round, mb, tb);
+        __syncthreads();   // the next tile overwrites mb
     }
 }
 
@@ -513,7 +536,8 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     A.counts = (int32_t*)counts;
     A.stats = B.stats ? B.stats : ctx->d_stats;
     R->later = later;
-    R->grid = std::max(1, std::min(A.ntiles, 8 * ctx->cu_count));
+    // u8 tiles run 3 waves/SIMD (12 per CU), 32-bit tiles 1 (LDS-bound)
+    R->grid = std::max(1, std::min(A.ntiles, (dtype == NSX_U8 ? 12 : 4) * ctx->cu_count));
     R->key = ((uint64_t)A.ntiles << 16) ^ ((uint64_t)mode << 8) ^ ((uint64_t)dtype << 4) ^ (uint64_t)conn;
     run->impl = R;
 
