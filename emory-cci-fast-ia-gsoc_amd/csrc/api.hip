@@ -399,6 +399,42 @@ nsx_status nsx_threshold_gt(nsx_ctx* ctx, const nsx_img* in, double k, nsx_img* 
     return from_device(ctx, vo, out, s);
 }
 
+nsx_status nsx_recon_label(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* mask, int conn, double thresh,
+                           int ccl_conn, nsx_img* recon_out, nsx_img* labels_out, int32_t* ncomp, void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(check_img(ctx, marker, "marker"));
+    NSX_TRY(check_img(ctx, mask, "mask"));
+    NSX_TRY(check_img(ctx, recon_out, "recon_out"));
+    NSX_TRY(check_img(ctx, labels_out, "labels_out"));
+    if (!same_shape(marker, mask) || !same_shape(mask, recon_out) || !same_shape(mask, labels_out))
+        return ctx->fail(NSX_ERR_INVALID, "all images must have the same size");
+    if (marker->dtype != mask->dtype || recon_out->dtype != mask->dtype)
+        return ctx->fail(NSX_ERR_INVALID, "marker, mask and recon_out must have the same type");
+    if (labels_out->dtype != NSX_I32) return ctx->fail(NSX_ERR_INVALID, "labels must be int32");
+    if (overlaps(recon_out, marker) || overlaps(recon_out, mask) || overlaps(labels_out, recon_out))
+        return ctx->fail(NSX_ERR_INVALID, "outputs alias inputs");
+    hipStream_t s = pick_stream(ctx, stream);
+    View vm, vk, vr, vl;
+    NSX_TRY(to_device(ctx, marker, WS_STAGE_A, true, s, &vm));
+    NSX_TRY(to_device(ctx, mask, WS_STAGE_B, true, s, &vk));
+    NSX_TRY(to_device(ctx, recon_out, WS_CHAIN_RECON, false, s, &vr));
+    NSX_TRY(to_device(ctx, labels_out, WS_STAGE_C, false, s, &vl));
+    NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vr, conn == 8 ? 8 : 4, s));
+    CclParams p;
+    p.conn = ccl_conn == 8 ? 8 : 4;
+    p.out_mode = CO_CANON;
+    p.threshold = true;
+    p.thr = thresh;
+    int32_t* dcount = (int32_t*)ctx->d_stats + 12;
+    NSX_TRY(ccl_run(ctx, vr, vl, p, dcount, s));
+    if (recon_out->mem == NSX_MEM_HOST) NSX_TRY(from_device(ctx, vr, recon_out, s));
+    NSX_TRY(from_device(ctx, vl, labels_out, s));
+    int32_t n = 0;
+    NSX_TRY(read_count(ctx, dcount, &n, s));
+    if (ncomp) *ncomp = n;
+    return NSX_OK;
+}
+
 nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* mask, int recon_conn, int fill_conn,
                              double thresh, int ccl_conn, int min_size, int max_size, nsx_img* recon_out,
                              nsx_img* fill_out, nsx_img* labels_out, int32_t* ncomp, void* stream) {

@@ -7,22 +7,29 @@
 // smaller index, ConnComponents.cpp:34-43, ccl_uf.cu:40-62); the canonical
 // relabel numbers those roots 1.. in raster order with background 0.
 //
+// One launch labels a batch of images (a single image is a batch of one).
+// Each image owns a range of a global pixel index space that starts at a
+// multiple of 32768, so the root bitmap's scan blocks never straddle images.
 // Kernels (tile = 64 columns x 32 rows, 256 threads, 8 pixels per thread):
-//   K1 local    union-find in LDS with ds_min; per local component append
-//               (root raster index, pixel count) to a compact table; write the
-//               local root index only for tile-boundary pixels and for the
-//               roots themselves (one sparse int32 array `lab`).
+//   K1 local    tile-local union-find in LDS: each row's runs of equal values
+//               get their first pixel as parent (wave ballot), then overlapping
+//               runs of adjacent rows are joined once with ds_min; finds halve
+//               paths.  Per local component (root global index, pixel count)
+//               goes to the tile's own slots of a component table; the local
+//               root index is written only for tile-edge pixels and roots (one
+//               sparse int32 array `lab`).
 //   K2 merge    pixel pairs that cross a tile edge are joined with global
 //               atomicMin union-find over `lab`.
 //   K3 area     per table entry: atomicAdd its count into area[find(root)].
 //   K4 keep     per table entry that is a global root: keep it if the area is
-//               in [min, max); set its bit in a raster-order bitmap.
-//   K5 scan     popcount prefix over the bitmap -> rank of every kept root.
+//               in [min, max); set its bit in the root bitmap.
+//   K5 scan     popcount prefix over the bitmap -> rank of every kept root and
+//               the kept count of every image.
 //   K6 final    recompute the tile-local labelling in LDS from the input (1 B
 //               per pixel instead of re-reading a 4 B label array), map each
 //               local root to its global root once, write the output.
 // HBM bytes per pixel: K1 reads 1, K6 reads 1 and writes 4 (u8 input, int32
-// output); everything else touches only boundary pixels and roots.
+// output); everything else touches only tile edges, roots and the bitmap.
 #include "nsx_internal.h"
 
 #include <algorithm>
@@ -33,36 +40,60 @@ namespace {
 
 constexpr int CW = 64;               // tile columns
 constexpr int CH = 32;               // tile rows
-constexpr int CN = CW * CH;          // pixels per tile
+constexpr int CN = CW * CH;          // pixels per tile (and table slots per tile)
 constexpr int CTHREADS = 256;
 constexpr int CPT = CN / CTHREADS;   // pixels per thread
 constexpr uint32_t BGL = 0xFFFFFFFFu;
+constexpr int BLOCK_BITS = 32768;    // bitmap scan block (1024 words)
+
+struct CclImg {
+    const char* in; int64_t pin;
+    char* out; int64_t pout;
+    int rows, cols, ntx, nty;
+    int tile_base;     // first tile of the image in the launch
+    int pix_base;      // global index of pixel (0,0); multiple of BLOCK_BITS
+};
 
 struct CclArgs {
-    const char* in; int64_t pin; int in_dtype;
-    bool thresh; int64_t thr_i; float thr_f;
-    int rows, cols;
-    int64_t npix;
-    int32_t* lab;        // sparse: boundary pixels + local roots
-    int32_t* area;       // sparse: roots
-    uint2* entries;      // (root raster index, count)
-    int32_t* nentries;   // [0] entries, [1] kept components
-    uint32_t* bits;      // kept-root bitmap
-    uint32_t* wordpre;   // exclusive popcount prefix inside each 1024-word block
-    uint32_t* blockpre;  // exclusive prefix of block totals
-    int64_t nwords;
+    CclImg img0;               // used when nimg == 1
+    const CclImg* imgs;        // device table when nimg > 1
+    const int32_t* tile_img;   // tile -> image when nimg > 1
+    int nimg, ntiles;
+    int64_t thr_i; float thr_f;
+    int32_t* lab;              // sparse: tile-edge pixels + local roots
+    int32_t* area;             // sparse: roots
+    uint2* entries;            // tile t's components at [t*CN, t*CN + tcount[t])
+    int32_t* tcount;           // components per tile
+    uint32_t* bits;            // kept-root bitmap over the global index space
+    uint32_t* wordpre;         // exclusive popcount prefix inside each 1024-word block
+    uint32_t* blockpre;        // exclusive prefix of block totals (+ total at [nblocks])
+    int64_t nwords; int nblocks;
+    int32_t* ncomp;            // kept components per image
     int min_size, max_size;
     bool use_area;
-    char* out; int64_t pout; int out_mode;
+    int out_mode;
 };
+
+__device__ __forceinline__ void tile_of(const CclArgs& A, int t, CclImg& D, int& tx, int& ty) {
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (A.nimg == 1) {
+        D = A.img0;
+    } else {
+        const int idx = __builtin_amdgcn_readfirstlane(A.tile_img[t]);
+        D = A.imgs[idx];
+    }
+    const int lt = t - D.tile_base;
+    ty = lt / D.ntx;
+    tx = lt - ty * D.ntx;
+}
 
 // Pixel value used for connectivity: raw u8, or (in > thr) for the fused
 // threshold (OpenCV `Mat > k`, 255/0 -> here 1/0).  The threshold is
 // pre-converted on the host to the input type's comparison (thr_i for
 // integers: v > k  <=>  v > floor(k); thr_f for float).
 template <typename InT, bool THR>
-__device__ __forceinline__ uint32_t pix(const CclArgs& A, int y, int x) {
-    const InT v = reinterpret_cast<const InT*>(A.in + (int64_t)y * A.pin)[x];
+__device__ __forceinline__ uint32_t pix(const CclArgs& A, const CclImg& D, int y, int x) {
+    const InT v = reinterpret_cast<const InT*>(D.in + (int64_t)y * D.pin)[x];
     if constexpr (!THR) {
         return (uint32_t)v;
     } else if constexpr (sizeof(InT) == 4 && !__is_same(InT, int32_t)) {
@@ -118,7 +149,8 @@ __device__ __forceinline__ void g_union(int32_t* lab, int32_t a, int32_t b) {
 // Tile-local labelling shared by K1 and K6.  On return val[] holds the pixel
 // values and par[i] the local root of pixel i (BGL for background).
 template <int CONN, typename InT, bool THR>
-__device__ __forceinline__ void local_label(const CclArgs& A, int x0, int y0, uint32_t* val, uint32_t* par) {
+__device__ __forceinline__ void local_label(const CclArgs& A, const CclImg& D, int x0, int y0, uint32_t* val,
+                                            uint32_t* par) {
     const int tid = threadIdx.x;
     const int lx = tid & 63, w = tid >> 6;
     const int gx = x0 + lx;
@@ -127,7 +159,7 @@ __device__ __forceinline__ void local_label(const CclArgs& A, int x0, int y0, ui
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int gy = y0 + w + 4 * j;
-        v[j] = (gx < A.cols && gy < A.rows) ? pix<InT, THR>(A, gy, gx) : 0u;
+        v[j] = (gx < D.cols && gy < D.rows) ? pix<InT, THR>(A, D, gy, gx) : 0u;
     }
     // Each wave holds whole tile rows (lane = column): a pixel's initial parent
     // is the first pixel of its horizontal run of equal values, found with a
@@ -174,8 +206,8 @@ __device__ __forceinline__ void local_label(const CclArgs& A, int x0, int y0, ui
     __syncthreads();
 }
 
-__device__ __forceinline__ int64_t gidx(const CclArgs& A, int y0, int x0, uint32_t li) {
-    return (int64_t)(y0 + (int)(li >> 6)) * A.cols + x0 + (int)(li & 63);
+__device__ __forceinline__ int32_t gidx(const CclImg& D, int y0, int x0, uint32_t li) {
+    return D.pix_base + (y0 + (int)(li >> 6)) * D.cols + x0 + (int)(li & 63);
 }
 
 template <int CONN, typename InT, bool THR>
@@ -183,73 +215,67 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
     __shared__ uint32_t val[CN];
     __shared__ uint32_t par[CN];
     __shared__ uint32_t cnt[CN];
-    __shared__ uint32_t nroots, base;
+    __shared__ uint32_t nroots;
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * CW, y0 = blockIdx.y * CH;
+    CclImg D;
+    int tx, ty;
+    tile_of(A, blockIdx.x, D, tx, ty);
+    const int x0 = tx * CW, y0 = ty * CH;
     for (int i = tid; i < CN; i += CTHREADS) cnt[i] = 0;
     if (tid == 0) nroots = 0;
-    local_label<CONN, InT, THR>(A, x0, y0, val, par);
+    local_label<CONN, InT, THR>(A, D, x0, y0, val, par);
     const int lx = tid & 63, w = tid >> 6;
     const int gx = x0 + lx;
-    // pixel counts per local root, bitmap words of this tile cleared
-#pragma unroll
-    for (int j = 0; j < CPT; ++j) {
-        const int ly = w + 4 * j;
-        const int i = ly * CW + lx;
-        const int gy = y0 + ly;
-        if (gx < A.cols && gy < A.rows) {
-            const int64_t g = (int64_t)gy * A.cols + gx;
-            if (A.bits && ((g & 31) == 0 || lx == 0)) A.bits[g >> 5] = 0u;
-        }
-        if (par[i] != BGL) atomicAdd(&cnt[par[i]], 1u);
-    }
-    __syncthreads();
-    uint32_t slot[CPT];
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int i = (w + 4 * j) * CW + lx;
-        slot[j] = (par[i] == (uint32_t)i) ? atomicAdd(&nroots, 1u) : 0xFFFFFFFFu;
+        if (par[i] != BGL) atomicAdd(&cnt[par[i]], 1u);
     }
     __syncthreads();
-    if (tid == 0) base = nroots ? (uint32_t)atomicAdd(&A.nentries[0], (int)nroots) : 0u;
-    __syncthreads();
+    uint2* slots = A.entries + (int64_t)blockIdx.x * CN;
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int ly = w + 4 * j;
         const int i = ly * CW + lx;
         const uint32_t r = par[i];
         if (r == BGL) continue;
-        const int64_t g = (int64_t)(y0 + ly) * A.cols + gx;
-        if (slot[j] != 0xFFFFFFFFu) {
-            A.entries[base + slot[j]] = make_uint2((uint32_t)g, cnt[i]);
-            A.lab[g] = (int32_t)g;
+        const int32_t g = D.pix_base + (y0 + ly) * D.cols + gx;
+        if (r == (uint32_t)i) {
+            const uint32_t k = atomicAdd(&nroots, 1u);
+            slots[k] = make_uint2((uint32_t)g, cnt[i]);
+            A.lab[g] = g;
             if (A.use_area) A.area[g] = 0;
         } else if (ly == 0 || ly == CH - 1 || lx == 0 || lx == CW - 1) {
-            A.lab[g] = (int32_t)gidx(A, y0, x0, r);
+            A.lab[g] = gidx(D, y0, x0, r);
         }
     }
+    __syncthreads();
+    if (tid == 0) A.tcount[blockIdx.x] = (int32_t)nroots;
 }
 
 // K2: one thread per tile-edge pixel whose backward neighbours cross the edge:
 // threads 0..63 the top row (up, up-left, up-right), 64..95 the left column
-// (left, and up-left below the top row), 96..127 the right column (up-right
+// (left, and up-left below the top row), 96..126 the right column (up-right
 // below the top row).
 template <int CONN, typename InT, bool THR>
 __global__ __launch_bounds__(128) void ccl_merge_kernel(CclArgs A) {
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * CW, y0 = blockIdx.y * CH;
+    CclImg D;
+    int tx, ty;
+    tile_of(A, blockIdx.x, D, tx, ty);
+    const int x0 = tx * CW, y0 = ty * CH;
     int y, x;
     if (tid < 64) { y = y0; x = x0 + tid; }
     else if (tid < 96) { y = y0 + (tid - 64); x = x0; }
     else { y = y0 + 1 + (tid - 96); x = x0 + CW - 1; if (tid == 127) return; }
-    if (x >= A.cols || y >= A.rows) return;
-    const uint32_t v = pix<InT, THR>(A,y, x);
+    if (x >= D.cols || y >= D.rows) return;
+    const uint32_t v = pix<InT, THR>(A, D, y, x);
     if (!v) return;
-    const int64_t g = (int64_t)y * A.cols + x;
+    const int32_t g = D.pix_base + y * D.cols + x;
     auto join = [&](int qy, int qx) {
-        if (qy < 0 || qx < 0 || qx >= A.cols) return;
-        if (pix<InT, THR>(A,qy, qx) != v) return;
-        const int64_t q = (int64_t)qy * A.cols + qx;
+        if (qy < 0 || qx < 0 || qx >= D.cols) return;
+        if (pix<InT, THR>(A, D, qy, qx) != v) return;
+        const int32_t q = D.pix_base + qy * D.cols + qx;
         g_union(A.lab, A.lab[g], A.lab[q]);
     };
     if (tid < 64) {
@@ -263,33 +289,33 @@ __global__ __launch_bounds__(128) void ccl_merge_kernel(CclArgs A) {
             if (CONN == 8 && y > y0) join(y - 1, x - 1);
         }
     } else {
-        if (CONN == 8 && x + 1 < A.cols) join(y - 1, x + 1);
+        if (CONN == 8 && x + 1 < D.cols) join(y - 1, x + 1);
     }
 }
 
-__global__ void ccl_area_kernel(CclArgs A) {
-    const int n = A.nentries[0];
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const uint2 en = A.entries[e];
+// K3 / K4: one block per tile over the tile's component slots.
+__global__ __launch_bounds__(256) void ccl_area_kernel(CclArgs A) {
+    const int n = A.tcount[blockIdx.x];
+    const uint2* slots = A.entries + (int64_t)blockIdx.x * CN;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const uint2 en = slots[e];
         const int32_t r = g_find(A.lab, (int32_t)en.x);
         atomicAdd(&A.area[r], (int)en.y);
     }
 }
 
-__global__ void ccl_keep_kernel(CclArgs A) {
-    const int n = A.nentries[0];
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const int32_t g = (int32_t)A.entries[e].x;
+__global__ __launch_bounds__(256) void ccl_keep_kernel(CclArgs A) {
+    const int n = A.tcount[blockIdx.x];
+    const uint2* slots = A.entries + (int64_t)blockIdx.x * CN;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int32_t g = (int32_t)slots[e].x;
         if (A.lab[g] != g) continue;
         bool keep = true;
         if (A.use_area) {
             const int a = A.area[g];
             keep = a >= A.min_size && a < A.max_size;
         }
-        if (keep) {
-            if (A.bits) atomicOr(&A.bits[g >> 5], 1u << (g & 31));
-            atomicAdd(&A.nentries[1], 1);
-        }
+        if (keep) atomicOr(&A.bits[g >> 5], 1u << (g & 31));
     }
 }
 
@@ -305,7 +331,6 @@ __global__ __launch_bounds__(256) void scan_words_kernel(const uint32_t* bits, i
         c[k] = (w0 + k < nwords) ? (uint32_t)__popc(bits[w0 + k]) : 0u;
         s += c[k];
     }
-    // inclusive wave scan of s
     const int lane = tid & 63, wv = tid >> 6;
     uint32_t inc = s;
 #pragma unroll
@@ -326,7 +351,8 @@ __global__ __launch_bounds__(256) void scan_words_kernel(const uint32_t* bits, i
     if (tid == 255) blocksum[blockIdx.x] = off + inc;
 }
 
-// K5b: one block of 1024 threads scans the block totals in place (exclusive).
+// K5b: one block of 1024 threads scans the block totals in place (exclusive)
+// and stores the grand total at blocksum[nb].
 __global__ __launch_bounds__(1024) void scan_blocks_kernel(uint32_t* blocksum, int nb) {
     __shared__ uint32_t part[1024];
     const int tid = threadIdx.x;
@@ -350,14 +376,27 @@ __global__ __launch_bounds__(1024) void scan_blocks_kernel(uint32_t* blocksum, i
             ex += v;
         }
     }
+    if (tid == 1023) blocksum[nb] = part[1023];
+}
+
+// K5c: kept components per image from the block prefix.
+__global__ void ccl_count_kernel(CclArgs A) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < A.nimg; b += gridDim.x * blockDim.x) {
+        const CclImg D = A.nimg == 1 ? A.img0 : A.imgs[b];
+        const int n = D.rows * D.cols;
+        const int blk0 = D.pix_base / BLOCK_BITS;
+        const int blk1 = (int)(((int64_t)D.pix_base + n + BLOCK_BITS - 1) / BLOCK_BITS);
+        A.ncomp[b] = (int32_t)(A.blockpre[blk1] - A.blockpre[blk0]);
+    }
 }
 
 __device__ __forceinline__ bool kept(const CclArgs& A, int32_t r) {
     return (A.bits[r >> 5] >> (r & 31)) & 1u;
 }
-__device__ __forceinline__ int32_t rank1(const CclArgs& A, int32_t r) {
+__device__ __forceinline__ int32_t rank1(const CclArgs& A, const CclImg& D, int32_t r) {
     const uint32_t w = A.bits[r >> 5];
-    return (int32_t)(A.blockpre[r >> 15] + A.wordpre[r >> 5] + __popc(w & ((1u << (r & 31)) - 1u))) + 1;
+    return (int32_t)(A.blockpre[r >> 15] - A.blockpre[D.pix_base >> 15] + A.wordpre[r >> 5] +
+                     __popc(w & ((1u << (r & 31)) - 1u))) + 1;
 }
 
 template <int CONN, typename InT, bool THR>
@@ -366,8 +405,11 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
     __shared__ uint32_t par[CN];
     __shared__ int32_t outv[CN];
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * CW, y0 = blockIdx.y * CH;
-    local_label<CONN, InT, THR>(A, x0, y0, val, par);
+    CclImg D;
+    int tx, ty;
+    tile_of(A, blockIdx.x, D, tx, ty);
+    const int x0 = tx * CW, y0 = ty * CH;
+    local_label<CONN, InT, THR>(A, D, x0, y0, val, par);
     const int lx = tid & 63, w = tid >> 6;
     const int gx = x0 + lx;
     // each local root resolves its global root once
@@ -376,12 +418,12 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
         const int ly = w + 4 * j;
         const int i = ly * CW + lx;
         if (par[i] != (uint32_t)i) continue;
-        const int32_t r = g_find(A.lab, (int32_t)((int64_t)(y0 + ly) * A.cols + gx));
+        const int32_t r = g_find(A.lab, D.pix_base + (y0 + ly) * D.cols + gx);
         int32_t o;
         switch (A.out_mode) {
-            case CO_ROOT: o = r; break;
-            case CO_ROOT_FILTERED: o = kept(A, r) ? r : -1; break;
-            case CO_CANON: o = kept(A, r) ? rank1(A, r) : 0; break;
+            case CO_ROOT: o = r - D.pix_base; break;
+            case CO_ROOT_FILTERED: o = kept(A, r) ? r - D.pix_base : -1; break;
+            case CO_CANON: o = kept(A, r) ? rank1(A, D, r) : 0; break;
             default: o = kept(A, r) ? 255 : 0; break;
         }
         outv[i] = o;
@@ -392,47 +434,127 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
     for (int j = 0; j < CPT; ++j) {
         const int ly = w + 4 * j;
         const int gy = y0 + ly;
-        if (gx >= A.cols || gy >= A.rows) continue;
+        if (gx >= D.cols || gy >= D.rows) continue;
         const uint32_t r = par[ly * CW + lx];
         const int32_t o = (r == BGL) ? bg : outv[r];
-        char* row = A.out + (int64_t)gy * A.pout;
+        char* row = D.out + (int64_t)gy * D.pout;
         if (A.out_mode == CO_FLAT_U8) ((uint8_t*)row)[gx] = (uint8_t)o;
         else ((int32_t*)row)[gx] = o;
     }
 }
 
-__global__ void zero_i32_kernel(int32_t* p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
+__global__ void zero_words_kernel(uint32_t* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
 }
 
 template <int CONN, typename InT, bool THR>
-nsx_status ccl_run_conn(nsx_ctx* ctx, CclArgs A, hipStream_t s) {
-    const dim3 tiles((A.cols + CW - 1) / CW, (A.rows + CH - 1) / CH);
+void ccl_launch_conn(nsx_ctx* ctx, const CclArgs& A, hipStream_t s) {
     const int stream_grid = 4 * ctx->cu_count;
-    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, s, A.nentries, 2);
+    hipLaunchKernelGGL(zero_words_kernel, dim3(stream_grid), dim3(256), 0, s, A.bits, A.nwords);
     int h = ctx->tic(TK_CCL_LOCAL, s);
-    hipLaunchKernelGGL((ccl_local_kernel<CONN, InT, THR>), tiles, dim3(CTHREADS), 0, s, A);
+    hipLaunchKernelGGL((ccl_local_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(CTHREADS), 0, s, A);
     ctx->toc(h, s);
     h = ctx->tic(TK_CCL_MERGE, s);
-    hipLaunchKernelGGL((ccl_merge_kernel<CONN, InT, THR>), tiles, dim3(128), 0, s, A);
+    hipLaunchKernelGGL((ccl_merge_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(128), 0, s, A);
     ctx->toc(h, s);
     h = ctx->tic(TK_CCL_SMALL, s);
-    if (A.use_area) hipLaunchKernelGGL(ccl_area_kernel, dim3(stream_grid), dim3(256), 0, s, A);
-    hipLaunchKernelGGL(ccl_keep_kernel, dim3(stream_grid), dim3(256), 0, s, A);
-    if (A.out_mode == CO_CANON) {
-        const int nb = (int)((A.nwords + 1023) / 1024);
-        hipLaunchKernelGGL(scan_words_kernel, dim3(nb), dim3(256), 0, s, A.bits, A.nwords, A.wordpre, A.blockpre);
-        hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, s, A.blockpre, nb);
-    }
+    if (A.use_area) hipLaunchKernelGGL(ccl_area_kernel, dim3(A.ntiles), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(ccl_keep_kernel, dim3(A.ntiles), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(scan_words_kernel, dim3(A.nblocks), dim3(256), 0, s, A.bits, A.nwords, A.wordpre, A.blockpre);
+    hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, s, A.blockpre, A.nblocks);
+    hipLaunchKernelGGL(ccl_count_kernel, dim3(std::max(1, (A.nimg + 255) / 256)), dim3(256), 0, s, A);
     ctx->toc(h, s);
     h = ctx->tic(TK_CCL_FINAL, s);
-    hipLaunchKernelGGL((ccl_final_kernel<CONN, InT, THR>), tiles, dim3(CTHREADS), 0, s, A);
+    hipLaunchKernelGGL((ccl_final_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(CTHREADS), 0, s, A);
     ctx->toc(h, s);
+}
+
+template <typename InT, bool THR>
+void ccl_launch_typed(nsx_ctx* ctx, const CclArgs& A, int conn, hipStream_t s) {
+    if (conn == 8) ccl_launch_conn<8, InT, THR>(ctx, A, s);
+    else ccl_launch_conn<4, InT, THR>(ctx, A, s);
+}
+
+}  // namespace
+
+size_t ccl_desc_bytes() { return sizeof(CclImg); }
+
+int ccl_tiles(int rows, int cols) {
+    if (rows <= 0 || cols <= 0) return 0;
+    return ((cols + CW - 1) / CW) * ((rows + CH - 1) / CH);
+}
+
+int64_t ccl_pix_span(int rows, int cols) {
+    const int64_t n = (int64_t)rows * cols;
+    return (n + BLOCK_BITS - 1) / BLOCK_BITS * BLOCK_BITS;
+}
+
+void ccl_fill_desc(void* out, const View& in, const View& o, int tile_base, int64_t pix_base) {
+    CclImg D{};
+    D.in = in.data; D.pin = in.pitch;
+    D.out = o.data; D.pout = o.pitch;
+    D.rows = in.rows; D.cols = in.cols;
+    D.ntx = (in.cols + CW - 1) / CW;
+    D.nty = (in.rows + CH - 1) / CH;
+    D.tile_base = tile_base;
+    D.pix_base = (int)pix_base;
+    *reinterpret_cast<CclImg*>(out) = D;
+}
+
+nsx_status ccl_launch(nsx_ctx* ctx, const CclBatch& B, const CclParams& p, hipStream_t s) {
+    if (B.ntiles <= 0) return NSX_OK;
+    if (B.pix_total >= (int64_t)INT32_MAX) return ctx->fail(NSX_ERR_INVALID, "batch too large for int32 labels");
+    if (!p.threshold && B.in_dtype != NSX_U8) return ctx->fail(NSX_ERR_INVALID, "labelling needs u8 input or a threshold");
+    CclArgs A{};
+    A.nimg = B.nimg;
+    A.ntiles = B.ntiles;
+    if (B.nimg == 1) A.img0 = *reinterpret_cast<const CclImg*>(B.host_desc);
+    A.imgs = reinterpret_cast<const CclImg*>(B.dev_desc);
+    A.tile_img = B.tile_img;
+    A.thr_i = (int64_t)std::floor(std::max(-9.0e18, std::min(9.0e18, p.thr)));
+    A.thr_f = (float)p.thr;
+    A.min_size = p.min_size; A.max_size = p.max_size; A.use_area = p.use_area;
+    A.out_mode = p.out_mode;
+    A.nwords = B.pix_total / 32;
+    A.nblocks = (int)(B.pix_total / BLOCK_BITS);
+    void *lab, *area = nullptr, *ent, *tcnt, *bits, *scan;
+    NSX_TRY(ctx->ws(B.ws_base + 0, sizeof(int32_t) * (size_t)B.pix_total, &lab));
+    if (A.use_area) NSX_TRY(ctx->ws(B.ws_base + 1, sizeof(int32_t) * (size_t)B.pix_total, &area));
+    NSX_TRY(ctx->ws(B.ws_base + 2, sizeof(uint2) * (size_t)B.ntiles * CN, &ent));
+    NSX_TRY(ctx->ws(B.ws_base + 3, sizeof(int32_t) * (size_t)B.ntiles, &tcnt));
+    NSX_TRY(ctx->ws(B.ws_base + 4, sizeof(uint32_t) * (size_t)A.nwords, &bits));
+    NSX_TRY(ctx->ws(B.ws_base + 5, sizeof(uint32_t) * (size_t)(A.nwords + A.nblocks + 1), &scan));
+    A.lab = (int32_t*)lab; A.area = (int32_t*)area; A.entries = (uint2*)ent; A.tcount = (int32_t*)tcnt;
+    A.bits = (uint32_t*)bits;
+    A.wordpre = (uint32_t*)scan;
+    A.blockpre = (uint32_t*)scan + A.nwords;
+    A.ncomp = B.ncomp_dev;
+    if (!p.threshold) ccl_launch_typed<uint8_t, false>(ctx, A, p.conn, s);
+    else if (B.in_dtype == NSX_U8) ccl_launch_typed<uint8_t, true>(ctx, A, p.conn, s);
+    else if (B.in_dtype == NSX_I32) ccl_launch_typed<int32_t, true>(ctx, A, p.conn, s);
+    else ccl_launch_typed<float, true>(ctx, A, p.conn, s);
     NSX_HIP_TRY(ctx, hipGetLastError());
     return NSX_OK;
 }
 
+nsx_status ccl_run(nsx_ctx* ctx, const View& in, const View& out, const CclParams& p, int32_t* ncomp_dev,
+                   hipStream_t s) {
+    if ((int64_t)in.rows * in.cols == 0) return NSX_OK;
+    CclImg D;
+    ccl_fill_desc(&D, in, out, 0, 0);
+    CclBatch B{};
+    B.nimg = 1;
+    B.ntiles = ccl_tiles(in.rows, in.cols);
+    B.pix_total = ccl_pix_span(in.rows, in.cols);
+    B.host_desc = &D;
+    B.in_dtype = in.dtype;
+    B.ws_base = WS_CCL_LABEL;
+    B.ncomp_dev = ncomp_dev ? ncomp_dev : (int32_t*)ctx->d_stats + 30;
+    return ccl_launch(ctx, B, p, s);
+}
+
+namespace {
 // ---- labelled-input area filter (bwareaopen2 with labeled=true) ---------------------
 __global__ void area_count_kernel(const char* in, int64_t pin, int rows, int cols, int32_t* area, int64_t npix,
                                   int32_t* bad) {
@@ -465,9 +587,9 @@ __global__ void area_apply_kernel(const char* in, int64_t pin, int rows, int col
     else ((int32_t*)(out + (int64_t)y * pout))[x] = keep ? l : -1;
 }
 
-__global__ void zero_words_kernel(uint32_t* p, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = 0u;
+__global__ void zero_i32_kernel(int32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // ---- relabel of an arbitrary int32 label image ---------------------------------------
@@ -513,54 +635,6 @@ __global__ void fill_i32_kernel(char* out, int64_t pout, int rows, int cols, int
 }
 
 }  // namespace
-
-nsx_status ccl_run(nsx_ctx* ctx, const View& in, const View& out, const CclParams& p, int32_t* ncomp_dev,
-                   hipStream_t s) {
-    CclArgs A{};
-    A.in = in.data; A.pin = in.pitch; A.in_dtype = in.dtype;
-    A.thresh = p.threshold;
-    // v > k for an integer v  <=>  v > floor(k); for float, compare in float
-    // (the reference compares a float Mat with a scalar converted to float).
-    A.thr_i = (int64_t)std::floor(std::max(-9.0e18, std::min(9.0e18, p.thr)));
-    A.thr_f = (float)p.thr;
-    A.rows = in.rows; A.cols = in.cols;
-    A.npix = (int64_t)in.rows * in.cols;
-    if (A.npix == 0) return NSX_OK;
-    if (A.npix >= (int64_t)INT32_MAX) return ctx->fail(NSX_ERR_INVALID, "image too large for int32 labels");
-    A.min_size = p.min_size; A.max_size = p.max_size; A.use_area = p.use_area;
-    A.out = out.data; A.pout = out.pitch; A.out_mode = p.out_mode;
-    const int64_t ntile = (int64_t)((A.cols + CW - 1) / CW) * ((A.rows + CH - 1) / CH);
-    A.nwords = (A.npix + 31) / 32;
-    const int64_t nb = (A.nwords + 1023) / 1024;
-    void *lab, *area = nullptr, *ent, *misc, *bits = nullptr, *scan = nullptr;
-    NSX_TRY(ctx->ws(WS_CCL_LABEL, sizeof(int32_t) * (size_t)A.npix, &lab));
-    if (A.use_area) NSX_TRY(ctx->ws(WS_CCL_AREA, sizeof(int32_t) * (size_t)A.npix, &area));
-    const int64_t max_entries = std::min<int64_t>(A.npix, A.npix / 2 + ntile + 64);
-    NSX_TRY(ctx->ws(WS_CCL_ROOTS, sizeof(uint2) * (size_t)max_entries, &ent));
-    NSX_TRY(ctx->ws(WS_CCL_MISC, 64, &misc));
-    const bool need_bits = p.out_mode != CO_ROOT;
-    if (need_bits) NSX_TRY(ctx->ws(WS_CCL_BITS, sizeof(uint32_t) * (size_t)A.nwords, &bits));
-    if (p.out_mode == CO_CANON) NSX_TRY(ctx->ws(WS_CCL_SCAN, sizeof(uint32_t) * (size_t)(A.nwords + nb + 1), &scan));
-    A.lab = (int32_t*)lab; A.area = (int32_t*)area; A.entries = (uint2*)ent;
-    A.nentries = (int32_t*)misc;
-    A.bits = (uint32_t*)bits;
-    A.wordpre = (uint32_t*)scan;
-    A.blockpre = scan ? (uint32_t*)scan + A.nwords : nullptr;
-    nsx_status st;
-    if (!p.threshold) {
-        if (in.dtype != NSX_U8) return ctx->fail(NSX_ERR_INVALID, "labelling needs u8 input or a threshold");
-        st = p.conn == 8 ? ccl_run_conn<8, uint8_t, false>(ctx, A, s) : ccl_run_conn<4, uint8_t, false>(ctx, A, s);
-    } else if (in.dtype == NSX_U8) {
-        st = p.conn == 8 ? ccl_run_conn<8, uint8_t, true>(ctx, A, s) : ccl_run_conn<4, uint8_t, true>(ctx, A, s);
-    } else if (in.dtype == NSX_I32) {
-        st = p.conn == 8 ? ccl_run_conn<8, int32_t, true>(ctx, A, s) : ccl_run_conn<4, int32_t, true>(ctx, A, s);
-    } else {
-        st = p.conn == 8 ? ccl_run_conn<8, float, true>(ctx, A, s) : ccl_run_conn<4, float, true>(ctx, A, s);
-    }
-    if (st != NSX_OK) return st;
-    if (ncomp_dev) NSX_HIP_TRY(ctx, hipMemcpyAsync(ncomp_dev, A.nentries + 1, sizeof(int32_t), hipMemcpyDeviceToDevice, s));
-    return NSX_OK;
-}
 
 nsx_status ccl_area_labeled(nsx_ctx* ctx, const View& in, const View& out, int min_size, int max_size,
                             bool flatten, int32_t* count_dev, hipStream_t s) {

@@ -188,8 +188,29 @@ struct CclParams {
 
 // Labels the image `in` (u8; or u8/i32/f32 with threshold) into `out`.
 // `ncomp_dev` (may be null) receives the kept component count on the device.
+// Enqueues work only.
 nsx_status ccl_run(nsx_ctx* ctx, const View& in, const View& out, const CclParams& p,
                    int32_t* ncomp_dev, hipStream_t s);
+
+// Several images labelled by one set of launches.  Image b's pixels occupy a
+// global index range starting at its pix_base (a multiple of 32768, see
+// ccl_pix_span); tiles are numbered across images.
+struct CclBatch {
+    int nimg = 0;
+    int ntiles = 0;
+    int64_t pix_total = 0;            // end of the last image's index range
+    const void* host_desc = nullptr;  // the descriptor when nimg == 1
+    const void* dev_desc = nullptr;   // device array of nimg descriptors
+    const int32_t* tile_img = nullptr;  // device: tile -> image (nimg > 1)
+    int in_dtype = NSX_U8;
+    int ws_base = 0;                  // first of 6 consecutive workspace slots
+    int32_t* ncomp_dev = nullptr;     // device: kept components per image
+};
+size_t ccl_desc_bytes();
+int ccl_tiles(int rows, int cols);
+int64_t ccl_pix_span(int rows, int cols);
+void ccl_fill_desc(void* out, const View& in, const View& o, int tile_base, int64_t pix_base);
+nsx_status ccl_launch(nsx_ctx* ctx, const CclBatch& B, const CclParams& p, hipStream_t s);
 
 // Area filter on int32 root labels (background -1), out int32 or u8.
 nsx_status ccl_area_labeled(nsx_ctx* ctx, const View& in, const View& out, int min_size, int max_size,

@@ -419,6 +419,10 @@ __global__ __launch_bounds__(64) void recon_round_kernel(ReconArgs A, int round)
     }
 }
 
+__global__ void copy_word_kernel(int32_t* dst, const int32_t* src) {
+    if (threadIdx.x == 0) *dst = *src;
+}
+
 __global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts, int64_t* stats) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     for (int j = i; j < ntiles; j += gridDim.x * blockDim.x) stamp[j] = 0u;
@@ -556,9 +560,9 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     }
     R->next_round = hint + 1;
     NSX_HIP_TRY(ctx, hipGetLastError());
-    if (pending_dev)
-        NSX_HIP_TRY(ctx, hipMemcpyAsync(pending_dev, &A.counts[R->next_round % 3], sizeof(int32_t),
-                                        hipMemcpyDeviceToDevice, s));
+    if (pending_dev)   // a kernel rather than a copy node, so the sequence can be graph-captured
+        hipLaunchKernelGGL(copy_word_kernel, dim3(1), dim3(64), 0, s, pending_dev, &A.counts[R->next_round % 3]);
+    NSX_HIP_TRY(ctx, hipGetLastError());
     return NSX_OK;
 }
 

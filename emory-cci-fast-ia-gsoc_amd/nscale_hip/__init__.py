@@ -282,6 +282,70 @@ def segment_chain(marker, mask, recon_conn=8, fill_conn=4, thresh=128.0, ccl_con
     return recon_out, fill_out, labels_out, int(n.value)
 
 
+def recon_label(marker, mask, conn=8, thresh=128.0, ccl_conn=8, *, recon_out=None, labels_out=None, ctx=None,
+                stream=None):
+    """One tile of the batched workload: imreconstruct then bwlabel(recon > thresh).
+
+    Returns (recon, labels, ncomp)."""
+    c = _ctx(mask, ctx)
+    recon_out = _like(mask, _np_dtype(mask)) if recon_out is None else recon_out
+    labels_out = _like(mask, np.int32) if labels_out is None else labels_out
+    n = C.c_int32(0)
+    c.check(c.lib.nsx_recon_label(c.h, C.byref(describe(marker)), C.byref(describe(mask)), conn, float(thresh),
+                                  ccl_conn, C.byref(describe(recon_out)), C.byref(describe(labels_out)), C.byref(n),
+                                  _torch_stream(mask, stream)))
+    return recon_out, labels_out, int(n.value)
+
+
+class Batch:
+    """Many device-resident tiles of uneven size in one set of launches,
+    captured as a hipGraph (nsx_batch_*).  Per tile: recon =
+    imreconstruct(marker, mask, conn); labels = bwlabel(recon > thresh)."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or default_context(0)
+        self.lib = self.ctx.lib
+        h = C.c_void_p()
+        self.ctx.check(self.lib.nsx_batch_create(self.ctx.h, C.byref(h)))
+        self.h = h
+        self.n = 0
+        self._keep = []   # tensors referenced by the batch
+
+    def add(self, marker, mask, recon_out, labels_out, conn=8, thresh=128.0, ccl_conn=8) -> int:
+        for t in (marker, mask, recon_out, labels_out):
+            if not _is_device_tensor(t):
+                raise NsxError(_lib.NSX_ERR_INVALID, "batch tiles must be device tensors")
+        self.ctx.check(self.lib.nsx_batch_add(self.h, C.byref(describe(marker)), C.byref(describe(mask)),
+                                              C.byref(describe(recon_out)), C.byref(describe(labels_out)), conn,
+                                              float(thresh), ccl_conn))
+        self._keep.append((marker, mask, recon_out, labels_out))
+        self.n += 1
+        return self.n - 1
+
+    def finalize(self, use_graph: bool = True) -> None:
+        self.ctx.check(self.lib.nsx_batch_finalize(self.h, int(bool(use_graph))))
+
+    def run(self, stream=None) -> None:
+        s = _stream_arg(stream) if stream is not None else C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+        self.ctx.check(self.lib.nsx_batch_run(self.h, s))
+
+    def counts(self) -> list:
+        arr = (C.c_int32 * max(1, self.n))()
+        self.ctx.check(self.lib.nsx_batch_counts(self.h, arr, self.n))
+        return [int(arr[i]) for i in range(self.n)]
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.nsx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---- synthetic inputs ------------------------------------------------------------
 def gen_disks_u8(rows: int, cols: int, ndisks: int, seed: int) -> np.ndarray:
     lib = _lib.load()

@@ -55,6 +55,8 @@ SIGNATURES = {
     "nsx_threshold_gt": (C.c_int, [_CTX, _IMG, C.c_double, _IMG, C.c_void_p]),
     "nsx_segment_chain": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                     _IMG, _IMG, _IMG, _I32P, C.c_void_p]),
+    "nsx_recon_label": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_double, C.c_int, _IMG, _IMG, _I32P,
+                                  C.c_void_p]),
     "nsx_batch_create": (C.c_int, [_CTX, P(C.c_void_p)]),
     "nsx_batch_add": (C.c_int, [C.c_void_p, _IMG, _IMG, _IMG, _IMG, C.c_int, C.c_double, C.c_int]),
     "nsx_batch_finalize": (C.c_int, [C.c_void_p, C.c_int]),

@@ -150,6 +150,16 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
                              int min_size, int max_size, nsx_img* recon_out, nsx_img* fill_out,
                              nsx_img* labels_out, int32_t* ncomp, void* stream);
 
+/* One tile of the batched workload on its own: recon = imreconstruct(marker,
+ * mask, conn); labels = bwlabel(recon > thresh, ccl_conn, relabel=1); *ncomp =
+ * number of labels.  u8, i32, f32 marker/mask; device or host images.  The
+ * per-tile launch path that nsx_batch_* is compared against (the reference's
+ * per-image gpu::imreconstruct + gpu::bwlabel, MorphologicOperationsGPU.cpp:
+ * 172-200,878-900). */
+nsx_status nsx_recon_label(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* mask, int conn, double thresh,
+                           int ccl_conn, nsx_img* recon_out, nsx_img* labels_out, int32_t* ncomp,
+                           void* stream);
+
 /* ---- batched tiles -------------------------------------------------------- */
 /* Many independent device-resident tiles of uneven size run reconstruction +
  * threshold + labelling together, captured once in a hipGraph and replayed.
