@@ -42,7 +42,115 @@ def parse():
     p.add_argument("--no-check", action="store_true", help="skip the oracle comparison")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-runs", type=int, default=10, help="oracle chain runs in the CPU sample")
+    p.add_argument("--workload", default="chain4096",
+                   choices=["chain4096", "batch", "maze", "serve"],
+                   help="chain4096: the headline 4096^2 chain (default); batch: 256 u8 + 32 f32 uneven tiles "
+                        "batched vs per-tile; maze: 16384^2 maze stress; serve: continuous-batching server")
+    p.add_argument("--tiles-u8", type=int, default=256)
+    p.add_argument("--tiles-f32", type=int, default=32)
+    p.add_argument("--check-tiles", type=int, default=6, help="batch tiles also checked against the oracle")
     return p.parse_args()
+
+
+def batch_tiles(nh, n_u8: int, n_f32: int):
+    """Uneven tiles: sides uniform in 512..4096 (multiples of 16), disks in
+    proportion to area (200 per 512^2), seed 0x5EED0002 + index; float tiles are
+    the same construction divided by 255."""
+    import numpy as np
+    rng = np.random.default_rng(0x5EED0002)
+    out = []
+    for k in range(n_u8 + n_f32):
+        h = 512 + 16 * int(rng.integers(0, 225))
+        w = 512 + 16 * int(rng.integers(0, 225))
+        mask = nh.gen_disks_u8(h, w, round(200 * h * w / 512 ** 2), 0x5EED0002 + k)
+        marker = nh.gen_marker_u8(mask, 30)
+        if k >= n_u8:
+            out.append((nh.scale_f32(marker), nh.scale_f32(mask), float(np.float32(128) / np.float32(255))))
+        else:
+            out.append((marker, mask, 128.0))
+    return out
+
+
+def bench_batch(args):
+    """All tiles through one set of launches (graph replay) vs one call per tile."""
+    import numpy as np
+    import torch
+
+    import nscale_hip as nh
+    t0 = time.perf_counter()
+    tiles = batch_tiles(nh, args.tiles_u8, args.tiles_f32)
+    log(f"generated {len(tiles)} tiles in {time.perf_counter() - t0:.1f} s")
+    dev = torch.device("cuda:0")
+    dtiles = []
+    for marker, mask, thr in tiles:
+        dm = torch.from_numpy(marker).to(dev)
+        dk = torch.from_numpy(mask).to(dev)
+        dtiles.append((dm, dk, torch.empty_like(dk), torch.empty(dk.shape, dtype=torch.int32, device=dev),
+                       torch.empty_like(dk), torch.empty(dk.shape, dtype=torch.int32, device=dev), thr))
+    total_px = sum(m.size for _, m, _ in tiles)
+    ctx = nh.default_context(0)
+    # per-tile launches
+    def per_tile():
+        return [nh.recon_label(dm, dk, 8, thr, 8, recon_out=r2, labels_out=l2, ctx=ctx)[2]
+                for dm, dk, _, _, r2, l2, thr in dtiles]
+    per_tile()
+    torch.cuda.synchronize()
+    reps = max(1, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        n_tile = per_tile()
+    torch.cuda.synchronize()
+    t_tile = (time.perf_counter() - t0) / reps
+    # batched: one set of launches, captured once
+    b = nh.Batch(ctx)
+    for dm, dk, r, lab, _, _, thr in dtiles:
+        b.add(dm, dk, r, lab, 8, thr, 8)
+    t0 = time.perf_counter()
+    b.finalize(use_graph=True)
+    t_fin = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        b.run()
+    counts = b.counts()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        b.run()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - s0) * 1e3)
+    t_batch = (time.perf_counter() - t0) / args.steps
+    counts = b.counts()
+    same = all(torch.equal(r, r2) and torch.equal(lab, l2) for _, _, r, lab, r2, l2, _ in dtiles) and counts == n_tile
+    exact = None
+    if args.check_tiles > 0:
+        from oracle import oracle as orc
+        idx = list(range(min(args.check_tiles // 2, args.tiles_u8))) + \
+            list(range(args.tiles_u8, args.tiles_u8 + min(args.check_tiles - args.check_tiles // 2, args.tiles_f32)))
+        exact = True
+        for i in idx:
+            marker, mask, thr = tiles[i]
+            wr = orc.imreconstruct(marker, mask, 8)
+            bw = np.where(wr > (np.float32(thr) if wr.dtype == np.float32 else thr), 255, 0).astype(np.uint8)
+            wl, wn = orc.bwlabel2(bw, 8, relab=True)
+            _, _, r, lab, _, _, _ = dtiles[i]
+            exact &= bool(np.array_equal(r.cpu().numpy(), wr) and np.array_equal(lab.cpu().numpy(), wl)
+                          and counts[i] == wn)
+    lat.sort()
+    out = {
+        "metric": "Mpix/s recon+bwlabel, uneven tiles batched in one graph vs per-tile launches",
+        "value": round(total_px / t_batch / 1e6, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 3),
+        "p50_ms": round(statistics.median(lat), 3), "p99_ms": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8+f32",
+        "data": "synthetic", "bitexact": exact, "batched_equals_per_tile": bool(same),
+        "per_tile_mpix_s": round(total_px / t_tile / 1e6, 2), "per_tile_ms": round(t_tile * 1e3, 3),
+        "speedup_vs_per_tile": round(t_tile / t_batch, 3), "finalize_s": round(t_fin, 3),
+        "config": {"workload": f"{args.tiles_u8} u8 + {args.tiles_f32} f32 tiles, sides 512-4096 step 16: "
+                               "recon8 -> >128 (f32: >128/255) -> bwlabel8", "pixels": total_px,
+                   "tiles": len(tiles)},
+    }
+    b.close()
+    print(json.dumps(out))
 
 
 TILE_PX = 62 * 62   # interior pixels of one reconstruction tile visit
@@ -105,6 +213,8 @@ def main():
     import nscale_hip as nh
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.workload == "batch":
+        return bench_batch(args)
     n = args.size
     seed = 0x5EED0001
     mask = nh.gen_disks_u8(n, n, args.disks, seed)
