@@ -24,7 +24,9 @@ nsx_status nsx_ctx::ws(int slot, size_t bytes, void** out) {
             return fail(NSX_ERR_BUDGET, "workspace request exceeds the context's HBM budget");
     }
     if (b.ptr) {
-        hipError_t e = hipStreamSynchronize(stream);
+        // Work enqueued by earlier calls (on whatever stream the caller passed)
+        // may still read this buffer: wait for the whole device before freeing.
+        hipError_t e = hipDeviceSynchronize();
         (void)e;
         hipFree(b.ptr);
         held -= b.bytes;
