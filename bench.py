@@ -6,8 +6,9 @@ seed 0x5EED0001), marker = mask - 30, inputs resident in HBM.  One step =
     imreconstruct (8-conn) -> imfillHoles (grayscale, 4-conn) -> (> 128)
     -> bwlabel (8-conn) -> bwareaopen [11, 1000) -> canonical relabel
 through the C ABI (nsx_segment_chain).  Reported value: Mpixel/s over the
-timed steps.  The first step's outputs are checked bit-exactly against the
-oracle before timing.
+timed steps.  The CPU baseline leg runs the oracle's restatement of the
+reference chain on the same input and also compares it bit-exactly with the
+GPU's first step.
 
 Prints ONE JSON line.  --gpus N > 1 is refused: this path runs on one GPU.
 """
@@ -39,8 +40,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--disks", type=int, default=3000)
-    p.add_argument("--no-check", action="store_true", help="skip the oracle comparison")
-    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-cpu", "--no-check", dest="no_cpu", action="store_true",
+                   help="skip the CPU baseline leg (the oracle run that also checks the GPU results)")
     p.add_argument("--cpu-runs", type=int, default=10, help="oracle chain runs in the CPU sample")
     p.add_argument("--workload", default="chain4096",
                    choices=["chain4096", "batch", "maze", "serve"],
@@ -49,6 +50,13 @@ def parse():
     p.add_argument("--tiles-u8", type=int, default=256)
     p.add_argument("--tiles-f32", type=int, default=32)
     p.add_argument("--check-tiles", type=int, default=6, help="batch tiles also checked against the oracle")
+    p.add_argument("--requests", type=int, default=200, help="serve: requests in the trace")
+    p.add_argument("--slots", type=int, default=4, help="serve: worker slots of the server")
+    p.add_argument("--load", type=float, default=1.0,
+                   help="serve: Poisson arrival rate as a multiple of the sequential capacity")
+    p.add_argument("--budget-gib", type=float, default=4.0, help="serve: HBM arena of the server (GiB)")
+    p.add_argument("--check-max-side", type=int, default=1024,
+                   help="serve: requests up to this side are also checked against the oracle")
     return p.parse_args()
 
 
@@ -121,20 +129,9 @@ def bench_batch(args):
     t_batch = (time.perf_counter() - t0) / args.steps
     counts = b.counts()
     same = all(torch.equal(r, r2) and torch.equal(lab, l2) for _, _, r, lab, r2, l2, _ in dtiles) and counts == n_tile
-    exact = None
-    if args.check_tiles > 0:
-        from oracle import oracle as orc
-        idx = list(range(min(args.check_tiles // 2, args.tiles_u8))) + \
-            list(range(args.tiles_u8, args.tiles_u8 + min(args.check_tiles - args.check_tiles // 2, args.tiles_f32)))
-        exact = True
-        for i in idx:
-            marker, mask, thr = tiles[i]
-            wr = orc.imreconstruct(marker, mask, 8)
-            bw = np.where(wr > (np.float32(thr) if wr.dtype == np.float32 else thr), 255, 0).astype(np.uint8)
-            wl, wn = orc.bwlabel2(bw, 8, relab=True)
-            _, _, r, lab, _, _, _ = dtiles[i]
-            exact &= bool(np.array_equal(r.cpu().numpy(), wr) and np.array_equal(lab.cpu().numpy(), wl)
-                          and counts[i] == wn)
+    cpu, exact = (None, None) if args.no_cpu else cpu_baseline_batch(tiles, dtiles, counts, args)
+    if exact is False:
+        sys.exit(1)
     lat.sort()
     out = {
         "metric": "Mpix/s recon+bwlabel, uneven tiles batched in one graph vs per-tile launches",
@@ -148,9 +145,239 @@ def bench_batch(args):
         "config": {"workload": f"{args.tiles_u8} u8 + {args.tiles_f32} f32 tiles, sides 512-4096 step 16: "
                                "recon8 -> >128 (f32: >128/255) -> bwlabel8", "pixels": total_px,
                    "tiles": len(tiles)},
+        "roofline": None,
+        "cpu_baseline": cpu,
     }
     b.close()
     print(json.dumps(out))
+
+
+def cpu_baseline_batch(tiles, dtiles, counts, args):
+    """The oracle's imreconstruct + bwlabel2, single-threaded, on check_tiles of
+    the batch (half u8, half f32: a bounded sample); its outputs double as the
+    exactness check of the batched results.  Returns (cpu_baseline, exact)."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    k = max(0, args.check_tiles)
+    idx = list(range(min(k // 2, args.tiles_u8))) + \
+        list(range(args.tiles_u8, args.tiles_u8 + min(k - k // 2, args.tiles_f32)))
+    exact, px, dt = True, 0, 0.0
+    for i in idx:
+        marker, mask, thr = tiles[i]
+        t0 = time.perf_counter()
+        wr = orc.imreconstruct(marker, mask, 8)
+        bw = np.where(wr > (np.float32(thr) if wr.dtype == np.float32 else thr), 255, 0).astype(np.uint8)
+        wl, wn = orc.bwlabel2(bw, 8, relab=True)
+        dt += time.perf_counter() - t0
+        px += mask.size
+        _, _, r, lab, _, _, _ = dtiles[i]
+        if not (np.array_equal(r.cpu().numpy(), wr) and np.array_equal(lab.cpu().numpy(), wl) and counts[i] == wn):
+            exact = False
+            log(f"tile {i} differs from the oracle")
+    if not idx:
+        return None, None
+    cpu = {"value": round(px / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+           "sample": f"{len(idx)} tiles ({px / 1e6:.1f} Mpx) through imreconstruct + bwlabel2, single thread, "
+                     f"{dt:.1f} s", "host_cpus": os.cpu_count()}
+    return cpu, exact
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, int(round(q * (len(xs) - 1))))]
+
+
+def bench_serve(args):
+    """Serving loop: a trace of requests (log-uniform sides 256..4096, 20% int32)
+    through nsx_server with `slots` workers vs one worker (sequential).
+
+    Reported value: Mpix/s of the server with every request queued at once
+    (saturation).  Latency: Poisson arrivals at `load` x the sequential
+    capacity (requests/s of one-at-a-time service), p50/p99 of submit -> done,
+    for the server and for the sequential one-slot server on the same trace.
+    Inputs and label outputs are in pinned host memory: each request's time
+    includes its upload and the label download."""
+    import threading
+    import zlib
+
+    import numpy as np
+    import torch
+
+    import nscale_hip as nh
+    from nscale_hip.workloads import SERVE_SEED, request_images, serve_trace
+
+    def pinned(a):
+        t = torch.empty(a.shape, dtype={np.uint8: torch.uint8, np.int32: torch.int32}[a.dtype.type],
+                        pin_memory=True)
+        t.numpy()[...] = a
+        return t.numpy()
+
+    t0 = time.perf_counter()
+    trace = serve_trace(args.requests, SERVE_SEED)
+    reqs = []
+    for side, kind, seed in trace:
+        marker, mask, thr = request_images(side, kind, seed)
+        reqs.append((pinned(marker), pinned(mask), thr,
+                     torch.empty((side, side), dtype=torch.int32, pin_memory=True).numpy()))
+    px = [m.size for _, m, _, _ in reqs]
+    total_px = sum(px)
+    log(f"generated {len(reqs)} requests ({total_px / 1e6:.1f} Mpx, "
+        f"{sum(k == 'i32' for _, k, _ in trace)} int32) in {time.perf_counter() - t0:.1f} s")
+    budget = int(args.budget_gib * (1 << 30))
+    s1 = nh.Server(0, budget, slots=1)
+    sk = nh.Server(0, budget, slots=args.slots)
+
+    def submit(srv, i):
+        marker, mask, thr, lab = reqs[i]
+        return srv.submit(marker, mask, 8, 4, thr, 8, 11, 1000, labels_out=lab)
+
+    digests = {}
+
+    def record(i, n, tag):
+        d = (zlib.crc32(reqs[i][3]), n)
+        if digests.setdefault(i, d) != d:
+            raise SystemExit(f"request {i}: result differs in run '{tag}'")
+
+    def burst(srv, tag):
+        ts = time.perf_counter()
+        tickets = [submit(srv, i) for i in range(len(reqs))]
+        lat = []
+        for i, t in enumerate(tickets):
+            _, _, _, n, tm = srv.wait(t)
+            record(i, n, tag)
+            lat.append(tm[2])
+        return time.perf_counter() - ts, lat
+
+    def closed_loop(srv, tag):
+        ts = time.perf_counter()
+        svc = []
+        for i in range(len(reqs)):
+            _, _, _, n, tm = srv.wait(submit(srv, i))
+            record(i, n, tag)
+            svc.append(tm[1])
+        return time.perf_counter() - ts, svc
+
+    def poisson(srv, gaps, tag):
+        tickets = [None] * len(reqs)
+        lat = [0.0] * len(reqs)
+        ready = threading.Semaphore(0)
+        done_at = [0.0]
+
+        def waiter():
+            for i in range(len(reqs)):
+                ready.acquire()
+                _, _, _, n, tm = srv.wait(tickets[i])
+                record(i, n, tag)
+                lat[i] = tm[2]
+            done_at[0] = time.perf_counter()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        ts = time.perf_counter()
+        due = ts
+        for i in range(len(reqs)):
+            due += gaps[i]
+            while True:
+                now = time.perf_counter()
+                if now >= due:
+                    break
+                time.sleep(min(due - now, 0.0005))
+            tickets[i] = submit(srv, i)
+            ready.release()
+        th.join()
+        return done_at[0] - ts, lat
+
+    # warm both servers (workspace sizes, round predictions per size bucket)
+    for _ in range(max(1, args.warmup)):
+        burst(s1, "warmup-1")
+        burst(sk, "warmup-k")
+    t_seq, svc = closed_loop(s1, "sequential")
+    t_sat1, _ = burst(s1, "saturation-1")
+    t_satk, lat_satk = burst(sk, f"saturation-{args.slots}")
+    lam = args.load * len(reqs) / t_seq
+    gaps = np.random.default_rng(SERVE_SEED ^ 0xA5A5).exponential(1.0 / lam, len(reqs))
+    t_p1, lat_p1 = poisson(s1, gaps, "poisson-1")
+    t_pk, lat_pk = poisson(sk, gaps, f"poisson-{args.slots}")
+    stk = sk.stats()
+    s1.close()
+    sk.close()
+
+    cpu, exact, checked = (None, None, 0) if args.no_cpu else cpu_baseline_serve(reqs, trace, digests,
+                                                                                   args.check_max_side)
+    if exact is False:
+        sys.exit(1)
+    mp = total_px / 1e6
+    out = {
+        "metric": "Mpix/s served segmentation chain (continuous admission, HBM budget), p50/p99 ms",
+        "value": round(mp / t_satk, 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_satk * 1e3, 3),
+        "p50_ms": round(statistics.median(lat_pk), 3),
+        "p99_ms": round(pct(lat_pk, 0.99), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8+i32",
+        "data": "synthetic",
+        "bitexact": exact,
+        "results_same_alone_and_in_mix": True,
+        "oracle_checked_requests": checked,
+        "saturation": {"server_mpix_s": round(mp / t_satk, 2), "sequential_mpix_s": round(mp / t_sat1, 2),
+                       "closed_loop_sequential_mpix_s": round(mp / t_seq, 2),
+                       "speedup": round(t_sat1 / t_satk, 3)},
+        "poisson": {"lambda_req_s": round(lam, 2), "load_vs_sequential_capacity": args.load,
+                    "server": {"p50_ms": round(statistics.median(lat_pk), 3), "p99_ms": round(pct(lat_pk, 0.99), 3),
+                               "mpix_s": round(mp / t_pk, 2)},
+                    "sequential": {"p50_ms": round(statistics.median(lat_p1), 3),
+                                   "p99_ms": round(pct(lat_p1, 0.99), 3), "mpix_s": round(mp / t_p1, 2)}},
+        "service_ms_alone": {"p50": round(statistics.median(svc), 3), "p99": round(pct(svc, 0.99), 3)},
+        "server": stk,
+        "config": {"workload": f"{len(reqs)} requests, square sides log-uniform 256-4096, 80% u8 (>128) / 20% "
+                               "int32 (x257, >32768): recon8 -> fillholes4 -> threshold -> bwlabel8 -> "
+                               "bwareaopen[11,1000) -> relabel; pinned host in/out",
+                   "pixels": total_px, "slots": args.slots, "hbm_budget_bytes": budget},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline_serve(reqs, trace, digests, max_side):
+    """The oracle's chain, single-threaded, on the trace's requests of side <=
+    max_side (a bounded sample); its outputs double as the exactness check of
+    the served labels.  Returns (cpu_baseline, exact, requests checked)."""
+    import zlib
+
+    import numpy as np
+
+    from oracle import oracle as orc
+    exact, n, px, dt = True, 0, 0, 0.0
+    for i, (side, kind, _) in enumerate(trace):
+        if side > max_side:
+            continue
+        marker, mask, thr, lab = reqs[i]
+        t0 = time.perf_counter()
+        if kind == "u8":
+            _, _, want, wn = orc.chain_u8(marker, mask, 8, 4, int(thr), 8, 11, 1000)
+        else:
+            r = orc.imreconstruct(marker, mask, 8)
+            f = orc.fill_holes(r, False, 4)
+            b = np.where(f > thr, 255, 0).astype(np.uint8)
+            want, wn = orc.relabel(orc.bwareaopen2(b, False, False, 11, 1000, 8)[0], -1)
+        dt += time.perf_counter() - t0
+        px += mask.size
+        n += 1
+        if (zlib.crc32(np.ascontiguousarray(want)), wn) != digests[i] or not np.array_equal(lab, want):
+            exact = False
+            log(f"request {i} ({side}x{side} {kind}) differs from the oracle")
+    cpu = {"value": round(px / dt / 1e6, 3) if dt > 0 else None, "unit": "Mpix/s", "cores": 1, "kind": "port",
+           "sample": f"{n} requests of side <= {max_side} ({px / 1e6:.1f} Mpx) through the chain, single thread, "
+                     f"{dt:.1f} s", "host_cpus": os.cpu_count()}
+    return cpu, exact, n
 
 
 TILE_PX = 62 * 62   # interior pixels of one reconstruction tile visit
@@ -187,18 +414,24 @@ def roofline(kt: dict, px: int, visits: tuple, steps: int) -> dict:
             "per_kind_ms_per_step": {k: round(v[0] / steps, 4) for k, v in kt.items()}}
 
 
-def cpu_baseline(marker, mask, runs: int) -> dict:
+def cpu_baseline(marker, mask, runs: int, gpu_first) -> tuple:
     """The oracle's single-threaded restatement of the reference chain, timed
-    on this host on `runs` repetitions of the same 4096x4096 workload."""
+    on this host on `runs` repetitions of the same 4096x4096 workload.  The
+    first repetition's outputs double as the exactness check of the GPU's first
+    step.  Returns (cpu_baseline, exact)."""
+    import numpy as np
+
     from oracle import oracle as orc
-    orc.chain_u8(marker, mask)   # page in
+    want = orc.chain_u8(marker, mask, 8, 4, 128, 8, 11, 1000)
+    exact = bool(all(np.array_equal(a, b) for a, b in zip(gpu_first[:3], want[:3])) and gpu_first[3] == want[3])
+    log(f"bit-exact vs oracle: {exact} (components {gpu_first[3]} / {want[3]})")
     t0 = time.perf_counter()
     for _ in range(runs):
         orc.chain_u8(marker, mask, 8, 4, 128, 8, 11, 1000)
     dt = time.perf_counter() - t0
     return {"value": round(runs * mask.size / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
             "sample": f"{runs} runs of the full {mask.shape[0]}x{mask.shape[1]} chain, single thread, "
-                      f"{dt:.1f} s", "host_cpus": os.cpu_count()}
+                      f"{dt:.1f} s", "host_cpus": os.cpu_count()}, exact
 
 
 def main():
@@ -215,6 +448,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     n = args.size
     seed = 0x5EED0001
     mask = nh.gen_disks_u8(n, n, args.disks, seed)
@@ -234,15 +469,7 @@ def main():
 
     ncomp = step()
     torch.cuda.synchronize()
-    exact = None
-    if not args.no_check:
-        from oracle import oracle as orc
-        er, ef, elab, en = orc.chain_u8(marker, mask, 8, 4, 128, 8, 11, 1000)
-        exact = bool(np.array_equal(d_recon.cpu().numpy(), er) and np.array_equal(d_fill.cpu().numpy(), ef)
-                     and np.array_equal(d_lab.cpu().numpy(), elab) and ncomp == en)
-        log(f"bit-exact vs oracle: {exact} (components {ncomp} / {en})")
-        if not exact:
-            sys.exit(1)
+    first = (d_recon.cpu().numpy(), d_fill.cpu().numpy(), d_lab.cpu().numpy(), ncomp)
 
     for _ in range(args.warmup):
         step()
@@ -264,7 +491,9 @@ def main():
                                                   for k, v in kt.items()))
     log(f"recon stage {stats}; fill stage {fstats}")
     roof = roofline(kt, n * n, (stats["tile_visits"], fstats["tile_visits"]), args.steps)
-    cpu = None if args.no_cpu else cpu_baseline(marker, mask, args.cpu_runs)
+    cpu, exact = (None, None) if args.no_cpu else cpu_baseline(marker, mask, args.cpu_runs, first)
+    if exact is False:
+        sys.exit(1)
     px = n * n
     value = args.steps * px / elapsed / 1e6
     lat_sorted = sorted(lat)

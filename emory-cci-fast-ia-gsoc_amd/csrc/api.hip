@@ -116,22 +116,24 @@ nsx_status to_device(nsx_ctx* ctx, const nsx_img* im, int slot, bool copy_in, hi
         v->pitch = im->pitch_bytes;
         return NSX_OK;
     }
+    // staged planes are packed (pitch = row width) so contiguous host images
+    // move in one linear copy
     const size_t rowb = (size_t)im->cols * dtype_size(im->dtype);
-    const size_t pitch = (rowb + 255) & ~(size_t)255;
+    const size_t pitch = rowb > 0 ? rowb : 1;
     void* p;
     NSX_TRY(ctx->ws(slot, pitch * (size_t)(im->rows > 0 ? im->rows : 1), &p));
     v->data = (char*)p;
     v->pitch = (int64_t)pitch;
-    if (copy_in && im->rows > 0 && rowb > 0)
-        NSX_HIP_TRY(ctx, hipMemcpy2DAsync(p, pitch, im->data, im->pitch_bytes, rowb, im->rows, hipMemcpyHostToDevice, s));
+    if (copy_in && im->rows > 0)
+        NSX_HIP_TRY(ctx, copy_rows(p, pitch, im->data, im->pitch_bytes, rowb, im->rows, hipMemcpyHostToDevice, s));
     return NSX_OK;
 }
 
 nsx_status from_device(nsx_ctx* ctx, const View& v, const nsx_img* im, hipStream_t s) {
     if (im->mem == NSX_MEM_DEVICE) return NSX_OK;
     const size_t rowb = (size_t)im->cols * dtype_size(im->dtype);
-    if (im->rows > 0 && rowb > 0)
-        NSX_HIP_TRY(ctx, hipMemcpy2DAsync(im->data, im->pitch_bytes, v.data, v.pitch, rowb, im->rows, hipMemcpyDeviceToHost, s));
+    if (im->rows > 0)
+        NSX_HIP_TRY(ctx, copy_rows(im->data, im->pitch_bytes, v.data, v.pitch, rowb, im->rows, hipMemcpyDeviceToHost, s));
     NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
     return NSX_OK;
 }
@@ -215,6 +217,25 @@ nsx_status nsx_ctx_synchronize(nsx_ctx* ctx) {
 }
 
 size_t nsx_ctx_workspace_bytes(const nsx_ctx* ctx) { return ctx ? ctx->held : 0; }
+
+nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value) {
+    if (!ctx) return NSX_ERR_INVALID;
+    if (param == NSX_TUNE_VISIT_SWEEPS) {
+        if (value < 1 || value > 64) return ctx->fail(NSX_ERR_INVALID, "visit sweeps must be in 1..64");
+        ctx->visit_sweeps = value;
+        return NSX_OK;
+    }
+    return ctx->fail(NSX_ERR_INVALID, "unknown tuning parameter");
+}
+
+nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value) {
+    if (!ctx || !value) return NSX_ERR_INVALID;
+    if (param == NSX_TUNE_VISIT_SWEEPS) {
+        *value = ctx->visit_sweeps;
+        return NSX_OK;
+    }
+    return NSX_ERR_INVALID;
+}
 
 nsx_status nsx_ctx_set_timing(nsx_ctx* ctx, int enable) {
     if (!ctx) return NSX_ERR_INVALID;
@@ -462,7 +483,7 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
     const size_t es = dtype_size(mask->dtype);
     auto inter = [&](nsx_img* o, int slot, View* v) -> nsx_status {
         if (o && o->mem == NSX_MEM_DEVICE) return to_device(ctx, o, slot, false, s, v);
-        const size_t pitch = (((size_t)mask->cols * es) + 255) & ~(size_t)255;
+        const size_t pitch = std::max<size_t>(1, (size_t)mask->cols * es);
         void* p;
         NSX_TRY(ctx->ws(slot, pitch * (size_t)(mask->rows > 0 ? mask->rows : 1), &p));
         v->data = (char*)p; v->pitch = (int64_t)pitch; v->rows = mask->rows; v->cols = mask->cols; v->dtype = mask->dtype;

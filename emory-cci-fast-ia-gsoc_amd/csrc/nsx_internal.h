@@ -68,6 +68,9 @@ struct nsx_ctx {
     int32_t* h_pinned = nullptr;     // small pinned readback area
     int cu_count = 256;
 
+    // Tuning (nsx_ctx_set_tuning).
+    int visit_sweeps = 5;
+
     // Propagation rounds the last reconstruction of a given size/mode needed.
     std::unordered_map<uint64_t, int> round_hint;
 
@@ -126,6 +129,16 @@ struct View {
 };
 
 inline size_t dtype_size(int dt) { return dt == NSX_U8 ? 1 : 4; }
+
+// Host<->device copy of `rows` rows of `rowb` bytes.  When both sides are
+// contiguous (pitch == row width) it is one linear copy: a strided 2-D copy
+// between pinned host memory and the device is split into per-row transfers.
+inline hipError_t copy_rows(void* dst, size_t dpitch, const void* src, size_t spitch, size_t rowb, size_t rows,
+                            hipMemcpyKind kind, hipStream_t s) {
+    if (rows == 0 || rowb == 0) return hipSuccess;
+    if (dpitch == rowb && spitch == rowb) return hipMemcpyAsync(dst, src, rowb * rows, kind, s);
+    return hipMemcpy2DAsync(dst, dpitch, src, spitch, rowb, rows, kind, s);
+}
 
 // ---- reconstruction ---------------------------------------------------------------
 enum ReconMode { RM_RECON = 0, RM_BINARY = 1, RM_FILL = 2, RM_FILLBIN = 3 };

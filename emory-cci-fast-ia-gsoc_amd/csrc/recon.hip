@@ -37,6 +37,7 @@
 
 #include <climits>
 #include <cfloat>
+#include <cmath>
 #include <algorithm>
 
 namespace nsx {
@@ -45,7 +46,7 @@ namespace {
 constexpr int TS = 64;          // register tile side, halo included
 constexpr int TI = TS - 2;      // interior side written back
 constexpr int LDS_PITCH = TS + 1;
-constexpr int MAX_IT = 64;      // sweep iterations per visit before re-queueing itself
+constexpr int MAX_IT = 64;      // upper bound of the per-visit sweep-pair cap
 
 // ---- order-preserving 32-bit keys -------------------------------------------------
 template <typename T> struct Key;
@@ -139,13 +140,38 @@ template <typename T> struct Codec<T, RM_FILLBIN> {
     __device__ static T enc(uint32_t k, T) { return k ? inv(tmax<T>()) : inv(T(0)); }
 };
 
+// Image accesses go through global-address-space pointers: global_load/store
+// count only against vmcnt, so they do not serialise with the LDS traffic the
+// way flat accesses (vmcnt and lgkmcnt) do.
+template <typename T> using gptr = __attribute__((address_space(1))) T*;
 template <typename T>
 __device__ __forceinline__ T ld(const char* base, int64_t pitch, int y, int x) {
-    return *reinterpret_cast<const T*>(base + (int64_t)y * pitch + (int64_t)x * (int64_t)sizeof(T));
+    return *(gptr<const T>)(base + (int64_t)y * pitch + (int64_t)x * (int64_t)sizeof(T));
 }
 template <typename T>
 __device__ __forceinline__ void st(char* base, int64_t pitch, int y, int x, T v) {
-    *reinterpret_cast<T*>(base + (int64_t)y * pitch + (int64_t)x * (int64_t)sizeof(T)) = v;
+    *(gptr<T>)(base + (int64_t)y * pitch + (int64_t)x * (int64_t)sizeof(T)) = v;
+}
+
+// 32-bit register image of a pixel value and back.
+template <typename T> __device__ __forceinline__ uint32_t bits(T v);
+template <> __device__ __forceinline__ uint32_t bits<uint8_t>(uint8_t v) { return v; }
+template <> __device__ __forceinline__ uint32_t bits<int32_t>(int32_t v) { return (uint32_t)v; }
+template <> __device__ __forceinline__ uint32_t bits<float>(float v) { return __float_as_uint(v); }
+template <typename T> __device__ __forceinline__ T unbits(uint32_t b);
+template <> __device__ __forceinline__ uint8_t unbits<uint8_t>(uint32_t b) { return (uint8_t)b; }
+template <> __device__ __forceinline__ int32_t unbits<int32_t>(uint32_t b) { return (int32_t)b; }
+template <> __device__ __forceinline__ float unbits<float>(uint32_t b) { return __uint_as_float(b); }
+
+// Row loads are issued in chunks of LOAD_CH; an empty asm that takes a chunk's
+// registers as read-write operands marks the point where that chunk must have
+// arrived.  Without it the compiler sinks each load next to its first use and
+// waits for every row in turn.
+constexpr int LOAD_CH = 16;
+__device__ __forceinline__ void arrived(uint32_t (&r)[LOAD_CH]) {
+    asm volatile("" : "+v"(r[0]), "+v"(r[1]), "+v"(r[2]), "+v"(r[3]), "+v"(r[4]), "+v"(r[5]), "+v"(r[6]),
+                      "+v"(r[7]), "+v"(r[8]), "+v"(r[9]), "+v"(r[10]), "+v"(r[11]), "+v"(r[12]), "+v"(r[13]),
+                      "+v"(r[14]), "+v"(r[15]));
 }
 
 // lane i <- lane i-1 (lane 0 <- 0), lane i <- lane i+1 (lane 63 <- 0); 0 is the
@@ -259,6 +285,7 @@ struct ReconArgs {
     int32_t* list1;
     int32_t* counts;           // [3] rotating round counters
     int64_t* stats;            // [0] rounds with work [1] tile visits after round 0 [2] tiles
+    int visit_sweeps;          // sweep pairs per visit before re-queueing (1..MAX_IT)
 };
 
 // The tile id is wave-uniform; readfirstlane makes that visible to the
@@ -296,26 +323,54 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
     const uint32_t F = C::frame();
     const bool needA = FIRST ? C::kMarkerFirst : C::kMarkerLater;
 
+    // Pixels outside the image read a clamped (valid) address and take the
+    // frame value, so no load depends on a branch.
     auto key_at = [&](int gy, int gx, uint32_t& mk) -> uint32_t {
-        if (gx < 0 || gx >= D.cols || gy < 0 || gy >= D.rows) { mk = F; return F; }
-        const T mv = ld<T>(D.mask, D.pk, gy, gx);
-        const T av = needA ? ld<T>(D.marker, D.pm, gy, gx) : T(0);
-        mk = C::mkey(av, mv);
-        if (FIRST) return C::first(av, mv);
-        return C::dec(ld<T>(D.state, D.ps, gy, gx), av, mv);
+        const bool in = (unsigned)gx < (unsigned)D.cols && (unsigned)gy < (unsigned)D.rows;
+        const int cy = min(max(gy, 0), D.rows - 1), cx = min(max(gx, 0), D.cols - 1);
+        const T mv = ld<T>(D.mask, D.pk, cy, cx);
+        const T av = needA ? ld<T>(D.marker, D.pm, cy, cx) : T(0);
+        const uint32_t k = FIRST ? C::first(av, mv) : C::dec(ld<T>(D.state, D.ps, cy, cx), av, mv);
+        mk = in ? C::mkey(av, mv) : F;
+        return in ? k : F;
     };
-
-    uint32_t c[TS];
-    const int gx = x0 + lane;
-#pragma unroll
-    for (int k = 0; k < TS; ++k) {
-        uint32_t mk;
-        c[k] = key_at(y0 + k, gx, mk);
-        mb[k * LDS_PITCH + lane] = (LT)mk;
-    }
     uint32_t dummy;
     const uint32_t initL = key_at(y0 + lane, x0, dummy);
     const uint32_t initR = key_at(y0 + lane, x0 + TS - 1, dummy);
+
+    // The tile's rows: chunk j+1 is in flight while chunk j is decoded.
+    uint32_t c[TS];
+    const int gx = x0 + lane;
+    const bool col_in = (unsigned)gx < (unsigned)D.cols;
+    const int cx = min(max(gx, 0), D.cols - 1);
+    uint32_t rm[2][LOAD_CH], ra[2][LOAD_CH], rs[2][LOAD_CH];
+    auto issue = [&](int b, int k0) {
+#pragma unroll
+        for (int j = 0; j < LOAD_CH; ++j) {
+            const int cy = min(max(y0 + k0 + j, 0), D.rows - 1);
+            rm[b][j] = bits<T>(ld<T>(D.mask, D.pk, cy, cx));
+            ra[b][j] = needA ? bits<T>(ld<T>(D.marker, D.pm, cy, cx)) : 0u;
+            rs[b][j] = FIRST ? 0u : bits<T>(ld<T>(D.state, D.ps, cy, cx));
+        }
+    };
+    issue(0, 0);
+#pragma unroll
+    for (int ch = 0; ch < TS / LOAD_CH; ++ch) {
+        const int b = ch & 1;
+        if (ch + 1 < TS / LOAD_CH) issue(b ^ 1, (ch + 1) * LOAD_CH);
+        arrived(rm[b]);
+        if (needA) arrived(ra[b]);
+        if (!FIRST) arrived(rs[b]);
+#pragma unroll
+        for (int j = 0; j < LOAD_CH; ++j) {
+            const int k = ch * LOAD_CH + j;
+            const bool in = col_in && (unsigned)(y0 + k) < (unsigned)D.rows;
+            const T mv = unbits<T>(rm[b][j]), av = unbits<T>(ra[b][j]);
+            const uint32_t key = FIRST ? C::first(av, mv) : C::dec(unbits<T>(rs[b][j]), av, mv);
+            c[k] = in ? key : F;
+            mb[k * LDS_PITCH + lane] = (LT)(in ? C::mkey(av, mv) : F);
+        }
+    }
     const uint32_t initT = c[0], initB = c[TS - 1];
     __syncthreads();
 
@@ -337,7 +392,7 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
         tot = t1;
         return CONN == 4 ? (quiet >= 1 && half >= 1) : (quiet >= 2);
     };
-    for (int it = 0; it < MAX_IT; ++it) {
+    for (int it = 0; it < A.visit_sweeps; ++it) {
         sweep_pair<CONN, false>(c, mb);               // columns (orientation A)
         if (settled(tile_sum<S>(c))) { converged = true; break; }
         ++half;
@@ -514,6 +569,15 @@ void ReconRun::release() {
     impl = nullptr;
 }
 
+// Records the rounds a problem needed.  A smaller count lowers the prediction
+// by one round per call only: an over-prediction costs empty launches of a few
+// microseconds, an under-prediction a host round trip and a re-run.
+static void note_hint(nsx_ctx* ctx, uint64_t key, int64_t rounds_with_work) {
+    const int need = (int)std::min<int64_t>(4096, rounds_with_work + 1);
+    auto it = ctx->round_hint.find(key);
+    ctx->round_hint[key] = it == ctx->round_hint.end() ? need : std::max(need, it->second - 1);
+}
+
 nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const ReconBatch& B, hipStream_t s,
                         ReconRun* run, int32_t* pending_dev) {
     run->release();
@@ -539,10 +603,14 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     A.list1 = (int32_t*)lists + A.ntiles;
     A.counts = (int32_t*)counts;
     A.stats = B.stats ? B.stats : ctx->d_stats;
+    A.visit_sweeps = std::max(1, std::min(MAX_IT, ctx->visit_sweeps));
     R->later = later;
     // u8 tiles run 3 waves/SIMD (12 per CU), 32-bit tiles 1 (LDS-bound)
     R->grid = std::max(1, std::min(A.ntiles, (dtype == NSX_U8 ? 12 : 4) * ctx->cu_count));
-    R->key = ((uint64_t)A.ntiles << 16) ^ ((uint64_t)mode << 8) ^ ((uint64_t)dtype << 4) ^ (uint64_t)conn;
+    // Round predictions are shared by problems within ~19% of each other in tile
+    // count (quarter-octave buckets), so mixed request sizes still hit.
+    const uint64_t bucket = (uint64_t)std::lround(4.0 * std::log2((double)A.ntiles));
+    R->key = (bucket << 16) ^ ((uint64_t)mode << 8) ^ ((uint64_t)dtype << 4) ^ (uint64_t)conn;
     run->impl = R;
 
     const int grid_reset = std::max(1, std::min(1024, (A.ntiles + 255) / 256));
@@ -593,13 +661,13 @@ nsx_status recon_complete(nsx_ctx* ctx, ReconRun* run, hipStream_t s, bool* exte
     NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 32, A.stats, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
     NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
     std::memcpy(st, ctx->h_pinned + 32, sizeof(st));
-    ctx->round_hint[R->key] = (int)std::min<int64_t>(4096, st[0] + 1);
+    note_hint(ctx, R->key, st[0]);
     run->release();
     return NSX_OK;
 }
 
 void recon_note_rounds(nsx_ctx* ctx, ReconRun* run, int64_t rounds_with_work) {
-    if (run->impl) ctx->round_hint[run->impl->key] = (int)std::min<int64_t>(4096, rounds_with_work + 1);
+    if (run->impl) note_hint(ctx, run->impl->key, rounds_with_work);
     run->release();
 }
 

@@ -143,6 +143,14 @@ class Context:
         self.check(self.lib.nsx_ctx_read_timing(self.h, ms, n, k))
         return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(self.TIMED_KINDS)}
 
+    def set_tuning(self, param: int, value: int) -> None:
+        self.check(self.lib.nsx_ctx_set_tuning(self.h, param, value))
+
+    def get_tuning(self, param: int) -> int:
+        v = C.c_int(0)
+        self.check(self.lib.nsx_ctx_get_tuning(self.h, param, C.byref(v)))
+        return int(v.value)
+
     def recon_stats(self, stage: int = 0) -> dict:
         """Statistics of the last reconstruction (stage 1: the chain's fill)."""
         s = (C.c_int64 * 4)()
@@ -338,6 +346,70 @@ class Batch:
         if getattr(self, "h", None):
             self.lib.nsx_batch_destroy(self.h)
             self.h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop for the segmentation chain (nsx_server_*): requests are
+    admitted in arrival order as worker slots and HBM arena space free up, and
+    the work of admitted requests overlaps on the GPU.  Each request computes
+    segment_chain(marker, mask, ...) on its own buffers."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 1 << 30, slots: int = 4):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        st = self.lib.nsx_server_create(device, hbm_budget_bytes, slots, C.byref(h))
+        if st != _lib.NSX_OK:
+            raise NsxError(st, f"nsx_server_create(device={device}, budget={hbm_budget_bytes}, slots={slots})")
+        self.h = h
+        self._keep = {}   # ticket -> buffers the worker may still read or write
+
+    def _check(self, st: int) -> None:
+        if st != _lib.NSX_OK:
+            raise NsxError(st, (self.lib.nsx_server_last_error(self.h) or b"").decode())
+
+    def submit(self, marker, mask, recon_conn=8, fill_conn=4, thresh=128.0, ccl_conn=8, min_size=11,
+               max_size=1000, *, labels_out=None, recon_out=None, fill_out=None) -> int:
+        """Queues one request; returns its ticket.  Host (numpy) or device
+        images; outputs are allocated like the mask when not given (labels
+        always, recon/fill only when passed)."""
+        labels_out = _like(mask, np.int32) if labels_out is None else labels_out
+        t = C.c_uint64(0)
+        r = C.byref(describe(recon_out)) if recon_out is not None else None
+        f = C.byref(describe(fill_out)) if fill_out is not None else None
+        self._check(self.lib.nsx_server_submit(self.h, C.byref(describe(marker)), C.byref(describe(mask)),
+                                               recon_conn, fill_conn, float(thresh), ccl_conn, min_size, max_size,
+                                               r, f, C.byref(describe(labels_out)), C.byref(t)))
+        self._keep[t.value] = (marker, mask, labels_out, recon_out, fill_out)
+        return t.value
+
+    def wait(self, ticket: int):
+        """Blocks until the request is done.  Returns (labels, recon, fill,
+        ncomp, times_ms) with times_ms = (queued, service, total)."""
+        n = C.c_int32(0)
+        tm = (C.c_double * 3)()
+        st = self.lib.nsx_server_wait(self.h, ticket, C.byref(n), tm)
+        bufs = self._keep.pop(ticket, None)
+        self._check(st)
+        return bufs[2], bufs[3], bufs[4], int(n.value), (tm[0], tm[1], tm[2])
+
+    def stats(self) -> dict:
+        a = (C.c_int64 * 8)()
+        self._check(self.lib.nsx_server_stats(self.h, a))
+        keys = ("completed", "in_flight", "queued", "peak_in_flight", "arena_bytes", "arena_peak_bytes",
+                "workspace_bytes", "slots")
+        return dict(zip(keys, (int(v) for v in a)))
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.nsx_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
 
     def __del__(self):  # pragma: no cover
         try:

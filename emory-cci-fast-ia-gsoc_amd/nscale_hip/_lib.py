@@ -16,6 +16,7 @@ NSX_ERR_BUDGET = 5
 NSX_ERR_NOT_CONVERGED = 6
 
 NSX_U8, NSX_I32, NSX_F32 = 0, 1, 2
+NSX_TUNE_VISIT_SWEEPS = 0
 NSX_MEM_DEVICE, NSX_MEM_HOST = 0, 1
 
 
@@ -45,6 +46,8 @@ SIGNATURES = {
     "nsx_last_recon_stats": (C.c_int, [_CTX, C.c_int, P(C.c_int64)]),
     "nsx_ctx_set_timing": (C.c_int, [_CTX, C.c_int]),
     "nsx_ctx_read_timing": (C.c_int, [_CTX, P(C.c_double), P(C.c_int64), C.c_int]),
+    "nsx_ctx_set_tuning": (C.c_int, [_CTX, C.c_int, C.c_int]),
+    "nsx_ctx_get_tuning": (C.c_int, [_CTX, C.c_int, P(C.c_int)]),
     "nsx_imreconstruct": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_imreconstruct_binary": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_fill_holes": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_int, C.c_void_p]),
@@ -63,6 +66,13 @@ SIGNATURES = {
     "nsx_batch_run": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nsx_batch_counts": (C.c_int, [C.c_void_p, _I32P, C.c_int]),
     "nsx_batch_destroy": (None, [C.c_void_p]),
+    "nsx_server_create": (C.c_int, [C.c_int, C.c_size_t, C.c_int, P(C.c_void_p)]),
+    "nsx_server_submit": (C.c_int, [C.c_void_p, _IMG, _IMG, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                    C.c_int, _IMG, _IMG, _IMG, P(C.c_uint64)]),
+    "nsx_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, _I32P, P(C.c_double)]),
+    "nsx_server_stats": (C.c_int, [C.c_void_p, P(C.c_int64)]),
+    "nsx_server_last_error": (C.c_char_p, [C.c_void_p]),
+    "nsx_server_destroy": (None, [C.c_void_p]),
     "nsx_splitmix64_next": (C.c_uint64, [P(C.c_uint64)]),
     "nsx_gen_disks_u8": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_long, C.c_int, C.c_uint64]),
     "nsx_gen_marker_u8": (None, [C.c_void_p, C.c_void_p, C.c_long, C.c_int]),

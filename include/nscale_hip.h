@@ -81,6 +81,15 @@ nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int stage, int64_t stats[4]);
 nsx_status nsx_ctx_set_timing(nsx_ctx* ctx, int enable);
 nsx_status nsx_ctx_read_timing(nsx_ctx* ctx, double* ms, int64_t* launches, int nkinds);
 
+/* Tuning parameters of the context's kernels (results never depend on them).
+ * NSX_TUNE_VISIT_SWEEPS: sweep pairs (column + row half-pass) a reconstruction
+ * tile runs per visit before it re-queues itself unfinished, 1..64 (default
+ * 5); small caps keep one slow tile from stretching a propagation round.
+ * No reference counterpart. */
+enum { NSX_TUNE_VISIT_SWEEPS = 0, NSX_TUNE_NPARAMS = 1 };
+nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value);
+nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value);
+
 /* ---- morphology ----------------------------------------------------------- */
 /* Grayscale reconstruction by dilation of `marker` under `mask`; conn 8 gives
  * 8-connectivity, anything else 4.  u8, i32, f32.
@@ -180,6 +189,40 @@ nsx_status nsx_batch_run(nsx_batch* batch, void* stream);
 /* Per-tile component counts after a run (array of n tiles). Synchronises. */
 nsx_status nsx_batch_counts(nsx_batch* batch, int32_t* counts, int n);
 void nsx_batch_destroy(nsx_batch* batch);
+
+/* ---- serving loop --------------------------------------------------------- */
+/* Continuous admission of segmentation-chain requests (the nsx_segment_chain
+ * pipeline) on one GPU.  `slots` worker threads each own a context and a
+ * stream; a request is admitted when a slot is idle and its device buffers fit
+ * in a fixed HBM arena of `hbm_budget_bytes` (allocated once; no per-request
+ * hipMalloc).  Requests are admitted in arrival order; the work of admitted
+ * requests overlaps on the device.  Each request's inputs are copied into its
+ * arena block, all stages run there, and the outputs the caller asked for are
+ * copied back before the request completes.  Results are those of
+ * nsx_segment_chain on the same inputs, however many requests are in flight.
+ * No reference counterpart: the reference's GPU functions are synchronous and
+ * not reentrant (global queue state in global_queue.cu:9-22); the request
+ * processing matches the chain in HistologicalEntities.cpp:195,219,246,567. */
+typedef struct nsx_server nsx_server;
+nsx_status nsx_server_create(int device, size_t hbm_budget_bytes, int slots, nsx_server** server);
+/* Queues one request and returns its ticket.  marker/mask: u8 or i32, host or
+ * device; labels_out int32 (required), recon_out/fill_out optional (NULL), host
+ * or device.  Host buffers must stay valid until nsx_server_wait returns for the
+ * ticket.  NSX_ERR_BUDGET when the request alone exceeds the arena. */
+nsx_status nsx_server_submit(nsx_server* server, const nsx_img* marker, const nsx_img* mask, int recon_conn,
+                             int fill_conn, double thresh, int ccl_conn, int min_size, int max_size,
+                             nsx_img* recon_out, nsx_img* fill_out, nsx_img* labels_out, uint64_t* ticket);
+/* Blocks until the request completes and releases the ticket.  Returns the
+ * request's status; *ncomp = components kept; times_ms (optional, 3 entries) =
+ * queueing (submit -> admit), service (admit -> done), total (submit -> done). */
+nsx_status nsx_server_wait(nsx_server* server, uint64_t ticket, int32_t* ncomp, double* times_ms);
+/* stats (8 entries): completed, in flight, queued, peak in flight, arena bytes,
+ * peak arena bytes in use, workspace bytes held by the slot contexts, slots. */
+nsx_status nsx_server_stats(nsx_server* server, int64_t* stats);
+/* Message of the last failed server call or request. */
+const char* nsx_server_last_error(nsx_server* server);
+/* Finishes every queued request, then releases the workers and the arena. */
+void nsx_server_destroy(nsx_server* server);
 
 /* ---- synthetic inputs (host) ---------------------------------------------- */
 uint64_t nsx_splitmix64_next(uint64_t* state);

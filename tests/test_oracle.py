@@ -227,3 +227,34 @@ def test_golden_fixtures(orc):
         assert np.array_equal(orc.imreconstruct_binary(g["bin_seed"], g["bin_mask"], conn), g[f"recon_bin_c{conn}"])
         assert np.array_equal(orc.bwlabel2(g["bin_mask"], conn, False)[0], g[f"label_root_c{conn}"])
         assert np.array_equal(orc.bwlabel2(g["bin_mask"], conn, True)[0], g[f"label_canon_c{conn}"])
+
+
+def test_chain_composition_matches_chain_u8(orc):
+    """The per-operation composition used for the int32 serving requests equals
+    the oracle's fused u8 chain."""
+    import numpy as np
+    from chain_expect import expected_chain
+    from nscale_hip import gen_disks_u8, gen_marker_u8
+    mask = gen_disks_u8(150, 170, 30, 0x5EED0004)
+    marker = gen_marker_u8(mask, 30)
+    want = orc.chain_u8(marker, mask, 8, 4, 128, 8, 11, 1000)
+    got = expected_chain(orc, marker, mask, 128)
+    for a, b in zip(want[:3], got[:3]):
+        assert np.array_equal(a, b)
+    assert want[3] == got[3]
+
+
+def test_chain_i32_scaling_is_order_isomorphic(orc):
+    """int32 requests are the u8 image scaled by 257 with threshold 32768:
+    reconstruction and fill commute with the scaling, and 257*v > 32768 holds
+    exactly when v > 127, so the labels match the u8 chain thresholded at 127."""
+    import numpy as np
+    from chain_expect import expected_chain
+    from nscale_hip import gen_disks_u8, gen_marker_u8, scale_i32
+    mask = gen_disks_u8(120, 140, 25, 0x5EED0005)
+    marker = gen_marker_u8(mask, 30)
+    r8, f8, l8, n8 = expected_chain(orc, marker, mask, 127)
+    r32, f32, l32, n32 = expected_chain(orc, scale_i32(marker, 257), scale_i32(mask, 257), 32768)
+    assert np.array_equal(r32, r8.astype(np.int32) * 257)
+    assert np.array_equal(f32, f8.astype(np.int32) * 257)
+    assert np.array_equal(l32, l8) and n32 == n8
