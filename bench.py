@@ -474,7 +474,6 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    ctx.set_timing(True)
     lat = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -483,10 +482,17 @@ def main():
         lat.append((time.perf_counter() - s0) * 1e3)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    kt = ctx.read_timing()
-    ctx.set_timing(False)
     stats = ctx.recon_stats(0)
     fstats = ctx.recon_stats(1)
+    # Per-kernel durations come from a second set of steps with HIP events
+    # around every launch (events add a gap between launches, so the timed
+    # steps above run without them).
+    ctx.set_timing(True)
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    kt = ctx.read_timing()
+    ctx.set_timing(False)
     log("kernel time per step (ms): " + ", ".join(f"{k}={v[0] / args.steps:.4f} ({v[1] / args.steps:.1f} launches)"
                                                   for k, v in kt.items()))
     log(f"recon stage {stats}; fill stage {fstats}")

@@ -298,14 +298,35 @@ __device__ __forceinline__ void tile_of(const ReconArgs& A, int32_t t, ImgDesc& 
     lt = t - D.tile_base;
 }
 
-__device__ __forceinline__ void activate(const ReconArgs& A, const ImgDesc& D, int ty, int tx, int round) {
-    if (ty < 0 || tx < 0 || ty >= D.nty || tx >= D.ntx) return;
-    const int t = D.tile_base + ty * D.ntx + tx;
-    const uint32_t want = (uint32_t)round + 2u;   // "queued for round+1"
-    if (atomicMax(&A.stamp[t], want) < want) {
-        const int idx = atomicAdd(&A.counts[(round + 1) % 3], 1);
+// Queues up to nine tiles for the next round at once: lane j < 9 handles
+// candidate j (the eight neighbours, then the tile itself) when bit j of
+// `want` is set.  Each lane claims its tile through the per-tile stamp
+// (atomicMax; a tile is queued at most once per round), and one atomicAdd
+// reserves list slots for every lane that won a claim.
+__device__ __forceinline__ void activate_all(const ReconArgs& A, const ImgDesc& D, int ty, int tx, int round,
+                                             uint32_t want) {
+    const int lane = threadIdx.x;
+    // candidate j: up, up-left, up-right, down, down-left, down-right, left,
+    // right, self; dx + 1 of candidate j is held in bits 2j..2j+1 of 0x18861
+    const int dy = lane < 3 ? -1 : lane < 6 ? 1 : 0;
+    const int dx = lane < 9 ? (int)((0x18861u >> (2 * lane)) & 3u) - 1 : 0;
+    const int ny = ty + dy, nx = tx + dx;
+    bool claim = false;
+    int t = 0;
+    if (lane < 9 && ((want >> lane) & 1u) && ny >= 0 && nx >= 0 && ny < D.nty && nx < D.ntx) {
+        t = D.tile_base + ny * D.ntx + nx;
+        const uint32_t stamp = (uint32_t)round + 2u;   // "queued for round+1"
+        claim = atomicMax(&A.stamp[t], stamp) < stamp;
+    }
+    const uint64_t won = __ballot(claim);
+    if (!won) return;
+    const int first = __builtin_ctzll(won);
+    int base = 0;
+    if (lane == first) base = atomicAdd(&A.counts[(round + 1) % 3], __builtin_popcountll(won));
+    base = __shfl(base, first, 64);
+    if (claim) {
         int32_t* nl = ((round + 1) & 1) ? A.list1 : A.list0;
-        nl[idx] = t;
+        nl[base + __builtin_popcountll(won & ((1ull << lane) - 1ull))] = t;
     }
 }
 
@@ -313,7 +334,8 @@ __device__ __forceinline__ void activate(const ReconArgs& A, const ImgDesc& D, i
 // back and queues the tiles whose pixels in the halo rose.
 template <typename T, int MODE, int CONN, bool FIRST>
 __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& D, int lt, int round,
-                                             typename LdsT<T>::type* mb, typename LdsT<T>::type* tb) {
+                                             typename LdsT<T>::type* mb, typename LdsT<T>::type* tb,
+                                             uint32_t* hs) {
     using LT = typename LdsT<T>::type;
     using C = Codec<T, MODE>;
     using S = typename SumT<T>::type;
@@ -325,18 +347,23 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
 
     // Pixels outside the image read a clamped (valid) address and take the
     // frame value, so no load depends on a branch.
-    auto key_at = [&](int gy, int gx, uint32_t& mk) -> uint32_t {
-        const bool in = (unsigned)gx < (unsigned)D.cols && (unsigned)gy < (unsigned)D.rows;
-        const int cy = min(max(gy, 0), D.rows - 1), cx = min(max(gx, 0), D.cols - 1);
-        const T mv = ld<T>(D.mask, D.pk, cy, cx);
-        const T av = needA ? ld<T>(D.marker, D.pm, cy, cx) : T(0);
-        const uint32_t k = FIRST ? C::first(av, mv) : C::dec(ld<T>(D.state, D.ps, cy, cx), av, mv);
+    auto key_of = [&](bool in, uint32_t m, uint32_t a, uint32_t st, uint32_t& mk) -> uint32_t {
+        const T mv = unbits<T>(m), av = unbits<T>(a);
+        const uint32_t key = FIRST ? C::first(av, mv) : C::dec(unbits<T>(st), av, mv);
         mk = in ? C::mkey(av, mv) : F;
-        return in ? k : F;
+        return in ? key : F;
     };
-    uint32_t dummy;
-    const uint32_t initL = key_at(y0 + lane, x0, dummy);
-    const uint32_t initR = key_at(y0 + lane, x0 + TS - 1, dummy);
+    // Left / right halo columns (lane = row): their keys before the sweeps.
+    // They are loaded with the first row chunk and decoded when it arrives.
+    const int hy = min(max(y0 + lane, 0), D.rows - 1);
+    const int hxl = min(max(x0, 0), D.cols - 1), hxr = min(max(x0 + TS - 1, 0), D.cols - 1);
+    uint32_t hraw[6];
+    hraw[0] = bits<T>(ld<T>(D.mask, D.pk, hy, hxl));
+    hraw[1] = bits<T>(ld<T>(D.mask, D.pk, hy, hxr));
+    hraw[2] = needA ? bits<T>(ld<T>(D.marker, D.pm, hy, hxl)) : 0u;
+    hraw[3] = needA ? bits<T>(ld<T>(D.marker, D.pm, hy, hxr)) : 0u;
+    hraw[4] = FIRST ? 0u : bits<T>(ld<T>(D.state, D.ps, hy, hxl));
+    hraw[5] = FIRST ? 0u : bits<T>(ld<T>(D.state, D.ps, hy, hxr));
 
     // The tile's rows: chunk j+1 is in flight while chunk j is decoded.
     uint32_t c[TS];
@@ -354,6 +381,8 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
         }
     };
     issue(0, 0);
+    // hs[0..255]: the halo's keys before the sweeps (top row, bottom row, left
+    // column, right column), parked in LDS while the sweeps hold the registers
 #pragma unroll
     for (int ch = 0; ch < TS / LOAD_CH; ++ch) {
         const int b = ch & 1;
@@ -361,17 +390,26 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
         arrived(rm[b]);
         if (needA) arrived(ra[b]);
         if (!FIRST) arrived(rs[b]);
+        if (ch == 0) {
+            asm volatile("" : "+v"(hraw[0]), "+v"(hraw[1]), "+v"(hraw[2]), "+v"(hraw[3]), "+v"(hraw[4]),
+                              "+v"(hraw[5]));
+            const bool rin = (unsigned)(y0 + lane) < (unsigned)D.rows;
+            uint32_t dummy;
+            hs[2 * TS + lane] = key_of(rin && (unsigned)x0 < (unsigned)D.cols, hraw[0], hraw[2], hraw[4], dummy);
+            hs[3 * TS + lane] =
+                key_of(rin && (unsigned)(x0 + TS - 1) < (unsigned)D.cols, hraw[1], hraw[3], hraw[5], dummy);
+        }
 #pragma unroll
         for (int j = 0; j < LOAD_CH; ++j) {
             const int k = ch * LOAD_CH + j;
             const bool in = col_in && (unsigned)(y0 + k) < (unsigned)D.rows;
-            const T mv = unbits<T>(rm[b][j]), av = unbits<T>(ra[b][j]);
-            const uint32_t key = FIRST ? C::first(av, mv) : C::dec(unbits<T>(rs[b][j]), av, mv);
-            c[k] = in ? key : F;
-            mb[k * LDS_PITCH + lane] = (LT)(in ? C::mkey(av, mv) : F);
+            uint32_t mk;
+            c[k] = key_of(in, rm[b][j], ra[b][j], rs[b][j], mk);
+            mb[k * LDS_PITCH + lane] = (LT)mk;
         }
     }
-    const uint32_t initT = c[0], initB = c[TS - 1];
+    hs[lane] = c[0];
+    hs[TS + lane] = c[TS - 1];
     __syncthreads();
 
     // Half-passes alternate between the column sweeps (orientation A: lane =
@@ -430,29 +468,26 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
     }
 
     // Halo pixels that rose above what was read: their owners must re-visit.
-    const uint64_t topm = __ballot(c[0] > initT);
-    const uint64_t botm = __ballot(c[TS - 1] > initB);
-    const uint64_t lefm = __ballot(finL > initL);
-    const uint64_t rigm = __ballot(finR > initR);
-    if (lane == 0) {
-        const uint64_t mid = 0x7FFFFFFFFFFFFFFEull;
-        if (topm & mid) activate(A, D, ty - 1, tx, round);
-        if (topm & 1ull) activate(A, D, ty - 1, tx - 1, round);
-        if (topm >> 63) activate(A, D, ty - 1, tx + 1, round);
-        if (botm & mid) activate(A, D, ty + 1, tx, round);
-        if (botm & 1ull) activate(A, D, ty + 1, tx - 1, round);
-        if (botm >> 63) activate(A, D, ty + 1, tx + 1, round);
-        if (lefm & mid) activate(A, D, ty, tx - 1, round);
-        if (rigm & mid) activate(A, D, ty, tx + 1, round);
-        if (!converged) activate(A, D, ty, tx, round);
-    }
+    const uint64_t topm = __ballot(c[0] > hs[lane]);
+    const uint64_t botm = __ballot(c[TS - 1] > hs[TS + lane]);
+    const uint64_t lefm = __ballot(finL > hs[2 * TS + lane]);
+    const uint64_t rigm = __ballot(finR > hs[3 * TS + lane]);
+    const uint64_t mid = 0x7FFFFFFFFFFFFFFEull;
+    const uint32_t want = ((topm & mid) ? 1u : 0u) | ((topm & 1ull) ? 2u : 0u) | ((topm >> 63) ? 4u : 0u) |
+                          ((botm & mid) ? 8u : 0u) | ((botm & 1ull) ? 16u : 0u) | ((botm >> 63) ? 32u : 0u) |
+                          ((lefm & mid) ? 64u : 0u) | ((rigm & mid) ? 128u : 0u) | (converged ? 0u : 256u);
+    if (want) activate_all(A, D, ty, tx, round, want);
 }
 
+// u8 tiles keep their mask and transpose staging in 8.3 KB of LDS and run
+// 3 waves per SIMD (at most 168 VGPRs); 32-bit tiles are LDS-bound at 1.
 template <typename T, int MODE, int CONN, bool FIRST>
-__global__ __launch_bounds__(64) void recon_round_kernel(ReconArgs A, int round) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 1 ? 3 : 1)))
+void recon_round_kernel(ReconArgs A, int round) {
     using LT = typename LdsT<T>::type;
     __shared__ LT mb[TS * LDS_PITCH];   // mask keys of the tile
     __shared__ LT tb[TS * LDS_PITCH];   // transpose staging
+    __shared__ uint32_t hs[4 * TS];     // halo keys before the sweeps
     int n;
     const int32_t* list = (round & 1) ? A.list1 : A.list0;
     if (FIRST) n = A.ntiles;
@@ -469,7 +504,7 @@ __global__ __launch_bounds__(64) void recon_round_kernel(ReconArgs A, int round)
         ImgDesc D;
         int lt;
         tile_of(A, t, D, lt);
-        process_tile<T, MODE, CONN, FIRST>(A, D, lt, round, mb, tb);
+        process_tile<T, MODE, CONN, FIRST>(A, D, lt, round, mb, tb, hs);
         __syncthreads();   // the next tile overwrites mb
     }
 }
