@@ -225,6 +225,11 @@ nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value) {
         ctx->visit_sweeps = value;
         return NSX_OK;
     }
+    if (param == NSX_TUNE_LEVELSET_LEVELS) {
+        if (value < 0 || value > 255) return ctx->fail(NSX_ERR_INVALID, "level-set levels must be in 0..255");
+        ctx->levelset_levels = value;
+        return NSX_OK;
+    }
     return ctx->fail(NSX_ERR_INVALID, "unknown tuning parameter");
 }
 
@@ -232,6 +237,10 @@ nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value) {
     if (!ctx || !value) return NSX_ERR_INVALID;
     if (param == NSX_TUNE_VISIT_SWEEPS) {
         *value = ctx->visit_sweeps;
+        return NSX_OK;
+    }
+    if (param == NSX_TUNE_LEVELSET_LEVELS) {
+        *value = ctx->levelset_levels;
         return NSX_OK;
     }
     return NSX_ERR_INVALID;

@@ -49,6 +49,7 @@ constexpr int BLOCK_BITS = 32768;    // bitmap scan block (1024 words)
 struct CclImg {
     const char* in; int64_t pin;
     char* out; int64_t pout;
+    const char* wgt; int64_t pw;   // u8 weight image (per-component maximum mode)
     int rows, cols, ntx, nty;
     int tile_base;     // first tile of the image in the launch
     int pix_base;      // global index of pixel (0,0); multiple of BLOCK_BITS
@@ -71,6 +72,8 @@ struct CclArgs {
     int32_t* ncomp;            // kept components per image
     int min_size, max_size;
     bool use_area;
+    bool wmax;                 // `area` holds each component's maximum weight, not its size
+    int level;                 // CO_LEVEL: out = max(out, min(level, maximum weight))
     int out_mode;
 };
 
@@ -210,7 +213,9 @@ __device__ __forceinline__ int32_t gidx(const CclImg& D, int y0, int x0, uint32_
     return D.pix_base + (y0 + (int)(li >> 6)) * D.cols + x0 + (int)(li & 63);
 }
 
-template <int CONN, typename InT, bool THR>
+// WMAX: each local component's slot carries the maximum of the u8 weight image
+// over its pixels instead of its pixel count.
+template <int CONN, typename InT, bool THR, bool WMAX>
 __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
     __shared__ uint32_t val[CN];
     __shared__ uint32_t par[CN];
@@ -223,13 +228,25 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
     const int x0 = tx * CW, y0 = ty * CH;
     for (int i = tid; i < CN; i += CTHREADS) cnt[i] = 0;
     if (tid == 0) nroots = 0;
-    local_label<CONN, InT, THR>(A, D, x0, y0, val, par);
     const int lx = tid & 63, w = tid >> 6;
     const int gx = x0 + lx;
+    uint32_t wv[WMAX ? CPT : 1];
+    if (WMAX) {
+        // issued before the labelling so they arrive while it runs
+        const int cx = min(gx, D.cols - 1);
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int cy = min(y0 + w + 4 * j, D.rows - 1);
+            wv[j] = ((const uint8_t*)(D.wgt + (int64_t)cy * D.pw))[cx];
+        }
+    }
+    local_label<CONN, InT, THR>(A, D, x0, y0, val, par);
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int i = (w + 4 * j) * CW + lx;
-        if (par[i] != BGL) atomicAdd(&cnt[par[i]], 1u);
+        if (par[i] == BGL) continue;
+        if (WMAX) atomicMax(&cnt[par[i]], wv[j]);
+        else atomicAdd(&cnt[par[i]], 1u);
     }
     __syncthreads();
     uint2* slots = A.entries + (int64_t)blockIdx.x * CN;
@@ -244,7 +261,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
             const uint32_t k = atomicAdd(&nroots, 1u);
             slots[k] = make_uint2((uint32_t)g, cnt[i]);
             A.lab[g] = g;
-            if (A.use_area) A.area[g] = 0;
+            if (A.use_area || WMAX) A.area[g] = 0;
         } else if (ly == 0 || ly == CH - 1 || lx == 0 || lx == CW - 1) {
             A.lab[g] = gidx(D, y0, x0, r);
         }
@@ -300,7 +317,8 @@ __global__ __launch_bounds__(256) void ccl_area_kernel(CclArgs A) {
     for (int e = threadIdx.x; e < n; e += blockDim.x) {
         const uint2 en = slots[e];
         const int32_t r = g_find(A.lab, (int32_t)en.x);
-        atomicAdd(&A.area[r], (int)en.y);
+        if (A.wmax) atomicMax(&A.area[r], (int)en.y);
+        else atomicAdd(&A.area[r], (int)en.y);
     }
 }
 
@@ -424,6 +442,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
             case CO_ROOT: o = r - D.pix_base; break;
             case CO_ROOT_FILTERED: o = kept(A, r) ? r - D.pix_base : -1; break;
             case CO_CANON: o = kept(A, r) ? rank1(A, D, r) : 0; break;
+            case CO_LEVEL: o = min(A.level, A.area[r]); break;
             default: o = kept(A, r) ? 255 : 0; break;
         }
         outv[i] = o;
@@ -436,8 +455,16 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
         const int gy = y0 + ly;
         if (gx >= D.cols || gy >= D.rows) continue;
         const uint32_t r = par[ly * CW + lx];
-        const int32_t o = (r == BGL) ? bg : outv[r];
         char* row = D.out + (int64_t)gy * D.pout;
+        if (A.out_mode == CO_LEVEL) {
+            // u8 reconstruction accumulated over levels: foreground only
+            if (r != BGL) {
+                uint8_t* p = (uint8_t*)row + gx;
+                *p = (uint8_t)max((int32_t)*p, outv[r]);
+            }
+            continue;
+        }
+        const int32_t o = (r == BGL) ? bg : outv[r];
         if (A.out_mode == CO_FLAT_U8) ((uint8_t*)row)[gx] = (uint8_t)o;
         else ((int32_t*)row)[gx] = o;
     }
@@ -450,10 +477,26 @@ __global__ void zero_words_kernel(uint32_t* p, int64_t n) {
 
 template <int CONN, typename InT, bool THR>
 void ccl_launch_conn(nsx_ctx* ctx, const CclArgs& A, hipStream_t s) {
+    if (A.out_mode == CO_LEVEL) {
+        // per-component maximum weight folded into a u8 image: no bitmap or ranks
+        int h = ctx->tic(TK_CCL_LOCAL, s);
+        hipLaunchKernelGGL((ccl_local_kernel<CONN, InT, THR, true>), dim3(A.ntiles), dim3(CTHREADS), 0, s, A);
+        ctx->toc(h, s);
+        h = ctx->tic(TK_CCL_MERGE, s);
+        hipLaunchKernelGGL((ccl_merge_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(128), 0, s, A);
+        ctx->toc(h, s);
+        h = ctx->tic(TK_CCL_SMALL, s);
+        hipLaunchKernelGGL(ccl_area_kernel, dim3(A.ntiles), dim3(256), 0, s, A);
+        ctx->toc(h, s);
+        h = ctx->tic(TK_CCL_FINAL, s);
+        hipLaunchKernelGGL((ccl_final_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(CTHREADS), 0, s, A);
+        ctx->toc(h, s);
+        return;
+    }
     const int stream_grid = 4 * ctx->cu_count;
     hipLaunchKernelGGL(zero_words_kernel, dim3(stream_grid), dim3(256), 0, s, A.bits, A.nwords);
     int h = ctx->tic(TK_CCL_LOCAL, s);
-    hipLaunchKernelGGL((ccl_local_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(CTHREADS), 0, s, A);
+    hipLaunchKernelGGL((ccl_local_kernel<CONN, InT, THR, false>), dim3(A.ntiles), dim3(CTHREADS), 0, s, A);
     ctx->toc(h, s);
     h = ctx->tic(TK_CCL_MERGE, s);
     hipLaunchKernelGGL((ccl_merge_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(128), 0, s, A);
@@ -515,12 +558,16 @@ nsx_status ccl_launch(nsx_ctx* ctx, const CclBatch& B, const CclParams& p, hipSt
     A.thr_i = (int64_t)std::floor(std::max(-9.0e18, std::min(9.0e18, p.thr)));
     A.thr_f = (float)p.thr;
     A.min_size = p.min_size; A.max_size = p.max_size; A.use_area = p.use_area;
+    A.wmax = p.out_mode == CO_LEVEL;
+    A.level = p.level;
+    if (A.wmax && (!p.threshold || B.in_dtype != NSX_U8 || B.nimg != 1))
+        return ctx->fail(NSX_ERR_INVALID, "level labelling needs one thresholded u8 image");
     A.out_mode = p.out_mode;
     A.nwords = B.pix_total / 32;
     A.nblocks = (int)(B.pix_total / BLOCK_BITS);
     void *lab, *area = nullptr, *ent, *tcnt, *bits, *scan;
     NSX_TRY(ctx->ws(B.ws_base + 0, sizeof(int32_t) * (size_t)B.pix_total, &lab));
-    if (A.use_area) NSX_TRY(ctx->ws(B.ws_base + 1, sizeof(int32_t) * (size_t)B.pix_total, &area));
+    if (A.use_area || A.wmax) NSX_TRY(ctx->ws(B.ws_base + 1, sizeof(int32_t) * (size_t)B.pix_total, &area));
     NSX_TRY(ctx->ws(B.ws_base + 2, sizeof(uint2) * (size_t)B.ntiles * CN, &ent));
     NSX_TRY(ctx->ws(B.ws_base + 3, sizeof(int32_t) * (size_t)B.ntiles, &tcnt));
     NSX_TRY(ctx->ws(B.ws_base + 4, sizeof(uint32_t) * (size_t)A.nwords, &bits));
@@ -552,6 +599,102 @@ nsx_status ccl_run(nsx_ctx* ctx, const View& in, const View& out, const CclParam
     B.ws_base = WS_CCL_LABEL;
     B.ncomp_dev = ncomp_dev ? ncomp_dev : (int32_t*)ctx->d_stats + 30;
     return ccl_launch(ctx, B, p, s);
+}
+
+nsx_status ccl_level_run(nsx_ctx* ctx, const View& mask, const View& weight, const View& out, int conn, int level,
+                         hipStream_t s) {
+    if ((int64_t)mask.rows * mask.cols == 0) return NSX_OK;
+    CclImg D;
+    ccl_fill_desc(&D, mask, out, 0, 0);
+    D.wgt = weight.data;
+    D.pw = weight.pitch;
+    CclBatch B{};
+    B.nimg = 1;
+    B.ntiles = ccl_tiles(mask.rows, mask.cols);
+    B.pix_total = ccl_pix_span(mask.rows, mask.cols);
+    B.host_desc = &D;
+    B.in_dtype = mask.dtype;
+    B.ws_base = WS_CCL_LABEL;
+    B.ncomp_dev = (int32_t*)ctx->d_stats + 30;
+    CclParams p;
+    p.conn = conn;
+    p.out_mode = CO_LEVEL;
+    p.threshold = true;
+    p.thr = level - 1;   // foreground: mask > level - 1, i.e. mask >= level
+    p.level = level;
+    return ccl_launch(ctx, B, p, s);
+}
+
+namespace {
+// Which u8 values occur in `mask` (bit v of present[v >> 5]) and the largest
+// marker value.  The LDS bitmap is read before it is set, so a value costs one
+// atomic per block, not one per pixel.
+__global__ __launch_bounds__(256) void levels_u8_kernel(const char* mask, int64_t pk, const char* marker,
+                                                        int64_t pm, int rows, int cols, uint32_t* present,
+                                                        uint32_t* mmax) {
+    __shared__ uint32_t lp[8];
+    __shared__ uint32_t lm;
+    if (threadIdx.x < 8) lp[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) lm = 0u;
+    __syncthreads();
+    uint32_t m = 0;
+    for (int y = blockIdx.x; y < rows; y += gridDim.x) {
+        const uint8_t* mr = (const uint8_t*)(mask + (int64_t)y * pk);
+        const uint8_t* ar = (const uint8_t*)(marker + (int64_t)y * pm);
+        for (int x = threadIdx.x; x < cols; x += blockDim.x) {
+            const uint32_t v = mr[x];
+            const uint32_t bit = 1u << (v & 31u);
+            if (!(lp[v >> 5] & bit)) atomicOr(&lp[v >> 5], bit);
+            m = max(m, (uint32_t)ar[x]);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&lm, m);
+    __syncthreads();
+    if (threadIdx.x < 8 && lp[threadIdx.x]) atomicOr(&present[threadIdx.x], lp[threadIdx.x]);
+    if (threadIdx.x == 0) atomicMax(mmax, lm);
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+// out = min(level, *mmax) everywhere: the reconstruction at the image's
+// smallest mask value, whose level set is the whole (connected) image.
+__global__ void level_floor_kernel(char* out, int64_t po, int rows, int cols, int level, const uint32_t* mmax) {
+    const uint8_t v = (uint8_t)min((uint32_t)level, *mmax);
+    for (int y = blockIdx.x; y < rows; y += gridDim.x) {
+        uint8_t* r = (uint8_t*)(out + (int64_t)y * po);
+        for (int x = threadIdx.x; x < cols; x += blockDim.x) r[x] = v;
+    }
+}
+}  // namespace
+
+nsx_status recon_levelset_u8(nsx_ctx* ctx, const View& marker, const View& mask, const View& out, int conn,
+                             int max_levels, bool* used, hipStream_t s) {
+    *used = false;
+    if (mask.dtype != NSX_U8 || max_levels <= 0 || (int64_t)mask.rows * mask.cols == 0) return NSX_OK;
+    // d_stats int32 words 40..47: value bitmap, 48: largest marker value
+    uint32_t* d = (uint32_t*)ctx->d_stats + 40;
+    const int grid = std::max(1, std::min(mask.rows, 4 * ctx->cu_count));
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, d, 9);
+    hipLaunchKernelGGL(levels_u8_kernel, dim3(grid), dim3(256), 0, s, mask.data, mask.pitch, marker.data,
+                       marker.pitch, mask.rows, mask.cols, d, d + 8);
+    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, d, 9 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+    const uint32_t* h = (const uint32_t*)ctx->h_pinned + 40;
+    std::vector<int> levels;   // descending
+    for (int v = 255; v >= 0; --v)
+        if ((h[v >> 5] >> (v & 31)) & 1u) levels.push_back(v);
+    if (levels.empty() || (int)levels.size() - 1 > max_levels) return NSX_OK;
+    hipLaunchKernelGGL(level_floor_kernel, dim3(grid), dim3(256), 0, s, out.data, out.pitch, out.rows, out.cols,
+                       levels.back(), d + 8);
+    for (size_t j = 0; j + 1 < levels.size(); ++j)
+        NSX_TRY(ccl_level_run(ctx, mask, marker, out, conn, levels[j], s));
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    *used = true;
+    return NSX_OK;
 }
 
 namespace {

@@ -70,6 +70,7 @@ struct nsx_ctx {
 
     // Tuning (nsx_ctx_set_tuning).
     int visit_sweeps = 5;
+    int levelset_levels = 3;
 
     // Propagation rounds the last reconstruction of a given size/mode needed.
     std::unordered_map<uint64_t, int> round_hint;
@@ -186,7 +187,8 @@ enum CclOut {
     CO_ROOT = 0,          // int32: root raster index, background -1
     CO_ROOT_FILTERED = 1, // int32: root index if kept by the area filter, else -1
     CO_CANON = 2,         // int32: 1..N in raster order of kept roots, background 0
-    CO_FLAT_U8 = 3        // u8: 255 if kept, else 0
+    CO_FLAT_U8 = 3,       // u8: 255 if kept, else 0
+    CO_LEVEL = 4          // u8: out = max(out, min(level, component's maximum weight)) on the foreground
 };
 
 struct CclParams {
@@ -197,6 +199,7 @@ struct CclParams {
     int min_size = 0;         // area filter (inclusive) ; filter on when use_area
     int max_size = 0;         // (exclusive)
     bool use_area = false;
+    int level = 0;            // CO_LEVEL
 };
 
 // Labels the image `in` (u8; or u8/i32/f32 with threshold) into `out`.
@@ -204,6 +207,20 @@ struct CclParams {
 // Enqueues work only.
 nsx_status ccl_run(nsx_ctx* ctx, const View& in, const View& out, const CclParams& p,
                    int32_t* ncomp_dev, hipStream_t s);
+
+// One level of the level-set reconstruction of a u8 image: components of
+// {mask >= level} (conn), each folded into `out` as min(level, its maximum of
+// `weight`).  Enqueues work only.
+nsx_status ccl_level_run(nsx_ctx* ctx, const View& mask, const View& weight, const View& out, int conn, int level,
+                         hipStream_t s);
+
+// Reconstruction by dilation of a u8 image through its level sets when the
+// mask holds at most max_levels + 1 distinct values: R = max over mask values
+// L of min(L, largest marker value in the component of {mask >= L}), one
+// labelling per level above the smallest.  Reads the value set back (one
+// synchronisation); *used = false (nothing written) when there are more levels.
+nsx_status recon_levelset_u8(nsx_ctx* ctx, const View& marker, const View& mask, const View& out, int conn,
+                             int max_levels, bool* used, hipStream_t s);
 
 // Several images labelled by one set of launches.  Image b's pixels occupy a
 // global index range starting at its pix_base (a multiple of 32768, see

@@ -17,6 +17,7 @@ NSX_ERR_NOT_CONVERGED = 6
 
 NSX_U8, NSX_I32, NSX_F32 = 0, 1, 2
 NSX_TUNE_VISIT_SWEEPS = 0
+NSX_TUNE_LEVELSET_LEVELS = 1
 NSX_MEM_DEVICE, NSX_MEM_HOST = 0, 1
 
 

@@ -85,8 +85,14 @@ nsx_status nsx_ctx_read_timing(nsx_ctx* ctx, double* ms, int64_t* launches, int 
  * NSX_TUNE_VISIT_SWEEPS: sweep pairs (column + row half-pass) a reconstruction
  * tile runs per visit before it re-queues itself unfinished, 1..64 (default
  * 5); small caps keep one slow tile from stretching a propagation round.
+ * NSX_TUNE_LEVELSET_LEVELS: a u8 reconstruction (nsx_imreconstruct,
+ * nsx_recon_label) whose mask holds at most this many distinct values above
+ * its smallest one is computed through the mask's level sets, one
+ * union-find labelling per level, instead of by propagation rounds; 0..255
+ * (default 3, 0 = never).  Long thin propagation paths (mazes, binary-like
+ * masks) then cost a few labellings instead of one round per tile crossing.
  * No reference counterpart. */
-enum { NSX_TUNE_VISIT_SWEEPS = 0, NSX_TUNE_NPARAMS = 1 };
+enum { NSX_TUNE_VISIT_SWEEPS = 0, NSX_TUNE_LEVELSET_LEVELS = 1, NSX_TUNE_NPARAMS = 2 };
 nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value);
 nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value);
 

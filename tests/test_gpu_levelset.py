@@ -1,0 +1,69 @@
+"""u8 reconstruction through the mask's level sets (few distinct mask values)
+against the oracle and against the propagation-round path on the same input."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def few_level_case(rng, shape, levels, p_marker=0.02):
+    mask = rng.choice(np.array(levels, np.uint8), size=shape).astype(np.uint8)
+    marker = np.zeros(shape, np.uint8)
+    sel = rng.random(shape) < p_marker
+    marker[sel] = rng.integers(0, 256, int(sel.sum()))
+    return marker, mask
+
+
+def contexts(nh):
+    ls = nh.Context(0)
+    ls.set_tuning(1, 3)
+    sweep = nh.Context(0)
+    sweep.set_tuning(1, 0)
+    return ls, sweep
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("levels,shape", [((0, 200), (97, 131)), ((20, 200), (256, 300)),
+                                          ((0, 90, 255), (200, 170)), ((10, 60, 61, 250), (64, 640)),
+                                          ((3, 7, 100, 101, 180), (150, 150))])
+def test_levelset_matches_oracle(nh, orc, gpu, conn, levels, shape):
+    rng = np.random.default_rng(sum(levels) + shape[0] * conn)
+    marker, mask = few_level_case(rng, shape, levels)
+    want = orc.imreconstruct(marker, mask, conn)
+    ls, sweep = contexts(nh)
+    got_ls = nh.imreconstruct(marker, mask, conn, ctx=ls)
+    got_sw = nh.imreconstruct(marker, mask, conn, ctx=sweep)
+    assert np.array_equal(got_ls, want)
+    assert np.array_equal(got_sw, want)
+    ls.close()
+    sweep.close()
+
+
+def test_levelset_marker_above_mask_and_uniform(nh, orc, gpu):
+    ls, _ = contexts(nh)
+    rng = np.random.default_rng(7)
+    mask = np.where(rng.random((90, 77)) < 0.55, 180, 40).astype(np.uint8)
+    marker = rng.integers(0, 256, mask.shape).astype(np.uint8)   # often above the mask
+    assert np.array_equal(nh.imreconstruct(marker, mask, 4, ctx=ls), orc.imreconstruct(marker, mask, 4))
+    flat = np.full((33, 70), 77, np.uint8)                          # one level: floor only
+    mk = np.zeros_like(flat)
+    mk[5, 5] = 30
+    assert np.array_equal(nh.imreconstruct(mk, flat, 8, ctx=ls), orc.imreconstruct(mk, flat, 8))
+    ls.close()
+
+
+@pytest.mark.parametrize("shape", [(1030, 1500), (611, 389)])
+def test_maze_recon_label_closed_form(nh, orc, gpu, shape):
+    """Maze: reconstruction from the entrance seed is 200 on the corridors
+    (one perfect maze, all reachable), 20 elsewhere; > 100 then 4-connected
+    labelling gives exactly one component."""
+    import torch
+    mask, marker = nh.gen_maze_u8(shape[0], shape[1], 0x5EED0003)
+    corridors = mask == 200
+    r, lab, n = nh.recon_label(torch.from_numpy(marker).cuda(), torch.from_numpy(mask).cuda(), 4, 100.0, 4)
+    r = r.cpu().numpy()
+    lab = lab.cpu().numpy()
+    assert n == 1
+    assert np.array_equal(r, np.where(corridors, 200, 20).astype(np.uint8))
+    assert np.array_equal(lab, corridors.astype(np.int32))
+    assert np.array_equal(r, orc.imreconstruct(marker, mask, 4))
