@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--tiles-u8", type=int, default=256)
     p.add_argument("--tiles-f32", type=int, default=32)
     p.add_argument("--check-tiles", type=int, default=6, help="batch tiles also checked against the oracle")
+    p.add_argument("--maze-size", type=int, default=16384, help="maze: side of the maze image")
+    p.add_argument("--maze-cpu-size", type=int, default=4096, help="maze: side of the CPU baseline sample")
     p.add_argument("--requests", type=int, default=200, help="serve: requests in the trace")
     p.add_argument("--slots", type=int, default=4, help="serve: worker slots of the server")
     p.add_argument("--load", type=float, default=1.0,
@@ -380,6 +382,113 @@ def cpu_baseline_serve(reqs, trace, digests, max_side):
     return cpu, exact, n
 
 
+def bench_maze(args):
+    """16384x16384 u8 maze (6-px cells, 3-px corridors at 200, walls 20, seed
+    0x5EED0003), marker = 5x5 block of 200 at the entrance.  One step =
+    imreconstruct (4-conn) -> (> 100) -> bwlabel (4-conn, canonical) through
+    nsx_recon_label on device-resident buffers.  Checked against the closed
+    form (200 on the corridors, 20 elsewhere; exactly one label, equal to the
+    corridor set)."""
+    import numpy as np
+    import torch
+
+    import nscale_hip as nh
+    n = args.maze_size
+    t0 = time.perf_counter()
+    mask, marker = nh.gen_maze_u8(n, n, 0x5EED0003)
+    log(f"generated {n}x{n} maze in {time.perf_counter() - t0:.1f} s")
+    dev = torch.device("cuda:0")
+    dm, dk = torch.from_numpy(marker).to(dev), torch.from_numpy(mask).to(dev)
+    dr = torch.empty_like(dk)
+    dl = torch.empty((n, n), dtype=torch.int32, device=dev)
+    ctx = nh.default_context(0)
+
+    def step():
+        return nh.recon_label(dm, dk, 4, 100.0, 4, recon_out=dr, labels_out=dl, ctx=ctx)[2]
+
+    ncomp = step()
+    torch.cuda.synchronize()
+    corridors = torch.from_numpy(mask == 200).to(dev)
+    exact = bool(ncomp == 1 and torch.equal(dr, torch.where(corridors, 200, 20).to(torch.uint8))
+                 and torch.equal(dl, corridors.to(torch.int32)))
+    del corridors
+    log(f"closed form: {exact} (components {ncomp})")
+    if not exact:
+        sys.exit(1)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        step()
+        lat.append((time.perf_counter() - s0) * 1e3)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_timing(True)
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    kt = ctx.read_timing()
+    ctx.set_timing(False)
+    px = n * n
+    # Algorithmic bytes per pixel of each launch kind over one step: the level
+    # labelling of {mask >= 200} reads mask + marker (local) and mask + the
+    # reconstruction, writing the reconstruction (final); the output labelling
+    # reads the reconstruction (local) and writes int32 labels (final).
+    alg = {"ccl_local": (2 + 1) * px, "ccl_final": (3 + 5) * px}
+    kind = max(kt, key=lambda k: kt[k][0])
+    roof = None
+    if kind in alg and kt[kind][1]:
+        ms = kt[kind][0] / kt[kind][1]
+        achieved = alg[kind] / (kt[kind][1] / args.steps) / (ms / 1e3) / 1e9
+        roof = {"bound": "hbm", "kernel": kind, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "alg_bytes_per_launch": int(alg[kind] / (kt[kind][1] / args.steps)), "avg_launch_us": round(ms * 1e3, 3),
+                "launches_per_step": round(kt[kind][1] / args.steps, 2),
+                "per_kind_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()}}
+    cpu = None if args.no_cpu else cpu_baseline_maze(nh, args.maze_cpu_size, ctx)
+    value = args.steps * px / elapsed / 1e6
+    out = {
+        "metric": "Mpix/s maze reconstruct+CCL @16384^2 u8 bit-exact",
+        "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4), "p50_ms": round(statistics.median(lat), 4),
+        "p99_ms": round(pct(lat, 0.99), 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8", "data": "synthetic", "bitexact": exact,
+        "config": {"workload": f"{n}x{n} u8 maze (6-px cells, 3-px corridors 200 / walls 20), 5x5 seed: recon4 "
+                               "-> >100 -> bwlabel4", "pixels": px, "components": ncomp,
+                   "step_alg_bytes": 8 * px,
+                   "step_hbm_frac": round(8 * px / (elapsed / args.steps) / (HBM_PEAK_GBS * 1e9), 5)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline_maze(nh, n, ctx):
+    """The oracle's imreconstruct + bwlabel2 (single thread) on an n x n maze of
+    the same construction (a bounded sample of the 16384^2 workload); the GPU
+    result on the same sample is compared with it.  Returns cpu_baseline."""
+    import numpy as np
+    import torch
+
+    from oracle import oracle as orc
+    mask, marker = nh.gen_maze_u8(n, n, 0x5EED0003)
+    t0 = time.perf_counter()
+    wr = orc.imreconstruct(marker, mask, 4)
+    wl, wn = orc.bwlabel2(np.where(wr > 100, 255, 0).astype(np.uint8), 4, relab=True)
+    dt = time.perf_counter() - t0
+    r, lab, gn = nh.recon_label(torch.from_numpy(marker).cuda(), torch.from_numpy(mask).cuda(), 4, 100.0, 4, ctx=ctx)
+    same = bool(np.array_equal(r.cpu().numpy(), wr) and np.array_equal(lab.cpu().numpy(), wl) and gn == wn)
+    if not same:
+        log("maze sample differs from the oracle")
+        sys.exit(1)
+    return {"value": round(n * n / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": f"one {n}x{n} maze of the same construction (recon4 + bwlabel4), single thread, {dt:.1f} s; "
+                      "GPU output on it identical", "host_cpus": os.cpu_count()}
+
+
 TILE_PX = 62 * 62   # interior pixels of one reconstruction tile visit
 
 
@@ -450,6 +559,8 @@ def main():
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "maze":
+        return bench_maze(args)
     n = args.size
     seed = 0x5EED0001
     mask = nh.gen_disks_u8(n, n, args.disks, seed)

@@ -317,8 +317,15 @@ __global__ __launch_bounds__(256) void ccl_area_kernel(CclArgs A) {
     for (int e = threadIdx.x; e < n; e += blockDim.x) {
         const uint2 en = slots[e];
         const int32_t r = g_find(A.lab, (int32_t)en.x);
-        if (A.wmax) atomicMax(&A.area[r], (int)en.y);
-        else atomicAdd(&A.area[r], (int)en.y);
+        // the output kernel finds the root of this same local root again
+        if (r != (int32_t)en.x) A.lab[en.x] = r;
+        if (A.wmax) {
+            // a large component has one slot per tile it crosses: only a slot
+            // that can raise the maximum pays for the atomic on its root
+            if ((int)en.y > A.area[r]) atomicMax(&A.area[r], (int)en.y);
+        } else {
+            atomicAdd(&A.area[r], (int)en.y);
+        }
     }
 }
 
@@ -637,20 +644,49 @@ __global__ __launch_bounds__(256) void levels_u8_kernel(const char* mask, int64_
     if (threadIdx.x < 8) lp[threadIdx.x] = 0u;
     if (threadIdx.x == 0) lm = 0u;
     __syncthreads();
+    // per-thread value set: quarter q of the byte range in q[...] (64 bits each)
+    uint64_t q0 = 0, q1 = 0, q2 = 0, q3 = 0;
     uint32_t m = 0;
+    auto add = [&](uint32_t v) {
+        const uint64_t b = 1ull << (v & 63u);
+        const uint32_t q = v >> 6;
+        q0 |= q == 0 ? b : 0ull;
+        q1 |= q == 1 ? b : 0ull;
+        q2 |= q == 2 ? b : 0ull;
+        q3 |= q == 3 ? b : 0ull;
+    };
     for (int y = blockIdx.x; y < rows; y += gridDim.x) {
         const uint8_t* mr = (const uint8_t*)(mask + (int64_t)y * pk);
         const uint8_t* ar = (const uint8_t*)(marker + (int64_t)y * pm);
-        for (int x = threadIdx.x; x < cols; x += blockDim.x) {
-            const uint32_t v = mr[x];
-            const uint32_t bit = 1u << (v & 31u);
-            if (!(lp[v >> 5] & bit)) atomicOr(&lp[v >> 5], bit);
-            m = max(m, (uint32_t)ar[x]);
+        for (int x0 = 4 * (int)threadIdx.x; x0 < cols; x0 += 4 * (int)blockDim.x) {
+            uint32_t v[4], a[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int x = min(x0 + k, cols - 1);   // repeats the last pixel: same value set
+                v[k] = mr[x];
+                a[k] = ar[x];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                add(v[k]);
+                m = max(m, a[k]);
+            }
         }
     }
+    uint32_t w[8] = {(uint32_t)q0, (uint32_t)(q0 >> 32), (uint32_t)q1, (uint32_t)(q1 >> 32),
+                     (uint32_t)q2, (uint32_t)(q2 >> 32), (uint32_t)q3, (uint32_t)(q3 >> 32)};
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(&lm, m);
+    for (int o = 32; o > 0; o >>= 1) {
+        m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+#pragma unroll
+        for (int k = 0; k < 8; ++k) w[k] |= (uint32_t)__shfl_xor((int)w[k], o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&lm, m);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (w[k]) atomicOr(&lp[k], w[k]);
+    }
     __syncthreads();
     if (threadIdx.x < 8 && lp[threadIdx.x]) atomicOr(&present[threadIdx.x], lp[threadIdx.x]);
     if (threadIdx.x == 0) atomicMax(mmax, lm);
