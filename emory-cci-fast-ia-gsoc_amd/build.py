@@ -61,6 +61,19 @@ def build_library(jobs: int = 6) -> Path:
     return LIB_PATH
 
 
+DROPIN_CHECK = REPO / "tests" / "cpp" / "dropin_check"
+
+
+def build_dropin_check() -> Path:
+    """Compile tests/cpp/dropin_check (the C++ drop-in header's check program)
+    against the in-tree library; found at run time through an $ORIGIN rpath."""
+    src = REPO / "tests" / "cpp" / "dropin_check.cpp"
+    rpath = "$ORIGIN/" + os.path.relpath(LIB_PATH.parent, DROPIN_CHECK.parent)
+    _run(["g++", "-std=c++17", "-O1", "-Wall", f"-I{REPO / 'include'}", f"-I{src.parent}", str(src),
+          f"-L{LIB_PATH.parent}", "-lnscale_hip", f"-Wl,-rpath,{rpath}", "-o", str(DROPIN_CHECK)])
+    return DROPIN_CHECK
+
+
 def build_oracle() -> None:
     """Build oracle/liboracle.so (and oracle/_ref when the reference is present)."""
     if shutil.which("make") is None:
@@ -70,4 +83,5 @@ def build_oracle() -> None:
 
 if __name__ == "__main__":
     print(build_library())
+    print(build_dropin_check())
     build_oracle()
