@@ -119,6 +119,16 @@ __device__ __forceinline__ uint32_t lds_find(uint32_t* par, uint32_t x) {
     return x;
 }
 
+// Root of x without modifying the forest.
+__device__ __forceinline__ uint32_t lds_root(const uint32_t* par, uint32_t x) {
+    uint32_t p = par[x];
+    while (p != x) {
+        x = p;
+        p = par[x];
+    }
+    return x;
+}
+
 __device__ __forceinline__ void lds_union(uint32_t* par, uint32_t a, uint32_t b) {
     while (true) {
         a = lds_find(par, a);
@@ -201,11 +211,18 @@ __device__ __forceinline__ void local_label(const CclArgs& A, const CclImg& D, i
         }
     }
     __syncthreads();
+    // Flatten: every pixel's root is found with reads only, and written after a
+    // barrier.  (A halving write made while other threads store their roots can
+    // land after one of them and leave a pixel pointing at a non-root.)
+    uint32_t root[CPT];
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int i = (w + 4 * j) * CW + lx;
-        if (par[i] != BGL) par[i] = lds_find(par, i);
+        root[j] = par[i] != BGL ? lds_root(par, i) : BGL;
     }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) par[(w + 4 * j) * CW + lx] = root[j];
     __syncthreads();
 }
 

@@ -196,3 +196,19 @@ def test_golden_fixtures(nh, gpu):
         assert np.array_equal(nh.bwlabel2(g["bin_mask"], conn, True), g[f"label_canon_c{conn}"])
         got, _ = nh.bwareaopen2(g["bin_mask"], False, False, 3, 12, conn)
         assert np.array_equal(got, g[f"areaopen_c{conn}"])
+
+
+def test_bwlabel_large_repeated(nh, orc, gpu):
+    """Labelling a 4096x4096 image several times gives the oracle's labels
+    every time (a race in the tile-local flatten once left single pixels
+    pointing at a non-root, about once per few hundred megapixels)."""
+    import torch
+    mask = nh.gen_disks_u8(4096, 4096, 3000, 0x5EED0101)
+    binimg = np.where(mask > 128, 255, 0).astype(np.uint8)
+    want, wn = orc.bwlabel2(binimg, 8, relab=True)
+    d = torch.from_numpy(binimg).cuda()
+    out = torch.empty(d.shape, dtype=torch.int32, device="cuda")
+    for rep in range(6):
+        out.fill_(-7)
+        _, n = nh.bwlabel2(d, 8, True, out=out)
+        assert n == wn and np.array_equal(out.cpu().numpy(), want), f"repetition {rep}"
