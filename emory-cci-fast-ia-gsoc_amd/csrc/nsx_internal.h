@@ -69,7 +69,7 @@ struct nsx_ctx {
     int cu_count = 256;
 
     // Tuning (nsx_ctx_set_tuning).
-    int visit_sweeps = 5;
+    int visit_sweeps = 4;
     int levelset_levels = 3;
 
     // Propagation rounds the last reconstruction of a given size/mode needed.

@@ -39,6 +39,7 @@
 #include <cfloat>
 #include <cmath>
 #include <algorithm>
+#include <type_traits>
 
 namespace nsx {
 namespace {
@@ -183,6 +184,9 @@ __device__ __forceinline__ uint32_t from_right(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xF, 0xF, true);  // wave_shl:1
 }
 
+// Median of three.  Kept opaque to the scheduler: written as min/max (which
+// the compiler also turns into v_med3_u32) it interleaves the sweep steps
+// until the tile no longer fits the 168 VGPRs of three waves per SIMD.
 __device__ __forceinline__ uint32_t med3u(uint32_t a, uint32_t b, uint32_t c) {
     uint32_t r;
     asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
@@ -205,21 +209,57 @@ __device__ __forceinline__ uint32_t mask_at(const LT* mb, int k) {
     return (uint32_t)(B ? mb[lane * LDS_PITCH + k] : mb[k * LDS_PITCH + lane]);
 }
 
-// Forward sweep over registers 1..TS-1 then backward over TS-2..0.
+// One sweep step, r = med3(a, b, c) with b the register's old value, plus the
+// change bookkeeping in the same asm statement (so each term is taken while the
+// value is in a register and the scheduler cannot stretch live ranges):
+//   sum_old: s += b (old value)      sum_new: s += r (new value)
+//   chg:     s |= r ^ b
+__device__ __forceinline__ uint32_t med3_sum_old(uint32_t a, uint32_t b, uint32_t c, uint32_t& s) {
+    uint32_t r;
+    asm("v_med3_u32 %0, %2, %3, %4\n\tv_add_u32 %1, %1, %3" : "=&v"(r), "+v"(s) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ uint32_t med3_sum_new(uint32_t a, uint32_t b, uint32_t c, uint32_t& s) {
+    uint32_t r;
+    asm("v_med3_u32 %0, %2, %3, %4\n\tv_add_u32 %1, %1, %0" : "=&v"(r), "+v"(s) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ uint32_t med3_chg(uint32_t a, uint32_t b, uint32_t c, uint32_t& s) {
+    uint32_t r, t;
+    asm("v_med3_u32 %0, %3, %4, %5\n\tv_xor_b32 %2, %0, %4\n\tv_or_b32 %1, %1, %2"
+        : "=&v"(r), "+v"(s), "=&v"(t) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// Forward sweep over registers 1..TS-1 then backward over TS-2..0.  Returns
+// whether any of the lane's registers changed.  Values only rise, so for u8
+// keys a lane changed iff the sum of its registers rose (old values summed on
+// the forward sweep, new ones on the backward sweep; 64 x 255 cannot wrap);
+// 32-bit keys OR the differences instead.  The lane's mask column (row in
+// orientation B) is read from LDS once, before both sweeps.
 template <int CONN, bool B, typename LT>
-__device__ __forceinline__ void sweep_pair(uint32_t (&c)[TS], const LT* mb) {
+__device__ __forceinline__ bool sweep_pair(uint32_t (&c)[TS], const LT* mb) {
+    uint32_t m[TS];
+#pragma unroll
+    for (int k = 0; k < TS; ++k) m[k] = mask_at<B>(mb, k);
+#pragma unroll
+    for (int q = 0; q < TS / LOAD_CH; ++q) arrived(*reinterpret_cast<uint32_t(*)[LOAD_CH]>(&m[q * LOAD_CH]));
+    constexpr bool kSum = sizeof(LT) == 1;
+    uint32_t before = kSum ? c[0] : 0u, after = 0;
     uint32_t p = c[0];
 #pragma unroll
     for (int k = 1; k < TS; ++k) {
-        c[k] = med3u(nbr<CONN>(p), c[k], mask_at<B>(mb, k));
+        c[k] = kSum ? med3_sum_old(nbr<CONN>(p), c[k], m[k], before) : med3_chg(nbr<CONN>(p), c[k], m[k], before);
         p = c[k];
     }
+    if (kSum) after += c[TS - 1];
     p = c[TS - 1];
 #pragma unroll
     for (int k = TS - 2; k >= 0; --k) {
-        c[k] = med3u(nbr<CONN>(p), c[k], mask_at<B>(mb, k));
+        c[k] = kSum ? med3_sum_new(nbr<CONN>(p), c[k], m[k], after) : med3_chg(nbr<CONN>(p), c[k], m[k], after);
         p = c[k];
     }
+    return kSum ? after != before : (before | after) != 0u;
 }
 
 // Register tile transpose through LDS: lane l, register k <-> lane k, register l.
@@ -234,27 +274,10 @@ __device__ __forceinline__ void transpose(uint32_t (&c)[TS], LT* lds) {
     __syncthreads();
 }
 
-template <typename T> struct SumT { typedef uint64_t type; };
-template <> struct SumT<uint8_t> { typedef uint32_t type; };
 // LDS element of the mask / transpose buffers: every key of a u8 image (value,
 // inverted value, 0/1 reached flag, frame) fits in a byte.
 template <typename T> struct LdsT { typedef uint32_t type; };
 template <> struct LdsT<uint8_t> { typedef uint8_t type; };
-
-template <typename S>
-__device__ __forceinline__ S wave_sum(S v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-template <typename S>
-__device__ __forceinline__ S tile_sum(const uint32_t (&c)[TS]) {
-    S s = 0;
-#pragma unroll
-    for (int k = 0; k < TS; ++k) s += (S)c[k];
-    return wave_sum<S>(s);
-}
 
 // ---- propagation rounds over a device tile list ---------------------------------------
 // Round 0 visits every tile.  A visit that raises a halo pixel appends the
@@ -338,7 +361,6 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
                                              uint32_t* hs) {
     using LT = typename LdsT<T>::type;
     using C = Codec<T, MODE>;
-    using S = typename SumT<T>::type;
     const int lane = threadIdx.x;
     const int ty = lt / D.ntx, tx = lt - ty * D.ntx;
     const int x0 = tx * TI - 1, y0 = ty * TI - 1;
@@ -414,29 +436,27 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
 
     // Half-passes alternate between the column sweeps (orientation A: lane =
     // column) and the row sweeps (after a transpose, orientation B: lane = row).
-    // Values only rise, so an unchanged tile sum means an unchanged tile.
+    // A half-pass changed the tile iff some lane's sweep saw a register rise.
     // 4-connectivity: a half-pass is an exact 1-D reconstruction of every
     // column (row) and so idempotent; one quiet half-pass after the other
     // orientation ran proves the fixed point.  8-connectivity: the diagonal
     // terms break idempotence, so two consecutive quiet half-passes are needed.
-    const S tot_in = tile_sum<S>(c);
-    S tot = tot_in;
     bool converged = false;
     bool orientB = false;
+    bool changed = false;
     int quiet = 0;
     int half = 0;
-    auto settled = [&](const S t1) {
-        quiet = (t1 == tot) ? quiet + 1 : 0;
-        tot = t1;
+    auto settled = [&](bool lane_moved) {
+        const bool moved = __ballot(lane_moved) != 0ull;
+        changed = changed || moved;
+        quiet = moved ? 0 : quiet + 1;
         return CONN == 4 ? (quiet >= 1 && half >= 1) : (quiet >= 2);
     };
     for (int it = 0; it < A.visit_sweeps; ++it) {
-        sweep_pair<CONN, false>(c, mb);               // columns (orientation A)
-        if (settled(tile_sum<S>(c))) { converged = true; break; }
+        if (settled(sweep_pair<CONN, false>(c, mb))) { converged = true; break; }   // columns (orientation A)
         ++half;
         transpose(c, tb);
-        sweep_pair<CONN, true>(c, mb);                // rows (orientation B)
-        if (settled(tile_sum<S>(c))) { converged = true; orientB = true; break; }
+        if (settled(sweep_pair<CONN, true>(c, mb))) { converged = true; orientB = true; break; }   // rows
         ++half;
         transpose(c, tb);
     }
@@ -454,7 +474,7 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
         transpose(c, tb);
     }
 
-    if (FIRST || tot != tot_in) {
+    if (FIRST || changed) {
         if (lane >= 1 && lane <= TI && gx < D.cols) {
 #pragma unroll
             for (int k = 1; k <= TI; ++k) {

@@ -210,5 +210,5 @@ def test_bwlabel_large_repeated(nh, orc, gpu):
     out = torch.empty(d.shape, dtype=torch.int32, device="cuda")
     for rep in range(6):
         out.fill_(-7)
-        _, n = nh.bwlabel2(d, 8, True, out=out)
+        _, n = nh.bwlabel2(d, 8, True, out=out, return_count=True)
         assert n == wn and np.array_equal(out.cpu().numpy(), want), f"repetition {rep}"
