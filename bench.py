@@ -44,7 +44,7 @@ def parse():
                    help="skip the CPU baseline leg (the oracle run that also checks the GPU results)")
     p.add_argument("--cpu-runs", type=int, default=10, help="oracle chain runs in the CPU sample")
     p.add_argument("--workload", default="chain4096",
-                   choices=["chain4096", "batch", "maze", "serve"],
+                   choices=["chain4096", "batch", "maze", "serve", "copy"],
                    help="chain4096: the headline 4096^2 chain (default); batch: 256 u8 + 32 f32 uneven tiles "
                         "batched vs per-tile; maze: 16384^2 maze stress; serve: continuous-batching server")
     p.add_argument("--tiles-u8", type=int, default=256)
@@ -466,6 +466,34 @@ def bench_maze(args):
     print(json.dumps(out))
 
 
+def bench_copy(args):
+    """Calibration workload for the HBM traffic counters: one elementwise
+    kernel (int32 negation, torch) that reads and writes exactly 256 MiB per
+    launch.  rocprofv3's FETCH_SIZE / WRITE_SIZE for this kernel, divided into
+    268435456 bytes, give the factor that converts those counters into bytes."""
+    import torch
+    n = 64 << 20
+    x = torch.ones(n, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    gbs = 2 * 4 * n / dt / 1e9
+    print(json.dumps({"metric": "GB/s device copy (int32 negation, 256 MiB read + 256 MiB written per launch)",
+                      "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 4), "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+                      "config": {"workload": "copy calibration", "bytes_read": 4 * n, "bytes_written": 4 * n},
+                      "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                      "cpu_baseline": None}))
+
+
 def cpu_baseline_maze(nh, n, ctx):
     """The oracle's imreconstruct + bwlabel2 (single thread) on an n x n maze of
     the same construction (a bounded sample of the 16384^2 workload); the GPU
@@ -561,6 +589,8 @@ def main():
         return bench_serve(args)
     if args.workload == "maze":
         return bench_maze(args)
+    if args.workload == "copy":
+        return bench_copy(args)
     n = args.size
     seed = 0x5EED0001
     mask = nh.gen_disks_u8(n, n, args.disks, seed)

@@ -375,18 +375,6 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
         mk = in ? C::mkey(av, mv) : F;
         return in ? key : F;
     };
-    // Left / right halo columns (lane = row): their keys before the sweeps.
-    // They are loaded with the first row chunk and decoded when it arrives.
-    const int hy = min(max(y0 + lane, 0), D.rows - 1);
-    const int hxl = min(max(x0, 0), D.cols - 1), hxr = min(max(x0 + TS - 1, 0), D.cols - 1);
-    uint32_t hraw[6];
-    hraw[0] = bits<T>(ld<T>(D.mask, D.pk, hy, hxl));
-    hraw[1] = bits<T>(ld<T>(D.mask, D.pk, hy, hxr));
-    hraw[2] = needA ? bits<T>(ld<T>(D.marker, D.pm, hy, hxl)) : 0u;
-    hraw[3] = needA ? bits<T>(ld<T>(D.marker, D.pm, hy, hxr)) : 0u;
-    hraw[4] = FIRST ? 0u : bits<T>(ld<T>(D.state, D.ps, hy, hxl));
-    hraw[5] = FIRST ? 0u : bits<T>(ld<T>(D.state, D.ps, hy, hxr));
-
     // The tile's rows: chunk j+1 is in flight while chunk j is decoded.
     uint32_t c[TS];
     const int gx = x0 + lane;
@@ -412,15 +400,6 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
         arrived(rm[b]);
         if (needA) arrived(ra[b]);
         if (!FIRST) arrived(rs[b]);
-        if (ch == 0) {
-            asm volatile("" : "+v"(hraw[0]), "+v"(hraw[1]), "+v"(hraw[2]), "+v"(hraw[3]), "+v"(hraw[4]),
-                              "+v"(hraw[5]));
-            const bool rin = (unsigned)(y0 + lane) < (unsigned)D.rows;
-            uint32_t dummy;
-            hs[2 * TS + lane] = key_of(rin && (unsigned)x0 < (unsigned)D.cols, hraw[0], hraw[2], hraw[4], dummy);
-            hs[3 * TS + lane] =
-                key_of(rin && (unsigned)(x0 + TS - 1) < (unsigned)D.cols, hraw[1], hraw[3], hraw[5], dummy);
-        }
 #pragma unroll
         for (int j = 0; j < LOAD_CH; ++j) {
             const int k = ch * LOAD_CH + j;
@@ -432,6 +411,13 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
     }
     hs[lane] = c[0];
     hs[TS + lane] = c[TS - 1];
+    // left / right halo columns are lanes 0 and TS-1: they write them out so
+    // that lane l later finds row l's two values (no column loads from HBM)
+    if (lane == 0 || lane == TS - 1) {
+        const int base = lane == 0 ? 2 * TS : 3 * TS;
+#pragma unroll
+        for (int k = 0; k < TS; ++k) hs[base + k] = c[k];
+    }
     __syncthreads();
 
     // Half-passes alternate between the column sweeps (orientation A: lane =
