@@ -519,8 +519,17 @@ def cpu_baseline_maze(nh, n, ctx):
 
 TILE_PX = 62 * 62   # interior pixels of one reconstruction tile visit
 
+# HBM bytes per launch of each kernel kind of the default 4096x4096 chain,
+# measured by two rocprofv3 runs of this bench (--steps 3 --warmup 1 --no-cpu,
+# one with --pmc FETCH_SIZE, one with --pmc WRITE_SIZE), averaged over every
+# launch of the kind.  FETCH_SIZE is scaled by 2.0 and WRITE_SIZE by 1.0: the
+# factors that make `--workload copy` (256 MiB read + 256 MiB written per
+# launch) read true.  Commands and raw figures: profiles/traffic_chain4096.md.
+TRAFFIC_CHAIN4096 = {"recon_round": 18464218, "recon_first": 104523601, "ccl_local": 37200903,
+                     "ccl_final": 101865173, "ccl_merge": 65550201}
 
-def roofline(kt: dict, px: int, visits: tuple, steps: int) -> dict:
+
+def roofline(kt: dict, px: int, visits: tuple, steps: int, traffic: dict | None = None) -> dict:
     """HBM roofline of the kernel kind with the most time per step.
 
     Algorithmic bytes per pixel (compulsory I/O of the operation a launch
@@ -545,7 +554,7 @@ def roofline(kt: dict, px: int, visits: tuple, steps: int) -> dict:
     avg_launch_s = ms_step / launches_step / 1e3 if launches_step else float("nan")
     achieved = alg_launch / avg_launch_s / 1e9 if launches_step else 0.0
     return {"bound": "hbm", "kernel": kind, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": (traffic or {}).get(kind),
             "alg_bytes_per_launch": int(alg_launch), "avg_launch_us": round(avg_launch_s * 1e6, 3),
             "launches_per_step": round(launches_step, 2),
             "per_kind_ms_per_step": {k: round(v[0] / steps, 4) for k, v in kt.items()}}
@@ -637,7 +646,9 @@ def main():
     log("kernel time per step (ms): " + ", ".join(f"{k}={v[0] / args.steps:.4f} ({v[1] / args.steps:.1f} launches)"
                                                   for k, v in kt.items()))
     log(f"recon stage {stats}; fill stage {fstats}")
-    roof = roofline(kt, n * n, (stats["tile_visits"], fstats["tile_visits"]), args.steps)
+    default_chain = n == 4096 and args.disks == 3000
+    roof = roofline(kt, n * n, (stats["tile_visits"], fstats["tile_visits"]), args.steps,
+                    TRAFFIC_CHAIN4096 if default_chain else None)
     cpu, exact = (None, None) if args.no_cpu else cpu_baseline(marker, mask, args.cpu_runs, first)
     if exact is False:
         sys.exit(1)
