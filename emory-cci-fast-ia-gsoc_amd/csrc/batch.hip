@@ -257,6 +257,7 @@ nsx_status nsx_batch_finalize(nsx_batch* b, int use_graph) {
         if (st == NSX_OK) st = c->ws(WS_CCL_BITS, sizeof(int32_t) * (size_t)max_ct, &p);
         if (st == NSX_OK) st = c->ws(WS_CCL_SCAN, sizeof(uint32_t) * (size_t)nw, &p);
         if (st == NSX_OK) st = c->ws(WS_CCL_MISC, sizeof(uint32_t) * (size_t)(nw + nb + 1), &p);
+        if (st == NSX_OK) st = c->ws(WS_CCL_LROOT, sizeof(uint16_t) * (size_t)max_pix, &p);
         if (st == NSX_OK) st = c->ws(WS_RECON_STAMP, sizeof(uint32_t) * (size_t)max_rt_even, &p);
         if (st == NSX_OK) st = c->ws(WS_RECON_LIST, 2 * sizeof(int32_t) * (size_t)max_rt_even, &p);
         if (st == NSX_OK && max_rt_odd) st = c->ws(WS_FILL_STAMP, sizeof(uint32_t) * (size_t)max_rt_odd, &p);

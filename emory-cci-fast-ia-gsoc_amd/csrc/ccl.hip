@@ -62,6 +62,7 @@ struct CclArgs {
     int nimg, ntiles;
     int64_t thr_i; float thr_f;
     int32_t* lab;              // sparse: tile-edge pixels + local roots
+    uint16_t* lroot;           // every pixel: tile-local root index, 0xFFFF background
     int32_t* area;             // sparse: roots
     uint2* entries;            // tile t's components at [t*CN, t*CN + tcount[t])
     int32_t* tcount;           // components per tile
@@ -272,8 +273,11 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
         const int ly = w + 4 * j;
         const int i = ly * CW + lx;
         const uint32_t r = par[i];
-        if (r == BGL) continue;
         const int32_t g = D.pix_base + (y0 + ly) * D.cols + gx;
+        // the output kernel reads each pixel's local root back (2 B) instead of
+        // labelling the tile again
+        if (gx < D.cols && y0 + ly < D.rows) A.lroot[g] = r == BGL ? (uint16_t)0xFFFF : (uint16_t)r;
+        if (r == BGL) continue;
         if (r == (uint32_t)i) {
             const uint32_t k = atomicAdd(&nroots, 1u);
             slots[k] = make_uint2((uint32_t)g, cnt[i]);
@@ -443,23 +447,29 @@ __device__ __forceinline__ int32_t rank1(const CclArgs& A, const CclImg& D, int3
 
 template <int CONN, typename InT, bool THR>
 __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
-    __shared__ uint32_t val[CN];
-    __shared__ uint32_t par[CN];
     __shared__ int32_t outv[CN];
     const int tid = threadIdx.x;
     CclImg D;
     int tx, ty;
     tile_of(A, blockIdx.x, D, tx, ty);
     const int x0 = tx * CW, y0 = ty * CH;
-    local_label<CONN, InT, THR>(A, D, x0, y0, val, par);
     const int lx = tid & 63, w = tid >> 6;
     const int gx = x0 + lx;
+    // each pixel's tile-local root, as the local kernel left it
+    uint32_t par[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int gy = y0 + w + 4 * j;
+        const bool in = gx < D.cols && gy < D.rows;
+        const uint16_t r = in ? A.lroot[D.pix_base + gy * D.cols + gx] : (uint16_t)0xFFFF;
+        par[j] = r == 0xFFFF ? BGL : (uint32_t)r;
+    }
     // each local root resolves its global root once
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int ly = w + 4 * j;
         const int i = ly * CW + lx;
-        if (par[i] != (uint32_t)i) continue;
+        if (par[j] != (uint32_t)i) continue;
         const int32_t r = g_find(A.lab, D.pix_base + (y0 + ly) * D.cols + gx);
         int32_t o;
         switch (A.out_mode) {
@@ -478,7 +488,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
         const int ly = w + 4 * j;
         const int gy = y0 + ly;
         if (gx >= D.cols || gy >= D.rows) continue;
-        const uint32_t r = par[ly * CW + lx];
+        const uint32_t r = par[j];
         char* row = D.out + (int64_t)gy * D.pout;
         if (A.out_mode == CO_LEVEL) {
             // u8 reconstruction accumulated over levels: foreground only
@@ -596,6 +606,9 @@ nsx_status ccl_launch(nsx_ctx* ctx, const CclBatch& B, const CclParams& p, hipSt
     NSX_TRY(ctx->ws(B.ws_base + 3, sizeof(int32_t) * (size_t)B.ntiles, &tcnt));
     NSX_TRY(ctx->ws(B.ws_base + 4, sizeof(uint32_t) * (size_t)A.nwords, &bits));
     NSX_TRY(ctx->ws(B.ws_base + 5, sizeof(uint32_t) * (size_t)(A.nwords + A.nblocks + 1), &scan));
+    void* lroot;
+    NSX_TRY(ctx->ws(WS_CCL_LROOT, sizeof(uint16_t) * (size_t)B.pix_total, &lroot));
+    A.lroot = (uint16_t*)lroot;
     A.lab = (int32_t*)lab; A.area = (int32_t*)area; A.entries = (uint2*)ent; A.tcount = (int32_t*)tcnt;
     A.bits = (uint32_t*)bits;
     A.wordpre = (uint32_t*)scan;

@@ -118,6 +118,7 @@ enum WsSlot {
     WS_STAGE_A,          // host<->device staging
     WS_STAGE_B,
     WS_STAGE_C,
+    WS_CCL_LROOT,        // u16 per pixel index: tile-local root (0xFFFF background)
     WS_NSLOTS
 };
 
