@@ -162,49 +162,62 @@ __device__ __forceinline__ void g_union(int32_t* lab, int32_t a, int32_t b) {
 
 // Tile-local labelling shared by K1 and K6.  On return val[] holds the pixel
 // values and par[i] the local root of pixel i (BGL for background).
+// Wave w holds the band of tile rows w*CPT .. w*CPT+CPT-1, lane = column, one
+// register per row.
+__device__ __forceinline__ int band_row(int w, int j) { return w * CPT + j; }
+
+// lane i <- lane i-1 (lane 0 <- 0), lane i <- lane i+1 (lane 63 <- 0): the
+// neighbour value across a tile edge counts as background.
+__device__ __forceinline__ uint32_t from_left(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, true);  // wave_shr:1
+}
+__device__ __forceinline__ uint32_t from_right(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xF, 0xF, true);  // wave_shl:1
+}
+
+// Tile-local labelling: v[j] receives the thread's pixel values (band rows),
+// par[i] the local root of pixel i (BGL for background); edge[] (4 x CW) is
+// scratch for the last row of each band.
 template <int CONN, typename InT, bool THR>
-__device__ __forceinline__ void local_label(const CclArgs& A, const CclImg& D, int x0, int y0, uint32_t* val,
-                                            uint32_t* par) {
+__device__ __forceinline__ void local_label(const CclArgs& A, const CclImg& D, int x0, int y0, uint32_t (&v)[CPT],
+                                            uint32_t* edge, uint32_t* par) {
     const int tid = threadIdx.x;
     const int lx = tid & 63, w = tid >> 6;
     const int gx = x0 + lx;
     // all loads first (independent, in flight together), then the LDS writes
-    uint32_t v[CPT];
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-        const int gy = y0 + w + 4 * j;
+        const int gy = y0 + band_row(w, j);
         v[j] = (gx < D.cols && gy < D.rows) ? pix<InT, THR>(A, D, gy, gx) : 0u;
     }
-    // Each wave holds whole tile rows (lane = column): a pixel's initial parent
-    // is the first pixel of its horizontal run of equal values, found with a
-    // ballot of run starts, so the unions below only join runs of adjacent rows.
+    // A pixel's initial parent is the first pixel of its horizontal run of
+    // equal values, found with a ballot of run starts, so the unions below only
+    // join runs of adjacent rows.
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-        const int i = (w + 4 * j) * CW + lx;
-        const uint32_t left = __shfl_up(v[j], 1, 64);
-        const bool start = v[j] != 0u && (lx == 0 || left != v[j]);
+        const int i = band_row(w, j) * CW + lx;
+        const bool start = v[j] != 0u && from_left(v[j]) != v[j];
         const uint64_t starts = __ballot(start);
         const uint64_t upto = lx == 63 ? ~0ull : ((2ull << lx) - 1ull);
         const int rs = 63 - __clzll(starts & upto);
-        val[i] = v[j];
-        par[i] = v[j] ? (uint32_t)((w + 4 * j) * CW + rs) : BGL;
+        par[i] = v[j] ? (uint32_t)(band_row(w, j) * CW + rs) : BGL;
     }
+    edge[w * CW + lx] = v[CPT - 1];
     __syncthreads();
     // Join each pair of overlapping runs once, at the leftmost column where they
     // touch: a vertical (or diagonal) link that a left neighbour in the same run
-    // already makes through the same run above is skipped.
+    // already makes through the same run above is skipped.  The row above is the
+    // previous register, or for a band's first row the band above's last row.
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-        const int ly = w + 4 * j;
+        const int ly = band_row(w, j);
         const int i = ly * CW + lx;
-        const uint32_t vv = val[i];
+        const uint32_t vv = v[j];
+        const uint32_t vu = j > 0 ? v[j - 1] : (w > 0 ? edge[(w - 1) * CW + lx] : 0u);
+        const uint32_t vl = from_left(vv), vr = from_right(vv);
+        const uint32_t vul = from_left(vu), vur = from_right(vu);
         if (!vv || ly == 0) continue;
         const int u = i - CW;
-        const uint32_t vu = val[u];
-        const uint32_t vl = lx > 0 ? val[i - 1] : 0u;
-        const uint32_t vul = lx > 0 ? val[u - 1] : 0u;
-        const uint32_t vur = lx < CW - 1 ? val[u + 1] : 0u;
-        const uint32_t vr = lx < CW - 1 ? val[i + 1] : 0u;
         if (vu == vv && !(vl == vv && vul == vv)) lds_union(par, i, u);
         if (CONN == 8) {
             if (vul == vv && vu != vv && vl != vv) lds_union(par, i, u - 1);
@@ -218,12 +231,12 @@ __device__ __forceinline__ void local_label(const CclArgs& A, const CclImg& D, i
     uint32_t root[CPT];
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-        const int i = (w + 4 * j) * CW + lx;
+        const int i = band_row(w, j) * CW + lx;
         root[j] = par[i] != BGL ? lds_root(par, i) : BGL;
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < CPT; ++j) par[(w + 4 * j) * CW + lx] = root[j];
+    for (int j = 0; j < CPT; ++j) par[band_row(w, j) * CW + lx] = root[j];
     __syncthreads();
 }
 
@@ -235,7 +248,7 @@ __device__ __forceinline__ int32_t gidx(const CclImg& D, int y0, int x0, uint32_
 // over its pixels instead of its pixel count.
 template <int CONN, typename InT, bool THR, bool WMAX>
 __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
-    __shared__ uint32_t val[CN];
+    __shared__ uint32_t edge[(CTHREADS / 64) * CW];
     __shared__ uint32_t par[CN];
     __shared__ uint32_t cnt[CN];
     __shared__ uint32_t nroots;
@@ -254,23 +267,32 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
         const int cx = min(gx, D.cols - 1);
 #pragma unroll
         for (int j = 0; j < CPT; ++j) {
-            const int cy = min(y0 + w + 4 * j, D.rows - 1);
+            const int cy = min(y0 + band_row(w, j), D.rows - 1);
             wv[j] = ((const uint8_t*)(D.wgt + (int64_t)cy * D.pw))[cx];
         }
     }
-    local_label<CONN, InT, THR>(A, D, x0, y0, val, par);
+    uint32_t v[CPT];
+    local_label<CONN, InT, THR>(A, D, x0, y0, v, edge, par);
+    // Sizes: a run's pixels share a root, so the lane at a run's start adds the
+    // run's length (first end at or after it, from a ballot of run ends).
+    // Maximum weights: every pixel.
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-        const int i = (w + 4 * j) * CW + lx;
-        if (par[i] == BGL) continue;
-        if (WMAX) atomicMax(&cnt[par[i]], wv[j]);
-        else atomicAdd(&cnt[par[i]], 1u);
+        const int i = band_row(w, j) * CW + lx;
+        if (WMAX) {
+            if (par[i] != BGL) atomicMax(&cnt[par[i]], wv[j]);
+        } else {
+            const bool start = v[j] != 0u && from_left(v[j]) != v[j];
+            const bool end = v[j] != 0u && from_right(v[j]) != v[j];
+            const uint64_t ends = __ballot(end);
+            if (start) atomicAdd(&cnt[par[i]], (uint32_t)__builtin_ctzll(ends >> lx) + 1u);
+        }
     }
     __syncthreads();
     uint2* slots = A.entries + (int64_t)blockIdx.x * CN;
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-        const int ly = w + 4 * j;
+        const int ly = band_row(w, j);
         const int i = ly * CW + lx;
         const uint32_t r = par[i];
         const int32_t g = D.pix_base + (y0 + ly) * D.cols + gx;
