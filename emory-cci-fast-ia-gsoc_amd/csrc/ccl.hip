@@ -148,10 +148,25 @@ __device__ __forceinline__ int32_t g_find(const int32_t* lab, int32_t x) {
     return x;
 }
 
+// Find with path halving for the merge kernel.  Parents only ever decrease
+// (every link points to a smaller index), so a halving write made from a stale
+// read still points at an ancestor; a union that lands on a non-root carries
+// on from the parent its atomicMin returned.
+__device__ __forceinline__ int32_t g_find_halve(int32_t* lab, int32_t x) {
+    int32_t p = lab[x];
+    while (p != x) {
+        const int32_t g = lab[p];
+        if (g != p) lab[x] = g;
+        x = p;
+        p = g;
+    }
+    return x;
+}
+
 __device__ __forceinline__ void g_union(int32_t* lab, int32_t a, int32_t b) {
     while (true) {
-        a = g_find(lab, a);
-        b = g_find(lab, b);
+        a = g_find_halve(lab, a);
+        b = g_find_halve(lab, b);
         if (a == b) return;
         if (a < b) { int32_t t = a; a = b; b = t; }
         const int32_t old = atomicMin(&lab[a], b);
@@ -280,7 +295,9 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
     for (int j = 0; j < CPT; ++j) {
         const int i = band_row(w, j) * CW + lx;
         if (WMAX) {
-            if (par[i] != BGL) atomicMax(&cnt[par[i]], wv[j]);
+            // a weight that does not exceed the current maximum (0 included)
+            // changes nothing: most pixels of a component skip the atomic
+            if (par[i] != BGL && wv[j] > cnt[par[i]]) atomicMax(&cnt[par[i]], wv[j]);
         } else {
             const bool start = v[j] != 0u && from_left(v[j]) != v[j];
             const bool end = v[j] != 0u && from_right(v[j]) != v[j];
