@@ -11,23 +11,22 @@
 // Each image owns a range of a global pixel index space that starts at a
 // multiple of 32768, so the root bitmap's scan blocks never straddle images.
 // Kernels (tile = 64 columns x 32 rows, 256 threads, 8 pixels per thread):
-//   K1 local    tile-local union-find in LDS: each row's runs of equal values
-//               get their first pixel as parent (wave ballot), then overlapping
-//               runs of adjacent rows are joined once with ds_min; finds halve
-//               paths.  Per local component (root global index, pixel count)
-//               goes to the tile's own slots of a component table; the local
-//               root index is written only for tile-edge pixels and roots (one
-//               sparse int32 array `lab`).
-//   K2 merge    pixel pairs that cross a tile edge are joined with global
-//               atomicMin union-find over `lab`.
+//   K1 local    tile-local union-find in LDS: each wave holds a band of rows;
+//               each row's runs of equal values get their first pixel as
+//               parent (wave ballot), then overlapping runs of adjacent rows are
+//               joined once with ds_min; finds halve paths.  Per local
+//               component (root global index, pixel count) goes to the tile's
+//               own slots of a component table; every pixel's tile-local root
+//               goes to a u16 array; `lab` (int32, sparse) holds the roots.
+//   K2 merge    pixel pairs that cross a tile edge unite their local roots
+//               with global atomicMin union-find over `lab`.
 //   K3 area     per table entry: atomicAdd its count into area[find(root)].
 //   K4 keep     per table entry that is a global root: keep it if the area is
 //               in [min, max); set its bit in the root bitmap.
 //   K5 scan     popcount prefix over the bitmap -> rank of every kept root and
 //               the kept count of every image.
-//   K6 final    recompute the tile-local labelling in LDS from the input (1 B
-//               per pixel instead of re-reading a 4 B label array), map each
-//               local root to its global root once, write the output.
+//   K6 final    read every pixel's tile-local root (2 B), map each local root
+//               to its global root once, write the output.
 // HBM bytes per pixel: K1 reads 1, K6 reads 1 and writes 4 (u8 input, int32
 // output); everything else touches only tile edges, roots and the bitmap.
 #include "nsx_internal.h"
@@ -255,10 +254,6 @@ __device__ __forceinline__ void local_label(const CclArgs& A, const CclImg& D, i
     __syncthreads();
 }
 
-__device__ __forceinline__ int32_t gidx(const CclImg& D, int y0, int x0, uint32_t li) {
-    return D.pix_base + (y0 + (int)(li >> 6)) * D.cols + x0 + (int)(li & 63);
-}
-
 // WMAX: each local component's slot carries the maximum of the u8 weight image
 // over its pixels instead of its pixel count.
 template <int CONN, typename InT, bool THR, bool WMAX>
@@ -322,51 +317,86 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
             slots[k] = make_uint2((uint32_t)g, cnt[i]);
             A.lab[g] = g;
             if (A.use_area || WMAX) A.area[g] = 0;
-        } else if (ly == 0 || ly == CH - 1 || lx == 0 || lx == CW - 1) {
-            A.lab[g] = gidx(D, y0, x0, r);
         }
     }
     __syncthreads();
     if (tid == 0) A.tcount[blockIdx.x] = (int32_t)nroots;
 }
 
+// Global index of the tile-local root lr of the tile holding pixel (qy, qx).
+__device__ __forceinline__ int32_t root_gidx(const CclImg& D, int qy, int qx, uint32_t lr) {
+    const int ty0 = qy - qy % CH, tx0 = qx - qx % CW;
+    return D.pix_base + (ty0 + (int)(lr >> 6)) * D.cols + tx0 + (int)(lr & 63u);
+}
+
 // K2: one thread per tile-edge pixel whose backward neighbours cross the edge:
-// threads 0..63 the top row (up, up-left, up-right), 64..95 the left column
-// (left, and up-left below the top row), 96..126 the right column (up-right
-// below the top row).
+// threads 0..63 (wave 0) the top row (up, up-left, up-right), 64..95 (wave 1,
+// lanes 0..31) the left column (left, and up-left below the top row), 96..126
+// (lanes 32..62) the right column (up-right below the top row).  Pixels and
+// their neighbours are compared through the local roots K1 stored (a root
+// means foreground; raw u8 input also compares values) and the two local
+// roots are united.  Redundant unions are skipped:
+//   * the previous pixel along the same edge joined the same pair of roots;
+//   * a diagonal whose straight neighbour across the edge already joins: the
+//     two neighbours are adjacent inside one tile, hence one local component
+//     (except at the tile corners, where the diagonal lies in a third tile);
+//   * the right column's up-right, when the pixel above is in the same local
+//     component: the right tile's left column joins that pair.
 template <int CONN, typename InT, bool THR>
 __global__ __launch_bounds__(128) void ccl_merge_kernel(CclArgs A) {
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
     CclImg D;
     int tx, ty;
     tile_of(A, blockIdx.x, D, tx, ty);
     const int x0 = tx * CW, y0 = ty * CH;
+    const int kind = tid < 64 ? 0 : tid < 96 ? 1 : 2;   // top, left, right
     int y, x;
-    if (tid < 64) { y = y0; x = x0 + tid; }
-    else if (tid < 96) { y = y0 + (tid - 64); x = x0; }
-    else { y = y0 + 1 + (tid - 96); x = x0 + CW - 1; if (tid == 127) return; }
-    if (x >= D.cols || y >= D.rows) return;
-    const uint32_t v = pix<InT, THR>(A, D, y, x);
-    if (!v) return;
-    const int32_t g = D.pix_base + y * D.cols + x;
-    auto join = [&](int qy, int qx) {
-        if (qy < 0 || qx < 0 || qx >= D.cols) return;
-        if (pix<InT, THR>(A, D, qy, qx) != v) return;
-        const int32_t q = D.pix_base + qy * D.cols + qx;
-        g_union(A.lab, A.lab[g], A.lab[q]);
+    if (kind == 0) { y = y0; x = x0 + tid; }
+    else if (kind == 1) { y = y0 + (tid - 64); x = x0; }
+    else { y = y0 + 1 + (tid - 96); x = x0 + CW - 1; }
+    const bool valid = tid != 127 && x < D.cols && y < D.rows && y < y0 + CH;
+    auto root_at = [&](int qy, int qx) -> uint32_t {
+        if (qy < 0 || qx < 0 || qx >= D.cols || qy >= D.rows) return 0xFFFFu;
+        return A.lroot[D.pix_base + qy * D.cols + qx];
     };
-    if (tid < 64) {
+    auto same = [&](int qy, int qx, uint32_t qr, uint32_t v) -> bool {
+        // fg pixel q with the value of this pixel
+        if (qr == 0xFFFFu) return false;
+        return THR || pix<InT, THR>(A, D, qy, qx) == v;
+    };
+    const uint32_t a = valid ? root_at(y, x) : 0xFFFFu;
+    const uint32_t v = (!THR && a != 0xFFFFu) ? pix<InT, THR>(A, D, y, x) : 1u;
+    // the straight neighbour across the edge (up for the top row, left for the
+    // left column) and its root, for the previous-lane test
+    int sy = y - 1, sx = x;
+    if (kind == 1) { sy = y; sx = x - 1; }
+    const uint32_t b = (a != 0xFFFFu && kind != 2) ? root_at(sy, sx) : 0xFFFFu;
+    const bool straight = kind != 2 && a != 0xFFFFu && same(sy, sx, b, v);
+    // previous pixel along the edge (lane - 1, same edge) with the same pair
+    const uint32_t pa = from_left(a), pb = from_left(b);
+    const bool prev_same_edge = kind == 0 ? lane > 0 : (kind == 1 ? lane > 0 : false);
+    const bool dup = prev_same_edge && pa == a && pb == b;
+    if (a == 0xFFFFu) return;
+    const int32_t ga = root_gidx(D, y, x, a);
+    if (straight && !dup) g_union(A.lab, ga, root_gidx(D, sy, sx, b));
+    if (CONN != 8) return;
+    auto diag = [&](int qy, int qx) {
+        const uint32_t q = root_at(qy, qx);
+        if (same(qy, qx, q, v)) g_union(A.lab, ga, root_gidx(D, qy, qx, q));
+    };
+    if (kind == 0) {
         if (y > 0) {
-            join(y - 1, x);
-            if (CONN == 8) { join(y - 1, x - 1); join(y - 1, x + 1); }
+            if (!straight || x == x0) diag(y - 1, x - 1);
+            if (!straight || x == x0 + CW - 1) diag(y - 1, x + 1);
         }
-    } else if (tid < 96) {
-        if (x > 0) {
-            join(y, x - 1);
-            if (CONN == 8 && y > y0) join(y - 1, x - 1);
-        }
+    } else if (kind == 1) {
+        if (x > 0 && y > y0 && !straight) diag(y - 1, x - 1);
     } else {
-        if (CONN == 8 && x + 1 < D.cols) join(y - 1, x + 1);
+        if (x + 1 < D.cols) {
+            const uint32_t u = root_at(y - 1, x);   // inside this tile (y > y0)
+            if (u != a) diag(y - 1, x + 1);
+        }
     }
 }
 
