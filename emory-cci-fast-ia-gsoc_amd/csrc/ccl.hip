@@ -424,7 +424,14 @@ __global__ __launch_bounds__(256) void ccl_keep_kernel(CclArgs A) {
     const uint2* slots = A.entries + (int64_t)blockIdx.x * CN;
     for (int e = threadIdx.x; e < n; e += blockDim.x) {
         const int32_t g = (int32_t)slots[e].x;
-        if (A.lab[g] != g) continue;
+        const int32_t p = A.lab[g];
+        if (p != g) {
+            // the output kernel finds this local root's root again: shorten it
+            // to one hop (the root is final: all unions are done)
+            const int32_t r = g_find(A.lab, p);
+            if (r != p) A.lab[g] = r;
+            continue;
+        }
         bool keep = true;
         if (A.use_area) {
             const int a = A.area[g];
