@@ -99,9 +99,14 @@ def bench_batch(args):
                        torch.empty_like(dk), torch.empty(dk.shape, dtype=torch.int32, device=dev), thr))
     total_px = sum(m.size for _, m, _ in tiles)
     ctx = nh.default_context(0)
-    # per-tile launches
+    # per-tile launches, with the same algorithm as the batch: the level-set
+    # reconstruction (whose value-set scan costs a synchronisation per call)
+    # is switched off, as the batched path never takes it
+    ctx_tile = nh.Context(0)
+    ctx_tile.set_tuning(nh._lib.NSX_TUNE_LEVELSET_LEVELS, 0)
+
     def per_tile():
-        return [nh.recon_label(dm, dk, 8, thr, 8, recon_out=r2, labels_out=l2, ctx=ctx)[2]
+        return [nh.recon_label(dm, dk, 8, thr, 8, recon_out=r2, labels_out=l2, ctx=ctx_tile)[2]
                 for dm, dk, _, _, r2, l2, thr in dtiles]
     per_tile()
     torch.cuda.synchronize()
