@@ -193,10 +193,19 @@ __device__ __forceinline__ uint32_t med3u(uint32_t a, uint32_t b, uint32_t c) {
     return r;
 }
 
+// Maximum over the previous row's 1- or 3-neighbourhood.  For 8-connectivity
+// the two maxima are kept apart (an empty asm holds the first result) so that
+// each becomes a v_max_u32 with its DPP shift folded in, two instructions
+// instead of two DPP moves and a v_max3.
 template <int CONN>
 __device__ __forceinline__ uint32_t nbr(uint32_t p) {
-    if constexpr (CONN == 8) return max(p, max(from_left(p), from_right(p)));
-    else return p;
+    if constexpr (CONN == 8) {
+        uint32_t t = max(p, from_left(p));
+        asm volatile("" : "+v"(t));
+        return max(t, from_right(p));
+    } else {
+        return p;
+    }
 }
 
 // The mask lives in LDS as mb[row * LDS_PITCH + col] (u8 keys for u8 images,
