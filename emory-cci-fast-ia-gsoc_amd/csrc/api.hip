@@ -105,6 +105,10 @@ nsx_status check_img(nsx_ctx* ctx, const nsx_img* im, const char* name) {
 
 bool same_shape(const nsx_img* a, const nsx_img* b) { return a->rows == b->rows && a->cols == b->cols; }
 
+// A 0-pixel image: entry points that report a component count return 0
+// without touching the device.
+bool is_empty(const nsx_img* im) { return (int64_t)im->rows * (int64_t)im->cols == 0; }
+
 // Device view of an image: device images are used in place; host images are
 // staged through a workspace slot (copied in when `copy_in`).
 nsx_status to_device(nsx_ctx* ctx, const nsx_img* im, int slot, bool copy_in, hipStream_t s, View* v) {
@@ -351,6 +355,10 @@ nsx_status nsx_bwlabel(nsx_ctx* ctx, const nsx_img* bin, nsx_img* labels, int co
     if (labels->dtype != NSX_I32) return ctx->fail(NSX_ERR_INVALID, "labels must be int32");
     if (!same_shape(bin, labels)) return ctx->fail(NSX_ERR_INVALID, "bin and labels must have the same size");
     if (overlaps(bin, labels)) return ctx->fail(NSX_ERR_INVALID, "labels aliases bin");
+    if (is_empty(bin)) {
+        if (ncomp) *ncomp = 0;
+        return NSX_OK;
+    }
     hipStream_t s = pick_stream(ctx, stream);
     View vi, vo;
     NSX_TRY(to_device(ctx, bin, WS_STAGE_B, true, s, &vi));
@@ -378,6 +386,10 @@ nsx_status nsx_bwareaopen(nsx_ctx* ctx, const nsx_img* in, int labeled, int flat
         return ctx->fail(NSX_ERR_INVALID, flatten ? "flattened output must be u8" : "output must be int32");
     if (!same_shape(in, out)) return ctx->fail(NSX_ERR_INVALID, "in and out must have the same size");
     if (overlaps(in, out)) return ctx->fail(NSX_ERR_INVALID, "out aliases in");
+    if (is_empty(in)) {
+        if (count) *count = 0;
+        return NSX_OK;
+    }
     hipStream_t s = pick_stream(ctx, stream);
     View vi, vo;
     NSX_TRY(to_device(ctx, in, WS_STAGE_B, true, s, &vi));
@@ -405,6 +417,10 @@ nsx_status nsx_relabel(nsx_ctx* ctx, nsx_img* labels, int bgval, int32_t* ncomp,
     if (!ctx) return NSX_ERR_INVALID;
     NSX_TRY(check_img(ctx, labels, "labels"));
     if (labels->dtype != NSX_I32) return ctx->fail(NSX_ERR_INVALID, "labels must be int32");
+    if (is_empty(labels)) {
+        if (ncomp) *ncomp = 0;
+        return NSX_OK;
+    }
     hipStream_t s = pick_stream(ctx, stream);
     View v;
     NSX_TRY(to_device(ctx, labels, WS_STAGE_C, true, s, &v));
@@ -445,6 +461,10 @@ nsx_status nsx_recon_label(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* m
     if (labels_out->dtype != NSX_I32) return ctx->fail(NSX_ERR_INVALID, "labels must be int32");
     if (overlaps(recon_out, marker) || overlaps(recon_out, mask) || overlaps(labels_out, recon_out))
         return ctx->fail(NSX_ERR_INVALID, "outputs alias inputs");
+    if (is_empty(mask)) {
+        if (ncomp) *ncomp = 0;
+        return NSX_OK;
+    }
     hipStream_t s = pick_stream(ctx, stream);
     View vm, vk, vr, vl;
     NSX_TRY(to_device(ctx, marker, WS_STAGE_A, true, s, &vm));
@@ -484,6 +504,10 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
         NSX_TRY(check_img(ctx, o, "chain output"));
         if (!same_shape(o, mask) || o->dtype != mask->dtype)
             return ctx->fail(NSX_ERR_INVALID, "chain outputs must match the mask's size and type");
+    }
+    if (is_empty(mask)) {
+        if (ncomp) *ncomp = 0;
+        return NSX_OK;
     }
     hipStream_t s = pick_stream(ctx, stream);
     View vm, vk, vr, vf, vl;

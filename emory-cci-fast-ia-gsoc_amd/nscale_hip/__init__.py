@@ -75,6 +75,9 @@ def describe(x) -> NsxImg:
         raise NsxError(_lib.NSX_ERR_INVALID, "image must be a 2-D numpy array or device tensor (one channel)")
     if a.dtype not in _DT:
         raise NsxError(_lib.NSX_ERR_INVALID, f"unsupported dtype {a.dtype}")
+    if a.size == 0:  # numpy gives empty arrays zero strides
+        return NsxImg(a.ctypes.data, a.shape[1] * a.itemsize, a.shape[0], a.shape[1], _DT[a.dtype],
+                      _lib.NSX_MEM_HOST)
     if a.strides[1] != a.itemsize:
         raise NsxError(_lib.NSX_ERR_INVALID, "host image must have unit column stride")
     return NsxImg(a.ctypes.data, a.strides[0], a.shape[0], a.shape[1], _DT[a.dtype], _lib.NSX_MEM_HOST)

@@ -212,3 +212,40 @@ def test_bwlabel_large_repeated(nh, orc, gpu):
         out.fill_(-7)
         _, n = nh.bwlabel2(d, 8, True, out=out, return_count=True)
         assert n == wn and np.array_equal(out.cpu().numpy(), want), f"repetition {rep}"
+
+
+def test_empty_images(nh, gpu):
+    """0-pixel images are accepted by every entry point and give 0 components,
+    also right after a call that found some (no stale count)."""
+    img = np.where(np.random.default_rng(3).random((40, 50)) < 0.5, 255, 0).astype(np.uint8)
+    for e in (np.zeros((0, 7), np.uint8), np.zeros((5, 0), np.uint8)):
+        _, n = nh.bwlabel2(img, 8, True, return_count=True)
+        assert n > 0
+        assert nh.imreconstruct(e, e, 8).shape == e.shape
+        assert nh.imreconstructBinary(e, e, 4).shape == e.shape
+        assert nh.imfillHoles(e, False, 4).shape == e.shape
+        lab, n = nh.bwlabel2(e, 8, True, return_count=True)
+        assert lab.shape == e.shape and n == 0
+        out, n = nh.bwareaopen2(e, False, False, 1, 10, 8)
+        assert out.shape == e.shape and n == 0
+        _, _, lab, n = nh.segment_chain(e, e)
+        assert lab.shape == e.shape and n == 0
+        _, lab, n = nh.recon_label(e, e)
+        assert n == 0
+
+
+def test_more_than_int32_pixels_rejected(nh, gpu):
+    """Images of 2^31-1 pixels or more are refused by the host checks before
+    any device work (labels and tile lists index pixels with int32)."""
+    import ctypes as C
+    import torch
+    buf = torch.zeros(64, dtype=torch.uint8, device="cuda")
+    lab = torch.zeros(64, dtype=torch.int32, device="cuda")
+    L = nh._lib
+    big = L.NsxImg(buf.data_ptr(), 50000, 50000, 50000, L.NSX_U8, L.NSX_MEM_DEVICE)
+    bigl = L.NsxImg(lab.data_ptr(), 200000, 50000, 50000, L.NSX_I32, L.NSX_MEM_DEVICE)
+    ctx = nh.default_context(0)
+    n = C.c_int32(0)
+    assert ctx.lib.nsx_bwlabel(ctx.h, C.byref(big), C.byref(bigl), 8, 1, C.byref(n), None) == L.NSX_ERR_INVALID
+    assert ctx.lib.nsx_imreconstruct(ctx.h, C.byref(big), C.byref(big), C.byref(big), 8, None) == L.NSX_ERR_INVALID
+    assert b"2^31" in ctx.lib.nsx_last_error(ctx.h)
