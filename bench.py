@@ -42,16 +42,17 @@ def parse():
     p.add_argument("--disks", type=int, default=3000)
     p.add_argument("--no-cpu", "--no-check", dest="no_cpu", action="store_true",
                    help="skip the CPU baseline leg (the oracle run that also checks the GPU results)")
-    p.add_argument("--cpu-runs", type=int, default=10, help="oracle chain runs in the CPU sample")
+    p.add_argument("--cpu-runs", type=int, default=20, help="oracle chain runs in the CPU sample")
     p.add_argument("--workload", default="chain4096",
                    choices=["chain4096", "batch", "maze", "serve", "copy"],
                    help="chain4096: the headline 4096^2 chain (default); batch: 256 u8 + 32 f32 uneven tiles "
                         "batched vs per-tile; maze: 16384^2 maze stress; serve: continuous-batching server")
     p.add_argument("--tiles-u8", type=int, default=256)
     p.add_argument("--tiles-f32", type=int, default=32)
-    p.add_argument("--check-tiles", type=int, default=6, help="batch tiles also checked against the oracle")
+    p.add_argument("--check-tiles", type=int, default=64, help="batch tiles also checked against the oracle")
     p.add_argument("--maze-size", type=int, default=16384, help="maze: side of the maze image")
-    p.add_argument("--maze-cpu-size", type=int, default=4096, help="maze: side of the CPU baseline sample")
+    p.add_argument("--maze-cpu-size", type=int, default=16384,
+                   help="maze: side of the maze the CPU baseline runs (default: the whole workload)")
     p.add_argument("--requests", type=int, default=200, help="serve: requests in the trace")
     p.add_argument("--slots", type=int, default=4, help="serve: worker slots of the server")
     p.add_argument("--load", type=float, default=1.0,
@@ -507,8 +508,9 @@ def bench_copy(args):
 
 def cpu_baseline_maze(nh, n, ctx):
     """The oracle's imreconstruct + bwlabel2 (single thread) on an n x n maze of
-    the same construction (a bounded sample of the 16384^2 workload); the GPU
-    result on the same sample is compared with it.  Returns cpu_baseline."""
+    the same construction (by default the whole 16384^2 workload, a few seconds
+    of CPU work); the GPU result on the same input is compared with it.
+    Returns cpu_baseline."""
     import numpy as np
     import torch
 
@@ -536,8 +538,8 @@ TILE_PX = 62 * 62   # interior pixels of one reconstruction tile visit
 # launch of the kind.  FETCH_SIZE is scaled by 2.0 and WRITE_SIZE by 1.0: the
 # factors that make `--workload copy` (256 MiB read + 256 MiB written per
 # launch) read true.  Commands and raw figures: profiles/traffic_chain4096.md.
-TRAFFIC_CHAIN4096 = {"recon_round": 18464218, "recon_first": 104523601, "ccl_local": 37200903,
-                     "ccl_final": 101865173, "ccl_merge": 65550201}
+TRAFFIC_CHAIN4096 = {"recon_round": 18486001, "recon_first": 104510512, "ccl_local": 68011315,
+                     "ccl_final": 101627904, "ccl_merge": 40422400}
 
 
 def roofline(kt: dict, px: int, visits: tuple, steps: int, traffic: dict | None = None) -> dict:

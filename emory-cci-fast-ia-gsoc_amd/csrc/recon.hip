@@ -288,6 +288,43 @@ __device__ __forceinline__ void transpose(uint32_t (&c)[TS], LT* lds) {
 template <typename T> struct LdsT { typedef uint32_t type; };
 template <> struct LdsT<uint8_t> { typedef uint8_t type; };
 
+// ---- u8 tile loads in 16-byte blocks ----------------------------------------------
+// A u8 tile row is 64 bytes at an arbitrary offset, so one load per row (lane =
+// column) moves one byte per lane: 64 load instructions per image and tile.
+// Instead each row's window is read as five 16-byte-aligned blocks (320 blocks,
+// five load instructions per image), staged in LDS at WPITCH bytes per row, and
+// each lane then picks its column's byte.  Row y's window starts at the block
+// holding column xs = max(x0, 0) and needs the blocks up to the one holding
+// column xe - 1 = min(x0 + 64, cols) - 1; the remaining block slots read that
+// last block again (their bytes are never picked).  So every block read holds a
+// byte of the row, and an aligned 16-byte block never straddles a page.
+constexpr int WBLK = 5;
+constexpr int WPITCH = 16 * WBLK;
+static_assert(TS * WBLK == 5 * 64, "one block per lane per load instruction");
+
+template <typename T> struct Stage { static constexpr int elems = TS * LDS_PITCH; };
+template <> struct Stage<uint8_t> { static constexpr int elems = TS * WPITCH > TS * LDS_PITCH ? TS * WPITCH : TS * LDS_PITCH; };
+
+__device__ __forceinline__ uintptr_t row_addr(const char* base, int64_t pitch, int y) {
+    return (uintptr_t)base + (uintptr_t)((int64_t)y * pitch);
+}
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Block j (0..WBLK-1) of the window of row y.
+__device__ __forceinline__ u32x4 ld_block(const char* base, int64_t pitch, int y, int xs, int xe, int j) {
+    const uintptr_t row = row_addr(base, pitch, y);
+    const uintptr_t first = (row + (uintptr_t)xs) & ~(uintptr_t)15;
+    const uintptr_t last = (row + (uintptr_t)xe - 1u) & ~(uintptr_t)15;
+    const uintptr_t a = min(first + 16u * (uintptr_t)j, last);
+    return *(gptr<const u32x4>)a;
+}
+
+// Byte offset, in row y's staged window, of column xs.
+__device__ __forceinline__ int win_off(const char* base, int64_t pitch, int y, int xs) {
+    return (int)((row_addr(base, pitch, y) + (uintptr_t)xs) & 15u);
+}
+
 // ---- propagation rounds over a device tile list ---------------------------------------
 // Round 0 visits every tile.  A visit that raises a halo pixel appends the
 // owning tile to the next round's list (a per-tile stamp keeps each tile at
@@ -384,11 +421,96 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
         mk = in ? C::mkey(av, mv) : F;
         return in ? key : F;
     };
-    // The tile's rows: chunk j+1 is in flight while chunk j is decoded.
     uint32_t c[TS];
     const int gx = x0 + lane;
     const bool col_in = (unsigned)gx < (unsigned)D.cols;
     const int cx = min(max(gx, 0), D.cols - 1);
+    // u8 images other than imreconstructBinary (whose later rounds read three
+    // images) load in 16-byte blocks through the staging buffer tb.
+    constexpr bool kWide = sizeof(T) == 1 && MODE != RM_BINARY;
+    if constexpr (kWide) {
+        const bool kSec = FIRST ? C::kMarkerFirst : true;   // marker (first pass) or state
+        const char* sec = FIRST ? D.marker : D.state;
+        const int64_t psec = FIRST ? D.pm : D.ps;
+        const int xs = max(x0, 0), xe = min(x0 + TS, D.cols);
+        u32x4 bm[WBLK], bs[WBLK];
+#pragma unroll
+        for (int i = 0; i < WBLK; ++i) {
+            const int q = i * TS + lane;
+            const int k = q / WBLK, j = q - k * WBLK;
+            const int cy = min(max(y0 + k, 0), D.rows - 1);
+            bm[i] = ld_block(D.mask, D.pk, cy, xs, xe, j);
+            if (kSec) bs[i] = ld_block(sec, psec, cy, xs, xe, j);
+        }
+        u32x4* t4 = reinterpret_cast<u32x4*>(tb);
+        const int lo = cx - xs;
+        // Row k's window offset: lane k computes it once.  When both pitches
+        // are multiples of 16 every row has the same offset.
+        const int cyl = min(max(y0 + lane, 0), D.rows - 1);
+        const int offm_l = win_off(D.mask, D.pk, cyl, xs);
+        const int offs_l = kSec ? win_off(sec, psec, cyl, xs) : 0;
+        // Staged bytes are read 16 rows at a time with no condition (so the
+        // reads issue back to back); the frame is applied afterwards, in tiles
+        // that reach past the image only.
+        auto decode = [&](auto off_of) {
+#pragma unroll
+            for (int i = 0; i < WBLK; ++i) t4[i * TS + lane] = bm[i];
+            __syncthreads();
+#pragma unroll
+            for (int ch = 0; ch < TS / LOAD_CH; ++ch) {
+                uint32_t v[LOAD_CH];
+#pragma unroll
+                for (int j = 0; j < LOAD_CH; ++j) {
+                    const int k = ch * LOAD_CH + j;
+                    v[j] = tb[k * WPITCH + off_of(offm_l, k) + lo];
+                }
+                arrived(v);
+#pragma unroll
+                for (int j = 0; j < LOAD_CH; ++j) {
+                    const int k = ch * LOAD_CH + j;
+                    mb[k * LDS_PITCH + lane] = (LT)C::mkey(T(0), (T)v[j]);
+                    // imreconstruct's first key is min(marker, mask): keep the mask value
+                    if (FIRST && MODE == RM_RECON) c[k] = v[j];
+                }
+            }
+            __syncthreads();
+            if (kSec) {
+#pragma unroll
+                for (int i = 0; i < WBLK; ++i) t4[i * TS + lane] = bs[i];
+                __syncthreads();
+            }
+#pragma unroll
+            for (int ch = 0; ch < TS / LOAD_CH; ++ch) {
+                uint32_t v[LOAD_CH];
+#pragma unroll
+                for (int j = 0; j < LOAD_CH; ++j) {
+                    const int k = ch * LOAD_CH + j;
+                    v[j] = kSec ? (uint32_t)tb[k * WPITCH + off_of(offs_l, k) + lo] : 0u;
+                }
+                if (kSec) arrived(v);
+#pragma unroll
+                for (int j = 0; j < LOAD_CH; ++j) {
+                    const int k = ch * LOAD_CH + j;
+                    if constexpr (FIRST) c[k] = C::first((T)v[j], MODE == RM_RECON ? (T)c[k] : T(0));
+                    else c[k] = C::dec((T)v[j], T(0), T(0));
+                }
+            }
+            __syncthreads();   // tb is the transpose buffer from here on
+        };
+        if (((D.pk | psec) & 15) == 0)
+            decode([](int v, int) { return __builtin_amdgcn_readfirstlane(v); });
+        else
+            decode([](int v, int k) { return __builtin_amdgcn_readlane(v, k); });
+        if (x0 < 0 || y0 < 0 || x0 + TS > D.cols || y0 + TS > D.rows) {
+#pragma unroll
+            for (int k = 0; k < TS; ++k) {
+                const bool in = col_in && (unsigned)(y0 + k) < (unsigned)D.rows;
+                c[k] = in ? c[k] : F;
+                if (!in) mb[k * LDS_PITCH + lane] = (LT)F;
+            }
+        }
+    } else {
+    // The tile's rows: chunk j+1 is in flight while chunk j is decoded.
     uint32_t rm[2][LOAD_CH], ra[2][LOAD_CH], rs[2][LOAD_CH];
     auto issue = [&](int b, int k0) {
 #pragma unroll
@@ -417,6 +539,7 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
             c[k] = key_of(in, rm[b][j], ra[b][j], rs[b][j], mk);
             mb[k * LDS_PITCH + lane] = (LT)mk;
         }
+    }
     }
     hs[lane] = c[0];
     hs[TS + lane] = c[TS - 1];
@@ -494,14 +617,15 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
     if (want) activate_all(A, D, ty, tx, round, want);
 }
 
-// u8 tiles keep their mask and transpose staging in 8.3 KB of LDS and run
-// 3 waves per SIMD (at most 168 VGPRs); 32-bit tiles are LDS-bound at 1.
+// u8 tiles keep their mask, block staging / transpose buffer and halo keys in
+// 10.3 KB of LDS and run 3 waves per SIMD (at most 168 VGPRs); 32-bit tiles
+// are LDS-bound at 1.
 template <typename T, int MODE, int CONN, bool FIRST>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 1 ? 3 : 1)))
 void recon_round_kernel(ReconArgs A, int round) {
     using LT = typename LdsT<T>::type;
-    __shared__ LT mb[TS * LDS_PITCH];   // mask keys of the tile
-    __shared__ LT tb[TS * LDS_PITCH];   // transpose staging
+    __shared__ LT mb[TS * LDS_PITCH];                     // mask keys of the tile
+    __shared__ __align__(16) LT tb[Stage<T>::elems];      // u8 block staging, then transposes
     __shared__ uint32_t hs[4 * TS];     // halo keys before the sweeps
     int n;
     const int32_t* list = (round & 1) ? A.list1 : A.list0;
