@@ -259,9 +259,9 @@ nsx_status nsx_batch_finalize(nsx_batch* b, int use_graph) {
         if (st == NSX_OK) st = c->ws(WS_CCL_MISC, sizeof(uint32_t) * (size_t)(nw + nb + 1), &p);
         if (st == NSX_OK) st = c->ws(WS_CCL_LROOT, sizeof(uint16_t) * (size_t)max_pix, &p);
         if (st == NSX_OK) st = c->ws(WS_RECON_STAMP, sizeof(uint32_t) * (size_t)max_rt_even, &p);
-        if (st == NSX_OK) st = c->ws(WS_RECON_LIST, 2 * sizeof(int32_t) * (size_t)max_rt_even, &p);
+        if (st == NSX_OK) st = c->ws(WS_RECON_LIST, recon_list_bytes(max_rt_even), &p);
         if (st == NSX_OK && max_rt_odd) st = c->ws(WS_FILL_STAMP, sizeof(uint32_t) * (size_t)max_rt_odd, &p);
-        if (st == NSX_OK && max_rt_odd) st = c->ws(WS_FILL_LIST, 2 * sizeof(int32_t) * (size_t)max_rt_odd, &p);
+        if (st == NSX_OK && max_rt_odd) st = c->ws(WS_FILL_LIST, recon_list_bytes(max_rt_odd), &p);
         if (st != NSX_OK) return fail_to_owner(b, st);
         std::vector<ReconRun> runs;
         st = enqueue(b, s, &runs);

@@ -102,8 +102,8 @@ namespace nsx {
 // Workspace slots.
 enum WsSlot {
     WS_RECON_STAMP = 0,  // per-tile queue stamps
-    WS_RECON_LIST,       // 2 tile lists
-    WS_RECON_COUNTS,     // 3 round counters (+ pad)
+    WS_RECON_LIST,       // 2 segmented tile lists
+    WS_RECON_COUNTS,     // 3 rounds x 8 segment counters (spaced) + a sum word
     WS_CCL_LABEL,        // int32 per pixel (when the caller's output cannot hold it)
     WS_CCL_AREA,         // int32 per pixel (roots only are touched)
     WS_CCL_ROOTS,        // compact local-root table: (global index, count) pairs
@@ -170,6 +170,8 @@ struct ReconRun {
 };
 int recon_tiles(int rows, int cols);
 size_t recon_desc_bytes();
+// Bytes of the two round lists (second of a reconstruction's ws slots).
+size_t recon_list_bytes(int ntiles);
 void recon_fill_desc(void* out, const View* marker, const View& mask, const View& state, int tile_base);
 // Enqueues round 0 and the predicted number of further rounds (no host sync).
 // When pending_dev is given, the number of tiles still queued after the last

@@ -325,6 +325,19 @@ __device__ __forceinline__ int win_off(const char* base, int64_t pitch, int y, i
     return (int)((row_addr(base, pitch, y) + (uintptr_t)xs) & 15u);
 }
 
+// Inclusive prefix sum over the wave: Hillis-Steele within each row of 16
+// (DPP row_shr 1, 2, 4, 8), then row_bcast15 / row_bcast31 add the totals of
+// the rows below.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return v;
+}
+
 // ---- propagation rounds over a device tile list ---------------------------------------
 // Round 0 visits every tile.  A visit that raises a halo pixel appends the
 // owning tile to the next round's list (a per-tile stamp keeps each tile at
@@ -333,6 +346,16 @@ __device__ __forceinline__ int win_off(const char* base, int64_t pitch, int y, i
 // the round after.  Kernel boundaries order the rounds, so a tile sees every
 // write of the previous round; visits inside a round only read neighbours,
 // whose values are valid lower bounds whether or not they are mid-update.
+//
+// Each list is split into NSEG segments with a counter each (256 bytes apart):
+// a workgroup appends to segment blockIdx % NSEG, so appends from the XCDs
+// (workgroups are dealt to them round-robin) go to different counters instead
+// of queueing on one.  The next round reads the segments as one list, through
+// the prefix sum of the counters.
+constexpr int NSEG = 8;
+constexpr int CSTRIDE = 64;   // ints between counters
+__host__ __device__ inline int cnt_idx(int round, int seg) { return ((round % 3) * NSEG + seg) * CSTRIDE; }
+constexpr int CNT_SUM = 3 * NSEG * CSTRIDE;   // scratch word: sum of one round's counters
 
 // One image of a reconstruction launch.
 struct ImgDesc {
@@ -350,9 +373,9 @@ struct ReconArgs {
     const int32_t* tile_img;   // global tile -> image index when nimg > 1
     int nimg, ntiles;
     uint32_t* stamp;           // per tile: last round (+2) it was queued for
-    int32_t* list0;
+    int32_t* list0;            // [NSEG][ntiles]
     int32_t* list1;
-    int32_t* counts;           // [3] rotating round counters
+    int32_t* counts;           // [3][NSEG] rotating round counters, CSTRIDE ints apart
     int64_t* stats;            // [0] rounds with work [1] tile visits after round 0 [2] tiles
     int visit_sweeps;          // sweep pairs per visit before re-queueing (1..MAX_IT)
 };
@@ -391,10 +414,11 @@ __device__ __forceinline__ void activate_all(const ReconArgs& A, const ImgDesc& 
     if (!won) return;
     const int first = __builtin_ctzll(won);
     int base = 0;
-    if (lane == first) base = atomicAdd(&A.counts[(round + 1) % 3], __builtin_popcountll(won));
+    const int seg = (int)(blockIdx.x % NSEG);
+    if (lane == first) base = atomicAdd(&A.counts[cnt_idx(round + 1, seg)], __builtin_popcountll(won));
     base = __shfl(base, first, 64);
     if (claim) {
-        int32_t* nl = ((round + 1) & 1) ? A.list1 : A.list0;
+        int32_t* nl = (((round + 1) & 1) ? A.list1 : A.list0) + (int64_t)seg * A.ntiles;
         nl[base + __builtin_popcountll(won & ((1ull << lane) - 1ull))] = t;
     }
 }
@@ -627,19 +651,37 @@ void recon_round_kernel(ReconArgs A, int round) {
     __shared__ LT mb[TS * LDS_PITCH];                     // mask keys of the tile
     __shared__ __align__(16) LT tb[Stage<T>::elems];      // u8 block staging, then transposes
     __shared__ uint32_t hs[4 * TS];     // halo keys before the sweeps
-    int n;
-    const int32_t* list = (round & 1) ? A.list1 : A.list0;
-    if (FIRST) n = A.ntiles;
-    else n = A.counts[round % 3];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        A.counts[(round + 2) % 3] = 0;
-        if (!FIRST && n > 0) {
-            A.stats[0] += 1;
-            A.stats[1] += n;
+    const int lane = threadIdx.x;
+    if (blockIdx.x == 0) {
+        if (lane < NSEG) A.counts[cnt_idx(round + 2, lane)] = 0;
+        if (!FIRST && lane == 0) {
+            int n = 0;
+            for (int q = 0; q < NSEG; ++q) n += A.counts[cnt_idx(round, q)];
+            if (n > 0) {
+                A.stats[0] += 1;
+                A.stats[1] += n;
+            }
         }
     }
+    if (FIRST) {
+        for (int i = blockIdx.x; i < A.ntiles; i += gridDim.x) {
+            ImgDesc D;
+            int lt;
+            tile_of(A, i, D, lt);
+            process_tile<T, MODE, CONN, FIRST>(A, D, lt, round, mb, tb, hs);
+            __syncthreads();   // the next tile overwrites mb
+        }
+        return;
+    }
+    // lane q < NSEG: segment q's count and the inclusive prefix over segments
+    const uint32_t cq = lane < NSEG ? (uint32_t)A.counts[cnt_idx(round, lane)] : 0u;
+    const uint32_t incl = wave_incl_scan(cq);
+    const int n = __builtin_amdgcn_readlane((int)incl, NSEG - 1);
+    const int32_t* list = (round & 1) ? A.list1 : A.list0;
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
-        const int t = FIRST ? i : list[i];
+        const int q = __builtin_ctzll(__ballot(lane < NSEG && incl > (uint32_t)i));
+        const int off = i - (__builtin_amdgcn_readlane((int)incl, q) - __builtin_amdgcn_readlane((int)cq, q));
+        const int t = __builtin_amdgcn_readfirstlane(list[(int64_t)q * A.ntiles + off]);
         ImgDesc D;
         int lt;
         tile_of(A, t, D, lt);
@@ -648,14 +690,19 @@ void recon_round_kernel(ReconArgs A, int round) {
     }
 }
 
-__global__ void copy_word_kernel(int32_t* dst, const int32_t* src) {
-    if (threadIdx.x == 0) *dst = *src;
+// dst = sum of one round's NSEG counters (zero iff the round has no work).
+__global__ void sum_counts_kernel(int32_t* dst, const int32_t* round_counts) {
+    if (threadIdx.x == 0) {
+        int n = 0;
+        for (int q = 0; q < NSEG; ++q) n += round_counts[q * CSTRIDE];
+        *dst = n;
+    }
 }
 
 __global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts, int64_t* stats) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     for (int j = i; j < ntiles; j += gridDim.x * blockDim.x) stamp[j] = 0u;
-    if (i < 3) counts[i] = 0;
+    if (i < 3 * NSEG) counts[i * CSTRIDE] = 0;
     if (i == 0) { stats[0] = 0; stats[1] = 0; stats[2] = ntiles; stats[3] = 0; }
 }
 
@@ -734,6 +781,8 @@ int recon_tiles(int rows, int cols) {
 
 size_t recon_desc_bytes() { return sizeof(ImgDesc); }
 
+size_t recon_list_bytes(int ntiles) { return 2 * sizeof(int32_t) * (size_t)NSEG * (size_t)ntiles; }
+
 void recon_fill_desc(void* out, const View* marker, const View& mask, const View& state, int tile_base) {
     *reinterpret_cast<ImgDesc*>(out) = make_desc(marker, mask, state, tile_base);
 }
@@ -769,12 +818,12 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     A.tile_img = B.tile_img;
     void *stamp, *lists, *counts;
     nsx_status st = ctx->ws(B.ws_base + 0, sizeof(uint32_t) * (size_t)A.ntiles, &stamp);
-    if (st == NSX_OK) st = ctx->ws(B.ws_base + 1, 2 * sizeof(int32_t) * (size_t)A.ntiles, &lists);
-    if (st == NSX_OK) st = ctx->ws(B.ws_base + 2, 64, &counts);
+    if (st == NSX_OK) st = ctx->ws(B.ws_base + 1, recon_list_bytes(A.ntiles), &lists);
+    if (st == NSX_OK) st = ctx->ws(B.ws_base + 2, sizeof(int32_t) * (3 * NSEG * CSTRIDE + 64), &counts);
     if (st != NSX_OK) { delete R; return st; }
     A.stamp = (uint32_t*)stamp;
     A.list0 = (int32_t*)lists;
-    A.list1 = (int32_t*)lists + A.ntiles;
+    A.list1 = (int32_t*)lists + (size_t)NSEG * A.ntiles;
     A.counts = (int32_t*)counts;
     A.stats = B.stats ? B.stats : ctx->d_stats;
     A.visit_sweeps = std::max(1, std::min(MAX_IT, ctx->visit_sweeps));
@@ -803,7 +852,7 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     R->next_round = hint + 1;
     NSX_HIP_TRY(ctx, hipGetLastError());
     if (pending_dev)   // a kernel rather than a copy node, so the sequence can be graph-captured
-        hipLaunchKernelGGL(copy_word_kernel, dim3(1), dim3(64), 0, s, pending_dev, &A.counts[R->next_round % 3]);
+        hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, pending_dev, &A.counts[cnt_idx(R->next_round, 0)]);
     NSX_HIP_TRY(ctx, hipGetLastError());
     return NSX_OK;
 }
@@ -815,8 +864,9 @@ nsx_status recon_complete(nsx_ctx* ctx, ReconRun* run, hipStream_t s, bool* exte
     const ReconArgs& A = R->A;
     int batch = 2;
     while (true) {
-        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, &A.counts[R->next_round % 3], sizeof(int32_t),
-                                        hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, &A.counts[CNT_SUM],
+                           &A.counts[cnt_idx(R->next_round, 0)]);
+        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, &A.counts[CNT_SUM], sizeof(int32_t), hipMemcpyDeviceToHost, s));
         NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
         if (ctx->h_pinned[0] == 0) break;
         if (extended) *extended = true;

@@ -249,3 +249,18 @@ def test_more_than_int32_pixels_rejected(nh, gpu):
     assert ctx.lib.nsx_bwlabel(ctx.h, C.byref(big), C.byref(bigl), 8, 1, C.byref(n), None) == L.NSX_ERR_INVALID
     assert ctx.lib.nsx_imreconstruct(ctx.h, C.byref(big), C.byref(big), C.byref(big), 8, None) == L.NSX_ERR_INVALID
     assert b"2^31" in ctx.lib.nsx_last_error(ctx.h)
+
+
+@pytest.mark.parametrize("cols", [7936, 7937])
+def test_recon_fill_large(nh, orc, gpu, cols):
+    """3968 x 7936 is exactly 8192 reconstruction tiles (62 x 62); one more
+    column adds a partial tile column (8256 tiles).  Both give the oracle's
+    recon and fillHoles."""
+    import torch
+    mask = nh.gen_disks_u8(3968, cols, 2000, 0x5EED0202)
+    marker = nh.gen_marker_u8(mask, 30)
+    r = nh.imreconstruct(torch.from_numpy(marker).cuda(), torch.from_numpy(mask).cuda(), 8).cpu().numpy()
+    want = orc.imreconstruct(marker, mask, 8)
+    assert np.array_equal(r, want)
+    f = nh.imfillHoles(torch.from_numpy(want).cuda(), False, 4).cpu().numpy()
+    assert np.array_equal(f, orc.fill_holes(want, False, 4))
