@@ -538,8 +538,8 @@ TILE_PX = 62 * 62   # interior pixels of one reconstruction tile visit
 # launch of the kind.  FETCH_SIZE is scaled by 2.0 and WRITE_SIZE by 1.0: the
 # factors that make `--workload copy` (256 MiB read + 256 MiB written per
 # launch) read true.  Commands and raw figures: profiles/traffic_chain4096.md.
-TRAFFIC_CHAIN4096 = {"recon_round": 18486001, "recon_first": 104510512, "ccl_local": 68011315,
-                     "ccl_final": 101627904, "ccl_merge": 40422400}
+TRAFFIC_CHAIN4096 = {"recon_round": 19825756, "recon_first": 111417675, "ccl_local": 68011315,
+                     "ccl_final": 101627699, "ccl_merge": 40415846}
 
 
 def roofline(kt: dict, px: int, visits: tuple, steps: int, traffic: dict | None = None) -> dict:
