@@ -378,6 +378,7 @@ struct ReconArgs {
     int32_t* counts;           // [3][NSEG] rotating round counters, CSTRIDE ints apart
     int64_t* stats;            // [0] rounds with work [1] tile visits after round 0 [2] tiles
     int visit_sweeps;          // sweep pairs per visit before re-queueing (1..MAX_IT)
+    int first_list;            // round 0 takes its tiles from the list (fill: the border tiles)
 };
 
 // The tile id is wave-uniform; readfirstlane makes that visible to the
@@ -663,7 +664,7 @@ void recon_round_kernel(ReconArgs A, int round) {
             }
         }
     }
-    if (FIRST) {
+    if (FIRST && !A.first_list) {
         for (int i = blockIdx.x; i < A.ntiles; i += gridDim.x) {
             ImgDesc D;
             int lt;
@@ -699,11 +700,41 @@ __global__ void sum_counts_kernel(int32_t* dst, const int32_t* round_counts) {
     }
 }
 
-__global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts, int64_t* stats) {
+// Number of tiles on the border of an ntx x nty tile grid.
+__host__ __device__ inline int border_tiles(int ntx, int nty) {
+    return (ntx <= 2 || nty <= 2) ? ntx * nty : 2 * ntx + 2 * (nty - 2);
+}
+
+// Clears the stamps, the round counters and the stats.  With ntx > 0 it also
+// lists the border tiles of a single ntx x nty image as round 0's work.
+__global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts, int64_t* stats, int32_t* list0,
+                                   int ntx, int nty) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     for (int j = i; j < ntiles; j += gridDim.x * blockDim.x) stamp[j] = 0u;
-    if (i < 3 * NSEG) counts[i * CSTRIDE] = 0;
+    const int nb = ntx > 0 ? border_tiles(ntx, nty) : 0;
+    for (int b = i; b < nb; b += gridDim.x * blockDim.x) {
+        int t;
+        if (ntx <= 2 || nty <= 2) t = b;
+        else if (b < ntx) t = b;                                   // top row
+        else if (b < 2 * ntx) t = (nty - 1) * ntx + (b - ntx);     // bottom row
+        else {                                                     // left / right column
+            const int q = b - 2 * ntx;
+            t = (1 + q / 2) * ntx + ((q & 1) ? ntx - 1 : 0);
+        }
+        list0[b] = t;
+    }
+    if (i < 3 * NSEG) counts[i * CSTRIDE] = i == 0 ? nb : 0;
     if (i == 0) { stats[0] = 0; stats[1] = 0; stats[2] = ntiles; stats[3] = 0; }
+}
+
+// Writes v into every pixel of an image (the fill's initial state).
+template <typename T>
+__global__ void fill_state_kernel(char* state, int64_t ps, int rows, int cols, T v) {
+    const int64_t n = (int64_t)rows * cols;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        const int y = (int)(j / cols), x = (int)(j - (int64_t)y * cols);
+        st<T>(state, ps, y, x, v);
+    }
 }
 
 typedef void (*RoundFn)(const ReconArgs&, int, int, hipStream_t);
@@ -836,10 +867,27 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     R->key = (bucket << 16) ^ ((uint64_t)mode << 8) ^ ((uint64_t)dtype << 4) ^ (uint64_t)conn;
     run->impl = R;
 
+    // Hole filling starts from the minimum everywhere inside the frame, so in
+    // round 0 only the tiles that touch the frame can change: the others are
+    // written with the initial state by one fill kernel and join when activated.
+    const bool border_start = (mode == RM_FILL || mode == RM_FILLBIN) && B.nimg == 1;
+    const ImgDesc& D0 = A.img0;
+    A.first_list = border_start ? 1 : 0;
     const int grid_reset = std::max(1, std::min(1024, (A.ntiles + 255) / 256));
-    hipLaunchKernelGGL(recon_reset_kernel, dim3(grid_reset), dim3(256), 0, s, A.stamp, A.ntiles, A.counts, A.stats);
+    hipLaunchKernelGGL(recon_reset_kernel, dim3(grid_reset), dim3(256), 0, s, A.stamp, A.ntiles, A.counts, A.stats,
+                       A.list0, border_start ? D0.ntx : 0, D0.nty);
+    if (border_start) {
+        const int grid_fill = std::max(1, std::min(4 * ctx->cu_count, (int)(((int64_t)D0.rows * D0.cols + 255) / 256)));
+        // the stored state of the initial key: invert(min) = max for both fill modes
+        if (dtype == NSX_U8)
+            hipLaunchKernelGGL(fill_state_kernel<uint8_t>, dim3(grid_fill), dim3(256), 0, s, D0.state, D0.ps, D0.rows,
+                               D0.cols, (uint8_t)255);
+        else
+            hipLaunchKernelGGL(fill_state_kernel<int32_t>, dim3(grid_fill), dim3(256), 0, s, D0.state, D0.ps, D0.rows,
+                               D0.cols, (int32_t)INT_MAX);
+    }
     const int h0 = ctx->tic(TK_RECON_FIRST, s);
-    first(A, 0, A.ntiles, s);
+    first(A, 0, border_start ? std::min(R->grid, border_tiles(D0.ntx, D0.nty)) : A.ntiles, s);
     ctx->toc(h0, s);
     // Rounds the same-sized problem needed last time, launched without a host check.
     auto it = ctx->round_hint.find(R->key);
