@@ -533,8 +533,8 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
     p.use_area = true;
     p.min_size = min_size;
     p.max_size = max_size;
-    // d_stats words: int32 [12] components, [13] recon tiles still queued,
-    // [14] fill tiles still queued; int64 [8..10] the fill stage's statistics.
+    // d_stats words: int32 [12] components, int64 [8..11] the fill stage's
+    // statistics, int32 [32..39] the status gathered for the host.
     int32_t* dw = (int32_t*)ctx->d_stats;
     const int rconn = recon_conn == 8 ? 8 : 4, fconn = fill_conn == 8 ? 8 : 4;
     std::vector<uint8_t> dr(recon_desc_bytes()), df(recon_desc_bytes());
@@ -551,17 +551,19 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
     ReconRun rr, rf;
     // All stages are enqueued back to back; convergence of both propagations
     // is checked once, together with the component count.
-    NSX_TRY(recon_launch(ctx, RM_RECON, mask->dtype, rconn, br, s, &rr, dw + 13));
-    NSX_TRY(recon_launch(ctx, RM_FILL, mask->dtype, fconn, bf, s, &rf, dw + 14));
+    NSX_TRY(recon_launch(ctx, RM_RECON, mask->dtype, rconn, br, s, &rr, nullptr));
+    NSX_TRY(recon_launch(ctx, RM_FILL, mask->dtype, fconn, bf, s, &rf, nullptr));
     NSX_TRY(ccl_run(ctx, vf, vl, p, dw + 12, s));
-    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 12, dw + 12, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 16, ctx->d_stats, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 18, ctx->d_stats + 8, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    recon_status_gather(&rr, &rf, dw + 12, ctx->d_stats, ctx->d_stats + 8, dw + 32, s);
+    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, dw + 32, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
-    if (ctx->h_pinned[13] != 0 || ctx->h_pinned[14] != 0) {
+    int32_t status[8];
+    std::memcpy(status, ctx->h_pinned + 40, sizeof(status));
+    ctx->h_pinned[12] = status[0];
+    if (status[1] != 0 || status[2] != 0) {
         // a propagation needed more rounds than predicted: finish it and redo
         // the stages downstream of it
-        if (ctx->h_pinned[13] != 0) {
+        if (status[1] != 0) {
             NSX_TRY(recon_complete(ctx, &rr, s, nullptr));
             NSX_TRY(recon_launch(ctx, RM_FILL, mask->dtype, fconn, bf, s, &rf, nullptr));
         }
@@ -571,8 +573,8 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
         NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
     } else {
         int64_t wr, wf;   // rounds that had work, for the next call's prediction
-        std::memcpy(&wr, ctx->h_pinned + 16, sizeof(wr));
-        std::memcpy(&wf, ctx->h_pinned + 18, sizeof(wf));
+        std::memcpy(&wr, status + 4, sizeof(wr));
+        std::memcpy(&wf, status + 6, sizeof(wf));
         recon_note_rounds(ctx, &rr, wr);
         recon_note_rounds(ctx, &rf, wf);
     }

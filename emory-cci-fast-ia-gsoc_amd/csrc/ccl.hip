@@ -475,7 +475,7 @@ __global__ __launch_bounds__(256) void scan_words_kernel(const uint32_t* bits, i
 
 // K5b: one block of 1024 threads scans the block totals in place (exclusive)
 // and stores the grand total at blocksum[nb].
-__global__ __launch_bounds__(1024) void scan_blocks_kernel(uint32_t* blocksum, int nb) {
+__device__ __forceinline__ void scan_blocks(uint32_t* blocksum, int nb) {
     __shared__ uint32_t part[1024];
     const int tid = threadIdx.x;
     const int per = (nb + 1023) / 1024;
@@ -501,9 +501,14 @@ __global__ __launch_bounds__(1024) void scan_blocks_kernel(uint32_t* blocksum, i
     if (tid == 1023) blocksum[nb] = part[1023];
 }
 
-// K5c: kept components per image from the block prefix.
-__global__ void ccl_count_kernel(CclArgs A) {
-    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < A.nimg; b += gridDim.x * blockDim.x) {
+__global__ __launch_bounds__(1024) void scan_blocks_kernel(uint32_t* blocksum, int nb) { scan_blocks(blocksum, nb); }
+
+// K5b + K5c: the block scan, then the kept components per image from the
+// block prefix (one block: its own writes are visible after the barrier).
+__global__ __launch_bounds__(1024) void scan_blocks_count_kernel(CclArgs A) {
+    scan_blocks(A.blockpre, A.nblocks);
+    __syncthreads();
+    for (int b = threadIdx.x; b < A.nimg; b += blockDim.x) {
         const CclImg D = A.nimg == 1 ? A.img0 : A.imgs[b];
         const int n = D.rows * D.cols;
         const int blk0 = D.pix_base / BLOCK_BITS;
@@ -615,8 +620,7 @@ void ccl_launch_conn(nsx_ctx* ctx, const CclArgs& A, hipStream_t s) {
     if (A.use_area) hipLaunchKernelGGL(ccl_area_kernel, dim3(A.ntiles), dim3(256), 0, s, A);
     hipLaunchKernelGGL(ccl_keep_kernel, dim3(A.ntiles), dim3(256), 0, s, A);
     hipLaunchKernelGGL(scan_words_kernel, dim3(A.nblocks), dim3(256), 0, s, A.bits, A.nwords, A.wordpre, A.blockpre);
-    hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, s, A.blockpre, A.nblocks);
-    hipLaunchKernelGGL(ccl_count_kernel, dim3(std::max(1, (A.nimg + 255) / 256)), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(scan_blocks_count_kernel, dim3(1), dim3(1024), 0, s, A);
     ctx->toc(h, s);
     h = ctx->tic(TK_CCL_FINAL, s);
     hipLaunchKernelGGL((ccl_final_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(CTHREADS), 0, s, A);

@@ -183,6 +183,12 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
 nsx_status recon_complete(nsx_ctx* ctx, ReconRun* run, hipStream_t s, bool* extended);
 // For a run already known to have converged: records how many rounds had work
 // (read by the caller from the run's stats word 0) and releases the run.
+// One kernel that gathers a chain's status for a single copy: out[0] = *ncomp,
+// out[1], out[2] = tiles still queued by rr and rf (0 when their predicted
+// rounds sufficed), int64 out[2], out[3] = rstats[0], fstats[0].  Both runs
+// must hold a launch (non-empty images).
+void recon_status_gather(const ReconRun* rr, const ReconRun* rf, const int32_t* ncomp, const int64_t* rstats,
+                         const int64_t* fstats, int32_t* out, hipStream_t s);
 void recon_note_rounds(nsx_ctx* ctx, ReconRun* run, int64_t rounds_with_work);
 
 // ---- connected components ---------------------------------------------------------

@@ -705,11 +705,27 @@ __host__ __device__ inline int border_tiles(int ntx, int nty) {
     return (ntx <= 2 || nty <= 2) ? ntx * nty : 2 * ntx + 2 * (nty - 2);
 }
 
+// The initial state written into every pixel when a hole filling starts from
+// its border tiles: invert(min) = max, one byte (u8) or one int32.
+struct StateFill {
+    char* state; int64_t ps;
+    int rows, cols;   // rows == 0: nothing to write
+    int esize;        // 1 or 4
+    uint32_t value;
+};
+
 // Clears the stamps, the round counters and the stats.  With ntx > 0 it also
-// lists the border tiles of a single ntx x nty image as round 0's work.
+// lists the border tiles of a single ntx x nty image as round 0's work, and
+// writes the initial state F.
 __global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts, int64_t* stats, int32_t* list0,
-                                   int ntx, int nty) {
+                                   int ntx, int nty, StateFill F) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t npx = (int64_t)F.rows * F.cols;
+    for (int64_t j = i; j < npx; j += (int64_t)gridDim.x * blockDim.x) {
+        const int y = (int)(j / F.cols), x = (int)(j - (int64_t)y * F.cols);
+        if (F.esize == 1) st<uint8_t>(F.state, F.ps, y, x, (uint8_t)F.value);
+        else st<int32_t>(F.state, F.ps, y, x, (int32_t)F.value);
+    }
     for (int j = i; j < ntiles; j += gridDim.x * blockDim.x) stamp[j] = 0u;
     const int nb = ntx > 0 ? border_tiles(ntx, nty) : 0;
     for (int b = i; b < nb; b += gridDim.x * blockDim.x) {
@@ -727,15 +743,28 @@ __global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts,
     if (i == 0) { stats[0] = 0; stats[1] = 0; stats[2] = ntiles; stats[3] = 0; }
 }
 
-// Writes v into every pixel of an image (the fill's initial state).
-template <typename T>
-__global__ void fill_state_kernel(char* state, int64_t ps, int rows, int cols, T v) {
-    const int64_t n = (int64_t)rows * cols;
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
-        const int y = (int)(j / cols), x = (int)(j - (int64_t)y * cols);
-        st<T>(state, ps, y, x, v);
+// out[0] = *ncomp; out[1], out[2] = tiles queued after the launched rounds of
+// two reconstructions (the sums of their next round's counters); int64 out[2],
+// out[3] = their rounds with work.
+__global__ void chain_status_kernel(int32_t* out, const int32_t* ncomp, const int32_t* rc, const int32_t* fc,
+                                    const int64_t* rs, const int64_t* fs) {
+    if (threadIdx.x == 0) {
+        int a = 0, b = 0;
+        for (int q = 0; q < NSEG; ++q) {
+            a += rc[q * CSTRIDE];
+            b += fc[q * CSTRIDE];
+        }
+        out[0] = *ncomp;
+        out[1] = a;
+        out[2] = b;
+        out[3] = 0;
+        int64_t* o64 = reinterpret_cast<int64_t*>(out);
+        o64[2] = rs[0];
+        o64[3] = fs[0];
     }
 }
+
+
 
 typedef void (*RoundFn)(const ReconArgs&, int, int, hipStream_t);
 
@@ -873,19 +902,16 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     const bool border_start = (mode == RM_FILL || mode == RM_FILLBIN) && B.nimg == 1;
     const ImgDesc& D0 = A.img0;
     A.first_list = border_start ? 1 : 0;
-    const int grid_reset = std::max(1, std::min(1024, (A.ntiles + 255) / 256));
-    hipLaunchKernelGGL(recon_reset_kernel, dim3(grid_reset), dim3(256), 0, s, A.stamp, A.ntiles, A.counts, A.stats,
-                       A.list0, border_start ? D0.ntx : 0, D0.nty);
+    StateFill F{};
+    int grid_reset = std::max(1, std::min(1024, (A.ntiles + 255) / 256));
     if (border_start) {
-        const int grid_fill = std::max(1, std::min(4 * ctx->cu_count, (int)(((int64_t)D0.rows * D0.cols + 255) / 256)));
         // the stored state of the initial key: invert(min) = max for both fill modes
-        if (dtype == NSX_U8)
-            hipLaunchKernelGGL(fill_state_kernel<uint8_t>, dim3(grid_fill), dim3(256), 0, s, D0.state, D0.ps, D0.rows,
-                               D0.cols, (uint8_t)255);
-        else
-            hipLaunchKernelGGL(fill_state_kernel<int32_t>, dim3(grid_fill), dim3(256), 0, s, D0.state, D0.ps, D0.rows,
-                               D0.cols, (int32_t)INT_MAX);
+        F = StateFill{D0.state, D0.ps, D0.rows, D0.cols, dtype == NSX_U8 ? 1 : 4,
+                      dtype == NSX_U8 ? 255u : (uint32_t)INT_MAX};
+        grid_reset = std::max(grid_reset, std::min(4 * ctx->cu_count, (int)(((int64_t)D0.rows * D0.cols + 255) / 256)));
     }
+    hipLaunchKernelGGL(recon_reset_kernel, dim3(grid_reset), dim3(256), 0, s, A.stamp, A.ntiles, A.counts, A.stats,
+                       A.list0, border_start ? D0.ntx : 0, D0.nty, F);
     const int h0 = ctx->tic(TK_RECON_FIRST, s);
     first(A, 0, border_start ? std::min(R->grid, border_tiles(D0.ntx, D0.nty)) : A.ntiles, s);
     ctx->toc(h0, s);
@@ -936,6 +962,14 @@ nsx_status recon_complete(nsx_ctx* ctx, ReconRun* run, hipStream_t s, bool* exte
     note_hint(ctx, R->key, st[0]);
     run->release();
     return NSX_OK;
+}
+
+void recon_status_gather(const ReconRun* rr, const ReconRun* rf, const int32_t* ncomp, const int64_t* rstats,
+                         const int64_t* fstats, int32_t* out, hipStream_t s) {
+    const ReconRunImpl* a = rr->impl;
+    const ReconRunImpl* b = rf->impl;
+    hipLaunchKernelGGL(chain_status_kernel, dim3(1), dim3(64), 0, s, out, ncomp,
+                       &a->A.counts[cnt_idx(a->next_round, 0)], &b->A.counts[cnt_idx(b->next_round, 0)], rstats, fstats);
 }
 
 void recon_note_rounds(nsx_ctx* ctx, ReconRun* run, int64_t rounds_with_work) {
