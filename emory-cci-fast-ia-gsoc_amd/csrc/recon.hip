@@ -271,6 +271,26 @@ __device__ __forceinline__ bool sweep_pair(uint32_t (&c)[TS], const LT* mb) {
     return kSum ? after != before : (before | after) != 0u;
 }
 
+// 32-bit keys with the lane's mask column (row in orientation B) held in
+// registers for the whole visit (no LDS mask reads).
+template <int CONN>
+__device__ __forceinline__ bool sweep_pair_reg(uint32_t (&c)[TS], const uint32_t (&m)[TS]) {
+    uint32_t chg = 0;
+    uint32_t p = c[0];
+#pragma unroll
+    for (int k = 1; k < TS; ++k) {
+        c[k] = med3_chg(nbr<CONN>(p), c[k], m[k], chg);
+        p = c[k];
+    }
+    p = c[TS - 1];
+#pragma unroll
+    for (int k = TS - 2; k >= 0; --k) {
+        c[k] = med3_chg(nbr<CONN>(p), c[k], m[k], chg);
+        p = c[k];
+    }
+    return chg != 0u;
+}
+
 // Register tile transpose through LDS: lane l, register k <-> lane k, register l.
 template <typename LT>
 __device__ __forceinline__ void transpose(uint32_t (&c)[TS], LT* lds) {
@@ -437,6 +457,9 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
     const int x0 = tx * TI - 1, y0 = ty * TI - 1;
     const uint32_t F = C::frame();
     const bool needA = FIRST ? C::kMarkerFirst : C::kMarkerLater;
+    // 32-bit keys: the mask keys stay in registers, column and row orientation
+    constexpr bool kRegMask = sizeof(T) == 4;
+    uint32_t mA[kRegMask ? TS : 1], mB[kRegMask ? TS : 1];
 
     // Pixels outside the image read a clamped (valid) address and take the
     // frame value, so no load depends on a branch.
@@ -562,9 +585,15 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
             const bool in = col_in && (unsigned)(y0 + k) < (unsigned)D.rows;
             uint32_t mk;
             c[k] = key_of(in, rm[b][j], ra[b][j], rs[b][j], mk);
-            mb[k * LDS_PITCH + lane] = (LT)mk;
+            if constexpr (kRegMask) mA[k] = mk;
+            else mb[k * LDS_PITCH + lane] = (LT)mk;
         }
     }
+    }
+    if constexpr (kRegMask) {
+#pragma unroll
+        for (int k = 0; k < TS; ++k) mB[k] = mA[k];
+        transpose(mB, tb);
     }
     hs[lane] = c[0];
     hs[TS + lane] = c[TS - 1];
@@ -596,10 +625,16 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
         return CONN == 4 ? (quiet >= 1 && half >= 1) : (quiet >= 2);
     };
     for (int it = 0; it < A.visit_sweeps; ++it) {
-        if (settled(sweep_pair<CONN, false>(c, mb))) { converged = true; break; }   // columns (orientation A)
+        bool done;   // columns (orientation A)
+        if constexpr (kRegMask) done = settled(sweep_pair_reg<CONN>(c, mA));
+        else done = settled(sweep_pair<CONN, false>(c, mb));
+        if (done) { converged = true; break; }
         ++half;
         transpose(c, tb);
-        if (settled(sweep_pair<CONN, true>(c, mb))) { converged = true; orientB = true; break; }   // rows
+        // rows (orientation B)
+        if constexpr (kRegMask) done = settled(sweep_pair_reg<CONN>(c, mB));
+        else done = settled(sweep_pair<CONN, true>(c, mb));
+        if (done) { converged = true; orientB = true; break; }
         ++half;
         transpose(c, tb);
     }
@@ -646,10 +681,10 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
 // 10.3 KB of LDS and run 3 waves per SIMD (at most 168 VGPRs); 32-bit tiles
 // are LDS-bound at 1.
 template <typename T, int MODE, int CONN, bool FIRST>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 1 ? 3 : 1)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 1 ? 3 : 2)))
 void recon_round_kernel(ReconArgs A, int round) {
     using LT = typename LdsT<T>::type;
-    __shared__ LT mb[TS * LDS_PITCH];                     // mask keys of the tile
+    __shared__ LT mb[sizeof(T) == 1 ? TS * LDS_PITCH : 1];   // u8 mask keys of the tile
     __shared__ __align__(16) LT tb[Stage<T>::elems];      // u8 block staging, then transposes
     __shared__ uint32_t hs[4 * TS];     // halo keys before the sweeps
     const int lane = threadIdx.x;
@@ -888,8 +923,8 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     A.stats = B.stats ? B.stats : ctx->d_stats;
     A.visit_sweeps = std::max(1, std::min(MAX_IT, ctx->visit_sweeps));
     R->later = later;
-    // u8 tiles run 3 waves/SIMD (12 per CU), 32-bit tiles 1 (LDS-bound)
-    R->grid = std::max(1, std::min(A.ntiles, (dtype == NSX_U8 ? 12 : 4) * ctx->cu_count));
+    // u8 tiles run 3 waves/SIMD (12 per CU), 32-bit tiles 2
+    R->grid = std::max(1, std::min(A.ntiles, (dtype == NSX_U8 ? 12 : 8) * ctx->cu_count));
     // Round predictions are shared by problems within ~19% of each other in tile
     // count (quarter-octave buckets), so mixed request sizes still hit.
     const uint64_t bucket = (uint64_t)std::lround(4.0 * std::log2((double)A.ntiles));
