@@ -801,11 +801,298 @@ __global__ void chain_status_kernel(int32_t* out, const int32_t* ncomp, const in
 
 
 
+// ---- 4-connectivity tiles on four waves -------------------------------------------
+// For 4-connectivity a sweep step sets a register to med3(previous, cur, mask)
+// = clamp(previous, cur, mask) (cur <= mask), and clamps compose into clamps:
+//     clamp(clamp(x, a1, b1), a2, b2) = clamp(x, med3(a1, a2, b2), med3(b1, a2, b2)).
+// A tile is split over four waves of QR registers each (rows in orientation A,
+// columns in orientation B).  In a sweep the wave holding the starting halo
+// line runs its registers while the other three compose their QR clamps (two
+// independent chains); after one barrier each of them clamps its entry value
+// through the composed blocks on the way and reruns its registers.  A 63-step
+// dependent chain becomes 16 + up to 3 + 16 steps on four SIMDs.  Hole filling
+// with 4-connectivity uses this: its rounds hold few tiles each, so a round
+// lasts as long as one visit's dependent chain.
+constexpr int QW = 4;
+constexpr int QR = TS / QW;
+
+struct QShared {
+    uint32_t xch[2][(2 * QW - 1) * TS];   // per sweep direction: slot 0 the starting wave's exit,
+                                          // slots 2p-1, 2p the composed clamp at position p
+    uint32_t hs[4 * TS];               // halo keys before the sweeps
+    uint32_t flag[QW];                 // per wave: a register rose in this half-pass
+    uint32_t want[QW];                 // per wave: neighbour tiles to queue
+};
+
+// One sweep direction over the four waves.  DOWN: registers 0..QR-1 of waves
+// 0..3 in order, starting from wave 0's register 0; otherwise the reverse,
+// starting from wave 3's register QR-1.  One barrier, between the composing
+// and the rerun; each direction has its own exchange slots, and the next
+// writes to them come after the other direction's barrier.
+template <bool DOWN>
+__device__ __forceinline__ void qsweep(uint32_t (&c)[QR], const uint32_t (&m)[QR], int w, uint32_t* xch) {
+    const int lane = threadIdx.x & 63;
+    const int ws = DOWN ? 0 : QW - 1;
+    const int pos = DOWN ? w : QW - 1 - w;   // position along the sweep, 0 = starting wave
+    if (w == ws) {
+        uint32_t p = DOWN ? c[0] : c[QR - 1];
+#pragma unroll
+        for (int s = 1; s < QR; ++s) {
+            const int k = DOWN ? s : QR - 1 - s;
+            c[k] = med3u(p, c[k], m[k]);
+            p = c[k];
+        }
+        xch[lane] = p;
+    } else {
+        const int k0 = DOWN ? 0 : QR - 1;
+        uint32_t lo = c[k0], hi = m[k0];
+#pragma unroll
+        for (int s = 1; s < QR; ++s) {
+            const int k = DOWN ? s : QR - 1 - s;
+            lo = med3u(lo, c[k], m[k]);
+            hi = med3u(hi, c[k], m[k]);
+        }
+        xch[(2 * pos - 1) * TS + lane] = lo;
+        xch[(2 * pos) * TS + lane] = hi;
+    }
+    __syncthreads();
+    if (w != ws) {
+        uint32_t x = xch[lane];
+        for (int r = 1; r < pos; ++r) x = med3u(x, xch[(2 * r - 1) * TS + lane], xch[(2 * r) * TS + lane]);
+#pragma unroll
+        for (int s = 0; s < QR; ++s) {
+            const int k = DOWN ? s : QR - 1 - s;
+            c[k] = med3u(x, c[k], m[k]);
+            x = c[k];
+        }
+    }
+}
+
+// Orientation A (wave w: rows QR*w.., lane = column) <-> orientation B (wave
+// w: columns QR*w.., lane = row) through LDS.  TAIL: a closing barrier, for
+// when tb is written again before another barrier.
+template <bool TO_B, bool TAIL, typename LT>
+__device__ __forceinline__ void qtranspose(uint32_t (&c)[QR], LT* tb, int w) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < QR; ++k) {
+        const int posA = (QR * w + k) * LDS_PITCH + lane, posB = lane * LDS_PITCH + QR * w + k;
+        tb[TO_B ? posA : posB] = (LT)c[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < QR; ++k) {
+        const int posA = (QR * w + k) * LDS_PITCH + lane, posB = lane * LDS_PITCH + QR * w + k;
+        c[k] = tb[TO_B ? posB : posA];
+    }
+    if (TAIL) __syncthreads();
+}
+
+// A visit of one tile by four waves (4-connectivity; modes without a marker
+// in later rounds, i.e. the hole-filling modes).
+template <typename T, int MODE, bool FIRST>
+__device__ __forceinline__ void process_tile_q(const ReconArgs& A, const ImgDesc& D, int lt, int round,
+                                               typename LdsT<T>::type* tb, QShared& Q) {
+    using C = Codec<T, MODE>;
+    static_assert(!C::kMarkerLater, "four-wave tiles read mask and state only after round 0");
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ty = lt / D.ntx, tx = lt - ty * D.ntx;
+    const int x0 = tx * TI - 1, y0 = ty * TI - 1;
+    const uint32_t F = C::frame();
+    constexpr bool needA = FIRST && C::kMarkerFirst;
+    const int gx = x0 + lane;
+    const bool col_in = (unsigned)gx < (unsigned)D.cols;
+    const int cx = min(max(gx, 0), D.cols - 1);
+    // this wave's QR rows, every load in flight at once
+    uint32_t rm[QR], ra[QR], rs[QR];
+#pragma unroll
+    for (int k = 0; k < QR; ++k) {
+        const int cy = min(max(y0 + QR * w + k, 0), D.rows - 1);
+        rm[k] = bits<T>(ld<T>(D.mask, D.pk, cy, cx));
+        ra[k] = needA ? bits<T>(ld<T>(D.marker, D.pm, cy, cx)) : 0u;
+        rs[k] = FIRST ? 0u : bits<T>(ld<T>(D.state, D.ps, cy, cx));
+    }
+    uint32_t c[QR], mA[QR], mB[QR];
+#pragma unroll
+    for (int k = 0; k < QR; ++k) {
+        const bool in = col_in && (unsigned)(y0 + QR * w + k) < (unsigned)D.rows;
+        const T mv = unbits<T>(rm[k]), av = unbits<T>(ra[k]);
+        const uint32_t key = FIRST ? C::first(av, mv) : C::dec(unbits<T>(rs[k]), av, mv);
+        c[k] = in ? key : F;
+        mA[k] = in ? C::mkey(av, mv) : F;
+    }
+#pragma unroll
+    for (int k = 0; k < QR; ++k) mB[k] = mA[k];
+    qtranspose<true, false>(mB, tb, w);
+    if (w == 0) Q.hs[lane] = c[0];
+    if (w == QW - 1) Q.hs[TS + lane] = c[QR - 1];
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < QR; ++k) Q.hs[2 * TS + QR * w + k] = c[k];
+    }
+    if (lane == TS - 1) {
+#pragma unroll
+        for (int k = 0; k < QR; ++k) Q.hs[3 * TS + QR * w + k] = c[k];
+    }
+    __syncthreads();
+
+    // As in the one-wave visit: 4-connectivity half-passes are idempotent, so
+    // one quiet half-pass after the other orientation ran is the fixed point.
+    bool converged = false, orientB = false, changed = false;
+    int quiet = 0, half = 0;
+    // A half-pass: both sweeps; then each wave writes its "a register rose"
+    // flag and its registers for the transpose, and one barrier serves both.
+    // A converged visit keeps its registers and skips the transpose's reads.
+    // (Flags and tb are written again only after the next half-pass's two
+    // sweep barriers, which every reader passes first.)
+    auto half_pass = [&](const uint32_t (&m)[QR], bool toB) {
+        uint32_t before[QR];
+#pragma unroll
+        for (int k = 0; k < QR; ++k) before[k] = c[k];
+        qsweep<true>(c, m, w, Q.xch[0]);
+        qsweep<false>(c, m, w, Q.xch[1]);
+        uint32_t d = 0;
+#pragma unroll
+        for (int k = 0; k < QR; ++k) d |= c[k] ^ before[k];
+        const bool mine = __ballot(d != 0u) != 0ull;   // every lane votes
+        if (lane == 0) Q.flag[w] = mine ? 1u : 0u;
+#pragma unroll
+        for (int k = 0; k < QR; ++k) {
+            const int posA = (QR * w + k) * LDS_PITCH + lane, posB = lane * LDS_PITCH + QR * w + k;
+            tb[toB ? posA : posB] = (typename LdsT<T>::type)c[k];
+        }
+        __syncthreads();
+        const bool moved = (Q.flag[0] | Q.flag[1] | Q.flag[2] | Q.flag[3]) != 0u;
+        changed = changed || moved;
+        quiet = moved ? 0 : quiet + 1;
+        const bool done = quiet >= 1 && half >= 1;
+        if (!done) {
+#pragma unroll
+            for (int k = 0; k < QR; ++k) {
+                const int posA = (QR * w + k) * LDS_PITCH + lane, posB = lane * LDS_PITCH + QR * w + k;
+                c[k] = tb[toB ? posB : posA];
+            }
+        }
+        return done;
+    };
+    for (int it = 0; it < A.visit_sweeps; ++it) {
+        if (half_pass(mA, true)) { converged = true; break; }   // columns (orientation A), then to B
+        ++half;
+        if (half_pass(mB, false)) { converged = true; orientB = true; break; }   // rows, then back to A
+        ++half;
+    }
+    if (threadIdx.x == 0) atomicAdd((unsigned long long*)&A.stats[3], (unsigned long long)(half + 1));
+    // left / right halo columns: orientation B's register 0 of wave 0 and
+    // register QR-1 of the last wave
+    uint32_t finL, finR;
+    if (orientB) {
+        finL = c[0];
+        finR = c[QR - 1];
+        qtranspose<false, true>(c, tb, w);
+    } else {
+        qtranspose<true, true>(c, tb, w);
+        finL = c[0];
+        finR = c[QR - 1];
+        qtranspose<false, true>(c, tb, w);
+    }
+
+    if (FIRST || changed) {
+        if (lane >= 1 && lane <= TI && gx < D.cols) {
+#pragma unroll
+            for (int k = 0; k < QR; ++k) {
+                const int r = QR * w + k, gy = y0 + r;
+                if (r >= 1 && r <= TI && gy < D.rows) st<T>(D.state, D.ps, gy, gx, C::enc(c[k], T(0)));
+            }
+        }
+    }
+
+    // halo pixels that rose above what was read: their owners must re-visit
+    const uint64_t mid = 0x7FFFFFFFFFFFFFFEull;
+    uint32_t wb = 0;
+    if (w == 0) {
+        const uint64_t topm = __ballot(c[0] > Q.hs[lane]);
+        const uint64_t lefm = __ballot(finL > Q.hs[2 * TS + lane]);
+        wb = ((topm & mid) ? 1u : 0u) | ((topm & 1ull) ? 2u : 0u) | ((topm >> 63) ? 4u : 0u) | ((lefm & mid) ? 64u : 0u);
+    } else if (w == QW - 1) {
+        const uint64_t botm = __ballot(c[QR - 1] > Q.hs[TS + lane]);
+        const uint64_t rigm = __ballot(finR > Q.hs[3 * TS + lane]);
+        wb = ((botm & mid) ? 8u : 0u) | ((botm & 1ull) ? 16u : 0u) | ((botm >> 63) ? 32u : 0u) |
+             ((rigm & mid) ? 128u : 0u);
+    }
+    if (lane == 0) Q.want[w] = wb;
+    __syncthreads();
+    if (w == 0) {
+        const uint32_t want = Q.want[0] | Q.want[QW - 1] | (converged ? 0u : 256u);
+        if (want) activate_all(A, D, ty, tx, round, want);
+    }
+}
+
+template <typename T, int MODE, bool FIRST>
+__global__ __launch_bounds__(QW * 64) void recon_quad_kernel(ReconArgs A, int round) {
+    using LT = typename LdsT<T>::type;
+    __shared__ LT tb[TS * LDS_PITCH];   // transposes
+    __shared__ QShared Q;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (blockIdx.x == 0 && w == 0) {
+        if (lane < NSEG) A.counts[cnt_idx(round + 2, lane)] = 0;
+        if (!FIRST && lane == 0) {
+            int n = 0;
+            for (int q = 0; q < NSEG; ++q) n += A.counts[cnt_idx(round, q)];
+            if (n > 0) {
+                A.stats[0] += 1;
+                A.stats[1] += n;
+            }
+        }
+    }
+    if (FIRST && !A.first_list) {
+        for (int i = blockIdx.x; i < A.ntiles; i += gridDim.x) {
+            ImgDesc D;
+            int lt;
+            tile_of(A, i, D, lt);
+            process_tile_q<T, MODE, FIRST>(A, D, lt, round, tb, Q);
+            __syncthreads();
+        }
+        return;
+    }
+    const uint32_t cq = lane < NSEG ? (uint32_t)A.counts[cnt_idx(round, lane)] : 0u;
+    const uint32_t incl = wave_incl_scan(cq);
+    const int n = __builtin_amdgcn_readlane((int)incl, NSEG - 1);
+    const int32_t* list = (round & 1) ? A.list1 : A.list0;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int q = __builtin_ctzll(__ballot(lane < NSEG && incl > (uint32_t)i));
+        const int off = i - (__builtin_amdgcn_readlane((int)incl, q) - __builtin_amdgcn_readlane((int)cq, q));
+        const int t = __builtin_amdgcn_readfirstlane(list[(int64_t)q * A.ntiles + off]);
+        ImgDesc D;
+        int lt;
+        tile_of(A, t, D, lt);
+        process_tile_q<T, MODE, FIRST>(A, D, lt, round, tb, Q);
+        __syncthreads();
+    }
+}
+
 typedef void (*RoundFn)(const ReconArgs&, int, int, hipStream_t);
 
 template <typename T, int MODE, int CONN, bool FIRST>
 void launch_round(const ReconArgs& A, int round, int grid, hipStream_t s) {
     hipLaunchKernelGGL((recon_round_kernel<T, MODE, CONN, FIRST>), dim3(grid), dim3(64), 0, s, A, round);
+}
+
+template <typename T, int MODE, bool FIRST>
+void launch_quad(const ReconArgs& A, int round, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((recon_quad_kernel<T, MODE, FIRST>), dim3(grid), dim3(QW * 64), 0, s, A, round);
+}
+
+// Hole filling with 4-connectivity runs on four-wave tiles.
+template <typename T, int MODE>
+bool pick_fill(int conn, RoundFn* first, RoundFn* later) {
+    if (conn == 8) {
+        *first = launch_round<T, MODE, 8, true>;
+        *later = launch_round<T, MODE, 8, false>;
+    } else {
+        *first = launch_quad<T, MODE, true>;
+        *later = launch_quad<T, MODE, false>;
+    }
+    return true;
 }
 
 template <typename T, int MODE>
@@ -831,11 +1118,11 @@ bool select_fns(int mode, int dtype, int conn, RoundFn* first, RoundFn* later) {
             if (dtype == NSX_I32) return pick<int32_t, RM_BINARY>(conn, first, later);
             return false;
         case RM_FILL:
-            if (dtype == NSX_U8) return pick<uint8_t, RM_FILL>(conn, first, later);
-            if (dtype == NSX_I32) return pick<int32_t, RM_FILL>(conn, first, later);
+            if (dtype == NSX_U8) return pick_fill<uint8_t, RM_FILL>(conn, first, later);
+            if (dtype == NSX_I32) return pick_fill<int32_t, RM_FILL>(conn, first, later);
             return false;
         case RM_FILLBIN:
-            if (dtype == NSX_U8) return pick<uint8_t, RM_FILLBIN>(conn, first, later);
+            if (dtype == NSX_U8) return pick_fill<uint8_t, RM_FILLBIN>(conn, first, later);
             return false;
         default:
             return false;
