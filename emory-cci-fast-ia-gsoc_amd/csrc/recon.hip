@@ -945,15 +945,25 @@ __device__ __forceinline__ void process_tile_q(const ReconArgs& A, const ImgDesc
     // A converged visit keeps its registers and skips the transpose's reads.
     // (Flags and tb are written again only after the next half-pass's two
     // sweep barriers, which every reader passes first.)
+    // u8 keys: values only rise and 16 x 255 cannot wrap, so a lane changed iff
+    // its sum rose; 32-bit keys compare against a copy.
     auto half_pass = [&](const uint32_t (&m)[QR], bool toB) {
-        uint32_t before[QR];
+        constexpr bool kSum = sizeof(T) == 1;
+        uint32_t before[kSum ? 1 : QR], s0 = 0;
 #pragma unroll
-        for (int k = 0; k < QR; ++k) before[k] = c[k];
+        for (int k = 0; k < QR; ++k) {
+            if (kSum) s0 += c[k];
+            else before[kSum ? 0 : k] = c[k];
+        }
         qsweep<true>(c, m, w, Q.xch[0]);
         qsweep<false>(c, m, w, Q.xch[1]);
-        uint32_t d = 0;
+        uint32_t d = 0, s1 = 0;
 #pragma unroll
-        for (int k = 0; k < QR; ++k) d |= c[k] ^ before[k];
+        for (int k = 0; k < QR; ++k) {
+            if (kSum) s1 += c[k];
+            else d |= c[k] ^ before[kSum ? 0 : k];
+        }
+        if (kSum) d = s1 ^ s0;
         const bool mine = __ballot(d != 0u) != 0ull;   // every lane votes
         if (lane == 0) Q.flag[w] = mine ? 1u : 0u;
 #pragma unroll
@@ -1027,8 +1037,10 @@ __device__ __forceinline__ void process_tile_q(const ReconArgs& A, const ImgDesc
     }
 }
 
+// u8: at most 128 VGPRs, so four tiles (16 waves) fit on a CU.
 template <typename T, int MODE, bool FIRST>
-__global__ __launch_bounds__(QW * 64) void recon_quad_kernel(ReconArgs A, int round) {
+__global__ __launch_bounds__(QW * 64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 1 ? 4 : 3)))
+void recon_quad_kernel(ReconArgs A, int round) {
     using LT = typename LdsT<T>::type;
     __shared__ LT tb[TS * LDS_PITCH];   // transposes
     __shared__ QShared Q;
