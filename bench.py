@@ -538,22 +538,31 @@ TILE_PX = 62 * 62   # interior pixels of one reconstruction tile visit
 # launch of the kind.  FETCH_SIZE is scaled by 2.0 and WRITE_SIZE by 1.0: the
 # factors that make `--workload copy` (256 MiB read + 256 MiB written per
 # launch) read true.  Commands and raw figures: profiles/traffic_chain4096.md.
-TRAFFIC_CHAIN4096 = {"recon_round": 19825756, "recon_first": 111417675, "ccl_local": 68011315,
-                     "ccl_final": 101627699, "ccl_merge": 40415846}
+TRAFFIC_CHAIN4096 = {"recon_round": 20151356, "recon_first": 64785396, "ccl_local": 68009984,
+                     "ccl_final": 101634253, "ccl_merge": 40455578}
 
 
-def roofline(kt: dict, px: int, visits: tuple, steps: int, traffic: dict | None = None) -> dict:
+def border_tiles(rows: int, cols: int) -> int:
+    """Reconstruction tiles (62 x 62) that touch the image border."""
+    ntx, nty = -(-cols // 62), -(-rows // 62)
+    return ntx * nty if ntx <= 2 or nty <= 2 else 2 * ntx + 2 * (nty - 2)
+
+
+def roofline(kt: dict, px: int, visits: tuple, steps: int, traffic: dict | None = None,
+             fill_first_tiles: int = 0) -> dict:
     """HBM roofline of the kernel kind with the most time per step.
 
     Algorithmic bytes per pixel (compulsory I/O of the operation a launch
     performs): reconstruction 3 B (marker, mask in; state out), fillHoles 2 B
     (image in; state out), labelling read 1 B, label output 1 B in + 4 B out.
-    A reconstruction first pass processes every pixel of the image; a
-    propagation round processes the tiles it visits (62x62 pixels each).
+    The reconstruction's first pass processes every pixel of the image; the
+    fill's first pass only the border tiles (the others start from a plain
+    fill of the output); a propagation round processes the tiles it visits
+    (62x62 pixels each).
     """
     v_recon, v_fill = visits
     alg_step = {
-        "recon_first": (3 + 2) * px,
+        "recon_first": 3 * px + 2 * fill_first_tiles * TILE_PX,
         "recon_round": (3 * v_recon + 2 * v_fill) * TILE_PX,
         "ccl_local": 1 * px,
         "ccl_final": 5 * px,
@@ -661,7 +670,7 @@ def main():
     log(f"recon stage {stats}; fill stage {fstats}")
     default_chain = n == 4096 and args.disks == 3000
     roof = roofline(kt, n * n, (stats["tile_visits"], fstats["tile_visits"]), args.steps,
-                    TRAFFIC_CHAIN4096 if default_chain else None)
+                    TRAFFIC_CHAIN4096 if default_chain else None, border_tiles(n, n))
     cpu, exact = (None, None) if args.no_cpu else cpu_baseline(marker, mask, args.cpu_runs, first)
     if exact is False:
         sys.exit(1)
