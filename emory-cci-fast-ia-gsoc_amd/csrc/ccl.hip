@@ -545,13 +545,15 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
         const uint16_t r = in ? A.lroot[D.pix_base + gy * D.cols + gx] : (uint16_t)0xFFFF;
         par[j] = r == 0xFFFF ? BGL : (uint32_t)r;
     }
-    // each local root resolves its global root once
+    // each local root resolves its global root once: the area kernel (level
+    // mode) or the keep kernel (otherwise) left every local root's `lab` entry
+    // pointing at its global root (a root points at itself), so one load does
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int ly = w + 4 * j;
         const int i = ly * CW + lx;
         if (par[j] != (uint32_t)i) continue;
-        const int32_t r = g_find(A.lab, D.pix_base + (y0 + ly) * D.cols + gx);
+        const int32_t r = A.lab[D.pix_base + (y0 + ly) * D.cols + gx];
         int32_t o;
         switch (A.out_mode) {
             case CO_ROOT: o = r - D.pix_base; break;
