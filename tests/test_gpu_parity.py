@@ -82,7 +82,8 @@ def test_recon_binary(nh, orc, gpu, dtype, conn):
 @pytest.mark.parametrize("binary", [False, True])
 def test_fill_holes_u8(nh, orc, gpu, conn, binary):
     rng = np.random.default_rng(7 + conn + 2 * binary)
-    for shape in [(1, 1), (5, 70), (62, 62), (128, 190)]:
+    # tile edges: 62 (8-connectivity) and 126 (4-connectivity) interior pixels
+    for shape in [(1, 1), (5, 70), (62, 62), (128, 190), (126, 126), (127, 253), (253, 127), (300, 380)]:
         bw = np.where(rng.random(shape) < 0.6, 255, 0).astype(np.uint8)
         assert np.array_equal(nh.imfillHoles(bw, binary, conn), orc.fill_holes(bw, binary, conn))
         g = rng.integers(0, 256, size=shape).astype(np.uint8)
