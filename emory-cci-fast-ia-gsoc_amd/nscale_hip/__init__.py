@@ -180,10 +180,25 @@ def _ctx(x, ctx):
     return default_context(dev)
 
 
+# HIP's handle of the legacy default stream (hipStreamLegacy).  torch reports
+# its default stream as handle 0, which the C library reads as "the context's
+# own stream" (a non-blocking stream, unordered with torch's work), so a torch
+# stream with handle 0 is passed as this handle instead.
+HIP_STREAM_LEGACY = 1
+
+
+def _torch_handle(ts) -> C.c_void_p:
+    h = ts.cuda_stream
+    return C.c_void_p(h if h else HIP_STREAM_LEGACY)
+
+
 def _stream_arg(stream):
+    """A torch stream object, a raw hipStream_t handle, or None (the context's stream)."""
     if stream is None:
         return None
-    return C.c_void_p(getattr(stream, "cuda_stream", stream))
+    if hasattr(stream, "cuda_stream"):
+        return _torch_handle(stream)
+    return C.c_void_p(stream)
 
 
 def _torch_stream(x, stream):
@@ -191,7 +206,7 @@ def _torch_stream(x, stream):
     if stream is not None:
         return _stream_arg(stream)
     if _is_device_tensor(x):
-        return C.c_void_p(_torch().cuda.current_stream(x.device).cuda_stream)
+        return _torch_handle(_torch().cuda.current_stream(x.device))
     return None
 
 
@@ -337,7 +352,7 @@ class Batch:
         self.ctx.check(self.lib.nsx_batch_finalize(self.h, int(bool(use_graph))))
 
     def run(self, stream=None) -> None:
-        s = _stream_arg(stream) if stream is not None else C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+        s = _stream_arg(stream) if stream is not None else _torch_handle(_torch().cuda.current_stream())
         self.ctx.check(self.lib.nsx_batch_run(self.h, s))
 
     def counts(self) -> list:
