@@ -804,6 +804,145 @@ __global__ __launch_bounds__(256) void levels_u8_kernel(const char* mask, int64_
     if (threadIdx.x == 0) atomicMax(mmax, lm);
 }
 
+// The same scan for a small value limit: each wave keeps the list of values it
+// has seen (wave-uniform, at most LV_KCAP) and checks every byte against it,
+// a few compares per byte instead of a 256-bit bitmap update.  A wave that
+// meets more than `kmax` distinct values raises *overflow and every wave stops
+// at its next row: the mask then has too many values for the level-set path.
+// Rows are read as 16-byte aligned blocks; bytes of an edge block outside the
+// row are replaced by a value already listed (mask) or by 0 (marker).  An
+// aligned 16-byte block holding a byte of the row lies in that byte's page.
+constexpr int LV_KCAP = 16;
+typedef uint32_t lv_u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short lv_u16x2 __attribute__((ext_vector_type(2)));
+
+// 0xFF in each byte k of word j (block bytes 4j..4j+3) with lo <= 4j+k < hi
+__device__ __forceinline__ uint32_t lv_keep(int lo, int hi, int j) {
+    uint32_t m = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) m |= (4 * j + k >= lo && 4 * j + k < hi) ? (0xFFu << (8 * k)) : 0u;
+    return m;
+}
+
+// Loads the block of row [start, end) with index i, zero for i >= nblk; the
+// bytes outside the row become `fill4`'s byte.
+__device__ __forceinline__ void lv_block(uintptr_t start, uintptr_t end, int i, int nblk, uint32_t fill4,
+                                         uint32_t (&w)[4]) {
+    const uintptr_t a = (start & ~(uintptr_t)15) + 16 * (uintptr_t)i;
+    if (i < nblk) {
+        const lv_u32x4 v = *(const lv_u32x4*)a;
+        w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+        if (a < start || a + 16 > end) {
+            const int lo = a < start ? (int)(start - a) : 0;
+            const int hi = a + 16 > end ? (int)(end - a) : 16;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t km = lv_keep(lo, hi, j);
+                w[j] = (w[j] & km) | (fill4 & ~km);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) w[j] = fill4;
+    }
+}
+
+__global__ __launch_bounds__(256) void levels_few_u8_kernel(const char* mask, int64_t pk, const char* marker,
+                                                            int64_t pm, int rows, int cols, int kmax,
+                                                            uint32_t* present, uint32_t* mmax, uint32_t* overflow) {
+    __shared__ uint32_t lp[8];
+    __shared__ uint32_t lm;
+    if (threadIdx.x < 8) lp[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) lm = 0u;
+    __syncthreads();
+    // the list starts with the image's first pixel, so a listed value exists
+    // to stand in for bytes outside the row
+    uint32_t sv[LV_KCAP];
+    const uint32_t first = *(const uint8_t*)mask;
+#pragma unroll
+    for (int j = 0; j < LV_KCAP; ++j) sv[j] = first;
+    int cnt = 1;
+    bool over = false;
+    lv_u16x2 ml = {0, 0}, mh = {0, 0};
+    for (int y = blockIdx.x; y < rows; y += gridDim.x) {
+        if (__builtin_amdgcn_readfirstlane(*(volatile uint32_t*)overflow)) {
+            over = true;
+            break;
+        }
+        const uintptr_t ms = (uintptr_t)(mask + (int64_t)y * pk), me = ms + (uintptr_t)cols;
+        const int mblk = (int)((me - (ms & ~(uintptr_t)15) + 15) >> 4);
+        for (int b0 = 0; b0 < mblk; b0 += (int)blockDim.x) {   // uniform trip count: every lane takes part
+            uint32_t w[4];
+            lv_block(ms, me, b0 + (int)threadIdx.x, mblk, sv[0] * 0x01010101u, w);
+            // high bit of each byte of um: the byte differs from every listed
+            // value.  (x & 0x7F..) + 0x7F.. sets a byte's high bit iff its low
+            // seven bits are not all zero, with no carry into the next byte.
+            uint32_t um[4] = {~0u, ~0u, ~0u, ~0u};
+#pragma unroll
+            for (int j = 0; j < LV_KCAP; ++j) {
+                if (j < cnt) {   // uniform
+                    const uint32_t s4 = sv[j] * 0x01010101u;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const uint32_t x = w[q] ^ s4;
+                        um[q] &= ((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x;
+                    }
+                }
+            }
+            const bool any = ((um[0] | um[1] | um[2] | um[3]) & 0x80808080u) != 0u;
+            if (__ballot(any) == 0ull) continue;
+            // rare: add the new values one at a time
+            for (int b = 0; b < 16 && !over; ++b) {
+                const uint32_t x = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+                while (true) {
+                    bool in = false;
+#pragma unroll
+                    for (int j = 0; j < LV_KCAP; ++j)
+                        if (j < cnt) in = in || x == sv[j];
+                    const uint64_t m = __ballot(!in);
+                    if (m == 0ull) break;
+                    if (cnt >= kmax) {
+                        over = true;
+                        break;
+                    }
+                    const uint32_t nv = (uint32_t)__builtin_amdgcn_readlane((int)x, (int)__builtin_ctzll(m));
+#pragma unroll
+                    for (int j = 0; j < LV_KCAP; ++j)
+                        if (j == cnt) sv[j] = nv;
+                    ++cnt;
+                }
+            }
+            if (over) break;
+        }
+        if (over) break;
+        const uintptr_t as = (uintptr_t)(marker + (int64_t)y * pm), ae = as + (uintptr_t)cols;
+        const int ablk = (int)((ae - (as & ~(uintptr_t)15) + 15) >> 4);
+        for (int i = (int)threadIdx.x; i < ablk; i += (int)blockDim.x) {
+            uint32_t w[4];
+            lv_block(as, ae, i, ablk, 0u, w);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                ml = __builtin_elementwise_max(ml, __builtin_bit_cast(lv_u16x2, w[j] & 0x00FF00FFu));
+                mh = __builtin_elementwise_max(mh, __builtin_bit_cast(lv_u16x2, (w[j] >> 8) & 0x00FF00FFu));
+            }
+        }
+    }
+    uint32_t m = max(max((uint32_t)ml.x, (uint32_t)ml.y), max((uint32_t)mh.x, (uint32_t)mh.y));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+    if ((threadIdx.x & 63) == 0) {
+        if (over) {
+            atomicOr(overflow, 1u);
+        } else {
+            atomicMax(&lm, m);
+            for (int j = 0; j < cnt; ++j) atomicOr(&lp[sv[j] >> 5], 1u << (sv[j] & 31u));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && lp[threadIdx.x]) atomicOr(&present[threadIdx.x], lp[threadIdx.x]);
+    if (threadIdx.x == 0) atomicMax(mmax, lm);
+}
+
 __global__ void zero_u32_kernel(uint32_t* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
@@ -823,15 +962,21 @@ nsx_status recon_levelset_u8(nsx_ctx* ctx, const View& marker, const View& mask,
                              int max_levels, bool* used, hipStream_t s) {
     *used = false;
     if (mask.dtype != NSX_U8 || max_levels <= 0 || (int64_t)mask.rows * mask.cols == 0) return NSX_OK;
-    // d_stats int32 words 40..47: value bitmap, 48: largest marker value
+    // d_stats int32 words 40..47: value bitmap, 48: largest marker value,
+    // 49: more distinct values than the level-set path takes
     uint32_t* d = (uint32_t*)ctx->d_stats + 40;
     const int grid = std::max(1, std::min(mask.rows, 4 * ctx->cu_count));
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, d, 9);
-    hipLaunchKernelGGL(levels_u8_kernel, dim3(grid), dim3(256), 0, s, mask.data, mask.pitch, marker.data,
-                       marker.pitch, mask.rows, mask.cols, d, d + 8);
-    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, d, 9 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, d, 10);
+    if (max_levels + 1 <= LV_KCAP)
+        hipLaunchKernelGGL(levels_few_u8_kernel, dim3(grid), dim3(256), 0, s, mask.data, mask.pitch, marker.data,
+                           marker.pitch, mask.rows, mask.cols, max_levels + 1, d, d + 8, d + 9);
+    else
+        hipLaunchKernelGGL(levels_u8_kernel, dim3(grid), dim3(256), 0, s, mask.data, mask.pitch, marker.data,
+                           marker.pitch, mask.rows, mask.cols, d, d + 8);
+    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, d, 10 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
     const uint32_t* h = (const uint32_t*)ctx->h_pinned + 40;
+    if (h[9] != 0u) return NSX_OK;
     std::vector<int> levels;   // descending
     for (int v = 255; v >= 0; --v)
         if ((h[v >> 5] >> (v & 31)) & 1u) levels.push_back(v);

@@ -67,3 +67,53 @@ def test_maze_recon_label_closed_form(nh, orc, gpu, shape):
     assert np.array_equal(r, np.where(corridors, 200, 20).astype(np.uint8))
     assert np.array_equal(lab, corridors.astype(np.int32))
     assert np.array_equal(r, orc.imreconstruct(marker, mask, 4))
+
+
+def test_levelset_values_split_between_waves(nh, orc, gpu):
+    """Each half of every row holds four values, the image seven: no single
+    scan wave sees more than the limit, the union exceeds it, and the rounds
+    path runs.  Both contexts give the oracle's result."""
+    rng = np.random.default_rng(23)
+    shape = (64, 4096)
+    mask = np.empty(shape, np.uint8)
+    mask[:, :2048] = rng.choice(np.array([0, 10, 20, 30], np.uint8), size=(64, 2048))
+    mask[:, 2048:] = rng.choice(np.array([0, 40, 50, 60], np.uint8), size=(64, 2048))
+    marker = np.where(rng.random(shape) < 0.02, rng.integers(0, 256, shape), 0).astype(np.uint8)
+    ls, sweep = contexts(nh)
+    want = orc.imreconstruct(marker, mask, 8)
+    assert np.array_equal(nh.imreconstruct(marker, mask, 8, ctx=ls), want)
+    assert np.array_equal(nh.imreconstruct(marker, mask, 8, ctx=sweep), want)
+    ls.close()
+    sweep.close()
+
+
+@pytest.mark.parametrize("x0,cols", [(3, 517), (16, 1024), (7, 15), (1, 1)])
+def test_levelset_unaligned_rows(nh, orc, gpu, x0, cols):
+    """Device views whose rows start and end off 16-byte boundaries (column
+    slices of wider, odd-pitched parents)."""
+    import torch
+    rng = np.random.default_rng(x0 * 1000 + cols)
+    rows = 211
+    marker, mask = few_level_case(rng, (rows, cols), (5, 90, 200))
+    pitch_m, pitch_k = x0 + cols + 13, x0 + cols + 29
+    big_m = torch.full((rows, pitch_m), 77, dtype=torch.uint8, device="cuda")   # 77: a value outside the image
+    big_k = torch.full((rows, pitch_k), 250, dtype=torch.uint8, device="cuda")  # above every marker value
+    big_m[:, x0:x0 + cols] = torch.from_numpy(mask).cuda()
+    big_k[:, x0:x0 + cols] = torch.from_numpy(marker).cuda()
+    ls, _ = contexts(nh)
+    for conn in (4, 8):
+        got = nh.imreconstruct(big_k[:, x0:x0 + cols], big_m[:, x0:x0 + cols], conn, ctx=ls).cpu().numpy()
+        assert np.array_equal(got, orc.imreconstruct(marker, mask, conn))
+    ls.close()
+
+
+def test_levelset_many_levels_setting(nh, orc, gpu):
+    """A level limit above the per-wave list (16 values) takes the bitmap scan."""
+    rng = np.random.default_rng(31)
+    levels = tuple(range(0, 200, 10))   # 20 values
+    marker, mask = few_level_case(rng, (150, 333), levels)
+    ctx = nh.Context(0)
+    ctx.set_tuning(1, 40)
+    for conn in (4, 8):
+        assert np.array_equal(nh.imreconstruct(marker, mask, conn, ctx=ctx), orc.imreconstruct(marker, mask, conn))
+    ctx.close()

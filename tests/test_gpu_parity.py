@@ -265,3 +265,37 @@ def test_recon_fill_large(nh, orc, gpu, cols):
     assert np.array_equal(r, want)
     f = nh.imfillHoles(torch.from_numpy(want).cuda(), False, 4).cpu().numpy()
     assert np.array_equal(f, orc.fill_holes(want, False, 4))
+
+
+def test_default_stream_ordering(nh, orc, gpu):
+    """Device tensors on torch's default stream: inputs written by torch just
+    before the call (behind a spin kernel that keeps the stream busy) are seen
+    by the call, and outputs are complete for torch's next reads."""
+    import torch
+    assert torch.cuda.current_stream().cuda_stream == 0
+    mask = nh.gen_disks_u8(300, 333, 80, 0x5EED0404)
+    marker = nh.gen_marker_u8(mask, 30)
+    dmask, dmarker = torch.from_numpy(mask).cuda(), torch.from_numpy(marker).cuda()
+    rng = np.random.default_rng(5)
+    few = rng.choice(np.array([5, 90, 200], np.uint8), size=(211, 517))
+    fmark = np.where(rng.random(few.shape) < 0.02, rng.integers(0, 256, few.shape), 0).astype(np.uint8)
+    dfew, dfmark = torch.from_numpy(few).cuda(), torch.from_numpy(fmark).cuda()
+    torch.cuda.synchronize()
+
+    def busy_then(*xs):
+        torch.cuda._sleep(20_000_000)   # the default stream stays busy ...
+        return [x.clone() for x in xs]  # ... and these copies land after it
+
+    k, m = busy_then(dmarker, dmask)
+    assert np.array_equal(nh.imreconstruct(k, m, 8).cpu().numpy(), orc.imreconstruct(marker, mask, 8))
+    k, m = busy_then(dfmark, dfew)   # few values: the level-set path
+    assert np.array_equal(nh.imreconstruct(k, m, 8).cpu().numpy(), orc.imreconstruct(fmark, few, 8))
+    (m,) = busy_then(dmask)
+    assert np.array_equal(nh.imfillHoles(m, False, 4).cpu().numpy(), orc.fill_holes(mask, False, 4))
+    bw = np.where(mask > 128, 255, 0).astype(np.uint8)
+    (b,) = busy_then(torch.from_numpy(bw).cuda())
+    assert np.array_equal(nh.bwlabel2(b, 8, True).cpu().numpy(), orc.bwlabel2(bw, 8, True)[0])
+    k, m = busy_then(dmarker, dmask)
+    r, f, lab, n = nh.segment_chain(k, m)
+    er, ef, elab, en = orc.chain_u8(marker, mask)
+    assert np.array_equal(lab.cpu().numpy(), elab) and n == en
