@@ -471,14 +471,23 @@ nsx_status nsx_recon_label(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* m
     NSX_TRY(to_device(ctx, mask, WS_STAGE_B, true, s, &vk));
     NSX_TRY(to_device(ctx, recon_out, WS_CHAIN_RECON, false, s, &vr));
     NSX_TRY(to_device(ctx, labels_out, WS_STAGE_C, false, s, &vl));
-    NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vr, conn == 8 ? 8 : 4, s));
-    CclParams p;
-    p.conn = ccl_conn == 8 ? 8 : 4;
-    p.out_mode = CO_CANON;
-    p.threshold = true;
-    p.thr = thresh;
     int32_t* dcount = (int32_t*)ctx->d_stats + 12;
-    NSX_TRY(ccl_run(ctx, vr, vl, p, dcount, s));
+    const int rconn = conn == 8 ? 8 : 4, lconn = ccl_conn == 8 ? 8 : 4;
+    // a u8 reconstruction through its level sets labels {recon > thresh} from
+    // its own labelling when both connectivities agree
+    LevelLabelReq req;
+    req.labels = vl;
+    req.thresh = thresh;
+    req.ncomp_dev = dcount;
+    NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vr, rconn, s, rconn == lconn ? &req : nullptr));
+    if (!req.done) {
+        CclParams p;
+        p.conn = lconn;
+        p.out_mode = CO_CANON;
+        p.threshold = true;
+        p.thr = thresh;
+        NSX_TRY(ccl_run(ctx, vr, vl, p, dcount, s));
+    }
     if (recon_out->mem == NSX_MEM_HOST) NSX_TRY(from_device(ctx, vr, recon_out, s));
     NSX_TRY(from_device(ctx, vl, labels_out, s));
     int32_t n = 0;

@@ -75,6 +75,7 @@ struct CclArgs {
     bool wmax;                 // `area` holds each component's maximum weight, not its size
     int level;                 // CO_LEVEL: out = max(out, min(level, maximum weight))
     int out_mode;
+    bool from_last;            // reuse the previous CO_LEVEL run's components
 };
 
 __device__ __forceinline__ void tile_of(const CclArgs& A, int t, CclImg& D, int& tx, int& ty) {
@@ -594,6 +595,22 @@ __global__ void zero_words_kernel(uint32_t* p, int64_t n) {
 
 template <int CONN, typename InT, bool THR>
 void ccl_launch_conn(nsx_ctx* ctx, const CclArgs& A, hipStream_t s) {
+    if (A.from_last) {
+        // components, roots (one hop from every local root) and per-root
+        // maxima are the previous CO_LEVEL run's: keep, rank, write
+        const int stream_grid = 4 * ctx->cu_count;
+        int h = ctx->tic(TK_CCL_SMALL, s);
+        hipLaunchKernelGGL(zero_words_kernel, dim3(stream_grid), dim3(256), 0, s, A.bits, A.nwords);
+        hipLaunchKernelGGL(ccl_keep_kernel, dim3(A.ntiles), dim3(256), 0, s, A);
+        hipLaunchKernelGGL(scan_words_kernel, dim3(A.nblocks), dim3(256), 0, s, A.bits, A.nwords, A.wordpre,
+                           A.blockpre);
+        hipLaunchKernelGGL(scan_blocks_count_kernel, dim3(1), dim3(1024), 0, s, A);
+        ctx->toc(h, s);
+        h = ctx->tic(TK_CCL_FINAL, s);
+        hipLaunchKernelGGL((ccl_final_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(CTHREADS), 0, s, A);
+        ctx->toc(h, s);
+        return;
+    }
     if (A.out_mode == CO_LEVEL) {
         // per-component maximum weight folded into a u8 image: no bitmap or ranks
         int h = ctx->tic(TK_CCL_LOCAL, s);
@@ -676,6 +693,9 @@ nsx_status ccl_launch(nsx_ctx* ctx, const CclBatch& B, const CclParams& p, hipSt
     A.min_size = p.min_size; A.max_size = p.max_size; A.use_area = p.use_area;
     A.wmax = p.out_mode == CO_LEVEL;
     A.level = p.level;
+    A.from_last = p.from_last;
+    if (p.from_last && (p.out_mode != CO_CANON || !p.use_area || B.nimg != 1))
+        return ctx->fail(NSX_ERR_INVALID, "relabelling the last level labelling needs CO_CANON with its maxima");
     if (A.wmax && (!p.threshold || B.in_dtype != NSX_U8 || B.nimg != 1))
         return ctx->fail(NSX_ERR_INVALID, "level labelling needs one thresholded u8 image");
     A.out_mode = p.out_mode;
@@ -943,6 +963,15 @@ __global__ __launch_bounds__(256) void levels_few_u8_kernel(const char* mask, in
     if (threadIdx.x == 0) atomicMax(mmax, lm);
 }
 
+// labels = v everywhere, *count = v (0 or 1 component)
+__global__ void fill_labels_kernel(char* out, int64_t po, int rows, int cols, int v, int32_t* count) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count = v;
+    for (int y = blockIdx.x; y < rows; y += gridDim.x) {
+        int32_t* r = (int32_t*)(out + (int64_t)y * po);
+        for (int x = threadIdx.x; x < cols; x += blockDim.x) r[x] = v;
+    }
+}
+
 __global__ void zero_u32_kernel(uint32_t* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
@@ -959,7 +988,7 @@ __global__ void level_floor_kernel(char* out, int64_t po, int rows, int cols, in
 }  // namespace
 
 nsx_status recon_levelset_u8(nsx_ctx* ctx, const View& marker, const View& mask, const View& out, int conn,
-                             int max_levels, bool* used, hipStream_t s) {
+                             int max_levels, bool* used, hipStream_t s, LevelLabelReq* req) {
     *used = false;
     if (mask.dtype != NSX_U8 || max_levels <= 0 || (int64_t)mask.rows * mask.cols == 0) return NSX_OK;
     // d_stats int32 words 40..47: value bitmap, 48: largest marker value,
@@ -983,8 +1012,49 @@ nsx_status recon_levelset_u8(nsx_ctx* ctx, const View& marker, const View& mask,
     if (levels.empty() || (int)levels.size() - 1 > max_levels) return NSX_OK;
     hipLaunchKernelGGL(level_floor_kernel, dim3(grid), dim3(256), 0, s, out.data, out.pitch, out.rows, out.cols,
                        levels.back(), d + 8);
-    for (size_t j = 0; j + 1 < levels.size(); ++j)
-        NSX_TRY(ccl_level_run(ctx, mask, marker, out, conn, levels[j], s));
+    // the smallest level above the labelling threshold (-1: none) runs last,
+    // so its components are still in the workspace for the labels
+    const int nl = (int)levels.size();
+    int lstar = -1;
+    if (req)
+        for (int j = nl - 1; j >= 0 && lstar < 0; --j)
+            if ((double)levels[j] > req->thresh) lstar = j;
+    for (int j = 0; j + 1 < nl; ++j)
+        if (j != lstar) NSX_TRY(ccl_level_run(ctx, mask, marker, out, conn, levels[j], s));
+    if (req) {
+        const View& L = req->labels;
+        if (lstar >= 0 && lstar + 1 < nl) {
+            NSX_TRY(ccl_level_run(ctx, mask, marker, out, conn, levels[lstar], s));
+            CclImg D;
+            ccl_fill_desc(&D, mask, L, 0, 0);
+            CclBatch B{};
+            B.nimg = 1;
+            B.ntiles = ccl_tiles(mask.rows, mask.cols);
+            B.pix_total = ccl_pix_span(mask.rows, mask.cols);
+            B.host_desc = &D;
+            B.in_dtype = NSX_U8;
+            B.ws_base = WS_CCL_LABEL;
+            B.ncomp_dev = req->ncomp_dev;
+            CclParams p;
+            p.conn = conn;
+            p.out_mode = CO_CANON;
+            p.threshold = true;
+            p.thr = levels[lstar] - 1;
+            p.use_area = true;   // kept: largest marker value > thresh
+            p.min_size = (int)std::floor(std::max(-1.0, std::min(256.0, req->thresh))) + 1;
+            p.max_size = INT32_MAX;
+            p.from_last = true;
+            NSX_TRY(ccl_launch(ctx, B, p, s));
+        } else {
+            // none above the threshold: no foreground; the smallest level: the
+            // whole (connected) image, foreground iff the largest marker value
+            // is above the threshold (h[8])
+            const bool all = lstar == nl - 1 && (double)h[8] > req->thresh;
+            hipLaunchKernelGGL(fill_labels_kernel, dim3(grid), dim3(256), 0, s, L.data, L.pitch, L.rows, L.cols,
+                               all ? 1 : 0, req->ncomp_dev);
+        }
+        req->done = true;
+    }
     NSX_HIP_TRY(ctx, hipGetLastError());
     *used = true;
     return NSX_OK;

@@ -148,8 +148,9 @@ enum ReconMode { RM_RECON = 0, RM_BINARY = 1, RM_FILL = 2, RM_FILLBIN = 3 };
 // Runs reconstruction in `mode` on device views (marker may be null for the
 // fill modes) to convergence.  `state` is the output buffer and holds the
 // propagation state.  Synchronises the stream at least once.
+struct LevelLabelReq;
 nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& state,
-                     int conn, hipStream_t s);
+                     int conn, hipStream_t s, LevelLabelReq* req = nullptr);
 
 // Several images of one dtype/mode/connectivity in one set of launches.
 struct ReconBatch {
@@ -209,6 +210,10 @@ struct CclParams {
     int max_size = 0;         // (exclusive)
     bool use_area = false;
     int level = 0;            // CO_LEVEL
+    // CO_CANON over the components, roots and per-root maxima that the previous
+    // CO_LEVEL run on the same workspace left (no new labelling); the area
+    // filter then applies to those maxima
+    bool from_last = false;
 };
 
 // Labels the image `in` (u8; or u8/i32/f32 with threshold) into `out`.
@@ -228,8 +233,19 @@ nsx_status ccl_level_run(nsx_ctx* ctx, const View& mask, const View& weight, con
 // L of min(L, largest marker value in the component of {mask >= L}), one
 // labelling per level above the smallest.  Reads the value set back (one
 // synchronisation); *used = false (nothing written) when there are more levels.
+// A labelling of {out > thresh} wanted together with a u8 reconstruction
+// (nsx_recon_label, same connectivity).  {R > t} is the union of the
+// components of {mask >= L*} whose largest marker value exceeds t, L* the
+// smallest mask value above t (none: empty; the smallest mask value: the whole
+// image), so the level-set path labels it from its own labelling of L*.
+struct LevelLabelReq {
+    View labels;              // int32 output
+    double thresh = 0.0;
+    int32_t* ncomp_dev = nullptr;
+    bool done = false;        // set when the labels (and the count) were enqueued
+};
 nsx_status recon_levelset_u8(nsx_ctx* ctx, const View& marker, const View& mask, const View& out, int conn,
-                             int max_levels, bool* used, hipStream_t s);
+                             int max_levels, bool* used, hipStream_t s, LevelLabelReq* req = nullptr);
 
 // Several images labelled by one set of launches.  Image b's pixels occupy a
 // global index range starting at its pix_base (a multiple of 32768, see

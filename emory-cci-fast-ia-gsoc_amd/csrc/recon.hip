@@ -1312,11 +1312,11 @@ void recon_note_rounds(nsx_ctx* ctx, ReconRun* run, int64_t rounds_with_work) {
 }
 
 nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& state,
-                     int conn, hipStream_t s) {
+                     int conn, hipStream_t s, LevelLabelReq* req) {
     if (mask.rows <= 0 || mask.cols <= 0) return NSX_OK;
     if (mode == RM_RECON && mask.dtype == NSX_U8 && marker && ctx->levelset_levels > 0) {
         bool used = false;
-        NSX_TRY(recon_levelset_u8(ctx, *marker, mask, state, conn == 8 ? 8 : 4, ctx->levelset_levels, &used, s));
+        NSX_TRY(recon_levelset_u8(ctx, *marker, mask, state, conn == 8 ? 8 : 4, ctx->levelset_levels, &used, s, req));
         if (used) return NSX_OK;
     }
     ImgDesc D = make_desc(marker, mask, state, 0);

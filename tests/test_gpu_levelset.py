@@ -117,3 +117,33 @@ def test_levelset_many_levels_setting(nh, orc, gpu):
     for conn in (4, 8):
         assert np.array_equal(nh.imreconstruct(marker, mask, conn, ctx=ctx), orc.imreconstruct(marker, mask, conn))
     ctx.close()
+
+
+@pytest.mark.parametrize("rconn,lconn", [(4, 4), (8, 8), (4, 8), (8, 4)])
+def test_recon_label_levels(nh, orc, gpu, rconn, lconn):
+    """nsx_recon_label on few-level masks: with equal connectivities the labels
+    come from the level sets' own labelling (threshold below every level, between
+    levels, at and above the largest; largest marker value below the threshold),
+    otherwise from a separate labelling.  Both must equal the oracle's
+    bwlabel2(recon > thresh)."""
+    import torch
+    rng = np.random.default_rng(41 + 10 * rconn + lconn)
+    cases = []
+    for levels, shape in [((5, 90, 200), (211, 517)), ((20, 200), (300, 300)), ((0, 60, 61, 250), (97, 640))]:
+        marker, mask = few_level_case(rng, shape, levels)
+        for t in (-3.0, levels[0] - 1.0, levels[0] + 0.5, float(levels[1]), levels[-1] - 0.5, float(levels[-1]), 300.0):
+            cases.append((marker, mask, t))
+    marker, mask = few_level_case(rng, (150, 170), (10, 100))
+    cases.append((np.minimum(marker, 3).astype(np.uint8), mask, 5.0))   # every marker value below the threshold
+    ls, sweep = contexts(nh)
+    for marker, mask, t in cases:
+        r = orc.imreconstruct(marker, mask, rconn)
+        want, wn = orc.bwlabel2(np.where(r > t, 255, 0).astype(np.uint8), lconn, True)
+        for c in (ls, sweep):
+            gr, gl, n = nh.recon_label(torch.from_numpy(marker).cuda(), torch.from_numpy(mask).cuda(), rconn, t, lconn,
+                                       ctx=c)
+            assert np.array_equal(gr.cpu().numpy(), r)
+            assert np.array_equal(gl.cpu().numpy(), want), (mask.shape, t)
+            assert n == wn
+    ls.close()
+    sweep.close()
