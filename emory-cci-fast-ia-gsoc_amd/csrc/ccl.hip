@@ -49,6 +49,7 @@ struct CclImg {
     const char* in; int64_t pin;
     char* out; int64_t pout;
     const char* wgt; int64_t pw;   // u8 weight image (per-component maximum mode)
+    char* lout; int64_t plout;     // int32 labels written with a CO_LEVEL output
     int rows, cols, ntx, nty;
     int tile_base;     // first tile of the image in the launch
     int pix_base;      // global index of pixel (0,0); multiple of BLOCK_BITS
@@ -75,7 +76,7 @@ struct CclArgs {
     bool wmax;                 // `area` holds each component's maximum weight, not its size
     int level;                 // CO_LEVEL: out = max(out, min(level, maximum weight))
     int out_mode;
-    bool from_last;            // reuse the previous CO_LEVEL run's components
+    bool level_labels;         // CO_LEVEL also writes canonical labels (kept: area filter on the maxima)
 };
 
 __device__ __forceinline__ void tile_of(const CclArgs& A, int t, CclImg& D, int& tx, int& ty) {
@@ -527,9 +528,12 @@ __device__ __forceinline__ int32_t rank1(const CclArgs& A, const CclImg& D, int3
                      __popc(w & ((1u << (r & 31)) - 1u))) + 1;
 }
 
-template <int CONN, typename InT, bool THR>
+// LL: CO_LEVEL that also writes the canonical labels of the kept components
+// to D.lout (0 for the others and the background).
+template <int CONN, typename InT, bool THR, bool LL = false>
 __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
     __shared__ int32_t outv[CN];
+    __shared__ int32_t outl[LL ? CN : 1];
     const int tid = threadIdx.x;
     CclImg D;
     int tx, ty;
@@ -564,6 +568,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
             default: o = kept(A, r) ? 255 : 0; break;
         }
         outv[i] = o;
+        if (LL) outl[i] = kept(A, r) ? rank1(A, D, r) : 0;
     }
     __syncthreads();
     const int32_t bg = (A.out_mode == CO_ROOT || A.out_mode == CO_ROOT_FILTERED) ? -1 : 0;
@@ -580,6 +585,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
                 uint8_t* p = (uint8_t*)row + gx;
                 *p = (uint8_t)max((int32_t)*p, outv[r]);
             }
+            if (LL) ((int32_t*)(D.lout + (int64_t)gy * D.plout))[gx] = r == BGL ? 0 : outl[r];
             continue;
         }
         const int32_t o = (r == BGL) ? bg : outv[r];
@@ -595,22 +601,6 @@ __global__ void zero_words_kernel(uint32_t* p, int64_t n) {
 
 template <int CONN, typename InT, bool THR>
 void ccl_launch_conn(nsx_ctx* ctx, const CclArgs& A, hipStream_t s) {
-    if (A.from_last) {
-        // components, roots (one hop from every local root) and per-root
-        // maxima are the previous CO_LEVEL run's: keep, rank, write
-        const int stream_grid = 4 * ctx->cu_count;
-        int h = ctx->tic(TK_CCL_SMALL, s);
-        hipLaunchKernelGGL(zero_words_kernel, dim3(stream_grid), dim3(256), 0, s, A.bits, A.nwords);
-        hipLaunchKernelGGL(ccl_keep_kernel, dim3(A.ntiles), dim3(256), 0, s, A);
-        hipLaunchKernelGGL(scan_words_kernel, dim3(A.nblocks), dim3(256), 0, s, A.bits, A.nwords, A.wordpre,
-                           A.blockpre);
-        hipLaunchKernelGGL(scan_blocks_count_kernel, dim3(1), dim3(1024), 0, s, A);
-        ctx->toc(h, s);
-        h = ctx->tic(TK_CCL_FINAL, s);
-        hipLaunchKernelGGL((ccl_final_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(CTHREADS), 0, s, A);
-        ctx->toc(h, s);
-        return;
-    }
     if (A.out_mode == CO_LEVEL) {
         // per-component maximum weight folded into a u8 image: no bitmap or ranks
         int h = ctx->tic(TK_CCL_LOCAL, s);
@@ -621,9 +611,20 @@ void ccl_launch_conn(nsx_ctx* ctx, const CclArgs& A, hipStream_t s) {
         ctx->toc(h, s);
         h = ctx->tic(TK_CCL_SMALL, s);
         hipLaunchKernelGGL(ccl_area_kernel, dim3(A.ntiles), dim3(256), 0, s, A);
+        if (A.level_labels) {
+            // keep and rank the components whose maximum passes the filter
+            hipLaunchKernelGGL(zero_words_kernel, dim3(4 * ctx->cu_count), dim3(256), 0, s, A.bits, A.nwords);
+            hipLaunchKernelGGL(ccl_keep_kernel, dim3(A.ntiles), dim3(256), 0, s, A);
+            hipLaunchKernelGGL(scan_words_kernel, dim3(A.nblocks), dim3(256), 0, s, A.bits, A.nwords, A.wordpre,
+                               A.blockpre);
+            hipLaunchKernelGGL(scan_blocks_count_kernel, dim3(1), dim3(1024), 0, s, A);
+        }
         ctx->toc(h, s);
         h = ctx->tic(TK_CCL_FINAL, s);
-        hipLaunchKernelGGL((ccl_final_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(CTHREADS), 0, s, A);
+        if (A.level_labels)
+            hipLaunchKernelGGL((ccl_final_kernel<CONN, InT, THR, true>), dim3(A.ntiles), dim3(CTHREADS), 0, s, A);
+        else
+            hipLaunchKernelGGL((ccl_final_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(CTHREADS), 0, s, A);
         ctx->toc(h, s);
         return;
     }
@@ -693,9 +694,9 @@ nsx_status ccl_launch(nsx_ctx* ctx, const CclBatch& B, const CclParams& p, hipSt
     A.min_size = p.min_size; A.max_size = p.max_size; A.use_area = p.use_area;
     A.wmax = p.out_mode == CO_LEVEL;
     A.level = p.level;
-    A.from_last = p.from_last;
-    if (p.from_last && (p.out_mode != CO_CANON || !p.use_area || B.nimg != 1))
-        return ctx->fail(NSX_ERR_INVALID, "relabelling the last level labelling needs CO_CANON with its maxima");
+    A.level_labels = A.wmax && A.use_area;
+    if (A.level_labels && !A.img0.lout)
+        return ctx->fail(NSX_ERR_INVALID, "level labels need a labels image");
     if (A.wmax && (!p.threshold || B.in_dtype != NSX_U8 || B.nimg != 1))
         return ctx->fail(NSX_ERR_INVALID, "level labelling needs one thresholded u8 image");
     A.out_mode = p.out_mode;
@@ -741,12 +742,16 @@ nsx_status ccl_run(nsx_ctx* ctx, const View& in, const View& out, const CclParam
 }
 
 nsx_status ccl_level_run(nsx_ctx* ctx, const View& mask, const View& weight, const View& out, int conn, int level,
-                         hipStream_t s) {
+                         hipStream_t s, const View* labels, double thresh, int32_t* ncomp_dev) {
     if ((int64_t)mask.rows * mask.cols == 0) return NSX_OK;
     CclImg D;
     ccl_fill_desc(&D, mask, out, 0, 0);
     D.wgt = weight.data;
     D.pw = weight.pitch;
+    if (labels) {
+        D.lout = labels->data;
+        D.plout = labels->pitch;
+    }
     CclBatch B{};
     B.nimg = 1;
     B.ntiles = ccl_tiles(mask.rows, mask.cols);
@@ -761,6 +766,13 @@ nsx_status ccl_level_run(nsx_ctx* ctx, const View& mask, const View& weight, con
     p.threshold = true;
     p.thr = level - 1;   // foreground: mask > level - 1, i.e. mask >= level
     p.level = level;
+    if (labels) {
+        // kept: maximum > thresh, i.e. >= floor(thresh) + 1
+        p.use_area = true;
+        p.min_size = (int)std::floor(std::max(-1.0, std::min(256.0, thresh))) + 1;
+        p.max_size = INT32_MAX;
+        B.ncomp_dev = ncomp_dev;
+    }
     return ccl_launch(ctx, B, p, s);
 }
 
@@ -1012,8 +1024,8 @@ nsx_status recon_levelset_u8(nsx_ctx* ctx, const View& marker, const View& mask,
     if (levels.empty() || (int)levels.size() - 1 > max_levels) return NSX_OK;
     hipLaunchKernelGGL(level_floor_kernel, dim3(grid), dim3(256), 0, s, out.data, out.pitch, out.rows, out.cols,
                        levels.back(), d + 8);
-    // the smallest level above the labelling threshold (-1: none) runs last,
-    // so its components are still in the workspace for the labels
+    // the smallest level above the labelling threshold (-1: none) also writes
+    // the labels, from its own components
     const int nl = (int)levels.size();
     int lstar = -1;
     if (req)
@@ -1024,27 +1036,7 @@ nsx_status recon_levelset_u8(nsx_ctx* ctx, const View& marker, const View& mask,
     if (req) {
         const View& L = req->labels;
         if (lstar >= 0 && lstar + 1 < nl) {
-            NSX_TRY(ccl_level_run(ctx, mask, marker, out, conn, levels[lstar], s));
-            CclImg D;
-            ccl_fill_desc(&D, mask, L, 0, 0);
-            CclBatch B{};
-            B.nimg = 1;
-            B.ntiles = ccl_tiles(mask.rows, mask.cols);
-            B.pix_total = ccl_pix_span(mask.rows, mask.cols);
-            B.host_desc = &D;
-            B.in_dtype = NSX_U8;
-            B.ws_base = WS_CCL_LABEL;
-            B.ncomp_dev = req->ncomp_dev;
-            CclParams p;
-            p.conn = conn;
-            p.out_mode = CO_CANON;
-            p.threshold = true;
-            p.thr = levels[lstar] - 1;
-            p.use_area = true;   // kept: largest marker value > thresh
-            p.min_size = (int)std::floor(std::max(-1.0, std::min(256.0, req->thresh))) + 1;
-            p.max_size = INT32_MAX;
-            p.from_last = true;
-            NSX_TRY(ccl_launch(ctx, B, p, s));
+            NSX_TRY(ccl_level_run(ctx, mask, marker, out, conn, levels[lstar], s, &L, req->thresh, req->ncomp_dev));
         } else {
             // none above the threshold: no foreground; the smallest level: the
             // whole (connected) image, foreground iff the largest marker value

@@ -210,10 +210,6 @@ struct CclParams {
     int max_size = 0;         // (exclusive)
     bool use_area = false;
     int level = 0;            // CO_LEVEL
-    // CO_CANON over the components, roots and per-root maxima that the previous
-    // CO_LEVEL run on the same workspace left (no new labelling); the area
-    // filter then applies to those maxima
-    bool from_last = false;
 };
 
 // Labels the image `in` (u8; or u8/i32/f32 with threshold) into `out`.
@@ -224,9 +220,12 @@ nsx_status ccl_run(nsx_ctx* ctx, const View& in, const View& out, const CclParam
 
 // One level of the level-set reconstruction of a u8 image: components of
 // {mask >= level} (conn), each folded into `out` as min(level, its maximum of
-// `weight`).  Enqueues work only.
+// `weight`).  With `labels`, the same pass also writes the canonical labels of
+// the components whose maximum exceeds `thresh` (0 elsewhere) and their count
+// to *ncomp_dev.  Enqueues work only.
 nsx_status ccl_level_run(nsx_ctx* ctx, const View& mask, const View& weight, const View& out, int conn, int level,
-                         hipStream_t s);
+                         hipStream_t s, const View* labels = nullptr, double thresh = 0.0,
+                         int32_t* ncomp_dev = nullptr);
 
 // Reconstruction by dilation of a u8 image through its level sets when the
 // mask holds at most max_levels + 1 distinct values: R = max over mask values
