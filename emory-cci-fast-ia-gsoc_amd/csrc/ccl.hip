@@ -41,6 +41,7 @@ constexpr int CW = 64;               // tile columns
 constexpr int CH = 32;               // tile rows
 constexpr int CN = CW * CH;          // pixels per tile (and table slots per tile)
 constexpr int CTHREADS = 256;
+static_assert(CN <= 65536 && CTHREADS <= CN, "slot packing: 16-bit local index and size");
 constexpr int CPT = CN / CTHREADS;   // pixels per thread
 constexpr uint32_t BGL = 0xFFFFFFFFu;
 constexpr int BLOCK_BITS = 32768;    // bitmap scan block (1024 words)
@@ -316,7 +317,9 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
         if (r == BGL) continue;
         if (r == (uint32_t)i) {
             const uint32_t k = atomicAdd(&nroots, 1u);
-            slots[k] = make_uint2((uint32_t)g, cnt[i]);
+            // .y: size or maximum weight (<= CN) in the low 16 bits, the
+            // tile-local index in the high 16
+            slots[k] = make_uint2((uint32_t)g, cnt[i] | ((uint32_t)i << 16));
             A.lab[g] = g;
             if (A.use_area || WMAX) A.area[g] = 0;
         }
@@ -408,15 +411,16 @@ __global__ __launch_bounds__(256) void ccl_area_kernel(CclArgs A) {
     const uint2* slots = A.entries + (int64_t)blockIdx.x * CN;
     for (int e = threadIdx.x; e < n; e += blockDim.x) {
         const uint2 en = slots[e];
+        const int v = (int)(en.y & 0xFFFFu);
         const int32_t r = g_find(A.lab, (int32_t)en.x);
         // the output kernel finds the root of this same local root again
         if (r != (int32_t)en.x) A.lab[en.x] = r;
         if (A.wmax) {
             // a large component has one slot per tile it crosses: only a slot
             // that can raise the maximum pays for the atomic on its root
-            if ((int)en.y > A.area[r]) atomicMax(&A.area[r], (int)en.y);
+            if (v > A.area[r]) atomicMax(&A.area[r], v);
         } else {
-            atomicAdd(&A.area[r], (int)en.y);
+            atomicAdd(&A.area[r], v);
         }
     }
 }
@@ -541,24 +545,29 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
     const int x0 = tx * CW, y0 = ty * CH;
     const int lx = tid & 63, w = tid >> 6;
     const int gx = x0 + lx;
-    // each pixel's tile-local root, as the local kernel left it
-    uint32_t par[CPT];
+    // each pixel's tile-local root, as the local kernel left it, and for a
+    // level output the value so far (loaded now, used after the barrier)
+    const bool level = A.out_mode == CO_LEVEL;
+    uint32_t par[CPT], prev[CPT];
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int gy = y0 + w + 4 * j;
         const bool in = gx < D.cols && gy < D.rows;
         const uint16_t r = in ? A.lroot[D.pix_base + gy * D.cols + gx] : (uint16_t)0xFFFF;
         par[j] = r == 0xFFFF ? BGL : (uint32_t)r;
+        prev[j] = (level && in) ? ((const uint8_t*)(D.out + (int64_t)gy * D.pout))[gx] : 0u;
     }
-    // each local root resolves its global root once: the area kernel (level
-    // mode) or the keep kernel (otherwise) left every local root's `lab` entry
-    // pointing at its global root (a root points at itself), so one load does
-#pragma unroll
-    for (int j = 0; j < CPT; ++j) {
-        const int ly = w + 4 * j;
-        const int i = ly * CW + lx;
-        if (par[j] != (uint32_t)i) continue;
-        const int32_t r = A.lab[D.pix_base + (y0 + ly) * D.cols + gx];
+    // each local root resolves its output once, one thread per component slot
+    // of the tile (the local kernel's list of its local roots); the area kernel
+    // (level mode) or the keep kernel (otherwise) left every local root's `lab`
+    // entry pointing at its global root (a root points at itself)
+    const int nslot = A.tcount[blockIdx.x];
+    const uint2* slots = A.entries + (int64_t)blockIdx.x * CN;
+    uint2 en = slots[tid];   // loaded with the count (a tile has CN slot places)
+    for (int k = tid; k < nslot; k += CTHREADS) {
+        if (k != tid) en = slots[k];
+        const int32_t r = A.lab[(int32_t)en.x];
+        const int i = (int)(en.y >> 16);
         int32_t o;
         switch (A.out_mode) {
             case CO_ROOT: o = r - D.pix_base; break;
@@ -581,10 +590,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
         char* row = D.out + (int64_t)gy * D.pout;
         if (A.out_mode == CO_LEVEL) {
             // u8 reconstruction accumulated over levels: foreground only
-            if (r != BGL) {
-                uint8_t* p = (uint8_t*)row + gx;
-                *p = (uint8_t)max((int32_t)*p, outv[r]);
-            }
+            if (r != BGL) ((uint8_t*)row)[gx] = (uint8_t)max((int32_t)prev[j], outv[r]);
             if (LL) ((int32_t*)(D.lout + (int64_t)gy * D.plout))[gx] = r == BGL ? 0 : outl[r];
             continue;
         }
