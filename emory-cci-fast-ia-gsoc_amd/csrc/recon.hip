@@ -1220,7 +1220,10 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     A.list1 = (int32_t*)lists + (size_t)NSEG * A.ntiles;
     A.counts = (int32_t*)counts;
     A.stats = B.stats ? B.stats : ctx->d_stats;
-    A.visit_sweeps = std::max(1, std::min(MAX_IT, ctx->visit_sweeps));
+    // four-wave hole-filling tiles run one pair more: their pairs are shorter
+    // (4096^2 chain: 36 fill rounds instead of 41, 0.8 % faster)
+    const bool quad = (mode == RM_FILL || mode == RM_FILLBIN) && conn == 4;
+    A.visit_sweeps = std::max(1, std::min(MAX_IT, ctx->visit_sweeps + (quad ? 1 : 0)));
     R->later = later;
     // u8 tiles run 3 waves/SIMD (12 per CU), 32-bit tiles 2
     R->grid = std::max(1, std::min(A.ntiles, (dtype == NSX_U8 ? 12 : 8) * ctx->cu_count));

@@ -84,7 +84,9 @@ nsx_status nsx_ctx_read_timing(nsx_ctx* ctx, double* ms, int64_t* launches, int 
 /* Tuning parameters of the context's kernels (results never depend on them).
  * NSX_TUNE_VISIT_SWEEPS: sweep pairs (column + row half-pass) a reconstruction
  * tile runs per visit before it re-queues itself unfinished, 1..64 (default
- * 4); small caps keep one slow tile from stretching a propagation round.
+ * 4; 4-connected hole filling, whose tiles run on four waves with shorter
+ * pairs, runs one pair more); small caps keep one slow tile from stretching a
+ * propagation round.
  * NSX_TUNE_LEVELSET_LEVELS: a u8 reconstruction (nsx_imreconstruct,
  * nsx_recon_label) whose mask holds at most this many distinct values above
  * its smallest one is computed through the mask's level sets, one
