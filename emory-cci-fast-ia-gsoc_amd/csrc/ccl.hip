@@ -41,7 +41,7 @@ constexpr int CW = 64;               // tile columns
 constexpr int CH = 32;               // tile rows
 constexpr int CN = CW * CH;          // pixels per tile (and table slots per tile)
 constexpr int CTHREADS = 256;
-static_assert(CN <= 65536 && CTHREADS <= CN, "slot packing: 16-bit local index and size");
+static_assert(CN <= 65536 && CTHREADS == 256, "u16 local indices; four waves per tile");
 constexpr int CPT = CN / CTHREADS;   // pixels per thread
 constexpr uint32_t BGL = 0xFFFFFFFFu;
 constexpr int BLOCK_BITS = 32768;    // bitmap scan block (1024 words)
@@ -317,9 +317,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
         if (r == BGL) continue;
         if (r == (uint32_t)i) {
             const uint32_t k = atomicAdd(&nroots, 1u);
-            // .y: size or maximum weight (<= CN) in the low 16 bits, the
-            // tile-local index in the high 16
-            slots[k] = make_uint2((uint32_t)g, cnt[i] | ((uint32_t)i << 16));
+            slots[k] = make_uint2((uint32_t)g, cnt[i]);
             A.lab[g] = g;
             if (A.use_area || WMAX) A.area[g] = 0;
         }
@@ -411,7 +409,7 @@ __global__ __launch_bounds__(256) void ccl_area_kernel(CclArgs A) {
     const uint2* slots = A.entries + (int64_t)blockIdx.x * CN;
     for (int e = threadIdx.x; e < n; e += blockDim.x) {
         const uint2 en = slots[e];
-        const int v = (int)(en.y & 0xFFFFu);
+        const int v = (int)en.y;
         const int32_t r = g_find(A.lab, (int32_t)en.x);
         // the output kernel finds the root of this same local root again
         if (r != (int32_t)en.x) A.lab[en.x] = r;
@@ -538,6 +536,7 @@ template <int CONN, typename InT, bool THR, bool LL = false>
 __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
     __shared__ int32_t outv[CN];
     __shared__ int32_t outl[LL ? CN : 1];
+    __shared__ uint16_t rlist[CN];   // local indices of the local roots, CN/4 per wave
     const int tid = threadIdx.x;
     CclImg D;
     int tx, ty;
@@ -557,17 +556,30 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
         par[j] = r == 0xFFFF ? BGL : (uint32_t)r;
         prev[j] = (level && in) ? ((const uint8_t*)(D.out + (int64_t)gy * D.pout))[gx] : 0u;
     }
-    // each local root resolves its output once, one thread per component slot
-    // of the tile (the local kernel's list of its local roots); the area kernel
+    // Each local root resolves its output once, one lane per local root: each
+    // wave compacts the roots among its pixels into its own LDS list (ballot
+    // per row) and resolves them with its lanes, so roots in several rows of a
+    // wave do not chain their global lookups row after row.  The area kernel
     // (level mode) or the keep kernel (otherwise) left every local root's `lab`
-    // entry pointing at its global root (a root points at itself)
-    const int nslot = A.tcount[blockIdx.x];
-    const uint2* slots = A.entries + (int64_t)blockIdx.x * CN;
-    uint2 en = slots[tid];   // loaded with the count (a tile has CN slot places)
-    for (int k = tid; k < nslot; k += CTHREADS) {
-        if (k != tid) en = slots[k];
-        const int32_t r = A.lab[(int32_t)en.x];
-        const int i = (int)(en.y >> 16);
+    // entry pointing at its global root (a root points at itself).
+    static_assert(CPT * 64 * (CTHREADS / 64) == CN, "one list region per wave");
+    uint16_t* wl = rlist + w * (CPT * 64);
+    uint32_t nw = 0;   // this wave's roots so far (wave-uniform)
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int i = (w + 4 * j) * CW + lx;
+        const uint64_t m = __ballot(par[j] == (uint32_t)i);
+        if (par[j] == (uint32_t)i) wl[nw + __builtin_popcountll(m & ((1ull << lx) - 1ull))] = (uint16_t)i;
+        nw += (uint32_t)__builtin_popcountll(m);
+    }
+    // the list is read back by other lanes of the same wave: LDS accesses of a
+    // wave complete in order, so a wave-scope fence suffices
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int k = lx; k < (int)nw; k += 64) {
+        const int i = wl[k];
+        const int32_t r = A.lab[D.pix_base + (y0 + i / CW) * D.cols + x0 + (i % CW)];
         int32_t o;
         switch (A.out_mode) {
             case CO_ROOT: o = r - D.pix_base; break;
