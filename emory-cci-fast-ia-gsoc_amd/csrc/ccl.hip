@@ -403,11 +403,20 @@ __global__ __launch_bounds__(128) void ccl_merge_kernel(CclArgs A) {
     }
 }
 
-// K3 / K4: one block per tile over the tile's component slots.
+// K3 / K4: over each tile's component slots.
+// SLOT_LANES lanes per tile, SLOT_TILES tiles per workgroup: tiles hold few
+// components, so a workgroup per tile would leave most lanes idle and cost a
+// dispatch per tile.
+constexpr int SLOT_LANES = 16;
+constexpr int SLOT_TILES = 256 / SLOT_LANES;
+inline int slot_grid(int ntiles) { return (ntiles + SLOT_TILES - 1) / SLOT_TILES; }
+
 __global__ __launch_bounds__(256) void ccl_area_kernel(CclArgs A) {
-    const int n = A.tcount[blockIdx.x];
-    const uint2* slots = A.entries + (int64_t)blockIdx.x * CN;
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const int t = (int)blockIdx.x * SLOT_TILES + (int)(threadIdx.x / SLOT_LANES);
+    if (t >= A.ntiles) return;
+    const int n = A.tcount[t];
+    const uint2* slots = A.entries + (int64_t)t * CN;
+    for (int e = (int)(threadIdx.x % SLOT_LANES); e < n; e += SLOT_LANES) {
         const uint2 en = slots[e];
         const int v = (int)en.y;
         const int32_t r = g_find(A.lab, (int32_t)en.x);
@@ -424,9 +433,11 @@ __global__ __launch_bounds__(256) void ccl_area_kernel(CclArgs A) {
 }
 
 __global__ __launch_bounds__(256) void ccl_keep_kernel(CclArgs A) {
-    const int n = A.tcount[blockIdx.x];
-    const uint2* slots = A.entries + (int64_t)blockIdx.x * CN;
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const int t = (int)blockIdx.x * SLOT_TILES + (int)(threadIdx.x / SLOT_LANES);
+    if (t >= A.ntiles) return;
+    const int n = A.tcount[t];
+    const uint2* slots = A.entries + (int64_t)t * CN;
+    for (int e = (int)(threadIdx.x % SLOT_LANES); e < n; e += SLOT_LANES) {
         const int32_t g = (int32_t)slots[e].x;
         const int32_t p = A.lab[g];
         if (p != g) {
@@ -628,11 +639,11 @@ void ccl_launch_conn(nsx_ctx* ctx, const CclArgs& A, hipStream_t s) {
         hipLaunchKernelGGL((ccl_merge_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(128), 0, s, A);
         ctx->toc(h, s);
         h = ctx->tic(TK_CCL_SMALL, s);
-        hipLaunchKernelGGL(ccl_area_kernel, dim3(A.ntiles), dim3(256), 0, s, A);
+        hipLaunchKernelGGL(ccl_area_kernel, dim3(slot_grid(A.ntiles)), dim3(256), 0, s, A);
         if (A.level_labels) {
             // keep and rank the components whose maximum passes the filter
             hipLaunchKernelGGL(zero_words_kernel, dim3(4 * ctx->cu_count), dim3(256), 0, s, A.bits, A.nwords);
-            hipLaunchKernelGGL(ccl_keep_kernel, dim3(A.ntiles), dim3(256), 0, s, A);
+            hipLaunchKernelGGL(ccl_keep_kernel, dim3(slot_grid(A.ntiles)), dim3(256), 0, s, A);
             hipLaunchKernelGGL(scan_words_kernel, dim3(A.nblocks), dim3(256), 0, s, A.bits, A.nwords, A.wordpre,
                                A.blockpre);
             hipLaunchKernelGGL(scan_blocks_count_kernel, dim3(1), dim3(1024), 0, s, A);
@@ -655,8 +666,8 @@ void ccl_launch_conn(nsx_ctx* ctx, const CclArgs& A, hipStream_t s) {
     hipLaunchKernelGGL((ccl_merge_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(128), 0, s, A);
     ctx->toc(h, s);
     h = ctx->tic(TK_CCL_SMALL, s);
-    if (A.use_area) hipLaunchKernelGGL(ccl_area_kernel, dim3(A.ntiles), dim3(256), 0, s, A);
-    hipLaunchKernelGGL(ccl_keep_kernel, dim3(A.ntiles), dim3(256), 0, s, A);
+    if (A.use_area) hipLaunchKernelGGL(ccl_area_kernel, dim3(slot_grid(A.ntiles)), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(ccl_keep_kernel, dim3(slot_grid(A.ntiles)), dim3(256), 0, s, A);
     hipLaunchKernelGGL(scan_words_kernel, dim3(A.nblocks), dim3(256), 0, s, A.bits, A.nwords, A.wordpre, A.blockpre);
     hipLaunchKernelGGL(scan_blocks_count_kernel, dim3(1), dim3(1024), 0, s, A);
     ctx->toc(h, s);
