@@ -346,12 +346,15 @@ __device__ __forceinline__ int32_t root_gidx(const CclImg& D, int qy, int qx, ui
 //   * the right column's up-right, when the pixel above is in the same local
 //     component: the right tile's left column joins that pair.
 template <int CONN, typename InT, bool THR>
-__global__ __launch_bounds__(128) void ccl_merge_kernel(CclArgs A) {
-    const int tid = threadIdx.x;
+__global__ __launch_bounds__(256) void ccl_merge_kernel(CclArgs A) {
+    // two tiles per workgroup, 128 threads (two whole waves) each
+    const int t = (int)blockIdx.x * 2 + (int)(threadIdx.x >> 7);
+    if (t >= A.ntiles) return;
+    const int tid = threadIdx.x & 127;
     const int lane = tid & 63;
     CclImg D;
     int tx, ty;
-    tile_of(A, blockIdx.x, D, tx, ty);
+    tile_of(A, t, D, tx, ty);
     const int x0 = tx * CW, y0 = ty * CH;
     const int kind = tid < 64 ? 0 : tid < 96 ? 1 : 2;   // top, left, right
     int y, x;
@@ -636,7 +639,7 @@ void ccl_launch_conn(nsx_ctx* ctx, const CclArgs& A, hipStream_t s) {
         hipLaunchKernelGGL((ccl_local_kernel<CONN, InT, THR, true>), dim3(A.ntiles), dim3(CTHREADS), 0, s, A);
         ctx->toc(h, s);
         h = ctx->tic(TK_CCL_MERGE, s);
-        hipLaunchKernelGGL((ccl_merge_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(128), 0, s, A);
+        hipLaunchKernelGGL((ccl_merge_kernel<CONN, InT, THR>), dim3((A.ntiles + 1) / 2), dim3(256), 0, s, A);
         ctx->toc(h, s);
         h = ctx->tic(TK_CCL_SMALL, s);
         hipLaunchKernelGGL(ccl_area_kernel, dim3(slot_grid(A.ntiles)), dim3(256), 0, s, A);
@@ -663,7 +666,7 @@ void ccl_launch_conn(nsx_ctx* ctx, const CclArgs& A, hipStream_t s) {
     hipLaunchKernelGGL((ccl_local_kernel<CONN, InT, THR, false>), dim3(A.ntiles), dim3(CTHREADS), 0, s, A);
     ctx->toc(h, s);
     h = ctx->tic(TK_CCL_MERGE, s);
-    hipLaunchKernelGGL((ccl_merge_kernel<CONN, InT, THR>), dim3(A.ntiles), dim3(128), 0, s, A);
+    hipLaunchKernelGGL((ccl_merge_kernel<CONN, InT, THR>), dim3((A.ntiles + 1) / 2), dim3(256), 0, s, A);
     ctx->toc(h, s);
     h = ctx->tic(TK_CCL_SMALL, s);
     if (A.use_area) hipLaunchKernelGGL(ccl_area_kernel, dim3(slot_grid(A.ntiles)), dim3(256), 0, s, A);
