@@ -78,6 +78,8 @@ struct CclArgs {
     int level;                 // CO_LEVEL: out = max(out, min(level, maximum weight))
     int out_mode;
     bool level_labels;         // CO_LEVEL also writes canonical labels (kept: area filter on the maxima)
+    int floor_level;           // CO_LEVEL with floor_mmax: writes every pixel (see CclParams)
+    const uint32_t* floor_mmax;
 };
 
 __device__ __forceinline__ void tile_of(const CclArgs& A, int t, CclImg& D, int& tx, int& ty) {
@@ -561,6 +563,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
     // each pixel's tile-local root, as the local kernel left it, and for a
     // level output the value so far (loaded now, used after the barrier)
     const bool level = A.out_mode == CO_LEVEL;
+    const bool fold = level && !A.floor_mmax;   // level value folded into `out`
     uint32_t par[CPT], prev[CPT];
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
@@ -568,7 +571,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
         const bool in = gx < D.cols && gy < D.rows;
         const uint16_t r = in ? A.lroot[D.pix_base + gy * D.cols + gx] : (uint16_t)0xFFFF;
         par[j] = r == 0xFFFF ? BGL : (uint32_t)r;
-        prev[j] = (level && in) ? ((const uint8_t*)(D.out + (int64_t)gy * D.pout))[gx] : 0u;
+        prev[j] = (fold && in) ? ((const uint8_t*)(D.out + (int64_t)gy * D.pout))[gx] : 0u;
     }
     // Each local root resolves its output once, one lane per local root: each
     // wave compacts the roots among its pixels into its own LDS list (ballot
@@ -607,6 +610,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
     }
     __syncthreads();
     const int32_t bg = (A.out_mode == CO_ROOT || A.out_mode == CO_ROOT_FILTERED) ? -1 : 0;
+    const int32_t floorv = (level && !fold) ? min(A.floor_level, (int32_t)*A.floor_mmax) : 0;
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int ly = w + 4 * j;
@@ -616,7 +620,8 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
         char* row = D.out + (int64_t)gy * D.pout;
         if (A.out_mode == CO_LEVEL) {
             // u8 reconstruction accumulated over levels: foreground only
-            if (r != BGL) ((uint8_t*)row)[gx] = (uint8_t)max((int32_t)prev[j], outv[r]);
+            if (!fold) ((uint8_t*)row)[gx] = (uint8_t)(r == BGL ? floorv : max(floorv, outv[r]));
+            else if (r != BGL) ((uint8_t*)row)[gx] = (uint8_t)max((int32_t)prev[j], outv[r]);
             if (LL) ((int32_t*)(D.lout + (int64_t)gy * D.plout))[gx] = r == BGL ? 0 : outl[r];
             continue;
         }
@@ -727,6 +732,8 @@ nsx_status ccl_launch(nsx_ctx* ctx, const CclBatch& B, const CclParams& p, hipSt
     A.wmax = p.out_mode == CO_LEVEL;
     A.level = p.level;
     A.level_labels = A.wmax && A.use_area;
+    A.floor_level = p.floor_level;
+    A.floor_mmax = A.wmax ? p.floor_mmax : nullptr;
     if (A.level_labels && !A.img0.lout)
         return ctx->fail(NSX_ERR_INVALID, "level labels need a labels image");
     if (A.wmax && (!p.threshold || B.in_dtype != NSX_U8 || B.nimg != 1))
@@ -774,7 +781,8 @@ nsx_status ccl_run(nsx_ctx* ctx, const View& in, const View& out, const CclParam
 }
 
 nsx_status ccl_level_run(nsx_ctx* ctx, const View& mask, const View& weight, const View& out, int conn, int level,
-                         hipStream_t s, const View* labels, double thresh, int32_t* ncomp_dev) {
+                         hipStream_t s, const View* labels, double thresh, int32_t* ncomp_dev, int floor_level,
+                         const uint32_t* floor_mmax) {
     if ((int64_t)mask.rows * mask.cols == 0) return NSX_OK;
     CclImg D;
     ccl_fill_desc(&D, mask, out, 0, 0);
@@ -798,6 +806,8 @@ nsx_status ccl_level_run(nsx_ctx* ctx, const View& mask, const View& weight, con
     p.threshold = true;
     p.thr = level - 1;   // foreground: mask > level - 1, i.e. mask >= level
     p.level = level;
+    p.floor_level = floor_level;
+    p.floor_mmax = floor_mmax;
     if (labels) {
         // kept: maximum > thresh, i.e. >= floor(thresh) + 1
         p.use_area = true;
@@ -1054,22 +1064,33 @@ nsx_status recon_levelset_u8(nsx_ctx* ctx, const View& marker, const View& mask,
     for (int v = 255; v >= 0; --v)
         if ((h[v >> 5] >> (v & 31)) & 1u) levels.push_back(v);
     if (levels.empty() || (int)levels.size() - 1 > max_levels) return NSX_OK;
-    hipLaunchKernelGGL(level_floor_kernel, dim3(grid), dim3(256), 0, s, out.data, out.pitch, out.rows, out.cols,
-                       levels.back(), d + 8);
-    // the smallest level above the labelling threshold (-1: none) also writes
-    // the labels, from its own components
+    // the smallest level above the labelling threshold (-1: none) runs last
+    // and also writes the labels, from its own components
     const int nl = (int)levels.size();
     int lstar = -1;
     if (req)
         for (int j = nl - 1; j >= 0 && lstar < 0; --j)
             if ((double)levels[j] > req->thresh) lstar = j;
+    std::vector<int> order;
     for (int j = 0; j + 1 < nl; ++j)
-        if (j != lstar) NSX_TRY(ccl_level_run(ctx, mask, marker, out, conn, levels[j], s));
+        if (j != lstar) order.push_back(j);
+    if (lstar >= 0 && lstar + 1 < nl) order.push_back(lstar);
+    // the smallest level's value min(level, largest marker value) holds
+    // everywhere: the first level run writes it with its own values (every
+    // pixel, no read), the others fold into the result; no run: a fill
+    if (order.empty())
+        hipLaunchKernelGGL(level_floor_kernel, dim3(grid), dim3(256), 0, s, out.data, out.pitch, out.rows, out.cols,
+                           levels.back(), d + 8);
+    for (size_t o = 0; o < order.size(); ++o) {
+        const int j = order[o];
+        const bool lab = req && j == lstar;
+        NSX_TRY(ccl_level_run(ctx, mask, marker, out, conn, levels[j], s, lab ? &req->labels : nullptr,
+                              req ? req->thresh : 0.0, lab ? req->ncomp_dev : nullptr, levels.back(),
+                              o == 0 ? d + 8 : nullptr));
+    }
     if (req) {
         const View& L = req->labels;
-        if (lstar >= 0 && lstar + 1 < nl) {
-            NSX_TRY(ccl_level_run(ctx, mask, marker, out, conn, levels[lstar], s, &L, req->thresh, req->ncomp_dev));
-        } else {
+        if (lstar < 0 || lstar + 1 >= nl) {
             // none above the threshold: no foreground; the smallest level: the
             // whole (connected) image, foreground iff the largest marker value
             // is above the threshold (h[8])

@@ -210,6 +210,11 @@ struct CclParams {
     int max_size = 0;         // (exclusive)
     bool use_area = false;
     int level = 0;            // CO_LEVEL
+    // CO_LEVEL: with floor_mmax, every pixel is written (no read of `out`):
+    // background min(floor_level, *floor_mmax), foreground the larger of that
+    // and the level value
+    int floor_level = 0;
+    const uint32_t* floor_mmax = nullptr;
 };
 
 // Labels the image `in` (u8; or u8/i32/f32 with threshold) into `out`.
@@ -222,10 +227,11 @@ nsx_status ccl_run(nsx_ctx* ctx, const View& in, const View& out, const CclParam
 // {mask >= level} (conn), each folded into `out` as min(level, its maximum of
 // `weight`).  With `labels`, the same pass also writes the canonical labels of
 // the components whose maximum exceeds `thresh` (0 elsewhere) and their count
-// to *ncomp_dev.  Enqueues work only.
+// to *ncomp_dev.  With floor_mmax, `out` is written everywhere instead of
+// folded (see CclParams).  Enqueues work only.
 nsx_status ccl_level_run(nsx_ctx* ctx, const View& mask, const View& weight, const View& out, int conn, int level,
                          hipStream_t s, const View* labels = nullptr, double thresh = 0.0,
-                         int32_t* ncomp_dev = nullptr);
+                         int32_t* ncomp_dev = nullptr, int floor_level = 0, const uint32_t* floor_mmax = nullptr);
 
 // Reconstruction by dilation of a u8 image through its level sets when the
 // mask holds at most max_levels + 1 distinct values: R = max over mask values
