@@ -445,11 +445,11 @@ def bench_maze(args):
     kt = ctx.read_timing()
     ctx.set_timing(False)
     px = n * n
-    # Algorithmic bytes per pixel of each launch kind over one step: the level
-    # labelling of {mask >= 200} reads mask + marker (local) and mask + the
-    # reconstruction, writing the reconstruction (final); the output labelling
-    # reads the reconstruction (local) and writes int32 labels (final).
-    alg = {"ccl_local": (2 + 1) * px, "ccl_final": (3 + 5) * px}
+    # Algorithmic bytes per pixel of each launch kind over one step: one
+    # labelling of {mask >= 200} reads mask + marker (local); its output pass
+    # writes the reconstruction and the int32 labels of {recon > 100}, which
+    # are that labelling's components (final).  Step: 2 B in, 5 B out.
+    alg = {"ccl_local": 2 * px, "ccl_final": 5 * px}
     kind = max(kt, key=lambda k: kt[k][0])
     roof = None
     if kind in alg and kt[kind][1]:
@@ -470,8 +470,8 @@ def bench_maze(args):
         "dtype": "u8", "data": "synthetic", "bitexact": exact,
         "config": {"workload": f"{n}x{n} u8 maze (6-px cells, 3-px corridors 200 / walls 20), 5x5 seed: recon4 "
                                "-> >100 -> bwlabel4", "pixels": px, "components": ncomp,
-                   "step_alg_bytes": 8 * px,
-                   "step_hbm_frac": round(8 * px / (elapsed / args.steps) / (HBM_PEAK_GBS * 1e9), 5)},
+                   "step_alg_bytes": 7 * px,
+                   "step_hbm_frac": round(7 * px / (elapsed / args.steps) / (HBM_PEAK_GBS * 1e9), 5)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -538,8 +538,8 @@ TILE_PX = 62 * 62   # interior pixels of one reconstruction tile visit
 # launch of the kind.  FETCH_SIZE is scaled by 2.0 and WRITE_SIZE by 1.0: the
 # factors that make `--workload copy` (256 MiB read + 256 MiB written per
 # launch) read true.  Commands and raw figures: profiles/traffic_chain4096.md.
-TRAFFIC_CHAIN4096 = {"recon_round": 20151356, "recon_first": 64785396, "ccl_local": 68009984,
-                     "ccl_final": 101634253, "ccl_merge": 40455578}
+TRAFFIC_CHAIN4096 = {"recon_round": 21234994, "recon_first": 64782430, "ccl_local": 68013931,
+                     "ccl_final": 101539598, "ccl_merge": 37431420}
 
 
 def border_tiles(rows: int, cols: int) -> int:
