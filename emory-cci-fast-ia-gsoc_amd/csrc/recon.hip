@@ -755,11 +755,28 @@ struct StateFill {
 __global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts, int64_t* stats, int32_t* list0,
                                    int ntx, int nty, StateFill F) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t npx = (int64_t)F.rows * F.cols;
-    for (int64_t j = i; j < npx; j += (int64_t)gridDim.x * blockDim.x) {
-        const int y = (int)(j / F.cols), x = (int)(j - (int64_t)y * F.cols);
-        if (F.esize == 1) st<uint8_t>(F.state, F.ps, y, x, (uint8_t)F.value);
-        else st<int32_t>(F.state, F.ps, y, x, (int32_t)F.value);
+    // the initial state, a row per block: 16-byte stores between the row's first
+    // and last 16-byte boundaries, element stores before and after (and for a
+    // whole int32 row that is not 4-byte aligned, where the pattern would be
+    // out of phase)
+    const int64_t rowbytes = (int64_t)F.cols * F.esize;
+    const uint32_t pat = F.esize == 1 ? (F.value & 0xFFu) * 0x01010101u : F.value;
+    for (int y = blockIdx.x; y < F.rows; y += gridDim.x) {
+        char* p = F.state + (int64_t)y * F.ps;
+        const uintptr_t a = (uintptr_t)p;
+        int64_t h = std::min((int64_t)((16 - (a & 15)) & 15), rowbytes);
+        if (F.esize == 4 && (a & 3)) h = rowbytes;
+        const int64_t body = (rowbytes - h) / 16 * 16;
+        for (int part = 0; part < 2; ++part) {   // before and after the 16-byte body
+            const int64_t e0 = part == 0 ? 0 : h + body, e1 = part == 0 ? h : rowbytes;
+            for (int64_t e = e0 + (int64_t)threadIdx.x * F.esize; e < e1; e += (int64_t)blockDim.x * F.esize) {
+                if (F.esize == 1) *(uint8_t*)(p + e) = (uint8_t)F.value;
+                else *(int32_t*)(p + e) = (int32_t)F.value;
+            }
+        }
+        const u32x4 v4 = {pat, pat, pat, pat};
+        for (int64_t o = h + 16 * (int64_t)threadIdx.x; o < h + body; o += 16 * (int64_t)blockDim.x)
+            *(u32x4*)(p + o) = v4;
     }
     for (int j = i; j < ntiles; j += gridDim.x * blockDim.x) stamp[j] = 0u;
     const int nb = ntx > 0 ? border_tiles(ntx, nty) : 0;
@@ -1245,7 +1262,7 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
         // the stored state of the initial key: invert(min) = max for both fill modes
         F = StateFill{D0.state, D0.ps, D0.rows, D0.cols, dtype == NSX_U8 ? 1 : 4,
                       dtype == NSX_U8 ? 255u : (uint32_t)INT_MAX};
-        grid_reset = std::max(grid_reset, std::min(4 * ctx->cu_count, (int)(((int64_t)D0.rows * D0.cols + 255) / 256)));
+        grid_reset = std::max(grid_reset, std::min(4 * ctx->cu_count, D0.rows));
     }
     hipLaunchKernelGGL(recon_reset_kernel, dim3(grid_reset), dim3(256), 0, s, A.stamp, A.ntiles, A.counts, A.stats,
                        A.list0, border_start ? D0.ntx : 0, D0.nty, F);
