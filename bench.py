@@ -49,6 +49,8 @@ def parse():
                         "batched vs per-tile; maze: 16384^2 maze stress; serve: continuous-batching server")
     p.add_argument("--tiles-u8", type=int, default=256)
     p.add_argument("--tiles-f32", type=int, default=32)
+    p.add_argument("--tune-reps", type=int, default=2,
+                   help="batch: graph replays per setting tried by the autotuner (0: no autotuning)")
     p.add_argument("--check-tiles", type=int, default=64, help="batch tiles also checked against the oracle")
     p.add_argument("--maze-size", type=int, default=16384, help="maze: side of the maze image")
     p.add_argument("--maze-cpu-size", type=int, default=16384,
@@ -124,6 +126,11 @@ def bench_batch(args):
     t0 = time.perf_counter()
     b.finalize(use_graph=True)
     t_fin = time.perf_counter() - t0
+    # per tile group, the sweep pairs per visit with the fastest replays
+    t0 = time.perf_counter()
+    pairs = b.autotune(reps=args.tune_reps) if args.tune_reps > 0 else []
+    t_tune = time.perf_counter() - t0
+    log(f"autotune: sweep pairs per group {pairs} in {t_tune:.2f} s")
     for _ in range(args.warmup):
         b.run()
     counts = b.counts()
@@ -150,6 +157,7 @@ def bench_batch(args):
         "data": "synthetic", "bitexact": exact, "batched_equals_per_tile": bool(same),
         "per_tile_mpix_s": round(total_px / t_tile / 1e6, 2), "per_tile_ms": round(t_tile * 1e3, 3),
         "speedup_vs_per_tile": round(t_tile / t_batch, 3), "finalize_s": round(t_fin, 3),
+        "autotune": {"visit_pairs_per_group": pairs, "s": round(t_tune, 3)},
         "config": {"workload": f"{args.tiles_u8} u8 + {args.tiles_f32} f32 tiles, sides 512-4096 step 16: "
                                "recon8 -> >128 (f32: >128/255) -> bwlabel8", "pixels": total_px,
                    "tiles": len(tiles)},

@@ -49,6 +49,7 @@ struct Group {
     int32_t* d_ncomp = nullptr;      // per item of the group
     int32_t* d_pending = nullptr;    // tiles still queued after the captured rounds
     int ws_recon = 0, ws_ccl = 0;    // workspace slot bases in the batch context
+    int visit_sweeps = 0;            // sweep pairs per visit (0: the context's), see autotune
 };
 
 }  // namespace
@@ -74,6 +75,21 @@ View view_of(const nsx_img& im) {
     return View{(char*)im.data, im.pitch_bytes, im.rows, im.cols, im.dtype};
 }
 
+// The reconstruction of group gi as one batch of images.
+ReconBatch recon_batch(nsx_batch* b, size_t gi) {
+    const Group& g = b->groups[gi];
+    ReconBatch rb{};
+    rb.nimg = (int)g.items.size();
+    rb.ntiles = g.rtiles;
+    rb.host_desc = g.rdesc.data();
+    rb.dev_desc = g.d_rdesc;
+    rb.tile_img = g.d_rtile;
+    rb.ws_base = g.ws_recon;
+    rb.stats = b->ws->d_stats + 8 * (gi % 2);
+    rb.visit_sweeps = g.visit_sweeps;
+    return rb;
+}
+
 // Enqueues every group's reconstruction (first pass + predicted rounds) and
 // labelling, then gathers the per-item counts.  No host synchronisation.
 nsx_status enqueue(nsx_batch* b, hipStream_t s, std::vector<ReconRun>* runs) {
@@ -82,14 +98,7 @@ nsx_status enqueue(nsx_batch* b, hipStream_t s, std::vector<ReconRun>* runs) {
     runs->resize(b->groups.size());
     for (size_t gi = 0; gi < b->groups.size(); ++gi) {
         Group& g = b->groups[gi];
-        ReconBatch rb{};
-        rb.nimg = (int)g.items.size();
-        rb.ntiles = g.rtiles;
-        rb.host_desc = g.rdesc.data();
-        rb.dev_desc = g.d_rdesc;
-        rb.tile_img = g.d_rtile;
-        rb.ws_base = g.ws_recon;
-        rb.stats = c->d_stats + 8 * (gi % 2);
+        const ReconBatch rb = recon_batch(b, gi);
         NSX_TRY(recon_launch(c, RM_RECON, g.dtype, g.conn, rb, s, &(*runs)[gi], g.d_pending));
         CclBatch cb{};
         cb.nimg = (int)g.items.size();
@@ -123,6 +132,42 @@ nsx_status gather_counts(nsx_batch* b, hipStream_t s) {
 nsx_status fail_to_owner(nsx_batch* b, nsx_status st) {
     if (st != NSX_OK && b->ws && b->owner != b->ws) b->owner->last_error = b->ws->last_error;
     return st;
+}
+
+// Runs every group's reconstruction eagerly to convergence, which records the
+// rounds each one needs (the next enqueue launches that many).
+nsx_status learn_rounds(nsx_batch* b, hipStream_t s) {
+    nsx_ctx* c = b->ws;
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        const Group& g = b->groups[gi];
+        const ReconBatch rb = recon_batch(b, gi);
+        ReconRun run;
+        nsx_status st = recon_launch(c, RM_RECON, g.dtype, g.conn, rb, s, &run, nullptr);
+        if (st == NSX_OK) st = recon_complete(c, &run, s, nullptr);
+        if (st != NSX_OK) return fail_to_owner(b, st);
+    }
+    return NSX_OK;
+}
+
+// (Re-)captures the batch's launches into its graph.
+nsx_status capture(nsx_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    if (!b->cap_stream) NSX_HIP_TRY(b->owner, hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+    std::vector<ReconRun> runs;
+    NSX_HIP_TRY(b->owner, hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeRelaxed));
+    nsx_status st = enqueue(b, b->cap_stream, &runs);
+    if (st == NSX_OK) st = gather_counts(b, b->cap_stream);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->cap_stream, &gr);
+    for (auto& r : runs) r.release();
+    if (st != NSX_OK) return fail_to_owner(b, st);
+    if (e != hipSuccess) return b->owner->fail(NSX_ERR_HIP, "batch graph capture failed");
+    b->graph = gr;
+    NSX_HIP_TRY(b->owner, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return NSX_OK;
 }
 
 }  // namespace
@@ -230,17 +275,7 @@ nsx_status nsx_batch_finalize(nsx_batch* b, int use_graph) {
     // that sizes every workspace slot for the largest group, so that nothing
     // is reallocated while launches are captured
     hipStream_t s = c->stream;
-    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-        Group& g = b->groups[gi];
-        ReconBatch rb{};
-        rb.nimg = (int)g.items.size(); rb.ntiles = g.rtiles; rb.host_desc = g.rdesc.data();
-        rb.dev_desc = g.d_rdesc; rb.tile_img = g.d_rtile; rb.ws_base = g.ws_recon;
-        rb.stats = c->d_stats + 8 * (gi % 2);
-        ReconRun run;
-        nsx_status st = recon_launch(c, RM_RECON, g.dtype, g.conn, rb, s, &run, nullptr);
-        if (st == NSX_OK) st = recon_complete(c, &run, s, nullptr);
-        if (st != NSX_OK) return fail_to_owner(b, st);
-    }
+    NSX_TRY(learn_rounds(b, s));
     {
         int64_t max_pix = 0;
         int max_ct = 0, max_rt_even = 0, max_rt_odd = 0;
@@ -270,20 +305,7 @@ nsx_status nsx_batch_finalize(nsx_batch* b, int use_graph) {
         if (st != NSX_OK) return fail_to_owner(b, st);
         NSX_HIP_TRY(b->owner, hipStreamSynchronize(s));
     }
-    if (b->use_graph) {
-        NSX_HIP_TRY(b->owner, hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
-        std::vector<ReconRun> runs;
-        NSX_HIP_TRY(b->owner, hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeRelaxed));
-        nsx_status st = enqueue(b, b->cap_stream, &runs);
-        if (st == NSX_OK) st = gather_counts(b, b->cap_stream);
-        hipGraph_t gr = nullptr;
-        const hipError_t e = hipStreamEndCapture(b->cap_stream, &gr);
-        for (auto& r : runs) r.release();
-        if (st != NSX_OK) return fail_to_owner(b, st);
-        if (e != hipSuccess) return b->owner->fail(NSX_ERR_HIP, "batch graph capture failed");
-        b->graph = gr;
-        NSX_HIP_TRY(b->owner, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-    }
+    if (b->use_graph) NSX_TRY(capture(b));
     return NSX_OK;
 }
 
@@ -313,10 +335,7 @@ nsx_status nsx_batch_counts(nsx_batch* b, int32_t* counts, int n) {
         NSX_HIP_TRY(b->owner, hipMemcpy(b->h_pending, g.d_pending, sizeof(int32_t), hipMemcpyDeviceToHost));
         if (b->h_pending[0] == 0) continue;
         redo = true;
-        ReconBatch rb{};
-        rb.nimg = (int)g.items.size(); rb.ntiles = g.rtiles; rb.host_desc = g.rdesc.data();
-        rb.dev_desc = g.d_rdesc; rb.tile_img = g.d_rtile; rb.ws_base = g.ws_recon;
-        rb.stats = c->d_stats + 8 * (gi % 2);
+        const ReconBatch rb = recon_batch(b, gi);
         ReconRun run;
         nsx_status st = recon_launch(c, RM_RECON, g.dtype, g.conn, rb, c->stream, &run, nullptr);
         if (st == NSX_OK) st = recon_complete(c, &run, c->stream, nullptr);
@@ -332,25 +351,64 @@ nsx_status nsx_batch_counts(nsx_batch* b, int32_t* counts, int n) {
     if (redo) {
         NSX_TRY(gather_counts(b, c->stream));
         NSX_HIP_TRY(b->owner, hipStreamSynchronize(c->stream));
-        if (b->exec) {
-            hipGraphExecDestroy(b->exec);
-            hipGraphDestroy(b->graph);
-            b->exec = nullptr;
-            b->graph = nullptr;
-            std::vector<ReconRun> runs;
-            NSX_HIP_TRY(b->owner, hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeRelaxed));
-            nsx_status st = enqueue(b, b->cap_stream, &runs);
-            if (st == NSX_OK) st = gather_counts(b, b->cap_stream);
-            hipGraph_t gr = nullptr;
-            const hipError_t e = hipStreamEndCapture(b->cap_stream, &gr);
-            for (auto& r : runs) r.release();
-            if (st != NSX_OK) return fail_to_owner(b, st);
-            if (e != hipSuccess) return b->owner->fail(NSX_ERR_HIP, "batch graph re-capture failed");
-            b->graph = gr;
-            NSX_HIP_TRY(b->owner, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-        }
+        if (b->exec) NSX_TRY(capture(b));
     }
     NSX_HIP_TRY(b->owner, hipMemcpy(counts, b->d_counts, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    return NSX_OK;
+}
+
+nsx_status nsx_batch_autotune(nsx_batch* b, void* stream, int reps, int32_t* pairs, int npairs) {
+    if (!b || !b->finalized || reps < 1 || npairs < 0 || (npairs > 0 && !pairs)) return NSX_ERR_INVALID;
+    if (!b->exec) return b->owner->fail(NSX_ERR_INVALID, "autotune needs a batch finalized with a graph");
+    hipStream_t s = stream ? (hipStream_t)stream : b->owner->stream;
+    hipEvent_t e0, e1;
+    NSX_HIP_TRY(b->owner, hipEventCreate(&e0));
+    NSX_HIP_TRY(b->owner, hipEventCreate(&e1));
+    // one setting: learn its round counts, capture, one replay to warm up,
+    // then the mean of `reps` timed replays
+    auto trial = [&](float* ms) -> nsx_status {
+        NSX_TRY(learn_rounds(b, b->ws->stream));
+        NSX_HIP_TRY(b->owner, hipStreamSynchronize(b->ws->stream));
+        NSX_TRY(capture(b));
+        NSX_HIP_TRY(b->owner, hipGraphLaunch(b->exec, s));
+        NSX_HIP_TRY(b->owner, hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) NSX_HIP_TRY(b->owner, hipGraphLaunch(b->exec, s));
+        NSX_HIP_TRY(b->owner, hipEventRecord(e1, s));
+        NSX_HIP_TRY(b->owner, hipEventSynchronize(e1));
+        NSX_HIP_TRY(b->owner, hipEventElapsedTime(ms, e0, e1));
+        *ms /= (float)reps;
+        return NSX_OK;
+    };
+    // coordinate descent over the groups (each is its own set of launches)
+    static const int kCand[] = {3, 4, 5, 6};
+    nsx_status st = NSX_OK;
+    for (size_t gi = 0; gi < b->groups.size() && st == NSX_OK; ++gi) {
+        Group& g = b->groups[gi];
+        int best = g.visit_sweeps > 0 ? g.visit_sweeps : b->ws->visit_sweeps;
+        float best_ms = 0.f;
+        g.visit_sweeps = best;
+        st = trial(&best_ms);
+        for (int cand : kCand) {
+            if (st != NSX_OK || cand == best) continue;
+            g.visit_sweeps = cand;
+            float ms = 0.f;
+            st = trial(&ms);
+            if (st == NSX_OK && ms < best_ms) {
+                best_ms = ms;
+                best = cand;
+            }
+        }
+        g.visit_sweeps = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    NSX_TRY(st);
+    // leave the graph captured with the chosen settings
+    NSX_TRY(learn_rounds(b, b->ws->stream));
+    NSX_HIP_TRY(b->owner, hipStreamSynchronize(b->ws->stream));
+    NSX_TRY(capture(b));
+    for (int i = 0; i < npairs; ++i)
+        pairs[i] = (size_t)i < b->groups.size() ? b->groups[i].visit_sweeps : 0;
     return NSX_OK;
 }
 

@@ -161,6 +161,7 @@ struct ReconBatch {
     const int32_t* tile_img = nullptr;  // device: tile -> image index (nimg > 1)
     int ws_base = 0;                // first of 3 consecutive workspace slots
     int64_t* stats = nullptr;       // device stats words (default ctx->d_stats)
+    int visit_sweeps = 0;           // sweep pairs per visit; 0: the context's setting
 };
 struct ReconRunImpl;
 // A launched reconstruction whose convergence has not been checked yet.

@@ -1240,14 +1240,16 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     // four-wave hole-filling tiles run one pair more: their pairs are shorter
     // (4096^2 chain: 36 fill rounds instead of 41, 0.8 % faster)
     const bool quad = (mode == RM_FILL || mode == RM_FILLBIN) && conn == 4;
-    A.visit_sweeps = std::max(1, std::min(MAX_IT, ctx->visit_sweeps + (quad ? 1 : 0)));
+    const int pairs = B.visit_sweeps > 0 ? B.visit_sweeps : ctx->visit_sweeps;
+    A.visit_sweeps = std::max(1, std::min(MAX_IT, pairs + (quad ? 1 : 0)));
     R->later = later;
     // u8 tiles run 3 waves/SIMD (12 per CU), 32-bit tiles 2
     R->grid = std::max(1, std::min(A.ntiles, (dtype == NSX_U8 ? 12 : 8) * ctx->cu_count));
     // Round predictions are shared by problems within ~19% of each other in tile
     // count (quarter-octave buckets), so mixed request sizes still hit.
     const uint64_t bucket = (uint64_t)std::lround(4.0 * std::log2((double)A.ntiles));
-    R->key = (bucket << 16) ^ ((uint64_t)mode << 8) ^ ((uint64_t)dtype << 4) ^ (uint64_t)conn;
+    R->key = ((uint64_t)A.visit_sweeps << 40) ^ (bucket << 16) ^ ((uint64_t)mode << 8) ^ ((uint64_t)dtype << 4) ^
+             (uint64_t)conn;
     run->impl = R;
 
     // Hole filling starts from the minimum everywhere inside the frame, so in

@@ -355,6 +355,15 @@ class Batch:
         s = _stream_arg(stream) if stream is not None else _torch_handle(_torch().cuda.current_stream())
         self.ctx.check(self.lib.nsx_batch_run(self.h, s))
 
+    def autotune(self, reps: int = 3, stream=None) -> list:
+        """Times graph replays with 3..6 sweep pairs per tile visit for each tile
+        group and keeps the fastest (nsx_batch_autotune); returns the pairs chosen
+        per group.  Results do not depend on it."""
+        s = _stream_arg(stream) if stream is not None else _torch_handle(_torch().cuda.current_stream())
+        pairs = (C.c_int32 * 4)()
+        self.ctx.check(self.lib.nsx_batch_autotune(self.h, s, int(reps), pairs, 4))
+        return [int(v) for v in pairs if v > 0]
+
     def counts(self) -> list:
         arr = (C.c_int32 * max(1, self.n))()
         self.ctx.check(self.lib.nsx_batch_counts(self.h, arr, self.n))

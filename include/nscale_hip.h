@@ -196,6 +196,16 @@ nsx_status nsx_batch_finalize(nsx_batch* batch, int use_graph);
 nsx_status nsx_batch_run(nsx_batch* batch, void* stream);
 /* Per-tile component counts after a run (array of n tiles). Synchronises. */
 nsx_status nsx_batch_counts(nsx_batch* batch, int32_t* counts, int n);
+/* Tunes a batch finalized with a graph: for each group of tiles with the same
+ * (dtype, conn, ccl_conn, thresh), in the order groups first appear, tries
+ * 3..6 sweep pairs per tile visit (NSX_TUNE_VISIT_SWEEPS for that group only),
+ * each timed as the mean of `reps` graph replays on `stream` after re-learning
+ * the round counts and re-capturing; keeps the fastest and leaves the graph
+ * captured with it.  pairs[g] (g < npairs) receives the choice, 0 past the last
+ * group.  Results never depend on it; it runs the batch about 4 x groups x
+ * (reps + 2) times.  No counterpart in the reference (its batched path has
+ * fixed launch shapes). */
+nsx_status nsx_batch_autotune(nsx_batch* batch, void* stream, int reps, int32_t* pairs, int npairs);
 void nsx_batch_destroy(nsx_batch* batch);
 
 /* ---- serving loop --------------------------------------------------------- */

@@ -79,3 +79,45 @@ def test_batch_rejects_host_images(nh, gpu):
     with pytest.raises(nh.NsxError):
         b.add(a, a, a, np.zeros((16, 16), np.int32))
     b.close()
+
+
+def test_batch_autotune_keeps_results(nh, orc, gpu):
+    """nsx_batch_autotune re-captures the graph per tried setting and leaves the
+    fastest one captured: the choices are in 3..6 per group, and replays after
+    it still give the oracle's results.  A batch without a graph is refused."""
+    import torch
+    tiles = make_tiles(nh, 6, 3)
+    want = [expected(orc, m, k, t) for m, k, t in tiles]
+    batch = nh.Batch()
+    outs = []
+    for marker, mask, thr in tiles:
+        dm = torch.from_numpy(marker).cuda()
+        dk = torch.from_numpy(mask).cuda()
+        r = torch.empty_like(dk)
+        lab = torch.empty(dk.shape, dtype=torch.int32, device="cuda")
+        batch.add(dm, dk, r, lab, 8, thr, 8)
+        outs.append((r, lab))
+    batch.finalize(use_graph=True)
+    pairs = batch.autotune(reps=1)
+    assert 1 <= len(pairs) <= 3 and all(3 <= p <= 6 for p in pairs)
+    for replay in range(2):
+        for r, lab in outs:
+            r.fill_(7)
+            lab.fill_(-5)
+        torch.cuda.synchronize()
+        batch.run()
+        counts = batch.counts()
+        for i, ((r, lab), (wr, wlab, wn)) in enumerate(zip(outs, want)):
+            assert np.array_equal(r.cpu().numpy(), wr), f"replay {replay} tile {i} recon"
+            assert np.array_equal(lab.cpu().numpy(), wlab), f"replay {replay} tile {i} labels"
+            assert counts[i] == wn
+    batch.close()
+    eager = nh.Batch()
+    marker, mask, thr = tiles[0]
+    dk = torch.from_numpy(mask).cuda()
+    eager.add(torch.from_numpy(marker).cuda(), dk, torch.empty_like(dk),
+              torch.empty(dk.shape, dtype=torch.int32, device="cuda"), 8, thr, 8)
+    eager.finalize(use_graph=False)
+    with pytest.raises(nh.NsxError):
+        eager.autotune(reps=1)
+    eager.close()
