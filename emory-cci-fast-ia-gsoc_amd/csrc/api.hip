@@ -162,6 +162,29 @@ nsx_status finish_recon(nsx_ctx* ctx, const View& v, const nsx_img* out, hipStre
     return from_device(ctx, v, out, s);
 }
 
+// imfill with seeds and bwselect share their checks and staging: u8 image and
+// seeds of one size, u8 output that aliases neither.
+nsx_status seeded_setup(nsx_ctx* ctx, const nsx_img* img, const nsx_img* seeds, const nsx_img* out,
+                               const char* what) {
+    NSX_TRY(check_img(ctx, img, "image"));
+    NSX_TRY(check_img(ctx, seeds, "seeds"));
+    NSX_TRY(check_img(ctx, out, "out"));
+    if (img->dtype != NSX_U8 || seeds->dtype != NSX_U8 || out->dtype != NSX_U8)
+        return ctx->fail(NSX_ERR_UNSUPPORTED, std::string(what) + " is instantiated for uchar only (seeds: CV_8U)");
+    if (!same_shape(img, seeds) || !same_shape(img, out))
+        return ctx->fail(NSX_ERR_INVALID, "image, seeds and out must have the same size");
+    if (overlaps(out, img) || overlaps(out, seeds)) return ctx->fail(NSX_ERR_INVALID, "out aliases an input");
+    return NSX_OK;
+}
+
+nsx_status scratch_u8(nsx_ctx* ctx, int slot, const View& like, View* v) {
+    void* p;
+    const size_t pitch = like.cols > 0 ? (size_t)like.cols : 1;
+    NSX_TRY(ctx->ws(slot, pitch * (size_t)(like.rows > 0 ? like.rows : 1), &p));
+    *v = View{(char*)p, (int64_t)pitch, like.rows, like.cols, NSX_U8};
+    return NSX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -343,6 +366,41 @@ nsx_status nsx_fill_holes(nsx_ctx* ctx, const nsx_img* in, nsx_img* out, int bin
     } else {
         NSX_TRY(recon_run(ctx, binary ? RM_FILLBIN : RM_FILL, nullptr, vi, vo, conn == 8 ? 8 : 4, s));
     }
+    return finish_recon(ctx, vo, out, s);
+}
+
+nsx_status nsx_imfill(nsx_ctx* ctx, const nsx_img* image, const nsx_img* seeds, nsx_img* out, int binary, int conn,
+                      void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(seeded_setup(ctx, image, seeds, out, "imfill"));
+    if (is_empty(image)) return NSX_OK;
+    hipStream_t s = pick_stream(ctx, stream);
+    View vi, vs, vo, vm, vk;
+    NSX_TRY(to_device(ctx, image, WS_STAGE_B, true, s, &vi));
+    NSX_TRY(to_device(ctx, seeds, WS_STAGE_A, true, s, &vs));
+    NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
+    NSX_TRY(scratch_u8(ctx, WS_CHAIN_RECON, vi, &vm));
+    NSX_TRY(scratch_u8(ctx, WS_CHAIN_FILL, vi, &vk));
+    NSX_TRY(seeded_marker_run(ctx, vi, vs, vm, &vk, s));
+    NSX_TRY(recon_run(ctx, binary ? RM_BINARY : RM_RECON, &vm, vk, vo, conn == 8 ? 8 : 4, s));
+    NSX_TRY(bitwise_u8_run(ctx, vi, vo, false, s));
+    return finish_recon(ctx, vo, out, s);
+}
+
+nsx_status nsx_bwselect(nsx_ctx* ctx, const nsx_img* binary_image, const nsx_img* seeds, nsx_img* out, int conn,
+                        void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(seeded_setup(ctx, binary_image, seeds, out, "bwselect"));
+    if (is_empty(binary_image)) return NSX_OK;
+    hipStream_t s = pick_stream(ctx, stream);
+    View vi, vs, vo, vm;
+    NSX_TRY(to_device(ctx, binary_image, WS_STAGE_B, true, s, &vi));
+    NSX_TRY(to_device(ctx, seeds, WS_STAGE_A, true, s, &vs));
+    NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
+    NSX_TRY(scratch_u8(ctx, WS_CHAIN_RECON, vi, &vm));
+    NSX_TRY(seeded_marker_run(ctx, vi, vs, vm, nullptr, s));
+    NSX_TRY(recon_run(ctx, RM_BINARY, &vm, vi, vo, conn == 8 ? 8 : 4, s));
+    NSX_TRY(bitwise_u8_run(ctx, vi, vo, true, s));
     return finish_recon(ctx, vo, out, s);
 }
 

@@ -1179,6 +1179,30 @@ __global__ void threshold_kernel(const char* in, int64_t pin, int dtype, int row
     ((uint8_t*)(out + (int64_t)y * pout))[x] = v > k ? 255 : 0;
 }
 
+// Seeded markers of imfill / bwselect (u8): marker = image where seeds != 0,
+// else 0; with `invert`, the image is first complemented (255 - x) and the
+// complement is also written to `mask` (imfill's reconstruction mask).
+__global__ void seeded_marker_kernel(const char* img, int64_t pi, const char* seeds, int64_t ps, char* marker,
+                                     int64_t pm, char* mask, int64_t pk, int rows, int cols, int invert) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= cols) return;
+    uint8_t v = ((const uint8_t*)(img + (int64_t)y * pi))[x];
+    if (invert) {
+        v = (uint8_t)(255 - v);
+        ((uint8_t*)(mask + (int64_t)y * pk))[x] = v;
+    }
+    ((uint8_t*)(marker + (int64_t)y * pm))[x] = ((const uint8_t*)(seeds + (int64_t)y * ps))[x] != 0 ? v : 0;
+}
+
+// out = image | out (op 0) or image & out (op 1), u8.
+__global__ void bitwise_u8_kernel(const char* img, int64_t pi, char* out, int64_t po, int rows, int cols, int op) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= cols) return;
+    const uint8_t a = ((const uint8_t*)(img + (int64_t)y * pi))[x];
+    uint8_t* o = (uint8_t*)(out + (int64_t)y * po) + x;
+    *o = op == 0 ? (uint8_t)(a | *o) : (uint8_t)(a & *o);
+}
+
 __global__ void fill_i32_kernel(char* out, int64_t pout, int rows, int cols, int32_t v) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= cols) return;
@@ -1245,6 +1269,26 @@ nsx_status threshold_run(nsx_ctx* ctx, const View& in, double k, const View& out
     const dim3 grid((in.cols + 255) / 256, in.rows);
     hipLaunchKernelGGL(threshold_kernel, grid, dim3(256), 0, s, in.data, in.pitch, in.dtype, in.rows, in.cols, k,
                        out.data, out.pitch);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    return NSX_OK;
+}
+
+nsx_status seeded_marker_run(nsx_ctx* ctx, const View& img, const View& seeds, const View& marker, const View* mask,
+                             hipStream_t s) {
+    if (img.rows == 0 || img.cols == 0) return NSX_OK;
+    const dim3 grid((img.cols + 255) / 256, img.rows);
+    hipLaunchKernelGGL(seeded_marker_kernel, grid, dim3(256), 0, s, img.data, img.pitch, seeds.data, seeds.pitch,
+                       marker.data, marker.pitch, mask ? mask->data : nullptr, mask ? mask->pitch : 0, img.rows,
+                       img.cols, mask ? 1 : 0);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    return NSX_OK;
+}
+
+nsx_status bitwise_u8_run(nsx_ctx* ctx, const View& img, const View& out, bool and_op, hipStream_t s) {
+    if (img.rows == 0 || img.cols == 0) return NSX_OK;
+    const dim3 grid((img.cols + 255) / 256, img.rows);
+    hipLaunchKernelGGL(bitwise_u8_kernel, grid, dim3(256), 0, s, img.data, img.pitch, out.data, out.pitch, img.rows,
+                       img.cols, and_op ? 1 : 0);
     NSX_HIP_TRY(ctx, hipGetLastError());
     return NSX_OK;
 }

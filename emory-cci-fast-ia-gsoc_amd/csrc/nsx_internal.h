@@ -283,6 +283,13 @@ nsx_status ccl_relabel(nsx_ctx* ctx, const View& lab, int bgval, int32_t* ncomp_
 // Pointwise `in > k` -> u8 255/0.
 nsx_status threshold_run(nsx_ctx* ctx, const View& in, double k, const View& out, hipStream_t s);
 
+// imfill / bwselect markers (u8): marker = (mask ? 255 - img : img) where
+// seeds != 0, else 0; with `mask`, 255 - img is also written there.
+nsx_status seeded_marker_run(nsx_ctx* ctx, const View& img, const View& seeds, const View& marker, const View* mask,
+                             hipStream_t s);
+// out = img | out, or img & out (and_op), u8.
+nsx_status bitwise_u8_run(nsx_ctx* ctx, const View& img, const View& out, bool and_op, hipStream_t s);
+
 // Pointwise i32 fill of a constant.
 nsx_status fill_const_i32(nsx_ctx* ctx, const View& out, int32_t v, hipStream_t s);
 

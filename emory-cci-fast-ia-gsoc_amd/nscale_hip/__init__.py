@@ -28,7 +28,7 @@ from ._lib import NsxImg
 
 __all__ = [
     "NsxError", "Context", "default_context",
-    "imreconstruct", "imreconstructBinary", "imfillHoles", "fillHoles",
+    "imreconstruct", "imreconstructBinary", "imfillHoles", "fillHoles", "imfill", "bwselect",
     "bwlabel2", "bwlabel", "bwareaopen2", "relabel", "threshold_gt", "segment_chain",
     "gen_disks_u8", "gen_marker_u8", "gen_maze_u8", "scale_i32", "scale_f32",
 ]
@@ -239,6 +239,25 @@ def imfillHoles(image, binary: bool = False, connectivity: int = 4, *, out=None,
 
 
 fillHoles = imfillHoles
+
+
+def imfill(image, seeds, binary: bool = False, connectivity: int = 4, *, out=None, ctx=None, stream=None):
+    """Seeded fill (nscale::imfill<uchar>): image | reconstruction of the
+    complement from the seeds (u8 image and seeds)."""
+    c = _ctx(image, ctx)
+    out = _like(image, np.uint8) if out is None else out
+    c.check(c.lib.nsx_imfill(c.h, C.byref(describe(image)), C.byref(describe(seeds)), C.byref(describe(out)),
+                             int(bool(binary)), connectivity, _torch_stream(image, stream)))
+    return out
+
+
+def bwselect(binaryImage, seeds, connectivity: int = 8, *, out=None, ctx=None, stream=None):
+    """Components holding a seed (nscale::bwselect<uchar>), with their values."""
+    c = _ctx(binaryImage, ctx)
+    out = _like(binaryImage, np.uint8) if out is None else out
+    c.check(c.lib.nsx_bwselect(c.h, C.byref(describe(binaryImage)), C.byref(describe(seeds)), C.byref(describe(out)),
+                               connectivity, _torch_stream(binaryImage, stream)))
+    return out
 
 
 def bwlabel2(binaryImage, connectivity: int = 8, relab: bool = True, *, out=None, ctx=None, stream=None,

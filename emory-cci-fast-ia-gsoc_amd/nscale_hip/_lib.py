@@ -52,6 +52,8 @@ SIGNATURES = {
     "nsx_imreconstruct": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_imreconstruct_binary": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_fill_holes": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_int, C.c_void_p]),
+    "nsx_imfill": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_int, C.c_void_p]),
+    "nsx_bwselect": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_bwlabel": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_int, _I32P, C.c_void_p]),
     "nsx_bwareaopen": (C.c_int, [_CTX, _IMG, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _IMG, _I32P,
                                  C.c_void_p]),

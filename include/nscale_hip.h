@@ -121,6 +121,20 @@ nsx_status nsx_imreconstruct_binary(nsx_ctx* ctx, const nsx_img* marker, const n
 nsx_status nsx_fill_holes(nsx_ctx* ctx, const nsx_img* in, nsx_img* out, int binary, int conn,
                           void* stream);
 
+/* imfill with seeds (MorphologicOperations.cpp:1833-1857, uchar only, :2516):
+ * mask = 255 - image; marker = mask where seeds != 0, else 0; marker =
+ * imreconstructBinary (binary) or imreconstruct (marker, mask, conn); out =
+ * image | marker.  image, seeds (any nonzero selects) and out are u8. */
+nsx_status nsx_imfill(nsx_ctx* ctx, const nsx_img* image, const nsx_img* seeds, nsx_img* out, int binary,
+                      int conn, void* stream);
+
+/* bwselect (MorphologicOperations.cpp:1929-1951, uchar only, :2522): marker =
+ * binary_image where seeds != 0, else 0; out = imreconstructBinary(marker,
+ * binary_image, conn) & binary_image, i.e. the components of the nonzero
+ * pixels that hold a seed, with their values.  All three images u8. */
+nsx_status nsx_bwselect(nsx_ctx* ctx, const nsx_img* binary_image, const nsx_img* seeds, nsx_img* out, int conn,
+                        void* stream);
+
 /* ---- connected components ------------------------------------------------- */
 /* Union-find labelling of a u8 image: 4/8-neighbours of equal nonzero value are
  * joined.  relabel=0: label = raster index of the component's first pixel,

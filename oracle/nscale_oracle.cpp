@@ -18,6 +18,11 @@
 //   or_imreconstruct_binary brute-force BFS flood fill + marker-kept quirk.
 //   or_fill_holes           MATLAB imfill 'holes' closed form (complement of
 //                           the border-reachable set / reconstruction from frame).
+//   or_imfill_u8            image | brute-force reconstruction of the seeded
+//                           complement (gray), image | BFS from the seeds
+//                           through the complement (binary).
+//   or_bwselect_u8          the components of {image != 0} holding a seed
+//                           (scipy ndimage.label), with the image's values.
 //   or_bwlabel / relabel /  the reference's own ConnComponents.cpp compiled
 //   or_bwareaopen           unmodified into oracle/_ref (exact), scipy
 //                           ndimage.label partitions, and the canonical-numbering
@@ -317,6 +322,39 @@ void imfillHoles(const T* image, T* out, int rows, int cols, bool binary, int co
     crop1<T>(output, rows, cols, out);                           // :1925
 }
 
+// imfill<T> with seeds (MorphologicOperations.cpp:1833-1857, instantiated for
+// uchar at :2516): mask = invert(image) (:1848); marker = mask where seeds != 0,
+// else 0 (:1850-1852, Mat::copyTo with the seeds as copy mask); marker =
+// imreconstructBinary or imreconstruct(marker, mask, conn) (:1854-1855);
+// result = image | marker (:1857).
+template <typename T>
+void imfill(const T* image, const uint8_t* seeds, T* out, int rows, int cols, bool binary, int connectivity) {
+    const size_t n = (size_t)rows * cols;
+    std::vector<T> mask(image, image + n);
+    invert_vec(mask);
+    std::vector<T> marker(n, T(0));
+    for (size_t i = 0; i < n; ++i)
+        if (seeds[i] != 0) marker[i] = mask[i];
+    std::vector<T> rec(n);
+    if (binary) imreconstructBinary<T>(marker.data(), mask.data(), rec.data(), rows, cols, connectivity);
+    else imreconstruct<T>(marker.data(), mask.data(), rec.data(), rows, cols, connectivity);
+    for (size_t i = 0; i < n; ++i) out[i] = (T)(image[i] | rec[i]);
+}
+
+// bwselect<T> (MorphologicOperations.cpp:1929-1951, instantiated for uchar at
+// :2522): marker = binaryImage where seeds != 0, else 0 (:1945-1946);
+// imreconstructBinary(marker, binaryImage, conn) (:1948); & binaryImage (:1950).
+template <typename T>
+void bwselect(const T* image, const uint8_t* seeds, T* out, int rows, int cols, int connectivity) {
+    const size_t n = (size_t)rows * cols;
+    std::vector<T> marker(n, T(0));
+    for (size_t i = 0; i < n; ++i)
+        if (seeds[i] != 0) marker[i] = image[i];
+    std::vector<T> rec(n);
+    imreconstructBinary<T>(marker.data(), image, rec.data(), rows, cols, connectivity);
+    for (size_t i = 0; i < n; ++i) out[i] = (T)(rec[i] & image[i]);
+}
+
 // ---- ConnComponents (src/segment/ConnComponents.cpp) ----------------------
 // find/flatten are written iteratively: the recursive originals (:24-32,
 // :137-147) compress every node on the path to the root, which the two-pass
@@ -443,6 +481,14 @@ int or_fill_holes_u8(const uint8_t* in, uint8_t* out, int rows, int cols, int bi
 }
 int or_fill_holes_i32(const int32_t* in, int32_t* out, int rows, int cols, int binary, int conn) {
     imfillHoles<int32_t>(in, out, rows, cols, binary != 0, conn);
+    return 0;
+}
+int or_imfill_u8(const uint8_t* in, const uint8_t* seeds, uint8_t* out, int rows, int cols, int binary, int conn) {
+    imfill<uint8_t>(in, seeds, out, rows, cols, binary != 0, conn);
+    return 0;
+}
+int or_bwselect_u8(const uint8_t* in, const uint8_t* seeds, uint8_t* out, int rows, int cols, int conn) {
+    bwselect<uint8_t>(in, seeds, out, rows, cols, conn);
     return 0;
 }
 void or_invert_i32(const int32_t* in, int32_t* out, long n) {

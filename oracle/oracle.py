@@ -73,6 +73,24 @@ def fill_holes(img, binary=False, conn=4):
     return out
 
 
+def imfill(img, seeds, binary=False, conn=4):
+    """imfill<uchar>(image, seeds, binary, conn) with seeds (any nonzero selects)."""
+    img = _c(img, np.uint8)
+    seeds = _c(seeds, np.uint8)
+    out = np.empty_like(img)
+    lib().or_imfill_u8(_p(img), _p(seeds), _p(out), img.shape[0], img.shape[1], int(bool(binary)), conn)
+    return out
+
+
+def bwselect(img, seeds, conn=8):
+    """bwselect<uchar>(binaryImage, seeds, conn)."""
+    img = _c(img, np.uint8)
+    seeds = _c(seeds, np.uint8)
+    out = np.empty_like(img)
+    lib().or_bwselect_u8(_p(img), _p(seeds), _p(out), img.shape[0], img.shape[1], conn)
+    return out
+
+
 def bwlabel2(binimg, conn=8, relab=True):
     b = _c(binimg, np.uint8)
     out = np.empty(b.shape, np.int32)

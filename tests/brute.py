@@ -117,3 +117,31 @@ def root_labels_from_canonical(canon: np.ndarray) -> np.ndarray:
         if l:
             out[i] = first[l]
     return out.reshape(canon.shape)
+
+
+def bwselect_scipy(img: np.ndarray, seeds: np.ndarray, conn: int) -> np.ndarray:
+    """The components of img != 0 that hold a pixel with seeds != 0, with the
+    image's values; 0 elsewhere."""
+    s = img != 0
+    lab, _ = ndimage.label(s, structure=structure(conn))
+    sel = np.unique(lab[(seeds != 0) & s])
+    sel = sel[sel > 0]
+    return np.where(np.isin(lab, sel) & s, img, 0).astype(img.dtype)
+
+
+def imfill_binary_flood(img: np.ndarray, seeds: np.ndarray, conn: int) -> np.ndarray:
+    """Seeded binary fill of a u8 image: flood from the seeds through the pixels
+    below 255 (the complement's nonzero pixels); reached pixels become 255."""
+    s = img != 255
+    lab, _ = ndimage.label(s, structure=structure(conn))
+    sel = np.unique(lab[(seeds != 0) & s])
+    sel = sel[sel > 0]
+    return np.where(np.isin(lab, sel) & s, 255, img).astype(np.uint8)
+
+
+def imfill_gray_fixpoint(img: np.ndarray, seeds: np.ndarray, conn: int) -> np.ndarray:
+    """Seeded grayscale fill of a u8 image: img | R, R the geodesic
+    reconstruction of the complement from the complement at the seeds."""
+    comp = (255 - img.astype(np.int32)).astype(np.uint8)
+    marker = np.where(seeds != 0, comp, 0).astype(np.uint8)
+    return (img | recon_fixpoint(marker, comp, conn)).astype(np.uint8)

@@ -299,3 +299,38 @@ def test_default_stream_ordering(nh, orc, gpu):
     r, f, lab, n = nh.segment_chain(k, m)
     er, ef, elab, en = orc.chain_u8(marker, mask)
     assert np.array_equal(lab.cpu().numpy(), elab) and n == en
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_imfill_seeded_and_bwselect(nh, orc, gpu, conn):
+    """Seeded imfill (binary and gray) and bwselect against the oracle, host
+    arrays and device tensors, at reconstruction tile edges."""
+    import torch
+    rng = np.random.default_rng(71 + conn)
+    for shape in [(1, 1), (5, 70), (62, 63), (127, 130), (300, 257)]:
+        bw = np.where(rng.random(shape) < 0.55, 255, 0).astype(np.uint8)
+        g = rng.integers(0, 256, size=shape).astype(np.uint8)
+        seeds = np.where(rng.random(shape) < 0.01, rng.integers(1, 256, shape), 0).astype(np.uint8)
+        for img in (bw, g):
+            for binary in (False, True):
+                want = orc.imfill(img, seeds, binary, conn)
+                assert np.array_equal(nh.imfill(img, seeds, binary, conn), want), (shape, binary)
+                got = nh.imfill(torch.from_numpy(img).cuda(), torch.from_numpy(seeds).cuda(), binary, conn)
+                assert np.array_equal(got.cpu().numpy(), want)
+            want = orc.bwselect(img, seeds, conn)
+            assert np.array_equal(nh.bwselect(img, seeds, conn), want), shape
+            got = nh.bwselect(torch.from_numpy(img).cuda(), torch.from_numpy(seeds).cuda(), conn)
+            assert np.array_equal(got.cpu().numpy(), want)
+
+
+def test_imfill_bwselect_checks(nh, gpu):
+    """Types the reference does not instantiate are refused; empty images are fine."""
+    a = np.zeros((4, 5), np.int32)
+    s = np.zeros((4, 5), np.uint8)
+    with pytest.raises(nh.NsxError):
+        nh.imfill(a, s)
+    with pytest.raises(nh.NsxError):
+        nh.bwselect(np.zeros((4, 5), np.uint8), np.zeros((4, 6), np.uint8))
+    e = np.zeros((0, 7), np.uint8)
+    assert nh.imfill(e, e).shape == e.shape
+    assert nh.bwselect(e, e).shape == e.shape

@@ -258,3 +258,29 @@ def test_chain_i32_scaling_is_order_isomorphic(orc):
     assert np.array_equal(r32, r8.astype(np.int32) * 257)
     assert np.array_equal(f32, f8.astype(np.int32) * 257)
     assert np.array_equal(l32, l8) and n32 == n8
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_bwselect_vs_scipy(orc, conn):
+    """bwselect keeps the components (of the nonzero pixels) that hold a seed,
+    with their values, for binary and multi-valued images."""
+    rng = np.random.default_rng(51 + conn)
+    for shape in [(1, 1), (7, 9), (40, 33)]:
+        for vals in ([0, 255], [0, 3, 200, 255]):
+            img = rng.choice(np.array(vals, np.uint8), size=shape).astype(np.uint8)
+            seeds = np.where(rng.random(shape) < 0.06, rng.integers(1, 256, shape), 0).astype(np.uint8)
+            assert np.array_equal(orc.bwselect(img, seeds, conn), brute.bwselect_scipy(img, seeds, conn))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_imfill_seeded_closed_forms(orc, conn):
+    """Seeded imfill: binary = flood from the seeds through the pixels below 255;
+    gray = image | brute-force reconstruction of the seeded complement."""
+    rng = np.random.default_rng(61 + conn)
+    for shape in [(1, 1), (6, 11), (31, 29)]:
+        bw = np.where(rng.random(shape) < 0.4, 255, 0).astype(np.uint8)
+        g = rng.integers(0, 256, size=shape).astype(np.uint8)
+        seeds = np.where(rng.random(shape) < 0.05, rng.integers(1, 256, shape), 0).astype(np.uint8)
+        for img in (bw, g):
+            assert np.array_equal(orc.imfill(img, seeds, True, conn), brute.imfill_binary_flood(img, seeds, conn))
+            assert np.array_equal(orc.imfill(img, seeds, False, conn), brute.imfill_gray_fixpoint(img, seeds, conn))
