@@ -6,7 +6,9 @@
 // fastIA-master/):
 //     imreconstruct<T>(seeds, image, connectivity)            :21-22
 //     imreconstructBinary<T>(seeds, binaryImage, connectivity) :43-44
+//     imfill<T>(image, seeds, binary, connectivity)            :48
 //     imfillHoles<T>(image, binary, connectivity)              :50-51
+//     bwselect<T>(binaryImage, seeds, connectivity)            :56
 //     bwlabel2(binaryImage, connectivity, relab)               :59
 //     bwareaopen2(binaryImage, labeled, flatten, minSize, maxSize, connectivity, count)  :74
 // At a call site, include this header instead of MorphologicOperations.h for
@@ -112,6 +114,32 @@ cv::Mat imfillHoles(const cv::Mat& image, bool binary, int connectivity) {
     nsx_img i = host(image, Type<T>::nsx), o = host(out, Type<T>::nsx);
     nsx_ctx* c = ctx();
     check(nsx_fill_holes(c, &i, &o, binary ? 1 : 0, connectivity, nullptr), c);
+    return out;
+}
+
+// imfill and bwselect are instantiated for uchar only in the reference
+// (MorphologicOperations.cpp:2516,2522); other T throw.
+template <typename T>
+cv::Mat imfill(const cv::Mat& image, const cv::Mat& seeds, bool binary, int connectivity) {
+    using namespace nsx_detail;
+    require<T>(image, "image");
+    require<unsigned char>(seeds, "seeds");
+    cv::Mat out(image.rows, image.cols, Type<T>::cv);
+    nsx_img i = host(image, Type<T>::nsx), s = host(seeds, NSX_U8), o = host(out, Type<T>::nsx);
+    nsx_ctx* c = ctx();
+    check(nsx_imfill(c, &i, &s, &o, binary ? 1 : 0, connectivity, nullptr), c);
+    return out;
+}
+
+template <typename T>
+cv::Mat bwselect(const cv::Mat& binaryImage, const cv::Mat& seeds, int connectivity) {
+    using namespace nsx_detail;
+    require<T>(binaryImage, "binaryImage");
+    require<unsigned char>(seeds, "seeds");
+    cv::Mat out(binaryImage.rows, binaryImage.cols, Type<T>::cv);
+    nsx_img b = host(binaryImage, Type<T>::nsx), s = host(seeds, NSX_U8), o = host(out, Type<T>::nsx);
+    nsx_ctx* c = ctx();
+    check(nsx_bwselect(c, &b, &s, &o, connectivity, nullptr), c);
     return out;
 }
 

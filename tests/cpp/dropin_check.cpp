@@ -18,13 +18,15 @@ int main(int argc, char** argv) {
     const std::string dir = argc > 1 ? argv[1] : ".";
     const int R = 61, C = 83;
     // mask(y, x) = (x * 37 + y * 91 + ((x * y) >> 3)) & 255; marker = saturating mask - 30
-    cv::Mat mask(R, C, CV_8U), marker(R, C, CV_8U), bin(R, C, CV_8U);
+    // seeds(y, x) = 1 where (13 x + 7 y) % 97 == 0
+    cv::Mat mask(R, C, CV_8U), marker(R, C, CV_8U), bin(R, C, CV_8U), seeds(R, C, CV_8U);
     for (int y = 0; y < R; ++y)
         for (int x = 0; x < C; ++x) {
             const int v = (x * 37 + y * 91 + ((x * y) >> 3)) & 255;
             mask.at<unsigned char>(y, x) = (unsigned char)v;
             marker.at<unsigned char>(y, x) = (unsigned char)(v > 30 ? v - 30 : 0);
             bin.at<unsigned char>(y, x) = v > 128 ? 255 : 0;
+            seeds.at<unsigned char>(y, x) = (x * 13 + y * 7) % 97 == 0 ? 1 : 0;
         }
     try {
         save(dir + "/recon8.bin", nscale::imreconstruct<unsigned char>(marker, mask, 8), 1);
@@ -32,6 +34,9 @@ int main(int argc, char** argv) {
         save(dir + "/reconbin8.bin", nscale::imreconstructBinary<unsigned char>(marker, bin, 8), 1);
         save(dir + "/fill4.bin", nscale::imfillHoles<unsigned char>(mask, false, 4), 1);
         save(dir + "/fillbin8.bin", nscale::imfillHoles<unsigned char>(bin, true, 8), 1);
+        save(dir + "/imfill4.bin", nscale::imfill<unsigned char>(mask, seeds, false, 4), 1);
+        save(dir + "/imfillbin8.bin", nscale::imfill<unsigned char>(bin, seeds, true, 8), 1);
+        save(dir + "/bwselect8.bin", nscale::bwselect<unsigned char>(bin, seeds, 8), 1);
         save(dir + "/label8.bin", nscale::bwlabel2(bin, 8, true), 4);
         int count = 0;
         save(dir + "/area8.bin", nscale::bwareaopen2(bin, false, false, 3, 40, 8, count), 4);

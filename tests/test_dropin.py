@@ -20,6 +20,11 @@ def inputs():
     return marker, mask, binimg
 
 
+def seeds_of():
+    y, x = np.mgrid[0:R, 0:C]
+    return ((x * 13 + y * 7) % 97 == 0).astype(np.uint8)
+
+
 def test_dropin_header_compiles_and_fails_cleanly_without_gpu(tmp_path):
     """Compiled against the in-tree library; without a GPU the first call
     throws the library's error (exit 3), with one it runs (exit 0)."""
@@ -53,6 +58,10 @@ def test_dropin_matches_oracle(orc, tmp_path):
     assert np.array_equal(load("reconbin8.bin", np.uint8), orc.imreconstruct_binary(marker, binimg, 8))
     assert np.array_equal(load("fill4.bin", np.uint8), orc.fill_holes(mask, False, 4))
     assert np.array_equal(load("fillbin8.bin", np.uint8), orc.fill_holes(binimg, True, 8))
+    seeds = seeds_of()
+    assert np.array_equal(load("imfill4.bin", np.uint8), orc.imfill(mask, seeds, False, 4))
+    assert np.array_equal(load("imfillbin8.bin", np.uint8), orc.imfill(binimg, seeds, True, 8))
+    assert np.array_equal(load("bwselect8.bin", np.uint8), orc.bwselect(binimg, seeds, 8))
     assert np.array_equal(load("label8.bin", np.int32), orc.bwlabel2(binimg, 8, True)[0])
     want_area, want_n = orc.bwareaopen2(binimg, False, False, 3, 40, 8)
     assert np.array_equal(load("area8.bin", np.int32), want_area)
