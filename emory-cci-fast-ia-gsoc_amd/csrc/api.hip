@@ -4,6 +4,7 @@
 #include "nsx_internal.h"
 
 #include <climits>
+#include <cmath>
 #include <cstring>
 #include <new>
 
@@ -177,11 +178,11 @@ nsx_status seeded_setup(nsx_ctx* ctx, const nsx_img* img, const nsx_img* seeds, 
     return NSX_OK;
 }
 
-nsx_status scratch_u8(nsx_ctx* ctx, int slot, const View& like, View* v) {
+nsx_status scratch_like(nsx_ctx* ctx, int slot, const View& like, View* v) {
     void* p;
-    const size_t pitch = like.cols > 0 ? (size_t)like.cols : 1;
+    const size_t pitch = like.cols > 0 ? (size_t)like.cols * dtype_size(like.dtype) : 1;
     NSX_TRY(ctx->ws(slot, pitch * (size_t)(like.rows > 0 ? like.rows : 1), &p));
-    *v = View{(char*)p, (int64_t)pitch, like.rows, like.cols, NSX_U8};
+    *v = View{(char*)p, (int64_t)pitch, like.rows, like.cols, like.dtype};
     return NSX_OK;
 }
 
@@ -379,11 +380,35 @@ nsx_status nsx_imfill(nsx_ctx* ctx, const nsx_img* image, const nsx_img* seeds, 
     NSX_TRY(to_device(ctx, image, WS_STAGE_B, true, s, &vi));
     NSX_TRY(to_device(ctx, seeds, WS_STAGE_A, true, s, &vs));
     NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
-    NSX_TRY(scratch_u8(ctx, WS_CHAIN_RECON, vi, &vm));
-    NSX_TRY(scratch_u8(ctx, WS_CHAIN_FILL, vi, &vk));
+    NSX_TRY(scratch_like(ctx, WS_CHAIN_RECON, vi, &vm));
+    NSX_TRY(scratch_like(ctx, WS_CHAIN_FILL, vi, &vk));
     NSX_TRY(seeded_marker_run(ctx, vi, vs, vm, &vk, s));
     NSX_TRY(recon_run(ctx, binary ? RM_BINARY : RM_RECON, &vm, vk, vo, conn == 8 ? 8 : 4, s));
     NSX_TRY(bitwise_u8_run(ctx, vi, vo, false, s));
+    return finish_recon(ctx, vo, out, s);
+}
+
+nsx_status nsx_imhmin(nsx_ctx* ctx, const nsx_img* image, double h, nsx_img* out, int conn, void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(check_img(ctx, image, "image"));
+    NSX_TRY(check_img(ctx, out, "out"));
+    if (image->dtype != NSX_U8 && image->dtype != NSX_F32)
+        return ctx->fail(NSX_ERR_UNSUPPORTED, "imhmin is instantiated for uchar and float only");
+    if (!same_shape(image, out) || image->dtype != out->dtype)
+        return ctx->fail(NSX_ERR_INVALID, "image and out must have the same size and type");
+    if (image->dtype == NSX_U8 && !(h >= 0.0 && h <= 255.0 && h == std::floor(h)))
+        return ctx->fail(NSX_ERR_INVALID, "uchar imhmin takes an integer h in 0..255");
+    if (overlaps(out, image)) return ctx->fail(NSX_ERR_INVALID, "out aliases image");
+    if (is_empty(image)) return NSX_OK;
+    hipStream_t s = pick_stream(ctx, stream);
+    View vi, vo, vk, vm;
+    NSX_TRY(to_device(ctx, image, WS_STAGE_B, true, s, &vi));
+    NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
+    NSX_TRY(scratch_like(ctx, WS_CHAIN_FILL, vi, &vk));
+    NSX_TRY(scratch_like(ctx, WS_CHAIN_RECON, vi, &vm));
+    NSX_TRY(hmin_prep_run(ctx, vi, h, vk, vm, s));
+    NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vo, conn == 8 ? 8 : 4, s));
+    NSX_TRY(invert_run(ctx, vo, s));
     return finish_recon(ctx, vo, out, s);
 }
 
@@ -397,7 +422,7 @@ nsx_status nsx_bwselect(nsx_ctx* ctx, const nsx_img* binary_image, const nsx_img
     NSX_TRY(to_device(ctx, binary_image, WS_STAGE_B, true, s, &vi));
     NSX_TRY(to_device(ctx, seeds, WS_STAGE_A, true, s, &vs));
     NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
-    NSX_TRY(scratch_u8(ctx, WS_CHAIN_RECON, vi, &vm));
+    NSX_TRY(scratch_like(ctx, WS_CHAIN_RECON, vi, &vm));
     NSX_TRY(seeded_marker_run(ctx, vi, vs, vm, nullptr, s));
     NSX_TRY(recon_run(ctx, RM_BINARY, &vm, vi, vo, conn == 8 ? 8 : 4, s));
     NSX_TRY(bitwise_u8_run(ctx, vi, vo, true, s));

@@ -1194,6 +1194,32 @@ __global__ void seeded_marker_kernel(const char* img, int64_t pi, const char* se
     ((uint8_t*)(marker + (int64_t)y * pm))[x] = ((const uint8_t*)(seeds + (int64_t)y * ps))[x] != 0 ? v : 0;
 }
 
+// imhmin's inputs: mask = invert(image) (u8 255 - x, f32 0 - x), marker =
+// mask - h (saturating for u8).  dtype NSX_U8 or NSX_F32.
+__global__ void hmin_prep_kernel(const char* img, int64_t pi, char* mask, int64_t pk, char* marker, int64_t pm,
+                                 int rows, int cols, int dtype, int hu, float hf) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= cols) return;
+    if (dtype == NSX_U8) {
+        const int v = 255 - ((const uint8_t*)(img + (int64_t)y * pi))[x];
+        ((uint8_t*)(mask + (int64_t)y * pk))[x] = (uint8_t)v;
+        ((uint8_t*)(marker + (int64_t)y * pm))[x] = (uint8_t)(v > hu ? v - hu : 0);
+    } else {
+        const float v = 0.0f - ((const float*)(img + (int64_t)y * pi))[x];
+        ((float*)(mask + (int64_t)y * pk))[x] = v;
+        ((float*)(marker + (int64_t)y * pm))[x] = v - hf;
+    }
+}
+
+// out = invert(out) in place (u8 255 - x, f32 0 - x).
+__global__ void invert_kernel(char* out, int64_t po, int rows, int cols, int dtype) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= cols) return;
+    char* row = out + (int64_t)y * po;
+    if (dtype == NSX_U8) ((uint8_t*)row)[x] = (uint8_t)(255 - ((uint8_t*)row)[x]);
+    else ((float*)row)[x] = 0.0f - ((float*)row)[x];
+}
+
 // out = image | out (op 0) or image & out (op 1), u8.
 __global__ void bitwise_u8_kernel(const char* img, int64_t pi, char* out, int64_t po, int rows, int cols, int op) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
@@ -1280,6 +1306,24 @@ nsx_status seeded_marker_run(nsx_ctx* ctx, const View& img, const View& seeds, c
     hipLaunchKernelGGL(seeded_marker_kernel, grid, dim3(256), 0, s, img.data, img.pitch, seeds.data, seeds.pitch,
                        marker.data, marker.pitch, mask ? mask->data : nullptr, mask ? mask->pitch : 0, img.rows,
                        img.cols, mask ? 1 : 0);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    return NSX_OK;
+}
+
+nsx_status hmin_prep_run(nsx_ctx* ctx, const View& img, double h, const View& mask, const View& marker,
+                         hipStream_t s) {
+    if (img.rows == 0 || img.cols == 0) return NSX_OK;
+    const dim3 grid((img.cols + 255) / 256, img.rows);
+    hipLaunchKernelGGL(hmin_prep_kernel, grid, dim3(256), 0, s, img.data, img.pitch, mask.data, mask.pitch,
+                       marker.data, marker.pitch, img.rows, img.cols, img.dtype, (int)h, (float)h);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    return NSX_OK;
+}
+
+nsx_status invert_run(nsx_ctx* ctx, const View& out, hipStream_t s) {
+    if (out.rows == 0 || out.cols == 0) return NSX_OK;
+    const dim3 grid((out.cols + 255) / 256, out.rows);
+    hipLaunchKernelGGL(invert_kernel, grid, dim3(256), 0, s, out.data, out.pitch, out.rows, out.cols, out.dtype);
     NSX_HIP_TRY(ctx, hipGetLastError());
     return NSX_OK;
 }

@@ -287,6 +287,11 @@ nsx_status threshold_run(nsx_ctx* ctx, const View& in, double k, const View& out
 // seeds != 0, else 0; with `mask`, 255 - img is also written there.
 nsx_status seeded_marker_run(nsx_ctx* ctx, const View& img, const View& seeds, const View& marker, const View* mask,
                              hipStream_t s);
+// imhmin's mask = invert(img) and marker = mask - h (u8 saturating, or f32).
+nsx_status hmin_prep_run(nsx_ctx* ctx, const View& img, double h, const View& mask, const View& marker,
+                         hipStream_t s);
+// out = invert(out) in place (u8 or f32).
+nsx_status invert_run(nsx_ctx* ctx, const View& out, hipStream_t s);
 // out = img | out, or img & out (and_op), u8.
 nsx_status bitwise_u8_run(nsx_ctx* ctx, const View& img, const View& out, bool and_op, hipStream_t s);
 

@@ -28,7 +28,7 @@ from ._lib import NsxImg
 
 __all__ = [
     "NsxError", "Context", "default_context",
-    "imreconstruct", "imreconstructBinary", "imfillHoles", "fillHoles", "imfill", "bwselect",
+    "imreconstruct", "imreconstructBinary", "imfillHoles", "fillHoles", "imfill", "bwselect", "imhmin",
     "bwlabel2", "bwlabel", "bwareaopen2", "relabel", "threshold_gt", "segment_chain",
     "gen_disks_u8", "gen_marker_u8", "gen_maze_u8", "scale_i32", "scale_f32",
 ]
@@ -248,6 +248,15 @@ def imfill(image, seeds, binary: bool = False, connectivity: int = 4, *, out=Non
     out = _like(image, np.uint8) if out is None else out
     c.check(c.lib.nsx_imfill(c.h, C.byref(describe(image)), C.byref(describe(seeds)), C.byref(describe(out)),
                              int(bool(binary)), connectivity, _torch_stream(image, stream)))
+    return out
+
+
+def imhmin(image, h, connectivity: int = 8, *, out=None, ctx=None, stream=None):
+    """Suppression of minima shallower than h (nscale::imhmin<T>, uchar or float)."""
+    c = _ctx(image, ctx)
+    out = _like(image, _np_dtype(image)) if out is None else out
+    c.check(c.lib.nsx_imhmin(c.h, C.byref(describe(image)), float(h), C.byref(describe(out)), connectivity,
+                             _torch_stream(image, stream)))
     return out
 
 

@@ -9,6 +9,7 @@
 //     imfill<T>(image, seeds, binary, connectivity)            :48
 //     imfillHoles<T>(image, binary, connectivity)              :50-51
 //     bwselect<T>(binaryImage, seeds, connectivity)            :56
+//     imhmin<T>(image, h, connectivity)                        :78
 //     bwlabel2(binaryImage, connectivity, relab)               :59
 //     bwareaopen2(binaryImage, labeled, flatten, minSize, maxSize, connectivity, count)  :74
 // At a call site, include this header instead of MorphologicOperations.h for
@@ -140,6 +141,17 @@ cv::Mat bwselect(const cv::Mat& binaryImage, const cv::Mat& seeds, int connectiv
     nsx_img b = host(binaryImage, Type<T>::nsx), s = host(seeds, NSX_U8), o = host(out, Type<T>::nsx);
     nsx_ctx* c = ctx();
     check(nsx_bwselect(c, &b, &s, &o, connectivity, nullptr), c);
+    return out;
+}
+
+template <typename T>
+cv::Mat imhmin(const cv::Mat& image, T h, int connectivity) {
+    using namespace nsx_detail;
+    require<T>(image, "image");
+    cv::Mat out(image.rows, image.cols, Type<T>::cv);
+    nsx_img i = host(image, Type<T>::nsx), o = host(out, Type<T>::nsx);
+    nsx_ctx* c = ctx();
+    check(nsx_imhmin(c, &i, (double)h, &o, connectivity, nullptr), c);
     return out;
 }
 

@@ -135,6 +135,14 @@ nsx_status nsx_imfill(nsx_ctx* ctx, const nsx_img* image, const nsx_img* seeds, 
 nsx_status nsx_bwselect(nsx_ctx* ctx, const nsx_img* binary_image, const nsx_img* seeds, nsx_img* out, int conn,
                         void* stream);
 
+/* imhmin (MorphologicOperations.cpp:2260-2282, uchar and float, :2527-2528):
+ * suppresses the minima shallower than h.  mask = invert(image) (uchar 255 - x,
+ * float -x); out = invert(imreconstruct(mask - h, mask, conn)), with uchar
+ * mask - h saturating at 0 (uchar h: an integer in 0..255).  Float images may
+ * be negative; the reconstruction's constant 0 frame applies as in the
+ * reference. */
+nsx_status nsx_imhmin(nsx_ctx* ctx, const nsx_img* image, double h, nsx_img* out, int conn, void* stream);
+
 /* ---- connected components ------------------------------------------------- */
 /* Union-find labelling of a u8 image: 4/8-neighbours of equal nonzero value are
  * joined.  relabel=0: label = raster index of the component's first pixel,

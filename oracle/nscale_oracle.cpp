@@ -23,6 +23,10 @@
 //                           through the complement (binary).
 //   or_bwselect_u8          the components of {image != 0} holding a seed
 //                           (scipy ndimage.label), with the image's values.
+//   or_imhmin_*             invert(brute-force reconstruction of
+//                           (invert(image) - h) under invert(image), 0 frame).
+//   float invert's zero     parity unpinned: -(+0.0) is taken as +0.0 (see
+//                           invert_one(float)); values are equal either way.
 //   or_bwlabel / relabel /  the reference's own ConnComponents.cpp compiled
 //   or_bwareaopen           unmodified into oracle/_ref (exact), scipy
 //                           ndimage.label partitions, and the canonical-numbering
@@ -296,7 +300,10 @@ void imreconstructBinary(const T* seeds, const T* image, T* out, int rows, int c
 // inputs are exact two's-complement negation.  (Parity unpinned at INT_MIN.)
 inline uint8_t invert_one(uint8_t v) { return (uint8_t)(255 - v); }
 inline int32_t invert_one(int32_t v) { return v == INT_MIN ? INT_MAX : -v; }
-inline float invert_one(float v) { return -v; }
+// OpenCV's unary minus on a Mat is a scale by -1 plus a shift of 0, which
+// turns -0.0 into +0.0; 0.0f - v gives the same bits (OpenCV is absent here, so
+// the sign of a zero result is unpinned; the value is exact).
+inline float invert_one(float v) { return 0.0f - v; }
 
 template <typename T>
 void invert_vec(std::vector<T>& v) {
@@ -353,6 +360,23 @@ void bwselect(const T* image, const uint8_t* seeds, T* out, int rows, int cols, 
     std::vector<T> rec(n);
     imreconstructBinary<T>(marker.data(), image, rec.data(), rows, cols, connectivity);
     for (size_t i = 0; i < n; ++i) out[i] = (T)(rec[i] & image[i]);
+}
+
+// imhmin<T> (MorphologicOperations.cpp:2260-2282, instantiated for uchar and
+// float at :2527-2528): mask = invert(image) (:2273); marker = mask - h
+// (:2274, OpenCV Mat - scalar: saturating for uchar); output =
+// invert(imreconstruct(marker, mask, conn)) (:2279-2280).
+inline uint8_t sub_h(uint8_t v, uint8_t h) { return v > h ? (uint8_t)(v - h) : (uint8_t)0; }
+inline float sub_h(float v, float h) { return v - h; }
+template <typename T>
+void imhmin(const T* image, T h, T* out, int rows, int cols, int connectivity) {
+    const size_t n = (size_t)rows * cols;
+    std::vector<T> mask(image, image + n);
+    invert_vec(mask);
+    std::vector<T> marker(n);
+    for (size_t i = 0; i < n; ++i) marker[i] = sub_h(mask[i], h);
+    imreconstruct<T>(marker.data(), mask.data(), out, rows, cols, connectivity);
+    for (size_t i = 0; i < n; ++i) out[i] = invert_one(out[i]);
 }
 
 // ---- ConnComponents (src/segment/ConnComponents.cpp) ----------------------
@@ -489,6 +513,14 @@ int or_imfill_u8(const uint8_t* in, const uint8_t* seeds, uint8_t* out, int rows
 }
 int or_bwselect_u8(const uint8_t* in, const uint8_t* seeds, uint8_t* out, int rows, int cols, int conn) {
     bwselect<uint8_t>(in, seeds, out, rows, cols, conn);
+    return 0;
+}
+int or_imhmin_u8(const uint8_t* in, int h, uint8_t* out, int rows, int cols, int conn) {
+    imhmin<uint8_t>(in, (uint8_t)h, out, rows, cols, conn);
+    return 0;
+}
+int or_imhmin_f32(const float* in, float h, float* out, int rows, int cols, int conn) {
+    imhmin<float>(in, h, out, rows, cols, conn);
     return 0;
 }
 void or_invert_i32(const int32_t* in, int32_t* out, long n) {

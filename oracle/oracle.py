@@ -91,6 +91,19 @@ def bwselect(img, seeds, conn=8):
     return out
 
 
+def imhmin(img, h, conn=8):
+    """imhmin<T>(image, h, conn) for uchar (integer h in 0..255) and float."""
+    img = np.ascontiguousarray(img)
+    out = np.empty_like(img)
+    if img.dtype == np.uint8:
+        lib().or_imhmin_u8(_p(img), int(h), _p(out), img.shape[0], img.shape[1], conn)
+    else:
+        img = _c(img, np.float32)
+        out = np.empty_like(img)
+        lib().or_imhmin_f32(_p(img), C.c_float(h), _p(out), img.shape[0], img.shape[1], conn)
+    return out
+
+
 def bwlabel2(binimg, conn=8, relab=True):
     b = _c(binimg, np.uint8)
     out = np.empty(b.shape, np.int32)

@@ -145,3 +145,15 @@ def imfill_gray_fixpoint(img: np.ndarray, seeds: np.ndarray, conn: int) -> np.nd
     comp = (255 - img.astype(np.int32)).astype(np.uint8)
     marker = np.where(seeds != 0, comp, 0).astype(np.uint8)
     return (img | recon_fixpoint(marker, comp, conn)).astype(np.uint8)
+
+
+def imhmin_fixpoint(img: np.ndarray, h, conn: int) -> np.ndarray:
+    """imhmin from the textbook form: complement, subtract h (saturating for
+    u8), geodesic reconstruction with a 0 frame, complement back."""
+    if img.dtype == np.uint8:
+        comp = (255 - img.astype(np.int32)).astype(np.uint8)
+        marker = np.maximum(comp.astype(np.int32) - int(h), 0).astype(np.uint8)
+        return (255 - recon_fixpoint(marker, comp, conn).astype(np.int32)).astype(np.uint8)
+    comp = (np.float32(0) - img).astype(np.float32)
+    marker = (comp - np.float32(h)).astype(np.float32)
+    return (np.float32(0) - recon_fixpoint(marker, comp, conn, frame=np.float32(0))).astype(np.float32)

@@ -334,3 +334,25 @@ def test_imfill_bwselect_checks(nh, gpu):
     e = np.zeros((0, 7), np.uint8)
     assert nh.imfill(e, e).shape == e.shape
     assert nh.bwselect(e, e).shape == e.shape
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_imhmin(nh, orc, gpu, conn):
+    """imhmin (uchar and float, negative floats included) against the oracle,
+    host arrays and device tensors; uchar h must be an integer in 0..255."""
+    import torch
+    rng = np.random.default_rng(91 + conn)
+    for shape in [(1, 1), (61, 62), (130, 257)]:
+        u = rng.integers(0, 256, size=shape).astype(np.uint8)
+        for h in (0, 9, 255):
+            want = orc.imhmin(u, h, conn)
+            assert np.array_equal(nh.imhmin(u, h, conn), want), (shape, h)
+            assert np.array_equal(nh.imhmin(torch.from_numpy(u).cuda(), h, conn).cpu().numpy(), want)
+        f = (rng.integers(-400, 400, size=shape) / np.float32(64)).astype(np.float32)
+        for h in (0.5, 2.0):
+            want = orc.imhmin(f, h, conn)
+            assert np.array_equal(nh.imhmin(f, h, conn), want), (shape, h)
+    with pytest.raises(nh.NsxError):
+        nh.imhmin(np.zeros((3, 3), np.uint8), 1.5)
+    with pytest.raises(nh.NsxError):
+        nh.imhmin(np.zeros((3, 3), np.int32), 1)

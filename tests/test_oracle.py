@@ -284,3 +284,25 @@ def test_imfill_seeded_closed_forms(orc, conn):
         for img in (bw, g):
             assert np.array_equal(orc.imfill(img, seeds, True, conn), brute.imfill_binary_flood(img, seeds, conn))
             assert np.array_equal(orc.imfill(img, seeds, False, conn), brute.imfill_gray_fixpoint(img, seeds, conn))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_imhmin_closed_form(orc, conn):
+    """imhmin against complement / subtract / brute-force reconstruction /
+    complement, for u8 and for float images with negative values (where the
+    reconstruction's 0 frame shows)."""
+    rng = np.random.default_rng(81 + conn)
+    for shape in [(1, 1), (9, 7), (27, 31)]:
+        u = rng.integers(0, 256, size=shape).astype(np.uint8)
+        for h in (0, 1, 17, 255):
+            assert np.array_equal(orc.imhmin(u, h, conn), brute.imhmin_fixpoint(u, h, conn))
+        f = (rng.integers(-400, 400, size=shape) / np.float32(64)).astype(np.float32)
+        for h in (0.25, 3.0):
+            got = orc.imhmin(f, h, conn)
+            assert np.array_equal(got, brute.imhmin_fixpoint(f, h, conn))
+    # the 0 frame matters for negative floats: a frame of -1000 changes the answer
+    f = -(rng.integers(1, 400, size=(15, 13)) / np.float32(8)).astype(np.float32)
+    comp = (np.float32(0) - f).astype(np.float32)
+    other = np.float32(0) - brute.recon_fixpoint((comp - np.float32(2)).astype(np.float32), comp, conn,
+                                                  frame=np.float32(1000))
+    assert not np.array_equal(orc.imhmin(f, 2.0, conn), other)
