@@ -412,6 +412,20 @@ nsx_status nsx_imhmin(nsx_ctx* ctx, const nsx_img* image, double h, nsx_img* out
     return finish_recon(ctx, vo, out, s);
 }
 
+nsx_status nsx_bounding_box(nsx_ctx* ctx, const nsx_img* labels, int bgval, int32_t* boxes, int capacity,
+                            int32_t* count, void* stream) {
+    if (!ctx || !count) return NSX_ERR_INVALID;
+    NSX_TRY(check_img(ctx, labels, "labels"));
+    if (labels->dtype != NSX_I32) return ctx->fail(NSX_ERR_INVALID, "labels must be int32");
+    if (boxes && capacity < 0) return ctx->fail(NSX_ERR_INVALID, "negative capacity");
+    *count = 0;
+    if (is_empty(labels)) return NSX_OK;
+    hipStream_t s = pick_stream(ctx, stream);
+    View vl;
+    NSX_TRY(to_device(ctx, labels, WS_STAGE_B, true, s, &vl));
+    return bbox_run(ctx, vl, bgval, boxes, capacity, count, s);
+}
+
 nsx_status nsx_bwselect(nsx_ctx* ctx, const nsx_img* binary_image, const nsx_img* seeds, nsx_img* out, int conn,
                         void* stream) {
     if (!ctx) return NSX_ERR_INVALID;

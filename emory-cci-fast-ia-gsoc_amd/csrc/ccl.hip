@@ -1194,6 +1194,97 @@ __global__ void seeded_marker_kernel(const char* img, int64_t pi, const char* se
     ((uint8_t*)(marker + (int64_t)y * pm))[x] = ((const uint8_t*)(seeds + (int64_t)y * ps))[x] != 0 ? v : 0;
 }
 
+// ---- bounding boxes (ConnComponents::boundingBox) ------------------------------
+// misc: [0] raster index of the first label -1 or 0 that is not the background
+// (the reference stops there), [1] first raster index of a label outside
+// [-1, rows*cols], [2] the largest label.  One atomic per wave and word.
+__global__ void bbox_scan_kernel(const char* lab, int64_t pl, int rows, int cols, int bgval, int32_t* misc) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    const int32_t i = y * cols + x;
+    int32_t stop = INT_MAX, bad = INT_MAX, mx = 0;
+    if (x < cols) {
+        const int32_t v = ((const int32_t*)(lab + (int64_t)y * pl))[x];
+        if (v != bgval) {
+            if (v == -1 || v == 0) stop = i;
+            else if (v < -1 || (int64_t)v > (int64_t)rows * cols) bad = i;
+            else mx = v;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        stop = min(stop, __shfl_xor(stop, o, 64));
+        bad = min(bad, __shfl_xor(bad, o, 64));
+        mx = max(mx, __shfl_xor(mx, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (stop != INT_MAX) atomicMin(&misc[0], stop);
+        if (bad != INT_MAX) atomicMin(&misc[1], bad);
+        if (mx > 0) atomicMax(&misc[2], mx);
+    }
+}
+
+__global__ void bbox_misc_init_kernel(int32_t* misc) {
+    if (threadIdx.x == 0) { misc[0] = INT_MAX; misc[1] = INT_MAX; misc[2] = 0; }
+}
+
+// tab[5][cap]: first raster index, min x, max x, min y, max y per label value
+__global__ void bbox_init_kernel(int32_t* tab, int64_t cap) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 5 * cap; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = i / cap;
+        tab[i] = (c == 2 || c == 4) ? -1 : INT_MAX;
+    }
+}
+
+// Runs of one label along a row, within each wave's 64 columns: the run's
+// first lane updates the label's box once.  Pixels from the stop index on
+// count as background.
+__global__ void bbox_runs_kernel(const char* lab, int64_t pl, int rows, int cols, int bgval, const int32_t* misc,
+                                 int32_t* tab, int64_t cap) {
+    const int lane = threadIdx.x & 63;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    const int32_t i = y * cols + x;
+    int32_t v = bgval;
+    if (x < cols && i < misc[0]) v = ((const int32_t*)(lab + (int64_t)y * pl))[x];
+    const int32_t left = __shfl_up(v, 1, 64), right = __shfl_down(v, 1, 64);
+    const bool fg = v != bgval;
+    const bool start = fg && (lane == 0 || left != v);
+    const bool end = fg && (lane == 63 || right != v);
+    const uint64_t ends = __ballot(end);
+    if (start) {
+        const int xe = x + __builtin_ctzll(ends >> lane);
+        atomicMin(&tab[v], i);
+        atomicMin(&tab[cap + v], x);
+        atomicMax(&tab[2 * cap + v], xe);
+        atomicMin(&tab[3 * cap + v], y);
+        atomicMax(&tab[4 * cap + v], y);
+    }
+}
+
+// Marks each present label's first raster index in a bitmap over positions.
+__global__ void bbox_mark_kernel(const int32_t* tab, int64_t cap, uint32_t* bits) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < cap; v += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t f = tab[v];
+        if (f != INT_MAX) atomicOr(&bits[f >> 5], 1u << (f & 31));
+    }
+}
+
+// Writes the boxes in order of first appearance: the rank of a label's first
+// position among all first positions is its slot; n = blockpre[nb].
+__global__ void bbox_scatter_kernel(const int32_t* tab, int64_t cap, const uint32_t* bits, const uint32_t* wordpre,
+                                    const uint32_t* blockpre, int nb, int32_t* out) {
+    const int n = (int)blockpre[nb];
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < cap; v += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t f = tab[v];
+        if (f == INT_MAX) continue;
+        const int r = (int)(blockpre[f >> 15] + wordpre[f >> 5] + __popc(bits[f >> 5] & ((1u << (f & 31)) - 1u)));
+        out[r] = (int32_t)v;
+        out[n + r] = tab[cap + v];
+        out[2 * n + r] = tab[2 * cap + v];
+        out[3 * n + r] = tab[3 * cap + v];
+        out[4 * n + r] = tab[4 * cap + v];
+    }
+}
+
 // imhmin's inputs: mask = invert(image) (u8 255 - x, f32 0 - x), marker =
 // mask - h (saturating for u8).  dtype NSX_U8 or NSX_F32.
 __global__ void hmin_prep_kernel(const char* img, int64_t pi, char* mask, int64_t pk, char* marker, int64_t pm,
@@ -1317,6 +1408,56 @@ nsx_status hmin_prep_run(nsx_ctx* ctx, const View& img, double h, const View& ma
     hipLaunchKernelGGL(hmin_prep_kernel, grid, dim3(256), 0, s, img.data, img.pitch, mask.data, mask.pitch,
                        marker.data, marker.pitch, img.rows, img.cols, img.dtype, (int)h, (float)h);
     NSX_HIP_TRY(ctx, hipGetLastError());
+    return NSX_OK;
+}
+
+nsx_status bbox_run(nsx_ctx* ctx, const View& lab, int bgval, int32_t* boxes, int capacity, int32_t* count,
+                    hipStream_t s) {
+    *count = 0;
+    const int64_t npix = (int64_t)lab.rows * lab.cols;
+    if (npix == 0) return NSX_OK;
+    void *misc_p, *tab_p, *bits_p, *scan_p, *out_p;
+    NSX_TRY(ctx->ws(WS_CCL_MISC, 64, &misc_p));
+    int32_t* misc = (int32_t*)misc_p;
+    const dim3 grid((lab.cols + 255) / 256, lab.rows);
+    hipLaunchKernelGGL(bbox_misc_init_kernel, dim3(1), dim3(64), 0, s, misc);
+    hipLaunchKernelGGL(bbox_scan_kernel, grid, dim3(256), 0, s, lab.data, lab.pitch, lab.rows, lab.cols, bgval, misc);
+    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, misc, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+    const int32_t stop = ctx->h_pinned[40], bad = ctx->h_pinned[41], maxv = ctx->h_pinned[42];
+    if (bad < stop)
+        return ctx->fail(NSX_ERR_UNSUPPORTED, "boundingBox labels must lie in [-1, rows*cols] (or be the background)");
+    const int64_t cap = (int64_t)maxv + 1;
+    const int64_t nwords = (npix + 31) / 32, nb = (nwords + 1023) / 1024;
+    NSX_TRY(ctx->ws(WS_CCL_LABEL, sizeof(int32_t) * 5 * (size_t)cap, &tab_p));
+    NSX_TRY(ctx->ws(WS_CCL_BITS, sizeof(uint32_t) * (size_t)nwords, &bits_p));
+    NSX_TRY(ctx->ws(WS_CCL_SCAN, sizeof(uint32_t) * (size_t)(nwords + nb + 1), &scan_p));
+    NSX_TRY(ctx->ws(WS_CCL_AREA, sizeof(int32_t) * 5 * (size_t)cap, &out_p));
+    int32_t* tab = (int32_t*)tab_p;
+    uint32_t* bits = (uint32_t*)bits_p;
+    uint32_t* wordpre = (uint32_t*)scan_p;
+    uint32_t* blockpre = wordpre + nwords;
+    const int g = 4 * ctx->cu_count;
+    hipLaunchKernelGGL(bbox_init_kernel, dim3(g), dim3(256), 0, s, tab, cap);
+    hipLaunchKernelGGL(bbox_runs_kernel, grid, dim3(256), 0, s, lab.data, lab.pitch, lab.rows, lab.cols, bgval,
+                       (const int32_t*)misc, tab, cap);
+    hipLaunchKernelGGL(zero_words_kernel, dim3(g), dim3(256), 0, s, bits, nwords);
+    hipLaunchKernelGGL(bbox_mark_kernel, dim3(g), dim3(256), 0, s, (const int32_t*)tab, cap, bits);
+    hipLaunchKernelGGL(scan_words_kernel, dim3(nb), dim3(256), 0, s, (const uint32_t*)bits, nwords, wordpre, blockpre);
+    hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, s, blockpre, (int)nb);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, blockpre + nb, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+    const int32_t n = ctx->h_pinned[40];
+    *count = n;
+    if (!boxes || n == 0) return NSX_OK;
+    if (n > capacity) return ctx->fail(NSX_ERR_INVALID, "boxes holds fewer components than the image has");
+    int32_t* out = (int32_t*)out_p;
+    hipLaunchKernelGGL(bbox_scatter_kernel, dim3(g), dim3(256), 0, s, (const int32_t*)tab, cap,
+                       (const uint32_t*)bits, (const uint32_t*)wordpre, (const uint32_t*)blockpre, (int)nb, out);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    NSX_HIP_TRY(ctx, hipMemcpyAsync(boxes, out, sizeof(int32_t) * 5 * (size_t)n, hipMemcpyDeviceToHost, s));
+    NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
     return NSX_OK;
 }
 

@@ -290,6 +290,11 @@ nsx_status seeded_marker_run(nsx_ctx* ctx, const View& img, const View& seeds, c
 // imhmin's mask = invert(img) and marker = mask - h (u8 saturating, or f32).
 nsx_status hmin_prep_run(nsx_ctx* ctx, const View& img, double h, const View& mask, const View& marker,
                          hipStream_t s);
+// ConnComponents::boundingBox of int32 labels into host `boxes` (5 x count
+// ints: labels, min x, max x, min y, max y in order of first appearance);
+// boxes == nullptr: count only.  Synchronises.
+nsx_status bbox_run(nsx_ctx* ctx, const View& lab, int bgval, int32_t* boxes, int capacity, int32_t* count,
+                    hipStream_t s);
 // out = invert(out) in place (u8 or f32).
 nsx_status invert_run(nsx_ctx* ctx, const View& out, hipStream_t s);
 // out = img | out, or img & out (and_op), u8.

@@ -28,7 +28,7 @@ from ._lib import NsxImg
 
 __all__ = [
     "NsxError", "Context", "default_context",
-    "imreconstruct", "imreconstructBinary", "imfillHoles", "fillHoles", "imfill", "bwselect", "imhmin",
+    "imreconstruct", "imreconstructBinary", "imfillHoles", "fillHoles", "imfill", "bwselect", "imhmin", "bounding_box",
     "bwlabel2", "bwlabel", "bwareaopen2", "relabel", "threshold_gt", "segment_chain",
     "gen_disks_u8", "gen_marker_u8", "gen_maze_u8", "scale_i32", "scale_f32",
 ]
@@ -249,6 +249,21 @@ def imfill(image, seeds, binary: bool = False, connectivity: int = 4, *, out=Non
     c.check(c.lib.nsx_imfill(c.h, C.byref(describe(image)), C.byref(describe(seeds)), C.byref(describe(out)),
                              int(bool(binary)), connectivity, _torch_stream(image, stream)))
     return out
+
+
+def bounding_box(labels, bgval: int = 0, *, ctx=None, stream=None):
+    """ConnComponents::boundingBox of an int32 label image: (count, 5 x count
+    int32 array of labels, min x, max x, min y, max y in order of first
+    appearance in raster order)."""
+    c = _ctx(labels, ctx)
+    n = C.c_int32(0)
+    img = describe(labels)
+    c.check(c.lib.nsx_bounding_box(c.h, C.byref(img), int(bgval), None, 0, C.byref(n), _torch_stream(labels, stream)))
+    boxes = np.empty(5 * max(1, n.value), np.int32)
+    if n.value:
+        c.check(c.lib.nsx_bounding_box(c.h, C.byref(img), int(bgval), boxes.ctypes.data_as(_lib._I32P), n.value,
+                                       C.byref(n), _torch_stream(labels, stream)))
+    return int(n.value), boxes[:5 * n.value].reshape(5, n.value)
 
 
 def imhmin(image, h, connectivity: int = 8, *, out=None, ctx=None, stream=None):

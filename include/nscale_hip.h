@@ -143,6 +143,18 @@ nsx_status nsx_bwselect(nsx_ctx* ctx, const nsx_img* binary_image, const nsx_img
  * reference. */
 nsx_status nsx_imhmin(nsx_ctx* ctx, const nsx_img* image, double h, nsx_img* out, int conn, void* stream);
 
+/* ConnComponents::boundingBox (ConnComponents.cpp:241-284): for the labels of
+ * an int32 image other than bgval, in order of first appearance in raster
+ * order, the bounding box.  boxes (host memory, room for 5 x capacity ints)
+ * receives the reference's layout: count labels, then their min x, max x, min
+ * y and max y.  A label -1 or 0 that is not bgval ends the scan, as in the
+ * reference.  Labels must lie in [-1, rows*cols] (the reference's hash map
+ * takes any int; every label bwlabel2 writes is in that range).  boxes NULL:
+ * *count only.  *count is set also when capacity is too small (error).
+ * Synchronises. */
+nsx_status nsx_bounding_box(nsx_ctx* ctx, const nsx_img* labels, int bgval, int32_t* boxes, int capacity,
+                            int32_t* count, void* stream);
+
 /* ---- connected components ------------------------------------------------- */
 /* Union-find labelling of a u8 image: 4/8-neighbours of equal nonzero value are
  * joined.  relabel=0: label = raster index of the component's first pixel,

@@ -25,6 +25,8 @@
 //                           (scipy ndimage.label), with the image's values.
 //   or_imhmin_*             invert(brute-force reconstruction of
 //                           (invert(image) - h) under invert(image), 0 frame).
+//   or_bounding_box         the reference's own ConnComponents::boundingBox
+//                           compiled into oracle/_ref (exact).
 //   float invert's zero     parity unpinned: -(+0.0) is taken as +0.0 (see
 //                           invert_one(float)); values are equal either way.
 //   or_bwlabel / relabel /  the reference's own ConnComponents.cpp compiled
@@ -515,6 +517,36 @@ int or_bwselect_u8(const uint8_t* in, const uint8_t* seeds, uint8_t* out, int ro
     bwselect<uint8_t>(in, seeds, out, rows, cols, conn);
     return 0;
 }
+// ConnComponents::boundingBox (ConnComponents.cpp:241-284): for each label !=
+// bgval, in order of first appearance in raster order, its min x, max x, min y,
+// max y; a label of -1 or 0 that is not bgval ends the scan (:251-252).
+// out: labels, min x, max x, min y, max y (count each), at most capacity ints.
+int or_bounding_box(const int32_t* label, int w, int h, int bgval, int32_t* out, int capacity) {
+    const long length = (long)w * h;
+    std::unordered_map<int32_t, int> index;
+    std::vector<int32_t> lab, mnx, mxx, mny, mxy;
+    for (long i = 0; i < length; ++i) {
+        const int32_t v = label[i];
+        if (v == bgval) continue;
+        if (v == -1 || v == 0) break;
+        const int x = (int)(i % w), y = (int)(i / w);
+        auto it = index.find(v);
+        if (it == index.end()) {
+            index[v] = (int)lab.size();
+            lab.push_back(v); mnx.push_back(x); mxx.push_back(x); mny.push_back(y); mxy.push_back(y);
+        } else {
+            const int j = it->second;
+            mnx[j] = std::min(mnx[j], x); mxx[j] = std::max(mxx[j], x);
+            mny[j] = std::min(mny[j], y); mxy[j] = std::max(mxy[j], y);
+        }
+    }
+    const int n = (int)lab.size();
+    const std::vector<int32_t>* cols[5] = {&lab, &mnx, &mxx, &mny, &mxy};
+    for (int c = 0; c < 5; ++c)
+        for (int j = 0; j < n && c * n + j < capacity; ++j) out[c * n + j] = (*cols[c])[j];
+    return n;
+}
+
 int or_imhmin_u8(const uint8_t* in, int h, uint8_t* out, int rows, int cols, int conn) {
     imhmin<uint8_t>(in, (uint8_t)h, out, rows, cols, conn);
     return 0;

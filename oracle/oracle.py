@@ -104,6 +104,24 @@ def imhmin(img, h, conn=8):
     return out
 
 
+def bounding_box(labels, bgval=0):
+    """ConnComponents::boundingBox: (n, [labels, minx, maxx, miny, maxy] as a 5 x n array)."""
+    a = _c(labels, np.int32)
+    cap = 5 * a.size + 5
+    out = np.empty(cap, np.int32)
+    n = lib().or_bounding_box(_p(a), a.shape[1], a.shape[0], int(bgval), _p(out), cap)
+    return int(n), out[:5 * n].reshape(5, n).copy()
+
+
+def ref_bounding_box(labels, bgval=0):
+    r = ref()
+    a = _c(labels, np.int32)
+    cap = 5 * a.size + 5
+    out = np.empty(cap, np.int32)
+    n = r.ref_bounding_box(_p(a), a.shape[1], a.shape[0], int(bgval), _p(out), cap)
+    return int(n), out[:5 * n].reshape(5, n).copy()
+
+
 def bwlabel2(binimg, conn=8, relab=True):
     b = _c(binimg, np.uint8)
     out = np.empty(b.shape, np.int32)

@@ -5,6 +5,8 @@
 // to pin the oracle's union-find CCL, relabel and area filter exactly.
 #include "ConnComponents.h"
 
+#include <cstdlib>
+
 extern "C" {
 
 void ref_label(unsigned char* img, int w, int h, int* label, int bgval, int conn) {
@@ -25,6 +27,18 @@ int ref_area_threshold_labeled(const int* label, int w, int h, int* n_label, int
 int ref_area_threshold(unsigned char* img, int w, int h, int* label, int bgval, int lower, int upper, int conn) {
     nscale::ConnComponents cc;
     return cc.areaThreshold(img, w, h, label, bgval, lower, upper, conn);
+}
+
+// boundingBox returns a malloc'ed 5 x count array: copied into out (at most
+// `capacity` ints) and freed.  Returns count.
+int ref_bounding_box(const int* label, int w, int h, int bgval, int* out, int capacity) {
+    nscale::ConnComponents cc;
+    int n = 0;
+    int* b = cc.boundingBox(w, h, label, bgval, n);
+    const int m = 5 * n < capacity ? 5 * n : capacity;
+    for (int i = 0; i < m; ++i) out[i] = b[i];
+    std::free(b);
+    return n;
 }
 
 }

@@ -356,3 +356,32 @@ def test_imhmin(nh, orc, gpu, conn):
         nh.imhmin(np.zeros((3, 3), np.uint8), 1.5)
     with pytest.raises(nh.NsxError):
         nh.imhmin(np.zeros((3, 3), np.int32), 1)
+
+
+def test_bounding_box(nh, orc, gpu):
+    """boundingBox against the oracle (pinned to the reference's own
+    ConnComponents.cpp): canonical labels (background 0), root-index labels
+    (background -1), host and device, a stray 0 label that ends the scan, and
+    labels outside [-1, rows*cols] refused."""
+    import torch
+    rng = np.random.default_rng(101)
+    for shape in [(1, 1), (33, 65), (257, 300), (700, 530)]:
+        img = np.where(rng.random(shape) < 0.45, 255, 0).astype(np.uint8)
+        canon, _ = orc.bwlabel2(img, 8, True)
+        roots, _ = orc.bwlabel2(img, 8, False)
+        for labels, bg in ((canon, 0), (roots, -1)):
+            wn, wb = orc.bounding_box(labels, bg)
+            n, b = nh.bounding_box(labels, bg)
+            assert n == wn and np.array_equal(b, wb), (shape, bg)
+            n, b = nh.bounding_box(torch.from_numpy(labels).cuda(), bg)
+            assert n == wn and np.array_equal(b, wb)
+        stray = roots.copy()
+        stray[shape[0] // 2, shape[1] // 2] = 0 if roots[shape[0] // 2, shape[1] // 2] != 0 else 5
+        wn, wb = orc.bounding_box(stray, -1)
+        n, b = nh.bounding_box(stray, -1)
+        assert n == wn and np.array_equal(b, wb)
+    bad = np.full((4, 4), 7, np.int32)
+    bad[1, 1] = 100000
+    with pytest.raises(nh.NsxError):
+        nh.bounding_box(bad, 0)
+    assert nh.bounding_box(np.zeros((0, 3), np.int32))[0] == 0

@@ -306,3 +306,24 @@ def test_imhmin_closed_form(orc, conn):
     other = np.float32(0) - brute.recon_fixpoint((comp - np.float32(2)).astype(np.float32), comp, conn,
                                                   frame=np.float32(1000))
     assert not np.array_equal(orc.imhmin(f, 2.0, conn), other)
+
+
+def test_bounding_box_vs_reference(orc):
+    """boundingBox against the reference's own ConnComponents.cpp: canonical
+    labels (background 0), root-index labels (background -1), and a stray 0
+    label that ends the reference's scan."""
+    if orc.ref() is None:
+        pytest.skip("reference ConnComponents not built here")
+    rng = np.random.default_rng(97)
+    for shape in [(1, 1), (7, 9), (40, 57)]:
+        img = np.where(rng.random(shape) < 0.5, 255, 0).astype(np.uint8)
+        canon, _ = orc.bwlabel2(img, 8, True)
+        roots, _ = orc.bwlabel2(img, 8, False)
+        for labels, bg in ((canon, 0), (roots, -1)):
+            assert [np.array_equal(a, b) if isinstance(a, np.ndarray) else a == b
+                    for a, b in zip(orc.bounding_box(labels, bg), orc.ref_bounding_box(labels, bg))] == [True, True]
+        stray = canon.copy()
+        stray[shape[0] // 2, shape[1] // 2] = 0
+        stray[stray == 0] = np.where(img[stray == 0] > 0, 0, -1)
+        got, want = orc.bounding_box(stray, -1), orc.ref_bounding_box(stray, -1)
+        assert got[0] == want[0] and np.array_equal(got[1], want[1])
