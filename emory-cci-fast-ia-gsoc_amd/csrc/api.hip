@@ -412,6 +412,33 @@ nsx_status nsx_imhmin(nsx_ctx* ctx, const nsx_img* image, double h, nsx_img* out
     return finish_recon(ctx, vo, out, s);
 }
 
+static nsx_status local_extrema(nsx_ctx* ctx, const nsx_img* image, nsx_img* out, int conn, bool minima,
+                                void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(check_img(ctx, image, "image"));
+    NSX_TRY(check_img(ctx, out, "out"));
+    if (image->dtype != NSX_U8 && image->dtype != NSX_F32)
+        return ctx->fail(NSX_ERR_UNSUPPORTED, "localMaxima/localMinima are instantiated for uchar and float only");
+    if (out->dtype != NSX_U8) return ctx->fail(NSX_ERR_INVALID, "the extrema image is u8");
+    if (!same_shape(image, out)) return ctx->fail(NSX_ERR_INVALID, "image and out must have the same size");
+    if (overlaps(out, image)) return ctx->fail(NSX_ERR_INVALID, "out aliases image");
+    if (is_empty(image)) return NSX_OK;
+    hipStream_t s = pick_stream(ctx, stream);
+    View vi, vo;
+    NSX_TRY(to_device(ctx, image, WS_STAGE_B, true, s, &vi));
+    NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
+    NSX_TRY(local_extrema_run(ctx, vi, vo, conn == 8 ? 8 : 4, minima, s));
+    return from_device(ctx, vo, out, s);
+}
+
+nsx_status nsx_local_maxima(nsx_ctx* ctx, const nsx_img* image, nsx_img* out, int conn, void* stream) {
+    return local_extrema(ctx, image, out, conn, false, stream);
+}
+
+nsx_status nsx_local_minima(nsx_ctx* ctx, const nsx_img* image, nsx_img* out, int conn, void* stream) {
+    return local_extrema(ctx, image, out, conn, true, stream);
+}
+
 nsx_status nsx_bounding_box(nsx_ctx* ctx, const nsx_img* labels, int bgval, int32_t* boxes, int capacity,
                             int32_t* count, void* stream) {
     if (!ctx || !count) return NSX_ERR_INVALID;

@@ -1194,6 +1194,111 @@ __global__ void seeded_marker_kernel(const char* img, int64_t pi, const char* se
     ((uint8_t*)(marker + (int64_t)y * pm))[x] = ((const uint8_t*)(seeds + (int64_t)y * ps))[x] != 0 ? v : 0;
 }
 
+// ---- local maxima / minima (localMaxima, localMinima) ---------------------------
+__device__ __forceinline__ float lx_get(const char* p, int64_t pitch, int y, int x, int dtype) {
+    const char* row = p + (int64_t)y * pitch;
+    return dtype == NSX_U8 ? (float)((const uint8_t*)row)[x] : ((const float*)row)[x];
+}
+__device__ __forceinline__ void lx_put(char* p, int64_t pitch, int y, int x, int dtype, float v) {
+    char* row = p + (int64_t)y * pitch;
+    if (dtype == NSX_U8) ((uint8_t*)row)[x] = (uint8_t)v;
+    else ((float*)row)[x] = v;
+}
+
+// work = image (minima: its complement, u8 255 - x, f32 0 - x); marker = work - 1
+// (u8 saturating).  u8 values are exact in float.
+__global__ void lx_prep_kernel(const char* img, int64_t pi, char* work, int64_t pw, char* marker, int64_t pm,
+                               int rows, int cols, int dtype, int minima) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= cols) return;
+    float v = lx_get(img, pi, y, x, dtype);
+    if (minima) v = dtype == NSX_U8 ? 255.0f - v : 0.0f - v;
+    lx_put(work, pw, y, x, dtype, v);
+    lx_put(marker, pm, y, x, dtype, dtype == NSX_U8 ? fmaxf(v - 1.0f, 0.0f) : v - 1.0f);
+}
+
+// flags: 0 not a candidate (the reconstruction rose above the marker), 1 a
+// candidate, 2 a candidate with a strictly higher neighbour (out-of-image
+// neighbours are never higher).
+__global__ void lx_mark_kernel(const char* work, int64_t pw, const char* marker, int64_t pm, const char* rec,
+                               int64_t pr, uint8_t* flags, int rows, int cols, int dtype, int conn) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= cols) return;
+    uint8_t f = 0;
+    if (!(lx_get(marker, pm, y, x, dtype) < lx_get(rec, pr, y, x, dtype))) {
+        f = 1;
+        const float v = lx_get(work, pw, y, x, dtype);
+        for (int k = 0; k < (conn == 8 ? 8 : 4); ++k) {
+            const int dy = k < 4 ? (k == 0 ? -1 : k == 1 ? 1 : 0) : (k < 6 ? -1 : 1);
+            const int dx = k < 4 ? (k == 2 ? -1 : k == 3 ? 1 : 0) : ((k & 1) ? 1 : -1);
+            const int qy = y + dy, qx = x + dx;
+            if (qy >= 0 && qy < rows && qx >= 0 && qx < cols && v < lx_get(work, pw, qy, qx, dtype)) f = 2;
+        }
+    }
+    flags[(int64_t)y * cols + x] = f;
+}
+
+// A candidate next to a marked candidate of the same value is marked (the
+// reference's flood of equal-valued candidates).  Marks only grow, so the
+// in-place updates within a launch are safe; *changed = 1 when one was made.
+__global__ void lx_spread_kernel(const char* work, int64_t pw, uint8_t* flags, int rows, int cols, int dtype,
+                                 int conn, int32_t* changed) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= cols || flags[(int64_t)y * cols + x] != 1) return;
+    const float v = lx_get(work, pw, y, x, dtype);
+    for (int k = 0; k < (conn == 8 ? 8 : 4); ++k) {
+        const int dy = k < 4 ? (k == 0 ? -1 : k == 1 ? 1 : 0) : (k < 6 ? -1 : 1);
+        const int dx = k < 4 ? (k == 2 ? -1 : k == 3 ? 1 : 0) : ((k & 1) ? 1 : -1);
+        const int qy = y + dy, qx = x + dx;
+        if (qy >= 0 && qy < rows && qx >= 0 && qx < cols && flags[(int64_t)qy * cols + qx] == 2 &&
+            lx_get(work, pw, qy, qx, dtype) == v) {
+            flags[(int64_t)y * cols + x] = 2;
+            *changed = 1;
+            return;
+        }
+    }
+}
+
+// The same rule run along whole lines, forward then back: one thread per row
+// (cols_dir 0) or per column (cols_dir 1), so a mark crosses a plateau's
+// straight runs in one launch.
+__global__ void lx_sweep_kernel(const char* work, int64_t pw, uint8_t* flags, int rows, int cols, int dtype,
+                                int cols_dir, int32_t* changed) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nlines = cols_dir ? cols : rows, len = cols_dir ? rows : cols;
+    if (t >= nlines) return;
+    bool any = false;
+    for (int pass = 0; pass < 2; ++pass) {
+        bool carry = false;
+        float prev = 0.0f;
+        for (int i = 0; i < len; ++i) {
+            const int j = pass ? len - 1 - i : i;
+            const int y = cols_dir ? j : t, x = cols_dir ? t : j;
+            uint8_t* f = flags + (int64_t)y * cols + x;
+            const float v = lx_get(work, pw, y, x, dtype);
+            uint8_t fv = *f;
+            if (fv == 1 && carry && v == prev) {
+                fv = 2;
+                *f = 2;
+                any = true;
+            }
+            carry = fv == 2;
+            prev = v;
+        }
+    }
+    if (any) *changed = 1;
+}
+
+__global__ void lx_out_kernel(const uint8_t* flags, char* out, int64_t po, int rows, int cols) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= cols) return;
+    ((uint8_t*)(out + (int64_t)y * po))[x] = flags[(int64_t)y * cols + x] == 1 ? 255 : 0;
+}
+
+__global__ void lx_zero_kernel(int32_t* w) {
+    if (threadIdx.x == 0) *w = 0;
+}
+
 // ---- bounding boxes (ConnComponents::boundingBox) ------------------------------
 // misc: [0] raster index of the first label -1 or 0 that is not the background
 // (the reference stops there), [1] first raster index of a label outside
@@ -1458,6 +1563,55 @@ nsx_status bbox_run(nsx_ctx* ctx, const View& lab, int bgval, int32_t* boxes, in
     NSX_HIP_TRY(ctx, hipGetLastError());
     NSX_HIP_TRY(ctx, hipMemcpyAsync(boxes, out, sizeof(int32_t) * 5 * (size_t)n, hipMemcpyDeviceToHost, s));
     NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+    return NSX_OK;
+}
+
+nsx_status local_extrema_run(nsx_ctx* ctx, const View& img, const View& out, int conn, bool minima, hipStream_t s) {
+    if (img.rows == 0 || img.cols == 0) return NSX_OK;
+    const size_t es = img.dtype == NSX_U8 ? 1 : 4;
+    const size_t plane = (size_t)img.rows * img.cols * es;
+    void *work_p, *marker_p, *rec_p;
+    NSX_TRY(ctx->ws(WS_CHAIN_FILL, plane, &work_p));
+    NSX_TRY(ctx->ws(WS_CHAIN_RECON, plane, &marker_p));
+    NSX_TRY(ctx->ws(WS_STAGE_A, plane, &rec_p));
+    const int64_t pitch = (int64_t)img.cols * (int64_t)es;
+    const View work{(char*)work_p, pitch, img.rows, img.cols, img.dtype};
+    const View marker{(char*)marker_p, pitch, img.rows, img.cols, img.dtype};
+    const View rec{(char*)rec_p, pitch, img.rows, img.cols, img.dtype};
+    const dim3 grid((img.cols + 255) / 256, img.rows);
+    hipLaunchKernelGGL(lx_prep_kernel, grid, dim3(256), 0, s, img.data, img.pitch, work.data, work.pitch, marker.data,
+                       marker.pitch, img.rows, img.cols, img.dtype, minima ? 1 : 0);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    NSX_TRY(recon_run(ctx, RM_RECON, &marker, work, rec, conn, s));
+    // after the reconstruction, whose level-set path may use the labelling slots
+    void *flags_p, *misc_p;
+    NSX_TRY(ctx->ws(WS_CCL_AREA, (size_t)img.rows * img.cols, &flags_p));
+    NSX_TRY(ctx->ws(WS_CCL_MISC, 64, &misc_p));
+    uint8_t* flags = (uint8_t*)flags_p;
+    int32_t* changed = (int32_t*)misc_p;
+    hipLaunchKernelGGL(lx_mark_kernel, grid, dim3(256), 0, s, work.data, work.pitch, marker.data, marker.pitch,
+                       rec.data, rec.pitch, flags, img.rows, img.cols, img.dtype, conn);
+    // spread the marks over equal-valued candidates: line sweeps both ways and
+    // neighbour steps (diagonals) per check; each checked batch with a change
+    // marks at least one more pixel, so the loop ends
+    const int64_t npix = (int64_t)img.rows * img.cols;
+    for (int64_t batch = 0; batch <= npix; ++batch) {
+        hipLaunchKernelGGL(lx_zero_kernel, dim3(1), dim3(64), 0, s, changed);
+        for (int k = 0; k < 2; ++k) {
+            hipLaunchKernelGGL(lx_sweep_kernel, dim3((img.rows + 63) / 64), dim3(64), 0, s, work.data, work.pitch,
+                               flags, img.rows, img.cols, img.dtype, 0, changed);
+            hipLaunchKernelGGL(lx_sweep_kernel, dim3((img.cols + 63) / 64), dim3(64), 0, s, work.data, work.pitch,
+                               flags, img.rows, img.cols, img.dtype, 1, changed);
+            hipLaunchKernelGGL(lx_spread_kernel, grid, dim3(256), 0, s, work.data, work.pitch, flags, img.rows,
+                               img.cols, img.dtype, conn, changed);
+        }
+        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, changed, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+        if (ctx->h_pinned[40] == 0) break;
+    }
+    hipLaunchKernelGGL(lx_out_kernel, grid, dim3(256), 0, s, (const uint8_t*)flags, out.data, out.pitch, img.rows,
+                       img.cols);
+    NSX_HIP_TRY(ctx, hipGetLastError());
     return NSX_OK;
 }
 

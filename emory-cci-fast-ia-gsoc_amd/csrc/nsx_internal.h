@@ -295,6 +295,9 @@ nsx_status hmin_prep_run(nsx_ctx* ctx, const View& img, double h, const View& ma
 // boxes == nullptr: count only.  Synchronises.
 nsx_status bbox_run(nsx_ctx* ctx, const View& lab, int bgval, int32_t* boxes, int capacity, int32_t* count,
                     hipStream_t s);
+// localMaxima / localMinima of a u8 or f32 image into u8 out (255 at the
+// extrema).  Synchronises.
+nsx_status local_extrema_run(nsx_ctx* ctx, const View& img, const View& out, int conn, bool minima, hipStream_t s);
 // out = invert(out) in place (u8 or f32).
 nsx_status invert_run(nsx_ctx* ctx, const View& out, hipStream_t s);
 // out = img | out, or img & out (and_op), u8.

@@ -28,7 +28,7 @@ from ._lib import NsxImg
 
 __all__ = [
     "NsxError", "Context", "default_context",
-    "imreconstruct", "imreconstructBinary", "imfillHoles", "fillHoles", "imfill", "bwselect", "imhmin", "bounding_box",
+    "imreconstruct", "imreconstructBinary", "imfillHoles", "fillHoles", "imfill", "bwselect", "imhmin", "bounding_box", "localMaxima", "localMinima",
     "bwlabel2", "bwlabel", "bwareaopen2", "relabel", "threshold_gt", "segment_chain",
     "gen_disks_u8", "gen_marker_u8", "gen_maze_u8", "scale_i32", "scale_f32",
 ]
@@ -248,6 +248,24 @@ def imfill(image, seeds, binary: bool = False, connectivity: int = 4, *, out=Non
     out = _like(image, np.uint8) if out is None else out
     c.check(c.lib.nsx_imfill(c.h, C.byref(describe(image)), C.byref(describe(seeds)), C.byref(describe(out)),
                              int(bool(binary)), connectivity, _torch_stream(image, stream)))
+    return out
+
+
+def localMaxima(image, connectivity: int = 8, *, out=None, ctx=None, stream=None):
+    """nscale::localMaxima<T> (uchar or float): u8, 255 at the local maxima."""
+    c = _ctx(image, ctx)
+    out = _like(image, np.uint8) if out is None else out
+    c.check(c.lib.nsx_local_maxima(c.h, C.byref(describe(image)), C.byref(describe(out)), connectivity,
+                                   _torch_stream(image, stream)))
+    return out
+
+
+def localMinima(image, connectivity: int = 8, *, out=None, ctx=None, stream=None):
+    """nscale::localMinima<T> (uchar or float): u8, 255 at the local minima."""
+    c = _ctx(image, ctx)
+    out = _like(image, np.uint8) if out is None else out
+    c.check(c.lib.nsx_local_minima(c.h, C.byref(describe(image)), C.byref(describe(out)), connectivity,
+                                   _torch_stream(image, stream)))
     return out
 
 

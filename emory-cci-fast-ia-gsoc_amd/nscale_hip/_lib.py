@@ -56,6 +56,8 @@ SIGNATURES = {
     "nsx_bwselect": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_imhmin": (C.c_int, [_CTX, _IMG, C.c_double, _IMG, C.c_int, C.c_void_p]),
     "nsx_bounding_box": (C.c_int, [_CTX, _IMG, C.c_int, _I32P, C.c_int, _I32P, C.c_void_p]),
+    "nsx_local_maxima": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_void_p]),
+    "nsx_local_minima": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_bwlabel": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_int, _I32P, C.c_void_p]),
     "nsx_bwareaopen": (C.c_int, [_CTX, _IMG, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _IMG, _I32P,
                                  C.c_void_p]),

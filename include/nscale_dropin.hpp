@@ -10,6 +10,8 @@
 //     imfillHoles<T>(image, binary, connectivity)              :50-51
 //     bwselect<T>(binaryImage, seeds, connectivity)            :56
 //     imhmin<T>(image, h, connectivity)                        :78
+//     localMaxima<T>(image, connectivity)                      :84
+//     localMinima<T>(image, connectivity)                      :86
 //     bwlabel2(binaryImage, connectivity, relab)               :59
 //     bwareaopen2(binaryImage, labeled, flatten, minSize, maxSize, connectivity, count)  :74
 // At a call site, include this header instead of MorphologicOperations.h for
@@ -24,7 +26,8 @@
 // OpenCV's <opencv2/core/core.hpp> is included unless the including file
 // defines NSCALE_DROPIN_CV_PROVIDED after providing a cv::Mat with the members
 // used here: data, step (convertible to size_t), rows, cols, type(),
-// channels(), and the constructors Mat(rows, cols, type) and Mat_<int>(rows, cols).
+// channels(), and the constructors Mat(rows, cols, type), Mat_<int>(rows, cols)
+// and Mat_<unsigned char>(rows, cols).
 #ifndef NSCALE_DROPIN_HPP_
 #define NSCALE_DROPIN_HPP_
 
@@ -152,6 +155,28 @@ cv::Mat imhmin(const cv::Mat& image, T h, int connectivity) {
     nsx_img i = host(image, Type<T>::nsx), o = host(out, Type<T>::nsx);
     nsx_ctx* c = ctx();
     check(nsx_imhmin(c, &i, (double)h, &o, connectivity, nullptr), c);
+    return out;
+}
+
+template <typename T>
+cv::Mat_<unsigned char> localMaxima(const cv::Mat& image, int connectivity) {
+    using namespace nsx_detail;
+    require<T>(image, "image");
+    cv::Mat_<unsigned char> out(image.rows, image.cols);
+    nsx_img i = host(image, Type<T>::nsx), o = host(out, NSX_U8);
+    nsx_ctx* c = ctx();
+    check(nsx_local_maxima(c, &i, &o, connectivity, nullptr), c);
+    return out;
+}
+
+template <typename T>
+cv::Mat_<unsigned char> localMinima(const cv::Mat& image, int connectivity) {
+    using namespace nsx_detail;
+    require<T>(image, "image");
+    cv::Mat_<unsigned char> out(image.rows, image.cols);
+    nsx_img i = host(image, Type<T>::nsx), o = host(out, NSX_U8);
+    nsx_ctx* c = ctx();
+    check(nsx_local_minima(c, &i, &o, connectivity, nullptr), c);
     return out;
 }
 

@@ -143,6 +143,15 @@ nsx_status nsx_bwselect(nsx_ctx* ctx, const nsx_img* binary_image, const nsx_img
  * reference. */
 nsx_status nsx_imhmin(nsx_ctx* ctx, const nsx_img* image, double h, nsx_img* out, int conn, void* stream);
 
+/* localMaxima / localMinima (MorphologicOperations.cpp:2374-2456, uchar and
+ * float, :2529-2532): u8 out, 255 at the local maxima (minima: of the
+ * complement).  Candidates are the pixels where imreconstruct(image - 1,
+ * image) stays at image - 1 (uchar saturating); a connected region of equal-
+ * valued candidates that touches a strictly higher neighbour is dropped (the
+ * reference's cv::floodFill, fixed range, mask only). */
+nsx_status nsx_local_maxima(nsx_ctx* ctx, const nsx_img* image, nsx_img* out, int conn, void* stream);
+nsx_status nsx_local_minima(nsx_ctx* ctx, const nsx_img* image, nsx_img* out, int conn, void* stream);
+
 /* ConnComponents::boundingBox (ConnComponents.cpp:241-284): for the labels of
  * an int32 image other than bgval, in order of first appearance in raster
  * order, the bounding box.  boxes (host memory, room for 5 x capacity ints)

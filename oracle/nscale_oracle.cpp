@@ -25,6 +25,11 @@
 //                           (scipy ndimage.label), with the image's values.
 //   or_imhmin_*             invert(brute-force reconstruction of
 //                           (invert(image) - h) under invert(image), 0 frame).
+//   or_local_maxima/minima  candidates from the brute-force reconstruction,
+//                           minus the equal-valued candidate plateaus that touch
+//                           a higher neighbour (scipy); the flood follows
+//                           OpenCV's documented floodFill (fixed range, mask
+//                           only, mask value 1): parity pinned to that document.
 //   or_bounding_box         the reference's own ConnComponents::boundingBox
 //                           compiled into oracle/_ref (exact).
 //   float invert's zero     parity unpinned: -(+0.0) is taken as +0.0 (see
@@ -381,6 +386,60 @@ void imhmin(const T* image, T h, T* out, int rows, int cols, int connectivity) {
     for (size_t i = 0; i < n; ++i) out[i] = invert_one(out[i]);
 }
 
+// localMaxima<T> (MorphologicOperations.cpp:2374-2446; uchar and float,
+// :2529-2532).  marker = image - 1 (uchar saturating) (:2379); candidates =
+// !(marker < imreconstruct(marker, image)) (:2380-2381: 0 marks a candidate);
+// padded with 255 (mask) and type min (image) (:2392-2395); every candidate
+// with a strictly higher neighbour (4, then the diagonals for 8) floods its
+// equal-valued region of candidates in the mask (:2425-2440) -- cv::floodFill
+// with FLOODFILL_FIXED_RANGE | FLOODFILL_MASK_ONLY, zero differences: pixels
+// equal to the seed, connected through zero mask pixels, get mask value 1;
+// result = 255 where the mask stayed 0 (:2446).
+template <typename T>
+void localMaxima(const T* image, uint8_t* out, int rows, int cols, int connectivity) {
+    const size_t n = (size_t)rows * cols;
+    std::vector<T> marker(n), rec(n);
+    for (size_t i = 0; i < n; ++i) marker[i] = sub_h(image[i], T(1));
+    imreconstruct<T>(marker.data(), image, rec.data(), rows, cols, connectivity);
+    const int R = rows + 2, C = cols + 2;
+    std::vector<uint8_t> mask((size_t)R * C, 255);
+    std::vector<T> in((size_t)R * C, type_min<T>());
+    for (int y = 0; y < rows; ++y)
+        for (int x = 0; x < cols; ++x) {
+            const size_t i = (size_t)y * cols + x;
+            mask[(size_t)(y + 1) * C + x + 1] = marker[i] < rec[i] ? 255 : 0;
+            in[(size_t)(y + 1) * C + x + 1] = image[i];
+        }
+    static const int dy[8] = {-1, 1, 0, 0, -1, -1, 1, 1}, dx[8] = {0, 0, -1, 1, -1, 1, -1, 1};
+    const int nn = connectivity == 8 ? 8 : 4;
+    std::vector<int> stack;
+    for (int y = 1; y < R - 1; ++y)
+        for (int x = 1; x < C - 1; ++x) {
+            if (mask[(size_t)y * C + x] > 0) continue;
+            const T v = in[(size_t)y * C + x];
+            bool higher = false;
+            for (int k = 0; k < nn && !higher; ++k) higher = v < in[(size_t)(y + dy[k]) * C + x + dx[k]];
+            if (!higher) continue;
+            // floodFill from (x, y): equal value, through zero mask pixels
+            stack.assign(1, y * C + x);
+            mask[(size_t)y * C + x] = 1;
+            while (!stack.empty()) {
+                const int p = stack.back();
+                stack.pop_back();
+                const int py = p / C, px = p % C;
+                for (int k = 0; k < nn; ++k) {
+                    const int q = (py + dy[k]) * C + px + dx[k];
+                    if (mask[q] == 0 && in[q] == v) {
+                        mask[q] = 1;
+                        stack.push_back(q);
+                    }
+                }
+            }
+        }
+    for (int y = 0; y < rows; ++y)
+        for (int x = 0; x < cols; ++x) out[(size_t)y * cols + x] = mask[(size_t)(y + 1) * C + x + 1] == 0 ? 255 : 0;
+}
+
 // ---- ConnComponents (src/segment/ConnComponents.cpp) ----------------------
 // find/flatten are written iteratively: the recursive originals (:24-32,
 // :137-147) compress every node on the path to the root, which the two-pass
@@ -545,6 +604,28 @@ int or_bounding_box(const int32_t* label, int w, int h, int bgval, int32_t* out,
     for (int c = 0; c < 5; ++c)
         for (int j = 0; j < n && c * n + j < capacity; ++j) out[c * n + j] = (*cols[c])[j];
     return n;
+}
+
+int or_local_maxima_u8(const uint8_t* in, uint8_t* out, int rows, int cols, int conn) {
+    localMaxima<uint8_t>(in, out, rows, cols, conn);
+    return 0;
+}
+int or_local_maxima_f32(const float* in, uint8_t* out, int rows, int cols, int conn) {
+    localMaxima<float>(in, out, rows, cols, conn);
+    return 0;
+}
+// localMinima<T> (:2449-2456): localMaxima of invert(image).
+int or_local_minima_u8(const uint8_t* in, uint8_t* out, int rows, int cols, int conn) {
+    std::vector<uint8_t> c(in, in + (size_t)rows * cols);
+    invert_vec(c);
+    localMaxima<uint8_t>(c.data(), out, rows, cols, conn);
+    return 0;
+}
+int or_local_minima_f32(const float* in, uint8_t* out, int rows, int cols, int conn) {
+    std::vector<float> c(in, in + (size_t)rows * cols);
+    invert_vec(c);
+    localMaxima<float>(c.data(), out, rows, cols, conn);
+    return 0;
 }
 
 int or_imhmin_u8(const uint8_t* in, int h, uint8_t* out, int rows, int cols, int conn) {

@@ -104,6 +104,19 @@ def imhmin(img, h, conn=8):
     return out
 
 
+def local_extrema(img, conn=8, minima=False):
+    """localMaxima / localMinima<T> (uchar or float): u8 255 at the extrema."""
+    img = np.ascontiguousarray(img)
+    out = np.empty(img.shape, np.uint8)
+    kind = "minima" if minima else "maxima"
+    if img.dtype == np.uint8:
+        getattr(lib(), f"or_local_{kind}_u8")(_p(img), _p(out), img.shape[0], img.shape[1], conn)
+    else:
+        img = _c(img, np.float32)
+        getattr(lib(), f"or_local_{kind}_f32")(_p(img), _p(out), img.shape[0], img.shape[1], conn)
+    return out
+
+
 def bounding_box(labels, bgval=0):
     """ConnComponents::boundingBox: (n, [labels, minx, maxx, miny, maxy] as a 5 x n array)."""
     a = _c(labels, np.int32)

@@ -157,3 +157,29 @@ def imhmin_fixpoint(img: np.ndarray, h, conn: int) -> np.ndarray:
     comp = (np.float32(0) - img).astype(np.float32)
     marker = (comp - np.float32(h)).astype(np.float32)
     return (np.float32(0) - recon_fixpoint(marker, comp, conn, frame=np.float32(0))).astype(np.float32)
+
+
+def local_maxima_closed_form(img: np.ndarray, conn: int) -> np.ndarray:
+    """Candidates: the reconstruction of image - 1 (u8 saturating) under the
+    image stays at the marker.  The result drops every candidate plateau (a
+    conn-connected set of equal-valued candidates) that holds a pixel with a
+    strictly higher neighbour; pixels outside the image never count as higher."""
+    if img.dtype == np.uint8:
+        marker = np.maximum(img.astype(np.int32) - 1, 0).astype(np.uint8)
+    else:
+        marker = (img - np.float32(1)).astype(np.float32)
+    frame = np.float32(0) if img.dtype == np.float32 else 0
+    cand = ~(marker < recon_fixpoint(marker, img, conn, frame=frame))
+    lo = np.iinfo(np.uint8).min if img.dtype == np.uint8 else -np.inf
+    pad = np.pad(img.astype(np.float64), 1, constant_values=lo)
+    higher = np.zeros(img.shape, bool)
+    offs = [(-1, 0), (1, 0), (0, -1), (0, 1)] + ([(-1, -1), (-1, 1), (1, -1), (1, 1)] if conn == 8 else [])
+    H, W = img.shape
+    for dy, dx in offs:
+        higher |= pad[1 + dy:1 + dy + H, 1 + dx:1 + dx + W] > img
+    out = cand.copy()
+    for v in np.unique(img[cand]):
+        lab, _ = ndimage.label(cand & (img == v), structure=structure(conn))
+        bad = np.unique(lab[higher & (lab > 0)])
+        out &= ~np.isin(lab, bad[bad > 0])
+    return np.where(out, 255, 0).astype(np.uint8)

@@ -38,6 +38,8 @@ int main(int argc, char** argv) {
         save(dir + "/imfillbin8.bin", nscale::imfill<unsigned char>(bin, seeds, true, 8), 1);
         save(dir + "/bwselect8.bin", nscale::bwselect<unsigned char>(bin, seeds, 8), 1);
         save(dir + "/hmin8.bin", nscale::imhmin<unsigned char>(mask, (unsigned char)20, 8), 1);
+        save(dir + "/lmax8.bin", nscale::localMaxima<unsigned char>(mask, 8), 1);
+        save(dir + "/lmin4.bin", nscale::localMinima<unsigned char>(mask, 4), 1);
         save(dir + "/label8.bin", nscale::bwlabel2(bin, 8, true), 4);
         int count = 0;
         save(dir + "/area8.bin", nscale::bwareaopen2(bin, false, false, 3, 40, 8, count), 4);

@@ -44,6 +44,10 @@ template <> class Mat_<int> : public Mat {
 public:
     Mat_(int r, int c) : Mat(r, c, CV_32S) {}
 };
+template <> class Mat_<unsigned char> : public Mat {
+public:
+    Mat_(int r, int c) : Mat(r, c, CV_8U) {}
+};
 
 }  // namespace cv
 

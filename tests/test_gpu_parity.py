@@ -385,3 +385,53 @@ def test_bounding_box(nh, orc, gpu):
     with pytest.raises(nh.NsxError):
         nh.bounding_box(bad, 0)
     assert nh.bounding_box(np.zeros((0, 3), np.int32))[0] == 0
+
+
+def _plateau_images(rng, shape):
+    """Few-valued noise (many plateaus), a blocky smooth field and a constant."""
+    few = rng.integers(0, 4, size=shape).astype(np.uint8) * 60
+    by, bx = max(1, shape[0] // 7), max(1, shape[1] // 9)
+    coarse = rng.integers(0, 6, size=(shape[0] // by + 1, shape[1] // bx + 1)).astype(np.uint8) * 40
+    blocky = np.kron(coarse, np.ones((by, bx), np.uint8))[: shape[0], : shape[1]].copy()
+    return [few, blocky, np.full(shape, 17, np.uint8)]
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_local_extrema(nh, orc, gpu, conn):
+    """localMaxima / localMinima (uchar and float) against the oracle: random
+    and plateau-heavy images, constants, a serpentine plateau that needs many
+    propagation steps, host arrays and device tensors."""
+    import torch
+    rng = np.random.default_rng(131 + conn)
+    for shape in [(1, 1), (1, 40), (61, 62), (130, 257), (300, 517)]:
+        imgs = [rng.integers(0, 256, size=shape).astype(np.uint8)] + _plateau_images(rng, shape)
+        for u in imgs:
+            f = (u.astype(np.float32) - 100.0) / np.float32(8)
+            for img in (u, f):
+                for minima in (False, True):
+                    want = orc.local_extrema(img, conn, minima)
+                    fn = nh.localMinima if minima else nh.localMaxima
+                    assert np.array_equal(fn(img, conn), want), (shape, img.dtype, minima)
+            want = orc.local_extrema(u, conn, False)
+            assert np.array_equal(nh.localMaxima(torch.from_numpy(u).cuda(), conn).cpu().numpy(), want)
+    # a serpentine plateau at 1e9 (where v - 1 rounds back to v in float, so
+    # the whole snake stays a candidate) between walls at -1: with a higher
+    # pixel at its far end the whole snake is dropped, one mark step at a time
+    snake = np.full((99, 101), -1, np.float32)
+    snake[1::4, 1:-1] = 1e9
+    for k in range(24):
+        snake[4 * k + 2 : 4 * k + 5, -2 if k % 2 == 0 else 1] = 1e9
+    for end in (None, 2e9):
+        img = snake.copy()
+        if end is not None:
+            img[97, -2] = end
+        for minima in (False, True):
+            want = orc.local_extrema(img, conn, minima)
+            fn = nh.localMinima if minima else nh.localMaxima
+            assert np.array_equal(fn(img, conn), want), (end, minima)
+    with pytest.raises(nh.NsxError):
+        nh.localMaxima(np.zeros((3, 3), np.int32))
+    with pytest.raises(nh.NsxError):
+        nh.localMaxima(np.zeros((3, 3), np.uint8), out=np.zeros((3, 3), np.float32))
+    e = np.zeros((0, 7), np.uint8)
+    assert nh.localMaxima(e).shape == e.shape

@@ -327,3 +327,22 @@ def test_bounding_box_vs_reference(orc):
         stray[stray == 0] = np.where(img[stray == 0] > 0, 0, -1)
         got, want = orc.bounding_box(stray, -1), orc.ref_bounding_box(stray, -1)
         assert got[0] == want[0] and np.array_equal(got[1], want[1])
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_local_extrema_closed_form(orc, conn):
+    """localMaxima / localMinima against the plateau definition, for u8 and for
+    float images with plateaus (few distinct values) and with fine values."""
+    rng = np.random.default_rng(111 + conn)
+    for shape in [(1, 1), (8, 9), (30, 27)]:
+        u = rng.integers(0, 6, size=shape).astype(np.uint8) * 40
+        f1 = (rng.integers(0, 5, size=shape) * np.float32(0.5)).astype(np.float32)   # plateaus, steps < 1
+        f2 = (rng.integers(-300, 300, size=shape) / np.float32(37)).astype(np.float32)
+        for img in (u, f1, f2, np.zeros(shape, np.uint8)):
+            assert np.array_equal(orc.local_extrema(img, conn), brute.local_maxima_closed_form(img, conn))
+            if img.dtype == np.uint8:
+                inv = (255 - img.astype(np.int32)).astype(np.uint8)
+            else:
+                inv = (np.float32(0) - img).astype(np.float32)
+            assert np.array_equal(orc.local_extrema(img, conn, minima=True),
+                                  brute.local_maxima_closed_form(inv, conn))
