@@ -439,6 +439,24 @@ nsx_status nsx_local_minima(nsx_ctx* ctx, const nsx_img* image, nsx_img* out, in
     return local_extrema(ctx, image, out, conn, true, stream);
 }
 
+nsx_status nsx_distance_transform(nsx_ctx* ctx, const nsx_img* mask, nsx_img* out, int calc_dist, void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(check_img(ctx, mask, "mask"));
+    NSX_TRY(check_img(ctx, out, "out"));
+    if (mask->dtype != NSX_U8) return ctx->fail(NSX_ERR_INVALID, "distanceTransform takes a uchar mask");
+    if (out->dtype != (calc_dist ? NSX_F32 : NSX_I32))
+        return ctx->fail(NSX_ERR_INVALID, "out is float32 with calc_dist, int32 without");
+    if (!same_shape(mask, out)) return ctx->fail(NSX_ERR_INVALID, "mask and out must have the same size");
+    if (overlaps(out, mask)) return ctx->fail(NSX_ERR_INVALID, "out aliases mask");
+    if (is_empty(mask)) return NSX_OK;
+    hipStream_t s = pick_stream(ctx, stream);
+    View vm, vo;
+    NSX_TRY(to_device(ctx, mask, WS_STAGE_B, true, s, &vm));
+    NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
+    NSX_TRY(dist_run(ctx, vm, vo, calc_dist != 0, s));
+    return from_device(ctx, vo, out, s);
+}
+
 nsx_status nsx_bounding_box(nsx_ctx* ctx, const nsx_img* labels, int bgval, int32_t* boxes, int capacity,
                             int32_t* count, void* stream) {
     if (!ctx || !count) return NSX_ERR_INVALID;

@@ -1,0 +1,197 @@
+// dist.hip -- distanceTransform (MorphologicOperations.cpp:197-275) on gfx950.
+//
+// The reference propagates nearest-background labels through a FIFO queue:
+// every pixel starts at the point (0, 3*rows) below the image (:214), a
+// background pixel holds itself (:218), and a queued pixel offers its label to
+// its 8 neighbours, which take it when the distance to it, sqrt in double
+// rounded to float, is strictly below the distance to their own (:90-119).  It
+// ends at a fixed point of that rule: no pixel is strictly closer to a
+// neighbour's label than to its own.
+//
+// Here the same rule runs to a fixed point in 64x64 tiles.  A tile loads its
+// labels and a one-pixel halo into LDS, applies the rule in place until no
+// label in it changes, and writes the tile back; rounds of tile launches repeat
+// until a round changes nothing.  A tile runs in a round only when it or one of
+// its 8 neighbour tiles changed in the previous round.
+//
+// Labels are packed (y << 16) | x inside the kernels, with DT_FAR for the
+// starting point, so no integer division is needed per offer; the output turns
+// them into the reference's raster index (far = rows*cols*3) or its float
+// distance (:254-266).
+//
+// Where several backgrounds are equally far, the reference keeps the one its
+// queue offered first and this keeps the one its sweep offered first, so the
+// index maps may differ there; the distances agree.
+#include "nsx_internal.h"
+
+#include <cmath>
+
+namespace nsx {
+namespace {
+
+constexpr int DT = 64;
+constexpr int DH = DT + 2;
+constexpr uint32_t DT_FAR = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint64_t dt_d2(int x, int y, uint32_t lab, int far_y) {
+    const int lx = lab == DT_FAR ? 0 : (int)(lab & 0xFFFFu);
+    const int ly = lab == DT_FAR ? far_y : (int)(lab >> 16);
+    const int64_t dx = x - lx, dy = y - ly;
+    return (uint64_t)(dx * dx + dy * dy);
+}
+
+// The reference's test: float(sqrt(double(a))) < float(sqrt(double(b))).  Below
+// 2^20 distinct squares stay distinct after rounding to float (their square
+// roots differ by more than 1/2048, a float ulp there is 2^-14), so the integer
+// test decides; above, ties from rounding are possible and the roots are taken.
+__device__ __forceinline__ bool dt_closer(uint64_t a, uint64_t b) {
+    if (a >= b) return false;
+    if (b <= (1ull << 20)) return true;
+    return (float)sqrt((double)a) < (float)sqrt((double)b);
+}
+
+__global__ void dt_init_kernel(const char* mask, int64_t pm, uint32_t* lab, int rows, int cols, uint8_t* flags,
+                               int ntiles) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (y == 0)
+        for (int t = x; t < ntiles; t += gridDim.x * blockDim.x) flags[t] = 1;
+    if (x >= cols) return;
+    const uint8_t m = ((const uint8_t*)(mask + (int64_t)y * pm))[x];
+    lab[(int64_t)y * cols + x] = m == 0 ? ((uint32_t)y << 16) | (uint32_t)x : DT_FAR;
+}
+
+__global__ __launch_bounds__(256) void dt_tile_kernel(uint32_t* lab, int rows, int cols, int tiles_x, int tiles_y,
+                                                      int far_y, const uint8_t* prev, uint8_t* next,
+                                                      int32_t* changed) {
+    __shared__ uint32_t s[DH * DH];
+    __shared__ int active;
+    const int tile = blockIdx.x;
+    const int tx = tile % tiles_x, ty = tile / tiles_x;
+    if (threadIdx.x == 0) {
+        int a = 0;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int nx = tx + dx, ny = ty + dy;
+                if (nx >= 0 && nx < tiles_x && ny >= 0 && ny < tiles_y && prev[ny * tiles_x + nx]) a = 1;
+            }
+        active = a;
+    }
+    __syncthreads();
+    if (!active) {
+        if (threadIdx.x == 0) next[tile] = 0;
+        return;
+    }
+    const int x0 = tx * DT - 1, y0 = ty * DT - 1;
+    // halo outside the image reads as the starting point, which never wins
+    for (int i = threadIdx.x; i < DH * DH; i += 256) {
+        const int gx = x0 + i % DH, gy = y0 + i / DH;
+        s[i] = (gx >= 0 && gx < cols && gy >= 0 && gy < rows) ? lab[(int64_t)gy * cols + gx] : DT_FAR;
+    }
+    __syncthreads();
+    // offers in the reference's order: up-left, up-right, up, down-left,
+    // down-right, down, left, right
+    const int off[8] = {-DH - 1, -DH + 1, -DH, DH - 1, DH + 1, DH, -1, 1};
+    bool any = false;
+    for (;;) {
+        int ch = 0;
+        for (int k = 0; k < DT * DT / 256; ++k) {
+            const int p = threadIdx.x + 256 * k;
+            const int px = p & (DT - 1), py = p / DT;
+            const int gx = tx * DT + px, gy = ty * DT + py;
+            if (gx >= cols || gy >= rows) continue;
+            const int c = (py + 1) * DH + px + 1;
+            const uint32_t own = s[c];
+            uint32_t best = own;
+            uint64_t bd = dt_d2(gx, gy, own, far_y);
+            if (bd == 0) continue;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t cand = s[c + off[j]];
+                if (cand == best) continue;
+                const uint64_t d = dt_d2(gx, gy, cand, far_y);
+                if (dt_closer(d, bd)) {
+                    best = cand;
+                    bd = d;
+                }
+            }
+            if (best != own) {
+                s[c] = best;
+                ch = 1;
+            }
+        }
+        if (!__syncthreads_or(ch)) break;
+        any = true;
+    }
+    if (any) {
+        for (int i = threadIdx.x; i < DT * DT; i += 256) {
+            const int px = i & (DT - 1), py = i / DT;
+            const int gx = tx * DT + px, gy = ty * DT + py;
+            if (gx < cols && gy < rows) lab[(int64_t)gy * cols + gx] = s[(py + 1) * DH + px + 1];
+        }
+    }
+    if (threadIdx.x == 0) {
+        next[tile] = any ? 1 : 0;
+        if (any) *changed = 1;
+    }
+}
+
+__global__ void dt_zero_kernel(int32_t* w) {
+    if (threadIdx.x == 0) *w = 0;
+}
+
+__global__ void dt_out_kernel(const uint32_t* lab, char* out, int64_t po, int rows, int cols, int far_y,
+                              int calc_dist) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= cols) return;
+    const uint32_t l = lab[(int64_t)y * cols + x];
+    char* row = out + (int64_t)y * po;
+    if (calc_dist) {
+        ((float*)row)[x] = (float)sqrt((double)dt_d2(x, y, l, far_y));
+    } else {
+        ((int32_t*)row)[x] =
+            l == DT_FAR ? (int32_t)((int64_t)rows * cols * 3) : (int32_t)((l >> 16) * (uint32_t)cols + (l & 0xFFFFu));
+    }
+}
+
+}  // namespace
+
+nsx_status dist_run(nsx_ctx* ctx, const View& mask, const View& out, bool calc_dist, hipStream_t s) {
+    const int rows = mask.rows, cols = mask.cols;
+    if (rows == 0 || cols == 0) return NSX_OK;
+    if (rows >= 65535 || cols >= 65535 || (int64_t)rows * cols * 3 > INT32_MAX)
+        return ctx->fail(NSX_ERR_INVALID, "distanceTransform: rows, cols < 65535 and rows*cols*3 within int32");
+    const int64_t npix = (int64_t)rows * cols;
+    const int tiles_x = (cols + DT - 1) / DT, tiles_y = (rows + DT - 1) / DT;
+    const int ntiles = tiles_x * tiles_y;
+    void *lab_p, *flags_p, *misc_p;
+    NSX_TRY(ctx->ws(WS_CCL_LABEL, sizeof(uint32_t) * (size_t)npix, &lab_p));
+    NSX_TRY(ctx->ws(WS_CCL_BITS, 2 * (size_t)ntiles, &flags_p));
+    NSX_TRY(ctx->ws(WS_CCL_MISC, 64, &misc_p));
+    uint32_t* lab = (uint32_t*)lab_p;
+    uint8_t* fa = (uint8_t*)flags_p;
+    uint8_t* fb = fa + ntiles;
+    int32_t* changed = (int32_t*)misc_p;
+    const dim3 grid((cols + 255) / 256, rows);
+    hipLaunchKernelGGL(dt_init_kernel, grid, dim3(256), 0, s, mask.data, mask.pitch, lab, rows, cols, fa, ntiles);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    // each round that changes something moves a label strictly closer, so the
+    // rounds end; the bound only guards the loop
+    for (int64_t round = 0; round <= npix; ++round) {
+        hipLaunchKernelGGL(dt_zero_kernel, dim3(1), dim3(64), 0, s, changed);
+        hipLaunchKernelGGL(dt_tile_kernel, dim3(ntiles), dim3(256), 0, s, lab, rows, cols, tiles_x, tiles_y, 3 * rows,
+                           (const uint8_t*)fa, fb, changed);
+        NSX_HIP_TRY(ctx, hipGetLastError());
+        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, changed, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+        if (ctx->h_pinned[40] == 0) break;
+        uint8_t* t = fa;
+        fa = fb;
+        fb = t;
+    }
+    hipLaunchKernelGGL(dt_out_kernel, grid, dim3(256), 0, s, (const uint32_t*)lab, out.data, out.pitch, rows, cols,
+                       3 * rows, calc_dist ? 1 : 0);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    return NSX_OK;
+}
+
+}  // namespace nsx

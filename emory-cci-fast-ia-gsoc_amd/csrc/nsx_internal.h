@@ -300,6 +300,9 @@ nsx_status bbox_run(nsx_ctx* ctx, const View& lab, int bgval, int32_t* boxes, in
 nsx_status local_extrema_run(nsx_ctx* ctx, const View& img, const View& out, int conn, bool minima, hipStream_t s);
 // out = invert(out) in place (u8 or f32).
 nsx_status invert_run(nsx_ctx* ctx, const View& out, hipStream_t s);
+// distanceTransform of a u8 mask into out: f32 distances (calc_dist) or int32
+// nearest-background raster indices.  Synchronises.
+nsx_status dist_run(nsx_ctx* ctx, const View& mask, const View& out, bool calc_dist, hipStream_t s);
 // out = img | out, or img & out (and_op), u8.
 nsx_status bitwise_u8_run(nsx_ctx* ctx, const View& img, const View& out, bool and_op, hipStream_t s);
 

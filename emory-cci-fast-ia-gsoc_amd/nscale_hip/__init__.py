@@ -28,7 +28,7 @@ from ._lib import NsxImg
 
 __all__ = [
     "NsxError", "Context", "default_context",
-    "imreconstruct", "imreconstructBinary", "imfillHoles", "fillHoles", "imfill", "bwselect", "imhmin", "bounding_box", "localMaxima", "localMinima",
+    "imreconstruct", "imreconstructBinary", "imfillHoles", "fillHoles", "imfill", "bwselect", "imhmin", "bounding_box", "localMaxima", "localMinima", "distanceTransform",
     "bwlabel2", "bwlabel", "bwareaopen2", "relabel", "threshold_gt", "segment_chain",
     "gen_disks_u8", "gen_marker_u8", "gen_maze_u8", "scale_i32", "scale_f32",
 ]
@@ -266,6 +266,16 @@ def localMinima(image, connectivity: int = 8, *, out=None, ctx=None, stream=None
     out = _like(image, np.uint8) if out is None else out
     c.check(c.lib.nsx_local_minima(c.h, C.byref(describe(image)), C.byref(describe(out)), connectivity,
                                    _torch_stream(image, stream)))
+    return out
+
+
+def distanceTransform(mask, calcDist: bool = True, *, out=None, ctx=None, stream=None):
+    """nscale::distanceTransform (uchar mask): float32 distances to the nearest
+    background, or with calcDist=False int32 nearest-background raster indices."""
+    c = _ctx(mask, ctx)
+    out = _like(mask, np.float32 if calcDist else np.int32) if out is None else out
+    c.check(c.lib.nsx_distance_transform(c.h, C.byref(describe(mask)), C.byref(describe(out)), 1 if calcDist else 0,
+                                         _torch_stream(mask, stream)))
     return out
 
 

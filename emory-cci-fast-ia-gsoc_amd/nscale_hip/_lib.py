@@ -58,6 +58,7 @@ SIGNATURES = {
     "nsx_bounding_box": (C.c_int, [_CTX, _IMG, C.c_int, _I32P, C.c_int, _I32P, C.c_void_p]),
     "nsx_local_maxima": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_local_minima": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_void_p]),
+    "nsx_distance_transform": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_bwlabel": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_int, _I32P, C.c_void_p]),
     "nsx_bwareaopen": (C.c_int, [_CTX, _IMG, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _IMG, _I32P,
                                  C.c_void_p]),

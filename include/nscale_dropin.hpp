@@ -12,6 +12,7 @@
 //     imhmin<T>(image, h, connectivity)                        :78
 //     localMaxima<T>(image, connectivity)                      :84
 //     localMinima<T>(image, connectivity)                      :86
+//     distanceTransform(mask, calcDist)                        :92
 //     bwlabel2(binaryImage, connectivity, relab)               :59
 //     bwareaopen2(binaryImage, labeled, flatten, minSize, maxSize, connectivity, count)  :74
 // At a call site, include this header instead of MorphologicOperations.h for
@@ -177,6 +178,16 @@ cv::Mat_<unsigned char> localMinima(const cv::Mat& image, int connectivity) {
     nsx_img i = host(image, Type<T>::nsx), o = host(out, NSX_U8);
     nsx_ctx* c = ctx();
     check(nsx_local_minima(c, &i, &o, connectivity, nullptr), c);
+    return out;
+}
+
+inline cv::Mat distanceTransform(const cv::Mat& mask, bool calcDist = true) {
+    using namespace nsx_detail;
+    require<unsigned char>(mask, "mask");
+    cv::Mat out(mask.rows, mask.cols, calcDist ? CV_32F : CV_32S);
+    nsx_img m = host(mask, NSX_U8), o = host(out, calcDist ? NSX_F32 : NSX_I32);
+    nsx_ctx* c = ctx();
+    check(nsx_distance_transform(c, &m, &o, calcDist ? 1 : 0, nullptr), c);
     return out;
 }
 

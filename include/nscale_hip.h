@@ -152,6 +152,17 @@ nsx_status nsx_imhmin(nsx_ctx* ctx, const nsx_img* image, double h, nsx_img* out
 nsx_status nsx_local_maxima(nsx_ctx* ctx, const nsx_img* image, nsx_img* out, int conn, void* stream);
 nsx_status nsx_local_minima(nsx_ctx* ctx, const nsx_img* image, nsx_img* out, int conn, void* stream);
 
+/* distanceTransform (MorphologicOperations.cpp:197-275): u8 mask, zero pixels
+ * are background.  calc_dist != 0: out is f32, the Euclidean distance to the
+ * nearest background found by 8-neighbour propagation of nearest-background
+ * labels (a neighbour's label is taken when strictly closer after rounding the
+ * distance to float); calc_dist == 0: out is int32, those labels as raster
+ * indices (rows*cols*3 where no background was found, the reference's initial
+ * value).  Where backgrounds tie the index may differ from the reference's,
+ * whose choice follows its queue order; the distances do not.  rows and cols
+ * must be below 65535 and rows*cols*3 must fit int32. */
+nsx_status nsx_distance_transform(nsx_ctx* ctx, const nsx_img* mask, nsx_img* out, int calc_dist, void* stream);
+
 /* ConnComponents::boundingBox (ConnComponents.cpp:241-284): for the labels of
  * an int32 image other than bgval, in order of first appearance in raster
  * order, the bounding box.  boxes (host memory, room for 5 x capacity ints)

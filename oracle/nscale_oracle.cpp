@@ -45,6 +45,8 @@
 #include <climits>
 #include <cfloat>
 #include <limits>
+#include <cmath>
+#include <deque>
 #include <queue>
 #include <vector>
 #include <algorithm>
@@ -604,6 +606,61 @@ int or_bounding_box(const int32_t* label, int w, int h, int bgval, int32_t* out,
     for (int c = 0; c < 5; ++c)
         for (int j = 0; j < n && c * n + j < capacity; ++j) out[c * n + j] = (*cols[c])[j];
     return n;
+}
+
+// distanceTransform (MorphologicOperations.cpp:197-275).  Every pixel starts at
+// the index rows*cols*3 (:214), a point (0, 3*rows) below the image; a background
+// pixel holds its own index (:218) and enters a FIFO queue, in raster order,
+// when any 8-neighbour is nonzero (checkDistNeighbors8, :33-88).  A popped pixel
+// offers its index to its 8 neighbours in the order up-left, up-right, up,
+// down-left, down-right, down, left, right (propagateDist8, :122-193); a
+// neighbour takes it when the distance to it is strictly below the distance to
+// its own (propagateDist, :90-119: sqrt in double, compared after rounding to
+// float) and is queued again.  out_nn gets the indices; out_dist (optional) the
+// float distances to them (:254-266).
+int or_distance_transform(const uint8_t* mask, int rows, int cols, int32_t* out_nn, float* out_dist) {
+    const int64_t n = (int64_t)rows * cols;
+    const int32_t far = (int32_t)(n * 3);
+    std::deque<int64_t> q;
+    for (int y = 0; y < rows; ++y)
+        for (int x = 0; x < cols; ++x) {
+            const int64_t i = (int64_t)y * cols + x;
+            out_nn[i] = far;
+            if (mask[i] != 0) continue;
+            out_nn[i] = (int32_t)i;
+            bool front = false;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int qy = y + dy, qx = x + dx;
+                    if ((dy || dx) && qy >= 0 && qy < rows && qx >= 0 && qx < cols && mask[(int64_t)qy * cols + qx])
+                        front = true;
+                }
+            if (front) q.push_back(i);
+        }
+    auto dist = [&](int x, int y, int32_t label) {
+        const int lx = label % cols, ly = label / cols;
+        const double ddx = (double)(x - lx), ddy = (double)(y - ly);
+        return (float)std::sqrt(ddx * ddx + ddy * ddy);
+    };
+    static const int order[8][2] = {{-1, -1}, {1, -1}, {0, -1}, {-1, 1}, {1, 1}, {0, 1}, {-1, 0}, {1, 0}};
+    while (!q.empty()) {
+        const int64_t i = q.front();
+        q.pop_front();
+        const int y = (int)(i / cols), x = (int)(i % cols);
+        const int32_t cur = out_nn[i];
+        for (const auto& o : order) {
+            const int tx = x + o[0], ty = y + o[1];
+            if (tx < 0 || tx >= cols || ty < 0 || ty >= rows) continue;
+            const int64_t t = (int64_t)ty * cols + tx;
+            if (dist(tx, ty, cur) < dist(tx, ty, out_nn[t])) {
+                out_nn[t] = cur;
+                q.push_back(t);
+            }
+        }
+    }
+    if (out_dist)
+        for (int64_t i = 0; i < n; ++i) out_dist[i] = dist((int)(i % cols), (int)(i / cols), out_nn[i]);
+    return 0;
 }
 
 int or_local_maxima_u8(const uint8_t* in, uint8_t* out, int rows, int cols, int conn) {

@@ -117,6 +117,15 @@ def local_extrema(img, conn=8, minima=False):
     return out
 
 
+def distance_transform(mask):
+    """distanceTransform (uchar mask): (nearest-background index map int32, float distances)."""
+    m = _c(mask, np.uint8)
+    nn = np.empty(m.shape, np.int32)
+    dist = np.empty(m.shape, np.float32)
+    lib().or_distance_transform(_p(m), m.shape[0], m.shape[1], _p(nn), _p(dist))
+    return nn, dist
+
+
 def bounding_box(labels, bgval=0):
     """ConnComponents::boundingBox: (n, [labels, minx, maxx, miny, maxy] as a 5 x n array)."""
     a = _c(labels, np.int32)

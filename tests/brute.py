@@ -183,3 +183,54 @@ def local_maxima_closed_form(img: np.ndarray, conn: int) -> np.ndarray:
         bad = np.unique(lab[higher & (lab > 0)])
         out &= ~np.isin(lab, bad[bad > 0])
     return np.where(out, 255, 0).astype(np.uint8)
+
+
+def exact_distance_with_far(mask):
+    """Exact Euclidean distance of every pixel to the nearest zero pixel of a
+    u8 mask (scipy's separable EDT), or to the point (0, 3*rows) when that is
+    nearer, as float32(sqrt(double(d^2))).  Also returns the squared distances."""
+    from scipy import ndimage
+    r, c = mask.shape
+    yy, xx = np.mgrid[0:r, 0:c]
+    far = (xx.astype(np.int64) ** 2 + (3 * r - yy.astype(np.int64)) ** 2)
+    if (mask == 0).any():
+        d2 = np.rint(ndimage.distance_transform_edt(mask != 0) ** 2).astype(np.int64)
+        d2 = np.minimum(d2, far)
+    else:
+        d2 = far
+    return np.sqrt(d2.astype(np.float64)).astype(np.float32), d2
+
+
+def label_distance(nn, shape):
+    """float32 distance from every pixel to the point its raster-index label
+    names (rows*cols*3 names (0, 3*rows))."""
+    r, c = shape
+    yy, xx = np.mgrid[0:r, 0:c]
+    lx = (nn.astype(np.int64) % c)
+    ly = (nn.astype(np.int64) // c)
+    d2 = (xx - lx) ** 2 + (yy - ly) ** 2
+    return np.sqrt(d2.astype(np.float64)).astype(np.float32)
+
+
+def distance_fixed_point_violations(nn, shape):
+    """Pixels p with an 8-neighbour q whose label is strictly closer to p than
+    p's own, comparing float32-rounded distances (the propagation's rule)."""
+    r, c = shape
+    yy, xx = np.mgrid[0:r, 0:c]
+
+    def dist_to(lab):
+        lx, ly = lab.astype(np.int64) % c, lab.astype(np.int64) // c
+        return np.sqrt(((xx - lx) ** 2 + (yy - ly) ** 2).astype(np.float64)).astype(np.float32)
+
+    own = dist_to(nn)
+    bad = np.zeros(shape, bool)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            if dy == 0 and dx == 0:
+                continue
+            shifted = np.full(shape, r * c * 3, np.int64)
+            ys, yd = (slice(max(dy, 0), r + min(dy, 0)), slice(max(-dy, 0), r + min(-dy, 0)))
+            xs, xd = (slice(max(dx, 0), c + min(dx, 0)), slice(max(-dx, 0), c + min(-dx, 0)))
+            shifted[yd, xd] = nn[ys, xs]
+            bad |= dist_to(shifted) < own
+    return int(bad.sum())

@@ -435,3 +435,75 @@ def test_local_extrema(nh, orc, gpu, conn):
         nh.localMaxima(np.zeros((3, 3), np.uint8), out=np.zeros((3, 3), np.float32))
     e = np.zeros((0, 7), np.uint8)
     assert nh.localMaxima(e).shape == e.shape
+
+
+def _dt_gpu_masks(rng):
+    from scipy import ndimage
+    out = []
+    for shape, p in [((1, 1), 0.5), ((1, 300), 0.01), ((300, 1), 0.01), ((63, 65), 0.02), ((130, 257), 0.003),
+                     ((257, 300), 0.0005)]:
+        out.append(np.where(rng.random(shape) < p, 0, 255).astype(np.uint8))
+    blob = ndimage.gaussian_filter(rng.random((300, 410)), 4) > 0.5
+    out.append(np.where(blob, 255, 0).astype(np.uint8))
+    out.append(np.full((70, 130), 255, np.uint8))
+    out.append(np.zeros((65, 66), np.uint8))
+    wide = np.full((2, 400), 255, np.uint8)
+    wide[0, 399] = 0
+    out.append(wide)
+    # distances above 1024, where float rounding makes ties between squares
+    corner = np.full((1300, 1100), 255, np.uint8)
+    corner[0, 0] = 0
+    corner[1299, 1099] = 0
+    out.append(corner)
+    return out
+
+
+def test_distance_transform(nh, orc, gpu):
+    """distanceTransform: float distances equal the oracle's bit for bit; every
+    int32 label is a background pixel (or rows*cols*3) at that distance, and no
+    neighbour's label is strictly closer; host and device inputs."""
+    import torch
+    rng = np.random.default_rng(151)
+    for m in _dt_gpu_masks(rng):
+        _, want = orc.distance_transform(m)
+        got = nh.distanceTransform(m)
+        assert got.dtype == np.float32
+        assert np.array_equal(got, want), m.shape
+        nn = nh.distanceTransform(m, False)
+        assert nn.dtype == np.int32
+        real = nn != 3 * m.size
+        assert (m.reshape(-1)[nn[real]] == 0).all()
+        assert np.array_equal(brute.label_distance(nn, m.shape), want)
+        if m.size <= 200_000:
+            assert brute.distance_fixed_point_violations(nn, m.shape) == 0
+    m = _dt_gpu_masks(np.random.default_rng(151))[6]
+    _, want = orc.distance_transform(m)
+    assert np.array_equal(nh.distanceTransform(torch.from_numpy(m).cuda()).cpu().numpy(), want)
+    with pytest.raises(nh.NsxError):
+        nh.distanceTransform(np.zeros((3, 3), np.int32))
+    with pytest.raises(nh.NsxError):
+        nh.distanceTransform(np.zeros((3, 3), np.uint8), out=np.zeros((3, 3), np.int32))
+    e = np.zeros((0, 7), np.uint8)
+    assert nh.distanceTransform(e).shape == e.shape
+
+
+def test_distance_transform_stuck_pixel(nh, orc, gpu):
+    """A sparse 1000x1000 background where the oracle's FIFO propagation stops
+    short of the exact distance at one pixel.  Which fixed point such a pixel
+    reaches depends on the order of the offers, so the GPU result is held to
+    what the rule fixes: the same distances elsewhere, never nearer than the
+    exact distance, and no neighbour's label strictly closer."""
+    rng = np.random.default_rng(2)
+    m = None
+    for p in (0.00002, 0.0001, 0.0005):
+        m = np.where(rng.random((1000, 1000)) < p, 0, 255).astype(np.uint8)
+    _, want = orc.distance_transform(m)
+    exact, _ = brute.exact_distance_with_far(m)
+    got = nh.distanceTransform(m)
+    differ = got != want
+    assert int(differ.sum()) <= 10
+    assert (got >= exact).all()
+    assert (want[differ] > exact[differ]).all() or (got[differ] > exact[differ]).all()
+    nn = nh.distanceTransform(m, False)
+    assert np.array_equal(brute.label_distance(nn, m.shape), got)
+    assert brute.distance_fixed_point_violations(nn, m.shape) == 0

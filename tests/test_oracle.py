@@ -346,3 +346,54 @@ def test_local_extrema_closed_form(orc, conn):
                 inv = (np.float32(0) - img).astype(np.float32)
             assert np.array_equal(orc.local_extrema(img, conn, minima=True),
                                   brute.local_maxima_closed_form(inv, conn))
+
+
+def _dt_masks(rng):
+    from scipy import ndimage
+    out = []
+    for shape, p in [((1, 1), 0.5), ((1, 40), 0.1), ((40, 1), 0.1), ((23, 37), 0.05), ((64, 65), 0.01),
+                     ((130, 97), 0.002)]:
+        out.append(np.where(rng.random(shape) < p, 0, 255).astype(np.uint8))
+    blob = ndimage.gaussian_filter(rng.random((90, 110)), 3) > 0.5
+    out.append(np.where(blob, 255, 0).astype(np.uint8))
+    out.append(np.full((17, 19), 255, np.uint8))            # no background: the start point stays
+    out.append(np.zeros((5, 6), np.uint8))                  # all background
+    wide = np.full((2, 40), 255, np.uint8)
+    wide[0, 39] = 0                                         # the start point (0, 6) beats it on the left
+    out.append(wide)
+    return out
+
+
+def test_distance_transform_exact(orc):
+    """distanceTransform: on these inputs the propagation finds the exact
+    nearest background (or the start point (0, 3*rows) where that is nearer):
+    distances equal scipy's EDT, every label is a background pixel (or
+    rows*cols*3) at the reported distance, and no neighbour's label is closer."""
+    rng = np.random.default_rng(141)
+    for m in _dt_masks(rng):
+        nn, dist = orc.distance_transform(m)
+        want, _ = brute.exact_distance_with_far(m)
+        assert np.array_equal(dist, want), m.shape
+        n = m.size
+        real = nn != 3 * n
+        assert (m.reshape(-1)[nn[real]] == 0).all()
+        assert np.array_equal(brute.label_distance(nn, m.shape), dist)
+        assert brute.distance_fixed_point_violations(nn, m.shape) == 0
+    wide = _dt_masks(np.random.default_rng(141))[-1]
+    nn, _ = orc.distance_transform(wide)
+    assert nn[0, 0] == 3 * wide.size and nn[0, 38] == 39
+
+
+def test_distance_transform_stuck_pixel(orc):
+    """Far from a sparse background the 8-neighbour propagation can stop short of
+    the true nearest background: the result is still a fixed point of the rule,
+    never nearer than the exact distance, and equals it almost everywhere."""
+    rng = np.random.default_rng(2)
+    m = None
+    for _ in range(3):
+        m = np.where(rng.random((1000, 1000)) < [0.00002, 0.0001, 0.0005][_], 0, 255).astype(np.uint8)
+    nn, dist = orc.distance_transform(m)
+    want, _ = brute.exact_distance_with_far(m)
+    assert (dist >= want).all()
+    assert 1 <= int((dist != want).sum()) <= 10
+    assert brute.distance_fixed_point_violations(nn, m.shape) == 0
