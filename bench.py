@@ -44,9 +44,11 @@ def parse():
                    help="skip the CPU baseline leg (the oracle run that also checks the GPU results)")
     p.add_argument("--cpu-runs", type=int, default=20, help="oracle chain runs in the CPU sample")
     p.add_argument("--workload", default="chain4096",
-                   choices=["chain4096", "batch", "maze", "serve", "copy"],
+                   choices=["chain4096", "batch", "maze", "serve", "copy", "ops"],
                    help="chain4096: the headline 4096^2 chain (default); batch: 256 u8 + 32 f32 uneven tiles "
-                        "batched vs per-tile; maze: 16384^2 maze stress; serve: continuous-batching server")
+                        "batched vs per-tile; maze: 16384^2 maze stress; serve: continuous-batching server; "
+                        "ops: seeded imfill, bwselect, imhmin, localMaxima/Minima, boundingBox and "
+                        "distanceTransform, each on the 4096^2 disk image")
     p.add_argument("--tiles-u8", type=int, default=256)
     p.add_argument("--tiles-f32", type=int, default=32)
     p.add_argument("--tune-reps", type=int, default=2,
@@ -514,6 +516,97 @@ def bench_copy(args):
                       "cpu_baseline": None}))
 
 
+def bench_ops(args):
+    """The operators around the chain, each timed alone on the 4096^2 disk
+    image (inputs resident in HBM, outputs to HBM; each call ends with the
+    stream synchronised, as the entry points do), each checked against the
+    oracle, whose single-thread time is the CPU baseline."""
+    import numpy as np
+    import torch
+
+    import nscale_hip as nh
+    n = args.size
+    mask = nh.gen_disks_u8(n, n, args.disks, 0x5EED0001)
+    binm = np.where(mask > 128, 255, 0).astype(np.uint8)
+    rng = np.random.default_rng(0x5EED0003)
+    seeds = np.where(rng.random((n, n)) < 1 / 2000, 255, 0).astype(np.uint8)
+    dev = torch.device("cuda:0")
+    d_mask, d_bin, d_seeds = (torch.from_numpy(a).to(dev) for a in (mask, binm, seeds))
+    d_labels = nh.bwlabel2(d_bin, 8, True)
+    labels = d_labels.cpu().numpy()
+    orc = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+    npx = n * n
+    # name, GPU call, oracle call, HBM bytes the operator must move at least
+    ops = [
+        ("imfill gray 4-conn", lambda: nh.imfill(d_mask, d_seeds, False, 4),
+         lambda: orc.imfill(mask, seeds, False, 4), 3 * npx),
+        ("bwselect 8-conn", lambda: nh.bwselect(d_bin, d_seeds, 8), lambda: orc.bwselect(binm, seeds, 8), 3 * npx),
+        ("imhmin h=10 8-conn", lambda: nh.imhmin(d_mask, 10, 8), lambda: orc.imhmin(mask, 10, 8), 2 * npx),
+        ("localMaxima 8-conn", lambda: nh.localMaxima(d_mask, 8), lambda: orc.local_extrema(mask, 8), 2 * npx),
+        ("localMinima 8-conn", lambda: nh.localMinima(d_mask, 8), lambda: orc.local_extrema(mask, 8, True),
+         2 * npx),
+        ("boundingBox", lambda: nh.bounding_box(d_labels), lambda: orc.bounding_box(labels), 4 * npx),
+        ("distanceTransform", lambda: nh.distanceTransform(d_bin), lambda: orc.distance_transform(binm)[1],
+         5 * npx),
+    ]
+    reps = max(1, min(args.steps, 10))
+    res = {}
+    total_ms = 0.0
+    for name, gpu, cpu, alg in ops:
+        for _ in range(max(1, args.warmup)):
+            got = gpu()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            got = gpu()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        ms = statistics.median(ts)
+        total_ms += ms
+        r = {"ms": round(ms, 4), "mpix_s": round(npx / ms / 1e3, 1),
+             "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if orc is not None:
+            t0 = time.perf_counter()
+            want = cpu()
+            cpu_ms = (time.perf_counter() - t0) * 1e3
+            if name == "boundingBox":
+                same = got[0] == want[0] and np.array_equal(got[1], want[1])
+            else:
+                g = got.cpu().numpy()
+                same = bool(np.array_equal(g, want))
+                if name == "distanceTransform":
+                    # the propagation can stop short of the exact distance at
+                    # a few pixels, at places that depend on the order of the
+                    # offers: count them on both sides against scipy's EDT
+                    from scipy import ndimage
+                    yy, xx = np.mgrid[0:n, 0:n]
+                    d2 = np.rint(ndimage.distance_transform_edt(binm != 0) ** 2).astype(np.int64)
+                    d2 = np.minimum(d2, xx.astype(np.int64) ** 2 + (3 * n - yy.astype(np.int64)) ** 2)
+                    exact = np.sqrt(d2.astype(np.float64)).astype(np.float32)
+                    r.update({"pixels_differing": int((g != want).sum()),
+                              "oracle_not_exact": int((want != exact).sum()),
+                              "gpu_not_exact": int((g != exact).sum()),
+                              "gpu_below_exact": int((g < exact).sum())})
+            r.update({"cpu_ms": round(cpu_ms, 1), "speedup": round(cpu_ms / ms, 1), "bitexact": bool(same)})
+        res[name] = r
+        log(name, r)
+    value = len(ops) * npx / (total_ms * 1e-3) / 1e6
+    print(json.dumps({
+        "metric": "Mpix/s over the operators around the chain @4096^2 (each run once)", "value": round(value, 1),
+        "unit": "Mpix/s", "n_gpus": 1, "steps": reps, "warmup": args.warmup, "ms_per_step": round(total_ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": f"{n}x{n} u8 disks({args.disks}); binary = >128; seeds 1 in 2000 pixels; "
+                               "labels = bwlabel8", "ops": res},
+        "roofline": None,
+        "cpu_baseline": None if orc is None else {
+            "value": round(len(ops) * npx / (sum(r["cpu_ms"] for r in res.values()) * 1e-3) / 1e6, 3),
+            "unit": "Mpix/s", "cores": 1, "kind": "port", "sample": "each operator once on the same image"},
+    }))
+
+
 def cpu_baseline_maze(nh, n, ctx):
     """The oracle's imreconstruct + bwlabel2 (single thread) on an n x n maze of
     the same construction (by default the whole 16384^2 workload, a few seconds
@@ -630,6 +723,8 @@ def main():
         return bench_maze(args)
     if args.workload == "copy":
         return bench_copy(args)
+    if args.workload == "ops":
+        return bench_ops(args)
     n = args.size
     seed = 0x5EED0001
     mask = nh.gen_disks_u8(n, n, args.disks, seed)

@@ -1239,54 +1239,78 @@ __global__ void lx_mark_kernel(const char* work, int64_t pw, const char* marker,
 }
 
 // A candidate next to a marked candidate of the same value is marked (the
-// reference's flood of equal-valued candidates).  Marks only grow, so the
-// in-place updates within a launch are safe; *changed = 1 when one was made.
-__global__ void lx_spread_kernel(const char* work, int64_t pw, uint8_t* flags, int rows, int cols, int dtype,
-                                 int conn, int32_t* changed) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= cols || flags[(int64_t)y * cols + x] != 1) return;
-    const float v = lx_get(work, pw, y, x, dtype);
-    for (int k = 0; k < (conn == 8 ? 8 : 4); ++k) {
-        const int dy = k < 4 ? (k == 0 ? -1 : k == 1 ? 1 : 0) : (k < 6 ? -1 : 1);
-        const int dx = k < 4 ? (k == 2 ? -1 : k == 3 ? 1 : 0) : ((k & 1) ? 1 : -1);
-        const int qy = y + dy, qx = x + dx;
-        if (qy >= 0 && qy < rows && qx >= 0 && qx < cols && flags[(int64_t)qy * cols + qx] == 2 &&
-            lx_get(work, pw, qy, qx, dtype) == v) {
-            flags[(int64_t)y * cols + x] = 2;
-            *changed = 1;
-            return;
+// reference's flood of equal-valued candidates).  One 64x64 tile per block:
+// values and marks with a one-pixel halo in LDS, the rule applied until the
+// tile is stable, the marks written back.  A tile runs only when it or one of
+// its 8 neighbour tiles changed in the previous round (prev); marks only grow,
+// so reading a halo that a neighbour is writing is harmless.
+constexpr int LX_T = 64;
+constexpr int LX_H = LX_T + 2;
+
+__global__ __launch_bounds__(256) void lx_tile_kernel(const char* work, int64_t pw, uint8_t* flags, int rows,
+                                                      int cols, int dtype, int conn, int tiles_x, int tiles_y,
+                                                      const uint8_t* prev, uint8_t* next, int32_t* changed) {
+    __shared__ float sv[LX_H * LX_H];
+    __shared__ uint8_t sf[LX_H * LX_H];
+    __shared__ int active;
+    const int tile = blockIdx.x;
+    const int tx = tile % tiles_x, ty = tile / tiles_x;
+    if (threadIdx.x == 0) {
+        int a = 0;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int nx = tx + dx, ny = ty + dy;
+                if (nx >= 0 && nx < tiles_x && ny >= 0 && ny < tiles_y && prev[ny * tiles_x + nx]) a = 1;
+            }
+        active = a;
+    }
+    __syncthreads();
+    if (!active) {
+        if (threadIdx.x == 0) next[tile] = 0;
+        return;
+    }
+    const int x0 = tx * LX_T - 1, y0 = ty * LX_T - 1;
+    for (int i = threadIdx.x; i < LX_H * LX_H; i += 256) {
+        const int gx = x0 + i % LX_H, gy = y0 + i / LX_H;
+        const bool in = gx >= 0 && gx < cols && gy >= 0 && gy < rows;
+        sf[i] = in ? flags[(int64_t)gy * cols + gx] : 0;
+        sv[i] = in ? lx_get(work, pw, gy, gx, dtype) : 0.0f;
+    }
+    __syncthreads();
+    const int off[8] = {-LX_H, LX_H, -1, 1, -LX_H - 1, -LX_H + 1, LX_H - 1, LX_H + 1};
+    const int nn = conn == 8 ? 8 : 4;
+    bool any = false;
+    for (;;) {
+        int ch = 0;
+        for (int k = 0; k < LX_T * LX_T / 256; ++k) {
+            const int p = threadIdx.x + 256 * k;
+            const int c = (p / LX_T + 1) * LX_H + (p & (LX_T - 1)) + 1;
+            if (sf[c] != 1) continue;
+            const float v = sv[c];
+            for (int j = 0; j < nn; ++j)
+                if (sf[c + off[j]] == 2 && sv[c + off[j]] == v) {
+                    sf[c] = 2;
+                    ch = 1;
+                    break;
+                }
         }
+        if (!__syncthreads_or(ch)) break;
+        any = true;
+    }
+    if (any) {
+        for (int i = threadIdx.x; i < LX_T * LX_T; i += 256) {
+            const int gx = tx * LX_T + (i & (LX_T - 1)), gy = ty * LX_T + i / LX_T;
+            if (gx < cols && gy < rows) flags[(int64_t)gy * cols + gx] = sf[(i / LX_T + 1) * LX_H + (i & (LX_T - 1)) + 1];
+        }
+    }
+    if (threadIdx.x == 0) {
+        next[tile] = any ? 1 : 0;
+        if (any) *changed = 1;
     }
 }
 
-// The same rule run along whole lines, forward then back: one thread per row
-// (cols_dir 0) or per column (cols_dir 1), so a mark crosses a plateau's
-// straight runs in one launch.
-__global__ void lx_sweep_kernel(const char* work, int64_t pw, uint8_t* flags, int rows, int cols, int dtype,
-                                int cols_dir, int32_t* changed) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nlines = cols_dir ? cols : rows, len = cols_dir ? rows : cols;
-    if (t >= nlines) return;
-    bool any = false;
-    for (int pass = 0; pass < 2; ++pass) {
-        bool carry = false;
-        float prev = 0.0f;
-        for (int i = 0; i < len; ++i) {
-            const int j = pass ? len - 1 - i : i;
-            const int y = cols_dir ? j : t, x = cols_dir ? t : j;
-            uint8_t* f = flags + (int64_t)y * cols + x;
-            const float v = lx_get(work, pw, y, x, dtype);
-            uint8_t fv = *f;
-            if (fv == 1 && carry && v == prev) {
-                fv = 2;
-                *f = 2;
-                any = true;
-            }
-            carry = fv == 2;
-            prev = v;
-        }
-    }
-    if (any) *changed = 1;
+__global__ void lx_tiles_init_kernel(uint8_t* f, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) f[i] = 1;
 }
 
 __global__ void lx_out_kernel(const uint8_t* flags, char* out, int64_t po, int rows, int cols) {
@@ -1302,17 +1326,22 @@ __global__ void lx_zero_kernel(int32_t* w) {
 // ---- bounding boxes (ConnComponents::boundingBox) ------------------------------
 // misc: [0] raster index of the first label -1 or 0 that is not the background
 // (the reference stops there), [1] first raster index of a label outside
-// [-1, rows*cols], [2] the largest label.  One atomic per wave and word.
-__global__ void bbox_scan_kernel(const char* lab, int64_t pl, int rows, int cols, int bgval, int32_t* misc) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    const int32_t i = y * cols + x;
+// [-1, rows*cols], [2] the largest label.  Blocks stride over rows; one
+// atomic per block and word.
+__global__ __launch_bounds__(256) void bbox_scan_kernel(const char* lab, int64_t pl, int rows, int cols, int bgval,
+                                                        int32_t* misc) {
+    __shared__ int32_t red[3][4];
     int32_t stop = INT_MAX, bad = INT_MAX, mx = 0;
-    if (x < cols) {
-        const int32_t v = ((const int32_t*)(lab + (int64_t)y * pl))[x];
-        if (v != bgval) {
-            if (v == -1 || v == 0) stop = i;
-            else if (v < -1 || (int64_t)v > (int64_t)rows * cols) bad = i;
-            else mx = v;
+    const int64_t lim = (int64_t)rows * cols;
+    for (int y = blockIdx.x; y < rows; y += gridDim.x) {
+        const int32_t* row = (const int32_t*)(lab + (int64_t)y * pl);
+        for (int x = threadIdx.x; x < cols; x += 256) {
+            const int32_t v = row[x];
+            if (v == bgval) continue;
+            const int32_t i = y * cols + x;
+            if (v == -1 || v == 0) stop = min(stop, i);
+            else if (v < -1 || (int64_t)v > lim) bad = min(bad, i);
+            else mx = max(mx, v);
         }
     }
 #pragma unroll
@@ -1321,7 +1350,19 @@ __global__ void bbox_scan_kernel(const char* lab, int64_t pl, int rows, int cols
         bad = min(bad, __shfl_xor(bad, o, 64));
         mx = max(mx, __shfl_xor(mx, o, 64));
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        red[0][w] = stop;
+        red[1][w] = bad;
+        red[2][w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; ++k) {
+            stop = min(stop, red[0][k]);
+            bad = min(bad, red[1][k]);
+            mx = max(mx, red[2][k]);
+        }
         if (stop != INT_MAX) atomicMin(&misc[0], stop);
         if (bad != INT_MAX) atomicMin(&misc[1], bad);
         if (mx > 0) atomicMax(&misc[2], mx);
@@ -1526,7 +1567,8 @@ nsx_status bbox_run(nsx_ctx* ctx, const View& lab, int bgval, int32_t* boxes, in
     int32_t* misc = (int32_t*)misc_p;
     const dim3 grid((lab.cols + 255) / 256, lab.rows);
     hipLaunchKernelGGL(bbox_misc_init_kernel, dim3(1), dim3(64), 0, s, misc);
-    hipLaunchKernelGGL(bbox_scan_kernel, grid, dim3(256), 0, s, lab.data, lab.pitch, lab.rows, lab.cols, bgval, misc);
+    hipLaunchKernelGGL(bbox_scan_kernel, dim3(std::min(lab.rows, 8 * ctx->cu_count)), dim3(256), 0, s, lab.data,
+                       lab.pitch, lab.rows, lab.cols, bgval, misc);
     NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, misc, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
     const int32_t stop = ctx->h_pinned[40], bad = ctx->h_pinned[41], maxv = ctx->h_pinned[42];
@@ -1591,23 +1633,27 @@ nsx_status local_extrema_run(nsx_ctx* ctx, const View& img, const View& out, int
     int32_t* changed = (int32_t*)misc_p;
     hipLaunchKernelGGL(lx_mark_kernel, grid, dim3(256), 0, s, work.data, work.pitch, marker.data, marker.pitch,
                        rec.data, rec.pitch, flags, img.rows, img.cols, img.dtype, conn);
-    // spread the marks over equal-valued candidates: line sweeps both ways and
-    // neighbour steps (diagonals) per check; each checked batch with a change
-    // marks at least one more pixel, so the loop ends
+    // spread the marks over equal-valued candidates, tile by tile; each round
+    // with a change marks at least one more pixel, so the rounds end
     const int64_t npix = (int64_t)img.rows * img.cols;
-    for (int64_t batch = 0; batch <= npix; ++batch) {
+    const int tiles_x = (img.cols + LX_T - 1) / LX_T, tiles_y = (img.rows + LX_T - 1) / LX_T;
+    const int ntiles = tiles_x * tiles_y;
+    void* tf_p;
+    NSX_TRY(ctx->ws(WS_CCL_BITS, 2 * (size_t)ntiles, &tf_p));
+    uint8_t* fa = (uint8_t*)tf_p;
+    uint8_t* fb = fa + ntiles;
+    hipLaunchKernelGGL(lx_tiles_init_kernel, dim3(std::min(1024, (ntiles + 255) / 256)), dim3(256), 0, s, fa, ntiles);
+    for (int64_t round = 0; round <= npix; ++round) {
         hipLaunchKernelGGL(lx_zero_kernel, dim3(1), dim3(64), 0, s, changed);
-        for (int k = 0; k < 2; ++k) {
-            hipLaunchKernelGGL(lx_sweep_kernel, dim3((img.rows + 63) / 64), dim3(64), 0, s, work.data, work.pitch,
-                               flags, img.rows, img.cols, img.dtype, 0, changed);
-            hipLaunchKernelGGL(lx_sweep_kernel, dim3((img.cols + 63) / 64), dim3(64), 0, s, work.data, work.pitch,
-                               flags, img.rows, img.cols, img.dtype, 1, changed);
-            hipLaunchKernelGGL(lx_spread_kernel, grid, dim3(256), 0, s, work.data, work.pitch, flags, img.rows,
-                               img.cols, img.dtype, conn, changed);
-        }
+        hipLaunchKernelGGL(lx_tile_kernel, dim3(ntiles), dim3(256), 0, s, work.data, work.pitch, flags, img.rows,
+                           img.cols, img.dtype, conn, tiles_x, tiles_y, (const uint8_t*)fa, fb, changed);
+        NSX_HIP_TRY(ctx, hipGetLastError());
         NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, changed, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
         if (ctx->h_pinned[40] == 0) break;
+        uint8_t* t = fa;
+        fa = fb;
+        fb = t;
     }
     hipLaunchKernelGGL(lx_out_kernel, grid, dim3(256), 0, s, (const uint8_t*)flags, out.data, out.pitch, img.rows,
                        img.cols);

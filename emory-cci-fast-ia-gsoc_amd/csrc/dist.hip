@@ -9,10 +9,12 @@
 // neighbour's label than to its own.
 //
 // Here the same rule runs to a fixed point in 64x64 tiles.  A tile loads its
-// labels and a one-pixel halo into LDS, applies the rule in place until no
-// label in it changes, and writes the tile back; rounds of tile launches repeat
+// labels and a one-pixel halo into LDS, applies the rule in synchronous passes
+// (read one LDS buffer, write the other) until no label in it changes, and
+// writes the tile to the other global buffer; rounds of tile launches repeat
 // until a round changes nothing.  A tile runs in a round only when it or one of
-// its 8 neighbour tiles changed in the previous round.
+// its 8 neighbour tiles changed in the previous round (the others are copied).
+// Every read is of a finished buffer, so the result is deterministic.
 //
 // Labels are packed (y << 16) | x inside the kernels, with DT_FAR for the
 // starting point, so no integer division is needed per offer; the output turns
@@ -25,6 +27,7 @@
 #include "nsx_internal.h"
 
 #include <cmath>
+#include <utility>
 
 namespace nsx {
 namespace {
@@ -60,10 +63,10 @@ __global__ void dt_init_kernel(const char* mask, int64_t pm, uint32_t* lab, int 
     lab[(int64_t)y * cols + x] = m == 0 ? ((uint32_t)y << 16) | (uint32_t)x : DT_FAR;
 }
 
-__global__ __launch_bounds__(256) void dt_tile_kernel(uint32_t* lab, int rows, int cols, int tiles_x, int tiles_y,
-                                                      int far_y, const uint8_t* prev, uint8_t* next,
-                                                      int32_t* changed) {
-    __shared__ uint32_t s[DH * DH];
+__global__ __launch_bounds__(256) void dt_tile_kernel(const uint32_t* src, uint32_t* dst, int rows, int cols,
+                                                      int tiles_x, int tiles_y, int far_y, const uint8_t* prev,
+                                                      uint8_t* next, int32_t* changed) {
+    __shared__ uint32_t s[2][DH * DH];
     __shared__ int active;
     const int tile = blockIdx.x;
     const int tx = tile % tiles_x, ty = tile / tiles_x;
@@ -78,6 +81,11 @@ __global__ __launch_bounds__(256) void dt_tile_kernel(uint32_t* lab, int rows, i
     }
     __syncthreads();
     if (!active) {
+        // carry the tile into the other buffer unchanged
+        for (int i = threadIdx.x; i < DT * DT; i += 256) {
+            const int gx = tx * DT + (i & (DT - 1)), gy = ty * DT + i / DT;
+            if (gx < cols && gy < rows) dst[(int64_t)gy * cols + gx] = src[(int64_t)gy * cols + gx];
+        }
         if (threadIdx.x == 0) next[tile] = 0;
         return;
     }
@@ -85,49 +93,54 @@ __global__ __launch_bounds__(256) void dt_tile_kernel(uint32_t* lab, int rows, i
     // halo outside the image reads as the starting point, which never wins
     for (int i = threadIdx.x; i < DH * DH; i += 256) {
         const int gx = x0 + i % DH, gy = y0 + i / DH;
-        s[i] = (gx >= 0 && gx < cols && gy >= 0 && gy < rows) ? lab[(int64_t)gy * cols + gx] : DT_FAR;
+        const uint32_t v = (gx >= 0 && gx < cols && gy >= 0 && gy < rows) ? src[(int64_t)gy * cols + gx] : DT_FAR;
+        s[0][i] = v;
+        s[1][i] = v;
     }
     __syncthreads();
     // offers in the reference's order: up-left, up-right, up, down-left,
-    // down-right, down, left, right
+    // down-right, down, left, right.  Passes read one buffer and write the
+    // other, so the result does not depend on the order threads run in.
     const int off[8] = {-DH - 1, -DH + 1, -DH, DH - 1, DH + 1, DH, -1, 1};
     bool any = false;
+    int cur = 0;
     for (;;) {
+        const uint32_t* rd = s[cur];
+        uint32_t* wr = s[cur ^ 1];
         int ch = 0;
         for (int k = 0; k < DT * DT / 256; ++k) {
             const int p = threadIdx.x + 256 * k;
             const int px = p & (DT - 1), py = p / DT;
             const int gx = tx * DT + px, gy = ty * DT + py;
-            if (gx >= cols || gy >= rows) continue;
             const int c = (py + 1) * DH + px + 1;
-            const uint32_t own = s[c];
+            const uint32_t own = rd[c];
             uint32_t best = own;
-            uint64_t bd = dt_d2(gx, gy, own, far_y);
-            if (bd == 0) continue;
+            if (gx < cols && gy < rows) {
+                uint64_t bd = dt_d2(gx, gy, own, far_y);
+                if (bd != 0) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint32_t cand = s[c + off[j]];
-                if (cand == best) continue;
-                const uint64_t d = dt_d2(gx, gy, cand, far_y);
-                if (dt_closer(d, bd)) {
-                    best = cand;
-                    bd = d;
+                    for (int j = 0; j < 8; ++j) {
+                        const uint32_t cand = rd[c + off[j]];
+                        if (cand == best) continue;
+                        const uint64_t d = dt_d2(gx, gy, cand, far_y);
+                        if (dt_closer(d, bd)) {
+                            best = cand;
+                            bd = d;
+                        }
+                    }
                 }
             }
-            if (best != own) {
-                s[c] = best;
-                ch = 1;
-            }
+            wr[c] = best;
+            ch |= best != own;
         }
+        cur ^= 1;
         if (!__syncthreads_or(ch)) break;
         any = true;
     }
-    if (any) {
-        for (int i = threadIdx.x; i < DT * DT; i += 256) {
-            const int px = i & (DT - 1), py = i / DT;
-            const int gx = tx * DT + px, gy = ty * DT + py;
-            if (gx < cols && gy < rows) lab[(int64_t)gy * cols + gx] = s[(py + 1) * DH + px + 1];
-        }
+    for (int i = threadIdx.x; i < DT * DT; i += 256) {
+        const int px = i & (DT - 1), py = i / DT;
+        const int gx = tx * DT + px, gy = ty * DT + py;
+        if (gx < cols && gy < rows) dst[(int64_t)gy * cols + gx] = s[cur][(py + 1) * DH + px + 1];
     }
     if (threadIdx.x == 0) {
         next[tile] = any ? 1 : 0;
@@ -163,30 +176,32 @@ nsx_status dist_run(nsx_ctx* ctx, const View& mask, const View& out, bool calc_d
     const int64_t npix = (int64_t)rows * cols;
     const int tiles_x = (cols + DT - 1) / DT, tiles_y = (rows + DT - 1) / DT;
     const int ntiles = tiles_x * tiles_y;
-    void *lab_p, *flags_p, *misc_p;
+    void *lab_p, *lab2_p, *flags_p, *misc_p;
     NSX_TRY(ctx->ws(WS_CCL_LABEL, sizeof(uint32_t) * (size_t)npix, &lab_p));
+    NSX_TRY(ctx->ws(WS_CCL_AREA, sizeof(uint32_t) * (size_t)npix, &lab2_p));
     NSX_TRY(ctx->ws(WS_CCL_BITS, 2 * (size_t)ntiles, &flags_p));
     NSX_TRY(ctx->ws(WS_CCL_MISC, 64, &misc_p));
     uint32_t* lab = (uint32_t*)lab_p;
+    uint32_t* lab2 = (uint32_t*)lab2_p;
     uint8_t* fa = (uint8_t*)flags_p;
     uint8_t* fb = fa + ntiles;
     int32_t* changed = (int32_t*)misc_p;
     const dim3 grid((cols + 255) / 256, rows);
     hipLaunchKernelGGL(dt_init_kernel, grid, dim3(256), 0, s, mask.data, mask.pitch, lab, rows, cols, fa, ntiles);
     NSX_HIP_TRY(ctx, hipGetLastError());
-    // each round that changes something moves a label strictly closer, so the
-    // rounds end; the bound only guards the loop
+    // rounds read one label buffer and write the other; each round that
+    // changes something moves a label strictly closer, so the rounds end (the
+    // bound only guards the loop)
     for (int64_t round = 0; round <= npix; ++round) {
         hipLaunchKernelGGL(dt_zero_kernel, dim3(1), dim3(64), 0, s, changed);
-        hipLaunchKernelGGL(dt_tile_kernel, dim3(ntiles), dim3(256), 0, s, lab, rows, cols, tiles_x, tiles_y, 3 * rows,
-                           (const uint8_t*)fa, fb, changed);
+        hipLaunchKernelGGL(dt_tile_kernel, dim3(ntiles), dim3(256), 0, s, (const uint32_t*)lab, lab2, rows, cols,
+                           tiles_x, tiles_y, 3 * rows, (const uint8_t*)fa, fb, changed);
         NSX_HIP_TRY(ctx, hipGetLastError());
         NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, changed, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+        std::swap(lab, lab2);
         if (ctx->h_pinned[40] == 0) break;
-        uint8_t* t = fa;
-        fa = fb;
-        fb = t;
+        std::swap(fa, fb);
     }
     hipLaunchKernelGGL(dt_out_kernel, grid, dim3(256), 0, s, (const uint32_t*)lab, out.data, out.pitch, rows, cols,
                        3 * rows, calc_dist ? 1 : 0);

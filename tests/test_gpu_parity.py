@@ -507,3 +507,6 @@ def test_distance_transform_stuck_pixel(nh, orc, gpu):
     nn = nh.distanceTransform(m, False)
     assert np.array_equal(brute.label_distance(nn, m.shape), got)
     assert brute.distance_fixed_point_violations(nn, m.shape) == 0
+    # the same fixed point on every call
+    assert np.array_equal(nh.distanceTransform(m, False), nn)
+    assert np.array_equal(nh.distanceTransform(m), got)
