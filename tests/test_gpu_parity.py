@@ -483,6 +483,8 @@ def test_distance_transform(nh, orc, gpu):
         nh.distanceTransform(np.zeros((3, 3), np.int32))
     with pytest.raises(nh.NsxError):
         nh.distanceTransform(np.zeros((3, 3), np.uint8), out=np.zeros((3, 3), np.int32))
+    with pytest.raises(nh.NsxError):
+        nh.distanceTransform(np.zeros((1, 65535), np.uint8))   # packed labels need cols < 65535
     e = np.zeros((0, 7), np.uint8)
     assert nh.distanceTransform(e).shape == e.shape
 
