@@ -100,13 +100,26 @@ __global__ __launch_bounds__(256) void dt_tile_kernel(const uint32_t* src, uint3
     __syncthreads();
     // offers in the reference's order: up-left, up-right, up, down-left,
     // down-right, down, left, right.  Passes read one buffer and write the
-    // other, so the result does not depend on the order threads run in.
+    // other, so the result does not depend on the order threads run in.  A
+    // pixel none of whose neighbours changed in the previous pass would find
+    // the same best label as then, so it is skipped: rm[r] holds, per tile row,
+    // the lanes (columns) that changed (the wave of a step owns one row).
+    __shared__ uint64_t rm[2][DT + 2];
+    if (threadIdx.x < DT + 2) {
+        rm[0][threadIdx.x] = 0;
+        rm[1][threadIdx.x] = 0;
+    }
+    __syncthreads();
     const int off[8] = {-DH - 1, -DH + 1, -DH, DH - 1, DH + 1, DH, -1, 1};
+    const int lane = threadIdx.x & 63;
+    const uint64_t near = lane == 0 ? 3ull : (7ull << (lane - 1));
     bool any = false;
     int cur = 0;
-    for (;;) {
+    for (int pass = 0;; ++pass) {
         const uint32_t* rd = s[cur];
         uint32_t* wr = s[cur ^ 1];
+        const uint64_t* rmr = rm[cur];
+        uint64_t* rmw = rm[cur ^ 1];
         int ch = 0;
         for (int k = 0; k < DT * DT / 256; ++k) {
             const int p = threadIdx.x + 256 * k;
@@ -115,7 +128,10 @@ __global__ __launch_bounds__(256) void dt_tile_kernel(const uint32_t* src, uint3
             const int c = (py + 1) * DH + px + 1;
             const uint32_t own = rd[c];
             uint32_t best = own;
-            if (gx < cols && gy < rows) {
+            // rows py, py+1, py+2 of rm are tile rows py-1, py, py+1; the halo
+            // rows and columns only count in the first pass
+            const bool look = pass == 0 || ((rmr[py] | rmr[py + 1] | rmr[py + 2]) & near) != 0;
+            if (look && gx < cols && gy < rows) {
                 uint64_t bd = dt_d2(gx, gy, own, far_y);
                 if (bd != 0) {
 #pragma unroll
@@ -131,7 +147,9 @@ __global__ __launch_bounds__(256) void dt_tile_kernel(const uint32_t* src, uint3
                 }
             }
             wr[c] = best;
-            ch |= best != own;
+            const uint64_t moved = __ballot(best != own);
+            if (lane == 0) rmw[py + 1] = moved;
+            ch |= moved != 0;
         }
         cur ^= 1;
         if (!__syncthreads_or(ch)) break;
