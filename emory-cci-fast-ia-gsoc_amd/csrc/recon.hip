@@ -699,8 +699,18 @@ void recon_round_kernel(ReconArgs A, int round) {
             }
         }
     }
+    // Blocks go to the XCDs in turn (block b to XCD b mod 8).  For one image
+    // XCD x takes the x-th eighth of the work items, so each XCD works on one
+    // region of the image and neighbouring tiles meet in its L2.  A batch
+    // keeps the plain stride: its tiles differ in cost, and contiguous chunks
+    // would pile the costly ones onto one XCD.
+    const int nx = A.nimg > 1 ? 1 : (gridDim.x < 8 ? (int)gridDim.x : 8);
+    const int xcd = (int)blockIdx.x % nx, xb = (int)blockIdx.x / nx;
+    const int xstride = ((int)gridDim.x - xcd + nx - 1) / nx;
     if (FIRST && !A.first_list) {
-        for (int i = blockIdx.x; i < A.ntiles; i += gridDim.x) {
+        const int chunk = (A.ntiles + nx - 1) / nx;
+        const int lo = min(A.ntiles, xcd * chunk), hi = min(A.ntiles, lo + chunk);
+        for (int i = lo + xb; i < hi; i += xstride) {
             ImgDesc D;
             int lt;
             tile_of(A, i, D, lt);
@@ -714,7 +724,9 @@ void recon_round_kernel(ReconArgs A, int round) {
     const uint32_t incl = wave_incl_scan(cq);
     const int n = __builtin_amdgcn_readlane((int)incl, NSEG - 1);
     const int32_t* list = (round & 1) ? A.list1 : A.list0;
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int chunk = (n + nx - 1) / nx;
+    const int lo = min(n, xcd * chunk), hi = min(n, lo + chunk);
+    for (int i = lo + xb; i < hi; i += xstride) {
         const int q = __builtin_ctzll(__ballot(lane < NSEG && incl > (uint32_t)i));
         const int off = i - (__builtin_amdgcn_readlane((int)incl, q) - __builtin_amdgcn_readlane((int)cq, q));
         const int t = __builtin_amdgcn_readfirstlane(list[(int64_t)q * A.ntiles + off]);
@@ -1073,8 +1085,18 @@ void recon_quad_kernel(ReconArgs A, int round) {
             }
         }
     }
+    // Blocks go to the XCDs in turn (block b to XCD b mod 8).  For one image
+    // XCD x takes the x-th eighth of the work items, so each XCD works on one
+    // region of the image and neighbouring tiles meet in its L2.  A batch
+    // keeps the plain stride: its tiles differ in cost, and contiguous chunks
+    // would pile the costly ones onto one XCD.
+    const int nx = A.nimg > 1 ? 1 : (gridDim.x < 8 ? (int)gridDim.x : 8);
+    const int xcd = (int)blockIdx.x % nx, xb = (int)blockIdx.x / nx;
+    const int xstride = ((int)gridDim.x - xcd + nx - 1) / nx;
     if (FIRST && !A.first_list) {
-        for (int i = blockIdx.x; i < A.ntiles; i += gridDim.x) {
+        const int chunk = (A.ntiles + nx - 1) / nx;
+        const int lo = min(A.ntiles, xcd * chunk), hi = min(A.ntiles, lo + chunk);
+        for (int i = lo + xb; i < hi; i += xstride) {
             ImgDesc D;
             int lt;
             tile_of(A, i, D, lt);
@@ -1087,7 +1109,9 @@ void recon_quad_kernel(ReconArgs A, int round) {
     const uint32_t incl = wave_incl_scan(cq);
     const int n = __builtin_amdgcn_readlane((int)incl, NSEG - 1);
     const int32_t* list = (round & 1) ? A.list1 : A.list0;
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int chunk = (n + nx - 1) / nx;
+    const int lo = min(n, xcd * chunk), hi = min(n, lo + chunk);
+    for (int i = lo + xb; i < hi; i += xstride) {
         const int q = __builtin_ctzll(__ballot(lane < NSEG && incl > (uint32_t)i));
         const int off = i - (__builtin_amdgcn_readlane((int)incl, q) - __builtin_amdgcn_readlane((int)cq, q));
         const int t = __builtin_amdgcn_readfirstlane(list[(int64_t)q * A.ntiles + off]);
