@@ -37,6 +37,15 @@
 namespace nsx {
 namespace {
 
+// One tile per block, blocks dispatched to the 8 XCDs in turn (block b on XCD
+// b mod 8).  Renumbered XCD-major, the blocks of one XCD take consecutive tiles,
+// so each XCD works on one region of the image and neighbouring tiles, whose
+// rows share cache lines, meet in its L2.  A bijection of [0, G).
+__device__ __forceinline__ int xcd_tile(int b, int G) {
+    const int x = b & 7;
+    return x * (G >> 3) + min(x, G & 7) + (b >> 3);
+}
+
 constexpr int CW = 64;               // tile columns
 constexpr int CH = 32;               // tile rows
 constexpr int CN = CW * CH;          // pixels per tile (and table slots per tile)
@@ -270,7 +279,8 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
     const int tid = threadIdx.x;
     CclImg D;
     int tx, ty;
-    tile_of(A, blockIdx.x, D, tx, ty);
+    const int tb = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    tile_of(A, tb, D, tx, ty);
     const int x0 = tx * CW, y0 = ty * CH;
     for (int i = tid; i < CN; i += CTHREADS) cnt[i] = 0;
     if (tid == 0) nroots = 0;
@@ -306,7 +316,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
         }
     }
     __syncthreads();
-    uint2* slots = A.entries + (int64_t)blockIdx.x * CN;
+    uint2* slots = A.entries + (int64_t)tb * CN;
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int ly = band_row(w, j);
@@ -325,7 +335,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
         }
     }
     __syncthreads();
-    if (tid == 0) A.tcount[blockIdx.x] = (int32_t)nroots;
+    if (tid == 0) A.tcount[tb] = (int32_t)nroots;
 }
 
 // Global index of the tile-local root lr of the tile holding pixel (qy, qx).
@@ -556,7 +566,8 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
     const int tid = threadIdx.x;
     CclImg D;
     int tx, ty;
-    tile_of(A, blockIdx.x, D, tx, ty);
+    const int tb = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    tile_of(A, tb, D, tx, ty);
     const int x0 = tx * CW, y0 = ty * CH;
     const int lx = tid & 63, w = tid >> 6;
     const int gx = x0 + lx;
