@@ -1264,7 +1264,7 @@ __global__ __launch_bounds__(256) void lx_tile_kernel(const char* work, int64_t 
     __shared__ float sv[LX_H * LX_H];
     __shared__ uint8_t sf[LX_H * LX_H];
     __shared__ int active;
-    const int tile = blockIdx.x;
+    const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int tx = tile % tiles_x, ty = tile / tiles_x;
     if (threadIdx.x == 0) {
         int a = 0;

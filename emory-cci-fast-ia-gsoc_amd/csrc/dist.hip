@@ -36,6 +36,13 @@ constexpr int DT = 64;
 constexpr int DH = DT + 2;
 constexpr uint32_t DT_FAR = 0xFFFFFFFFu;
 
+// One tile per block, blocks dispatched to the 8 XCDs in turn; renumbered
+// XCD-major, each XCD takes consecutive tiles (one region of the image).
+__device__ __forceinline__ int xcd_tile(int b, int G) {
+    const int x = b & 7;
+    return x * (G >> 3) + min(x, G & 7) + (b >> 3);
+}
+
 __device__ __forceinline__ uint64_t dt_d2(int x, int y, uint32_t lab, int far_y) {
     const int lx = lab == DT_FAR ? 0 : (int)(lab & 0xFFFFu);
     const int ly = lab == DT_FAR ? far_y : (int)(lab >> 16);
@@ -68,7 +75,7 @@ __global__ __launch_bounds__(256) void dt_tile_kernel(const uint32_t* src, uint3
                                                       uint8_t* next, int32_t* changed) {
     __shared__ uint32_t s[2][DH * DH];
     __shared__ int active;
-    const int tile = blockIdx.x;
+    const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int tx = tile % tiles_x, ty = tile / tiles_x;
     if (threadIdx.x == 0) {
         int a = 0;
