@@ -68,22 +68,10 @@ def parse():
 
 
 def batch_tiles(nh, n_u8: int, n_f32: int):
-    """Uneven tiles: sides uniform in 512..4096 (multiples of 16), disks in
-    proportion to area (200 per 512^2), seed 0x5EED0002 + index; float tiles are
-    the same construction divided by 255."""
-    import numpy as np
-    rng = np.random.default_rng(0x5EED0002)
-    out = []
-    for k in range(n_u8 + n_f32):
-        h = 512 + 16 * int(rng.integers(0, 225))
-        w = 512 + 16 * int(rng.integers(0, 225))
-        mask = nh.gen_disks_u8(h, w, round(200 * h * w / 512 ** 2), 0x5EED0002 + k)
-        marker = nh.gen_marker_u8(mask, 30)
-        if k >= n_u8:
-            out.append((nh.scale_f32(marker), nh.scale_f32(mask), float(np.float32(128) / np.float32(255))))
-        else:
-            out.append((marker, mask, 128.0))
-    return out
+    """The batch workload's tiles (nscale_hip.workloads.batch_tiles, seed
+    0x5EED0002 + index)."""
+    from nscale_hip.workloads import batch_tiles as gen
+    return gen(n_u8, n_f32)
 
 
 def bench_batch(args):

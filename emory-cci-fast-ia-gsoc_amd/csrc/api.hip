@@ -87,20 +87,24 @@ hipStream_t pick_stream(nsx_ctx* ctx, void* stream) {
     return stream ? (hipStream_t)stream : ctx->stream;
 }
 
-nsx_status check_img(nsx_ctx* ctx, const nsx_img* im, const char* name) {
-    if (!im) return ctx->fail(NSX_ERR_INVALID, std::string(name) + " is NULL");
-    if (im->rows < 0 || im->cols < 0) return ctx->fail(NSX_ERR_INVALID, std::string(name) + ": negative size");
-    if (im->dtype != NSX_U8 && im->dtype != NSX_I32 && im->dtype != NSX_F32)
-        return ctx->fail(NSX_ERR_INVALID, std::string(name) + ": unknown dtype");
-    if (im->mem != NSX_MEM_DEVICE && im->mem != NSX_MEM_HOST)
-        return ctx->fail(NSX_ERR_INVALID, std::string(name) + ": unknown memory kind");
+// What is wrong with an image descriptor (nullptr: nothing).  Host-side only:
+// no device call, so it also runs where there is no GPU.
+const char* img_problem(const nsx_img* im) {
+    if (!im) return "is NULL";
+    if (im->rows < 0 || im->cols < 0) return "negative size";
+    if (im->dtype != NSX_U8 && im->dtype != NSX_I32 && im->dtype != NSX_F32) return "unknown dtype";
+    if (im->mem != NSX_MEM_DEVICE && im->mem != NSX_MEM_HOST) return "unknown memory kind";
     if (im->rows > 0 && im->cols > 0) {
-        if (!im->data) return ctx->fail(NSX_ERR_INVALID, std::string(name) + ": data is NULL");
-        if (im->pitch_bytes < (int64_t)im->cols * (int64_t)dtype_size(im->dtype))
-            return ctx->fail(NSX_ERR_INVALID, std::string(name) + ": pitch smaller than a row");
-        if ((int64_t)im->rows * im->cols >= (int64_t)INT32_MAX)
-            return ctx->fail(NSX_ERR_INVALID, std::string(name) + ": more than 2^31-1 pixels");
+        if (!im->data) return "data is NULL";
+        if (im->pitch_bytes < (int64_t)im->cols * (int64_t)dtype_size(im->dtype)) return "pitch smaller than a row";
+        if ((int64_t)im->rows * im->cols >= (int64_t)INT32_MAX) return "more than 2^31-1 pixels";
     }
+    return nullptr;
+}
+
+nsx_status check_img(nsx_ctx* ctx, const nsx_img* im, const char* name) {
+    const char* why = img_problem(im);
+    if (why) return ctx->fail(NSX_ERR_INVALID, std::string(name) + (im ? ": " : " ") + why);
     return NSX_OK;
 }
 
@@ -178,6 +182,13 @@ nsx_status seeded_setup(nsx_ctx* ctx, const nsx_img* img, const nsx_img* seeds, 
     return NSX_OK;
 }
 
+// The constant every int32 imfillHoles result takes, except the gray fill
+// under saturating negation: INT_MIN under wrapping negation, INT_MAX (binary
+// only) under saturation.
+int32_t i32_fill_const(const nsx_ctx* ctx) {
+    return ctx->i32_invert == NSX_I32_INVERT_WRAP ? INT_MIN : INT_MAX;
+}
+
 nsx_status scratch_like(nsx_ctx* ctx, int slot, const View& like, View* v) {
     void* p;
     const size_t pitch = like.cols > 0 ? (size_t)like.cols * dtype_size(like.dtype) : 1;
@@ -191,6 +202,12 @@ nsx_status scratch_like(nsx_ctx* ctx, int slot, const View& like, View* v) {
 extern "C" {
 
 const char* nsx_version(void) { return kVersion; }
+
+nsx_status nsx_image_check(const nsx_img* img, const char** why) {
+    const char* w = img_problem(img);
+    if (why) *why = w ? w : "";
+    return w ? NSX_ERR_INVALID : NSX_OK;
+}
 
 nsx_status nsx_ctx_create(nsx_ctx** out, int device, size_t hbm_budget_bytes) {
     if (!out) return NSX_ERR_INVALID;
@@ -269,6 +286,26 @@ nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value) {
     }
     if (param == NSX_TUNE_LEVELSET_LEVELS) {
         *value = ctx->levelset_levels;
+        return NSX_OK;
+    }
+    return NSX_ERR_INVALID;
+}
+
+nsx_status nsx_ctx_set_option(nsx_ctx* ctx, int option, int value) {
+    if (!ctx) return NSX_ERR_INVALID;
+    if (option == NSX_OPT_I32_INVERT) {
+        if (value != NSX_I32_INVERT_WRAP && value != NSX_I32_INVERT_SATURATE)
+            return ctx->fail(NSX_ERR_INVALID, "NSX_OPT_I32_INVERT takes NSX_I32_INVERT_WRAP or NSX_I32_INVERT_SATURATE");
+        ctx->i32_invert = value;
+        return NSX_OK;
+    }
+    return ctx->fail(NSX_ERR_INVALID, "unknown option");
+}
+
+nsx_status nsx_ctx_get_option(const nsx_ctx* ctx, int option, int* value) {
+    if (!ctx || !value) return NSX_ERR_INVALID;
+    if (option == NSX_OPT_I32_INVERT) {
+        *value = ctx->i32_invert;
         return NSX_OK;
     }
     return NSX_ERR_INVALID;
@@ -360,10 +397,13 @@ nsx_status nsx_fill_holes(nsx_ctx* ctx, const nsx_img* in, nsx_img* out, int bin
     View vi, vo;
     NSX_TRY(to_device(ctx, in, WS_STAGE_B, true, s, &vi));
     NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
-    if (binary && in->dtype == NSX_I32) {
-        // int marker interior is min<int> != 0, so imreconstructBinary keeps it
-        // everywhere and the inverted result is max<int> (MorphologicOperations.cpp:1889-1925).
-        NSX_TRY(fill_const_i32(ctx, vo, INT_MAX, s));
+    if (in->dtype == NSX_I32 && (binary || ctx->i32_invert == NSX_I32_INVERT_WRAP)) {
+        // int32 closed forms (MorphologicOperations.cpp:1889-1925): with
+        // wrapping negation the inverted mask's frame is INT_MIN, the
+        // reconstruction stays INT_MIN everywhere and so does its inverse.
+        // With saturation, binary: the marker's interior min<int> is nonzero,
+        // imreconstructBinary keeps it everywhere, and its inverse is max<int>.
+        NSX_TRY(fill_const_i32(ctx, vo, i32_fill_const(ctx), s));
     } else {
         NSX_TRY(recon_run(ctx, binary ? RM_FILLBIN : RM_FILL, nullptr, vi, vo, conn == 8 ? 8 : 4, s));
     }
@@ -688,6 +728,20 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
     // statistics, int32 [32..39] the status gathered for the host.
     int32_t* dw = (int32_t*)ctx->d_stats;
     const int rconn = recon_conn == 8 ? 8 : 4, fconn = fill_conn == 8 ? 8 : 4;
+    if (mask->dtype == NSX_I32 && ctx->i32_invert == NSX_I32_INVERT_WRAP) {
+        // int32 fill under wrapping negation is the constant INT_MIN (see
+        // nsx_fill_holes); the labelling still runs on it
+        NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vr, rconn, s));
+        NSX_TRY(fill_const_i32(ctx, vf, INT_MIN, s));
+        NSX_TRY(ccl_run(ctx, vf, vl, p, dw + 12, s));
+        int32_t n = 0;
+        NSX_TRY(read_count(ctx, dw + 12, &n, s));
+        if (recon_out && recon_out->mem == NSX_MEM_HOST) NSX_TRY(from_device(ctx, vr, recon_out, s));
+        if (fill_out && fill_out->mem == NSX_MEM_HOST) NSX_TRY(from_device(ctx, vf, fill_out, s));
+        NSX_TRY(from_device(ctx, vl, labels_out, s));
+        if (ncomp) *ncomp = n;
+        return NSX_OK;
+    }
     std::vector<uint8_t> dr(recon_desc_bytes()), df(recon_desc_bytes());
     recon_fill_desc(dr.data(), &vm, vk, vr, 0);
     recon_fill_desc(df.data(), nullptr, vr, vf, 0);

@@ -72,6 +72,9 @@ struct nsx_ctx {
     int visit_sweeps = 4;
     int levelset_levels = 3;
 
+    // Options (nsx_ctx_set_option).
+    int i32_invert = NSX_I32_INVERT_WRAP;
+
     // Propagation rounds the last reconstruction of a given size/mode needed.
     std::unordered_map<uint64_t, int> round_hint;
 

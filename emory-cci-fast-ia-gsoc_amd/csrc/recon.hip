@@ -71,8 +71,10 @@ template <> struct Key<float> {
     }
 };
 
-// PixelOperations::invert<T> (PixelOperations.cpp:17-39): 255-x, saturating
-// two's-complement negation, -x.
+// PixelOperations::invert<T> (PixelOperations.cpp:17-39): 255-x, -x, and for
+// int32 saturating negation.  The int32 form is only reached by gray
+// imfillHoles under NSX_I32_INVERT_SATURATE; under the default wrapping
+// negation the int32 fill is a constant written by the caller.
 __device__ inline uint8_t inv(uint8_t v) { return (uint8_t)(255 - v); }
 __device__ inline int32_t inv(int32_t v) { return v == INT_MIN ? INT_MAX : -v; }
 __device__ inline float inv(float v) { return -v; }

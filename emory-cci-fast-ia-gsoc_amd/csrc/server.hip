@@ -379,10 +379,15 @@ nsx_status nsx_server_stats(nsx_server* S, int64_t* stats) {
     return NSX_OK;
 }
 
+// The message is copied under the lock into the calling thread's own buffer:
+// workers may replace last_error at any time, so a pointer into it could
+// dangle.  The returned pointer stays valid until this thread's next call.
 const char* nsx_server_last_error(nsx_server* S) {
     if (!S) return "null server";
+    thread_local std::string copy;
     std::lock_guard<std::mutex> lk(S->mu);
-    return S->last_error.c_str();
+    copy = S->last_error;
+    return copy.c_str();
 }
 
 void nsx_server_destroy(nsx_server* S) {

@@ -25,9 +25,12 @@ import numpy as np
 
 from . import _lib
 from ._lib import NsxImg
+from ._lib import NSX_OPT_I32_INVERT as OPT_I32_INVERT
+from ._lib import NSX_I32_INVERT_WRAP as I32_INVERT_WRAP
+from ._lib import NSX_I32_INVERT_SATURATE as I32_INVERT_SATURATE
 
 __all__ = [
-    "NsxError", "Context", "default_context",
+    "NsxError", "Context", "default_context", "OPT_I32_INVERT", "I32_INVERT_WRAP", "I32_INVERT_SATURATE",
     "imreconstruct", "imreconstructBinary", "imfillHoles", "fillHoles", "imfill", "bwselect", "imhmin", "bounding_box", "localMaxima", "localMinima", "distanceTransform",
     "bwlabel2", "bwlabel", "bwareaopen2", "relabel", "threshold_gt", "segment_chain",
     "gen_disks_u8", "gen_marker_u8", "gen_maze_u8", "scale_i32", "scale_f32",
@@ -152,6 +155,16 @@ class Context:
     def get_tuning(self, param: int) -> int:
         v = C.c_int(0)
         self.check(self.lib.nsx_ctx_get_tuning(self.h, param, C.byref(v)))
+        return int(v.value)
+
+    def set_option(self, option: int, value: int) -> None:
+        """Semantic options (these change results), e.g.
+        set_option(OPT_I32_INVERT, I32_INVERT_SATURATE)."""
+        self.check(self.lib.nsx_ctx_set_option(self.h, option, value))
+
+    def get_option(self, option: int) -> int:
+        v = C.c_int(0)
+        self.check(self.lib.nsx_ctx_get_option(self.h, option, C.byref(v)))
         return int(v.value)
 
     def recon_stats(self, stage: int = 0) -> dict:

@@ -18,6 +18,9 @@ NSX_ERR_NOT_CONVERGED = 6
 NSX_U8, NSX_I32, NSX_F32 = 0, 1, 2
 NSX_TUNE_VISIT_SWEEPS = 0
 NSX_TUNE_LEVELSET_LEVELS = 1
+NSX_OPT_I32_INVERT = 0
+NSX_I32_INVERT_WRAP = 0
+NSX_I32_INVERT_SATURATE = 1
 NSX_MEM_DEVICE, NSX_MEM_HOST = 0, 1
 
 
@@ -49,6 +52,9 @@ SIGNATURES = {
     "nsx_ctx_read_timing": (C.c_int, [_CTX, P(C.c_double), P(C.c_int64), C.c_int]),
     "nsx_ctx_set_tuning": (C.c_int, [_CTX, C.c_int, C.c_int]),
     "nsx_ctx_get_tuning": (C.c_int, [_CTX, C.c_int, P(C.c_int)]),
+    "nsx_image_check": (C.c_int, [P(NsxImg), P(C.c_char_p)]),
+    "nsx_ctx_set_option": (C.c_int, [_CTX, C.c_int, C.c_int]),
+    "nsx_ctx_get_option": (C.c_int, [_CTX, C.c_int, P(C.c_int)]),
     "nsx_imreconstruct": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_imreconstruct_binary": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_fill_holes": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_int, C.c_void_p]),

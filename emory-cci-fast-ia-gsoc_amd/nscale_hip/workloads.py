@@ -9,9 +9,37 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import gen_disks_u8, gen_marker_u8, scale_i32
+from . import gen_disks_u8, gen_marker_u8, scale_f32, scale_i32
 
 SERVE_SEED = 0x5EED0004
+BATCH_SEED = 0x5EED0002
+CHAIN_SEED = 0x5EED0001
+MAZE_SEED = 0x5EED0003
+
+
+def chain_images(n: int = 4096, ndisks: int = 3000, seed: int = CHAIN_SEED):
+    """The single-tile chain workload: an n x n disks image and its marker."""
+    mask = gen_disks_u8(n, n, ndisks, seed)
+    return gen_marker_u8(mask, 30), mask
+
+
+def batch_tiles(n_u8: int = 256, n_f32: int = 32, seed: int = BATCH_SEED):
+    """Uneven batch tiles: sides uniform in 512..4096 (multiples of 16), disks
+    in proportion to area (200 per 512^2), generator seed `seed` + index; the
+    float tiles are the same construction divided by 255 (threshold 128/255).
+    Returns [(marker, mask, threshold)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_u8 + n_f32):
+        h = 512 + 16 * int(rng.integers(0, 225))
+        w = 512 + 16 * int(rng.integers(0, 225))
+        mask = gen_disks_u8(h, w, round(200 * h * w / 512 ** 2), seed + k)
+        marker = gen_marker_u8(mask, 30)
+        if k >= n_u8:
+            out.append((scale_f32(marker), scale_f32(mask), float(np.float32(128) / np.float32(255))))
+        else:
+            out.append((marker, mask, 128.0))
+    return out
 
 
 def request_images(side: int, kind: str, seed: int):

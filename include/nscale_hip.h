@@ -98,6 +98,30 @@ enum { NSX_TUNE_VISIT_SWEEPS = 0, NSX_TUNE_LEVELSET_LEVELS = 1, NSX_TUNE_NPARAMS
 nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value);
 nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value);
 
+/* Semantic options of a context (these change results).
+ * NSX_OPT_I32_INVERT: how PixelOperations::invert<int> (PixelOperations.cpp:
+ * 17-39: bitwise_not(x) + 1 through OpenCV's CV_32S Mat + Scalar) treats
+ * INT_MIN, the frame value imfillHoles<int> pads with
+ * (MorphologicOperations.cpp:1889-1903).
+ *   NSX_I32_INVERT_WRAP (default): two's-complement wrap, INT_MIN -> INT_MIN,
+ *     as the reference's own GPU kernel computes it (~x + 1 in int,
+ *     src/segment/cuda/pixel-ops.cu:39-48, instantiated for imfillholes,
+ *     PixelOperationsGPU.cpp:301) and as OpenCV documents CV_32S arithmetic
+ *     (no saturation).  The inverted mask's frame is then INT_MIN, so the
+ *     reconstruction is INT_MIN everywhere and every int32 imfillHoles result,
+ *     gray or binary, is the constant INT_MIN.
+ *   NSX_I32_INVERT_SATURATE: INT_MIN -> INT_MAX.  Gray int32 imfillHoles is
+ *     then the textbook fill (MATLAB imfill 'holes'); binary int32
+ *     imfillHoles is the constant INT_MAX (the marker's interior min<int> is
+ *     nonzero and is kept).
+ * Applies to nsx_fill_holes and nsx_segment_chain on int32 images.  No
+ * reference counterpart (OpenCV 2.x is not available to settle the
+ * reference's CPU behaviour, so the default follows its GPU code). */
+enum { NSX_OPT_I32_INVERT = 0 };
+enum { NSX_I32_INVERT_WRAP = 0, NSX_I32_INVERT_SATURATE = 1 };
+nsx_status nsx_ctx_set_option(nsx_ctx* ctx, int option, int value);
+nsx_status nsx_ctx_get_option(const nsx_ctx* ctx, int option, int* value);
+
 /* ---- morphology ----------------------------------------------------------- */
 /* Grayscale reconstruction by dilation of `marker` under `mask`; conn 8 gives
  * 8-connectivity, anything else 4.  u8, i32, f32.
@@ -246,9 +270,17 @@ nsx_status nsx_batch_add(nsx_batch* batch, const nsx_img* marker, const nsx_img*
 /* Builds the work tables and (use_graph=1) captures the graph.  Call once after
  * the last add. */
 nsx_status nsx_batch_finalize(nsx_batch* batch, int use_graph);
-/* Runs the whole batch on the stream (graph replay when captured). */
+/* Runs the whole batch on the stream (graph replay when captured).  The
+ * reconstruction runs the number of propagation rounds each tile group needed
+ * when it was last completed; recon_out and labels_out are final only after
+ * nsx_batch_counts, which checks every group's convergence and, where tile
+ * contents changed since and need more rounds, finishes that group eagerly,
+ * relabels it and re-captures the graph.  A caller that rewrites tile
+ * contents between runs must therefore call nsx_batch_counts before reading
+ * the outputs (with unchanged contents every run is already final). */
 nsx_status nsx_batch_run(nsx_batch* batch, void* stream);
-/* Per-tile component counts after a run (array of n tiles). Synchronises. */
+/* Completes the last run (see nsx_batch_run) and returns the per-tile
+ * component counts (array of n tiles).  Synchronises. */
 nsx_status nsx_batch_counts(nsx_batch* batch, int32_t* counts, int n);
 /* Tunes a batch finalized with a graph: for each group of tiles with the same
  * (dtype, conn, ccl_conn, thresh), in the order groups first appear, tries
@@ -291,7 +323,8 @@ nsx_status nsx_server_wait(nsx_server* server, uint64_t ticket, int32_t* ncomp, 
 /* stats (8 entries): completed, in flight, queued, peak in flight, arena bytes,
  * peak arena bytes in use, workspace bytes held by the slot contexts, slots. */
 nsx_status nsx_server_stats(nsx_server* server, int64_t* stats);
-/* Message of the last failed server call or request. */
+/* Message of the last failed server call or request: a copy private to the
+ * calling thread, valid until that thread calls this function again. */
 const char* nsx_server_last_error(nsx_server* server);
 /* Finishes every queued request, then releases the workers and the arena. */
 void nsx_server_destroy(nsx_server* server);
@@ -303,6 +336,13 @@ void nsx_gen_marker_u8(const uint8_t* mask, uint8_t* marker, long n, int sub);
 void nsx_gen_scale_i32(const uint8_t* in, int32_t* out, long n, int scale);
 void nsx_gen_scale_f32(const uint8_t* in, float* out, long n);
 int nsx_gen_maze_u8(uint8_t* mask, uint8_t* marker, int rows, int cols, uint64_t seed);
+
+/* The descriptor checks every entry point applies to its images (NULL data,
+ * unknown dtype or memory kind, pitch below a row, 2^31-1 pixels or more),
+ * without a context or a device: NSX_OK, or NSX_ERR_INVALID with *why (may be
+ * NULL) set to a static message.  No reference counterpart (the reference's
+ * CV_Assert sites check channels and types inside each call). */
+nsx_status nsx_image_check(const nsx_img* img, const char** why);
 
 /* Library version string. */
 const char* nsx_version(void);

@@ -38,8 +38,12 @@
 //   or_bwareaopen           unmodified into oracle/_ref (exact), scipy
 //                           ndimage.label partitions, and the canonical-numbering
 //                           rule.
-//   i32 invert at INT_MIN   parity unpinned (OpenCV 2.x unavailable): saturating
-//                           negation chosen, see or_invert_i32.
+//   i32 invert at INT_MIN   parity unpinned (OpenCV 2.x unavailable): both
+//                           readings are restated (wrap, the default, and
+//                           saturate); each one's int32 imfillHoles closed form
+//                           pins it (constant INT_MIN under wrap; the textbook
+//                           fill, or constant INT_MAX for binary, under
+//                           saturation).  See invert_i32.
 #include <cstdint>
 #include <cstring>
 #include <climits>
@@ -303,12 +307,20 @@ void imreconstructBinary(const T* seeds, const T* image, T* out, int rows, int c
 
 // PixelOperations::invert<T> (PixelOperations.cpp:17-39).
 //   unsigned: max - x ;  float: -x ;  signed int: bitwise_not(x) + 1.
-// The signed case goes through OpenCV Mat+Scalar on CV_32S, whose overflow
-// behaviour at x == INT_MIN cannot be checked here (OpenCV is absent).  We take
-// the saturate_cast contract: ~INT_MIN + 1 saturates to INT_MAX.  All other
-// inputs are exact two's-complement negation.  (Parity unpinned at INT_MIN.)
+// The signed case goes through OpenCV Mat+Scalar on CV_32S, whose behaviour at
+// x == INT_MIN cannot be run here (OpenCV is absent).  The reference's own GPU
+// kernel computes ~x + 1 in int and wraps (src/segment/cuda/pixel-ops.cu:
+// 39-48, instantiated "for imfillholes" at PixelOperationsGPU.cpp:301), and
+// OpenCV documents no saturation for CV_32S results, so wrapping is the
+// default; saturation (INT_MIN -> INT_MAX) is the alternative reading.  All
+// other inputs are exact two's-complement negation either way.
+thread_local bool g_i32_saturate = false;   // set per call by the exported functions
+inline int32_t invert_i32(int32_t v, bool saturate) {
+    if (v == INT_MIN) return saturate ? INT_MAX : INT_MIN;
+    return -v;
+}
 inline uint8_t invert_one(uint8_t v) { return (uint8_t)(255 - v); }
-inline int32_t invert_one(int32_t v) { return v == INT_MIN ? INT_MAX : -v; }
+inline int32_t invert_one(int32_t v) { return invert_i32(v, g_i32_saturate); }
 // OpenCV's unary minus on a Mat is a scale by -1 plus a shift of 0, which
 // turns -0.0 into +0.0; 0.0f - v gives the same bits (OpenCV is absent here, so
 // the sign of a zero result is unpinned; the value is exact).
@@ -566,8 +578,10 @@ int or_fill_holes_u8(const uint8_t* in, uint8_t* out, int rows, int cols, int bi
     imfillHoles<uint8_t>(in, out, rows, cols, binary != 0, conn);
     return 0;
 }
-int or_fill_holes_i32(const int32_t* in, int32_t* out, int rows, int cols, int binary, int conn) {
+int or_fill_holes_i32(const int32_t* in, int32_t* out, int rows, int cols, int binary, int conn, int saturate) {
+    g_i32_saturate = saturate != 0;
     imfillHoles<int32_t>(in, out, rows, cols, binary != 0, conn);
+    g_i32_saturate = false;
     return 0;
 }
 int or_imfill_u8(const uint8_t* in, const uint8_t* seeds, uint8_t* out, int rows, int cols, int binary, int conn) {
@@ -693,8 +707,8 @@ int or_imhmin_f32(const float* in, float h, float* out, int rows, int cols, int 
     imhmin<float>(in, h, out, rows, cols, conn);
     return 0;
 }
-void or_invert_i32(const int32_t* in, int32_t* out, long n) {
-    for (long i = 0; i < n; ++i) out[i] = invert_one(in[i]);
+void or_invert_i32(const int32_t* in, int32_t* out, long n, int saturate) {
+    for (long i = 0; i < n; ++i) out[i] = invert_i32(in[i], saturate != 0);
 }
 
 // bwlabel2 (MorphologicOperations.cpp:2012-2035).  relab=0 returns root-index

@@ -65,11 +65,16 @@ def imreconstruct_binary(marker, mask, conn=4):
     return out
 
 
-def fill_holes(img, binary=False, conn=4):
+def fill_holes(img, binary=False, conn=4, i32_saturate=False):
+    """imfillHoles<T>.  For int32, invert<int> wraps at INT_MIN unless
+    i32_saturate (the two readings of PixelOperations.cpp:24-27)."""
     img = np.ascontiguousarray(img)
     out = np.empty_like(img)
-    fn = {np.uint8: "or_fill_holes_u8", np.int32: "or_fill_holes_i32"}[img.dtype.type]
-    getattr(lib(), fn)(_p(img), _p(out), img.shape[0], img.shape[1], int(bool(binary)), conn)
+    if img.dtype == np.int32:
+        lib().or_fill_holes_i32(_p(img), _p(out), img.shape[0], img.shape[1], int(bool(binary)), conn,
+                                int(bool(i32_saturate)))
+    else:
+        lib().or_fill_holes_u8(_p(img), _p(out), img.shape[0], img.shape[1], int(bool(binary)), conn)
     return out
 
 

@@ -121,3 +121,47 @@ def test_batch_autotune_keeps_results(nh, orc, gpu):
     with pytest.raises(nh.NsxError):
         eager.autotune(reps=1)
     eager.close()
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_contents_changed_after_finalize(nh, orc, gpu, use_graph):
+    """Tile contents rewritten in place after finalize so that a replay needs
+    more propagation rounds than were captured (a maze where disks were):
+    nsx_batch_counts finishes the short group, and recon, labels and counts
+    match the oracle, on that replay and on the next (re-captured) one."""
+    import torch
+    tiles = make_tiles(nh, 4, 4, dtypes=("u8",))
+    batch = nh.Batch()
+    ins, outs = [], []
+    for marker, mask, thr in tiles:
+        dm = torch.from_numpy(marker).cuda()
+        dk = torch.from_numpy(mask).cuda()
+        r = torch.empty_like(dk)
+        lab = torch.empty(dk.shape, dtype=torch.int32, device="cuda")
+        batch.add(dm, dk, r, lab, 8, 128.0, 8)
+        ins.append((dm, dk))
+        outs.append((r, lab))
+    batch.finalize(use_graph=use_graph)
+    batch.counts()
+    new = []
+    for i, (dm, dk) in enumerate(ins):
+        if i % 2 == 0:
+            mask, marker = nh.gen_maze_u8(dk.shape[0], dk.shape[1], 0x5EED0302 + i)
+        else:
+            marker, mask = tiles[i][0], tiles[i][1]
+        dm.copy_(torch.from_numpy(marker))
+        dk.copy_(torch.from_numpy(mask))
+        new.append(expected(orc, marker, mask, 128.0))
+    torch.cuda.synchronize()
+    for replay in range(2):
+        for r, lab in outs:
+            r.fill_(7)
+            lab.fill_(-5)
+        torch.cuda.synchronize()
+        batch.run()
+        counts = batch.counts()
+        for i, ((r, lab), (wr, wlab, wn)) in enumerate(zip(outs, new)):
+            assert np.array_equal(r.cpu().numpy(), wr), f"replay {replay} tile {i} recon"
+            assert np.array_equal(lab.cpu().numpy(), wlab), f"replay {replay} tile {i} labels"
+            assert counts[i] == wn, f"replay {replay} tile {i} count"
+    batch.close()

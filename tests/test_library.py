@@ -67,3 +67,29 @@ def test_ctx_without_gpu_fails_cleanly(nh):
     h = C.c_void_p()
     st = nh._lib.load().nsx_ctx_create(C.byref(h), 0, 0)
     assert st != 0
+
+
+def test_image_check_host_only(nh):
+    """nsx_image_check applies the entry points' descriptor checks without a
+    context or a device (runs here, without a GPU)."""
+    import ctypes as C
+    L = nh._lib
+    lib = L.load()
+    buf = np.zeros(64, np.uint8)
+    why = C.c_char_p()
+
+    def check(rows, cols, pitch, dtype=L.NSX_U8, mem=L.NSX_MEM_DEVICE, data=buf.ctypes.data):
+        return lib.nsx_image_check(C.byref(L.NsxImg(data, pitch, rows, cols, dtype, mem)), C.byref(why))
+
+    assert check(8, 8, 8) == L.NSX_OK and why.value == b""
+    assert check(0, 5, 5, data=None) == L.NSX_OK            # empty images need no data
+    assert check(50000, 50000, 50000) == L.NSX_ERR_INVALID and b"2^31" in why.value
+    assert check(46341, 46341, 46341, mem=L.NSX_MEM_HOST) == L.NSX_ERR_INVALID       # 2^31 + 1 pixels
+    assert check(46340, 46340, 46340) == L.NSX_OK                                      # 2147395600 pixels
+    assert check(4, 8, 7) == L.NSX_ERR_INVALID and b"pitch" in why.value
+    assert check(4, 8, 31, dtype=L.NSX_I32) == L.NSX_ERR_INVALID
+    assert check(4, 8, 8, dtype=9) == L.NSX_ERR_INVALID and b"dtype" in why.value
+    assert check(4, 8, 8, mem=5) == L.NSX_ERR_INVALID and b"memory" in why.value
+    assert check(4, 8, 8, data=None) == L.NSX_ERR_INVALID and b"NULL" in why.value
+    assert check(-1, 8, 8) == L.NSX_ERR_INVALID
+    assert lib.nsx_image_check(None, None) == L.NSX_ERR_INVALID

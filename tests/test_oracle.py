@@ -142,20 +142,67 @@ def test_fill_holes_binary_treats_only_255_as_wall(orc):
     assert (got[1:6, 1:6] == 255).all() and got[0].sum() == 0
 
 
+def _fill_i32_fixpoint(img, conn, saturate):
+    """imfillHoles<int> from its definition: pad with INT_MIN, negate (wrapping
+    or saturating at INT_MIN), reconstruct the frame marker by brute-force
+    geodesic dilation, negate back, crop."""
+    lo, hi = np.iinfo(np.int32).min, np.iinfo(np.int32).max
+
+    def neg(a):
+        out = np.negative(a)                 # numpy int32 negation wraps
+        if saturate:
+            out[a == lo] = hi
+        return out
+
+    mk = neg(np.pad(img, 1, constant_values=lo))
+    mr = np.full_like(mk, lo)
+    mr[0, :] = mr[-1, :] = mr[:, 0] = mr[:, -1] = hi
+    r = brute.recon_fixpoint(mr, mk, conn, frame=0)
+    return neg(r)[1:-1, 1:-1]
+
+
 def test_fill_holes_i32(orc):
+    """Both readings of invert<int> at INT_MIN, each against the brute-force
+    fixed point built with that negation and against its closed form:
+    wrapping gives the constant INT_MIN (gray and binary), saturation the
+    textbook fill (gray) and the constant INT_MAX (binary)."""
+    lo, hi = np.iinfo(np.int32).min, np.iinfo(np.int32).max
     rng = np.random.default_rng(9)
     img = rng.integers(0, 65536, size=(25, 31)).astype(np.int32)
     for conn in (4, 8):
-        got = orc.fill_holes(img, False, conn)
-        # -R(-I) from the frame, by fixed point
-        inv = -img
-        mk = np.pad(inv, 1, constant_values=np.iinfo(np.int32).max)
-        mr = np.full_like(mk, np.iinfo(np.int32).min)
-        mr[0, :] = mr[-1, :] = mr[:, 0] = mr[:, -1] = np.iinfo(np.int32).max
-        r = brute.recon_fixpoint(mr, mk, conn, frame=0)[1:-1, 1:-1]
-        assert np.array_equal(got, -r)
-        # binary int fill: the int marker interior min<int> is nonzero and kept
-        assert (orc.fill_holes(img, True, conn) == np.iinfo(np.int32).max).all()
+        wrap = orc.fill_holes(img, False, conn)
+        assert np.array_equal(wrap, _fill_i32_fixpoint(img, conn, False))
+        assert (wrap == lo).all()
+        assert (orc.fill_holes(img, True, conn) == lo).all()
+        sat = orc.fill_holes(img, False, conn, i32_saturate=True)
+        assert np.array_equal(sat, _fill_i32_fixpoint(img, conn, True))
+        assert (orc.fill_holes(img, True, conn, i32_saturate=True) == hi).all()
+
+
+def test_fill_holes_i32_tiny_fixture(orc):
+    """A 5x5 ring of 9 around a 0 hole, with a 0 channel to the border on
+    one side in the second image.  Saturation: the enclosed hole is filled to
+    9, the open one is not.  Wrap: INT_MIN everywhere."""
+    lo = np.iinfo(np.int32).min
+    ring = np.zeros((5, 5), np.int32)
+    ring[1:4, 1:4] = 9
+    ring[2, 2] = 0
+    filled = ring.copy()
+    filled[2, 2] = 9
+    opened = ring.copy()
+    opened[2, 3] = 0          # hole joined to the background on the right
+    for conn in (4, 8):
+        assert np.array_equal(orc.fill_holes(ring, False, conn, i32_saturate=True), filled)
+        assert np.array_equal(orc.fill_holes(opened, False, 4, i32_saturate=True), opened)
+        assert (orc.fill_holes(ring, False, conn) == lo).all()
+        assert (orc.fill_holes(opened, True, conn) == lo).all()
+    # wrap and saturate agree everywhere but at INT_MIN
+    a = np.array([lo, lo + 1, -1, 0, 1, np.iinfo(np.int32).max], np.int32)
+    w, t = np.empty_like(a), np.empty_like(a)
+    orc.lib().or_invert_i32(orc._p(a), orc._p(w), a.size, 0)
+    orc.lib().or_invert_i32(orc._p(a), orc._p(t), a.size, 1)
+    assert w.tolist() == [lo, -(lo + 1), 1, 0, -1, -np.iinfo(np.int32).max]
+    assert t.tolist() == [np.iinfo(np.int32).max] + w.tolist()[1:]
 
 
 def test_bwlabel_vs_reference_and_scipy(orc):
@@ -245,19 +292,27 @@ def test_chain_composition_matches_chain_u8(orc):
 
 
 def test_chain_i32_scaling_is_order_isomorphic(orc):
-    """int32 requests are the u8 image scaled by 257 with threshold 32768:
-    reconstruction and fill commute with the scaling, and 257*v > 32768 holds
-    exactly when v > 127, so the labels match the u8 chain thresholded at 127."""
+    """int32 requests are the u8 image scaled by 257 with threshold 32768.
+    Under saturating negation reconstruction and fill commute with the
+    scaling, and 257*v > 32768 holds exactly when v > 127, so the labels
+    match the u8 chain thresholded at 127.  Under wrapping negation (the
+    default) the reconstruction still commutes, the fill is INT_MIN
+    everywhere and no component survives the threshold."""
     import numpy as np
     from chain_expect import expected_chain
     from nscale_hip import gen_disks_u8, gen_marker_u8, scale_i32
     mask = gen_disks_u8(120, 140, 25, 0x5EED0005)
     marker = gen_marker_u8(mask, 30)
     r8, f8, l8, n8 = expected_chain(orc, marker, mask, 127)
-    r32, f32, l32, n32 = expected_chain(orc, scale_i32(marker, 257), scale_i32(mask, 257), 32768)
+    m32, k32 = scale_i32(marker, 257), scale_i32(mask, 257)
+    r32, f32, l32, n32 = expected_chain(orc, m32, k32, 32768, i32_saturate=True)
     assert np.array_equal(r32, r8.astype(np.int32) * 257)
     assert np.array_equal(f32, f8.astype(np.int32) * 257)
     assert np.array_equal(l32, l8) and n32 == n8
+    r32, f32, l32, n32 = expected_chain(orc, m32, k32, 32768)
+    assert np.array_equal(r32, r8.astype(np.int32) * 257)
+    assert (f32 == np.iinfo(np.int32).min).all()
+    assert not l32.any() and n32 == 0
 
 
 @pytest.mark.parametrize("conn", [4, 8])
