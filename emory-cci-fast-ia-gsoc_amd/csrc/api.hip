@@ -235,6 +235,12 @@ nsx_status nsx_ctx_create(nsx_ctx** out, int device, size_t hbm_budget_bytes) {
     }
     hipMemset(c->d_stats, 0, 256);
     c->held = 256;
+    // the allocation-free entry point's scratch array: its device address,
+    // once per device, with the device current
+    if (recon_noalloc_init(device) != NSX_OK) {
+        nsx_ctx_destroy(c);
+        return NSX_ERR_HIP;
+    }
     *out = c;
     return NSX_OK;
 }
@@ -361,6 +367,25 @@ nsx_status nsx_imreconstruct(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
     NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
     NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vo, conn == 8 ? 8 : 4, s));
     return finish_recon(ctx, vo, out, s);
+}
+
+nsx_status nsx_imreconstruct_noalloc(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* mask, nsx_img* out,
+                                     int conn, void* stream) {
+    if (!ctx) return NSX_ERR_INVALID;
+    NSX_TRY(check_img(ctx, marker, "marker"));
+    NSX_TRY(check_img(ctx, mask, "mask"));
+    NSX_TRY(check_img(ctx, out, "out"));
+    if (!same_shape(marker, mask) || !same_shape(mask, out))
+        return ctx->fail(NSX_ERR_INVALID, "marker, mask and out must have the same size");
+    if (marker->dtype != mask->dtype || out->dtype != mask->dtype)
+        return ctx->fail(NSX_ERR_INVALID, "marker, mask and out must have the same type");
+    if (marker->mem != NSX_MEM_DEVICE || mask->mem != NSX_MEM_DEVICE || out->mem != NSX_MEM_DEVICE)
+        return ctx->fail(NSX_ERR_INVALID, "nsx_imreconstruct_noalloc takes device images only");
+    if (overlaps(out, marker) || overlaps(out, mask)) return ctx->fail(NSX_ERR_INVALID, "out aliases an input");
+    const View vm{(char*)marker->data, marker->pitch_bytes, marker->rows, marker->cols, marker->dtype};
+    const View vk{(char*)mask->data, mask->pitch_bytes, mask->rows, mask->cols, mask->dtype};
+    const View vo{(char*)out->data, out->pitch_bytes, out->rows, out->cols, out->dtype};
+    return recon_run_noalloc(ctx, vm, vk, vo, conn == 8 ? 8 : 4, pick_stream(ctx, stream));
 }
 
 nsx_status nsx_imreconstruct_binary(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* mask, nsx_img* out,

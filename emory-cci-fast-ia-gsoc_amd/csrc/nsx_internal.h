@@ -196,6 +196,18 @@ void recon_status_gather(const ReconRun* rr, const ReconRun* rf, const int32_t* 
                          const int64_t* fstats, int32_t* out, hipStream_t s);
 void recon_note_rounds(nsx_ctx* ctx, ReconRun* run, int64_t rounds_with_work);
 
+// The allocation-free reconstruction (nsx_imreconstruct_noalloc): work lists
+// in a file-scope device array of recon_noalloc_scratch_bytes(), shared by
+// every call on a device.  recon_noalloc_init looks up the array's device
+// address for `device` (which must be current); recon_run_noalloc runs
+// imreconstruct (propagation rounds only) on device views with at most
+// recon_noalloc_max_tiles() tiles, allocating nothing.
+nsx_status recon_noalloc_init(int device);
+size_t recon_noalloc_scratch_bytes();
+int recon_noalloc_max_tiles();
+nsx_status recon_run_noalloc(nsx_ctx* ctx, const View& marker, const View& mask, const View& state, int conn,
+                             hipStream_t s);
+
 // ---- connected components ---------------------------------------------------------
 enum CclOut {
     CO_ROOT = 0,          // int32: root raster index, background -1

@@ -40,6 +40,7 @@
 #include <cmath>
 #include <algorithm>
 #include <type_traits>
+#include <mutex>
 
 namespace nsx {
 namespace {
@@ -1378,6 +1379,113 @@ nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mas
     nsx_status st = recon_complete(ctx, &run, s, nullptr);
     run.release();
     return st;
+}
+
+// ---- the allocation-free reconstruction ---------------------------------------
+// nsx_imreconstruct_noalloc keeps its tile work lists, stamps, round counters and
+// statistics in one file-scope device array instead of context workspace, so a
+// call allocates no device memory.  Every call on a device shares that device's
+// copy of the array; overlapping calls on one device race.  The array's device
+// address is looked up once per device, with the device current, when the
+// first context on it is created; host code uses only that address.
+namespace {
+
+constexpr int NOALLOC_MAX_TILES = 16384;
+constexpr size_t NA_STAMP = 0;                                                    // u32 [tiles]
+constexpr size_t NA_LISTS = NA_STAMP + sizeof(uint32_t) * NOALLOC_MAX_TILES;      // i32 [2][NSEG][tiles]
+constexpr size_t NA_COUNTS = NA_LISTS + 2 * sizeof(int32_t) * NSEG * (size_t)NOALLOC_MAX_TILES;
+constexpr size_t NA_STATS = NA_COUNTS + ((sizeof(int32_t) * (3 * NSEG * CSTRIDE + 64) + 255) & ~(size_t)255);
+constexpr size_t NA_BYTES = NA_STATS + 256;                                       // int64 [4] + spare
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NA_MAX_DEVICES] = {};
+
+}  // namespace
+
+size_t recon_noalloc_scratch_bytes() { return NA_BYTES; }
+int recon_noalloc_max_tiles() { return NOALLOC_MAX_TILES; }
+
+nsx_status recon_noalloc_init(int device) {
+    if (device < 0 || device >= NA_MAX_DEVICES) return NSX_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return NSX_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return NSX_ERR_HIP;
+    }
+    g_scratch_ptr[device] = ptr;
+    return NSX_OK;
+}
+
+nsx_status recon_run_noalloc(nsx_ctx* ctx, const View& marker, const View& mask, const View& state, int conn,
+                             hipStream_t s) {
+    if (mask.rows <= 0 || mask.cols <= 0) return NSX_OK;
+    char* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (ctx->device >= 0 && ctx->device < NA_MAX_DEVICES) base = (char*)g_scratch_ptr[ctx->device];
+    }
+    if (!base) return ctx->fail(NSX_ERR_HIP, "allocation-free scratch array has no device address on this device");
+    ImgDesc D = make_desc(&marker, mask, state, 0);
+    const int ntiles = D.ntx * D.nty;
+    if ((int64_t)D.ntx * D.nty > NOALLOC_MAX_TILES)
+        return ctx->fail(NSX_ERR_INVALID, "image too large for nsx_imreconstruct_noalloc: at most 16384 tiles of "
+                                          "62 x 62 pixels (e.g. 7936 x 7936)");
+    RoundFn first, later;
+    if (!select_fns(RM_RECON, mask.dtype, conn, &first, &later))
+        return ctx->fail(NSX_ERR_UNSUPPORTED, "reconstruction dtype not supported");
+    ReconArgs A{};
+    A.img0 = D;
+    A.nimg = 1;
+    A.ntiles = ntiles;
+    A.stamp = (uint32_t*)(base + NA_STAMP);
+    A.list0 = (int32_t*)(base + NA_LISTS);
+    A.list1 = A.list0 + (size_t)NSEG * ntiles;
+    A.counts = (int32_t*)(base + NA_COUNTS);
+    A.stats = (int64_t*)(base + NA_STATS);
+    A.visit_sweeps = std::max(1, std::min(MAX_IT, ctx->visit_sweeps));
+    A.first_list = 0;
+    const int grid = std::max(1, std::min(ntiles, (mask.dtype == NSX_U8 ? 12 : 8) * ctx->cu_count));
+    // the stamps, counters and statistics start cleared on every call (a kernel,
+    // not a memset, so the sequence can also be captured)
+    hipLaunchKernelGGL(recon_reset_kernel, dim3(std::max(1, std::min(1024, (ntiles + 255) / 256))), dim3(256), 0, s,
+                       A.stamp, A.ntiles, A.counts, A.stats, A.list0, 0, 0, StateFill{});
+    first(A, 0, ntiles, s);
+    // the round count of the last same-sized reconstruction on this context
+    // (read only: the prediction table is not extended here)
+    const uint64_t bucket = (uint64_t)std::lround(4.0 * std::log2((double)ntiles));
+    const uint64_t key = ((uint64_t)A.visit_sweeps << 40) ^ (bucket << 16) ^ ((uint64_t)RM_RECON << 8) ^
+                         ((uint64_t)mask.dtype << 4) ^ (uint64_t)conn;
+    auto it = ctx->round_hint.find(key);
+    int next = 1;
+    const int hint = it != ctx->round_hint.end() ? it->second : 4;
+    for (; next <= hint; ++next) later(A, next, grid, s);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    int batch = 2;
+    while (true) {
+        hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, &A.counts[CNT_SUM], &A.counts[cnt_idx(next, 0)]);
+        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, &A.counts[CNT_SUM], sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+        if (ctx->h_pinned[0] == 0) break;
+        if (next > (1 << 24)) return ctx->fail(NSX_ERR_NOT_CONVERGED, "reconstruction did not converge");
+        for (int j = 0; j < batch; ++j, ++next) later(A, next, grid, s);
+        NSX_HIP_TRY(ctx, hipGetLastError());
+        batch = std::min(batch * 2, 64);
+    }
+    if (it != ctx->round_hint.end()) {
+        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 32, A.stats, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+        int64_t rounds;
+        std::memcpy(&rounds, ctx->h_pinned + 32, sizeof(rounds));
+        const int need = (int)std::min<int64_t>(4096, rounds + 1);
+        it->second = std::max(need, it->second - 1);
+    }
+    return NSX_OK;
 }
 
 }  // namespace nsx

@@ -56,6 +56,7 @@ SIGNATURES = {
     "nsx_ctx_set_option": (C.c_int, [_CTX, C.c_int, C.c_int]),
     "nsx_ctx_get_option": (C.c_int, [_CTX, C.c_int, P(C.c_int)]),
     "nsx_imreconstruct": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_void_p]),
+    "nsx_imreconstruct_noalloc": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_imreconstruct_binary": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_void_p]),
     "nsx_fill_holes": (C.c_int, [_CTX, _IMG, _IMG, C.c_int, C.c_int, C.c_void_p]),
     "nsx_imfill": (C.c_int, [_CTX, _IMG, _IMG, _IMG, C.c_int, C.c_int, C.c_void_p]),

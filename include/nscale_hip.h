@@ -132,6 +132,22 @@ nsx_status nsx_ctx_get_option(const nsx_ctx* ctx, int option, int* value);
 nsx_status nsx_imreconstruct(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* mask,
                              nsx_img* out, int conn, void* stream);
 
+/* nsx_imreconstruct without allocating: no hipMalloc, hipFree or allocating
+ * library call.  No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  Device images only (marker, mask, out);
+ * u8, i32, f32; conn 8 or 4; the same result as nsx_imreconstruct.  The tile
+ * work lists live in a static device array of the library (about 1.1 MB per
+ * GPU, held until the process ends), which limits images to 16384
+ * reconstruction tiles of 62 x 62 pixels (e.g. 7936 x 7936); larger images are
+ * refused with NSX_ERR_INVALID.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other threads, other streams) race and must be
+ * serialised by the caller.  The context supplies the stream and the pinned
+ * word the convergence check reads; it must have been created (which looks up
+ * the array's device address) before the call.  Synchronises the stream. */
+nsx_status nsx_imreconstruct_noalloc(nsx_ctx* ctx, const nsx_img* marker, const nsx_img* mask,
+                                     nsx_img* out, int conn, void* stream);
+
 /* Binary reconstruction: pixels of `mask` != 0 connected to a seed (marker != 0
  * and mask != 0) become max(T); nonzero marker pixels are kept as given.
  * u8, i32.  Replaces nscale::imreconstructBinary<T> (MorphologicOperations.h:43-44,
