@@ -281,6 +281,12 @@ nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value) {
         ctx->levelset_levels = value;
         return NSX_OK;
     }
+    if (param == NSX_TUNE_ENGINE) {
+        if (value != NSX_ENGINE_ROUNDS && value != NSX_ENGINE_TREE)
+            return ctx->fail(NSX_ERR_INVALID, "engine must be NSX_ENGINE_ROUNDS or NSX_ENGINE_TREE");
+        ctx->engine = value;
+        return NSX_OK;
+    }
     return ctx->fail(NSX_ERR_INVALID, "unknown tuning parameter");
 }
 
@@ -292,6 +298,10 @@ nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value) {
     }
     if (param == NSX_TUNE_LEVELSET_LEVELS) {
         *value = ctx->levelset_levels;
+        return NSX_OK;
+    }
+    if (param == NSX_TUNE_ENGINE) {
+        *value = ctx->engine;
         return NSX_OK;
     }
     return NSX_ERR_INVALID;
@@ -753,6 +763,28 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
     // statistics, int32 [32..39] the status gathered for the host.
     int32_t* dw = (int32_t*)ctx->d_stats;
     const int rconn = recon_conn == 8 ? 8 : 4, fconn = fill_conn == 8 ? 8 : 4;
+    if (mask->dtype == NSX_U8 && ctx->engine == NSX_ENGINE_TREE) {
+        // NSX_ENGINE_TREE: both reconstructions through the component tree
+        // (fixed launches), then the labelling; one synchronisation reads the
+        // count and both error words.  A bound hit in either falls back to the
+        // rounds below.
+        uint32_t *er = nullptr, *ef = nullptr;
+        NSX_TRY(mtree_launch(ctx, RM_RECON, &vm, vk, vr, rconn, s, &er, WS_MT_PAR));
+        NSX_TRY(mtree_launch(ctx, RM_FILL, nullptr, vr, vf, fconn, s, &ef, WS_MT2_PAR));
+        NSX_TRY(ccl_run(ctx, vf, vl, p, dw + 12, s));
+        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 12, dw + 12, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 56, er, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 57, ef, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+        if (ctx->h_pinned[56] == 0 && ctx->h_pinned[57] == 0) {
+            const int32_t n = ctx->h_pinned[12];
+            if (recon_out && recon_out->mem == NSX_MEM_HOST) NSX_TRY(from_device(ctx, vr, recon_out, s));
+            if (fill_out && fill_out->mem == NSX_MEM_HOST) NSX_TRY(from_device(ctx, vf, fill_out, s));
+            NSX_TRY(from_device(ctx, vl, labels_out, s));
+            if (ncomp) *ncomp = n;
+            return NSX_OK;
+        }
+    }
     if (mask->dtype == NSX_I32 && ctx->i32_invert == NSX_I32_INVERT_WRAP) {
         // int32 fill under wrapping negation is the constant INT_MIN (see
         // nsx_fill_holes); the labelling still runs on it

@@ -71,6 +71,7 @@ struct nsx_ctx {
     // Tuning (nsx_ctx_set_tuning).
     int visit_sweeps = 4;
     int levelset_levels = 3;
+    int engine = NSX_ENGINE_ROUNDS;   // NSX_TUNE_ENGINE
 
     // Options (nsx_ctx_set_option).
     int i32_invert = NSX_I32_INVERT_WRAP;
@@ -122,6 +123,22 @@ enum WsSlot {
     WS_STAGE_B,
     WS_STAGE_C,
     WS_CCL_LROOT,        // u16 per pixel index: tile-local root (0xFFFF background)
+    WS_MT_PAR,           // component-tree reconstruction: parent of each node (int32)
+    WS_MT_M,             // subtree maximum of the marker per node (u32)
+    WS_MT_G,             // level per node (u8)
+    WS_MT_R,             // result per node (u8)
+    WS_MT_LNODE,         // node of each pixel within its tile (u16)
+    WS_MT_TCOUNT,        // nodes per tile
+    WS_MT_WALK,          // nodes whose parent changed in the tile-edge merge
+    WS_MT_CTL,           // counters and error bits
+    WS_MT2_PAR,          // a second component-tree reconstruction in flight (the chain's fill)
+    WS_MT2_M,
+    WS_MT2_G,
+    WS_MT2_R,
+    WS_MT2_LNODE,
+    WS_MT2_TCOUNT,
+    WS_MT2_WALK,
+    WS_MT2_CTL,
     WS_NSLOTS
 };
 
@@ -207,6 +224,17 @@ size_t recon_noalloc_scratch_bytes();
 int recon_noalloc_max_tiles();
 nsx_status recon_run_noalloc(nsx_ctx* ctx, const View& marker, const View& mask, const View& state, int conn,
                              hipStream_t s);
+
+// ---- reconstruction through the component tree (u8) ------------------------------------
+// Enqueues the component-tree reconstruction of a u8 image (mode RM_RECON:
+// marker under mask; RM_FILL: hole filling of `mask`, marker unused) into
+// `out`, a fixed number of launches.  *err_dev receives the device word that
+// is nonzero when a bound was hit (the result is then invalid and must be
+// recomputed by propagation); it stays valid until the next launch on the same
+// workspace slots.  ws_base: WS_MT_PAR or WS_MT2_PAR.
+nsx_status mtree_launch(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& out, int conn,
+                        hipStream_t s, uint32_t** err_dev, int ws_base = WS_MT_PAR);
+int mtree_tiles(int rows, int cols);
 
 // ---- connected components ---------------------------------------------------------
 enum CclOut {

@@ -1368,6 +1368,17 @@ nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mas
         NSX_TRY(recon_levelset_u8(ctx, *marker, mask, state, conn == 8 ? 8 : 4, ctx->levelset_levels, &used, s, req));
         if (used) return NSX_OK;
     }
+    if ((mode == RM_RECON || mode == RM_FILL) && mask.dtype == NSX_U8 && ctx->engine == NSX_ENGINE_TREE) {
+        // NSX_ENGINE_TREE: the component tree; propagation only when one of
+        // its bounds was hit
+        uint32_t* err = nullptr;
+        NSX_TRY(mtree_launch(ctx, mode, marker, mask, state, conn == 8 ? 8 : 4, s, &err));
+        if (err) {
+            NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 56, err, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+            if (ctx->h_pinned[56] == 0) return NSX_OK;
+        }
+    }
     ImgDesc D = make_desc(marker, mask, state, 0);
     ReconBatch B{};
     B.nimg = 1;

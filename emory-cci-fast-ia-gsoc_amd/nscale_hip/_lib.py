@@ -18,6 +18,9 @@ NSX_ERR_NOT_CONVERGED = 6
 NSX_U8, NSX_I32, NSX_F32 = 0, 1, 2
 NSX_TUNE_VISIT_SWEEPS = 0
 NSX_TUNE_LEVELSET_LEVELS = 1
+NSX_TUNE_ENGINE = 2
+NSX_ENGINE_ROUNDS = 0
+NSX_ENGINE_TREE = 1
 NSX_OPT_I32_INVERT = 0
 NSX_I32_INVERT_WRAP = 0
 NSX_I32_INVERT_SATURATE = 1

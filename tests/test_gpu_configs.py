@@ -35,7 +35,9 @@ def _chain_dev(nh, ctx, dm, dk):
     return r.cpu().numpy(), f.cpu().numpy(), lab.cpu().numpy(), n
 
 
-def test_chain_4096_full(nh, orc, gpu):
+@pytest.mark.parametrize("engine", ["rounds", "tree"])
+def test_chain_4096_full(nh, orc, gpu, engine):
+    """The default engine (propagation rounds) and the component tree."""
     import torch
     from nscale_hip.workloads import chain_images
     marker, mask = chain_images()
@@ -43,13 +45,17 @@ def test_chain_4096_full(nh, orc, gpu):
     assert en == 629
     dm, dk = torch.from_numpy(marker).cuda(), torch.from_numpy(mask).cuda()
     ctx = nh.Context(0)
+    if engine == "tree":
+        ctx.set_tuning(nh._lib.NSX_TUNE_ENGINE, nh._lib.NSX_ENGINE_TREE)
     try:
-        # fresh context: the default prediction (4 rounds) is short of the 10
-        # reconstruction and 36 fill rounds, so the call re-launches both
+        # fresh context: with rounds, the default prediction (4 rounds) is
+        # short of the 10 reconstruction and 36 fill rounds, so the call
+        # re-launches both
         r, f, lab, n = _chain_dev(nh, ctx, dm, dk)
         assert np.array_equal(r, er) and np.array_equal(f, ef)
         assert np.array_equal(lab, elab) and n == en
-        assert ctx.recon_stats(1)["rounds"] > 4
+        if engine == "rounds":
+            assert ctx.recon_stats(1)["rounds"] > 4
         for rep in range(2):
             r, f, lab, n = _chain_dev(nh, ctx, dm, dk)
             assert np.array_equal(r, er) and np.array_equal(f, ef), f"repetition {rep}"
@@ -58,16 +64,20 @@ def test_chain_4096_full(nh, orc, gpu):
         ctx.close()
 
 
-def test_chain_4096_after_easy_input(nh, orc, gpu):
+@pytest.mark.parametrize("engine", ["rounds", "tree"])
+def test_chain_4096_after_easy_input(nh, orc, gpu, engine):
     """An easy input of the same size (marker == mask: nothing propagates)
     sets a low round prediction for the size; the full chain that follows
-    needs more rounds than predicted and must still be exact."""
+    needs more rounds than predicted and must still be exact (with rounds:
+    the re-launch branch; the component tree has no prediction)."""
     import torch
     from nscale_hip.workloads import chain_images
     marker, mask = chain_images()
     er, ef, elab, en = orc.chain_u8(marker, mask, 8, 4, 128, 8, 11, 1000)
     easy = np.zeros_like(mask)
     ctx = nh.Context(0)
+    if engine == "tree":
+        ctx.set_tuning(nh._lib.NSX_TUNE_ENGINE, nh._lib.NSX_ENGINE_TREE)
     try:
         de = torch.from_numpy(easy).cuda()
         r, f, lab, n = _chain_dev(nh, ctx, de, de)
@@ -101,8 +111,17 @@ def test_batch_full(nh, orc, gpu):
             counts = b.counts()
             for i, (dm, dk, r, lab, thr) in enumerate(dt):
                 r2, lab2, n2 = nh.recon_label(dm, dk, 8, thr, 8, ctx=per_tile)
-                assert torch.equal(r, r2) and torch.equal(lab, lab2) and counts[i] == n2, \
-                    f"replay {rep}: tile {i} ({tuple(dk.shape)}, {dk.dtype}) batched != per-tile"
+                if not (torch.equal(r, r2) and torch.equal(lab, lab2) and counts[i] == n2):
+                    # name the side that disagrees with the oracle
+                    rn = r.cpu().numpy()
+                    bw = np.where(rn > (np.float32(thr) if rn.dtype == np.float32 else thr), 255, 0).astype(np.uint8)
+                    wl, wn = orc.bwlabel2(bw, 8, relab=True)
+                    a, c = lab.cpu().numpy(), lab2.cpu().numpy()
+                    raise AssertionError(
+                        f"replay {rep}: tile {i} ({tuple(dk.shape)}, {dk.dtype}) batched != per-tile: recon equal "
+                        f"{torch.equal(r, r2)}; labels vs oracle: batched {int((a != wl).sum())} px, per-tile "
+                        f"{int((c != wl).sum())} px (first batched {np.argwhere(a != wl)[:3].tolist()}, per-tile "
+                        f"{np.argwhere(c != wl)[:3].tolist()}); counts {counts[i]} {n2} oracle {wn}")
         big_u8 = max(range(256), key=lambda i: tiles[i][1].size)
         big_f32 = max(range(256, 288), key=lambda i: tiles[i][1].size)
         for i in sorted({0, 1, 255, 256, 287, big_u8, big_f32}):

@@ -1,0 +1,81 @@
+"""u8 reconstruction and hole filling through the component tree
+(NSX_ENGINE_TREE) against the oracle: tile-edge shapes, plateaus, few-valued and
+noisy masks, both connectivities; the same results as the propagation rounds
+(NSX_ENGINE_ROUNDS)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(1, 1), (1, 300), (300, 1), (63, 64), (64, 64), (65, 65), (64, 129), (128, 128), (129, 200),
+          (257, 300), (700, 530)]
+
+
+def _ctx(nh, engine, levelset=0):
+    c = nh.Context(0)
+    c.set_tuning(nh._lib.NSX_TUNE_ENGINE, engine)
+    c.set_tuning(nh._lib.NSX_TUNE_LEVELSET_LEVELS, levelset)
+    return c
+
+
+def _masks(rng, shape):
+    yield "noise", rng.integers(0, 256, size=shape).astype(np.uint8)
+    yield "few", (rng.integers(0, 4, size=shape) * 60).astype(np.uint8)
+    yield "flat", np.full(shape, 77, np.uint8)
+    yield "zeros", np.zeros(shape, np.uint8)
+    smooth = np.cumsum(np.cumsum(rng.integers(-1, 2, size=shape), 0), 1)
+    yield "smooth", np.clip(smooth - smooth.min(), 0, 255).astype(np.uint8)
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_mtree_recon_vs_oracle(nh, orc, gpu, conn):
+    ctx = _ctx(nh, nh._lib.NSX_ENGINE_TREE)
+    rng = np.random.default_rng(300 + conn)
+    try:
+        for shape in SHAPES:
+            for name, mask in _masks(rng, shape):
+                for mk in ("sparse", "sub30"):
+                    if mk == "sparse":
+                        marker = np.where(rng.random(shape) < 0.02, rng.integers(0, 256, size=shape), 0).astype(np.uint8)
+                    else:
+                        marker = np.clip(mask.astype(int) - 30, 0, 255).astype(np.uint8)
+                    print(f"recon conn={conn} shape={shape} mask={name} marker={mk}", flush=True)
+                    got = nh.imreconstruct(marker, mask, conn, ctx=ctx)
+                    assert np.array_equal(got, orc.imreconstruct(marker, mask, conn)), (shape, name, mk)
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_mtree_fill_vs_oracle(nh, orc, gpu, conn):
+    ctx = _ctx(nh, nh._lib.NSX_ENGINE_TREE)
+    rng = np.random.default_rng(400 + conn)
+    try:
+        for shape in SHAPES:
+            for name, img in _masks(rng, shape):
+                print(f"fill conn={conn} shape={shape} image={name}", flush=True)
+                got = nh.imfillHoles(img, False, conn, ctx=ctx)
+                assert np.array_equal(got, orc.fill_holes(img, False, conn)), (shape, name)
+    finally:
+        ctx.close()
+
+
+def test_mtree_disks_match_rounds(nh, orc, gpu):
+    """The disks construction at 1500 x 1700: component tree == rounds ==
+    oracle for recon (8) and fill (4), device buffers."""
+    import torch
+    mask = nh.gen_disks_u8(1500, 1700, 500, 0x5EED0601)
+    marker = nh.gen_marker_u8(mask, 30)
+    auto, rounds = _ctx(nh, nh._lib.NSX_ENGINE_TREE, 3), _ctx(nh, nh._lib.NSX_ENGINE_ROUNDS, 3)
+    try:
+        dm, dk = torch.from_numpy(marker).cuda(), torch.from_numpy(mask).cuda()
+        a = nh.imreconstruct(dm, dk, 8, ctx=auto)
+        b = nh.imreconstruct(dm, dk, 8, ctx=rounds)
+        want = orc.imreconstruct(marker, mask, 8)
+        assert np.array_equal(a.cpu().numpy(), want) and torch.equal(a, b)
+        fa = nh.imfillHoles(a, False, 4, ctx=auto)
+        fb = nh.imfillHoles(a, False, 4, ctx=rounds)
+        assert np.array_equal(fa.cpu().numpy(), orc.fill_holes(want, False, 4)) and torch.equal(fa, fb)
+    finally:
+        auto.close()
+        rounds.close()
