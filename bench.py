@@ -44,9 +44,11 @@ def parse():
                    help="skip the CPU baseline leg (the oracle run that also checks the GPU results)")
     p.add_argument("--cpu-runs", type=int, default=20, help="oracle chain runs in the CPU sample")
     p.add_argument("--workload", default="chain4096",
-                   choices=["chain4096", "batch", "maze", "serve", "copy", "ops"],
+                   choices=["chain4096", "batch", "maze", "maze_levels", "serve", "copy", "ops"],
                    help="chain4096: the headline 4096^2 chain (default); batch: 256 u8 + 32 f32 uneven tiles "
-                        "batched vs per-tile; maze: 16384^2 maze stress; serve: continuous-batching server; "
+                        "batched vs per-tile; maze: 16384^2 maze stress; maze_levels: the same maze with "
+                        "corridors at 150..220 plus noise (propagation's worst case); serve: continuous-batching "
+                        "server; "
                         "ops: seeded imfill, bwselect, imhmin, localMaxima/Minima, boundingBox and "
                         "distanceTransform, each on the 4096^2 disk image")
     p.add_argument("--tiles-u8", type=int, default=256)
@@ -57,6 +59,8 @@ def parse():
     p.add_argument("--maze-size", type=int, default=16384, help="maze: side of the maze image")
     p.add_argument("--maze-cpu-size", type=int, default=16384,
                    help="maze: side of the maze the CPU baseline runs (default: the whole workload)")
+    p.add_argument("--maze-levels-cpu-size", type=int, default=4096,
+                   help="maze_levels: side of the many-valued maze the CPU baseline runs")
     p.add_argument("--requests", type=int, default=200, help="serve: requests in the trace")
     p.add_argument("--slots", type=int, default=4, help="serve: worker slots of the server")
     p.add_argument("--load", type=float, default=1.0,
@@ -476,6 +480,107 @@ def bench_maze(args):
     print(json.dumps(out))
 
 
+def bench_maze_levels(args):
+    """The many-valued maze (nscale_hip.workloads.maze_levels_images: the 16384^2
+    maze with corridor pixels uniform in 150..220, noise -8..8 on every pixel,
+    a 5x5 seed of 255 at the entrance).  The level-set path declines it (over
+    100 distinct values) and propagation needs one round per tile crossing of
+    paths millions of pixels long.  One step = imreconstruct (4-conn) -> (> 100)
+    -> bwlabel (4-conn) through nsx_recon_label with the default engine
+    (NSX_ENGINE_AUTO: rounds, switching to the component tree past 256 rounds
+    and starting there on later calls of the size).  Checked: every corridor
+    pixel is reached above 100 and every wall stays below it, so the labels are
+    exactly one component, the corridor set; the reconstruction equals the one
+    propagation rounds alone compute (timed once, reported as rounds_ms) and,
+    on a sample of the same construction, the oracle's."""
+    import numpy as np
+    import torch
+
+    import nscale_hip as nh
+    from nscale_hip.workloads import maze_levels_images
+    n = args.maze_size
+    t0 = time.perf_counter()
+    marker, mask = maze_levels_images(n)
+    log(f"generated {n}x{n} many-valued maze in {time.perf_counter() - t0:.1f} s "
+        f"({len(np.unique(mask))} distinct values)")
+    dev = torch.device("cuda:0")
+    dm, dk = torch.from_numpy(marker).to(dev), torch.from_numpy(mask).to(dev)
+    corridors = torch.from_numpy(mask > 100).to(dev)
+    del marker, mask
+    dr = torch.empty_like(dk)
+    dl = torch.empty((n, n), dtype=torch.int32, device=dev)
+    ctx = nh.Context(0)
+
+    def step(c=ctx, r=dr, lab=dl):
+        return nh.recon_label(dm, dk, 4, 100.0, 4, recon_out=r, labels_out=lab, ctx=c)[2]
+
+    t0 = time.perf_counter()
+    ncomp = step()
+    torch.cuda.synchronize()
+    first_ms = (time.perf_counter() - t0) * 1e3
+    exact = bool(ncomp == 1 and torch.equal(dl, corridors.to(torch.int32)))
+    # propagation rounds alone, once: its time and its (identical) result
+    rctx = nh.Context(0)
+    rctx.set_tuning(nh._lib.NSX_TUNE_ENGINE, nh._lib.NSX_ENGINE_ROUNDS)
+    rr = torch.empty_like(dk)
+    rl = torch.empty_like(dl)
+    t0 = time.perf_counter()
+    step(rctx, rr, rl)
+    torch.cuda.synchronize()
+    rounds_ms = (time.perf_counter() - t0) * 1e3
+    rounds = rctx.recon_stats(0)["rounds"]
+    exact = exact and bool(torch.equal(rr, dr) and torch.equal(rl, dl))
+    del rr, rl, corridors
+    rctx.close()
+    log(f"closed form and rounds agree: {exact}; first call {first_ms:.1f} ms; rounds alone {rounds_ms:.1f} ms "
+        f"({rounds} rounds)")
+    if not exact:
+        sys.exit(1)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - s0) * 1e3)
+    elapsed = time.perf_counter() - t0
+    px = n * n
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        cn = min(n, args.maze_levels_cpu_size)
+        cm, ck = maze_levels_images(cn)
+        t0 = time.perf_counter()
+        wr = orc.imreconstruct(cm, ck, 4)
+        wl, wn = orc.bwlabel2(np.where(wr > 100, 255, 0).astype(np.uint8), 4, relab=True)
+        dt = time.perf_counter() - t0
+        gr, gl, gn = nh.recon_label(torch.from_numpy(cm).cuda(), torch.from_numpy(ck).cuda(), 4, 100.0, 4, ctx=ctx)
+        same = bool(np.array_equal(gr.cpu().numpy(), wr) and np.array_equal(gl.cpu().numpy(), wl) and gn == wn)
+        if not same:
+            log("many-valued maze sample differs from the oracle")
+            sys.exit(1)
+        cpu = {"value": round(cn * cn / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+               "sample": f"one {cn}x{cn} many-valued maze of the same construction (recon4 + bwlabel4), single "
+                         f"thread, {dt:.1f} s; GPU output on it identical", "host_cpus": os.cpu_count()}
+    ctx.close()
+    out = {
+        "metric": "Mpix/s many-valued maze reconstruct+CCL @16384^2 u8 bit-exact",
+        "value": round(args.steps * px / elapsed / 1e6, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "p50_ms": round(statistics.median(lat), 3), "p99_ms": round(pct(lat, 0.99), 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic", "bitexact": exact,
+        "first_call_ms": round(first_ms, 1), "rounds_ms": round(rounds_ms, 1), "rounds": rounds,
+        "config": {"workload": f"{n}x{n} u8 maze, corridors 150..220 + noise, 5x5 seed: recon4 -> >100 -> bwlabel4 "
+                               "(engine auto)", "pixels": px},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def bench_copy(args):
     """Calibration workload for the HBM traffic counters: one elementwise
     kernel (int32 negation, torch) that reads and writes exactly 256 MiB per
@@ -709,6 +814,8 @@ def main():
         return bench_serve(args)
     if args.workload == "maze":
         return bench_maze(args)
+    if args.workload == "maze_levels":
+        return bench_maze_levels(args)
     if args.workload == "copy":
         return bench_copy(args)
     if args.workload == "ops":

@@ -282,8 +282,8 @@ nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value) {
         return NSX_OK;
     }
     if (param == NSX_TUNE_ENGINE) {
-        if (value != NSX_ENGINE_ROUNDS && value != NSX_ENGINE_TREE)
-            return ctx->fail(NSX_ERR_INVALID, "engine must be NSX_ENGINE_ROUNDS or NSX_ENGINE_TREE");
+        if (value != NSX_ENGINE_ROUNDS && value != NSX_ENGINE_TREE && value != NSX_ENGINE_AUTO)
+            return ctx->fail(NSX_ERR_INVALID, "engine must be NSX_ENGINE_ROUNDS, NSX_ENGINE_TREE or NSX_ENGINE_AUTO");
         ctx->engine = value;
         return NSX_OK;
     }
@@ -811,6 +811,22 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
     bf.ws_base = WS_FILL_STAMP;
     bf.stats = ctx->d_stats + 8;
     ReconRun rr, rf;
+    // NSX_ENGINE_AUTO on u8: stages whose last run of this size passed the
+    // tree threshold go stage by stage through recon_run, which picks the tree
+    const bool auto_u8 = mask->dtype == NSX_U8 && ctx->engine == NSX_ENGINE_AUTO;
+    if (auto_u8 && (recon_predicted_rounds(ctx, RM_RECON, NSX_U8, rconn, br.ntiles, 0) > 256 ||
+                    recon_predicted_rounds(ctx, RM_FILL, NSX_U8, fconn, bf.ntiles, 0) > 256)) {
+        NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vr, rconn, s));
+        NSX_TRY(recon_run(ctx, RM_FILL, nullptr, vr, vf, fconn, s));
+        NSX_TRY(ccl_run(ctx, vf, vl, p, dw + 12, s));
+        int32_t n = 0;
+        NSX_TRY(read_count(ctx, dw + 12, &n, s));
+        if (recon_out && recon_out->mem == NSX_MEM_HOST) NSX_TRY(from_device(ctx, vr, recon_out, s));
+        if (fill_out && fill_out->mem == NSX_MEM_HOST) NSX_TRY(from_device(ctx, vf, fill_out, s));
+        NSX_TRY(from_device(ctx, vl, labels_out, s));
+        if (ncomp) *ncomp = n;
+        return NSX_OK;
+    }
     // All stages are enqueued back to back; convergence of both propagations
     // is checked once, together with the component count.
     NSX_TRY(recon_launch(ctx, RM_RECON, mask->dtype, rconn, br, s, &rr, nullptr));
@@ -826,10 +842,22 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
         // a propagation needed more rounds than predicted: finish it and redo
         // the stages downstream of it
         if (status[1] != 0) {
-            NSX_TRY(recon_complete(ctx, &rr, s, nullptr));
-            NSX_TRY(recon_launch(ctx, RM_FILL, mask->dtype, fconn, bf, s, &rf, nullptr));
+            if (auto_u8) {
+                rr.release();
+                rf.release();
+                NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vr, rconn, s));
+                NSX_TRY(recon_run(ctx, RM_FILL, nullptr, vr, vf, fconn, s));
+            } else {
+                NSX_TRY(recon_complete(ctx, &rr, s, nullptr));
+                NSX_TRY(recon_launch(ctx, RM_FILL, mask->dtype, fconn, bf, s, &rf, nullptr));
+                NSX_TRY(recon_complete(ctx, &rf, s, nullptr));
+            }
+        } else if (auto_u8) {
+            rf.release();
+            NSX_TRY(recon_run(ctx, RM_FILL, nullptr, vr, vf, fconn, s));
+        } else {
+            NSX_TRY(recon_complete(ctx, &rf, s, nullptr));
         }
-        NSX_TRY(recon_complete(ctx, &rf, s, nullptr));
         NSX_TRY(ccl_run(ctx, vf, vl, p, dw + 12, s));
         NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 12, dw + 12, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         NSX_HIP_TRY(ctx, hipStreamSynchronize(s));

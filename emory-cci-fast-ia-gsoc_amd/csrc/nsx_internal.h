@@ -71,7 +71,7 @@ struct nsx_ctx {
     // Tuning (nsx_ctx_set_tuning).
     int visit_sweeps = 4;
     int levelset_levels = 3;
-    int engine = NSX_ENGINE_ROUNDS;   // NSX_TUNE_ENGINE
+    int engine = NSX_ENGINE_AUTO;     // NSX_TUNE_ENGINE
 
     // Options (nsx_ctx_set_option).
     int i32_invert = NSX_I32_INVERT_WRAP;
@@ -202,7 +202,13 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
                         ReconRun* run, int32_t* pending_dev);
 // Synchronises, launches further rounds until the work list is empty, and
 // updates the round prediction.  *extended tells whether rounds were added.
-nsx_status recon_complete(nsx_ctx* ctx, ReconRun* run, hipStream_t s, bool* extended);
+// max_rounds > 0: once the run passes that many rounds unconverged, it is
+// released and *gave_up set (the caller finishes another way; the round
+// prediction for the size is raised so the next call can skip the rounds).
+nsx_status recon_complete(nsx_ctx* ctx, ReconRun* run, hipStream_t s, bool* extended, int max_rounds = 0,
+                          bool* gave_up = nullptr);
+// Rounds the last reconstruction of this kind and size needed (0 = none yet).
+int recon_predicted_rounds(nsx_ctx* ctx, int mode, int dtype, int conn, int ntiles, int visit_sweeps);
 // For a run already known to have converged: records how many rounds had work
 // (read by the caller from the run's stats word 0) and releases the run.
 // One kernel that gathers a chain's status for a single copy: out[0] = *ncomp,

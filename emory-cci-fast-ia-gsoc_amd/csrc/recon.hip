@@ -1314,8 +1314,10 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     return NSX_OK;
 }
 
-nsx_status recon_complete(nsx_ctx* ctx, ReconRun* run, hipStream_t s, bool* extended) {
+nsx_status recon_complete(nsx_ctx* ctx, ReconRun* run, hipStream_t s, bool* extended, int max_rounds,
+                          bool* gave_up) {
     if (extended) *extended = false;
+    if (gave_up) *gave_up = false;
     ReconRunImpl* R = run->impl;
     if (!R) return NSX_OK;
     const ReconArgs& A = R->A;
@@ -1327,6 +1329,14 @@ nsx_status recon_complete(nsx_ctx* ctx, ReconRun* run, hipStream_t s, bool* exte
         NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
         if (ctx->h_pinned[0] == 0) break;
         if (extended) *extended = true;
+        if (max_rounds > 0 && R->next_round > max_rounds) {
+            // the caller finishes by another method; the next call of this
+            // size goes there directly
+            note_hint(ctx, R->key, (int64_t)R->next_round);
+            run->release();
+            if (gave_up) *gave_up = true;
+            return NSX_OK;
+        }
         if (R->next_round > (1 << 24)) return ctx->fail(NSX_ERR_NOT_CONVERGED, "reconstruction did not converge");
         for (int j = 0; j < batch; ++j, ++R->next_round) {
             const int h = ctx->tic(TK_RECON_ROUND, s);
@@ -1360,6 +1370,33 @@ void recon_note_rounds(nsx_ctx* ctx, ReconRun* run, int64_t rounds_with_work) {
     run->release();
 }
 
+int recon_predicted_rounds(nsx_ctx* ctx, int mode, int dtype, int conn, int ntiles, int visit_sweeps) {
+    const bool quad = (mode == RM_FILL || mode == RM_FILLBIN) && conn == 4;
+    const int pairs = std::max(1, std::min(MAX_IT, (visit_sweeps > 0 ? visit_sweeps : ctx->visit_sweeps) + (quad ? 1 : 0)));
+    const uint64_t bucket = (uint64_t)std::lround(4.0 * std::log2((double)std::max(1, ntiles)));
+    const uint64_t key = ((uint64_t)pairs << 40) ^ (bucket << 16) ^ ((uint64_t)mode << 8) ^ ((uint64_t)dtype << 4) ^
+                         (uint64_t)conn;
+    auto it = ctx->round_hint.find(key);
+    return it != ctx->round_hint.end() ? it->second : 0;
+}
+
+// u8 reconstructions that take more propagation rounds than this are finished
+// through the component tree under NSX_ENGINE_AUTO (a round costs a tile
+// visit's latency, ~10-25 us; the tree costs a fixed few ms at 4096^2).
+constexpr int AUTO_TREE_ROUNDS = 256;
+
+static nsx_status recon_tree(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& state,
+                             int conn, hipStream_t s, bool* ok) {
+    *ok = false;
+    uint32_t* err = nullptr;
+    NSX_TRY(mtree_launch(ctx, mode, marker, mask, state, conn, s, &err));
+    if (!err) return NSX_OK;
+    NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 56, err, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+    *ok = ctx->h_pinned[56] == 0;
+    return NSX_OK;
+}
+
 nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& state,
                      int conn, hipStream_t s, LevelLabelReq* req) {
     if (mask.rows <= 0 || mask.cols <= 0) return NSX_OK;
@@ -1368,26 +1405,38 @@ nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mas
         NSX_TRY(recon_levelset_u8(ctx, *marker, mask, state, conn == 8 ? 8 : 4, ctx->levelset_levels, &used, s, req));
         if (used) return NSX_OK;
     }
-    if ((mode == RM_RECON || mode == RM_FILL) && mask.dtype == NSX_U8 && ctx->engine == NSX_ENGINE_TREE) {
-        // NSX_ENGINE_TREE: the component tree; propagation only when one of
-        // its bounds was hit
-        uint32_t* err = nullptr;
-        NSX_TRY(mtree_launch(ctx, mode, marker, mask, state, conn == 8 ? 8 : 4, s, &err));
-        if (err) {
-            NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 56, err, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-            NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
-            if (ctx->h_pinned[56] == 0) return NSX_OK;
-        }
-    }
+    conn = conn == 8 ? 8 : 4;
     ImgDesc D = make_desc(marker, mask, state, 0);
+    const bool tree_ok = (mode == RM_RECON || mode == RM_FILL) && mask.dtype == NSX_U8;
+    const bool tree_first = tree_ok && (ctx->engine == NSX_ENGINE_TREE ||
+                                        (ctx->engine == NSX_ENGINE_AUTO &&
+                                         recon_predicted_rounds(ctx, mode, mask.dtype, conn, D.ntx * D.nty, 0) >
+                                             AUTO_TREE_ROUNDS));
+    if (tree_first) {
+        // the component tree; propagation only when one of its bounds was hit
+        bool ok = false;
+        NSX_TRY(recon_tree(ctx, mode, marker, mask, state, conn, s, &ok));
+        if (ok) return NSX_OK;
+    }
     ReconBatch B{};
     B.nimg = 1;
     B.ntiles = D.ntx * D.nty;
     B.host_desc = &D;
     B.ws_base = WS_RECON_STAMP;
     ReconRun run;
-    NSX_TRY(recon_launch(ctx, mode, mask.dtype, conn == 8 ? 8 : 4, B, s, &run, nullptr));
-    nsx_status st = recon_complete(ctx, &run, s, nullptr);
+    NSX_TRY(recon_launch(ctx, mode, mask.dtype, conn, B, s, &run, nullptr));
+    bool gave_up = false;
+    const int cap = tree_ok && !tree_first && ctx->engine == NSX_ENGINE_AUTO ? AUTO_TREE_ROUNDS : 0;
+    nsx_status st = recon_complete(ctx, &run, s, nullptr, cap, &gave_up);
+    run.release();
+    if (st != NSX_OK || !gave_up) return st;
+    // many rounds: the content carries values far; the tree's cost does not
+    // depend on that (the rounds' partial state is a lower bound it overwrites)
+    bool ok = false;
+    NSX_TRY(recon_tree(ctx, mode, marker, mask, state, conn, s, &ok));
+    if (ok) return NSX_OK;
+    NSX_TRY(recon_launch(ctx, mode, mask.dtype, conn, B, s, &run, nullptr));
+    st = recon_complete(ctx, &run, s, nullptr);
     run.release();
     return st;
 }

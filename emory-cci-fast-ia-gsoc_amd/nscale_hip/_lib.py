@@ -21,6 +21,7 @@ NSX_TUNE_LEVELSET_LEVELS = 1
 NSX_TUNE_ENGINE = 2
 NSX_ENGINE_ROUNDS = 0
 NSX_ENGINE_TREE = 1
+NSX_ENGINE_AUTO = 2
 NSX_OPT_I32_INVERT = 0
 NSX_I32_INVERT_WRAP = 0
 NSX_I32_INVERT_SATURATE = 1

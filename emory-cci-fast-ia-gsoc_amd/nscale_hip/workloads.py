@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import gen_disks_u8, gen_marker_u8, scale_f32, scale_i32
+from . import gen_disks_u8, gen_marker_u8, gen_maze_u8, scale_f32, scale_i32
 
 SERVE_SEED = 0x5EED0004
 BATCH_SEED = 0x5EED0002
@@ -59,3 +59,23 @@ def serve_trace(n: int, seed: int = SERVE_SEED, min_side: int = 256, max_side: i
         kind = "i32" if rng.random() < i32_frac else "u8"
         out.append((side, kind, seed + 1 + i))
     return out
+
+
+def maze_levels_images(n: int = 16384, seed: int = MAZE_SEED, lo: int = 150, hi: int = 220):
+    """A many-valued maze: the maze of gen_maze_u8 (6-px cells, 3-px corridors,
+    seed `seed`) with every corridor pixel at an intensity uniform in lo..hi
+    instead of 200, then integer noise uniform in -8..8 on every pixel
+    (clamped to 0..255), as in the disks construction.  The marker is 0 except
+    a 5x5 block at the entrance at 255 (clamped by the mask).  Its many distinct
+    values keep the level-set path out, and its corridors carry values over
+    paths millions of pixels long.  Returns (marker, mask)."""
+    mask, _ = gen_maze_u8(n, n, seed)
+    rng = np.random.default_rng(seed ^ 0x5A5A)
+    corr = mask == 200
+    out = np.where(corr, rng.integers(lo, hi + 1, size=mask.shape, dtype=np.int16), mask.astype(np.int16))
+    out += rng.integers(-8, 9, size=mask.shape, dtype=np.int16)
+    np.clip(out, 0, 255, out=out)
+    mask = out.astype(np.uint8)
+    marker = np.zeros_like(mask)
+    marker[:5, :5] = 255
+    return marker, mask

@@ -94,15 +94,18 @@ nsx_status nsx_ctx_read_timing(nsx_ctx* ctx, double* ms, int64_t* launches, int 
  * (default 3, 0 = never).  Long thin propagation paths (mazes, binary-like
  * masks) then cost a few labellings instead of one round per tile crossing.
  * No reference counterpart. */
-/* NSX_TUNE_ENGINE: NSX_ENGINE_ROUNDS (default) computes every
- * reconstruction by propagation rounds (and the level-set path above);
- * NSX_ENGINE_TREE computes u8 imreconstruct and gray imfillHoles through the
- * mask's component tree instead: a fixed number of launches whatever the image
- * holds, so its time does not grow with the distances values travel (long
- * many-valued paths), at a higher cost on ordinary images.  Other types and
- * modes always use rounds.  Results are the same. */
+/* NSX_TUNE_ENGINE chooses how u8 imreconstruct and gray imfillHoles run:
+ * NSX_ENGINE_ROUNDS: propagation rounds (and the level-set path above);
+ * NSX_ENGINE_TREE: through the mask's component tree, a fixed number of
+ * launches whatever the image holds, so its time does not grow with the
+ * distances values travel (long many-valued paths), at a higher cost on
+ * ordinary images;
+ * NSX_ENGINE_AUTO (default): rounds, switching to the tree when a
+ * reconstruction passes 256 rounds, and starting with the tree when the last
+ * reconstruction of the same kind and size did.
+ * Other types and modes always use rounds.  Results are the same. */
 enum { NSX_TUNE_VISIT_SWEEPS = 0, NSX_TUNE_LEVELSET_LEVELS = 1, NSX_TUNE_ENGINE = 2, NSX_TUNE_NPARAMS = 3 };
-enum { NSX_ENGINE_ROUNDS = 0, NSX_ENGINE_TREE = 1 };
+enum { NSX_ENGINE_ROUNDS = 0, NSX_ENGINE_TREE = 1, NSX_ENGINE_AUTO = 2 };
 nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value);
 nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value);
 

@@ -79,3 +79,30 @@ def test_mtree_disks_match_rounds(nh, orc, gpu):
     finally:
         auto.close()
         rounds.close()
+
+
+def test_auto_engine_switches_on_long_paths(nh, orc, gpu):
+    """NSX_ENGINE_AUTO (the default) on a many-valued maze (1024^2, about 1500
+    propagation rounds): the first call passes 256 rounds and finishes through
+    the component tree, the next ones start with it; every result equals the
+    oracle, for imreconstruct, recon_label and the chain's fill."""
+    import torch
+    from nscale_hip.workloads import maze_levels_images
+    marker, mask = maze_levels_images(1024)
+    want = orc.imreconstruct(marker, mask, 4)
+    ctx = nh.Context(0)
+    try:
+        assert ctx.get_tuning(nh._lib.NSX_TUNE_ENGINE) == nh._lib.NSX_ENGINE_AUTO
+        dm, dk = torch.from_numpy(marker).cuda(), torch.from_numpy(mask).cuda()
+        for rep in range(3):
+            got = nh.imreconstruct(dm, dk, 4, ctx=ctx)
+            assert np.array_equal(got.cpu().numpy(), want), f"call {rep}"
+        r, lab, n = nh.recon_label(dm, dk, 4, 100.0, 4, ctx=ctx)
+        assert np.array_equal(r.cpu().numpy(), want)
+        assert n == 1 and np.array_equal(lab.cpu().numpy(), (mask > 100).astype(np.int32))
+        for rep in range(2):
+            rr, ff, ll, nn = nh.segment_chain(dm, dk, 4, 4, 100.0, 4, 11, 1 << 30, ctx=ctx)
+            assert np.array_equal(rr.cpu().numpy(), want)
+            assert np.array_equal(ff.cpu().numpy(), orc.fill_holes(want, False, 4)), f"chain {rep}"
+    finally:
+        ctx.close()
