@@ -55,7 +55,8 @@ def parse():
     p.add_argument("--tiles-f32", type=int, default=32)
     p.add_argument("--tune-reps", type=int, default=2,
                    help="batch: graph replays per setting tried by the autotuner (0: no autotuning)")
-    p.add_argument("--check-tiles", type=int, default=64, help="batch tiles also checked against the oracle")
+    p.add_argument("--check-tiles", type=int, default=288,
+                   help="batch: tiles run by the CPU baseline (all host threads) and checked against the oracle")
     p.add_argument("--maze-size", type=int, default=16384, help="maze: side of the maze image")
     p.add_argument("--maze-cpu-size", type=int, default=16384,
                    help="maze: side of the maze the CPU baseline runs (default: the whole workload)")
@@ -138,6 +139,7 @@ def bench_batch(args):
     t_batch = (time.perf_counter() - t0) / args.steps
     counts = b.counts()
     same = all(torch.equal(r, r2) and torch.equal(lab, l2) for _, _, r, lab, r2, l2, _ in dtiles) and counts == n_tile
+    roof = batch_roofline(nh, dtiles, args)
     cpu, exact = (None, None) if args.no_cpu else cpu_baseline_batch(tiles, dtiles, counts, args)
     if exact is False:
         sys.exit(1)
@@ -154,42 +156,138 @@ def bench_batch(args):
         "autotune": {"visit_pairs_per_group": pairs, "s": round(t_tune, 3)},
         "config": {"workload": f"{args.tiles_u8} u8 + {args.tiles_f32} f32 tiles, sides 512-4096 step 16: "
                                "recon8 -> >128 (f32: >128/255) -> bwlabel8", "pixels": total_px,
-                   "tiles": len(tiles)},
-        "roofline": None,
+                   "tiles": len(tiles), "step_alg_bytes": batch_alg_bytes(tiles),
+                   "step_hbm_frac": round(batch_alg_bytes(tiles) / t_batch / (HBM_PEAK_GBS * 1e9), 5)},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     b.close()
     print(json.dumps(out))
 
 
+def batch_alg_bytes(tiles) -> int:
+    """Algorithmic bytes of one batch step: per u8 tile 8 B/px (marker, mask in,
+    recon out: 3; thresholded labelling: 1 in, 4 out), per f32 tile 16 B/px
+    (12 + 4: the labelling reads the f32 reconstruction)."""
+    return int(sum(m.size * (8 if m.dtype.itemsize == 1 else 16) for _, m, _ in tiles))
+
+
+def batch_roofline(nh, dtiles, args) -> dict | None:
+    """Roofline of the batch's dominant kernel kind.  Graph replays record no
+    per-kernel events, so the same tiles run as an eager batch (finalized
+    without a graph: the same launches) with HIP events on its private
+    context; its statistics give the tile visits of each group's propagation
+    rounds.  Algorithmic bytes per launch: reconstruction rounds 3 B (u8) or
+    12 B (f32) per visited tile pixel (62 x 62), the first pass that per pixel
+    of every tile, labelling local 1 B (u8) or 4 B (f32) per pixel, output 5 B
+    or 8 B per pixel."""
+    import torch
+    be = nh.Batch(nh.Context(0))
+    u8px = f32px = 0
+    for dm, dk, r, lab, _, _, thr in dtiles:
+        be.add(dm, dk, r, lab, 8, thr, 8)
+        if dk.dtype == torch.uint8:
+            u8px += dk.numel()
+        else:
+            f32px += dk.numel()
+    be.finalize(use_graph=False)
+    bctx = be.context()
+    steps = max(2, args.steps // 2)
+    be.run()
+    be.counts()
+    bctx.set_timing(True)
+    for _ in range(steps):
+        be.run()
+    torch.cuda.synchronize()
+    kt = bctx.read_timing()
+    bctx.set_timing(False)
+    vis = [bctx.recon_stats(g)["tile_visits"] for g in (0, 1)]
+    be.close()
+    alg_step = {
+        "recon_first": 3 * u8px + 12 * f32px,
+        "recon_round": (3 * vis[0] + 12 * vis[1]) * TILE_PX,
+        "ccl_local": u8px + 4 * f32px,
+        "ccl_final": 5 * u8px + 8 * f32px,
+        "ccl_merge": 0,
+        "ccl_small": 0,
+    }
+    kind = max(kt, key=lambda k: kt[k][0])
+    ms_step = kt[kind][0] / steps
+    launches_step = kt[kind][1] / steps
+    if not launches_step:
+        return None
+    alg_launch = alg_step[kind] / launches_step
+    avg_launch_s = ms_step / launches_step / 1e3
+    achieved = alg_launch / avg_launch_s / 1e9
+    log("batch kernel time per step (ms, eager): " +
+        ", ".join(f"{k}={v[0] / steps:.3f} ({v[1] / steps:.1f} launches)" for k, v in kt.items()))
+    return {"bound": "hbm", "kernel": kind, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "alg_bytes_per_launch": int(alg_launch),
+            "avg_launch_us": round(avg_launch_s * 1e6, 3), "launches_per_step": round(launches_step, 2),
+            "per_kind_ms_per_step": {k: round(v[0] / steps, 4) for k, v in kt.items()},
+            "timed_as": "eager batch (same launches as the graph) with HIP events"}
+
+
+def host_threads() -> int:
+    """Host threads for the multi-core CPU baselines: the CPUs this process may
+    run on, at most 16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline_batch(tiles, dtiles, counts, args):
-    """The oracle's imreconstruct + bwlabel2, single-threaded, on check_tiles of
-    the batch (half u8, half f32: a bounded sample); its outputs double as the
-    exactness check of the batched results.  Returns (cpu_baseline, exact)."""
+    """The oracle's imreconstruct + bwlabel2 on every tile of the batch, one
+    tile per host thread on all host CPUs (the oracle's C calls release the
+    GIL); its outputs are also the exactness check of every batched tile.
+    Returns (cpu_baseline, exact)."""
+    import concurrent.futures as cf
+
     import numpy as np
 
     from oracle import oracle as orc
-    k = max(0, args.check_tiles)
-    idx = list(range(min(k // 2, args.tiles_u8))) + \
-        list(range(args.tiles_u8, args.tiles_u8 + min(k - k // 2, args.tiles_f32)))
-    exact, px, dt = True, 0, 0.0
-    for i in idx:
+    idx = list(range(len(tiles)))[: max(0, args.check_tiles)] if args.check_tiles < len(tiles) else \
+        list(range(len(tiles)))
+    if not idx:
+        return None, None
+
+    def one(i):
         marker, mask, thr = tiles[i]
-        t0 = time.perf_counter()
         wr = orc.imreconstruct(marker, mask, 8)
         bw = np.where(wr > (np.float32(thr) if wr.dtype == np.float32 else thr), 255, 0).astype(np.uint8)
         wl, wn = orc.bwlabel2(bw, 8, relab=True)
-        dt += time.perf_counter() - t0
-        px += mask.size
+        return i, wr, wl, wn
+
+    threads = host_threads()
+    exact, px = True, 0
+    t0 = time.perf_counter()
+    results = []
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        for res in ex.map(one, idx):
+            results.append(res)
+    dt = time.perf_counter() - t0
+    for i, wr, wl, wn in results:
+        px += tiles[i][1].size
         _, _, r, lab, _, _, _ = dtiles[i]
         if not (np.array_equal(r.cpu().numpy(), wr) and np.array_equal(lab.cpu().numpy(), wl) and counts[i] == wn):
             exact = False
             log(f"tile {i} differs from the oracle")
-    if not idx:
-        return None, None
-    cpu = {"value": round(px / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
-           "sample": f"{len(idx)} tiles ({px / 1e6:.1f} Mpx) through imreconstruct + bwlabel2, single thread, "
-                     f"{dt:.1f} s", "host_cpus": os.cpu_count()}
+    cpu = {"value": round(px / dt / 1e6, 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
+           "sample": f"{len(idx)} tiles ({px / 1e6:.1f} Mpx) through imreconstruct + bwlabel2, one tile per thread "
+                     f"on {threads} threads, {dt:.1f} s wall; every GPU tile checked against it",
+           "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
     return cpu, exact
 
 
@@ -392,7 +490,7 @@ def cpu_baseline_serve(reqs, trace, digests, max_side):
             log(f"request {i} ({side}x{side} {kind}) differs from the oracle")
     cpu = {"value": round(px / dt / 1e6, 3) if dt > 0 else None, "unit": "Mpix/s", "cores": 1, "kind": "port",
            "sample": f"{n} requests of side <= {max_side} ({px / 1e6:.1f} Mpx) through the chain, single thread, "
-                     f"{dt:.1f} s", "host_cpus": os.cpu_count()}
+                     f"{dt:.1f} s", "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
     return cpu, exact, n
 
 
@@ -564,7 +662,7 @@ def bench_maze_levels(args):
             sys.exit(1)
         cpu = {"value": round(cn * cn / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
                "sample": f"one {cn}x{cn} many-valued maze of the same construction (recon4 + bwlabel4), single "
-                         f"thread, {dt:.1f} s; GPU output on it identical", "host_cpus": os.cpu_count()}
+                         f"thread, {dt:.1f} s; GPU output on it identical", "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
     ctx.close()
     out = {
         "metric": "Mpix/s many-valued maze reconstruct+CCL @16384^2 u8 bit-exact",
@@ -721,7 +819,7 @@ def cpu_baseline_maze(nh, n, ctx):
         sys.exit(1)
     return {"value": round(n * n / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
             "sample": f"one {n}x{n} maze of the same construction (recon4 + bwlabel4), single thread, {dt:.1f} s; "
-                      "GPU output on it identical", "host_cpus": os.cpu_count()}
+                      "GPU output on it identical", "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
 
 
 TILE_PX = 62 * 62   # interior pixels of one reconstruction tile visit
@@ -793,7 +891,7 @@ def cpu_baseline(marker, mask, runs: int, gpu_first) -> tuple:
     dt = time.perf_counter() - t0
     return {"value": round(runs * mask.size / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
             "sample": f"{runs} runs of the full {mask.shape[0]}x{mask.shape[1]} chain, single thread, "
-                      f"{dt:.1f} s", "host_cpus": os.cpu_count()}, exact
+                      f"{dt:.1f} s", "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}, exact
 
 
 def main():

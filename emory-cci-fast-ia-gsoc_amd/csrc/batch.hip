@@ -412,6 +412,8 @@ nsx_status nsx_batch_autotune(nsx_batch* b, void* stream, int reps, int32_t* pai
     return NSX_OK;
 }
 
+nsx_ctx* nsx_batch_context(nsx_batch* b) { return b ? b->ws : nullptr; }
+
 void nsx_batch_destroy(nsx_batch* b) {
     if (!b) return;
     (void)hipDeviceSynchronize();

@@ -114,9 +114,9 @@ class Context:
         self.device = device
 
     def close(self) -> None:
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and not getattr(self, "_borrowed", False):
             self.lib.nsx_ctx_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):  # pragma: no cover
         try:
@@ -447,6 +447,16 @@ class Batch:
         pairs = (C.c_int32 * 4)()
         self.ctx.check(self.lib.nsx_batch_autotune(self.h, s, int(reps), pairs, 4))
         return [int(v) for v in pairs if v > 0]
+
+    def context(self) -> "Context":
+        """The batch's private context (timing of eager runs, statistics); a
+        borrowed handle, valid while the batch lives."""
+        c = Context.__new__(Context)
+        c.lib = self.lib
+        c.h = C.c_void_p(self.lib.nsx_batch_context(self.h))
+        c.device = self.ctx.device
+        c._borrowed = True
+        return c
 
     def counts(self) -> list:
         arr = (C.c_int32 * max(1, self.n))()

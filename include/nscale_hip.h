@@ -319,6 +319,12 @@ nsx_status nsx_batch_counts(nsx_batch* batch, int32_t* counts, int n);
  * (reps + 2) times.  No counterpart in the reference (its batched path has
  * fixed launch shapes). */
 nsx_status nsx_batch_autotune(nsx_batch* batch, void* stream, int reps, int32_t* pairs, int npairs);
+/* The batch's private context (owned by the batch; valid until
+ * nsx_batch_destroy): its kernel timing (nsx_ctx_set_timing /
+ * nsx_ctx_read_timing, recorded by eager runs of a batch finalized without a
+ * graph) and its reconstruction statistics (nsx_last_recon_stats, stage g =
+ * tile group g mod 2).  No reference counterpart. */
+nsx_ctx* nsx_batch_context(nsx_batch* batch);
 void nsx_batch_destroy(nsx_batch* batch);
 
 /* ---- serving loop --------------------------------------------------------- */
