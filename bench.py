@@ -64,10 +64,17 @@ def parse():
                    help="maze_levels: side of the many-valued maze the CPU baseline runs")
     p.add_argument("--requests", type=int, default=200, help="serve: requests in the trace")
     p.add_argument("--slots", type=int, default=4, help="serve: worker slots of the server")
+    p.add_argument("--serve-sides", default="256,4096", help="serve: smallest,largest request side")
+    p.add_argument("--serve-reps", type=int, default=5, help="serve: saturation bursts per multi-slot server")
+    p.add_argument("--share-max-px", type=int, default=0,
+                   help="serve: largest request that joins a shared set (0: the server's default)")
+    p.add_argument("--share-set-px", type=int, default=0, help="serve: pixels per shared set (0: default)")
+    p.add_argument("--max-batch", type=int, default=16,
+                   help="serve: requests a slot takes at once (those of one kind share launches)")
     p.add_argument("--load", type=float, default=1.0,
                    help="serve: Poisson arrival rate as a multiple of the sequential capacity")
     p.add_argument("--budget-gib", type=float, default=4.0, help="serve: HBM arena of the server (GiB)")
-    p.add_argument("--check-max-side", type=int, default=1024,
+    p.add_argument("--check-max-side", type=int, default=4096,
                    help="serve: requests up to this side are also checked against the oracle")
     return p.parse_args()
 
@@ -298,7 +305,10 @@ def pct(xs, q):
 
 def bench_serve(args):
     """Serving loop: a trace of requests (log-uniform sides 256..4096, 20% int32)
-    through nsx_server with `slots` workers vs one worker (sequential).
+    through nsx_server with `slots` workers, each taking up to `max_batch`
+    queued requests at once (those of one kind share launches), vs one worker
+    serving one request at a time (sequential) and vs the same slots serving
+    every request on its own (unshared).
 
     Reported value: Mpix/s of the server with every request queued at once
     (saturation).  Latency: Poisson arrivals at `load` x the sequential
@@ -322,7 +332,8 @@ def bench_serve(args):
         return t.numpy()
 
     t0 = time.perf_counter()
-    trace = serve_trace(args.requests, SERVE_SEED)
+    lo_side, hi_side = (int(v) for v in args.serve_sides.split(","))
+    trace = serve_trace(args.requests, SERVE_SEED, lo_side, hi_side)
     reqs = []
     for side, kind, seed in trace:
         marker, mask, thr = request_images(side, kind, seed)
@@ -333,8 +344,12 @@ def bench_serve(args):
     log(f"generated {len(reqs)} requests ({total_px / 1e6:.1f} Mpx, "
         f"{sum(k == 'i32' for _, k, _ in trace)} int32) in {time.perf_counter() - t0:.1f} s")
     budget = int(args.budget_gib * (1 << 30))
-    s1 = nh.Server(0, budget, slots=1)
+    # one request at a time; the server as configured; the same slots serving
+    # every request on its own (no shared launches)
+    s1 = nh.Server(0, budget, slots=1, max_batch=1)
     sk = nh.Server(0, budget, slots=args.slots)
+    sk.set_sharing(args.max_batch, args.share_max_px, args.share_set_px)
+    sn = nh.Server(0, budget, slots=args.slots, max_batch=1)
 
     def submit(srv, i):
         marker, mask, thr, lab = reqs[i]
@@ -406,16 +421,27 @@ def bench_serve(args):
     for _ in range(max(1, args.warmup)):
         burst(s1, "warmup-1")
         burst(sk, "warmup-k")
+        burst(sn, "warmup-n")
     t_seq, svc = closed_loop(s1, "sequential")
     t_sat1, _ = burst(s1, "saturation-1")
-    t_satk, lat_satk = burst(sk, f"saturation-{args.slots}")
+    # the two multi-slot servers alternate over a few bursts (the order flips
+    # each time) and each keeps its median, so that drift over the run does
+    # not favour either
+    tn, tk = [], []
+    for rep in range(args.serve_reps):
+        order = ((sn, tn, "unshared"), (sk, tk, "shared"))
+        for srv, acc, tag in (order if rep % 2 == 0 else order[::-1]):
+            acc.append(burst(srv, f"saturation-{args.slots}-{tag}")[0])
+    t_satn, t_satk = statistics.median(tn), statistics.median(tk)
     lam = args.load * len(reqs) / t_seq
     gaps = np.random.default_rng(SERVE_SEED ^ 0xA5A5).exponential(1.0 / lam, len(reqs))
     t_p1, lat_p1 = poisson(s1, gaps, "poisson-1")
     t_pk, lat_pk = poisson(sk, gaps, f"poisson-{args.slots}")
+    roof = serve_roofline(sk, burst, reqs, args.slots)
     stk = sk.stats()
     s1.close()
     sk.close()
+    sn.close()
 
     cpu, exact, checked = (None, None, 0) if args.no_cpu else cpu_baseline_serve(reqs, trace, digests,
                                                                                    args.check_max_side)
@@ -441,8 +467,11 @@ def bench_serve(args):
         "results_same_alone_and_in_mix": True,
         "oracle_checked_requests": checked,
         "saturation": {"server_mpix_s": round(mp / t_satk, 2), "sequential_mpix_s": round(mp / t_sat1, 2),
+                       "unshared_mpix_s": round(mp / t_satn, 2),
                        "closed_loop_sequential_mpix_s": round(mp / t_seq, 2),
-                       "speedup": round(t_sat1 / t_satk, 3)},
+                       "speedup": round(t_sat1 / t_satk, 3), "speedup_vs_unshared": round(t_satn / t_satk, 3),
+                       "bursts_ms": {"shared": [round(t * 1e3, 2) for t in tk],
+                                     "unshared": [round(t * 1e3, 2) for t in tn]}},
         "poisson": {"lambda_req_s": round(lam, 2), "load_vs_sequential_capacity": args.load,
                     "server": {"p50_ms": round(statistics.median(lat_pk), 3), "p99_ms": round(pct(lat_pk, 0.99), 3),
                                "mpix_s": round(mp / t_pk, 2)},
@@ -450,48 +479,111 @@ def bench_serve(args):
                                    "p99_ms": round(pct(lat_p1, 0.99), 3), "mpix_s": round(mp / t_p1, 2)}},
         "service_ms_alone": {"p50": round(statistics.median(svc), 3), "p99": round(pct(svc, 0.99), 3)},
         "server": stk,
-        "config": {"workload": f"{len(reqs)} requests, square sides log-uniform 256-4096, 80% u8 (>128) / 20% "
+        "config": {"workload": f"{len(reqs)} requests, square sides log-uniform {lo_side}-{hi_side}, 80% u8 (>128) / 20% "
                                "int32 (x257, >32768): recon8 -> fillholes4 -> threshold -> bwlabel8 -> "
                                "bwareaopen[11,1000) -> relabel; pinned host in/out",
-                   "pixels": total_px, "slots": args.slots, "hbm_budget_bytes": budget},
-        "roofline": None,
+                   "pixels": total_px, "slots": args.slots, "max_batch": args.max_batch,
+                   "share_max_request_px": stk["share_max_request_px"], "share_max_set_px": stk["share_max_set_px"],
+                   "hbm_budget_bytes": budget, "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
 
+def serve_roofline(srv, burst, reqs, slots) -> dict | None:
+    """Roofline of the serving loop's dominant kernel kind.  One more
+    saturation burst runs with HIP-event timing on every slot's context (events
+    add gaps between launches, so the timed bursts run without them); the
+    slots' kernel times and tile counts are summed.  Kernels of different slots
+    overlap on the device, so a launch's duration here is its duration while
+    sharing the GPU.  Algorithmic bytes: reconstruction 3 B per pixel per
+    element byte (marker, mask in; state out) and hole filling 2 B (image in,
+    state out), over the tiles (62 x 62) each first pass and round processed;
+    labelling local: the fill read once (1 or 4 B/px); label output: that read
+    plus 4 B written."""
+    ctxs = [srv.context(i) for i in range(slots)]
+    for c in ctxs:
+        c.set_timing(True)
+    burst(srv, "kernel-timing")
+    kt, tiles = {}, {}
+    for c in ctxs:
+        for k, (ms, n) in c.read_timing().items():
+            a = kt.get(k, (0.0, 0))
+            kt[k] = (a[0] + ms, a[1] + n)
+        for k, (f, r) in c.read_tile_counts().items():
+            a = tiles.get(k, (0, 0))
+            tiles[k] = (a[0] + f, a[1] + r)
+        c.set_timing(False)
+    per_px = {"recon": 3, "fill": 2, "binary": 3, "fillbin": 2}
+    es = {"u8": 1, "i32": 4, "f32": 4}
+    first_b = sum(f * TILE_PX * per_px[m] * es[d] for (m, d), (f, _) in tiles.items())
+    round_b = sum(r * TILE_PX * per_px[m] * es[d] for (m, d), (_, r) in tiles.items())
+    in_b = sum(mask.size * mask.itemsize for _, mask, _, _ in reqs)
+    alg = {"recon_first": first_b, "recon_round": round_b, "ccl_local": in_b, "ccl_final": in_b + 4 * sum(
+        mask.size for _, mask, _, _ in reqs), "ccl_merge": 0, "ccl_small": 0}
+    kind = max(kt, key=lambda k: kt[k][0])
+    ms, n = kt[kind]
+    if not n or not alg[kind]:
+        return None
+    alg_launch = alg[kind] / n
+    avg_s = ms / n / 1e3
+    achieved = alg_launch / avg_s / 1e9
+    log("serve kernel time per burst (ms, summed over slots): " +
+        ", ".join(f"{k}={v[0]:.3f} ({v[1]} launches)" for k, v in kt.items()))
+    log(f"serve tile counts (first pass, rounds): {tiles}")
+    return {"bound": "hbm", "kernel": kind, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "alg_bytes_per_launch": int(alg_launch),
+            "avg_launch_us": round(avg_s * 1e6, 3), "launches_per_burst": n,
+            "per_kind_ms_per_burst": {k: round(v[0], 4) for k, v in kt.items()},
+            "timed_as": f"one saturation burst with HIP events on all {slots} slot streams (kernels of different "
+                        "slots overlap)"}
+
+
 def cpu_baseline_serve(reqs, trace, digests, max_side):
-    """The oracle's chain, single-threaded, on the trace's requests of side <=
-    max_side (a bounded sample); its outputs double as the exactness check of
-    the served labels.  Returns (cpu_baseline, exact, requests checked)."""
+    """The oracle's chain on the trace's requests of side <= max_side (a
+    bounded sample), one request per host thread on all host CPUs (the
+    oracle's C calls release the GIL); its outputs double as the exactness
+    check of the served labels.  Returns (cpu_baseline, exact, requests
+    checked)."""
+    import concurrent.futures as cf
     import zlib
 
     import numpy as np
 
     from oracle import oracle as orc
-    exact, n, px, dt = True, 0, 0, 0.0
-    for i, (side, kind, _) in enumerate(trace):
-        if side > max_side:
-            continue
-        marker, mask, thr, lab = reqs[i]
-        t0 = time.perf_counter()
-        if kind == "u8":
+    idx = [i for i, (side, _, _) in enumerate(trace) if side <= max_side]
+
+    def one(i):
+        marker, mask, thr, _ = reqs[i]
+        if trace[i][1] == "u8":
             _, _, want, wn = orc.chain_u8(marker, mask, 8, 4, int(thr), 8, 11, 1000)
         else:
             r = orc.imreconstruct(marker, mask, 8)
             f = orc.fill_holes(r, False, 4)
             b = np.where(f > thr, 255, 0).astype(np.uint8)
             want, wn = orc.relabel(orc.bwareaopen2(b, False, False, 11, 1000, 8)[0], -1)
-        dt += time.perf_counter() - t0
-        px += mask.size
-        n += 1
+        return i, want, wn
+
+    threads = host_threads()
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        results = list(ex.map(one, idx))
+    dt = time.perf_counter() - t0
+    exact, px = True, 0
+    for i, want, wn in results:
+        lab = reqs[i][3]
+        px += lab.size
         if (zlib.crc32(np.ascontiguousarray(want)), wn) != digests[i] or not np.array_equal(lab, want):
             exact = False
-            log(f"request {i} ({side}x{side} {kind}) differs from the oracle")
-    cpu = {"value": round(px / dt / 1e6, 3) if dt > 0 else None, "unit": "Mpix/s", "cores": 1, "kind": "port",
-           "sample": f"{n} requests of side <= {max_side} ({px / 1e6:.1f} Mpx) through the chain, single thread, "
-                     f"{dt:.1f} s", "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
-    return cpu, exact, n
+            log(f"request {i} ({trace[i][0]}x{trace[i][0]} {trace[i][1]}) differs from the oracle")
+    cpu = {"value": round(px / dt / 1e6, 3) if dt > 0 else None, "unit": "Mpix/s", "cores": threads,
+           "kind": "port",
+           "sample": f"{len(idx)} requests of side <= {max_side} ({px / 1e6:.1f} Mpx) through the chain, one "
+                     f"request per thread on {threads} threads, {dt:.1f} s wall; every one checked against the "
+                     "served labels",
+           "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
+    return cpu, exact, len(idx)
 
 
 def bench_maze(args):
@@ -900,6 +992,13 @@ def main():
         print("bench.py: this hot path runs on one GPU; multi-GPU sharding is a design note only "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "serve":
+        # one hardware queue per stream (the slots of three servers, torch's
+        # and the default context's): with HIP's default of 4 queues per
+        # process, streams created later share queues and serialise, which
+        # would favour whichever server got its own queues
+        if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+            os.environ["GPU_MAX_HW_QUEUES"] = "16"
     import numpy as np
     import torch
 

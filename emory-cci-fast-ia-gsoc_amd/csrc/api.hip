@@ -10,6 +10,11 @@
 
 using namespace nsx;
 
+// d_stats: per-run statistics and status words (int64 0..31), then the
+// cumulative tile counters (int64 kCumWord..).
+constexpr size_t kStatsBytes = 512;
+static_assert(sizeof(int64_t) * (nsx_ctx::kCumWord + nsx_ctx::kCumKinds) <= kStatsBytes, "stats words");
+
 nsx_status nsx_ctx::ws(int slot, size_t bytes, void** out) {
     if (slot < 0 || slot >= WS_NSLOTS) return fail(NSX_ERR_INVALID, "bad workspace slot");
     if ((int)bufs.size() < WS_NSLOTS) bufs.resize(WS_NSLOTS);
@@ -228,13 +233,13 @@ nsx_status nsx_ctx_create(nsx_ctx** out, int device, size_t hbm_budget_bytes) {
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         c->cu_count = cus;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc((void**)&c->d_stats, 256) != hipSuccess ||
+        hipMalloc((void**)&c->d_stats, kStatsBytes) != hipSuccess ||
         hipHostMalloc((void**)&c->h_pinned, 64 * sizeof(int32_t)) != hipSuccess) {
         nsx_ctx_destroy(c);
         return NSX_ERR_HIP;
     }
-    hipMemset(c->d_stats, 0, 256);
-    c->held = 256;
+    hipMemset(c->d_stats, 0, kStatsBytes);
+    c->held = kStatsBytes;
     // the allocation-free entry point's scratch array: its device address,
     // once per device, with the device current
     if (recon_noalloc_init(device) != NSX_OK) {
@@ -332,7 +337,21 @@ nsx_status nsx_ctx_set_timing(nsx_ctx* ctx, int enable) {
     NSX_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->collect_timing();
     for (int k = 0; k < TK_NKINDS; ++k) { ctx->t_ms[k] = 0.0; ctx->t_n[k] = 0; }
+    for (int k = 0; k < nsx_ctx::kCumKinds; ++k) ctx->first_tiles[k] = 0;
+    NSX_HIP_TRY(ctx, hipMemset(ctx->d_stats + nsx_ctx::kCumWord, 0, sizeof(int64_t) * nsx_ctx::kCumKinds));
     ctx->timing = enable != 0;
+    return NSX_OK;
+}
+
+nsx_status nsx_ctx_read_tile_counts(nsx_ctx* ctx, int64_t* first_pass, int64_t* rounds, int n) {
+    if (!ctx || n < 0) return NSX_ERR_INVALID;
+    int64_t h[nsx_ctx::kCumKinds];
+    NSX_HIP_TRY(ctx, hipDeviceSynchronize());
+    NSX_HIP_TRY(ctx, hipMemcpy(h, ctx->d_stats + nsx_ctx::kCumWord, sizeof(h), hipMemcpyDeviceToHost));
+    for (int k = 0; k < n && k < nsx_ctx::kCumKinds; ++k) {
+        if (first_pass) first_pass[k] = ctx->first_tiles[k];
+        if (rounds) rounds[k] = h[k];
+    }
     return NSX_OK;
 }
 

@@ -64,7 +64,12 @@ struct nsx_ctx {
     size_t held = 0;         // bytes of workspace held
     std::string last_error;
     std::vector<nsx::DevBuf> bufs;   // indexed by workspace slot
-    int64_t* d_stats = nullptr;      // [rounds, tile passes, ntiles, spare]
+    int64_t* d_stats = nullptr;      // [rounds, tile passes, ntiles, spare] ..., see kCumWord
+    // Tiles processed by reconstructions since timing was last enabled, per
+    // (mode, dtype) at index mode * 3 + dtype: first passes counted here on
+    // the host, propagation-round visits in d_stats[kCumWord + index].
+    static constexpr int kCumWord = 32, kCumKinds = 12;
+    int64_t first_tiles[kCumKinds] = {};
     int32_t* h_pinned = nullptr;     // small pinned readback area
     int cu_count = 256;
 
@@ -139,6 +144,7 @@ enum WsSlot {
     WS_MT2_TCOUNT,
     WS_MT2_WALK,
     WS_MT2_CTL,
+    WS_SERVE_TABLES,     // serving: descriptors, tile tables and counts of a shared launch set
     WS_NSLOTS
 };
 

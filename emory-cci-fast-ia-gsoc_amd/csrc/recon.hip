@@ -400,6 +400,7 @@ struct ReconArgs {
     int32_t* list1;
     int32_t* counts;           // [3][NSEG] rotating round counters, CSTRIDE ints apart
     int64_t* stats;            // [0] rounds with work [1] tile visits after round 0 [2] tiles
+    unsigned long long* cum;   // tile visits summed over runs (may be null)
     int visit_sweeps;          // sweep pairs per visit before re-queueing (1..MAX_IT)
     int first_list;            // round 0 takes its tiles from the list (fill: the border tiles)
 };
@@ -699,6 +700,7 @@ void recon_round_kernel(ReconArgs A, int round) {
             if (n > 0) {
                 A.stats[0] += 1;
                 A.stats[1] += n;
+                if (A.cum) atomicAdd(A.cum, (unsigned long long)n);
             }
         }
     }
@@ -1085,6 +1087,7 @@ void recon_quad_kernel(ReconArgs A, int round) {
             if (n > 0) {
                 A.stats[0] += 1;
                 A.stats[1] += n;
+                if (A.cum) atomicAdd(A.cum, (unsigned long long)n);
             }
         }
     }
@@ -1264,6 +1267,8 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     A.list1 = (int32_t*)lists + (size_t)NSEG * A.ntiles;
     A.counts = (int32_t*)counts;
     A.stats = B.stats ? B.stats : ctx->d_stats;
+    const int cum_idx = mode * 3 + dtype;
+    A.cum = (unsigned long long*)(ctx->d_stats + nsx_ctx::kCumWord + cum_idx);
     // four-wave hole-filling tiles run one pair more: their pairs are shorter
     // (4096^2 chain: 36 fill rounds instead of 41, 0.8 % faster)
     const bool quad = (mode == RM_FILL || mode == RM_FILLBIN) && conn == 4;
@@ -1297,6 +1302,7 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
                        A.list0, border_start ? D0.ntx : 0, D0.nty, F);
     const int h0 = ctx->tic(TK_RECON_FIRST, s);
     first(A, 0, border_start ? std::min(R->grid, border_tiles(D0.ntx, D0.nty)) : A.ntiles, s);
+    ctx->first_tiles[cum_idx] += border_start ? border_tiles(D0.ntx, D0.nty) : A.ntiles;
     ctx->toc(h0, s);
     // Rounds the same-sized problem needed last time, launched without a host check.
     auto it = ctx->round_hint.find(R->key);

@@ -149,6 +149,21 @@ class Context:
         self.check(self.lib.nsx_ctx_read_timing(self.h, ms, n, k))
         return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(self.TIMED_KINDS)}
 
+    def read_tile_counts(self) -> dict:
+        """Tiles processed by propagation reconstructions since timing was last
+        switched: {(mode, dtype): (first_pass_tiles, round_tile_visits)} for the
+        kinds with any, mode in ("recon", "binary", "fill", "fillbin"), dtype in
+        ("u8", "i32", "f32")."""
+        first = (C.c_int64 * 12)()
+        rounds = (C.c_int64 * 12)()
+        self.check(self.lib.nsx_ctx_read_tile_counts(self.h, first, rounds, 12))
+        out = {}
+        for k in range(12):
+            if first[k] or rounds[k]:
+                out[(("recon", "binary", "fill", "fillbin")[k // 3], ("u8", "i32", "f32")[k % 3])] = \
+                    (int(first[k]), int(rounds[k]))
+        return out
+
     def set_tuning(self, param: int, value: int) -> None:
         self.check(self.lib.nsx_ctx_set_tuning(self.h, param, value))
 
@@ -481,14 +496,27 @@ class Server:
     the work of admitted requests overlaps on the GPU.  Each request computes
     segment_chain(marker, mask, ...) on its own buffers."""
 
-    def __init__(self, device: int = 0, hbm_budget_bytes: int = 1 << 30, slots: int = 4):
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 1 << 30, slots: int = 4,
+                 max_batch: "int | None" = None):
         self.lib = _lib.load()
         h = C.c_void_p()
         st = self.lib.nsx_server_create(device, hbm_budget_bytes, slots, C.byref(h))
         if st != _lib.NSX_OK:
             raise NsxError(st, f"nsx_server_create(device={device}, budget={hbm_budget_bytes}, slots={slots})")
         self.h = h
+        self.device = device
         self._keep = {}   # ticket -> buffers the worker may still read or write
+        if max_batch is not None:
+            self.set_sharing(max_batch)
+
+    def set_sharing(self, max_requests: int = 16, max_request_pixels: int = 0, max_set_pixels: int = 0) -> None:
+        """Sharing policy: an idle slot takes the oldest request and, when it
+        has at most max_request_pixels pixels, the small ones queued behind it
+        (at most max_requests, max_set_pixels in all); those of one kind share
+        their launches.  max_requests=1 serves every request on its own; a
+        pixel limit of 0 keeps the default (2^20, 2^22)."""
+        self._check(self.lib.nsx_server_set_sharing(self.h, int(max_requests), int(max_request_pixels),
+                                                    int(max_set_pixels)))
 
     def _check(self, st: int) -> None:
         if st != _lib.NSX_OK:
@@ -519,12 +547,31 @@ class Server:
         self._check(st)
         return bufs[2], bufs[3], bufs[4], int(n.value), (tm[0], tm[1], tm[2])
 
+    def context(self, slot: int) -> "Context":
+        """Worker slot `slot`'s context (timing, tile counts); a borrowed
+        handle, valid while the server lives.  Switch its timing only while no
+        request is in flight."""
+        h = self.lib.nsx_server_context(self.h, int(slot))
+        if not h:
+            raise NsxError(_lib.NSX_ERR_INVALID, f"no worker slot {slot}")
+        c = Context.__new__(Context)
+        c.lib = self.lib
+        c.h = C.c_void_p(h)
+        c.device = self.device
+        c._borrowed = True
+        return c
+
     def stats(self) -> dict:
         a = (C.c_int64 * 8)()
         self._check(self.lib.nsx_server_stats(self.h, a))
         keys = ("completed", "in_flight", "queued", "peak_in_flight", "arena_bytes", "arena_peak_bytes",
                 "workspace_bytes", "slots")
-        return dict(zip(keys, (int(v) for v in a)))
+        out = dict(zip(keys, (int(v) for v in a)))
+        b = (C.c_int64 * 7)()
+        self._check(self.lib.nsx_server_batch_stats(self.h, b))
+        out.update(zip(("shared_sets", "shared_requests", "largest_set", "max_batch", "shared_short_rounds",
+                        "share_max_request_px", "share_max_set_px"), (int(v) for v in b)))
+        return out
 
     def close(self) -> None:
         if getattr(self, "h", None):

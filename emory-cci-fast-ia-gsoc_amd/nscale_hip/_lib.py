@@ -54,6 +54,7 @@ SIGNATURES = {
     "nsx_last_recon_stats": (C.c_int, [_CTX, C.c_int, P(C.c_int64)]),
     "nsx_ctx_set_timing": (C.c_int, [_CTX, C.c_int]),
     "nsx_ctx_read_timing": (C.c_int, [_CTX, P(C.c_double), P(C.c_int64), C.c_int]),
+    "nsx_ctx_read_tile_counts": (C.c_int, [_CTX, P(C.c_int64), P(C.c_int64), C.c_int]),
     "nsx_ctx_set_tuning": (C.c_int, [_CTX, C.c_int, C.c_int]),
     "nsx_ctx_get_tuning": (C.c_int, [_CTX, C.c_int, P(C.c_int)]),
     "nsx_image_check": (C.c_int, [P(NsxImg), P(C.c_char_p)]),
@@ -92,6 +93,9 @@ SIGNATURES = {
                                     C.c_int, _IMG, _IMG, _IMG, P(C.c_uint64)]),
     "nsx_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, _I32P, P(C.c_double)]),
     "nsx_server_stats": (C.c_int, [C.c_void_p, P(C.c_int64)]),
+    "nsx_server_set_sharing": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_int64]),
+    "nsx_server_batch_stats": (C.c_int, [C.c_void_p, P(C.c_int64)]),
+    "nsx_server_context": (C.c_void_p, [C.c_void_p, C.c_int]),
     "nsx_server_last_error": (C.c_char_p, [C.c_void_p]),
     "nsx_server_destroy": (None, [C.c_void_p]),
     "nsx_splitmix64_next": (C.c_uint64, [P(C.c_uint64)]),

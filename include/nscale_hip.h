@@ -80,6 +80,15 @@ nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int stage, int64_t stats[4]);
  * src/segment/cuda/ccl_uf.cu:28-31). */
 nsx_status nsx_ctx_set_timing(nsx_ctx* ctx, int enable);
 nsx_status nsx_ctx_read_timing(nsx_ctx* ctx, double* ms, int64_t* launches, int nkinds);
+/* Tiles (62 x 62 pixels) processed by the context's propagation
+ * reconstructions since nsx_ctx_set_timing last started or stopped timing
+ * (counted whether timing is on or not): first_pass[k] by first passes,
+ * rounds[k] by propagation rounds, for k = mode * 3 + dtype (mode 0
+ * imreconstruct, 1 imreconstructBinary, 2 grayscale fill, 3 binary fill;
+ * dtype as nsx_dtype), n <= 12 entries.  With the kernel times these give the
+ * algorithmic bytes per launch of a mixed workload.  Synchronises the device.
+ * No reference counterpart. */
+nsx_status nsx_ctx_read_tile_counts(nsx_ctx* ctx, int64_t* first_pass, int64_t* rounds, int n);
 
 /* Tuning parameters of the context's kernels (results never depend on them).
  * NSX_TUNE_VISIT_SWEEPS: sweep pairs (column + row half-pass) a reconstruction
@@ -330,13 +339,18 @@ void nsx_batch_destroy(nsx_batch* batch);
 /* ---- serving loop --------------------------------------------------------- */
 /* Continuous admission of segmentation-chain requests (the nsx_segment_chain
  * pipeline) on one GPU.  `slots` worker threads each own a context and a
- * stream; a request is admitted when a slot is idle and its device buffers fit
- * in a fixed HBM arena of `hbm_budget_bytes` (allocated once; no per-request
- * hipMalloc).  Requests are admitted in arrival order; the work of admitted
- * requests overlaps on the device.  Each request's inputs are copied into its
- * arena block, all stages run there, and the outputs the caller asked for are
- * copied back before the request completes.  Results are those of
- * nsx_segment_chain on the same inputs, however many requests are in flight.
+ * stream; requests are admitted when a slot is idle and their device buffers
+ * fit in a fixed HBM arena of `hbm_budget_bytes` (allocated once; no
+ * per-request hipMalloc).  Requests are admitted in arrival order; an idle
+ * slot takes the oldest one and, when it is small, the small ones queued
+ * behind it, which then share their launches (nsx_server_set_sharing).  The
+ * work of different slots overlaps on the device: give the process at least
+ * slots + 2 hardware queues (environment variable GPU_MAX_HW_QUEUES, read when
+ * HIP initialises; HIP's default is 4), or streams share queues and serialise.
+ * Each request's inputs are copied into its arena block, all stages run
+ * there, and the outputs the caller asked for are copied back before the
+ * request completes.  Results are those of nsx_segment_chain on the same
+ * inputs, however many requests are in flight or share launches.
  * No reference counterpart: the reference's GPU functions are synchronous and
  * not reentrant (global queue state in global_queue.cu:9-22); the request
  * processing matches the chain in HistologicalEntities.cpp:195,219,246,567. */
@@ -356,6 +370,29 @@ nsx_status nsx_server_wait(nsx_server* server, uint64_t ticket, int32_t* ncomp, 
 /* stats (8 entries): completed, in flight, queued, peak in flight, arena bytes,
  * peak arena bytes in use, workspace bytes held by the slot contexts, slots. */
 nsx_status nsx_server_stats(nsx_server* server, int64_t* stats);
+/* Sharing policy.  An idle slot takes the oldest queued request and, when it
+ * has at most max_request_pixels pixels, the requests of at most that size
+ * queued behind it, in order, while they fit in the arena, the set holds at
+ * most max_set_pixels pixels and max_requests requests (1..64).  Those of one
+ * kind (element type, connectivities, threshold, area bounds) share their
+ * launches (one reconstruction, fill and labelling over all their tiles).
+ * Defaults 16 requests, 2^20 and 2^22 pixels (0 selects a default);
+ * max_requests = 1 serves every request on its own.  Results do not depend on
+ * it.  No reference counterpart (the reference's queue-throughput entry
+ * point, MorphologicOperationsGPU.cpp:447-545, batches reconstructions only
+ * and has no admission). */
+nsx_status nsx_server_set_sharing(nsx_server* server, int max_requests, int64_t max_request_pixels,
+                                  int64_t max_set_pixels);
+/* batch stats (7 entries): shared launch sets run, requests served in them,
+ * largest set, max_requests, sets whose predicted propagation rounds fell
+ * short (finished with a host check; the stages after it ran again),
+ * max_request_pixels, max_set_pixels. */
+nsx_status nsx_server_batch_stats(nsx_server* server, int64_t* stats);
+/* The context of worker slot `slot` (0..slots-1), owned by the server, for
+ * kernel timing and tile counts (nsx_ctx_set_timing / nsx_ctx_read_timing /
+ * nsx_ctx_read_tile_counts).  Change its timing only while no request is in
+ * flight.  NULL for a bad slot. */
+nsx_ctx* nsx_server_context(nsx_server* server, int slot);
 /* Message of the last failed server call or request: a copy private to the
  * calling thread, valid until that thread calls this function again. */
 const char* nsx_server_last_error(nsx_server* server);
