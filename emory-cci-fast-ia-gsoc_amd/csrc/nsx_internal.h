@@ -1,6 +1,7 @@
 // nsx_internal.h -- shared declarations of the gfx950 morphology / CCL library.
 #pragma once
 
+#include "round_hints.h"
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstddef>
@@ -82,7 +83,7 @@ struct nsx_ctx {
     int i32_invert = NSX_I32_INVERT_WRAP;
 
     // Propagation rounds the last reconstruction of a given size/mode needed.
-    std::unordered_map<uint64_t, int> round_hint;
+    nsx::RoundHints round_hint;
 
     // HIP-event kernel timing (off by default).
     bool timing = false;

@@ -1237,9 +1237,7 @@ void ReconRun::release() {
 // by one round per call only: an over-prediction costs empty launches of a few
 // microseconds, an under-prediction a host round trip and a re-run.
 static void note_hint(nsx_ctx* ctx, uint64_t key, int64_t rounds_with_work) {
-    const int need = (int)std::min<int64_t>(4096, rounds_with_work + 1);
-    auto it = ctx->round_hint.find(key);
-    ctx->round_hint[key] = it == ctx->round_hint.end() ? need : std::max(need, it->second - 1);
+    ctx->round_hint.note(key, rounds_with_work);
 }
 
 nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const ReconBatch& B, hipStream_t s,
@@ -1279,9 +1277,7 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     R->grid = std::max(1, std::min(A.ntiles, (dtype == NSX_U8 ? 12 : 8) * ctx->cu_count));
     // Round predictions are shared by problems within ~19% of each other in tile
     // count (quarter-octave buckets), so mixed request sizes still hit.
-    const uint64_t bucket = (uint64_t)std::lround(4.0 * std::log2((double)A.ntiles));
-    R->key = ((uint64_t)A.visit_sweeps << 40) ^ (bucket << 16) ^ ((uint64_t)mode << 8) ^ ((uint64_t)dtype << 4) ^
-             (uint64_t)conn;
+    R->key = RoundHints::key(A.visit_sweeps, A.ntiles, mode, dtype, conn);
     run->impl = R;
 
     // Hole filling starts from the minimum everywhere inside the frame, so in
@@ -1305,8 +1301,7 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     ctx->first_tiles[cum_idx] += border_start ? border_tiles(D0.ntx, D0.nty) : A.ntiles;
     ctx->toc(h0, s);
     // Rounds the same-sized problem needed last time, launched without a host check.
-    auto it = ctx->round_hint.find(R->key);
-    const int hint = it != ctx->round_hint.end() ? it->second : 4;
+    const int hint = ctx->round_hint.launch(R->key);
     for (int r = 1; r <= hint; ++r) {
         const int h = ctx->tic(TK_RECON_ROUND, s);
         later(A, r, R->grid, s);
@@ -1379,11 +1374,7 @@ void recon_note_rounds(nsx_ctx* ctx, ReconRun* run, int64_t rounds_with_work) {
 int recon_predicted_rounds(nsx_ctx* ctx, int mode, int dtype, int conn, int ntiles, int visit_sweeps) {
     const bool quad = (mode == RM_FILL || mode == RM_FILLBIN) && conn == 4;
     const int pairs = std::max(1, std::min(MAX_IT, (visit_sweeps > 0 ? visit_sweeps : ctx->visit_sweeps) + (quad ? 1 : 0)));
-    const uint64_t bucket = (uint64_t)std::lround(4.0 * std::log2((double)std::max(1, ntiles)));
-    const uint64_t key = ((uint64_t)pairs << 40) ^ (bucket << 16) ^ ((uint64_t)mode << 8) ^ ((uint64_t)dtype << 4) ^
-                         (uint64_t)conn;
-    auto it = ctx->round_hint.find(key);
-    return it != ctx->round_hint.end() ? it->second : 0;
+    return ctx->round_hint.predicted(RoundHints::key(pairs, ntiles, mode, dtype, conn));
 }
 
 // u8 reconstructions that take more propagation rounds than this are finished
@@ -1524,12 +1515,10 @@ nsx_status recon_run_noalloc(nsx_ctx* ctx, const View& marker, const View& mask,
     first(A, 0, ntiles, s);
     // the round count of the last same-sized reconstruction on this context
     // (read only: the prediction table is not extended here)
-    const uint64_t bucket = (uint64_t)std::lround(4.0 * std::log2((double)ntiles));
-    const uint64_t key = ((uint64_t)A.visit_sweeps << 40) ^ (bucket << 16) ^ ((uint64_t)RM_RECON << 8) ^
-                         ((uint64_t)mask.dtype << 4) ^ (uint64_t)conn;
-    auto it = ctx->round_hint.find(key);
+    const uint64_t key = RoundHints::key(A.visit_sweeps, ntiles, RM_RECON, mask.dtype, conn);
+    const bool known = ctx->round_hint.has(key);
     int next = 1;
-    const int hint = it != ctx->round_hint.end() ? it->second : 4;
+    const int hint = ctx->round_hint.launch(key);
     for (; next <= hint; ++next) later(A, next, grid, s);
     NSX_HIP_TRY(ctx, hipGetLastError());
     int batch = 2;
@@ -1543,13 +1532,12 @@ nsx_status recon_run_noalloc(nsx_ctx* ctx, const View& marker, const View& mask,
         NSX_HIP_TRY(ctx, hipGetLastError());
         batch = std::min(batch * 2, 64);
     }
-    if (it != ctx->round_hint.end()) {
+    if (known) {
         NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 32, A.stats, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
         int64_t rounds;
         std::memcpy(&rounds, ctx->h_pinned + 32, sizeof(rounds));
-        const int need = (int)std::min<int64_t>(4096, rounds + 1);
-        it->second = std::max(need, it->second - 1);
+        ctx->round_hint.note_existing(key, rounds);
     }
     return NSX_OK;
 }

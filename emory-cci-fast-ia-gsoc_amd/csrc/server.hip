@@ -17,6 +17,7 @@
 // numbered in raster order within each image), so a request's output does not
 // depend on what else it was served with.
 #include "nsx_internal.h"
+#include "server_core.h"
 
 #include <algorithm>
 #include <chrono>
@@ -35,58 +36,13 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
-constexpr size_t kAlign = 256;
+using nsx_srv::align_up;
+using nsx_srv::kAlign;
+
 constexpr int kMaxBatch = 64;        // requests one slot may take at once
 // Per slot, pinned host memory: readback words of a shared set (6 + n) in
 // the first kPinnedWordBytes, then the staging of its tables.
 constexpr size_t kPinnedBytes = 512 << 10, kPinnedWordBytes = 4096;
-// Default sharing policy: only requests of at most 2^20 pixels are served
-// together, at most 2^22 pixels per set.  A small image's chain is bound by
-// launch and round latency, which a shared set pays once, while a large image
-// fills the device alone and keeps its own slot (so its transfers overlap
-// other work).
-constexpr int64_t kShareMaxPx = 1 << 20;
-constexpr int64_t kShareSetPx = 1 << 22;
-size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
-
-// First-fit allocator over one device block; offsets are kAlign-aligned.
-struct Arena {
-    char* base = nullptr;
-    size_t size = 0, used = 0, peak = 0;
-    std::map<size_t, size_t> free_blocks;   // offset -> length, coalesced
-
-    bool alloc(size_t bytes, size_t* off) {
-        bytes = align_up(bytes);
-        for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
-            if (it->second < bytes) continue;
-            const size_t o = it->first, rest = it->second - bytes;
-            free_blocks.erase(it);
-            if (rest) free_blocks.emplace(o + bytes, rest);
-            used += bytes;
-            peak = std::max(peak, used);
-            *off = o;
-            return true;
-        }
-        return false;
-    }
-    void release(size_t off, size_t bytes) {
-        bytes = align_up(bytes);
-        used -= bytes;
-        auto it = free_blocks.emplace(off, bytes).first;
-        auto nx = std::next(it);
-        if (nx != free_blocks.end() && it->first + it->second == nx->first) {
-            it->second += nx->second;
-            free_blocks.erase(nx);
-        }
-        if (it != free_blocks.begin()) {
-            auto pv = std::prev(it);
-            if (pv->first + pv->second == it->first) {
-                pv->second += it->second;
-                free_blocks.erase(it);
-            }
-        }
-    }
-};
 
 size_t elem_size(int dtype) { return dtype == NSX_U8 ? 1 : 4; }
 
@@ -96,7 +52,8 @@ struct Request {
     bool want_recon = false, want_fill = false;
     int recon_conn = 8, fill_conn = 4, ccl_conn = 8, min_size = 0, max_size = 0;
     double thresh = 0;
-    size_t bytes = 0, off = 0;
+    size_t bytes = 0, off = 0;   // arena block
+    int64_t px = 0;              // pixels of the mask
     bool done = false;
     nsx_status status = NSX_OK;
     std::string error;
@@ -104,15 +61,14 @@ struct Request {
     Clock::time_point t_submit, t_admit, t_done;
 };
 
-int64_t pixels(const Request& r) { return (int64_t)r.mask.rows * (int64_t)r.mask.cols; }
-
 }  // namespace
 
 struct nsx_server {
     int device = 0;
     std::vector<nsx_ctx*> ctxs;
     std::vector<std::thread> workers;
-    Arena arena;
+    nsx_srv::Arena arena;                // offsets into arena_base
+    char* arena_base = nullptr;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> queue;
@@ -121,9 +77,7 @@ struct nsx_server {
     bool stopping = false;
     int inflight = 0, peak_inflight = 0;
     int64_t completed = 0;
-    int max_batch = 16;                  // requests a slot takes at once
-    int64_t share_max_px = kShareMaxPx;  // largest request that joins a set
-    int64_t share_set_px = kShareSetPx;  // pixels per set
+    nsx_srv::SharingPolicy policy;
     int64_t shared_sets = 0, shared_reqs = 0, largest_set = 0, shared_short = 0;
     std::vector<char*> pinned;           // per slot: kPinnedBytes of pinned host memory
     std::string last_error;
@@ -474,7 +428,7 @@ void nsx_server::serve(int slot, std::vector<Request*>& rs) {
     std::vector<Job> jobs(rs.size());
     for (size_t i = 0; i < rs.size(); ++i) {
         jobs[i].r = rs[i];
-        stage_in(arena.base, jobs[i], s);
+        stage_in(arena_base, jobs[i], s);
     }
     // group by kind, in arrival order; empty images and kinds with one
     // request run on their own
@@ -530,30 +484,13 @@ void nsx_server::worker(int slot) {
             std::unique_lock<std::mutex> lk(mu);
             // Admission: the oldest queued request, once its block fits.  Later
             // requests wait behind it, so a large request cannot be starved.
-            cv_work.wait(lk, [&] {
-                if (queue.empty()) return stopping;
-                size_t off;
-                if (!arena.alloc(queue.front()->bytes, &off)) return false;
-                queue.front()->off = off;
-                return true;
-            });
+            cv_work.wait(lk, [&] { return queue.empty() ? stopping : nsx_srv::admit_front(queue, arena); });
             if (queue.empty()) return;
             // then, behind a small request, the small requests that follow it
             // in order while they fit (large ones fill the device by themselves)
-            int64_t px = 0;
-            do {
-                Request* r = queue.front();
-                queue.pop_front();
-                r->t_admit = Clock::now();
-                rs.push_back(r);
-                px += pixels(*r);
-                if (pixels(*r) > share_max_px || (int)rs.size() >= max_batch || queue.empty()) break;
-                const Request& nx = *queue.front();
-                if (pixels(nx) > share_max_px || px + pixels(nx) > share_set_px) break;
-                size_t off;
-                if (!arena.alloc(nx.bytes, &off)) break;
-                queue.front()->off = off;
-            } while (true);
+            nsx_srv::take_set(queue, arena, policy, rs);
+            const Clock::time_point now = Clock::now();
+            for (Request* r : rs) r->t_admit = now;
             inflight += (int)rs.size();
             peak_inflight = std::max(peak_inflight, inflight);
         }
@@ -590,17 +527,16 @@ nsx_status nsx_server_create(int device, size_t hbm_budget_bytes, int slots, nsx
     nsx_server* S = new (std::nothrow) nsx_server();
     if (!S) return NSX_ERR_NOMEM;
     S->device = device;
-    S->arena.size = align_up(hbm_budget_bytes);
-    if (hipMalloc((void**)&S->arena.base, S->arena.size) != hipSuccess) {
+    S->arena.reset(hbm_budget_bytes);
+    if (hipMalloc((void**)&S->arena_base, S->arena.size) != hipSuccess) {
         (void)hipGetLastError();
         delete S;
         return NSX_ERR_NOMEM;
     }
-    S->arena.free_blocks.emplace(0, S->arena.size);
     auto undo = [&](nsx_status st) {
         for (nsx_ctx* x : S->ctxs) nsx_ctx_destroy(x);
         for (char* w : S->pinned) hipHostFree(w);
-        hipFree(S->arena.base);
+        hipFree(S->arena_base);
         delete S;
         return st;
     };
@@ -651,6 +587,7 @@ nsx_status nsx_server_submit(nsx_server* S, const nsx_img* marker, const nsx_img
     r->min_size = min_size;
     r->max_size = max_size;
     r->bytes = layout_of(*r).total;
+    r->px = (int64_t)mask->rows * (int64_t)mask->cols;
     if (align_up(r->bytes) > S->arena.size) {
         delete r;
         return S->fail(NSX_ERR_BUDGET, "request needs more device memory than the server's HBM budget");
@@ -734,7 +671,7 @@ void nsx_server_destroy(nsx_server* S) {
     (void)hipSetDevice(S->device);
     for (nsx_ctx* c : S->ctxs) nsx_ctx_destroy(c);
     for (char* w : S->pinned) hipHostFree(w);
-    hipFree(S->arena.base);
+    hipFree(S->arena_base);
     delete S;
 }
 
@@ -746,9 +683,10 @@ nsx_status nsx_server_set_sharing(nsx_server* S, int max_requests, int64_t max_r
     if (max_request_pixels < 0 || max_set_pixels < 0)
         return S->fail(NSX_ERR_INVALID, "pixel limits must be >= 0");
     std::lock_guard<std::mutex> lk(S->mu);
-    S->max_batch = max_requests;
-    S->share_max_px = max_request_pixels ? max_request_pixels : kShareMaxPx;
-    S->share_set_px = max_set_pixels ? max_set_pixels : kShareSetPx;
+    const nsx_srv::SharingPolicy def;
+    S->policy.max_requests = max_requests;
+    S->policy.max_request_px = max_request_pixels ? max_request_pixels : def.max_request_px;
+    S->policy.max_set_px = max_set_pixels ? max_set_pixels : def.max_set_px;
     return NSX_OK;
 }
 
@@ -763,10 +701,10 @@ nsx_status nsx_server_batch_stats(nsx_server* S, int64_t* stats) {
     stats[0] = S->shared_sets;
     stats[1] = S->shared_reqs;
     stats[2] = S->largest_set;
-    stats[3] = S->max_batch;
+    stats[3] = S->policy.max_requests;
     stats[4] = S->shared_short;
-    stats[5] = S->share_max_px;
-    stats[6] = S->share_set_px;
+    stats[5] = S->policy.max_request_px;
+    stats[6] = S->policy.max_set_px;
     return NSX_OK;
 }
 
