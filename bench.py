@@ -217,6 +217,7 @@ def batch_roofline(nh, dtiles, args) -> dict | None:
         "ccl_final": 5 * u8px + 8 * f32px,
         "ccl_merge": 0,
         "ccl_small": 0,
+        "recon_scan": 0,   # batches are multi-image: no image-wide scans
     }
     kind = max(kt, key=lambda k: kt[k][0])
     ms_step = kt[kind][0] / steps
@@ -521,7 +522,7 @@ def serve_roofline(srv, burst, reqs, slots) -> dict | None:
     round_b = sum(r * TILE_PX * per_px[m] * es[d] for (m, d), (_, r) in tiles.items())
     in_b = sum(mask.size * mask.itemsize for _, mask, _, _ in reqs)
     alg = {"recon_first": first_b, "recon_round": round_b, "ccl_local": in_b, "ccl_final": in_b + 4 * sum(
-        mask.size for _, mask, _, _ in reqs), "ccl_merge": 0, "ccl_small": 0}
+        mask.size for _, mask, _, _ in reqs), "ccl_merge": 0, "ccl_small": 0, "recon_scan": 0}
     kind = max(kt, key=lambda k: kt[k][0])
     ms, n = kt[kind]
     if not n or not alg[kind]:
@@ -932,37 +933,40 @@ def border_tiles(rows: int, cols: int) -> int:
     return ntx * nty if ntx <= 2 or nty <= 2 else 2 * ntx + 2 * (nty - 2)
 
 
-def roofline(kt: dict, px: int, visits: tuple, steps: int, traffic: dict | None = None,
-             fill_first_tiles: int = 0) -> dict:
+def roofline(kt: dict, px: int, tiles: dict, steps: int, traffic: dict | None = None, scan_pairs: int = 0) -> dict:
     """HBM roofline of the kernel kind with the most time per step.
 
     Algorithmic bytes per pixel (compulsory I/O of the operation a launch
     performs): reconstruction 3 B (marker, mask in; state out), fillHoles 2 B
     (image in; state out), labelling read 1 B, label output 1 B in + 4 B out.
-    The reconstruction's first pass processes every pixel of the image; the
-    fill's first pass only the border tiles (the others start from a plain
-    fill of the output); a propagation round processes the tiles it visits
-    (62x62 pixels each).
+    First passes and propagation rounds process the tiles (62 x 62 pixels)
+    the context counted over the timed steps (`tiles`, from
+    Context.read_tile_counts).  A pair of image-wide scans moves 6 B/px (row
+    scan: image and state in, state out; column scan the same), 5 B/px for the
+    first pair, whose row scan starts from the initial state.
     """
-    v_recon, v_fill = visits
-    alg_step = {
-        "recon_first": 3 * px + 2 * fill_first_tiles * TILE_PX,
-        "recon_round": (3 * v_recon + 2 * v_fill) * TILE_PX,
-        "ccl_local": 1 * px,
-        "ccl_final": 5 * px,
+    per_px = {"recon": 3, "fill": 2, "binary": 3, "fillbin": 2}
+    es = {"u8": 1, "i32": 4, "f32": 4}
+    first_b = sum(f * TILE_PX * per_px[m] * es[d] for (m, d), (f, _) in tiles.items())
+    round_b = sum(r * TILE_PX * per_px[m] * es[d] for (m, d), (_, r) in tiles.items())
+    alg_total = {
+        "recon_first": first_b,
+        "recon_round": round_b,
+        "ccl_local": 1 * px * steps,
+        "ccl_final": 5 * px * steps,
         "ccl_merge": 0,
         "ccl_small": 0,
+        "recon_scan": (6 * scan_pairs - 1) * px * steps if scan_pairs else 0,
     }
     kind = max(kt, key=lambda k: kt[k][0])
-    ms_step = kt[kind][0] / steps
-    launches_step = kt[kind][1] / steps
-    alg_launch = alg_step[kind] / launches_step if launches_step else 0.0
-    avg_launch_s = ms_step / launches_step / 1e3 if launches_step else float("nan")
-    achieved = alg_launch / avg_launch_s / 1e9 if launches_step else 0.0
+    ms_total, launches = kt[kind]
+    alg_launch = alg_total[kind] / launches if launches else 0.0
+    avg_launch_s = ms_total / launches / 1e3 if launches else float("nan")
+    achieved = alg_launch / avg_launch_s / 1e9 if launches else 0.0
     return {"bound": "hbm", "kernel": kind, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": (traffic or {}).get(kind),
             "alg_bytes_per_launch": int(alg_launch), "avg_launch_us": round(avg_launch_s * 1e6, 3),
-            "launches_per_step": round(launches_step, 2),
+            "launches_per_step": round(launches / steps, 2),
             "per_kind_ms_per_step": {k: round(v[0] / steps, 4) for k, v in kt.items()}}
 
 
@@ -1059,13 +1063,14 @@ def main():
         step()
     torch.cuda.synchronize()
     kt = ctx.read_timing()
+    tiles = ctx.read_tile_counts()
     ctx.set_timing(False)
     log("kernel time per step (ms): " + ", ".join(f"{k}={v[0] / args.steps:.4f} ({v[1] / args.steps:.1f} launches)"
                                                   for k, v in kt.items()))
-    log(f"recon stage {stats}; fill stage {fstats}")
+    log(f"recon stage {stats}; fill stage {fstats}; tiles over the timed steps {tiles}")
     default_chain = n == 4096 and args.disks == 3000
-    roof = roofline(kt, n * n, (stats["tile_visits"], fstats["tile_visits"]), args.steps,
-                    TRAFFIC_CHAIN4096 if default_chain else None, border_tiles(n, n))
+    scan_pairs = ctx.get_tuning(nh._lib.NSX_TUNE_FILL_SCANS) if kt.get("recon_scan", (0, 0))[1] else 0
+    roof = roofline(kt, n * n, tiles, args.steps, TRAFFIC_CHAIN4096 if default_chain else None, scan_pairs)
     cpu, exact = (None, None) if args.no_cpu else cpu_baseline(marker, mask, args.cpu_runs, first)
     if exact is False:
         sys.exit(1)

@@ -286,6 +286,11 @@ nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value) {
         ctx->levelset_levels = value;
         return NSX_OK;
     }
+    if (param == NSX_TUNE_FILL_SCANS) {
+        if (value < 0 || value > 8) return ctx->fail(NSX_ERR_INVALID, "fill scans must be in 0..8");
+        ctx->fill_scans = value;
+        return NSX_OK;
+    }
     if (param == NSX_TUNE_ENGINE) {
         if (value != NSX_ENGINE_ROUNDS && value != NSX_ENGINE_TREE && value != NSX_ENGINE_AUTO)
             return ctx->fail(NSX_ERR_INVALID, "engine must be NSX_ENGINE_ROUNDS, NSX_ENGINE_TREE or NSX_ENGINE_AUTO");
@@ -307,6 +312,10 @@ nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value) {
     }
     if (param == NSX_TUNE_ENGINE) {
         *value = ctx->engine;
+        return NSX_OK;
+    }
+    if (param == NSX_TUNE_FILL_SCANS) {
+        *value = ctx->fill_scans;
         return NSX_OK;
     }
     return NSX_ERR_INVALID;
@@ -836,7 +845,7 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
     if (auto_u8 && (recon_predicted_rounds(ctx, RM_RECON, NSX_U8, rconn, br.ntiles, 0) > 256 ||
                     recon_predicted_rounds(ctx, RM_FILL, NSX_U8, fconn, bf.ntiles, 0) > 256)) {
         NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vr, rconn, s));
-        NSX_TRY(recon_run(ctx, RM_FILL, nullptr, vr, vf, fconn, s));
+        NSX_TRY(recon_run(ctx, RM_FILL, nullptr, vr, vf, fconn, s, nullptr, ctx->d_stats + 8));
         NSX_TRY(ccl_run(ctx, vf, vl, p, dw + 12, s));
         int32_t n = 0;
         NSX_TRY(read_count(ctx, dw + 12, &n, s));
@@ -865,7 +874,7 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
                 rr.release();
                 rf.release();
                 NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vr, rconn, s));
-                NSX_TRY(recon_run(ctx, RM_FILL, nullptr, vr, vf, fconn, s));
+                NSX_TRY(recon_run(ctx, RM_FILL, nullptr, vr, vf, fconn, s, nullptr, ctx->d_stats + 8));
             } else {
                 NSX_TRY(recon_complete(ctx, &rr, s, nullptr));
                 NSX_TRY(recon_launch(ctx, RM_FILL, mask->dtype, fconn, bf, s, &rf, nullptr));
@@ -873,7 +882,7 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
             }
         } else if (auto_u8) {
             rf.release();
-            NSX_TRY(recon_run(ctx, RM_FILL, nullptr, vr, vf, fconn, s));
+            NSX_TRY(recon_run(ctx, RM_FILL, nullptr, vr, vf, fconn, s, nullptr, ctx->d_stats + 8));
         } else {
             NSX_TRY(recon_complete(ctx, &rf, s, nullptr));
         }

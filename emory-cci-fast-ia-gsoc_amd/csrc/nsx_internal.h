@@ -54,6 +54,7 @@ enum TimedKind {
     TK_CCL_MERGE,         // tile-edge union
     TK_CCL_FINAL,         // relabel + output write
     TK_CCL_SMALL,         // area / keep / scan kernels (one event pair per call)
+    TK_RECON_SCAN,        // image-wide row / column scans before a hole filling's rounds
     TK_NKINDS
 };
 }  // namespace nsx
@@ -78,6 +79,7 @@ struct nsx_ctx {
     int visit_sweeps = 4;
     int levelset_levels = 3;
     int engine = NSX_ENGINE_AUTO;     // NSX_TUNE_ENGINE
+    int fill_scans = 0;               // NSX_TUNE_FILL_SCANS
 
     // Options (nsx_ctx_set_option).
     int i32_invert = NSX_I32_INVERT_WRAP;
@@ -146,6 +148,7 @@ enum WsSlot {
     WS_MT2_WALK,
     WS_MT2_CTL,
     WS_SERVE_TABLES,     // serving: descriptors, tile tables and counts of a shared launch set
+    WS_SCAN_SEG,         // image-wide column scans: per-segment clamp compositions
     WS_NSLOTS
 };
 
@@ -176,8 +179,10 @@ enum ReconMode { RM_RECON = 0, RM_BINARY = 1, RM_FILL = 2, RM_FILLBIN = 3 };
 // fill modes) to convergence.  `state` is the output buffer and holds the
 // propagation state.  Synchronises the stream at least once.
 struct LevelLabelReq;
+// `stats`: the device statistics words to fill (default ctx->d_stats, stage 0;
+// a chain's fill passes stage 1, ctx->d_stats + 8).
 nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& state,
-                     int conn, hipStream_t s, LevelLabelReq* req = nullptr);
+                     int conn, hipStream_t s, LevelLabelReq* req = nullptr, int64_t* stats = nullptr);
 
 // Several images of one dtype/mode/connectivity in one set of launches.
 struct ReconBatch {

@@ -768,9 +768,9 @@ struct StateFill {
 
 // Clears the stamps, the round counters and the stats.  With ntx > 0 it also
 // lists the border tiles of a single ntx x nty image as round 0's work, and
-// writes the initial state F.
+// writes the initial state F; with all_tiles it lists every tile instead.
 __global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts, int64_t* stats, int32_t* list0,
-                                   int ntx, int nty, StateFill F) {
+                                   int ntx, int nty, StateFill F, int all_tiles) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     // the initial state, a row per block: 16-byte stores between the row's first
     // and last 16-byte boundaries, element stores before and after (and for a
@@ -796,10 +796,10 @@ __global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts,
             *(u32x4*)(p + o) = v4;
     }
     for (int j = i; j < ntiles; j += gridDim.x * blockDim.x) stamp[j] = 0u;
-    const int nb = ntx > 0 ? border_tiles(ntx, nty) : 0;
+    const int nb = all_tiles ? ntiles : ntx > 0 ? border_tiles(ntx, nty) : 0;
     for (int b = i; b < nb; b += gridDim.x * blockDim.x) {
         int t;
-        if (ntx <= 2 || nty <= 2) t = b;
+        if (all_tiles || ntx <= 2 || nty <= 2) t = b;
         else if (b < ntx) t = b;                                   // top row
         else if (b < 2 * ntx) t = (nty - 1) * ntx + (b - ntx);     // bottom row
         else {                                                     // left / right column
@@ -810,6 +810,247 @@ __global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts,
     }
     if (i < 3 * NSEG) counts[i * CSTRIDE] = i == 0 ? nb : 0;
     if (i == 0) { stats[0] = 0; stats[1] = 0; stats[2] = ntiles; stats[3] = 0; }
+}
+
+// ---- image-wide row and column scans (hole filling's start) -------------------
+// Along one line (a row or a column) the reconstruction of state s under mask
+// m is exact in two scans: the forward value f(x) = med3(f(x-1), s(x), m(x))
+// (keys; f(-1) = the frame) carries every source on the left, the backward
+// value b(x) likewise from the right, and the line's result is max(f, b), both
+// computed from the same s.  Each element is the clamp v -> med3(v, s, m)
+// (s <= m), and clamps compose into clamps, so a line splits into pieces whose
+// compositions are combined first.  Rows: one wave per row, lane = column
+// within 64-column pieces, the pieces' compositions combined by a wave scan.
+// Columns: one wave per 64-column strip and 32-row segment; the segments'
+// compositions go through global memory (scan_cols_seg_kernel), a pass over
+// them gives each segment's carries from above and below
+// (scan_cols_carry_kernel), and each segment then runs its rows from those
+// (scan_cols_apply_kernel).  Alternating row and column scans carry the
+// frame's value across the open parts of the image before the tile rounds
+// start, so the rounds are left with the twisted paths.  Results are the same
+// with or without scans: every scan value is a lower bound of the
+// reconstruction and at least the state it started from.
+constexpr int SCAN_SEG = 32;   // rows per column segment
+constexpr uint32_t KEY_ALL = 0xFFFFFFFFu;
+
+// (lo, hi) <- (lo, hi) after (plo, phi): the clamp composition v -> (lo,hi)((plo,phi)(v)).
+__device__ __forceinline__ void clamp_after(uint32_t& lo, uint32_t& hi, uint32_t plo, uint32_t phi) {
+    const uint32_t nlo = med3u(plo, lo, hi), nhi = med3u(phi, lo, hi);
+    lo = nlo;
+    hi = nhi;
+}
+
+// Keys of a pixel: its mask key and its state key (INIT: the state before any
+// propagation, Codec::first).
+template <typename T, int MODE, bool INIT>
+__device__ __forceinline__ void scan_keys(const ImgDesc& D, int y, int x, uint32_t& ks, uint32_t& km) {
+    using C = Codec<T, MODE>;
+    const T m = ld<T>(D.mask, D.pk, y, x);
+    km = C::mkey(T(0), m);
+    ks = INIT ? C::first(T(0), m) : C::dec(ld<T>(D.state, D.ps, y, x), T(0), m);
+}
+
+// One wave per row; NS 64-column pieces per row at most (cols <= 64 NS).
+template <typename T, int MODE, bool INIT, int NS>
+__global__ __launch_bounds__(256) void scan_rows_kernel(ImgDesc D) {
+    using C = Codec<T, MODE>;
+    const int lane = threadIdx.x & 63;
+    const int y = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (y >= D.rows) return;
+    const int npc = (D.cols + 63) >> 6;
+    const uint32_t F = C::frame();
+    uint32_t bw[NS];
+    // backward: pieces right to left; lane l's composition covers lanes l..63
+    uint32_t v = F;
+#pragma unroll
+    for (int i = NS - 1; i >= 0; --i) {
+        if (i < npc) {
+            const int x = i * 64 + lane;
+            uint32_t lo = 0u, hi = KEY_ALL;   // identity past the row's end
+            if (x < D.cols) scan_keys<T, MODE, INIT>(D, y, x, lo, hi);
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t plo = __shfl_down(lo, d, 64), phi = __shfl_down(hi, d, 64);
+                if (lane + d < 64) clamp_after(lo, hi, plo, phi);
+            }
+            const uint32_t out = med3u(v, lo, hi);
+            bw[i] = out;
+            v = __builtin_amdgcn_readlane(out, 0);
+        }
+    }
+    // forward: pieces left to right; the result is the larger of both values
+    v = F;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        if (i < npc) {
+            const int x = i * 64 + lane;
+            uint32_t lo = 0u, hi = KEY_ALL;
+            if (x < D.cols) scan_keys<T, MODE, INIT>(D, y, x, lo, hi);
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t plo = __shfl_up(lo, d, 64), phi = __shfl_up(hi, d, 64);
+                if (lane >= d) clamp_after(lo, hi, plo, phi);
+            }
+            const uint32_t out = med3u(v, lo, hi);
+            if (x < D.cols) st<T>(D.state, D.ps, y, x, C::enc(max(out, bw[i]), T(0)));
+            v = __builtin_amdgcn_readlane(out, 63);
+        }
+    }
+}
+
+// Column segments: the forward (top to bottom) and backward compositions of
+// each SCAN_SEG-row segment of each column, as four planes [nseg][colsp] of
+// segf: forward lo, forward hi, backward lo, backward hi.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void scan_cols_seg_kernel(ImgDesc D, uint32_t* segf, int nseg, int colsp) {
+    const int lane = threadIdx.x & 63;
+    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int strips = colsp >> 6;
+    if (wid >= strips * nseg) return;
+    const int strip = wid % strips, seg = wid / strips;
+    const int x = strip * 64 + lane, cx = min(x, D.cols - 1);
+    const int y0 = seg * SCAN_SEG;
+    uint32_t ks[SCAN_SEG], km[SCAN_SEG];
+#pragma unroll
+    for (int k = 0; k < SCAN_SEG; ++k) {
+        ks[k] = 0u;
+        km[k] = KEY_ALL;   // identity below the image
+        if (y0 + k < D.rows) scan_keys<T, MODE, false>(D, y0 + k, cx, ks[k], km[k]);
+    }
+    uint32_t flo = 0u, fhi = KEY_ALL, blo = 0u, bhi = KEY_ALL;
+#pragma unroll
+    for (int k = 0; k < SCAN_SEG; ++k) {
+        uint32_t lo = ks[k], hi = km[k];   // row k after the rows above it
+        clamp_after(lo, hi, flo, fhi);
+        flo = lo;
+        fhi = hi;
+        uint32_t lo2 = ks[SCAN_SEG - 1 - k], hi2 = km[SCAN_SEG - 1 - k];   // and after the rows below
+        clamp_after(lo2, hi2, blo, bhi);
+        blo = lo2;
+        bhi = hi2;
+    }
+    const size_t plane = (size_t)nseg * colsp, at = (size_t)seg * colsp + x;
+    segf[at] = flo;
+    segf[plane + at] = fhi;
+    segf[2 * plane + at] = blo;
+    segf[3 * plane + at] = bhi;
+}
+
+// The value entering each segment from above (forward) and from below
+// (backward), planes 4 and 5 of segf.  A block of four waves takes 64
+// columns (lane = column); wave w composes the segments of its quarter of the
+// column, the quarters' compositions meet in LDS, and each wave then walks its
+// quarter again from its carry.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void scan_cols_carry_kernel(uint32_t* segf, int nseg, int colsp) {
+    using C = Codec<T, MODE>;
+    __shared__ uint32_t part[4][4][64];   // [wave][flo fhi blo bhi][lane]
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int x = blockIdx.x * 64 + lane;   // colsp is a multiple of 64: always a column of segf
+    const size_t plane = (size_t)nseg * colsp;
+    const int ch = (nseg + 3) / 4, q0 = min(nseg, w * ch), q1 = min(nseg, q0 + ch);
+    uint32_t flo = 0u, fhi = KEY_ALL, blo = 0u, bhi = KEY_ALL;
+#pragma unroll 8
+    for (int q = q0; q < q1; ++q) {   // this quarter's forward composition, top down
+        const size_t at = (size_t)q * colsp + x;
+        uint32_t lo = segf[at], hi = segf[plane + at];
+        clamp_after(lo, hi, flo, fhi);
+        flo = lo;
+        fhi = hi;
+    }
+#pragma unroll 8
+    for (int q = q1 - 1; q >= q0; --q) {   // and its backward composition, bottom up
+        const size_t at = (size_t)q * colsp + x;
+        uint32_t lo = segf[2 * plane + at], hi = segf[3 * plane + at];
+        clamp_after(lo, hi, blo, bhi);
+        blo = lo;
+        bhi = hi;
+    }
+    part[w][0][lane] = flo;
+    part[w][1][lane] = fhi;
+    part[w][2][lane] = blo;
+    part[w][3][lane] = bhi;
+    __syncthreads();
+    uint32_t vf = C::frame(), vb = C::frame();
+    for (int u = 0; u < w; ++u) vf = med3u(vf, part[u][0][lane], part[u][1][lane]);
+    for (int u = 3; u > w; --u) vb = med3u(vb, part[u][2][lane], part[u][3][lane]);
+#pragma unroll 8
+    for (int q = q0; q < q1; ++q) {
+        const size_t at = (size_t)q * colsp + x;
+        segf[4 * plane + at] = vf;
+        vf = med3u(vf, segf[at], segf[plane + at]);
+    }
+#pragma unroll 8
+    for (int q = q1 - 1; q >= q0; --q) {
+        const size_t at = (size_t)q * colsp + x;
+        segf[5 * plane + at] = vb;
+        vb = med3u(vb, segf[2 * plane + at], segf[3 * plane + at]);
+    }
+}
+
+// Column segments, last pass: each segment's rows from its two carries.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void scan_cols_apply_kernel(ImgDesc D, const uint32_t* segf, int nseg, int colsp) {
+    using C = Codec<T, MODE>;
+    const int lane = threadIdx.x & 63;
+    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int strips = colsp >> 6;
+    if (wid >= strips * nseg) return;
+    const int strip = wid % strips, seg = wid / strips;
+    const int x = strip * 64 + lane, cx = min(x, D.cols - 1);
+    const int y0 = seg * SCAN_SEG;
+    const size_t plane = (size_t)nseg * colsp, at = (size_t)seg * colsp + x;
+    uint32_t vf = segf[4 * plane + at], vb = segf[5 * plane + at];
+    uint32_t ks[SCAN_SEG], km[SCAN_SEG], bw[SCAN_SEG];
+#pragma unroll
+    for (int k = 0; k < SCAN_SEG; ++k) {
+        ks[k] = 0u;
+        km[k] = KEY_ALL;
+        if (y0 + k < D.rows) scan_keys<T, MODE, false>(D, y0 + k, cx, ks[k], km[k]);
+    }
+#pragma unroll
+    for (int k = SCAN_SEG - 1; k >= 0; --k) {
+        vb = med3u(vb, ks[k], km[k]);
+        bw[k] = vb;
+    }
+#pragma unroll
+    for (int k = 0; k < SCAN_SEG; ++k) {
+        vf = med3u(vf, ks[k], km[k]);
+        if (y0 + k < D.rows && x < D.cols) st<T>(D.state, D.ps, y0 + k, x, C::enc(max(vf, bw[k]), T(0)));
+    }
+}
+
+template <typename T, int MODE, bool INIT>
+void launch_scan_rows(const ImgDesc& D, hipStream_t s) {
+    const dim3 grid((D.rows + 3) / 4), block(256);
+    if (D.cols <= 1024) hipLaunchKernelGGL((scan_rows_kernel<T, MODE, INIT, 16>), grid, block, 0, s, D);
+    else if (D.cols <= 2048) hipLaunchKernelGGL((scan_rows_kernel<T, MODE, INIT, 32>), grid, block, 0, s, D);
+    else hipLaunchKernelGGL((scan_rows_kernel<T, MODE, INIT, 64>), grid, block, 0, s, D);
+}
+
+// `pairs` row + column scan pairs over one image, the first row scan starting
+// from the initial state (the state buffer is written everywhere).  segf:
+// scan_segf_words() words.  cols <= SCAN_MAX_COLS.
+constexpr int SCAN_MAX_COLS = 4096;
+// Images whose tile grid is narrower than this on either side keep the
+// border start: their rounds are few, and the scans' launches would not pay.
+constexpr int SCAN_MIN_TILES = 24;
+template <typename T, int MODE>
+void run_scans(const ImgDesc& D, int pairs, uint32_t* segf, hipStream_t s) {
+    const int nseg = (D.rows + SCAN_SEG - 1) / SCAN_SEG, colsp = ((D.cols + 63) / 64) * 64;
+    const int waves = (colsp / 64) * nseg;
+    for (int p = 0; p < pairs; ++p) {
+        if (p == 0) launch_scan_rows<T, MODE, true>(D, s);
+        else launch_scan_rows<T, MODE, false>(D, s);
+        hipLaunchKernelGGL((scan_cols_seg_kernel<T, MODE>), dim3((waves + 3) / 4), dim3(256), 0, s, D, segf, nseg, colsp);
+        hipLaunchKernelGGL((scan_cols_carry_kernel<T, MODE>), dim3(colsp / 64), dim3(256), 0, s, segf, nseg, colsp);
+        hipLaunchKernelGGL((scan_cols_apply_kernel<T, MODE>), dim3((waves + 3) / 4), dim3(256), 0, s, D,
+                           (const uint32_t*)segf, nseg, colsp);
+    }
+}
+
+size_t scan_segf_words(int rows, int cols) {
+    return (size_t)6 * ((rows + SCAN_SEG - 1) / SCAN_SEG) * (((size_t)cols + 63) / 64) * 64;
 }
 
 // out[0] = *ncomp; out[1], out[2] = tiles queued after the launched rounds of
@@ -1283,8 +1524,21 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     // Hole filling starts from the minimum everywhere inside the frame, so in
     // round 0 only the tiles that touch the frame can change: the others are
     // written with the initial state by one fill kernel and join when activated.
-    const bool border_start = (mode == RM_FILL || mode == RM_FILLBIN) && B.nimg == 1;
+    // A gray fill of a large single image instead starts with image-wide row and
+    // column scans that carry the frame's value across the open parts; its
+    // round 0 then visits every tile from the scanned state.
     const ImgDesc& D0 = A.img0;
+    const bool scan_start = mode == RM_FILL && B.nimg == 1 && ctx->fill_scans > 0 &&
+                            (dtype == NSX_U8 || dtype == NSX_I32) && D0.cols <= SCAN_MAX_COLS &&
+                            std::min(D0.ntx, D0.nty) >= SCAN_MIN_TILES;
+    const bool border_start = (mode == RM_FILL || mode == RM_FILLBIN) && B.nimg == 1 && !scan_start;
+    uint32_t* segf = nullptr;
+    if (scan_start) {
+        void* p = nullptr;
+        nsx_status sst = ctx->ws(WS_SCAN_SEG, sizeof(uint32_t) * scan_segf_words(D0.rows, D0.cols), &p);
+        if (sst != NSX_OK) { run->release(); return sst; }
+        segf = (uint32_t*)p;
+    }
     A.first_list = border_start ? 1 : 0;
     StateFill F{};
     int grid_reset = std::max(1, std::min(1024, (A.ntiles + 255) / 256));
@@ -1295,11 +1549,21 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
         grid_reset = std::max(grid_reset, std::min(4 * ctx->cu_count, D0.rows));
     }
     hipLaunchKernelGGL(recon_reset_kernel, dim3(grid_reset), dim3(256), 0, s, A.stamp, A.ntiles, A.counts, A.stats,
-                       A.list0, border_start ? D0.ntx : 0, D0.nty, F);
-    const int h0 = ctx->tic(TK_RECON_FIRST, s);
-    first(A, 0, border_start ? std::min(R->grid, border_tiles(D0.ntx, D0.nty)) : A.ntiles, s);
-    ctx->first_tiles[cum_idx] += border_start ? border_tiles(D0.ntx, D0.nty) : A.ntiles;
-    ctx->toc(h0, s);
+                       A.list0, border_start ? D0.ntx : 0, D0.nty, F, scan_start ? 1 : 0);
+    if (scan_start) {
+        const int hs = ctx->tic(TK_RECON_SCAN, s);
+        if (dtype == NSX_U8) run_scans<uint8_t, RM_FILL>(D0, ctx->fill_scans, segf, s);
+        else run_scans<int32_t, RM_FILL>(D0, ctx->fill_scans, segf, s);
+        ctx->toc(hs, s);
+        const int h0 = ctx->tic(TK_RECON_ROUND, s);
+        later(A, 0, R->grid, s);
+        ctx->toc(h0, s);
+    } else {
+        const int h0 = ctx->tic(TK_RECON_FIRST, s);
+        first(A, 0, border_start ? std::min(R->grid, border_tiles(D0.ntx, D0.nty)) : A.ntiles, s);
+        ctx->first_tiles[cum_idx] += border_start ? border_tiles(D0.ntx, D0.nty) : A.ntiles;
+        ctx->toc(h0, s);
+    }
     // Rounds the same-sized problem needed last time, launched without a host check.
     const int hint = ctx->round_hint.launch(R->key);
     for (int r = 1; r <= hint; ++r) {
@@ -1395,7 +1659,7 @@ static nsx_status recon_tree(nsx_ctx* ctx, int mode, const View* marker, const V
 }
 
 nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& state,
-                     int conn, hipStream_t s, LevelLabelReq* req) {
+                     int conn, hipStream_t s, LevelLabelReq* req, int64_t* stats) {
     if (mask.rows <= 0 || mask.cols <= 0) return NSX_OK;
     if (mode == RM_RECON && mask.dtype == NSX_U8 && marker && ctx->levelset_levels > 0) {
         bool used = false;
@@ -1420,6 +1684,7 @@ nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mas
     B.ntiles = D.ntx * D.nty;
     B.host_desc = &D;
     B.ws_base = WS_RECON_STAMP;
+    B.stats = stats;
     ReconRun run;
     NSX_TRY(recon_launch(ctx, mode, mask.dtype, conn, B, s, &run, nullptr));
     bool gave_up = false;
@@ -1511,7 +1776,7 @@ nsx_status recon_run_noalloc(nsx_ctx* ctx, const View& marker, const View& mask,
     // the stamps, counters and statistics start cleared on every call (a kernel,
     // not a memset, so the sequence can also be captured)
     hipLaunchKernelGGL(recon_reset_kernel, dim3(std::max(1, std::min(1024, (ntiles + 255) / 256))), dim3(256), 0, s,
-                       A.stamp, A.ntiles, A.counts, A.stats, A.list0, 0, 0, StateFill{});
+                       A.stamp, A.ntiles, A.counts, A.stats, A.list0, 0, 0, StateFill{}, 0);
     first(A, 0, ntiles, s);
     // the round count of the last same-sized reconstruction on this context
     // (read only: the prediction table is not extended here)

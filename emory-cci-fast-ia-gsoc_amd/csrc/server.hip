@@ -347,7 +347,8 @@ void run_shared(nsx_ctx* ctx, std::vector<Job*>& g, char* pin, bool* short_round
         for (Job* j : g) {
             const View vm = j->view(j->dm);
             NSX_TRY(mode == RM_RECON ? recon_run(ctx, RM_RECON, &vm, j->view(j->dk), j->view(j->dr), rconn, s)
-                                     : recon_run(ctx, RM_FILL, nullptr, j->view(j->dr), j->view(j->df), fconn, s));
+                                     : recon_run(ctx, RM_FILL, nullptr, j->view(j->dr), j->view(j->df), fconn, s, nullptr,
+                                                 ctx->d_stats + 8));
         }
         return NSX_OK;
     };

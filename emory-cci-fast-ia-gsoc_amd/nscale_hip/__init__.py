@@ -135,7 +135,7 @@ class Context:
     def workspace_bytes(self) -> int:
         return int(self.lib.nsx_ctx_workspace_bytes(self.h))
 
-    TIMED_KINDS = ("recon_first", "recon_round", "ccl_local", "ccl_merge", "ccl_final", "ccl_small")
+    TIMED_KINDS = ("recon_first", "recon_round", "ccl_local", "ccl_merge", "ccl_final", "ccl_small", "recon_scan")
 
     def set_timing(self, enable: bool) -> None:
         """Start (and clear) or stop HIP-event timing of the library's kernels."""

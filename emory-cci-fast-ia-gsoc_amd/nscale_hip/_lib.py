@@ -19,6 +19,7 @@ NSX_U8, NSX_I32, NSX_F32 = 0, 1, 2
 NSX_TUNE_VISIT_SWEEPS = 0
 NSX_TUNE_LEVELSET_LEVELS = 1
 NSX_TUNE_ENGINE = 2
+NSX_TUNE_FILL_SCANS = 3
 NSX_ENGINE_ROUNDS = 0
 NSX_ENGINE_TREE = 1
 NSX_ENGINE_AUTO = 2

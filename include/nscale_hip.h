@@ -75,7 +75,8 @@ nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int stage, int64_t stats[4]);
  * nsx_ctx_read_timing fills ms[k] (summed kernel time) and launches[k] for the
  * kernel kinds k = 0 reconstruction first pass, 1 reconstruction propagation
  * launch, 2 tile-local labelling, 3 tile-edge union, 4 label output, 5 area /
- * keep / scan kernels (per call).  Synchronises the context.  No reference
+ * keep / scan kernels (per call), 6 a hole filling's image-wide row and column
+ * scans (per call).  Synchronises the context.  No reference
  * counterpart (the reference's CUDA event timing is commented out,
  * src/segment/cuda/ccl_uf.cu:28-31). */
 nsx_status nsx_ctx_set_timing(nsx_ctx* ctx, int enable);
@@ -113,7 +114,19 @@ nsx_status nsx_ctx_read_tile_counts(nsx_ctx* ctx, int64_t* first_pass, int64_t* 
  * reconstruction passes 256 rounds, and starting with the tree when the last
  * reconstruction of the same kind and size did.
  * Other types and modes always use rounds.  Results are the same. */
-enum { NSX_TUNE_VISIT_SWEEPS = 0, NSX_TUNE_LEVELSET_LEVELS = 1, NSX_TUNE_ENGINE = 2, NSX_TUNE_NPARAMS = 3 };
+/* NSX_TUNE_FILL_SCANS: image-wide row + column scan pairs run before the
+ * propagation rounds of a gray hole filling (u8, int32) of one image at most
+ * 4096 columns wide whose tile grid is at least 24 tiles (of 62 pixels) on
+ * each side, 0..8 (default 0 = never).  A row or column scan is the exact
+ * reconstruction along every line of the image, so the frame's value crosses
+ * the open parts of the image at once, and the rounds, which move values one
+ * tile per round, are left with the twisted paths.  Off by default: on noisy
+ * images the rounds that remain are set by how many sweep pairs a tile visit
+ * runs, and the rounds saved do not repay the scans.  Counterpart of the
+ * reference's 1-D passes (cuda/imreconstruct_int_kernel.cu:217-585), which
+ * alternate until nothing changes. */
+enum { NSX_TUNE_VISIT_SWEEPS = 0, NSX_TUNE_LEVELSET_LEVELS = 1, NSX_TUNE_ENGINE = 2, NSX_TUNE_FILL_SCANS = 3,
+       NSX_TUNE_NPARAMS = 4 };
 enum { NSX_ENGINE_ROUNDS = 0, NSX_ENGINE_TREE = 1, NSX_ENGINE_AUTO = 2 };
 nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value);
 nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value);
