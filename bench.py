@@ -678,7 +678,7 @@ def bench_maze_levels(args):
     100 distinct values) and propagation needs one round per tile crossing of
     paths millions of pixels long.  One step = imreconstruct (4-conn) -> (> 100)
     -> bwlabel (4-conn) through nsx_recon_label with the default engine
-    (NSX_ENGINE_AUTO: rounds, switching to the component tree past 256 rounds
+    (NSX_ENGINE_AUTO: rounds, switching to the bit planes past 256 rounds
     and starting there on later calls of the size).  Checked: every corridor
     pixel is reached above 100 and every wall stays below it, so the labels are
     exactly one component, the corridor set; the reconstruction equals the one
@@ -756,9 +756,36 @@ def bench_maze_levels(args):
         cpu = {"value": round(cn * cn / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
                "sample": f"one {cn}x{cn} many-valued maze of the same construction (recon4 + bwlabel4), single "
                          f"thread, {dt:.1f} s; GPU output on it identical", "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
+    # per-kind kernel times of a few more steps with HIP events: under the
+    # adaptive engine each step is the bit-plane reconstruction (8 passes of
+    # prep + labelling with a fold) and one labelling of {recon > 100}
+    ctx.set_timing(True)
+    ksteps = max(1, args.steps)
+    for _ in range(ksteps):
+        step()
+    torch.cuda.synchronize()
+    kt = ctx.read_timing()
+    ctx.set_timing(False)
+    log("maze_levels kernel time per step (ms): " +
+        ", ".join(f"{k}={v[0] / ksteps:.3f} ({v[1] / ksteps:.1f} launches)" for k, v in kt.items() if v[1]))
+    roof = None
+    if kt.get("ccl_local", (0, 0))[1]:
+        # labelling local pass: 2 B/px per bit-plane pass (class labels and
+        # seed weights read), 1 B/px for the final labelling (recon read)
+        ms_l, n_l = kt["ccl_local"]
+        passes = n_l / ksteps - 1
+        alg_launch = (passes * 2 * px + px) / (passes + 1)
+        avg_s = ms_l / n_l / 1e3
+        achieved = alg_launch / avg_s / 1e9
+        kind = max(kt, key=lambda k: kt[k][0])
+        roof = {"bound": "hbm", "kernel": "ccl_local", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "alg_bytes_per_launch": int(alg_launch), "avg_launch_us": round(avg_s * 1e6, 3),
+                "launches_per_step": round(n_l / ksteps, 2), "dominant_kind": kind,
+                "per_kind_ms_per_step": {k: round(v[0] / ksteps, 4) for k, v in kt.items() if v[1]}}
     ctx.close()
     out = {
-        "metric": "Mpix/s many-valued maze reconstruct+CCL @16384^2 u8 bit-exact",
+        "metric": f"Mpix/s many-valued maze reconstruct+CCL @{n}^2 u8 bit-exact",
         "value": round(args.steps * px / elapsed / 1e6, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
         "p50_ms": round(statistics.median(lat), 3), "p99_ms": round(pct(lat, 0.99), 3), "higher_is_better": True,
@@ -766,7 +793,7 @@ def bench_maze_levels(args):
         "first_call_ms": round(first_ms, 1), "rounds_ms": round(rounds_ms, 1), "rounds": rounds,
         "config": {"workload": f"{n}x{n} u8 maze, corridors 150..220 + noise, 5x5 seed: recon4 -> >100 -> bwlabel4 "
                                "(engine auto)", "pixels": px},
-        "roofline": None,
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

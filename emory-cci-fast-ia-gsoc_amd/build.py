@@ -28,7 +28,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unused-value",
 ]
-SOURCES = ["recon.hip", "mtree.hip", "ccl.hip", "dist.hip", "api.hip", "batch.hip", "server.hip", "nsx_gen.cpp"]
+SOURCES = ["recon.hip", "mtree.hip", "bitplane.hip", "ccl.hip", "dist.hip", "api.hip", "batch.hip", "server.hip", "nsx_gen.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -292,8 +292,10 @@ nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value) {
         return NSX_OK;
     }
     if (param == NSX_TUNE_ENGINE) {
-        if (value != NSX_ENGINE_ROUNDS && value != NSX_ENGINE_TREE && value != NSX_ENGINE_AUTO)
-            return ctx->fail(NSX_ERR_INVALID, "engine must be NSX_ENGINE_ROUNDS, NSX_ENGINE_TREE or NSX_ENGINE_AUTO");
+        if (value != NSX_ENGINE_ROUNDS && value != NSX_ENGINE_TREE && value != NSX_ENGINE_AUTO &&
+            value != NSX_ENGINE_PLANES)
+            return ctx->fail(NSX_ERR_INVALID,
+                             "engine must be NSX_ENGINE_ROUNDS, NSX_ENGINE_TREE, NSX_ENGINE_PLANES or NSX_ENGINE_AUTO");
         ctx->engine = value;
         return NSX_OK;
     }
@@ -812,6 +814,20 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
             if (ncomp) *ncomp = n;
             return NSX_OK;
         }
+    }
+    if (mask->dtype == NSX_U8 && ctx->engine == NSX_ENGINE_PLANES) {
+        // NSX_ENGINE_PLANES: both reconstructions through the bit planes
+        // (fixed launches), then the labelling; one synchronisation
+        NSX_TRY(recon_bitplane_u8(ctx, RM_RECON, &vm, vk, vr, rconn, s));
+        NSX_TRY(recon_bitplane_u8(ctx, RM_FILL, nullptr, vr, vf, fconn, s));
+        NSX_TRY(ccl_run(ctx, vf, vl, p, dw + 12, s));
+        int32_t n = 0;
+        NSX_TRY(read_count(ctx, dw + 12, &n, s));
+        if (recon_out && recon_out->mem == NSX_MEM_HOST) NSX_TRY(from_device(ctx, vr, recon_out, s));
+        if (fill_out && fill_out->mem == NSX_MEM_HOST) NSX_TRY(from_device(ctx, vf, fill_out, s));
+        NSX_TRY(from_device(ctx, vl, labels_out, s));
+        if (ncomp) *ncomp = n;
+        return NSX_OK;
     }
     if (mask->dtype == NSX_I32 && ctx->i32_invert == NSX_I32_INVERT_WRAP) {
         // int32 fill under wrapping negation is the constant INT_MIN (see

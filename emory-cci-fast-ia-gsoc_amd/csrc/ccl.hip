@@ -747,8 +747,8 @@ nsx_status ccl_launch(nsx_ctx* ctx, const CclBatch& B, const CclParams& p, hipSt
     A.floor_mmax = A.wmax ? p.floor_mmax : nullptr;
     if (A.level_labels && !A.img0.lout)
         return ctx->fail(NSX_ERR_INVALID, "level labels need a labels image");
-    if (A.wmax && (!p.threshold || B.in_dtype != NSX_U8 || B.nimg != 1))
-        return ctx->fail(NSX_ERR_INVALID, "level labelling needs one thresholded u8 image");
+    if (A.wmax && (B.in_dtype != NSX_U8 || B.nimg != 1))
+        return ctx->fail(NSX_ERR_INVALID, "level labelling needs one u8 image");
     A.out_mode = p.out_mode;
     A.nwords = B.pix_total / 32;
     A.nblocks = (int)(B.pix_total / BLOCK_BITS);
@@ -826,6 +826,29 @@ nsx_status ccl_level_run(nsx_ctx* ctx, const View& mask, const View& weight, con
         p.max_size = INT32_MAX;
         B.ncomp_dev = ncomp_dev;
     }
+    return ccl_launch(ctx, B, p, s);
+}
+
+nsx_status ccl_class_fold_run(nsx_ctx* ctx, const View& classes, const View& weight, const View& out, int conn,
+                              hipStream_t s) {
+    if ((int64_t)classes.rows * classes.cols == 0) return NSX_OK;
+    CclImg D;
+    ccl_fill_desc(&D, classes, out, 0, 0);
+    D.wgt = weight.data;
+    D.pw = weight.pitch;
+    CclBatch B{};
+    B.nimg = 1;
+    B.ntiles = ccl_tiles(classes.rows, classes.cols);
+    B.pix_total = ccl_pix_span(classes.rows, classes.cols);
+    B.host_desc = &D;
+    B.in_dtype = NSX_U8;
+    B.ws_base = WS_CCL_LABEL;
+    B.ncomp_dev = (int32_t*)ctx->d_stats + 30;
+    CclParams p;
+    p.conn = conn;
+    p.out_mode = CO_LEVEL;
+    p.threshold = false;   // components: equal-valued nonzero neighbours
+    p.level = 255;
     return ccl_launch(ctx, B, p, s);
 }
 

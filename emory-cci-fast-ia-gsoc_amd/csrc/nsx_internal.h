@@ -149,6 +149,9 @@ enum WsSlot {
     WS_MT2_CTL,
     WS_SERVE_TABLES,     // serving: descriptors, tile tables and counts of a shared launch set
     WS_SCAN_SEG,         // image-wide column scans: per-segment clamp compositions
+    WS_BP_L,             // bit-plane reconstruction: class labels of a pass (u8)
+    WS_BP_W,             //   and its seed weights (u8)
+    WS_BP_COPY,          //   an input that is also the output, copied first
     WS_NSLOTS
 };
 
@@ -254,6 +257,13 @@ nsx_status mtree_launch(nsx_ctx* ctx, int mode, const View* marker, const View& 
                         hipStream_t s, uint32_t** err_dev, int ws_base = WS_MT_PAR);
 int mtree_tiles(int rows, int cols);
 
+// ---- reconstruction through bit planes (u8) ---------------------------------------------
+// u8 imreconstruct (RM_RECON) or gray hole filling (RM_FILL, marker unused)
+// into `out` in eight labelling passes, one per bit from the top; enqueues
+// work only.  Any image content, a fixed number of launches.
+nsx_status recon_bitplane_u8(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& out, int conn,
+                             hipStream_t s);
+
 // ---- connected components ---------------------------------------------------------
 enum CclOut {
     CO_ROOT = 0,          // int32: root raster index, background -1
@@ -333,6 +343,12 @@ int ccl_tiles(int rows, int cols);
 int64_t ccl_pix_span(int rows, int cols);
 void ccl_fill_desc(void* out, const View& in, const View& o, int tile_base, int64_t pix_base);
 nsx_status ccl_launch(nsx_ctx* ctx, const CclBatch& B, const CclParams& p, hipStream_t s);
+
+// Components of equal nonzero values of the u8 image `classes`; each folds its
+// largest `weight` into the u8 image `out` (out = max(out, weight max) on
+// the nonzero pixels).  Enqueues work only.
+nsx_status ccl_class_fold_run(nsx_ctx* ctx, const View& classes, const View& weight, const View& out, int conn,
+                              hipStream_t s);
 
 // Area filter on int32 root labels (background -1), out int32 or u8.
 nsx_status ccl_area_labeled(nsx_ctx* ctx, const View& in, const View& out, int min_size, int max_size,

@@ -1642,13 +1642,21 @@ int recon_predicted_rounds(nsx_ctx* ctx, int mode, int dtype, int conn, int ntil
 }
 
 // u8 reconstructions that take more propagation rounds than this are finished
-// through the component tree under NSX_ENGINE_AUTO (a round costs a tile
-// visit's latency, ~10-25 us; the tree costs a fixed few ms at 4096^2).
+// through the bit planes under NSX_ENGINE_AUTO (a round costs a tile visit's
+// latency, ~10-25 us; the eight bit-plane passes a fixed ~0.8 ms at 4096^2).
 constexpr int AUTO_TREE_ROUNDS = 256;
 
+// A u8 reconstruction with a fixed number of launches: the component tree
+// under NSX_ENGINE_TREE (*ok false when one of its bounds was hit), the bit
+// planes otherwise.
 static nsx_status recon_tree(nsx_ctx* ctx, int mode, const View* marker, const View& mask, const View& state,
                              int conn, hipStream_t s, bool* ok) {
     *ok = false;
+    if (ctx->engine != NSX_ENGINE_TREE) {
+        NSX_TRY(recon_bitplane_u8(ctx, mode, marker, mask, state, conn, s));
+        *ok = true;
+        return NSX_OK;
+    }
     uint32_t* err = nullptr;
     NSX_TRY(mtree_launch(ctx, mode, marker, mask, state, conn, s, &err));
     if (!err) return NSX_OK;
@@ -1669,12 +1677,13 @@ nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mas
     conn = conn == 8 ? 8 : 4;
     ImgDesc D = make_desc(marker, mask, state, 0);
     const bool tree_ok = (mode == RM_RECON || mode == RM_FILL) && mask.dtype == NSX_U8;
-    const bool tree_first = tree_ok && (ctx->engine == NSX_ENGINE_TREE ||
+    const bool tree_first = tree_ok && (ctx->engine == NSX_ENGINE_TREE || ctx->engine == NSX_ENGINE_PLANES ||
                                         (ctx->engine == NSX_ENGINE_AUTO &&
                                          recon_predicted_rounds(ctx, mode, mask.dtype, conn, D.ntx * D.nty, 0) >
                                              AUTO_TREE_ROUNDS));
     if (tree_first) {
-        // the component tree; propagation only when one of its bounds was hit
+        // bit planes, or the component tree (propagation only when one of its
+        // bounds was hit)
         bool ok = false;
         NSX_TRY(recon_tree(ctx, mode, marker, mask, state, conn, s, &ok));
         if (ok) return NSX_OK;
@@ -1692,8 +1701,8 @@ nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mas
     nsx_status st = recon_complete(ctx, &run, s, nullptr, cap, &gave_up);
     run.release();
     if (st != NSX_OK || !gave_up) return st;
-    // many rounds: the content carries values far; the tree's cost does not
-    // depend on that (the rounds' partial state is a lower bound it overwrites)
+    // many rounds: the content carries values far; the bit planes' cost does
+    // not depend on that (they overwrite the rounds' partial state)
     bool ok = false;
     NSX_TRY(recon_tree(ctx, mode, marker, mask, state, conn, s, &ok));
     if (ok) return NSX_OK;

@@ -451,7 +451,7 @@ void nsx_server::serve(int slot, std::vector<Request*>& rs) {
         std::vector<Job*> live;
         for (Job* j : grp.second)
             if (j->st == NSX_OK) live.push_back(j);
-        if (live.size() >= 2 && ctx->engine != NSX_ENGINE_TREE) {
+        if (live.size() >= 2 && (ctx->engine == NSX_ENGINE_ROUNDS || ctx->engine == NSX_ENGINE_AUTO)) {
             bool short_rounds = false;
             run_shared(ctx, live, pinned[slot], &short_rounds);
             ++sets;

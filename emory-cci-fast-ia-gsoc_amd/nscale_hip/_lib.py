@@ -23,6 +23,7 @@ NSX_TUNE_FILL_SCANS = 3
 NSX_ENGINE_ROUNDS = 0
 NSX_ENGINE_TREE = 1
 NSX_ENGINE_AUTO = 2
+NSX_ENGINE_PLANES = 3
 NSX_OPT_I32_INVERT = 0
 NSX_I32_INVERT_WRAP = 0
 NSX_I32_INVERT_SATURATE = 1

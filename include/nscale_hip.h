@@ -110,8 +110,10 @@ nsx_status nsx_ctx_read_tile_counts(nsx_ctx* ctx, int64_t* first_pass, int64_t* 
  * launches whatever the image holds, so its time does not grow with the
  * distances values travel (long many-valued paths), at a higher cost on
  * ordinary images;
- * NSX_ENGINE_AUTO (default): rounds, switching to the tree when a
- * reconstruction passes 256 rounds, and starting with the tree when the last
+ * NSX_ENGINE_PLANES: through the bit planes of the result, eight labelling
+ * passes (one per bit, from the top) whatever the image holds;
+ * NSX_ENGINE_AUTO (default): rounds, switching to the bit planes when a
+ * reconstruction passes 256 rounds, and starting with them when the last
  * reconstruction of the same kind and size did.
  * Other types and modes always use rounds.  Results are the same. */
 /* NSX_TUNE_FILL_SCANS: image-wide row + column scan pairs run before the
@@ -127,7 +129,7 @@ nsx_status nsx_ctx_read_tile_counts(nsx_ctx* ctx, int64_t* first_pass, int64_t* 
  * alternate until nothing changes. */
 enum { NSX_TUNE_VISIT_SWEEPS = 0, NSX_TUNE_LEVELSET_LEVELS = 1, NSX_TUNE_ENGINE = 2, NSX_TUNE_FILL_SCANS = 3,
        NSX_TUNE_NPARAMS = 4 };
-enum { NSX_ENGINE_ROUNDS = 0, NSX_ENGINE_TREE = 1, NSX_ENGINE_AUTO = 2 };
+enum { NSX_ENGINE_ROUNDS = 0, NSX_ENGINE_TREE = 1, NSX_ENGINE_AUTO = 2, NSX_ENGINE_PLANES = 3 };
 nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value);
 nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value);
 

@@ -151,6 +151,30 @@ def test_maze_16384_closed_form(nh, gpu):
     assert torch.equal(lab, corridors.to(torch.int32))
 
 
+def test_maze_levels_16384_closed_form(nh, gpu):
+    """The many-valued maze at 16384^2 (corridors 150..220 plus noise; the
+    level-set path declines it) under the default engine: rounds give way to
+    the bit planes past 256 rounds.  Every corridor pixel is reached above 100
+    and every wall stays below it, so the labels are exactly one component, the
+    corridor set; the second call starts with the bit planes and agrees."""
+    import torch
+    from nscale_hip.workloads import maze_levels_images
+    marker, mask = maze_levels_images(16384)
+    dev = torch.device("cuda:0")
+    dm, dk = torch.from_numpy(marker).to(dev), torch.from_numpy(mask).to(dev)
+    del marker, mask
+    corridors = (dk > 100).to(torch.int32)
+    ctx = nh.Context(0)
+    try:
+        r, lab, ncomp = nh.recon_label(dm, dk, 4, 100.0, 4, ctx=ctx)
+        assert ncomp == 1 and torch.equal(lab, corridors)
+        assert bool(((r > 100) == (dk > 100)).all()) and bool((r <= dk).all())
+        r2, lab2, n2 = nh.recon_label(dm, dk, 4, 100.0, 4, ctx=ctx)
+        assert n2 == 1 and torch.equal(r2, r) and torch.equal(lab2, lab)
+    finally:
+        ctx.close()
+
+
 def test_serve_trace_poisson(nh, orc, gpu):
     """40 requests of the serving trace; arrivals replayed with exponential
     gaps at about the sequential service rate, on a 4-slot server whose

@@ -1,6 +1,7 @@
-"""u8 reconstruction and hole filling through the component tree
-(NSX_ENGINE_TREE) against the oracle: tile-edge shapes, plateaus, few-valued and
-noisy masks, both connectivities; the same results as the propagation rounds
+"""u8 reconstruction and hole filling through the fixed-launch engines, the
+component tree (NSX_ENGINE_TREE) and the bit planes (NSX_ENGINE_PLANES),
+against the oracle: tile-edge shapes, plateaus, few-valued and noisy masks,
+both connectivities; the same results as the propagation rounds
 (NSX_ENGINE_ROUNDS)."""
 import numpy as np
 import pytest
@@ -27,9 +28,17 @@ def _masks(rng, shape):
     yield "smooth", np.clip(smooth - smooth.min(), 0, 255).astype(np.uint8)
 
 
+ENGINES = ["tree", "planes"]
+
+
+def _engine(nh, name):
+    return {"tree": nh._lib.NSX_ENGINE_TREE, "planes": nh._lib.NSX_ENGINE_PLANES}[name]
+
+
+@pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("conn", [4, 8])
-def test_mtree_recon_vs_oracle(nh, orc, gpu, conn):
-    ctx = _ctx(nh, nh._lib.NSX_ENGINE_TREE)
+def test_mtree_recon_vs_oracle(nh, orc, gpu, conn, engine):
+    ctx = _ctx(nh, _engine(nh, engine))
     rng = np.random.default_rng(300 + conn)
     try:
         for shape in SHAPES:
@@ -46,9 +55,10 @@ def test_mtree_recon_vs_oracle(nh, orc, gpu, conn):
         ctx.close()
 
 
+@pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("conn", [4, 8])
-def test_mtree_fill_vs_oracle(nh, orc, gpu, conn):
-    ctx = _ctx(nh, nh._lib.NSX_ENGINE_TREE)
+def test_mtree_fill_vs_oracle(nh, orc, gpu, conn, engine):
+    ctx = _ctx(nh, _engine(nh, engine))
     rng = np.random.default_rng(400 + conn)
     try:
         for shape in SHAPES:
@@ -60,13 +70,14 @@ def test_mtree_fill_vs_oracle(nh, orc, gpu, conn):
         ctx.close()
 
 
-def test_mtree_disks_match_rounds(nh, orc, gpu):
-    """The disks construction at 1500 x 1700: component tree == rounds ==
+@pytest.mark.parametrize("engine", ENGINES)
+def test_mtree_disks_match_rounds(nh, orc, gpu, engine):
+    """The disks construction at 1500 x 1700: fixed-launch engine == rounds ==
     oracle for recon (8) and fill (4), device buffers."""
     import torch
     mask = nh.gen_disks_u8(1500, 1700, 500, 0x5EED0601)
     marker = nh.gen_marker_u8(mask, 30)
-    auto, rounds = _ctx(nh, nh._lib.NSX_ENGINE_TREE, 3), _ctx(nh, nh._lib.NSX_ENGINE_ROUNDS, 3)
+    auto, rounds = _ctx(nh, _engine(nh, engine), 3), _ctx(nh, nh._lib.NSX_ENGINE_ROUNDS, 3)
     try:
         dm, dk = torch.from_numpy(marker).cuda(), torch.from_numpy(mask).cuda()
         a = nh.imreconstruct(dm, dk, 8, ctx=auto)
@@ -84,7 +95,7 @@ def test_mtree_disks_match_rounds(nh, orc, gpu):
 def test_auto_engine_switches_on_long_paths(nh, orc, gpu):
     """NSX_ENGINE_AUTO (the default) on a many-valued maze (1024^2, about 1500
     propagation rounds): the first call passes 256 rounds and finishes through
-    the component tree, the next ones start with it; every result equals the
+    the bit planes, the next ones start with them; every result equals the
     oracle, for imreconstruct, recon_label and the chain's fill."""
     import torch
     from nscale_hip.workloads import maze_levels_images
@@ -104,5 +115,38 @@ def test_auto_engine_switches_on_long_paths(nh, orc, gpu):
             rr, ff, ll, nn = nh.segment_chain(dm, dk, 4, 4, 100.0, 4, 11, 1 << 30, ctx=ctx)
             assert np.array_equal(rr.cpu().numpy(), want)
             assert np.array_equal(ff.cpu().numpy(), orc.fill_holes(want, False, 4)), f"chain {rep}"
+    finally:
+        ctx.close()
+
+
+def test_planes_reject_output_aliasing_an_input(nh, gpu):
+    """As with every engine, an output that aliases an input is rejected at
+    the boundary (nothing runs)."""
+    import torch
+    mask = nh.gen_disks_u8(300, 200, 40, 0x5EED0602)
+    marker = nh.gen_marker_u8(mask, 30)
+    ctx = _ctx(nh, nh._lib.NSX_ENGINE_PLANES)
+    try:
+        dm, dk = torch.from_numpy(marker).cuda(), torch.from_numpy(mask).cuda()
+        with pytest.raises(nh.NsxError):
+            nh.imreconstruct(dm, dk, 8, out=dm, ctx=ctx)
+        with pytest.raises(nh.NsxError):
+            nh.imfillHoles(dk, False, 4, out=dk, ctx=ctx)
+    finally:
+        ctx.close()
+
+
+def test_planes_chain_matches_oracle(nh, orc, gpu):
+    """segment_chain under NSX_ENGINE_PLANES on a 1030 x 1500 disks image."""
+    import torch
+    mask = nh.gen_disks_u8(1030, 1500, 300, 0x5EED0603)
+    marker = nh.gen_marker_u8(mask, 30)
+    er, ef, elab, en = orc.chain_u8(marker, mask, 8, 4, 128, 8, 11, 1000)
+    ctx = _ctx(nh, nh._lib.NSX_ENGINE_PLANES)
+    try:
+        dm, dk = torch.from_numpy(marker).cuda(), torch.from_numpy(mask).cuda()
+        r, f, lab, n = nh.segment_chain(dm, dk, 8, 4, 128.0, 8, 11, 1000, ctx=ctx)
+        assert np.array_equal(r.cpu().numpy(), er) and np.array_equal(f.cpu().numpy(), ef)
+        assert np.array_equal(lab.cpu().numpy(), elab) and n == en
     finally:
         ctx.close()
