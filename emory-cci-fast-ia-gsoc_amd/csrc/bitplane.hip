@@ -61,6 +61,59 @@ __global__ __launch_bounds__(256) void bp_prep_kernel(const char* marker, int64_
     ((uint8_t*)(W + (int64_t)y * pw))[x] = (uint8_t)(seed ? t : 0u);
 }
 
+// The same for four pixels per thread, with 4-byte loads and stores: rows of
+// every plane 4-byte aligned and cols a multiple of 4.  The neighbours left of
+// x0 and right of x0 + 3 come from single bytes.
+template <bool FILL, int CONN>
+__global__ __launch_bounds__(256) void bp_prep4_kernel(const char* marker, int64_t pm, const char* mask, int64_t pk,
+                                                       const char* P, int64_t pp, char* L, int64_t pl, char* W,
+                                                       int64_t pw, int rows, int cols, uint32_t bit) {
+    const int x0 = (blockIdx.x * 64 + (threadIdx.x & 63)) * 4;
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x0 >= cols || y >= rows) return;
+    const uint32_t mw = *(const uint32_t*)(mask + (int64_t)y * pk + x0);
+    const uint32_t kw = FILL ? 0u : *(const uint32_t*)(marker + (int64_t)y * pm + x0);
+    // P around the four pixels: rows y-1, y, y+1, columns x0-1 .. x0+4
+    // (outside the image: 0, which no class exceeds... a prefix 0 never counts
+    // as larger, so it stands for "no neighbour")
+    uint32_t pr[3][6];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int yy = y + r - 1;
+        const bool in = yy >= 0 && yy < rows;
+        const uint8_t* row = (const uint8_t*)(P + (int64_t)(in ? yy : y) * pp);
+        const uint32_t w = in ? *(const uint32_t*)(row + x0) : 0u;
+        pr[r][0] = (in && x0 > 0) ? row[x0 - 1] : 0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) pr[r][1 + j] = (w >> (8 * j)) & 0xFFu;
+        pr[r][5] = (in && x0 + 4 < cols) ? row[x0 + 4] : 0u;
+    }
+    uint32_t lw = 0u, ww = 0u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int x = x0 + j;
+        const uint32_t m = (mw >> (8 * j)) & 0xFFu;
+        const uint32_t g = FILL ? 255u - m : m;
+        const uint32_t p = pr[1][1 + j];
+        const uint32_t t = p | bit;
+        const bool fg = g >= t;
+        bool seed;
+        if (FILL) {
+            seed = x == 0 || y == 0 || x == cols - 1 || y == rows - 1;
+        } else {
+            seed = min(m, (kw >> (8 * j)) & 0xFFu) >= t;
+        }
+        // neighbours outside the image read 0 above: never larger than p
+        seed = seed || pr[1][j] > p || pr[1][j + 2] > p || pr[0][1 + j] > p || pr[2][1 + j] > p;
+        if (CONN == 8) seed = seed || pr[0][j] > p || pr[0][j + 2] > p || pr[2][j] > p || pr[2][j + 2] > p;
+        seed = seed && fg;
+        lw |= (fg ? p + 1u : 0u) << (8 * j);
+        ww |= (seed ? t : 0u) << (8 * j);
+    }
+    *(uint32_t*)(L + (int64_t)y * pl + x0) = lw;
+    *(uint32_t*)(W + (int64_t)y * pw + x0) = ww;
+}
+
 __global__ void bp_zero_kernel(char* P, int64_t pp, int rows, int cols) {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -98,9 +151,27 @@ nsx_status recon_bitplane_u8(nsx_ctx* ctx, int mode, const View* marker, const V
         if (alias_k) { g.data = (char*)pc; g.pitch = (int64_t)pitch; }
     }
     hipLaunchKernelGGL(bp_zero_kernel, grid, block, 0, s, out.data, out.pitch, out.rows, out.cols);
+    // four pixels per thread when every plane's rows are 4-byte aligned
+    const auto a4 = [](const char* p, int64_t pitch) { return ((uintptr_t)p % 4 == 0) && pitch % 4 == 0; };
+    const bool wide = mask.cols % 4 == 0 && a4(g.data, g.pitch) && a4(out.data, out.pitch) && a4(L.data, L.pitch) &&
+                      a4(W.data, W.pitch) && (fill || a4(mk, pm));
+    const dim3 grid4((mask.cols / 4 + 63) / 64, (mask.rows + 3) / 4);
     for (int b = 7; b >= 0; --b) {
         const uint32_t bit = 1u << b;
-        if (fill && conn == 8)
+        if (wide) {
+            if (fill && conn == 8)
+                hipLaunchKernelGGL((bp_prep4_kernel<true, 8>), grid4, block, 0, s, mk, pm, g.data, g.pitch, out.data,
+                                   out.pitch, L.data, L.pitch, W.data, W.pitch, mask.rows, mask.cols, bit);
+            else if (fill)
+                hipLaunchKernelGGL((bp_prep4_kernel<true, 4>), grid4, block, 0, s, mk, pm, g.data, g.pitch, out.data,
+                                   out.pitch, L.data, L.pitch, W.data, W.pitch, mask.rows, mask.cols, bit);
+            else if (conn == 8)
+                hipLaunchKernelGGL((bp_prep4_kernel<false, 8>), grid4, block, 0, s, mk, pm, g.data, g.pitch, out.data,
+                                   out.pitch, L.data, L.pitch, W.data, W.pitch, mask.rows, mask.cols, bit);
+            else
+                hipLaunchKernelGGL((bp_prep4_kernel<false, 4>), grid4, block, 0, s, mk, pm, g.data, g.pitch, out.data,
+                                   out.pitch, L.data, L.pitch, W.data, W.pitch, mask.rows, mask.cols, bit);
+        } else if (fill && conn == 8)
             hipLaunchKernelGGL((bp_prep_kernel<true, 8>), grid, block, 0, s, mk, pm, g.data, g.pitch, out.data,
                                out.pitch, L.data, L.pitch, W.data, W.pitch, mask.rows, mask.cols, bit);
         else if (fill)
