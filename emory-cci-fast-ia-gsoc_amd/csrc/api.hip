@@ -291,6 +291,11 @@ nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value) {
         ctx->fill_scans = value;
         return NSX_OK;
     }
+    if (param == NSX_TUNE_DRAIN) {
+        if (value != 0 && value != 1) return ctx->fail(NSX_ERR_INVALID, "drain must be 0 or 1");
+        ctx->drain = value;
+        return NSX_OK;
+    }
     if (param == NSX_TUNE_ENGINE) {
         if (value != NSX_ENGINE_ROUNDS && value != NSX_ENGINE_TREE && value != NSX_ENGINE_AUTO &&
             value != NSX_ENGINE_PLANES)
@@ -320,7 +325,17 @@ nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value) {
         *value = ctx->fill_scans;
         return NSX_OK;
     }
+    if (param == NSX_TUNE_DRAIN) {
+        *value = ctx->drain;
+        return NSX_OK;
+    }
     return NSX_ERR_INVALID;
+}
+
+nsx_status nsx_ctx_drain_failures(const nsx_ctx* ctx, int64_t* n) {
+    if (!ctx || !n) return NSX_ERR_INVALID;
+    *n = ctx->drain_failures;
+    return NSX_OK;
 }
 
 nsx_status nsx_ctx_set_option(nsx_ctx* ctx, int option, int value) {
@@ -855,9 +870,11 @@ nsx_status nsx_segment_chain(nsx_ctx* ctx, const nsx_img* marker, const nsx_img*
     bf.ws_base = WS_FILL_STAMP;
     bf.stats = ctx->d_stats + 8;
     ReconRun rr, rf;
+    // a drain stops where NSX_ENGINE_AUTO would switch to the bit planes
     // NSX_ENGINE_AUTO on u8: stages whose last run of this size passed the
     // tree threshold go stage by stage through recon_run, which picks the tree
     const bool auto_u8 = mask->dtype == NSX_U8 && ctx->engine == NSX_ENGINE_AUTO;
+    if (auto_u8) br.drain_cap = bf.drain_cap = 256 + 1;
     if (auto_u8 && (recon_predicted_rounds(ctx, RM_RECON, NSX_U8, rconn, br.ntiles, 0) > 256 ||
                     recon_predicted_rounds(ctx, RM_FILL, NSX_U8, fconn, bf.ntiles, 0) > 256)) {
         NSX_TRY(recon_run(ctx, RM_RECON, &vm, vk, vr, rconn, s));

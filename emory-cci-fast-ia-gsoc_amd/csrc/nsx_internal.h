@@ -55,6 +55,7 @@ enum TimedKind {
     TK_CCL_FINAL,         // relabel + output write
     TK_CCL_SMALL,         // area / keep / scan kernels (one event pair per call)
     TK_RECON_SCAN,        // image-wide row / column scans before a hole filling's rounds
+    TK_RECON_DRAIN,       // the persistent drain of a reconstruction's rounds (one launch)
     TK_NKINDS
 };
 }  // namespace nsx
@@ -80,6 +81,8 @@ struct nsx_ctx {
     int levelset_levels = 3;
     int engine = NSX_ENGINE_AUTO;     // NSX_TUNE_ENGINE
     int fill_scans = 0;               // NSX_TUNE_FILL_SCANS
+    int drain = 0;                    // NSX_TUNE_DRAIN
+    int64_t drain_failures = 0;       // drains that gave up (restarted with per-round launches)
 
     // Options (nsx_ctx_set_option).
     int i32_invert = NSX_I32_INVERT_WRAP;
@@ -197,6 +200,8 @@ struct ReconBatch {
     int ws_base = 0;                // first of 3 consecutive workspace slots
     int64_t* stats = nullptr;       // device stats words (default ctx->d_stats)
     int visit_sweeps = 0;           // sweep pairs per visit; 0: the context's setting
+    int drain = 0;                  // rounds in one persistent launch: 1 on, -1 off, 0 the context's setting
+    int drain_cap = 0;              // the drain stops before this round (0: no cap); the host goes on from there
 };
 struct ReconRunImpl;
 // A launched reconstruction whose convergence has not been checked yet.

@@ -743,14 +743,168 @@ void recon_round_kernel(ReconArgs A, int round) {
     }
 }
 
-// dst = sum of one round's NSEG counters (zero iff the round has no work).
-__global__ void sum_counts_kernel(int32_t* dst, const int32_t* round_counts) {
+// ---- persistent drain of the round lists -------------------------------------------
+// One launch runs every propagation round after the first pass: its blocks stay
+// resident and go through the rounds' device lists in the same order as the
+// per-round launches, with a grid-wide barrier where a kernel boundary was.  The
+// grid is the number of blocks the GPU holds at once (occupancy x CUs), so every
+// block is running when the first one waits.  The launch sequence no longer
+// depends on the content: it needs no round prediction and no host check, and
+// a graph of it has the same nodes however many rounds a replay takes.
+//
+// A barrier: the block's first thread writes its XCD's L2 back (release, so
+// that the tiles it stored and the list entries it appended reach memory),
+// adds one to the arrival counter, polls it with coherent loads until every
+// block of this barrier has arrived, then invalidates its L2 (acquire) before
+// the block reads the next round's counters, list and tiles.  Arrivals only
+// grow: barrier k waits for k x grid.  Every wait is bounded: a barrier that
+// sees no new arrival for `stall` ticks of the 100 MHz real-time counter (a
+// block that never became resident, on a GPU shared with other work) sets the
+// error word, every block leaves at its next barrier, and the host restarts
+// the reconstruction with per-round launches.  Rounds that end with an empty
+// list make every block leave together (all read the same final counters).
+struct Drain {
+    uint32_t* ctl;      // [0] barrier arrivals, [1] error, [2] the round the drain stopped at
+    int max_round;      // rounds below this one run here; the host continues from there
+    uint32_t stall;     // real-time ticks a barrier waits without a new arrival
+};
+// The drain's control words sit in the counters' buffer, 128 bytes past the
+// sum word; the reset kernel clears them with the counters.
+constexpr int CTL_OFF = CNT_SUM + 32;
+constexpr int CTL_WORDS = 4;
+
+__device__ __forceinline__ uint32_t ld_agent(const void* p) {
+    return __hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(void* p, uint32_t v) {
+    __hip_atomic_store((uint32_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier number target / gridDim.x; false (in every thread of the
+// block) when the drain has failed and the block must leave.
+__device__ __forceinline__ bool drain_barrier(const Drain& P, uint32_t target, uint32_t* okw) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ok = 1u;
+        __threadfence();
+        atomicAdd(&P.ctl[0], 1u);
+        uint32_t last = 0u;
+        uint64_t since = __builtin_amdgcn_s_memrealtime();
+        for (;;) {
+            const uint32_t v = ld_agent(&P.ctl[0]);
+            if (v >= target) break;
+            if (ld_agent(&P.ctl[1]) != 0u) { ok = 0u; break; }
+            const uint64_t now = __builtin_amdgcn_s_memrealtime();
+            if (v != last) {
+                last = v;
+                since = now;
+            } else if (now - since > (uint64_t)P.stall) {
+                atomicOr(&P.ctl[1], 1u);
+                ok = 0u;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();
+        *okw = ok;
+    }
+    __syncthreads();
+    return *okw != 0u;
+}
+
+// Round header of the drain, by block 0: clears the counters of round + 2 and
+// counts the round's work into the statistics.
+__device__ __forceinline__ void drain_header(const ReconArgs& A, int round, int n, int lane) {
+    if (lane < NSEG) st_agent(&A.counts[cnt_idx(round + 2, lane)], 0u);
+    if (lane == 0) {
+        A.stats[0] += 1;
+        A.stats[1] += n;
+        if (A.cum) atomicAdd(A.cum, (unsigned long long)n);
+    }
+}
+
+// The work items of one round: the same XCD-aware split of the list as the
+// per-round kernels.  `excl` (LDS) holds the segments' exclusive prefix over
+// the counters, NSEG + 1 words.  visit(t, round) processes global tile t.
+template <typename Visit>
+__device__ __forceinline__ void drain_round_items(const ReconArgs& A, int round, const uint32_t* excl, int n,
+                                                  Visit visit) {
+    const int nx = A.nimg > 1 ? 1 : (gridDim.x < 8 ? (int)gridDim.x : 8);
+    const int xcd = (int)blockIdx.x % nx, xb = (int)blockIdx.x / nx;
+    const int xstride = ((int)gridDim.x - xcd + nx - 1) / nx;
+    const int32_t* list = (round & 1) ? A.list1 : A.list0;
+    const int chunk = (n + nx - 1) / nx;
+    const int lo = min(n, xcd * chunk), hi = min(n, lo + chunk);
+    for (int i = lo + xb; i < hi; i += xstride) {
+        int q = 0;
+#pragma unroll
+        for (int k = 1; k < NSEG; ++k) q += excl[k] <= (uint32_t)i ? 1 : 0;
+        q = __builtin_amdgcn_readfirstlane(q);
+        const int off = i - (int)excl[q];
+        visit(__builtin_amdgcn_readfirstlane(list[(int64_t)q * A.ntiles + off]), round);
+        __syncthreads();   // the next tile overwrites the tile's LDS
+    }
+}
+
+// The drain loop shared by the one-wave and four-wave tiles.  excl: NSEG + 1
+// words of LDS.
+template <typename Visit>
+__device__ __forceinline__ void drain_rounds(const ReconArgs& A, const Drain& P, uint32_t* okw, uint32_t* excl,
+                                             Visit visit) {
+    const int lane = threadIdx.x & 63;
+    uint32_t target = 0u;
+    for (int round = 1; round < P.max_round; ++round) {
+        // the counters of this round are final: their appends came before the barrier
+        const uint32_t cq = lane < NSEG ? ld_agent(&A.counts[cnt_idx(round, lane)]) : 0u;
+        const uint32_t incl = wave_incl_scan(cq);
+        const int n = __builtin_amdgcn_readlane((int)incl, NSEG - 1);
+        if (threadIdx.x < NSEG) excl[threadIdx.x] = incl - cq;
+        __syncthreads();
+        if (n == 0) {
+            // done; the previous round's counters are the only non-zero ones
+            // left: clear them, so every counter reads zero for the host's check
+            if (blockIdx.x == 0 && threadIdx.x < 64) {
+                if (lane < NSEG) st_agent(&A.counts[cnt_idx(round + 2, lane)], 0u);
+                if (lane == 0) st_agent(&P.ctl[2], (uint32_t)round);
+            }
+            return;
+        }
+        if (blockIdx.x == 0 && threadIdx.x < 64) drain_header(A, round, n, lane);
+        drain_round_items(A, round, excl, n, visit);
+        target += gridDim.x;
+        if (!drain_barrier(P, target, okw)) return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st_agent(&P.ctl[2], (uint32_t)P.max_round);
+}
+
+template <typename T, int MODE, int CONN>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 1 ? 3 : 2)))
+void recon_drain_kernel(ReconArgs A, Drain P) {
+    using LT = typename LdsT<T>::type;
+    __shared__ LT mb[sizeof(T) == 1 ? TS * LDS_PITCH : 1];
+    __shared__ __align__(16) LT tb[Stage<T>::elems];
+    __shared__ uint32_t hs[4 * TS];
+    __shared__ uint32_t okw, excl[NSEG];
+    drain_rounds(A, P, &okw, excl, [&](int t, int round) {
+        ImgDesc D;
+        int lt;
+        tile_of(A, t, D, lt);
+        process_tile<T, MODE, CONN, false>(A, D, lt, round, mb, tb, hs);
+    });
+}
+
+// dst = sum of one round's NSEG counters (zero iff the round has no work), or
+// -1 when the drain whose control words are `ctl` gave up.
+__global__ void sum_counts_kernel(int32_t* dst, const int32_t* round_counts, const uint32_t* ctl) {
     if (threadIdx.x == 0) {
         int n = 0;
         for (int q = 0; q < NSEG; ++q) n += round_counts[q * CSTRIDE];
-        *dst = n;
+        *dst = (ctl && ctl[1] != 0u) ? -1 : n;
     }
 }
+
+// A barrier of the drain gives up after 0.5 s without a new arrival.
+constexpr uint32_t DRAIN_STALL_TICKS = 50u * 1000u * 1000u;
 
 // Number of tiles on the border of an ntx x nty tile grid.
 __host__ __device__ inline int border_tiles(int ntx, int nty) {
@@ -809,6 +963,7 @@ __global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts,
         list0[b] = t;
     }
     if (i < 3 * NSEG) counts[i * CSTRIDE] = i == 0 ? nb : 0;
+    if (i < CTL_WORDS) counts[CTL_OFF + i] = 0;
     if (i == 0) { stats[0] = 0; stats[1] = 0; stats[2] = ntiles; stats[3] = 0; }
 }
 
@@ -1057,7 +1212,7 @@ size_t scan_segf_words(int rows, int cols) {
 // two reconstructions (the sums of their next round's counters); int64 out[2],
 // out[3] = their rounds with work.
 __global__ void chain_status_kernel(int32_t* out, const int32_t* ncomp, const int32_t* rc, const int32_t* fc,
-                                    const int64_t* rs, const int64_t* fs) {
+                                    const int64_t* rs, const int64_t* fs, const uint32_t* rctl, const uint32_t* fctl) {
     if (threadIdx.x == 0) {
         int a = 0, b = 0;
         for (int q = 0; q < NSEG; ++q) {
@@ -1065,8 +1220,8 @@ __global__ void chain_status_kernel(int32_t* out, const int32_t* ncomp, const in
             b += fc[q * CSTRIDE];
         }
         out[0] = *ncomp;
-        out[1] = a;
-        out[2] = b;
+        out[1] = (rctl && rctl[1] != 0u) ? -1 : a;   // -1: the drain gave up
+        out[2] = (fctl && fctl[1] != 0u) ? -1 : b;
         out[3] = 0;
         int64_t* o64 = reinterpret_cast<int64_t*>(out);
         o64[2] = rs[0];
@@ -1370,7 +1525,56 @@ void recon_quad_kernel(ReconArgs A, int round) {
     }
 }
 
+// The persistent drain on four-wave tiles (4-connected hole filling).
+template <typename T, int MODE>
+__global__ __launch_bounds__(QW * 64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 1 ? 4 : 3)))
+void recon_quad_drain_kernel(ReconArgs A, Drain P) {
+    using LT = typename LdsT<T>::type;
+    __shared__ LT tb[TS * LDS_PITCH];
+    __shared__ QShared Q;
+    __shared__ uint32_t okw, excl[NSEG];
+    drain_rounds(A, P, &okw, excl, [&](int t, int round) {
+        ImgDesc D;
+        int lt;
+        tile_of(A, t, D, lt);
+        process_tile_q<T, MODE, false>(A, D, lt, round, tb, Q);
+    });
+}
+
 typedef void (*RoundFn)(const ReconArgs&, int, int, hipStream_t);
+// Launches a persistent drain; returns the grid it used (0: not launched).
+typedef int (*DrainFn)(const ReconArgs&, const Drain&, int cu_count, hipStream_t);
+
+// Blocks of `kernel` one CU holds at once (at most `design`), from the
+// compiled kernel's registers and LDS.
+template <typename K>
+int resident_blocks(K kernel, int block, int design) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        nb = 0;
+    }
+    return std::max(0, std::min(nb, design));
+}
+
+template <typename T, int MODE, int CONN>
+int launch_drain(const ReconArgs& A, const Drain& P, int cu_count, hipStream_t s) {
+    static const int per_cu = resident_blocks(recon_drain_kernel<T, MODE, CONN>, 64, sizeof(T) == 1 ? 12 : 8);
+    if (per_cu <= 0) return 0;
+    const int grid = per_cu * cu_count;
+    hipLaunchKernelGGL((recon_drain_kernel<T, MODE, CONN>), dim3(grid), dim3(64), 0, s, A, P);
+    return grid;
+}
+
+template <typename T, int MODE>
+int launch_quad_drain(const ReconArgs& A, const Drain& P, int cu_count, hipStream_t s) {
+    static const int per_cu = resident_blocks(recon_quad_drain_kernel<T, MODE>, QW * 64, sizeof(T) == 1 ? 4 : 3);
+    if (per_cu <= 0) return 0;
+    const int grid = per_cu * cu_count;
+    hipLaunchKernelGGL((recon_quad_drain_kernel<T, MODE>), dim3(grid), dim3(QW * 64), 0, s, A, P);
+    return grid;
+}
+
 
 template <typename T, int MODE, int CONN, bool FIRST>
 void launch_round(const ReconArgs& A, int round, int grid, hipStream_t s) {
@@ -1382,47 +1586,56 @@ void launch_quad(const ReconArgs& A, int round, int grid, hipStream_t s) {
     hipLaunchKernelGGL((recon_quad_kernel<T, MODE, FIRST>), dim3(grid), dim3(QW * 64), 0, s, A, round);
 }
 
+struct RoundFns {
+    RoundFn first = nullptr, later = nullptr;
+    DrainFn drain = nullptr;
+};
+
 // Hole filling with 4-connectivity runs on four-wave tiles.
 template <typename T, int MODE>
-bool pick_fill(int conn, RoundFn* first, RoundFn* later) {
+bool pick_fill(int conn, RoundFns* f) {
     if (conn == 8) {
-        *first = launch_round<T, MODE, 8, true>;
-        *later = launch_round<T, MODE, 8, false>;
+        f->first = launch_round<T, MODE, 8, true>;
+        f->later = launch_round<T, MODE, 8, false>;
+        f->drain = launch_drain<T, MODE, 8>;
     } else {
-        *first = launch_quad<T, MODE, true>;
-        *later = launch_quad<T, MODE, false>;
+        f->first = launch_quad<T, MODE, true>;
+        f->later = launch_quad<T, MODE, false>;
+        f->drain = launch_quad_drain<T, MODE>;
     }
     return true;
 }
 
 template <typename T, int MODE>
-bool pick(int conn, RoundFn* first, RoundFn* later) {
+bool pick(int conn, RoundFns* f) {
     if (conn == 8) {
-        *first = launch_round<T, MODE, 8, true>;
-        *later = launch_round<T, MODE, 8, false>;
+        f->first = launch_round<T, MODE, 8, true>;
+        f->later = launch_round<T, MODE, 8, false>;
+        f->drain = launch_drain<T, MODE, 8>;
     } else {
-        *first = launch_round<T, MODE, 4, true>;
-        *later = launch_round<T, MODE, 4, false>;
+        f->first = launch_round<T, MODE, 4, true>;
+        f->later = launch_round<T, MODE, 4, false>;
+        f->drain = launch_drain<T, MODE, 4>;
     }
     return true;
 }
 
-bool select_fns(int mode, int dtype, int conn, RoundFn* first, RoundFn* later) {
+bool select_fns(int mode, int dtype, int conn, RoundFns* f) {
     switch (mode) {
         case RM_RECON:
-            if (dtype == NSX_U8) return pick<uint8_t, RM_RECON>(conn, first, later);
-            if (dtype == NSX_I32) return pick<int32_t, RM_RECON>(conn, first, later);
-            return pick<float, RM_RECON>(conn, first, later);
+            if (dtype == NSX_U8) return pick<uint8_t, RM_RECON>(conn, f);
+            if (dtype == NSX_I32) return pick<int32_t, RM_RECON>(conn, f);
+            return pick<float, RM_RECON>(conn, f);
         case RM_BINARY:
-            if (dtype == NSX_U8) return pick<uint8_t, RM_BINARY>(conn, first, later);
-            if (dtype == NSX_I32) return pick<int32_t, RM_BINARY>(conn, first, later);
+            if (dtype == NSX_U8) return pick<uint8_t, RM_BINARY>(conn, f);
+            if (dtype == NSX_I32) return pick<int32_t, RM_BINARY>(conn, f);
             return false;
         case RM_FILL:
-            if (dtype == NSX_U8) return pick_fill<uint8_t, RM_FILL>(conn, first, later);
-            if (dtype == NSX_I32) return pick_fill<int32_t, RM_FILL>(conn, first, later);
+            if (dtype == NSX_U8) return pick_fill<uint8_t, RM_FILL>(conn, f);
+            if (dtype == NSX_I32) return pick_fill<int32_t, RM_FILL>(conn, f);
             return false;
         case RM_FILLBIN:
-            if (dtype == NSX_U8) return pick_fill<uint8_t, RM_FILLBIN>(conn, first, later);
+            if (dtype == NSX_U8) return pick_fill<uint8_t, RM_FILLBIN>(conn, f);
             return false;
         default:
             return false;
@@ -1454,6 +1667,9 @@ struct ReconRunImpl {
     int grid;
     int next_round;   // first round not yet launched
     uint64_t key;     // round-count memory key
+    bool drain = false;   // rounds 1 .. next_round - 1 ran in a persistent drain
+    int mode = 0, dtype = 0, conn = 0;
+    ReconBatch batch;     // to start again with per-round launches if the drain fails
 };
 
 int recon_tiles(int rows, int cols) {
@@ -1485,10 +1701,15 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
                         ReconRun* run, int32_t* pending_dev) {
     run->release();
     if (B.ntiles <= 0) return NSX_OK;
-    RoundFn first, later;
-    if (!select_fns(mode, dtype, conn, &first, &later))
+    RoundFns fns;
+    if (!select_fns(mode, dtype, conn, &fns))
         return ctx->fail(NSX_ERR_UNSUPPORTED, "reconstruction mode/dtype not instantiated by the reference");
+    const RoundFn first = fns.first, later = fns.later;
     ReconRunImpl* R = new ReconRunImpl();
+    R->mode = mode;
+    R->dtype = dtype;
+    R->conn = conn;
+    R->batch = B;
     ReconArgs& A = R->A;
     A = ReconArgs{};
     A.nimg = B.nimg;
@@ -1564,17 +1785,37 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
         ctx->first_tiles[cum_idx] += border_start ? border_tiles(D0.ntx, D0.nty) : A.ntiles;
         ctx->toc(h0, s);
     }
-    // Rounds the same-sized problem needed last time, launched without a host check.
-    const int hint = ctx->round_hint.launch(R->key);
-    for (int r = 1; r <= hint; ++r) {
-        const int h = ctx->tic(TK_RECON_ROUND, s);
-        later(A, r, R->grid, s);
+    const bool drain = (B.drain > 0 || (B.drain == 0 && ctx->drain > 0)) && fns.drain;
+    if (drain) {
+        // every later round in one persistent launch, up to the cap; arrivals
+        // at its barriers stay below 2^31
+        Drain P{};
+        P.ctl = reinterpret_cast<uint32_t*>(A.counts) + CTL_OFF;
+        P.max_round = B.drain_cap > 0 ? B.drain_cap : (1 << 24);
+        P.max_round = std::min<int64_t>(P.max_round, (int64_t)INT32_MAX / (12 * (int64_t)ctx->cu_count) - 1);
+        P.stall = DRAIN_STALL_TICKS;
+        const int h = ctx->tic(TK_RECON_DRAIN, s);
+        const int g = fns.drain(A, P, ctx->cu_count, s);
         ctx->toc(h, s);
+        if (g > 0) {
+            R->drain = true;
+            R->next_round = P.max_round;
+        }
     }
-    R->next_round = hint + 1;
+    if (!R->drain) {
+        // Rounds the same-sized problem needed last time, launched without a host check.
+        const int hint = ctx->round_hint.launch(R->key);
+        for (int r = 1; r <= hint; ++r) {
+            const int h = ctx->tic(TK_RECON_ROUND, s);
+            later(A, r, R->grid, s);
+            ctx->toc(h, s);
+        }
+        R->next_round = hint + 1;
+    }
     NSX_HIP_TRY(ctx, hipGetLastError());
     if (pending_dev)   // a kernel rather than a copy node, so the sequence can be graph-captured
-        hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, pending_dev, &A.counts[cnt_idx(R->next_round, 0)]);
+        hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, pending_dev, &A.counts[cnt_idx(R->next_round, 0)],
+                           R->drain ? reinterpret_cast<const uint32_t*>(A.counts) + CTL_OFF : nullptr);
     NSX_HIP_TRY(ctx, hipGetLastError());
     return NSX_OK;
 }
@@ -1585,11 +1826,30 @@ nsx_status recon_complete(nsx_ctx* ctx, ReconRun* run, hipStream_t s, bool* exte
     if (gave_up) *gave_up = false;
     ReconRunImpl* R = run->impl;
     if (!R) return NSX_OK;
+    if (R->drain) {
+        // -1: the drain gave up (a block never arrived); start again with
+        // per-round launches.  Otherwise every counter is zero (converged) or
+        // the rounds go on from the drain's cap as below.
+        hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, &R->A.counts[CNT_SUM],
+                           &R->A.counts[cnt_idx(R->next_round, 0)],
+                           reinterpret_cast<const uint32_t*>(R->A.counts) + CTL_OFF);
+        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, &R->A.counts[CNT_SUM], sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+        if (ctx->h_pinned[0] < 0) {
+            ++ctx->drain_failures;
+            ReconBatch B = R->batch;
+            B.drain = -1;
+            const int mode = R->mode, dtype = R->dtype, conn = R->conn;
+            NSX_TRY(recon_launch(ctx, mode, dtype, conn, B, s, run, nullptr));
+            R = run->impl;
+            if (!R) return NSX_OK;
+        }
+    }
     const ReconArgs& A = R->A;
     int batch = 2;
     while (true) {
         hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, &A.counts[CNT_SUM],
-                           &A.counts[cnt_idx(R->next_round, 0)]);
+                           &A.counts[cnt_idx(R->next_round, 0)], (const uint32_t*)nullptr);
         NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, &A.counts[CNT_SUM], sizeof(int32_t), hipMemcpyDeviceToHost, s));
         NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
         if (ctx->h_pinned[0] == 0) break;
@@ -1627,7 +1887,9 @@ void recon_status_gather(const ReconRun* rr, const ReconRun* rf, const int32_t* 
     const ReconRunImpl* a = rr->impl;
     const ReconRunImpl* b = rf->impl;
     hipLaunchKernelGGL(chain_status_kernel, dim3(1), dim3(64), 0, s, out, ncomp,
-                       &a->A.counts[cnt_idx(a->next_round, 0)], &b->A.counts[cnt_idx(b->next_round, 0)], rstats, fstats);
+                       &a->A.counts[cnt_idx(a->next_round, 0)], &b->A.counts[cnt_idx(b->next_round, 0)], rstats, fstats,
+                       a->drain ? reinterpret_cast<const uint32_t*>(a->A.counts) + CTL_OFF : nullptr,
+                       b->drain ? reinterpret_cast<const uint32_t*>(b->A.counts) + CTL_OFF : nullptr);
 }
 
 void recon_note_rounds(nsx_ctx* ctx, ReconRun* run, int64_t rounds_with_work) {
@@ -1695,9 +1957,10 @@ nsx_status recon_run(nsx_ctx* ctx, int mode, const View* marker, const View& mas
     B.ws_base = WS_RECON_STAMP;
     B.stats = stats;
     ReconRun run;
+    const int cap = tree_ok && !tree_first && ctx->engine == NSX_ENGINE_AUTO ? AUTO_TREE_ROUNDS : 0;
+    if (cap) B.drain_cap = cap + 1;
     NSX_TRY(recon_launch(ctx, mode, mask.dtype, conn, B, s, &run, nullptr));
     bool gave_up = false;
-    const int cap = tree_ok && !tree_first && ctx->engine == NSX_ENGINE_AUTO ? AUTO_TREE_ROUNDS : 0;
     nsx_status st = recon_complete(ctx, &run, s, nullptr, cap, &gave_up);
     run.release();
     if (st != NSX_OK || !gave_up) return st;
@@ -1767,9 +2030,10 @@ nsx_status recon_run_noalloc(nsx_ctx* ctx, const View& marker, const View& mask,
     if ((int64_t)D.ntx * D.nty > NOALLOC_MAX_TILES)
         return ctx->fail(NSX_ERR_INVALID, "image too large for nsx_imreconstruct_noalloc: at most 16384 tiles of "
                                           "62 x 62 pixels (e.g. 7936 x 7936)");
-    RoundFn first, later;
-    if (!select_fns(RM_RECON, mask.dtype, conn, &first, &later))
+    RoundFns fns;
+    if (!select_fns(RM_RECON, mask.dtype, conn, &fns))
         return ctx->fail(NSX_ERR_UNSUPPORTED, "reconstruction dtype not supported");
+    const RoundFn first = fns.first, later = fns.later;
     ReconArgs A{};
     A.img0 = D;
     A.nimg = 1;
@@ -1797,7 +2061,8 @@ nsx_status recon_run_noalloc(nsx_ctx* ctx, const View& marker, const View& mask,
     NSX_HIP_TRY(ctx, hipGetLastError());
     int batch = 2;
     while (true) {
-        hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, &A.counts[CNT_SUM], &A.counts[cnt_idx(next, 0)]);
+        hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, &A.counts[CNT_SUM], &A.counts[cnt_idx(next, 0)],
+                           (const uint32_t*)nullptr);
         NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, &A.counts[CNT_SUM], sizeof(int32_t), hipMemcpyDeviceToHost, s));
         NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
         if (ctx->h_pinned[0] == 0) break;

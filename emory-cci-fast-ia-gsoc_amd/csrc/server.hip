@@ -317,6 +317,9 @@ void run_shared(nsx_ctx* ctx, std::vector<Job*>& g, char* pin, bool* short_round
     bf.ws_base = WS_FILL_STAMP;
     br.stats = ctx->d_stats;
     bf.stats = ctx->d_stats + 8;
+    // slots run side by side on their own streams: a persistent drain sized
+    // for the whole GPU would wait for blocks held by another slot's kernels
+    br.drain = bf.drain = -1;
     CclBatch cb{};
     cb.nimg = n;
     cb.ntiles = cbase;

@@ -135,7 +135,8 @@ class Context:
     def workspace_bytes(self) -> int:
         return int(self.lib.nsx_ctx_workspace_bytes(self.h))
 
-    TIMED_KINDS = ("recon_first", "recon_round", "ccl_local", "ccl_merge", "ccl_final", "ccl_small", "recon_scan")
+    TIMED_KINDS = ("recon_first", "recon_round", "ccl_local", "ccl_merge", "ccl_final", "ccl_small", "recon_scan",
+                   "recon_drain")
 
     def set_timing(self, enable: bool) -> None:
         """Start (and clear) or stop HIP-event timing of the library's kernels."""
@@ -170,6 +171,12 @@ class Context:
     def get_tuning(self, param: int) -> int:
         v = C.c_int(0)
         self.check(self.lib.nsx_ctx_get_tuning(self.h, param, C.byref(v)))
+        return int(v.value)
+
+    def drain_failures(self) -> int:
+        """Persistent drains (TUNE_DRAIN) that gave up and restarted with per-round launches."""
+        v = C.c_int64(0)
+        self.check(self.lib.nsx_ctx_drain_failures(self.h, C.byref(v)))
         return int(v.value)
 
     def set_option(self, option: int, value: int) -> None:

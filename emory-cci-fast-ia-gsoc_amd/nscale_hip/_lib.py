@@ -20,6 +20,7 @@ NSX_TUNE_VISIT_SWEEPS = 0
 NSX_TUNE_LEVELSET_LEVELS = 1
 NSX_TUNE_ENGINE = 2
 NSX_TUNE_FILL_SCANS = 3
+NSX_TUNE_DRAIN = 4
 NSX_ENGINE_ROUNDS = 0
 NSX_ENGINE_TREE = 1
 NSX_ENGINE_AUTO = 2
@@ -59,6 +60,7 @@ SIGNATURES = {
     "nsx_ctx_read_tile_counts": (C.c_int, [_CTX, P(C.c_int64), P(C.c_int64), C.c_int]),
     "nsx_ctx_set_tuning": (C.c_int, [_CTX, C.c_int, C.c_int]),
     "nsx_ctx_get_tuning": (C.c_int, [_CTX, C.c_int, P(C.c_int)]),
+    "nsx_ctx_drain_failures": (C.c_int, [_CTX, P(C.c_int64)]),
     "nsx_image_check": (C.c_int, [P(NsxImg), P(C.c_char_p)]),
     "nsx_ctx_set_option": (C.c_int, [_CTX, C.c_int, C.c_int]),
     "nsx_ctx_get_option": (C.c_int, [_CTX, C.c_int, P(C.c_int)]),

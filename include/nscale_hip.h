@@ -76,7 +76,8 @@ nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int stage, int64_t stats[4]);
  * kernel kinds k = 0 reconstruction first pass, 1 reconstruction propagation
  * launch, 2 tile-local labelling, 3 tile-edge union, 4 label output, 5 area /
  * keep / scan kernels (per call), 6 a hole filling's image-wide row and column
- * scans (per call).  Synchronises the context.  No reference
+ * scans (per call), 7 a persistent drain of a reconstruction's rounds
+ * (NSX_TUNE_DRAIN; one launch).  Synchronises the context.  No reference
  * counterpart (the reference's CUDA event timing is commented out,
  * src/segment/cuda/ccl_uf.cu:28-31). */
 nsx_status nsx_ctx_set_timing(nsx_ctx* ctx, int enable);
@@ -127,11 +128,24 @@ nsx_status nsx_ctx_read_tile_counts(nsx_ctx* ctx, int64_t* first_pass, int64_t* 
  * runs, and the rounds saved do not repay the scans.  Counterpart of the
  * reference's 1-D passes (cuda/imreconstruct_int_kernel.cu:217-585), which
  * alternate until nothing changes. */
+/* NSX_TUNE_DRAIN: 1 runs every propagation round after a reconstruction's
+ * first pass in one persistent launch (blocks sized to stay resident on the
+ * whole GPU drain the device tile lists round by round, separated by
+ * grid-wide barriers), 0 (default) launches one kernel per round.  The drain
+ * needs no round prediction or host check; a barrier that waits 0.5 s without
+ * progress (a GPU shared with other work) makes the call start again with
+ * per-round launches.  The serving loop's slots always use per-round
+ * launches; nsx_batch_autotune picks per tile group.  Counterpart of the
+ * reference's persistent queue kernel (global_queue.cu:746-857). */
 enum { NSX_TUNE_VISIT_SWEEPS = 0, NSX_TUNE_LEVELSET_LEVELS = 1, NSX_TUNE_ENGINE = 2, NSX_TUNE_FILL_SCANS = 3,
-       NSX_TUNE_NPARAMS = 4 };
+       NSX_TUNE_DRAIN = 4, NSX_TUNE_NPARAMS = 5 };
 enum { NSX_ENGINE_ROUNDS = 0, NSX_ENGINE_TREE = 1, NSX_ENGINE_AUTO = 2, NSX_ENGINE_PLANES = 3 };
 nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value);
 nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value);
+/* Persistent drains (NSX_TUNE_DRAIN) on this context that gave up and were
+ * restarted with per-round launches, since the context was created.  No
+ * reference counterpart. */
+nsx_status nsx_ctx_drain_failures(const nsx_ctx* ctx, int64_t* n);
 
 /* Semantic options of a context (these change results).
  * NSX_OPT_I32_INVERT: how PixelOperations::invert<int> (PixelOperations.cpp:
