@@ -51,6 +51,8 @@ def parse():
                         "server; "
                         "ops: seeded imfill, bwselect, imhmin, localMaxima/Minima, boundingBox and "
                         "distanceTransform, each on the 4096^2 disk image")
+    p.add_argument("--drain", type=int, default=0, choices=[0, 1],
+                   help="chain: run each reconstruction's propagation rounds in one persistent launch")
     p.add_argument("--tiles-u8", type=int, default=256)
     p.add_argument("--tiles-f32", type=int, default=32)
     p.add_argument("--tune-reps", type=int, default=2,
@@ -128,11 +130,26 @@ def bench_batch(args):
     t0 = time.perf_counter()
     b.finalize(use_graph=True)
     t_fin = time.perf_counter() - t0
-    # per tile group, the sweep pairs per visit with the fastest replays
+
+    def replay_ms(reps):
+        b.run()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            b.run()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps * 1e3
+    # the fixed launch shapes before tuning: every group drained in one
+    # persistent launch, then one launch per round (the default)
+    b.set_drain(True)
+    shape_ms = {"drain": replay_ms(max(3, args.steps // 4))}
+    b.set_drain(False)
+    shape_ms["per_round"] = replay_ms(max(3, args.steps // 4))
+    # per tile group, a coordinate descent over the launch shape
     t0 = time.perf_counter()
-    pairs = b.autotune(reps=args.tune_reps) if args.tune_reps > 0 else []
+    tuned = b.autotune(reps=args.tune_reps) if args.tune_reps > 0 else b.tuning()
     t_tune = time.perf_counter() - t0
-    log(f"autotune: sweep pairs per group {pairs} in {t_tune:.2f} s")
+    log(f"fixed shapes (ms per replay): {shape_ms}; autotune: {tuned} in {t_tune:.2f} s")
     for _ in range(args.warmup):
         b.run()
     counts = b.counts()
@@ -146,7 +163,7 @@ def bench_batch(args):
     t_batch = (time.perf_counter() - t0) / args.steps
     counts = b.counts()
     same = all(torch.equal(r, r2) and torch.equal(lab, l2) for _, _, r, lab, r2, l2, _ in dtiles) and counts == n_tile
-    roof = batch_roofline(nh, dtiles, args)
+    roof = batch_roofline(nh, dtiles, args, drain=bool(tuned and tuned[0]["drain"]))
     cpu, exact = (None, None) if args.no_cpu else cpu_baseline_batch(tiles, dtiles, counts, args)
     if exact is False:
         sys.exit(1)
@@ -160,7 +177,8 @@ def bench_batch(args):
         "data": "synthetic", "bitexact": exact, "batched_equals_per_tile": bool(same),
         "per_tile_mpix_s": round(total_px / t_tile / 1e6, 2), "per_tile_ms": round(t_tile * 1e3, 3),
         "speedup_vs_per_tile": round(t_tile / t_batch, 3), "finalize_s": round(t_fin, 3),
-        "autotune": {"visit_pairs_per_group": pairs, "s": round(t_tune, 3)},
+        "autotune": {"groups": tuned, "s": round(t_tune, 3),
+                     "fixed_shape_ms": {k: round(v, 3) for k, v in shape_ms.items()}},
         "config": {"workload": f"{args.tiles_u8} u8 + {args.tiles_f32} f32 tiles, sides 512-4096 step 16: "
                                "recon8 -> >128 (f32: >128/255) -> bwlabel8", "pixels": total_px,
                    "tiles": len(tiles), "step_alg_bytes": batch_alg_bytes(tiles),
@@ -179,12 +197,13 @@ def batch_alg_bytes(tiles) -> int:
     return int(sum(m.size * (8 if m.dtype.itemsize == 1 else 16) for _, m, _ in tiles))
 
 
-def batch_roofline(nh, dtiles, args) -> dict | None:
+def batch_roofline(nh, dtiles, args, drain: bool = True) -> dict | None:
     """Roofline of the batch's dominant kernel kind.  Graph replays record no
     per-kernel events, so the same tiles run as an eager batch (finalized
-    without a graph: the same launches) with HIP events on its private
-    context; its statistics give the tile visits of each group's propagation
-    rounds.  Algorithmic bytes per launch: reconstruction rounds 3 B (u8) or
+    without a graph: the same launches, drained or per round as the tuned
+    graph's first group) with HIP events on its private context; its
+    statistics give the tile visits of each group's propagation rounds (a
+    drain launch covers all of them).  Algorithmic bytes per launch: reconstruction rounds 3 B (u8) or
     12 B (f32) per visited tile pixel (62 x 62), the first pass that per pixel
     of every tile, labelling local 1 B (u8) or 4 B (f32) per pixel, output 5 B
     or 8 B per pixel."""
@@ -198,6 +217,7 @@ def batch_roofline(nh, dtiles, args) -> dict | None:
         else:
             f32px += dk.numel()
     be.finalize(use_graph=False)
+    be.set_drain(drain)
     bctx = be.context()
     steps = max(2, args.steps // 2)
     be.run()
@@ -213,6 +233,7 @@ def batch_roofline(nh, dtiles, args) -> dict | None:
     alg_step = {
         "recon_first": 3 * u8px + 12 * f32px,
         "recon_round": (3 * vis[0] + 12 * vis[1]) * TILE_PX,
+        "recon_drain": (3 * vis[0] + 12 * vis[1]) * TILE_PX,
         "ccl_local": u8px + 4 * f32px,
         "ccl_final": 5 * u8px + 8 * f32px,
         "ccl_merge": 0,
@@ -522,7 +543,8 @@ def serve_roofline(srv, burst, reqs, slots) -> dict | None:
     round_b = sum(r * TILE_PX * per_px[m] * es[d] for (m, d), (_, r) in tiles.items())
     in_b = sum(mask.size * mask.itemsize for _, mask, _, _ in reqs)
     alg = {"recon_first": first_b, "recon_round": round_b, "ccl_local": in_b, "ccl_final": in_b + 4 * sum(
-        mask.size for _, mask, _, _ in reqs), "ccl_merge": 0, "ccl_small": 0, "recon_scan": 0}
+        mask.size for _, mask, _, _ in reqs), "ccl_merge": 0, "ccl_small": 0, "recon_scan": 0,
+           "recon_drain": round_b}
     kind = max(kt, key=lambda k: kt[k][0])
     ms, n = kt[kind]
     if not n or not alg[kind]:
@@ -979,6 +1001,7 @@ def roofline(kt: dict, px: int, tiles: dict, steps: int, traffic: dict | None = 
     alg_total = {
         "recon_first": first_b,
         "recon_round": round_b,
+        "recon_drain": round_b,
         "ccl_local": 1 * px * steps,
         "ccl_final": 5 * px * steps,
         "ccl_merge": 0,
@@ -1059,6 +1082,7 @@ def main():
     d_fill = torch.empty_like(d_mask)
     d_lab = torch.empty((n, n), dtype=torch.int32, device=dev)
     ctx = nh.default_context(0)
+    ctx.set_tuning(nh._lib.NSX_TUNE_DRAIN, int(args.drain))
     stream = torch.cuda.current_stream(dev)
 
     def step():

@@ -50,6 +50,9 @@ struct Group {
     int32_t* d_pending = nullptr;    // tiles still queued after the captured rounds
     int ws_recon = 0, ws_ccl = 0;    // workspace slot bases in the batch context
     int visit_sweeps = 0;            // sweep pairs per visit (0: the context's), see autotune
+    int drain = 0;                   // rounds in one persistent launch: 1 on, -1 off, 0 the batch's default
+    int drain_occ = 0;               // drain blocks per CU (0: as many as stay resident)
+    int xcd_split = 0;               // 1: each XCD takes a contiguous share of a round's tiles; 0: plain stride
 };
 
 }  // namespace
@@ -67,6 +70,10 @@ struct nsx_batch {
     hipGraphExec_t exec = nullptr;
     hipStream_t cap_stream = nullptr;
     int32_t* h_pending = nullptr;    // pinned, one per group
+    // Groups run one launch per propagation round unless set or tuned to the
+    // persistent drain: on MI355X a kernel boundary (~2.5 us) is a cheaper
+    // grid-wide barrier than the drain's (~100-290 us with every block resident).
+    int drain_default = -1;
 };
 
 namespace {
@@ -87,6 +94,9 @@ ReconBatch recon_batch(nsx_batch* b, size_t gi) {
     rb.ws_base = g.ws_recon;
     rb.stats = b->ws->d_stats + 8 * (gi % 2);
     rb.visit_sweeps = g.visit_sweeps;
+    rb.drain = g.drain != 0 ? g.drain : b->drain_default;
+    rb.drain_occ = g.drain_occ;
+    rb.xcd_split = g.xcd_split;
     return rb;
 }
 
@@ -379,26 +389,45 @@ nsx_status nsx_batch_autotune(nsx_batch* b, void* stream, int reps, int32_t* pai
         *ms /= (float)reps;
         return NSX_OK;
     };
-    // coordinate descent over the groups (each is its own set of launches)
-    static const int kCand[] = {3, 4, 5, 6};
+    // Coordinate descent, group by group (each group is its own set of
+    // launches), one parameter at a time with the others held: the drain (one
+    // persistent launch, or one launch per round), the sweep pairs per tile
+    // visit, the work split over the XCDs, and the drain's blocks per CU.
+    // A second pass runs only if the first changed something.
+    static const int kSweeps[] = {3, 4, 5, 6};
+    static const int kOcc[] = {0, 8, 4};
     nsx_status st = NSX_OK;
-    for (size_t gi = 0; gi < b->groups.size() && st == NSX_OK; ++gi) {
-        Group& g = b->groups[gi];
-        int best = g.visit_sweeps > 0 ? g.visit_sweeps : b->ws->visit_sweeps;
-        float best_ms = 0.f;
-        g.visit_sweeps = best;
-        st = trial(&best_ms);
-        for (int cand : kCand) {
-            if (st != NSX_OK || cand == best) continue;
-            g.visit_sweeps = cand;
-            float ms = 0.f;
-            st = trial(&ms);
-            if (st == NSX_OK && ms < best_ms) {
-                best_ms = ms;
-                best = cand;
-            }
+    float cur_ms = 0.f;
+    for (auto& g : b->groups) {
+        if (g.visit_sweeps <= 0) g.visit_sweeps = b->ws->visit_sweeps;
+        if (g.drain == 0) g.drain = b->drain_default;
+    }
+    st = trial(&cur_ms);
+    for (int pass = 0; pass < 2 && st == NSX_OK; ++pass) {
+        bool changed = false;
+        for (size_t gi = 0; gi < b->groups.size() && st == NSX_OK; ++gi) {
+            Group& g = b->groups[gi];
+            // try one value of one parameter; keep it if faster
+            auto attempt = [&](int* field, int value) {
+                if (st != NSX_OK || *field == value) return;
+                const int old = *field;
+                *field = value;
+                float ms = 0.f;
+                st = trial(&ms);
+                if (st == NSX_OK && ms < cur_ms) {
+                    cur_ms = ms;
+                    changed = true;
+                } else {
+                    *field = old;
+                }
+            };
+            attempt(&g.drain, -g.drain);
+            for (int v : kSweeps) attempt(&g.visit_sweeps, v);
+            attempt(&g.xcd_split, 1 - g.xcd_split);
+            if (g.drain > 0)
+                for (int v : kOcc) attempt(&g.drain_occ, v);
         }
-        g.visit_sweeps = best;
+        if (!changed) break;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -409,6 +438,29 @@ nsx_status nsx_batch_autotune(nsx_batch* b, void* stream, int reps, int32_t* pai
     NSX_TRY(capture(b));
     for (int i = 0; i < npairs; ++i)
         pairs[i] = (size_t)i < b->groups.size() ? b->groups[i].visit_sweeps : 0;
+    return NSX_OK;
+}
+
+nsx_status nsx_batch_get_tuning(nsx_batch* b, int group, int32_t* out, int n) {
+    if (!b || !b->finalized || !out || n < 0 || group < 0) return NSX_ERR_INVALID;
+    if ((size_t)group >= b->groups.size()) return NSX_ERR_INVALID;
+    const Group& g = b->groups[group];
+    const int32_t v[4] = {g.visit_sweeps > 0 ? g.visit_sweeps : b->ws->visit_sweeps,
+                          (g.drain != 0 ? g.drain : b->drain_default) > 0 ? 1 : 0, g.xcd_split, g.drain_occ};
+    for (int i = 0; i < n && i < 4; ++i) out[i] = v[i];
+    return NSX_OK;
+}
+
+nsx_status nsx_batch_set_drain(nsx_batch* b, int drain) {
+    if (!b || (drain != 0 && drain != 1)) return NSX_ERR_INVALID;
+    b->drain_default = drain ? 1 : -1;
+    for (auto& g : b->groups) g.drain = 0;
+    if (b->finalized) {
+        // per-round launches replay the learned round counts
+        NSX_TRY(learn_rounds(b, b->ws->stream));
+        NSX_HIP_TRY(b->owner, hipStreamSynchronize(b->ws->stream));
+    }
+    if (b->exec) NSX_TRY(capture(b));
     return NSX_OK;
 }
 

@@ -202,6 +202,8 @@ struct ReconBatch {
     int visit_sweeps = 0;           // sweep pairs per visit; 0: the context's setting
     int drain = 0;                  // rounds in one persistent launch: 1 on, -1 off, 0 the context's setting
     int drain_cap = 0;              // the drain stops before this round (0: no cap); the host goes on from there
+    int drain_occ = 0;              // drain blocks per CU (0: as many as stay resident)
+    int xcd_split = -1;             // XCD-contiguous work split: 1 on, 0 off, -1 on for one image only
 };
 struct ReconRunImpl;
 // A launched reconstruction whose convergence has not been checked yet.

@@ -403,6 +403,7 @@ struct ReconArgs {
     unsigned long long* cum;   // tile visits summed over runs (may be null)
     int visit_sweeps;          // sweep pairs per visit before re-queueing (1..MAX_IT)
     int first_list;            // round 0 takes its tiles from the list (fill: the border tiles)
+    int xcd_split;             // each XCD takes a contiguous share of the work items (else a plain stride)
 };
 
 // The tile id is wave-uniform; readfirstlane makes that visible to the
@@ -704,12 +705,12 @@ void recon_round_kernel(ReconArgs A, int round) {
             }
         }
     }
-    // Blocks go to the XCDs in turn (block b to XCD b mod 8).  For one image
+    // Blocks go to the XCDs in turn (block b to XCD b mod 8).  With xcd_split
     // XCD x takes the x-th eighth of the work items, so each XCD works on one
-    // region of the image and neighbouring tiles meet in its L2.  A batch
-    // keeps the plain stride: its tiles differ in cost, and contiguous chunks
-    // would pile the costly ones onto one XCD.
-    const int nx = A.nimg > 1 ? 1 : (gridDim.x < 8 ? (int)gridDim.x : 8);
+    // region of the image and neighbouring tiles meet in its L2 (the default
+    // for one image).  A batch defaults to the plain stride: its tiles differ
+    // in cost, and contiguous chunks can pile the costly ones onto one XCD.
+    const int nx = !A.xcd_split ? 1 : (gridDim.x < 8 ? (int)gridDim.x : 8);
     const int xcd = (int)blockIdx.x % nx, xb = (int)blockIdx.x / nx;
     const int xstride = ((int)gridDim.x - xcd + nx - 1) / nx;
     if (FIRST && !A.first_list) {
@@ -829,7 +830,7 @@ __device__ __forceinline__ void drain_header(const ReconArgs& A, int round, int 
 template <typename Visit>
 __device__ __forceinline__ void drain_round_items(const ReconArgs& A, int round, const uint32_t* excl, int n,
                                                   Visit visit) {
-    const int nx = A.nimg > 1 ? 1 : (gridDim.x < 8 ? (int)gridDim.x : 8);
+    const int nx = !A.xcd_split ? 1 : (gridDim.x < 8 ? (int)gridDim.x : 8);
     const int xcd = (int)blockIdx.x % nx, xb = (int)blockIdx.x / nx;
     const int xstride = ((int)gridDim.x - xcd + nx - 1) / nx;
     const int32_t* list = (round & 1) ? A.list1 : A.list0;
@@ -1487,12 +1488,12 @@ void recon_quad_kernel(ReconArgs A, int round) {
             }
         }
     }
-    // Blocks go to the XCDs in turn (block b to XCD b mod 8).  For one image
+    // Blocks go to the XCDs in turn (block b to XCD b mod 8).  With xcd_split
     // XCD x takes the x-th eighth of the work items, so each XCD works on one
-    // region of the image and neighbouring tiles meet in its L2.  A batch
-    // keeps the plain stride: its tiles differ in cost, and contiguous chunks
-    // would pile the costly ones onto one XCD.
-    const int nx = A.nimg > 1 ? 1 : (gridDim.x < 8 ? (int)gridDim.x : 8);
+    // region of the image and neighbouring tiles meet in its L2 (the default
+    // for one image).  A batch defaults to the plain stride: its tiles differ
+    // in cost, and contiguous chunks can pile the costly ones onto one XCD.
+    const int nx = !A.xcd_split ? 1 : (gridDim.x < 8 ? (int)gridDim.x : 8);
     const int xcd = (int)blockIdx.x % nx, xb = (int)blockIdx.x / nx;
     const int xstride = ((int)gridDim.x - xcd + nx - 1) / nx;
     if (FIRST && !A.first_list) {
@@ -1543,7 +1544,7 @@ void recon_quad_drain_kernel(ReconArgs A, Drain P) {
 
 typedef void (*RoundFn)(const ReconArgs&, int, int, hipStream_t);
 // Launches a persistent drain; returns the grid it used (0: not launched).
-typedef int (*DrainFn)(const ReconArgs&, const Drain&, int cu_count, hipStream_t);
+typedef int (*DrainFn)(const ReconArgs&, const Drain&, int cu_count, int occ, hipStream_t);
 
 // Blocks of `kernel` one CU holds at once (at most `design`), from the
 // compiled kernel's registers and LDS.
@@ -1557,20 +1558,21 @@ int resident_blocks(K kernel, int block, int design) {
     return std::max(0, std::min(nb, design));
 }
 
+// occ > 0 asks for fewer blocks per CU than stay resident.
 template <typename T, int MODE, int CONN>
-int launch_drain(const ReconArgs& A, const Drain& P, int cu_count, hipStream_t s) {
+int launch_drain(const ReconArgs& A, const Drain& P, int cu_count, int occ, hipStream_t s) {
     static const int per_cu = resident_blocks(recon_drain_kernel<T, MODE, CONN>, 64, sizeof(T) == 1 ? 12 : 8);
     if (per_cu <= 0) return 0;
-    const int grid = per_cu * cu_count;
+    const int grid = (occ > 0 ? std::min(occ, per_cu) : per_cu) * cu_count;
     hipLaunchKernelGGL((recon_drain_kernel<T, MODE, CONN>), dim3(grid), dim3(64), 0, s, A, P);
     return grid;
 }
 
 template <typename T, int MODE>
-int launch_quad_drain(const ReconArgs& A, const Drain& P, int cu_count, hipStream_t s) {
+int launch_quad_drain(const ReconArgs& A, const Drain& P, int cu_count, int occ, hipStream_t s) {
     static const int per_cu = resident_blocks(recon_quad_drain_kernel<T, MODE>, QW * 64, sizeof(T) == 1 ? 4 : 3);
     if (per_cu <= 0) return 0;
-    const int grid = per_cu * cu_count;
+    const int grid = (occ > 0 ? std::min(occ, per_cu) : per_cu) * cu_count;
     hipLaunchKernelGGL((recon_quad_drain_kernel<T, MODE>), dim3(grid), dim3(QW * 64), 0, s, A, P);
     return grid;
 }
@@ -1714,6 +1716,7 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     A = ReconArgs{};
     A.nimg = B.nimg;
     A.ntiles = B.ntiles;
+    A.xcd_split = B.xcd_split >= 0 ? (B.xcd_split ? 1 : 0) : (B.nimg == 1 ? 1 : 0);
     if (B.nimg == 1) A.img0 = *reinterpret_cast<const ImgDesc*>(B.host_desc);
     A.imgs = reinterpret_cast<const ImgDesc*>(B.dev_desc);
     A.tile_img = B.tile_img;
@@ -1795,7 +1798,7 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
         P.max_round = std::min<int64_t>(P.max_round, (int64_t)INT32_MAX / (12 * (int64_t)ctx->cu_count) - 1);
         P.stall = DRAIN_STALL_TICKS;
         const int h = ctx->tic(TK_RECON_DRAIN, s);
-        const int g = fns.drain(A, P, ctx->cu_count, s);
+        const int g = fns.drain(A, P, ctx->cu_count, B.drain_occ, s);
         ctx->toc(h, s);
         if (g > 0) {
             R->drain = true;
@@ -2045,6 +2048,7 @@ nsx_status recon_run_noalloc(nsx_ctx* ctx, const View& marker, const View& mask,
     A.stats = (int64_t*)(base + NA_STATS);
     A.visit_sweeps = std::max(1, std::min(MAX_IT, ctx->visit_sweeps));
     A.first_list = 0;
+    A.xcd_split = 1;
     const int grid = std::max(1, std::min(ntiles, (mask.dtype == NSX_U8 ? 12 : 8) * ctx->cu_count));
     // the stamps, counters and statistics start cleared on every call (a kernel,
     // not a memset, so the sequence can also be captured)

@@ -462,13 +462,30 @@ class Batch:
         self.ctx.check(self.lib.nsx_batch_run(self.h, s))
 
     def autotune(self, reps: int = 3, stream=None) -> list:
-        """Times graph replays with 3..6 sweep pairs per tile visit for each tile
-        group and keeps the fastest (nsx_batch_autotune); returns the pairs chosen
-        per group.  Results do not depend on it."""
+        """Coordinate descent over each tile group's launch shape (drain or
+        per-round launches, sweep pairs per visit, XCD split, drain blocks per
+        CU) timed by graph replays (nsx_batch_autotune); returns tuning() of the
+        kept settings.  Results do not depend on it."""
         s = _stream_arg(stream) if stream is not None else _torch_handle(_torch().cuda.current_stream())
         pairs = (C.c_int32 * 4)()
         self.ctx.check(self.lib.nsx_batch_autotune(self.h, s, int(reps), pairs, 4))
-        return [int(v) for v in pairs if v > 0]
+        return self.tuning()
+
+    def tuning(self) -> list:
+        """Per tile group: {"visit_pairs", "drain", "xcd_split", "drain_blocks_per_cu"}."""
+        out = []
+        g = 0
+        while True:
+            v = (C.c_int32 * 4)()
+            if self.lib.nsx_batch_get_tuning(self.h, g, v, 4) != _lib.NSX_OK:
+                return out
+            out.append({"visit_pairs": int(v[0]), "drain": int(v[1]), "xcd_split": int(v[2]),
+                        "drain_blocks_per_cu": int(v[3])})
+            g += 1
+
+    def set_drain(self, on: bool) -> None:
+        """Every group to the persistent drain (default) or to per-round launches."""
+        self.ctx.check(self.lib.nsx_batch_set_drain(self.h, int(bool(on))))
 
     def context(self) -> "Context":
         """The batch's private context (timing of eager runs, statistics); a

@@ -335,28 +335,46 @@ nsx_status nsx_batch_add(nsx_batch* batch, const nsx_img* marker, const nsx_img*
 /* Builds the work tables and (use_graph=1) captures the graph.  Call once after
  * the last add. */
 nsx_status nsx_batch_finalize(nsx_batch* batch, int use_graph);
-/* Runs the whole batch on the stream (graph replay when captured).  The
- * reconstruction runs the number of propagation rounds each tile group needed
- * when it was last completed; recon_out and labels_out are final only after
- * nsx_batch_counts, which checks every group's convergence and, where tile
- * contents changed since and need more rounds, finishes that group eagerly,
- * relabels it and re-captures the graph.  A caller that rewrites tile
- * contents between runs must therefore call nsx_batch_counts before reading
- * the outputs (with unchanged contents every run is already final). */
+/* Runs the whole batch on the stream (graph replay when captured).  A tile
+ * group set to the persistent drain (nsx_batch_set_drain(b, 1) or the tuner's
+ * choice; NSX_TUNE_DRAIN) runs its propagation rounds in one launch that
+ * drains the group's device tile lists to convergence, so the graph's launches
+ * do not depend on the content and every run is final.  A group on per-round
+ * launches (the default: faster on MI355X) runs the number of rounds it needed
+ * when last completed; its outputs are final only after nsx_batch_counts,
+ * which checks every group's convergence (and a drain's failure) and, where
+ * tile contents changed since and need more rounds, finishes that group
+ * eagerly, relabels it and re-captures the graph.  A caller that rewrites
+ * tile contents between runs of such a group must therefore call
+ * nsx_batch_counts before reading the outputs. */
 nsx_status nsx_batch_run(nsx_batch* batch, void* stream);
 /* Completes the last run (see nsx_batch_run) and returns the per-tile
  * component counts (array of n tiles).  Synchronises. */
 nsx_status nsx_batch_counts(nsx_batch* batch, int32_t* counts, int n);
 /* Tunes a batch finalized with a graph: for each group of tiles with the same
- * (dtype, conn, ccl_conn, thresh), in the order groups first appear, tries
- * 3..6 sweep pairs per tile visit (NSX_TUNE_VISIT_SWEEPS for that group only),
- * each timed as the mean of `reps` graph replays on `stream` after re-learning
- * the round counts and re-capturing; keeps the fastest and leaves the graph
- * captured with it.  pairs[g] (g < npairs) receives the choice, 0 past the last
- * group.  Results never depend on it; it runs the batch about 4 x groups x
- * (reps + 2) times.  No counterpart in the reference (its batched path has
- * fixed launch shapes). */
+ * (dtype, conn, ccl_conn, thresh), in the order groups first appear, a
+ * coordinate descent over the group's launch shape: persistent drain or one
+ * launch per round, 3..6 sweep pairs per tile visit (NSX_TUNE_VISIT_SWEEPS for
+ * that group only), the split of a round's tiles over the XCDs (plain stride
+ * or a contiguous share per XCD) and the drain's blocks per CU (all that stay
+ * resident, 8 or 4).  Each setting is timed as the mean of `reps` graph
+ * replays on `stream` after re-learning the round counts and re-capturing;
+ * the fastest is kept and left captured (a second pass runs if the first
+ * changed anything).  pairs[g] (g < npairs) receives the sweep pairs, 0 past
+ * the last group; nsx_batch_get_tuning reads every choice.  Results never
+ * depend on it; it runs the batch about 10 x groups x (reps + 2) times.  No
+ * counterpart in the reference (its batched path has fixed launch shapes). */
 nsx_status nsx_batch_autotune(nsx_batch* batch, void* stream, int reps, int32_t* pairs, int npairs);
+/* Launch shape of tile group `group` (groups in order of first appearance):
+ * out[0] sweep pairs per visit, out[1] 1 persistent drain / 0 per-round
+ * launches, out[2] 1 XCD-contiguous split / 0 plain stride, out[3] drain
+ * blocks per CU (0: all that stay resident); n <= 4 entries.  NSX_ERR_INVALID
+ * past the last group.  No reference counterpart. */
+nsx_status nsx_batch_get_tuning(nsx_batch* batch, int group, int32_t* out, int n);
+/* Sets every group to the persistent drain (drain = 1) or to per-round
+ * launches (0, the default), dropping the tuner's choice of it; re-learns
+ * round counts and re-captures a finalized batch.  No reference counterpart. */
+nsx_status nsx_batch_set_drain(nsx_batch* batch, int drain);
 /* The batch's private context (owned by the batch; valid until
  * nsx_batch_destroy): its kernel timing (nsx_ctx_set_timing /
  * nsx_ctx_read_timing, recorded by eager runs of a batch finalized without a

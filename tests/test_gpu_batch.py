@@ -32,12 +32,14 @@ def expected(orc, marker, mask, thr):
     return r, lab, n
 
 
+@pytest.mark.parametrize("drain", [True, False])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_batch_matches_oracle_every_replay(nh, orc, gpu, use_graph):
+def test_batch_matches_oracle_every_replay(nh, orc, gpu, use_graph, drain):
     import torch
     tiles = make_tiles(nh, 9, 1)
     want = [expected(orc, m, k, t) for m, k, t in tiles]
     batch = nh.Batch()
+    batch.set_drain(drain)
     outs = []
     for marker, mask, thr in tiles:
         dm = torch.from_numpy(marker).cuda()
@@ -98,8 +100,11 @@ def test_batch_autotune_keeps_results(nh, orc, gpu):
         batch.add(dm, dk, r, lab, 8, thr, 8)
         outs.append((r, lab))
     batch.finalize(use_graph=True)
-    pairs = batch.autotune(reps=1)
-    assert 1 <= len(pairs) <= 3 and all(3 <= p <= 6 for p in pairs)
+    tuned = batch.autotune(reps=1)
+    assert 1 <= len(tuned) <= 3 and tuned == batch.tuning()
+    for g in tuned:
+        assert 3 <= g["visit_pairs"] <= 6 and g["drain"] in (0, 1) and g["xcd_split"] in (0, 1)
+        assert g["drain_blocks_per_cu"] in (0, 8, 4)
     for replay in range(2):
         for r, lab in outs:
             r.fill_(7)
@@ -123,15 +128,18 @@ def test_batch_autotune_keeps_results(nh, orc, gpu):
     eager.close()
 
 
+@pytest.mark.parametrize("drain", [True, False])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_batch_contents_changed_after_finalize(nh, orc, gpu, use_graph):
+def test_batch_contents_changed_after_finalize(nh, orc, gpu, use_graph, drain):
     """Tile contents rewritten in place after finalize so that a replay needs
-    more propagation rounds than were captured (a maze where disks were):
-    nsx_batch_counts finishes the short group, and recon, labels and counts
-    match the oracle, on that replay and on the next (re-captured) one."""
+    more propagation rounds than were captured (a maze where disks were).
+    Drained: the run itself is final (checked before nsx_batch_counts).  Per
+    round: nsx_batch_counts finishes the short group.  Either way recon, labels
+    and counts match the oracle, on that replay and on the next one."""
     import torch
     tiles = make_tiles(nh, 4, 4, dtypes=("u8",))
     batch = nh.Batch()
+    batch.set_drain(drain)
     ins, outs = [], []
     for marker, mask, thr in tiles:
         dm = torch.from_numpy(marker).cuda()
@@ -159,6 +167,11 @@ def test_batch_contents_changed_after_finalize(nh, orc, gpu, use_graph):
             lab.fill_(-5)
         torch.cuda.synchronize()
         batch.run()
+        if drain:
+            torch.cuda.synchronize()
+            for i, ((r, lab), (wr, wlab, wn)) in enumerate(zip(outs, new)):
+                assert np.array_equal(r.cpu().numpy(), wr), f"replay {replay} tile {i} recon before counts"
+                assert np.array_equal(lab.cpu().numpy(), wlab), f"replay {replay} tile {i} labels before counts"
         counts = batch.counts()
         for i, ((r, lab), (wr, wlab, wn)) in enumerate(zip(outs, new)):
             assert np.array_equal(r.cpu().numpy(), wr), f"replay {replay} tile {i} recon"

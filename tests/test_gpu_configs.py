@@ -35,9 +35,10 @@ def _chain_dev(nh, ctx, dm, dk):
     return r.cpu().numpy(), f.cpu().numpy(), lab.cpu().numpy(), n
 
 
-@pytest.mark.parametrize("engine", ["rounds", "tree"])
+@pytest.mark.parametrize("engine", ["rounds", "tree", "drain"])
 def test_chain_4096_full(nh, orc, gpu, engine):
-    """The default engine (propagation rounds) and the component tree."""
+    """The default engine (propagation rounds), the component tree, and the
+    rounds in one persistent drain launch per propagation."""
     import torch
     from nscale_hip.workloads import chain_images
     marker, mask = chain_images()
@@ -47,6 +48,8 @@ def test_chain_4096_full(nh, orc, gpu, engine):
     ctx = nh.Context(0)
     if engine == "tree":
         ctx.set_tuning(nh._lib.NSX_TUNE_ENGINE, nh._lib.NSX_ENGINE_TREE)
+    if engine == "drain":
+        ctx.set_tuning(nh._lib.NSX_TUNE_DRAIN, 1)
     try:
         # fresh context: with rounds, the default prediction (4 rounds) is
         # short of the 10 reconstruction and 36 fill rounds, so the call
@@ -54,8 +57,10 @@ def test_chain_4096_full(nh, orc, gpu, engine):
         r, f, lab, n = _chain_dev(nh, ctx, dm, dk)
         assert np.array_equal(r, er) and np.array_equal(f, ef)
         assert np.array_equal(lab, elab) and n == en
-        if engine == "rounds":
+        if engine != "tree":
             assert ctx.recon_stats(1)["rounds"] > 4
+        if engine == "drain":
+            assert ctx.drain_failures() == 0
         for rep in range(2):
             r, f, lab, n = _chain_dev(nh, ctx, dm, dk)
             assert np.array_equal(r, er) and np.array_equal(f, ef), f"repetition {rep}"
@@ -64,7 +69,7 @@ def test_chain_4096_full(nh, orc, gpu, engine):
         ctx.close()
 
 
-@pytest.mark.parametrize("engine", ["rounds", "tree"])
+@pytest.mark.parametrize("engine", ["rounds", "tree", "drain"])
 def test_chain_4096_after_easy_input(nh, orc, gpu, engine):
     """An easy input of the same size (marker == mask: nothing propagates)
     sets a low round prediction for the size; the full chain that follows
@@ -78,6 +83,8 @@ def test_chain_4096_after_easy_input(nh, orc, gpu, engine):
     ctx = nh.Context(0)
     if engine == "tree":
         ctx.set_tuning(nh._lib.NSX_TUNE_ENGINE, nh._lib.NSX_ENGINE_TREE)
+    if engine == "drain":
+        ctx.set_tuning(nh._lib.NSX_TUNE_DRAIN, 1)
     try:
         de = torch.from_numpy(easy).cuda()
         r, f, lab, n = _chain_dev(nh, ctx, de, de)
