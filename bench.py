@@ -140,11 +140,15 @@ def bench_batch(args):
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / reps * 1e3
     # the fixed launch shapes before tuning: every group drained in one
-    # persistent launch, then one launch per round (the default)
+    # persistent launch; one launch per round with the groups in turn; and
+    # with the odd group on a second stream (the default)
     b.set_drain(True)
     shape_ms = {"drain": replay_ms(max(3, args.steps // 4))}
     b.set_drain(False)
-    shape_ms["per_round"] = replay_ms(max(3, args.steps // 4))
+    b.set_overlap(False)
+    shape_ms["per_round_serial"] = replay_ms(max(3, args.steps // 4))
+    b.set_overlap(True)
+    shape_ms["per_round_overlap"] = replay_ms(max(3, args.steps // 4))
     # per tile group, a coordinate descent over the launch shape
     t0 = time.perf_counter()
     tuned = b.autotune(reps=args.tune_reps) if args.tune_reps > 0 else b.tuning()

@@ -49,6 +49,7 @@ struct Group {
     int32_t* d_ncomp = nullptr;      // per item of the group
     int32_t* d_pending = nullptr;    // tiles still queued after the captured rounds
     int ws_recon = 0, ws_ccl = 0;    // workspace slot bases in the batch context
+    int lroot_slot = WS_CCL_LROOT;
     int visit_sweeps = 0;            // sweep pairs per visit (0: the context's), see autotune
     int drain = 0;                   // rounds in one persistent launch: 1 on, -1 off, 0 the batch's default
     int drain_occ = 0;               // drain blocks per CU (0: as many as stay resident)
@@ -70,6 +71,13 @@ struct nsx_batch {
     hipGraphExec_t exec = nullptr;
     hipStream_t cap_stream = nullptr;
     int32_t* h_pending = nullptr;    // pinned, one per group
+    // Odd groups run on a second stream, forked from the caller's stream and
+    // joined back (also inside the graph), so that one group's late rounds,
+    // which hold few tiles, overlap the other group's launches.  Even and odd
+    // groups use separate reconstruction and labelling workspace slots.
+    bool overlap = true;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // Groups run one launch per propagation round unless set or tuned to the
     // persistent drain: on MI355X a kernel boundary (~2.5 us) is a cheaper
     // grid-wide barrier than the drain's (~100-290 us with every block resident).
@@ -102,12 +110,26 @@ ReconBatch recon_batch(nsx_batch* b, size_t gi) {
 
 // Enqueues every group's reconstruction (first pass + predicted rounds) and
 // labelling, then gathers the per-item counts.  No host synchronisation.
-nsx_status enqueue(nsx_batch* b, hipStream_t s, std::vector<ReconRun>* runs) {
+nsx_status enqueue(nsx_batch* b, hipStream_t s0, std::vector<ReconRun>* runs) {
     nsx_ctx* c = b->ws;
     runs->clear();
     runs->resize(b->groups.size());
+    // A persistent drain sizes its grid to the whole GPU: it runs with nothing
+    // beside it.  The fork needs a stream object to wait on: the special
+    // handles (NULL, hipStreamLegacy, hipStreamPerThread) are not accepted by
+    // hipStreamWaitEvent (an eager run from torch's default stream crashed in
+    // it), so eager runs on them take the groups in turn; graph captures run
+    // on the batch's own stream and always fork.
+    bool fork = b->overlap && b->groups.size() > 1 && s0 != nullptr && s0 != hipStreamLegacy &&
+                s0 != hipStreamPerThread;
+    for (const auto& g : b->groups) fork = fork && (g.drain != 0 ? g.drain : b->drain_default) < 0;
+    if (fork) {
+        NSX_HIP_TRY(b->owner, hipEventRecord(b->ev_fork, s0));
+        NSX_HIP_TRY(b->owner, hipStreamWaitEvent(b->side, b->ev_fork, 0));
+    }
     for (size_t gi = 0; gi < b->groups.size(); ++gi) {
         Group& g = b->groups[gi];
+        hipStream_t s = fork && (gi % 2) ? b->side : s0;
         const ReconBatch rb = recon_batch(b, gi);
         NSX_TRY(recon_launch(c, RM_RECON, g.dtype, g.conn, rb, s, &(*runs)[gi], g.d_pending));
         CclBatch cb{};
@@ -119,6 +141,7 @@ nsx_status enqueue(nsx_batch* b, hipStream_t s, std::vector<ReconRun>* runs) {
         cb.tile_img = g.d_ctile;
         cb.in_dtype = g.dtype;
         cb.ws_base = g.ws_ccl;
+        cb.lroot_slot = g.lroot_slot;
         cb.ncomp_dev = g.d_ncomp;
         CclParams p;
         p.conn = g.ccl_conn;
@@ -126,6 +149,10 @@ nsx_status enqueue(nsx_batch* b, hipStream_t s, std::vector<ReconRun>* runs) {
         p.threshold = true;
         p.thr = g.thresh;
         NSX_TRY(ccl_launch(c, cb, p, s));
+    }
+    if (fork) {
+        NSX_HIP_TRY(b->owner, hipEventRecord(b->ev_join, b->side));
+        NSX_HIP_TRY(b->owner, hipStreamWaitEvent(s0, b->ev_join, 0));
     }
     return NSX_OK;
 }
@@ -273,12 +300,16 @@ nsx_status nsx_batch_finalize(nsx_batch* b, int use_graph) {
         NSX_HIP_TRY(b->owner, hipMemcpy(g.d_rtile, rt.data(), sizeof(int32_t) * rt.size(), hipMemcpyHostToDevice));
         NSX_HIP_TRY(b->owner, hipMemcpy(g.d_cdesc, g.cdesc.data(), g.cdesc.size(), hipMemcpyHostToDevice));
         NSX_HIP_TRY(b->owner, hipMemcpy(g.d_ctile, ct.data(), sizeof(int32_t) * ct.size(), hipMemcpyHostToDevice));
-        // each group gets its own workspace slots in the private context
+        // even and odd groups get their own workspace slots in the private context
         g.ws_recon = gi % 2 == 0 ? WS_RECON_STAMP : WS_FILL_STAMP;
-        g.ws_ccl = WS_CCL_LABEL;
+        g.ws_ccl = gi % 2 == 0 ? WS_CCL_LABEL : WS_CCL2_LABEL;
+        g.lroot_slot = gi % 2 == 0 ? WS_CCL_LROOT : WS_CCL2_LROOT;
     }
-    // groups 2 and 3 reuse the reconstruction slots of 0 and 1 (and every
-    // group the labelling slots): they run one after another on one stream
+    // groups 2 and 3 reuse the slots of 0 and 1: they run after them on the
+    // same stream
+    NSX_HIP_TRY(b->owner, hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+    NSX_HIP_TRY(b->owner, hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    NSX_HIP_TRY(b->owner, hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
     b->finalized = true;
     b->use_graph = use_graph != 0;
     // warm-up: learn the round counts of each group, then one full eager pass
@@ -287,26 +318,35 @@ nsx_status nsx_batch_finalize(nsx_batch* b, int use_graph) {
     hipStream_t s = c->stream;
     NSX_TRY(learn_rounds(b, s));
     {
-        int64_t max_pix = 0;
-        int max_ct = 0, max_rt_even = 0, max_rt_odd = 0;
+        // size every workspace slot for the largest group of its parity, so
+        // that nothing is reallocated while launches are captured or while
+        // the other parity's launches run
+        int64_t max_pix[2] = {0, 0};
+        int max_ct[2] = {0, 0}, max_rt[2] = {0, 0};
         for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-            max_pix = std::max(max_pix, b->groups[gi].pix_total);
-            max_ct = std::max(max_ct, b->groups[gi].ctiles);
-            (gi % 2 ? max_rt_odd : max_rt_even) = std::max(gi % 2 ? max_rt_odd : max_rt_even, b->groups[gi].rtiles);
+            const int q = (int)(gi % 2);
+            max_pix[q] = std::max(max_pix[q], b->groups[gi].pix_total);
+            max_ct[q] = std::max(max_ct[q], b->groups[gi].ctiles);
+            max_rt[q] = std::max(max_rt[q], b->groups[gi].rtiles);
         }
         void* p;
-        const int64_t nw = max_pix / 32, nb = max_pix / 32768;
-        nsx_status st = c->ws(WS_CCL_LABEL, sizeof(int32_t) * (size_t)max_pix, &p);
-        if (st == NSX_OK) st = c->ws(WS_CCL_AREA, sizeof(int32_t) * (size_t)max_pix, &p);
-        if (st == NSX_OK) st = c->ws(WS_CCL_ROOTS, sizeof(uint2) * (size_t)max_ct * 2048, &p);
-        if (st == NSX_OK) st = c->ws(WS_CCL_BITS, sizeof(int32_t) * (size_t)max_ct, &p);
-        if (st == NSX_OK) st = c->ws(WS_CCL_SCAN, sizeof(uint32_t) * (size_t)nw, &p);
-        if (st == NSX_OK) st = c->ws(WS_CCL_MISC, sizeof(uint32_t) * (size_t)(nw + nb + 1), &p);
-        if (st == NSX_OK) st = c->ws(WS_CCL_LROOT, sizeof(uint16_t) * (size_t)max_pix, &p);
-        if (st == NSX_OK) st = c->ws(WS_RECON_STAMP, sizeof(uint32_t) * (size_t)max_rt_even, &p);
-        if (st == NSX_OK) st = c->ws(WS_RECON_LIST, recon_list_bytes(max_rt_even), &p);
-        if (st == NSX_OK && max_rt_odd) st = c->ws(WS_FILL_STAMP, sizeof(uint32_t) * (size_t)max_rt_odd, &p);
-        if (st == NSX_OK && max_rt_odd) st = c->ws(WS_FILL_LIST, recon_list_bytes(max_rt_odd), &p);
+        nsx_status st = NSX_OK;
+        for (int q = 0; q < 2 && st == NSX_OK; ++q) {
+            if (!max_pix[q]) continue;
+            const int base = q ? WS_CCL2_LABEL : WS_CCL_LABEL;
+            const int64_t nw = max_pix[q] / 32, nb = max_pix[q] / 32768;
+            if (st == NSX_OK) st = c->ws(base + 0, sizeof(int32_t) * (size_t)max_pix[q], &p);
+            if (st == NSX_OK) st = c->ws(base + 1, sizeof(int32_t) * (size_t)max_pix[q], &p);
+            if (st == NSX_OK) st = c->ws(base + 2, sizeof(uint2) * (size_t)max_ct[q] * 2048, &p);
+            if (st == NSX_OK) st = c->ws(base + 3, sizeof(int32_t) * (size_t)max_ct[q], &p);
+            if (st == NSX_OK) st = c->ws(base + 4, sizeof(uint32_t) * (size_t)nw, &p);
+            if (st == NSX_OK) st = c->ws(base + 5, sizeof(uint32_t) * (size_t)(nw + nb + 1), &p);
+            if (st == NSX_OK)
+                st = c->ws(q ? WS_CCL2_LROOT : WS_CCL_LROOT, sizeof(uint16_t) * (size_t)max_pix[q], &p);
+            const int rbase = q ? WS_FILL_STAMP : WS_RECON_STAMP;
+            if (st == NSX_OK) st = c->ws(rbase + 0, sizeof(uint32_t) * (size_t)max_rt[q], &p);
+            if (st == NSX_OK) st = c->ws(rbase + 1, recon_list_bytes(max_rt[q]), &p);
+        }
         if (st != NSX_OK) return fail_to_owner(b, st);
         std::vector<ReconRun> runs;
         st = enqueue(b, s, &runs);
@@ -352,7 +392,7 @@ nsx_status nsx_batch_counts(nsx_batch* b, int32_t* counts, int n) {
         CclBatch cb{};
         cb.nimg = (int)g.items.size(); cb.ntiles = g.ctiles; cb.pix_total = g.pix_total;
         cb.host_desc = g.cdesc.data(); cb.dev_desc = g.d_cdesc; cb.tile_img = g.d_ctile; cb.in_dtype = g.dtype;
-        cb.ws_base = g.ws_ccl; cb.ncomp_dev = g.d_ncomp;
+        cb.ws_base = g.ws_ccl; cb.lroot_slot = g.lroot_slot; cb.ncomp_dev = g.d_ncomp;
         CclParams p;
         p.conn = g.ccl_conn; p.out_mode = CO_CANON; p.threshold = true; p.thr = g.thresh;
         if (st == NSX_OK) st = ccl_launch(c, cb, p, c->stream);
@@ -405,6 +445,18 @@ nsx_status nsx_batch_autotune(nsx_batch* b, void* stream, int reps, int32_t* pai
     st = trial(&cur_ms);
     for (int pass = 0; pass < 2 && st == NSX_OK; ++pass) {
         bool changed = false;
+        if (b->groups.size() > 1 && st == NSX_OK) {
+            // odd groups on their own stream, or every group on one stream
+            b->overlap = !b->overlap;
+            float ms = 0.f;
+            st = trial(&ms);
+            if (st == NSX_OK && ms < cur_ms) {
+                cur_ms = ms;
+                changed = true;
+            } else {
+                b->overlap = !b->overlap;
+            }
+        }
         for (size_t gi = 0; gi < b->groups.size() && st == NSX_OK; ++gi) {
             Group& g = b->groups[gi];
             // try one value of one parameter; keep it if faster
@@ -445,9 +497,10 @@ nsx_status nsx_batch_get_tuning(nsx_batch* b, int group, int32_t* out, int n) {
     if (!b || !b->finalized || !out || n < 0 || group < 0) return NSX_ERR_INVALID;
     if ((size_t)group >= b->groups.size()) return NSX_ERR_INVALID;
     const Group& g = b->groups[group];
-    const int32_t v[4] = {g.visit_sweeps > 0 ? g.visit_sweeps : b->ws->visit_sweeps,
-                          (g.drain != 0 ? g.drain : b->drain_default) > 0 ? 1 : 0, g.xcd_split, g.drain_occ};
-    for (int i = 0; i < n && i < 4; ++i) out[i] = v[i];
+    const int32_t v[5] = {g.visit_sweeps > 0 ? g.visit_sweeps : b->ws->visit_sweeps,
+                          (g.drain != 0 ? g.drain : b->drain_default) > 0 ? 1 : 0, g.xcd_split, g.drain_occ,
+                          b->overlap && b->groups.size() > 1 ? 1 : 0};
+    for (int i = 0; i < n && i < 5; ++i) out[i] = v[i];
     return NSX_OK;
 }
 
@@ -464,6 +517,13 @@ nsx_status nsx_batch_set_drain(nsx_batch* b, int drain) {
     return NSX_OK;
 }
 
+nsx_status nsx_batch_set_overlap(nsx_batch* b, int overlap) {
+    if (!b || (overlap != 0 && overlap != 1)) return NSX_ERR_INVALID;
+    b->overlap = overlap != 0;
+    if (b->exec) NSX_TRY(capture(b));
+    return NSX_OK;
+}
+
 nsx_ctx* nsx_batch_context(nsx_batch* b) { return b ? b->ws : nullptr; }
 
 void nsx_batch_destroy(nsx_batch* b) {
@@ -472,6 +532,9 @@ void nsx_batch_destroy(nsx_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    if (b->side) (void)hipStreamDestroy(b->side);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
     for (auto& g : b->groups) {
         for (void* p : {g.d_rdesc, (void*)g.d_rtile, g.d_cdesc, (void*)g.d_ctile, (void*)g.d_ncomp, (void*)g.d_pending})
             if (p) (void)hipFree(p);

@@ -760,7 +760,7 @@ nsx_status ccl_launch(nsx_ctx* ctx, const CclBatch& B, const CclParams& p, hipSt
     NSX_TRY(ctx->ws(B.ws_base + 4, sizeof(uint32_t) * (size_t)A.nwords, &bits));
     NSX_TRY(ctx->ws(B.ws_base + 5, sizeof(uint32_t) * (size_t)(A.nwords + A.nblocks + 1), &scan));
     void* lroot;
-    NSX_TRY(ctx->ws(WS_CCL_LROOT, sizeof(uint16_t) * (size_t)B.pix_total, &lroot));
+    NSX_TRY(ctx->ws(B.lroot_slot, sizeof(uint16_t) * (size_t)B.pix_total, &lroot));
     A.lroot = (uint16_t*)lroot;
     A.lab = (int32_t*)lab; A.area = (int32_t*)area; A.entries = (uint2*)ent; A.tcount = (int32_t*)tcnt;
     A.bits = (uint32_t*)bits;

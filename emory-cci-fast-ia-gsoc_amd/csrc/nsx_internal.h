@@ -155,6 +155,13 @@ enum WsSlot {
     WS_BP_L,             // bit-plane reconstruction: class labels of a pass (u8)
     WS_BP_W,             //   and its seed weights (u8)
     WS_BP_COPY,          //   an input that is also the output, copied first
+    WS_CCL2_LABEL,       // a second labelling in flight (a batch's odd tile groups, on their own
+    WS_CCL2_AREA,        //   stream): the same six slots as WS_CCL_LABEL .. WS_CCL_MISC
+    WS_CCL2_ROOTS,
+    WS_CCL2_BITS,
+    WS_CCL2_SCAN,
+    WS_CCL2_MISC,
+    WS_CCL2_LROOT,
     WS_NSLOTS
 };
 
@@ -343,6 +350,7 @@ struct CclBatch {
     const int32_t* tile_img = nullptr;  // device: tile -> image (nimg > 1)
     int in_dtype = NSX_U8;
     int ws_base = 0;                  // first of 6 consecutive workspace slots
+    int lroot_slot = WS_CCL_LROOT;    // slot of the u16 tile-local roots
     int32_t* ncomp_dev = nullptr;     // device: kept components per image
 };
 size_t ccl_desc_bytes();

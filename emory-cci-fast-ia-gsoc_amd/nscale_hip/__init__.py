@@ -472,16 +472,21 @@ class Batch:
         return self.tuning()
 
     def tuning(self) -> list:
-        """Per tile group: {"visit_pairs", "drain", "xcd_split", "drain_blocks_per_cu"}."""
+        """Per tile group: {"visit_pairs", "drain", "xcd_split", "drain_blocks_per_cu",
+        "overlap"} (overlap: odd groups on a second stream; batch-wide)."""
         out = []
         g = 0
         while True:
-            v = (C.c_int32 * 4)()
-            if self.lib.nsx_batch_get_tuning(self.h, g, v, 4) != _lib.NSX_OK:
+            v = (C.c_int32 * 5)()
+            if self.lib.nsx_batch_get_tuning(self.h, g, v, 5) != _lib.NSX_OK:
                 return out
             out.append({"visit_pairs": int(v[0]), "drain": int(v[1]), "xcd_split": int(v[2]),
-                        "drain_blocks_per_cu": int(v[3])})
+                        "drain_blocks_per_cu": int(v[3]), "overlap": int(v[4])})
             g += 1
+
+    def set_overlap(self, on: bool) -> None:
+        """Odd tile groups on a second stream beside the even ones (default), or all in turn."""
+        self.ctx.check(self.lib.nsx_batch_set_overlap(self.h, int(bool(on))))
 
     def set_drain(self, on: bool) -> None:
         """Every group to the persistent drain (default) or to per-round launches."""

@@ -92,6 +92,7 @@ SIGNATURES = {
     "nsx_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, _I32P, C.c_int]),
     "nsx_batch_get_tuning": (C.c_int, [C.c_void_p, C.c_int, _I32P, C.c_int]),
     "nsx_batch_set_drain": (C.c_int, [C.c_void_p, C.c_int]),
+    "nsx_batch_set_overlap": (C.c_int, [C.c_void_p, C.c_int]),
     "nsx_batch_context": (C.c_void_p, [C.c_void_p]),
     "nsx_batch_destroy": (None, [C.c_void_p]),
     "nsx_server_create": (C.c_int, [C.c_int, C.c_size_t, C.c_int, P(C.c_void_p)]),

@@ -357,7 +357,9 @@ nsx_status nsx_batch_counts(nsx_batch* batch, int32_t* counts, int n);
  * launch per round, 3..6 sweep pairs per tile visit (NSX_TUNE_VISIT_SWEEPS for
  * that group only), the split of a round's tiles over the XCDs (plain stride
  * or a contiguous share per XCD) and the drain's blocks per CU (all that stay
- * resident, 8 or 4).  Each setting is timed as the mean of `reps` graph
+ * resident, 8 or 4); before each pass, whether odd groups run on a second
+ * stream, overlapping the even groups' launches (the default), or every group
+ * runs on the caller's stream.  Each setting is timed as the mean of `reps` graph
  * replays on `stream` after re-learning the round counts and re-capturing;
  * the fastest is kept and left captured (a second pass runs if the first
  * changed anything).  pairs[g] (g < npairs) receives the sweep pairs, 0 past
@@ -368,13 +370,22 @@ nsx_status nsx_batch_autotune(nsx_batch* batch, void* stream, int reps, int32_t*
 /* Launch shape of tile group `group` (groups in order of first appearance):
  * out[0] sweep pairs per visit, out[1] 1 persistent drain / 0 per-round
  * launches, out[2] 1 XCD-contiguous split / 0 plain stride, out[3] drain
- * blocks per CU (0: all that stay resident); n <= 4 entries.  NSX_ERR_INVALID
- * past the last group.  No reference counterpart. */
+ * blocks per CU (0: all that stay resident), out[4] 1 when odd groups run on
+ * a second stream beside the even ones (the batch-wide overlap setting);
+ * n <= 5 entries.  NSX_ERR_INVALID past the last group.  No reference
+ * counterpart. */
 nsx_status nsx_batch_get_tuning(nsx_batch* batch, int group, int32_t* out, int n);
 /* Sets every group to the persistent drain (drain = 1) or to per-round
  * launches (0, the default), dropping the tuner's choice of it; re-learns
  * round counts and re-captures a finalized batch.  No reference counterpart. */
 nsx_status nsx_batch_set_drain(nsx_batch* batch, int drain);
+/* overlap = 1 (default): odd tile groups run on a second stream beside the
+ * even ones (forked from and joined back to the run's stream, also inside the
+ * graph), so one group's late rounds, which hold few tiles, overlap the other
+ * group's launches; 0: every group on the run's stream in turn.  Groups on the
+ * persistent drain always run in turn.  Re-captures a finalized batch.  No
+ * reference counterpart. */
+nsx_status nsx_batch_set_overlap(nsx_batch* batch, int overlap);
 /* The batch's private context (owned by the batch; valid until
  * nsx_batch_destroy): its kernel timing (nsx_ctx_set_timing /
  * nsx_ctx_read_timing, recorded by eager runs of a batch finalized without a

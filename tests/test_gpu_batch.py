@@ -32,14 +32,18 @@ def expected(orc, marker, mask, thr):
     return r, lab, n
 
 
-@pytest.mark.parametrize("drain", [True, False])
+@pytest.mark.parametrize("shape", ["drain", "serial", "overlap"])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_batch_matches_oracle_every_replay(nh, orc, gpu, use_graph, drain):
+def test_batch_matches_oracle_every_replay(nh, orc, gpu, use_graph, shape):
+    """Three tile groups (u8, f32, i32) under each launch shape: drained,
+    per-round launches in turn, and per-round with the odd group on a second
+    stream (the default)."""
     import torch
     tiles = make_tiles(nh, 9, 1)
     want = [expected(orc, m, k, t) for m, k, t in tiles]
     batch = nh.Batch()
-    batch.set_drain(drain)
+    batch.set_drain(shape == "drain")
+    batch.set_overlap(shape == "overlap")
     outs = []
     for marker, mask, thr in tiles:
         dm = torch.from_numpy(marker).cuda()
@@ -55,7 +59,14 @@ def test_batch_matches_oracle_every_replay(nh, orc, gpu, use_graph, drain):
             lab.fill_(-5)
         junk = torch.randn(256, 256, device="cuda") @ torch.randn(256, 256, device="cuda")
         torch.cuda.synchronize()
-        batch.run()
+        if shape == "overlap" and not use_graph:
+            # an eager run forks only from a stream object (not torch's default stream)
+            side = torch.cuda.Stream()
+            with torch.cuda.stream(side):
+                batch.run()
+            side.synchronize()
+        else:
+            batch.run()
         counts = batch.counts()
         for i, ((r, lab), (wr, wlab, wn)) in enumerate(zip(outs, want)):
             assert np.array_equal(r.cpu().numpy(), wr), f"replay {replay} tile {i} recon"
@@ -104,7 +115,7 @@ def test_batch_autotune_keeps_results(nh, orc, gpu):
     assert 1 <= len(tuned) <= 3 and tuned == batch.tuning()
     for g in tuned:
         assert 3 <= g["visit_pairs"] <= 6 and g["drain"] in (0, 1) and g["xcd_split"] in (0, 1)
-        assert g["drain_blocks_per_cu"] in (0, 8, 4)
+        assert g["drain_blocks_per_cu"] in (0, 8, 4) and g["overlap"] in (0, 1)
     for replay in range(2):
         for r, lab in outs:
             r.fill_(7)
