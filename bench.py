@@ -1024,6 +1024,30 @@ def roofline(kt: dict, px: int, tiles: dict, steps: int, traffic: dict | None = 
             "per_kind_ms_per_step": {k: round(v[0] / steps, 4) for k, v in kt.items()}}
 
 
+def changed_input_ms(nh, ctx, n, args, d_marker, d_mask, d_recon, d_fill, d_lab, stream) -> dict:
+    """Steps whose input changes every time: the chain alternates between the
+    workload and a second image of the same size and construction (another
+    generator seed), so the round prediction carried from the previous call
+    is for other content (a short prediction costs a host round trip and the
+    stages after it again).  Reports ms per step over the alternation.  Runs
+    after the timed steps."""
+    import torch
+    mask2 = nh.gen_disks_u8(n, n, args.disks, 0x5EED0000 + 77)
+    marker2 = nh.gen_marker_u8(mask2, 30)
+    pairs = [(d_marker, d_mask), (torch.from_numpy(marker2).to(d_mask.device), torch.from_numpy(mask2).to(d_mask.device))]
+    steps = max(4, args.steps // 2) // 2 * 2
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        dm, dk = pairs[i % 2]
+        nh.segment_chain(dm, dk, 8, 4, 128.0, 8, 11, 1000, recon_out=d_recon, fill_out=d_fill, labels_out=d_lab,
+                         ctx=ctx, stream=stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    return {"ms_per_step": round(ms, 4), "steps": steps, "inputs": 2,
+            "note": f"alternating two {n}^2 disk images (generator seeds 0x5EED0001, 0x5EED004D)"}
+
+
 def cpu_baseline(marker, mask, runs: int, gpu_first) -> tuple:
     """The oracle's single-threaded restatement of the reference chain, timed
     on this host on `runs` repetitions of the same 4096x4096 workload.  The
@@ -1093,8 +1117,13 @@ def main():
         return nh.segment_chain(d_marker, d_mask, 8, 4, 128.0, 8, 11, 1000, recon_out=d_recon, fill_out=d_fill,
                                 labels_out=d_lab, ctx=ctx, stream=stream)[3]
 
+    # a cold call: a fresh context has no round prediction for this size, so
+    # both propagations run past the default and are completed by the host
+    torch.cuda.synchronize()
+    c0 = time.perf_counter()
     ncomp = step()
     torch.cuda.synchronize()
+    cold_ms = (time.perf_counter() - c0) * 1e3
     first = (d_recon.cpu().numpy(), d_fill.cpu().numpy(), d_lab.cpu().numpy(), ncomp)
 
     for _ in range(args.warmup):
@@ -1110,6 +1139,7 @@ def main():
     elapsed = time.perf_counter() - t0
     stats = ctx.recon_stats(0)
     fstats = ctx.recon_stats(1)
+    changed = changed_input_ms(nh, ctx, n, args, d_marker, d_mask, d_recon, d_fill, d_lab, stream)
     # Per-kernel durations come from a second set of steps with HIP events
     # around every launch (events add a gap between launches, so the timed
     # steps above run without them).
@@ -1149,6 +1179,8 @@ def main():
         "dtype": "u8",
         "data": "synthetic",
         "bitexact": exact,
+        "cold_call_ms": round(cold_ms, 3),
+        "changed_input": changed,
         "config": {"workload": f"{n}x{n} u8 disks({args.disks})+noise: recon8 -> fillholes4 -> >128 -> "
                                "bwlabel8 -> bwareaopen[11,1000) -> relabel",
                    "global_batch": 1, "pixels": px, "components": ncomp,

@@ -91,18 +91,32 @@ struct CclArgs {
     const uint32_t* floor_mmax;
 };
 
+// The tile -> image entry and the image descriptor are read through the
+// constant address space with a wave-uniform index, so they are scalar loads
+// into SGPRs and every row address computed from them stays on the scalar unit.
+template <typename T> using cp = const __attribute__((address_space(4))) T*;
+
 __device__ __forceinline__ void tile_of(const CclArgs& A, int t, CclImg& D, int& tx, int& ty) {
     t = __builtin_amdgcn_readfirstlane(t);
     if (A.nimg == 1) {
         D = A.img0;
     } else {
-        const int idx = __builtin_amdgcn_readfirstlane(A.tile_img[t]);
-        D = A.imgs[idx];
+        const int idx = __builtin_amdgcn_readfirstlane(((cp<int32_t>)A.tile_img)[t]);
+        static_assert(sizeof(CclImg) % 4 == 0, "descriptor copied as words");
+        const cp<uint32_t> src = (cp<uint32_t>)(A.imgs + idx);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&D);
+#pragma unroll
+        for (int k = 0; k < (int)(sizeof(CclImg) / 4); ++k) dst[k] = src[k];
     }
     const int lt = t - D.tile_base;
     ty = lt / D.ntx;
     tx = lt - ty * D.ntx;
 }
+
+// Loads and stores through global (not flat) pointers.
+template <typename T> using gp = __attribute__((address_space(1))) T*;
+template <typename T> __device__ __forceinline__ T gload(const T* p) { return *(gp<const T>)p; }
+template <typename T> __device__ __forceinline__ void gstore(T* p, T v) { *(gp<T>)p = v; }
 
 // Pixel value used for connectivity: raw u8, or (in > thr) for the fused
 // threshold (OpenCV `Mat > k`, 255/0 -> here 1/0).  The threshold is
@@ -110,7 +124,7 @@ __device__ __forceinline__ void tile_of(const CclArgs& A, int t, CclImg& D, int&
 // integers: v > k  <=>  v > floor(k); thr_f for float).
 template <typename InT, bool THR>
 __device__ __forceinline__ uint32_t pix(const CclArgs& A, const CclImg& D, int y, int x) {
-    const InT v = reinterpret_cast<const InT*>(D.in + (int64_t)y * D.pin)[x];
+    const InT v = gload(reinterpret_cast<const InT*>(D.in + (int64_t)y * D.pin) + x);
     if constexpr (!THR) {
         return (uint32_t)v;
     } else if constexpr (sizeof(InT) == 4 && !__is_same(InT, int32_t)) {
@@ -210,7 +224,8 @@ template <int CONN, typename InT, bool THR>
 __device__ __forceinline__ void local_label(const CclArgs& A, const CclImg& D, int x0, int y0, uint32_t (&v)[CPT],
                                             uint32_t* edge, uint32_t* par) {
     const int tid = threadIdx.x;
-    const int lx = tid & 63, w = tid >> 6;
+    // w is wave-uniform: readfirstlane lets the row arithmetic run on the scalar unit
+    const int lx = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int gx = x0 + lx;
     // all loads first (independent, in flight together), then the LDS writes
 #pragma unroll
@@ -284,7 +299,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
     const int x0 = tx * CW, y0 = ty * CH;
     for (int i = tid; i < CN; i += CTHREADS) cnt[i] = 0;
     if (tid == 0) nroots = 0;
-    const int lx = tid & 63, w = tid >> 6;
+    const int lx = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int gx = x0 + lx;
     uint32_t wv[WMAX ? CPT : 1];
     if (WMAX) {
@@ -293,7 +308,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
 #pragma unroll
         for (int j = 0; j < CPT; ++j) {
             const int cy = min(y0 + band_row(w, j), D.rows - 1);
-            wv[j] = ((const uint8_t*)(D.wgt + (int64_t)cy * D.pw))[cx];
+            wv[j] = gload((const uint8_t*)(D.wgt + (int64_t)cy * D.pw) + cx);
         }
     }
     uint32_t v[CPT];
@@ -325,7 +340,7 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
         const int32_t g = D.pix_base + (y0 + ly) * D.cols + gx;
         // the output kernel reads each pixel's local root back (2 B) instead of
         // labelling the tile again
-        if (gx < D.cols && y0 + ly < D.rows) A.lroot[g] = r == BGL ? (uint16_t)0xFFFF : (uint16_t)r;
+        if (gx < D.cols && y0 + ly < D.rows) gstore(A.lroot + g, r == BGL ? (uint16_t)0xFFFF : (uint16_t)r);
         if (r == BGL) continue;
         if (r == (uint32_t)i) {
             const uint32_t k = atomicAdd(&nroots, 1u);
@@ -569,6 +584,10 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
     const int tb = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     tile_of(A, tb, D, tx, ty);
     const int x0 = tx * CW, y0 = ty * CH;
+    // w stays a per-lane value here: made wave-uniform (as in the local kernel)
+    // the rows' loads and stores split into scalar branches and this kernel
+    // ran 12-25 % slower (same-session A/B on the 4096^2 chain, the batch and
+    // the many-valued maze)
     const int lx = tid & 63, w = tid >> 6;
     const int gx = x0 + lx;
     // each pixel's tile-local root, as the local kernel left it, and for a
@@ -580,9 +599,9 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
     for (int j = 0; j < CPT; ++j) {
         const int gy = y0 + w + 4 * j;
         const bool in = gx < D.cols && gy < D.rows;
-        const uint16_t r = in ? A.lroot[D.pix_base + gy * D.cols + gx] : (uint16_t)0xFFFF;
+        const uint16_t r = in ? gload(A.lroot + D.pix_base + gy * D.cols + gx) : (uint16_t)0xFFFF;
         par[j] = r == 0xFFFF ? BGL : (uint32_t)r;
-        prev[j] = (fold && in) ? ((const uint8_t*)(D.out + (int64_t)gy * D.pout))[gx] : 0u;
+        prev[j] = (fold && in) ? gload((const uint8_t*)(D.out + (int64_t)gy * D.pout) + gx) : 0u;
     }
     // Each local root resolves its output once, one lane per local root: each
     // wave compacts the roots among its pixels into its own LDS list (ballot
@@ -631,14 +650,14 @@ __global__ __launch_bounds__(CTHREADS) void ccl_final_kernel(CclArgs A) {
         char* row = D.out + (int64_t)gy * D.pout;
         if (A.out_mode == CO_LEVEL) {
             // u8 reconstruction accumulated over levels: foreground only
-            if (!fold) ((uint8_t*)row)[gx] = (uint8_t)(r == BGL ? floorv : max(floorv, outv[r]));
-            else if (r != BGL) ((uint8_t*)row)[gx] = (uint8_t)max((int32_t)prev[j], outv[r]);
-            if (LL) ((int32_t*)(D.lout + (int64_t)gy * D.plout))[gx] = r == BGL ? 0 : outl[r];
+            if (!fold) gstore((uint8_t*)row + gx, (uint8_t)(r == BGL ? floorv : max(floorv, outv[r])));
+            else if (r != BGL) gstore((uint8_t*)row + gx, (uint8_t)max((int32_t)prev[j], outv[r]));
+            if (LL) gstore((int32_t*)(D.lout + (int64_t)gy * D.plout) + gx, r == BGL ? 0 : outl[r]);
             continue;
         }
         const int32_t o = (r == BGL) ? bg : outv[r];
-        if (A.out_mode == CO_FLAT_U8) ((uint8_t*)row)[gx] = (uint8_t)o;
-        else ((int32_t*)row)[gx] = o;
+        if (A.out_mode == CO_FLAT_U8) gstore((uint8_t*)row + gx, (uint8_t)o);
+        else gstore((int32_t*)row + gx, o);
     }
 }
 

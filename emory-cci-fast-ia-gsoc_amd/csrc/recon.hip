@@ -407,12 +407,18 @@ struct ReconArgs {
 };
 
 // The tile id is wave-uniform; readfirstlane makes that visible to the
-// compiler so the descriptor is fetched with scalar loads into SGPRs.
+// compiler, and the tile -> image entry and the descriptor are read through
+// the constant address space, so they are scalar loads into SGPRs.
+template <typename T> using cptr = const __attribute__((address_space(4))) T*;
 __device__ __forceinline__ void tile_of(const ReconArgs& A, int32_t t, ImgDesc& D, int& lt) {
     t = __builtin_amdgcn_readfirstlane(t);
     if (A.nimg == 1) { D = A.img0; lt = t; return; }
-    const int idx = __builtin_amdgcn_readfirstlane(A.tile_img[t]);
-    D = A.imgs[idx];
+    const int idx = __builtin_amdgcn_readfirstlane(((cptr<int32_t>)A.tile_img)[t]);
+    static_assert(sizeof(ImgDesc) % 4 == 0, "descriptor copied as words");
+    const cptr<uint32_t> src = (cptr<uint32_t>)(A.imgs + idx);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&D);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(ImgDesc) / 4); ++k) dst[k] = src[k];
     lt = t - D.tile_base;
 }
 
