@@ -1139,6 +1139,9 @@ def main():
     elapsed = time.perf_counter() - t0
     stats = ctx.recon_stats(0)
     fstats = ctx.recon_stats(1)
+    # work-list occupancy of every propagation round of the last timed step
+    # (tiles of 62 x 62 pixels queued per round; entry 0 is the first pass)
+    work_lists = {"recon": ctx.round_log(0), "fill": ctx.round_log(1)}
     changed = changed_input_ms(nh, ctx, n, args, d_marker, d_mask, d_recon, d_fill, d_lab, stream)
     # Per-kernel durations come from a second set of steps with HIP events
     # around every launch (events add a gap between launches, so the timed
@@ -1181,6 +1184,7 @@ def main():
         "bitexact": exact,
         "cold_call_ms": round(cold_ms, 3),
         "changed_input": changed,
+        "work_lists": work_lists,
         "config": {"workload": f"{n}x{n} u8 disks({args.disks})+noise: recon8 -> fillholes4 -> >128 -> "
                                "bwlabel8 -> bwareaopen[11,1000) -> relabel",
                    "global_batch": 1, "pixels": px, "components": ncomp,

@@ -234,12 +234,14 @@ nsx_status nsx_ctx_create(nsx_ctx** out, int device, size_t hbm_budget_bytes) {
         c->cu_count = cus;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void**)&c->d_stats, kStatsBytes) != hipSuccess ||
+        hipMalloc((void**)&c->d_round_log, 2 * sizeof(int32_t) * nsx_ctx::kRoundLog) != hipSuccess ||
         hipHostMalloc((void**)&c->h_pinned, 64 * sizeof(int32_t)) != hipSuccess) {
         nsx_ctx_destroy(c);
         return NSX_ERR_HIP;
     }
     hipMemset(c->d_stats, 0, kStatsBytes);
-    c->held = kStatsBytes;
+    hipMemset(c->d_round_log, 0, 2 * sizeof(int32_t) * nsx_ctx::kRoundLog);
+    c->held = kStatsBytes + 2 * sizeof(int32_t) * nsx_ctx::kRoundLog;
     // the allocation-free entry point's scratch array: its device address,
     // once per device, with the device current
     if (recon_noalloc_init(device) != NSX_OK) {
@@ -257,6 +259,7 @@ void nsx_ctx_destroy(nsx_ctx* ctx) {
     for (auto& b : ctx->bufs)
         if (b.ptr) hipFree(b.ptr);
     if (ctx->d_stats) hipFree(ctx->d_stats);
+    if (ctx->d_round_log) hipFree(ctx->d_round_log);
     if (ctx->h_pinned) hipHostFree(ctx->h_pinned);
     ctx->collect_timing();
     for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
@@ -401,6 +404,20 @@ nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int stage, int64_t stats[4]) {
     stats[1] = h[1];
     stats[2] = h[2];
     stats[3] = h[3];
+    return NSX_OK;
+}
+
+nsx_status nsx_last_round_log(nsx_ctx* ctx, int stage, int32_t* counts, int n, int* rounds) {
+    if (!ctx || !counts || !rounds || n < 0 || stage < 0 || stage > 1) return NSX_ERR_INVALID;
+    int64_t h[4];
+    NSX_HIP_TRY(ctx, hipDeviceSynchronize());
+    NSX_HIP_TRY(ctx, hipMemcpy(h, ctx->d_stats + 8 * stage, sizeof(h), hipMemcpyDeviceToHost));
+    const int r = (int)std::min<int64_t>(h[0] + 1, nsx_ctx::kRoundLog);
+    *rounds = r;
+    const int k = std::min(n, r);
+    if (k > 0)
+        NSX_HIP_TRY(ctx, hipMemcpy(counts, ctx->d_round_log + (size_t)stage * nsx_ctx::kRoundLog,
+                                   sizeof(int32_t) * (size_t)k, hipMemcpyDeviceToHost));
     return NSX_OK;
 }
 

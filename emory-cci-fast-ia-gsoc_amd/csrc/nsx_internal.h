@@ -75,6 +75,11 @@ struct nsx_ctx {
     int64_t first_tiles[kCumKinds] = {};
     int32_t* h_pinned = nullptr;     // small pinned readback area
     int cu_count = 256;
+    // Work-list length of every propagation round of the last reconstruction
+    // per stats stage (0, 1): kRoundLog ints each, round r at [r] (r = 0 the
+    // first pass), written by the round kernels.
+    static constexpr int kRoundLog = 4096;
+    int32_t* d_round_log = nullptr;
 
     // Tuning (nsx_ctx_set_tuning).
     int visit_sweeps = 4;

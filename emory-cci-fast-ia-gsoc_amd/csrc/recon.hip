@@ -404,6 +404,7 @@ struct ReconArgs {
     int visit_sweeps;          // sweep pairs per visit before re-queueing (1..MAX_IT)
     int first_list;            // round 0 takes its tiles from the list (fill: the border tiles)
     int xcd_split;             // each XCD takes a contiguous share of the work items (else a plain stride)
+    int32_t* round_log;        // work-list length of each round, nsx_ctx::kRoundLog entries (may be null)
 };
 
 // The tile id is wave-uniform; readfirstlane makes that visible to the
@@ -709,6 +710,7 @@ void recon_round_kernel(ReconArgs A, int round) {
                 A.stats[1] += n;
                 if (A.cum) atomicAdd(A.cum, (unsigned long long)n);
             }
+            if (A.round_log && round < nsx_ctx::kRoundLog) A.round_log[round] = n;
         }
     }
     // Blocks go to the XCDs in turn (block b to XCD b mod 8).  With xcd_split
@@ -827,6 +829,7 @@ __device__ __forceinline__ void drain_header(const ReconArgs& A, int round, int 
         A.stats[0] += 1;
         A.stats[1] += n;
         if (A.cum) atomicAdd(A.cum, (unsigned long long)n);
+        if (A.round_log && round < nsx_ctx::kRoundLog) A.round_log[round] = n;
     }
 }
 
@@ -931,7 +934,7 @@ struct StateFill {
 // lists the border tiles of a single ntx x nty image as round 0's work, and
 // writes the initial state F; with all_tiles it lists every tile instead.
 __global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts, int64_t* stats, int32_t* list0,
-                                   int ntx, int nty, StateFill F, int all_tiles) {
+                                   int ntx, int nty, StateFill F, int all_tiles, int32_t* round_log) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     // the initial state, a row per block: 16-byte stores between the row's first
     // and last 16-byte boundaries, element stores before and after (and for a
@@ -972,6 +975,7 @@ __global__ void recon_reset_kernel(uint32_t* stamp, int ntiles, int32_t* counts,
     if (i < 3 * NSEG) counts[i * CSTRIDE] = i == 0 ? nb : 0;
     if (i < CTL_WORDS) counts[CTL_OFF + i] = 0;
     if (i == 0) { stats[0] = 0; stats[1] = 0; stats[2] = ntiles; stats[3] = 0; }
+    if (i == 0 && round_log) round_log[0] = nb > 0 ? nb : ntiles;   // round 0: the first pass
 }
 
 // ---- image-wide row and column scans (hole filling's start) -------------------
@@ -1492,6 +1496,7 @@ void recon_quad_kernel(ReconArgs A, int round) {
                 A.stats[1] += n;
                 if (A.cum) atomicAdd(A.cum, (unsigned long long)n);
             }
+            if (A.round_log && round < nsx_ctx::kRoundLog) A.round_log[round] = n;
         }
     }
     // Blocks go to the XCDs in turn (block b to XCD b mod 8).  With xcd_split
@@ -1736,6 +1741,10 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
     A.list1 = (int32_t*)lists + (size_t)NSEG * A.ntiles;
     A.counts = (int32_t*)counts;
     A.stats = B.stats ? B.stats : ctx->d_stats;
+    // the round log of the stats stage (0 or 1) these statistics belong to
+    A.round_log = A.stats == ctx->d_stats       ? ctx->d_round_log
+                  : A.stats == ctx->d_stats + 8 ? ctx->d_round_log + nsx_ctx::kRoundLog
+                                                : nullptr;
     const int cum_idx = mode * 3 + dtype;
     A.cum = (unsigned long long*)(ctx->d_stats + nsx_ctx::kCumWord + cum_idx);
     // four-wave hole-filling tiles run one pair more: their pairs are shorter
@@ -1779,7 +1788,7 @@ nsx_status recon_launch(nsx_ctx* ctx, int mode, int dtype, int conn, const Recon
         grid_reset = std::max(grid_reset, std::min(4 * ctx->cu_count, D0.rows));
     }
     hipLaunchKernelGGL(recon_reset_kernel, dim3(grid_reset), dim3(256), 0, s, A.stamp, A.ntiles, A.counts, A.stats,
-                       A.list0, border_start ? D0.ntx : 0, D0.nty, F, scan_start ? 1 : 0);
+                       A.list0, border_start ? D0.ntx : 0, D0.nty, F, scan_start ? 1 : 0, A.round_log);
     if (scan_start) {
         const int hs = ctx->tic(TK_RECON_SCAN, s);
         if (dtype == NSX_U8) run_scans<uint8_t, RM_FILL>(D0, ctx->fill_scans, segf, s);
@@ -2059,7 +2068,7 @@ nsx_status recon_run_noalloc(nsx_ctx* ctx, const View& marker, const View& mask,
     // the stamps, counters and statistics start cleared on every call (a kernel,
     // not a memset, so the sequence can also be captured)
     hipLaunchKernelGGL(recon_reset_kernel, dim3(std::max(1, std::min(1024, (ntiles + 255) / 256))), dim3(256), 0, s,
-                       A.stamp, A.ntiles, A.counts, A.stats, A.list0, 0, 0, StateFill{}, 0);
+                       A.stamp, A.ntiles, A.counts, A.stats, A.list0, 0, 0, StateFill{}, 0, (int32_t*)nullptr);
     first(A, 0, ntiles, s);
     // the round count of the last same-sized reconstruction on this context
     // (read only: the prediction table is not extended here)

@@ -189,6 +189,14 @@ class Context:
         self.check(self.lib.nsx_ctx_get_option(self.h, option, C.byref(v)))
         return int(v.value)
 
+    def round_log(self, stage: int = 0) -> list:
+        """Tiles in each propagation round's work list of the last reconstruction
+        (stage 1: the chain's fill); entry 0 is the first pass."""
+        buf = (C.c_int32 * 4096)()
+        n = C.c_int(0)
+        self.check(self.lib.nsx_last_round_log(self.h, stage, buf, 4096, C.byref(n)))
+        return [int(buf[i]) for i in range(n.value)]
+
     def recon_stats(self, stage: int = 0) -> dict:
         """Statistics of the last reconstruction (stage 1: the chain's fill)."""
         s = (C.c_int64 * 4)()

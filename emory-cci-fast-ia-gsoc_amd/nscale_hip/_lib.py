@@ -55,6 +55,7 @@ SIGNATURES = {
     "nsx_ctx_synchronize": (C.c_int, [_CTX]),
     "nsx_ctx_workspace_bytes": (C.c_size_t, [_CTX]),
     "nsx_last_recon_stats": (C.c_int, [_CTX, C.c_int, P(C.c_int64)]),
+    "nsx_last_round_log": (C.c_int, [_CTX, C.c_int, P(C.c_int32), C.c_int, P(C.c_int)]),
     "nsx_ctx_set_timing": (C.c_int, [_CTX, C.c_int]),
     "nsx_ctx_read_timing": (C.c_int, [_CTX, P(C.c_double), P(C.c_int64), C.c_int]),
     "nsx_ctx_read_tile_counts": (C.c_int, [_CTX, P(C.c_int64), P(C.c_int64), C.c_int]),

@@ -69,6 +69,14 @@ size_t nsx_ctx_workspace_bytes(const nsx_ctx* ctx);
  * first full pass, stats[1] = tile visits in those rounds, stats[2] = tiles,
  * stats[3] = sweep half-passes summed over all visits.  Synchronises the device. */
 nsx_status nsx_last_recon_stats(nsx_ctx* ctx, int stage, int64_t stats[4]);
+/* Work-list occupancy of the last propagation reconstruction of a stage (as
+ * nsx_last_recon_stats): counts[r] = tiles (62 x 62 pixels) in round r's device
+ * work list, r = 0 the first pass, for the rounds that had work (at most n
+ * entries, 4096 logged); *rounds receives how many are valid.  Reconstructions
+ * that did not run by rounds (level sets, bit planes, component tree) leave the
+ * log of the rounds before they switched, if any.  Synchronises the device.  No
+ * reference counterpart (the reference's queue kernels report nothing). */
+nsx_status nsx_last_round_log(nsx_ctx* ctx, int stage, int32_t* counts, int n, int* rounds);
 
 /* Kernel timing with HIP events recorded on the stream each kernel runs on.
  * nsx_ctx_set_timing(ctx, 1) starts recording (and clears the totals);

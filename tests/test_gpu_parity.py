@@ -572,3 +572,29 @@ def test_distance_transform_4096_bounds(nh, orc, gpu):
     assert int((got != exact).sum()) <= int((want != exact).sum())
     assert np.array_equal(brute.label_distance(nn, binm.shape), got)
     assert brute.distance_fixed_point_violations(nn, binm.shape) == 0
+
+
+def test_round_log(nh, orc, gpu):
+    """The per-round work-list log of the chain's two propagations: entry 0 is
+    the first pass (every tile for the reconstruction, the border tiles for the
+    hole filling), and the later entries add up to the tile visits the
+    statistics count; the same holds under the persistent drain."""
+    mask = nh.gen_disks_u8(1000, 1300, 250, 0x5EED0031)
+    marker = nh.gen_marker_u8(mask, 30)
+    ntx, nty = -(-1300 // 62), -(-1000 // 62)
+    for drain in (0, 1):
+        ctx = nh.Context(0)
+        ctx.set_tuning(nh._lib.NSX_TUNE_DRAIN, drain)
+        try:
+            r, f, lab, n = nh.segment_chain(marker, mask, 8, 4, 128.0, 8, 11, 1000, ctx=ctx)
+            er, ef, elab, en = orc.chain_u8(marker, mask, 8, 4, 128, 8, 11, 1000)
+            assert np.array_equal(f, ef) and np.array_equal(lab, elab) and n == en
+            for stage, first in ((0, ntx * nty), (1, 2 * ntx + 2 * (nty - 2))):
+                st = ctx.recon_stats(stage)
+                log = ctx.round_log(stage)
+                assert len(log) == st["rounds"] + 1, (drain, stage)
+                assert log[0] == first, (drain, stage)
+                assert sum(log[1:]) == st["tile_visits"], (drain, stage)
+                assert all(v > 0 for v in log), (drain, stage)
+        finally:
+            ctx.close()
