@@ -1413,7 +1413,10 @@ __device__ __forceinline__ void process_tile_q(const ReconArgs& A, const ImgDesc
             tb[toB ? posA : posB] = (typename LdsT<T>::type)c[k];
         }
         __syncthreads();
-        const bool moved = (Q.flag[0] | Q.flag[1] | Q.flag[2] | Q.flag[3]) != 0u;
+        uint32_t fl = 0u;
+#pragma unroll
+        for (int q = 0; q < QW; ++q) fl |= Q.flag[q];
+        const bool moved = fl != 0u;
         changed = changed || moved;
         quiet = moved ? 0 : quiet + 1;
         const bool done = quiet >= 1 && half >= 1;
