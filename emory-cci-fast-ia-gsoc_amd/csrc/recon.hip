@@ -691,9 +691,12 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
 
 // u8 tiles keep their mask, block staging / transpose buffer and halo keys in
 // 10.3 KB of LDS and run 3 waves per SIMD (at most 168 VGPRs); 32-bit tiles
-// are LDS-bound at 1.
+// run 2 (at most 256 VGPRs).  f32 later rounds run 1 wave per SIMD: at 2 they
+// spilled 43 registers, at 1 they take 296 with none (batch 29.37 -> 28.91 ms
+// in a same-session A/B).
 template <typename T, int MODE, int CONN, bool FIRST>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 1 ? 3 : 2)))
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 1 ? 3 : (std::is_same<T, float>::value && !FIRST ? 1 : 2))))
 void recon_round_kernel(ReconArgs A, int round) {
     using LT = typename LdsT<T>::type;
     __shared__ LT mb[sizeof(T) == 1 ? TS * LDS_PITCH : 1];   // u8 mask keys of the tile
