@@ -693,10 +693,13 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
 // 10.3 KB of LDS and run 3 waves per SIMD (at most 168 VGPRs); 32-bit tiles
 // run 2 (at most 256 VGPRs).  f32 later rounds run 1 wave per SIMD: at 2 they
 // spilled 43 registers, at 1 they take 296 with none (batch 29.37 -> 28.91 ms
-// in a same-session A/B).
+// in a same-session A/B).  Binary reconstruction's later rounds read three
+// images per tile; they run 2 (u8: 14 spills instead of 49) or 1 (int32: none
+// instead of 99): bwselect 0.354 -> 0.340 ms at 4096^2.
 template <typename T, int MODE, int CONN, bool FIRST>
 __global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 1 ? 3 : (std::is_same<T, float>::value && !FIRST ? 1 : 2))))
+__attribute__((amdgpu_waves_per_eu(MODE == RM_BINARY && !FIRST ? (sizeof(T) == 1 ? 2 : 1)
+                                    : sizeof(T) == 1 ? 3 : (std::is_same<T, float>::value && !FIRST ? 1 : 2))))
 void recon_round_kernel(ReconArgs A, int round) {
     using LT = typename LdsT<T>::type;
     __shared__ LT mb[sizeof(T) == 1 ? TS * LDS_PITCH : 1];   // u8 mask keys of the tile
