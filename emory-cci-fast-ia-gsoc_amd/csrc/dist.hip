@@ -11,10 +11,15 @@
 // Here the same rule runs to a fixed point in 64x64 tiles.  A tile loads its
 // labels and a one-pixel halo into LDS, applies the rule in synchronous passes
 // (read one LDS buffer, write the other) until no label in it changes, and
-// writes the tile to the other global buffer; rounds of tile launches repeat
+// writes the tile to its other global buffer; rounds of tile launches repeat
 // until a round changes nothing.  A tile runs in a round only when it or one of
-// its 8 neighbour tiles changed in the previous round (the others are copied).
-// Every read is of a finished buffer, so the result is deterministic.
+// its 8 neighbour tiles changed in the previous round.  Each tile's labels live
+// in one of two global buffers, named by a per-tile index (one byte per tile
+// and round parity): a tile that changes writes the buffer its index does not
+// name and flips the index for the next round; the others keep theirs, so
+// nothing is copied.  Every read is of the round-start state (the neighbours'
+// indices of this round), so the result is deterministic.  Rounds are
+// launched in batches with one host check of the last round's change word.
 //
 // Labels are packed (y << 16) | x inside the kernels, with DT_FAR for the
 // starting point, so no integer division is needed per offer; the output turns
@@ -78,13 +83,22 @@ __global__ void dt_init_kernel(const char* mask, int64_t pm, uint32_t* lab, int 
     lab[(int64_t)y * cols + x] = m == 0 ? ((uint32_t)y << 16) | (uint32_t)x : DT_FAR;
 }
 
-__global__ __launch_bounds__(256) void dt_tile_kernel(const uint32_t* src, uint32_t* dst, int rows, int cols,
+// buf[b] is label buffer b; idx_cur[t] names the buffer holding tile t's
+// labels at the start of this round, idx_next[t] receives it for the next.
+__global__ __launch_bounds__(256) void dt_tile_kernel(uint32_t* buf0, uint32_t* buf1, int rows, int cols,
                                                       int tiles_x, int tiles_y, int far_y, const uint8_t* prev,
-                                                      uint8_t* next, int32_t* changed) {
+                                                      uint8_t* next, const uint8_t* idx_cur, uint8_t* idx_next,
+                                                      int32_t* changed) {
     __shared__ uint32_t s[2][DH * DH];
     __shared__ int active;
+    __shared__ uint8_t nidx[9];   // buffer index of the 3x3 tiles around this one
     const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int tx = tile % tiles_x, ty = tile / tiles_x;
+    if (threadIdx.x < 9) {
+        const int nx = tx + (int)threadIdx.x % 3 - 1, ny = ty + (int)threadIdx.x / 3 - 1;
+        const bool in = nx >= 0 && nx < tiles_x && ny >= 0 && ny < tiles_y;
+        nidx[threadIdx.x] = in ? idx_cur[ny * tiles_x + nx] : 0;
+    }
     if (threadIdx.x == 0) {
         int a = 0;
         for (int dy = -1; dy <= 1; ++dy)
@@ -95,19 +109,23 @@ __global__ __launch_bounds__(256) void dt_tile_kernel(const uint32_t* src, uint3
         active = a;
     }
     __syncthreads();
+    const uint8_t own = nidx[4];
     if (!active) {
-        // carry the tile into the other buffer unchanged
-        for (int i = threadIdx.x; i < DT * DT; i += 256) {
-            const int gx = tx * DT + (i & (DT - 1)), gy = ty * DT + i / DT;
-            if (gx < cols && gy < rows) dst[(int64_t)gy * cols + gx] = src[(int64_t)gy * cols + gx];
+        // the labels stay where they are
+        if (threadIdx.x == 0) {
+            next[tile] = 0;
+            idx_next[tile] = own;
         }
-        if (threadIdx.x == 0) next[tile] = 0;
         return;
     }
     const int x0 = tx * DT - 1, y0 = ty * DT - 1;
-    // halo outside the image reads as the starting point, which never wins
+    // each pixel from the buffer its tile's index names; halo outside the
+    // image reads as the starting point, which never wins
     for (int i = threadIdx.x; i < DH * DH; i += 256) {
-        const int gx = x0 + i % DH, gy = y0 + i / DH;
+        const int hx = i % DH, hy = i / DH;
+        const int gx = x0 + hx, gy = y0 + hy;
+        const int q = (hy == 0 ? 0 : hy == DH - 1 ? 2 : 1) * 3 + (hx == 0 ? 0 : hx == DH - 1 ? 2 : 1);
+        const uint32_t* src = nidx[q] ? buf1 : buf0;
         const uint32_t v = (gx >= 0 && gx < cols && gy >= 0 && gy < rows) ? src[(int64_t)gy * cols + gx] : DT_FAR;
         s[0][i] = v;
         s[1][i] = v;
@@ -170,25 +188,37 @@ __global__ __launch_bounds__(256) void dt_tile_kernel(const uint32_t* src, uint3
         if (!__syncthreads_or(ch)) break;
         any = true;
     }
-    for (int i = threadIdx.x; i < DT * DT; i += 256) {
-        const int px = i & (DT - 1), py = i / DT;
-        const int gx = tx * DT + px, gy = ty * DT + py;
-        if (gx < cols && gy < rows) dst[(int64_t)gy * cols + gx] = s[cur][(py + 1) * DH + px + 1];
+    // a tile that changed writes its other buffer (which no tile reads this
+    // round) and names it for the next round; an unchanged one keeps its own
+    if (any) {
+        uint32_t* dst = own ? buf0 : buf1;
+        for (int i = threadIdx.x; i < DT * DT; i += 256) {
+            const int px = i & (DT - 1), py = i / DT;
+            const int gx = tx * DT + px, gy = ty * DT + py;
+            if (gx < cols && gy < rows) dst[(int64_t)gy * cols + gx] = s[cur][(py + 1) * DH + px + 1];
+        }
     }
     if (threadIdx.x == 0) {
         next[tile] = any ? 1 : 0;
+        idx_next[tile] = any ? (uint8_t)(own ^ 1) : own;
         if (any) *changed = 1;
     }
 }
 
-__global__ void dt_zero_kernel(int32_t* w) {
-    if (threadIdx.x == 0) *w = 0;
+__global__ void dt_zero_kernel(int32_t* w, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) w[i] = 0;
 }
 
-__global__ void dt_out_kernel(const uint32_t* lab, char* out, int64_t po, int rows, int cols, int far_y,
-                              int calc_dist) {
+// Every tile's labels start in buffer 0.
+__global__ void dt_idx_init_kernel(uint8_t* idx, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) idx[i] = 0;
+}
+
+__global__ void dt_out_kernel(const uint32_t* buf0, const uint32_t* buf1, const uint8_t* idx, int tiles_x, char* out,
+                              int64_t po, int rows, int cols, int far_y, int calc_dist) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= cols) return;
+    const uint32_t* lab = idx[(y / DT) * tiles_x + x / DT] ? buf1 : buf0;
     const uint32_t l = lab[(int64_t)y * cols + x];
     char* row = out + (int64_t)y * po;
     if (calc_dist) {
@@ -209,35 +239,50 @@ nsx_status dist_run(nsx_ctx* ctx, const View& mask, const View& out, bool calc_d
     const int64_t npix = (int64_t)rows * cols;
     const int tiles_x = (cols + DT - 1) / DT, tiles_y = (rows + DT - 1) / DT;
     const int ntiles = tiles_x * tiles_y;
+    // change words of a batch of rounds, at most kBatch rounds per host check
+    constexpr int kBatch = 16;
     void *lab_p, *lab2_p, *flags_p, *misc_p;
     NSX_TRY(ctx->ws(WS_CCL_LABEL, sizeof(uint32_t) * (size_t)npix, &lab_p));
     NSX_TRY(ctx->ws(WS_CCL_AREA, sizeof(uint32_t) * (size_t)npix, &lab2_p));
-    NSX_TRY(ctx->ws(WS_CCL_BITS, 2 * (size_t)ntiles, &flags_p));
-    NSX_TRY(ctx->ws(WS_CCL_MISC, 64, &misc_p));
-    uint32_t* lab = (uint32_t*)lab_p;
-    uint32_t* lab2 = (uint32_t*)lab2_p;
-    uint8_t* fa = (uint8_t*)flags_p;
-    uint8_t* fb = fa + ntiles;
+    NSX_TRY(ctx->ws(WS_CCL_BITS, 4 * (size_t)ntiles, &flags_p));
+    NSX_TRY(ctx->ws(WS_CCL_MISC, sizeof(int32_t) * kBatch, &misc_p));
+    uint32_t* buf0 = (uint32_t*)lab_p;
+    uint32_t* buf1 = (uint32_t*)lab2_p;
+    // per tile: active flags and buffer indices, each for two round parities
+    uint8_t* fl[2] = {(uint8_t*)flags_p, (uint8_t*)flags_p + ntiles};
+    uint8_t* ix[2] = {(uint8_t*)flags_p + 2 * (size_t)ntiles, (uint8_t*)flags_p + 3 * (size_t)ntiles};
     int32_t* changed = (int32_t*)misc_p;
     const dim3 grid((cols + 255) / 256, rows);
-    hipLaunchKernelGGL(dt_init_kernel, grid, dim3(256), 0, s, mask.data, mask.pitch, lab, rows, cols, fa, ntiles);
+    hipLaunchKernelGGL(dt_init_kernel, grid, dim3(256), 0, s, mask.data, mask.pitch, buf0, rows, cols, fl[0], ntiles);
+    hipLaunchKernelGGL(dt_idx_init_kernel, dim3(std::min(1024, (ntiles + 255) / 256)), dim3(256), 0, s, ix[0],
+                       ntiles);
     NSX_HIP_TRY(ctx, hipGetLastError());
-    // rounds read one label buffer and write the other; each round that
-    // changes something moves a label strictly closer, so the rounds end (the
-    // bound only guards the loop)
-    for (int64_t round = 0; round <= npix; ++round) {
-        hipLaunchKernelGGL(dt_zero_kernel, dim3(1), dim3(64), 0, s, changed);
-        hipLaunchKernelGGL(dt_tile_kernel, dim3(ntiles), dim3(256), 0, s, (const uint32_t*)lab, lab2, rows, cols,
-                           tiles_x, tiles_y, 3 * rows, (const uint8_t*)fa, fb, changed);
+    // Each round that changes something moves a label strictly closer, so the
+    // rounds end (the bound only guards the loop).  Rounds after the one that
+    // changed nothing find no active tile and change nothing either, so a
+    // batch may overshoot; batches grow 2, 4, 8, 16.
+    int64_t round = 0;
+    int batch = 2;
+    for (;;) {
+        hipLaunchKernelGGL(dt_zero_kernel, dim3(1), dim3(64), 0, s, changed, batch);
+        for (int k = 0; k < batch; ++k, ++round) {
+            const int p = (int)(round & 1);
+            hipLaunchKernelGGL(dt_tile_kernel, dim3(ntiles), dim3(256), 0, s, buf0, buf1, rows, cols, tiles_x,
+                               tiles_y, 3 * rows, (const uint8_t*)fl[p], fl[p ^ 1], (const uint8_t*)ix[p], ix[p ^ 1],
+                               changed + k);
+        }
         NSX_HIP_TRY(ctx, hipGetLastError());
-        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, changed, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, changed + batch - 1, sizeof(int32_t),
+                                        hipMemcpyDeviceToHost, s));
         NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
-        std::swap(lab, lab2);
         if (ctx->h_pinned[40] == 0) break;
-        std::swap(fa, fb);
+        if (round > npix) return ctx->fail(NSX_ERR_NOT_CONVERGED, "distanceTransform did not converge");
+        batch = std::min(2 * batch, kBatch);
     }
-    hipLaunchKernelGGL(dt_out_kernel, grid, dim3(256), 0, s, (const uint32_t*)lab, out.data, out.pitch, rows, cols,
-                       3 * rows, calc_dist ? 1 : 0);
+    // after `round` rounds the indices of parity round & 1 are current
+    hipLaunchKernelGGL(dt_out_kernel, grid, dim3(256), 0, s, (const uint32_t*)buf0, (const uint32_t*)buf1,
+                       (const uint8_t*)ix[round & 1], tiles_x, out.data, out.pitch, rows, cols, 3 * rows,
+                       calc_dist ? 1 : 0);
     NSX_HIP_TRY(ctx, hipGetLastError());
     return NSX_OK;
 }
