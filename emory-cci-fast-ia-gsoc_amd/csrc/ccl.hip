@@ -233,17 +233,31 @@ __device__ __forceinline__ void local_label(const CclArgs& A, const CclImg& D, i
         const int gy = y0 + band_row(w, j);
         v[j] = (gx < D.cols && gy < D.rows) ? pix<InT, THR>(A, D, gy, gx) : 0u;
     }
-    // A pixel's initial parent is the first pixel of its horizontal run of
-    // equal values, found with a ballot of run starts, so the unions below only
-    // join runs of adjacent rows.
+    const uint64_t upto = lx == 63 ? ~0ull : ((2ull << lx) - 1ull);   // lanes 0..lx
+    if constexpr (THR) {
+        // 0/1 values: each row is one 64-bit foreground mask (bit = lane), and
+        // the run starts and the links between rows below are bit operations
+        // on those wave-uniform masks instead of per-lane neighbour moves
 #pragma unroll
-    for (int j = 0; j < CPT; ++j) {
-        const int i = band_row(w, j) * CW + lx;
-        const bool start = v[j] != 0u && from_left(v[j]) != v[j];
-        const uint64_t starts = __ballot(start);
-        const uint64_t upto = lx == 63 ? ~0ull : ((2ull << lx) - 1ull);
-        const int rs = 63 - __clzll(starts & upto);
-        par[i] = v[j] ? (uint32_t)(band_row(w, j) * CW + rs) : BGL;
+        for (int j = 0; j < CPT; ++j) {
+            const int i = band_row(w, j) * CW + lx;
+            const uint64_t M = __ballot(v[j] != 0u);
+            const uint64_t S = M & ~(M << 1);   // first pixel of each run
+            const int rs = 63 - __clzll(S & upto);
+            par[i] = v[j] ? (uint32_t)(band_row(w, j) * CW + rs) : BGL;
+        }
+    } else {
+        // A pixel's initial parent is the first pixel of its horizontal run of
+        // equal values, found with a ballot of run starts, so the unions below
+        // only join runs of adjacent rows.
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int i = band_row(w, j) * CW + lx;
+            const bool start = v[j] != 0u && from_left(v[j]) != v[j];
+            const uint64_t starts = __ballot(start);
+            const int rs = 63 - __clzll(starts & upto);
+            par[i] = v[j] ? (uint32_t)(band_row(w, j) * CW + rs) : BGL;
+        }
     }
     edge[w * CW + lx] = v[CPT - 1];
     __syncthreads();
@@ -251,20 +265,47 @@ __device__ __forceinline__ void local_label(const CclArgs& A, const CclImg& D, i
     // touch: a vertical (or diagonal) link that a left neighbour in the same run
     // already makes through the same run above is skipped.  The row above is the
     // previous register, or for a band's first row the band above's last row.
+    if constexpr (THR) {
+        // the same links as below, for 0/1 values: with M the row's mask and U
+        // the row above's (bit b = lane b; << 1 moves a lane's left neighbour
+        // onto it, >> 1 its right one)
+        //   up:         M & U & ~(left & up-left)
+        //   up-left:    M & (U << 1) & ~U & ~(M << 1)          (8-conn)
+        //   up-right:   M & (U >> 1) & ~U & ~(M >> 1)          (8-conn)
+        uint64_t U = w > 0 ? __ballot(edge[(w - 1) * CW + lx] != 0u) : 0ull;
 #pragma unroll
-    for (int j = 0; j < CPT; ++j) {
-        const int ly = band_row(w, j);
-        const int i = ly * CW + lx;
-        const uint32_t vv = v[j];
-        const uint32_t vu = j > 0 ? v[j - 1] : (w > 0 ? edge[(w - 1) * CW + lx] : 0u);
-        const uint32_t vl = from_left(vv), vr = from_right(vv);
-        const uint32_t vul = from_left(vu), vur = from_right(vu);
-        if (!vv || ly == 0) continue;
-        const int u = i - CW;
-        if (vu == vv && !(vl == vv && vul == vv)) lds_union(par, i, u);
-        if (CONN == 8) {
-            if (vul == vv && vu != vv && vl != vv) lds_union(par, i, u - 1);
-            if (vur == vv && vu != vv && vr != vv) lds_union(par, i, u + 1);
+        for (int j = 0; j < CPT; ++j) {
+            const int ly = band_row(w, j);
+            const uint64_t M = __ballot(v[j] != 0u);
+            const uint64_t up = M & U & ~((M << 1) & (U << 1));
+            const uint64_t ul = CONN == 8 ? (M & (U << 1) & ~U & ~(M << 1)) : 0ull;
+            const uint64_t ur = CONN == 8 ? (M & (U >> 1) & ~U & ~(M >> 1)) : 0ull;
+            U = M;
+            if (ly == 0 || !(up | ul | ur)) continue;   // wave-uniform
+            const int i = ly * CW + lx;
+            const int u = i - CW;
+            if ((up >> lx) & 1ull) lds_union(par, i, u);
+            if (CONN == 8) {
+                if ((ul >> lx) & 1ull) lds_union(par, i, u - 1);
+                if ((ur >> lx) & 1ull) lds_union(par, i, u + 1);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int ly = band_row(w, j);
+            const int i = ly * CW + lx;
+            const uint32_t vv = v[j];
+            const uint32_t vu = j > 0 ? v[j - 1] : (w > 0 ? edge[(w - 1) * CW + lx] : 0u);
+            const uint32_t vl = from_left(vv), vr = from_right(vv);
+            const uint32_t vul = from_left(vu), vur = from_right(vu);
+            if (!vv || ly == 0) continue;
+            const int u = i - CW;
+            if (vu == vv && !(vl == vv && vul == vv)) lds_union(par, i, u);
+            if (CONN == 8) {
+                if (vul == vv && vu != vv && vl != vv) lds_union(par, i, u - 1);
+                if (vur == vv && vu != vv && vr != vv) lds_union(par, i, u + 1);
+            }
         }
     }
     __syncthreads();
@@ -323,6 +364,10 @@ __global__ __launch_bounds__(CTHREADS) void ccl_local_kernel(CclArgs A) {
             // a weight that does not exceed the current maximum (0 included)
             // changes nothing: most pixels of a component skip the atomic
             if (par[i] != BGL && wv[j] > cnt[par[i]]) atomicMax(&cnt[par[i]], wv[j]);
+        } else if constexpr (THR) {
+            const uint64_t M = __ballot(v[j] != 0u);
+            const uint64_t ends = M & ~(M >> 1);   // last pixel of each run
+            if (((M & ~(M << 1)) >> lx) & 1ull) atomicAdd(&cnt[par[i]], (uint32_t)__builtin_ctzll(ends >> lx) + 1u);
         } else {
             const bool start = v[j] != 0u && from_left(v[j]) != v[j];
             const bool end = v[j] != 0u && from_right(v[j]) != v[j];
