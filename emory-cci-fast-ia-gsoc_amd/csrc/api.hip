@@ -46,6 +46,7 @@ nsx_status nsx_ctx::ws(int slot, size_t bytes, void** out) {
     }
     b.ptr = p;
     b.bytes = want;
+    b.serial = ++ws_serial;
     held += want;
     *out = p;
     return NSX_OK;
@@ -299,6 +300,11 @@ nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value) {
         ctx->drain = value;
         return NSX_OK;
     }
+    if (param == NSX_TUNE_DT_SOLVE) {
+        if (value < 0 || value > 8) return ctx->fail(NSX_ERR_INVALID, "distanceTransform solve passes must be in 0..8");
+        ctx->dt_solve = value;
+        return NSX_OK;
+    }
     if (param == NSX_TUNE_ENGINE) {
         if (value != NSX_ENGINE_ROUNDS && value != NSX_ENGINE_TREE && value != NSX_ENGINE_AUTO &&
             value != NSX_ENGINE_PLANES)
@@ -332,12 +338,22 @@ nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value) {
         *value = ctx->drain;
         return NSX_OK;
     }
+    if (param == NSX_TUNE_DT_SOLVE) {
+        *value = ctx->dt_solve;
+        return NSX_OK;
+    }
     return NSX_ERR_INVALID;
 }
 
 nsx_status nsx_ctx_drain_failures(const nsx_ctx* ctx, int64_t* n) {
     if (!ctx || !n) return NSX_ERR_INVALID;
     *n = ctx->drain_failures;
+    return NSX_OK;
+}
+
+nsx_status nsx_ctx_dt_resumes(const nsx_ctx* ctx, int64_t* n) {
+    if (!ctx || !n) return NSX_ERR_INVALID;
+    *n = ctx->dtq_resumes;
     return NSX_OK;
 }
 
@@ -349,6 +365,12 @@ nsx_status nsx_ctx_set_option(nsx_ctx* ctx, int option, int value) {
         ctx->i32_invert = value;
         return NSX_OK;
     }
+    if (option == NSX_OPT_DT_ORDER) {
+        if (value != NSX_DT_ORDER_QUEUE && value != NSX_DT_ORDER_TILES)
+            return ctx->fail(NSX_ERR_INVALID, "NSX_OPT_DT_ORDER takes NSX_DT_ORDER_QUEUE or NSX_DT_ORDER_TILES");
+        ctx->dt_order = value;
+        return NSX_OK;
+    }
     return ctx->fail(NSX_ERR_INVALID, "unknown option");
 }
 
@@ -356,6 +378,10 @@ nsx_status nsx_ctx_get_option(const nsx_ctx* ctx, int option, int* value) {
     if (!ctx || !value) return NSX_ERR_INVALID;
     if (option == NSX_OPT_I32_INVERT) {
         *value = ctx->i32_invert;
+        return NSX_OK;
+    }
+    if (option == NSX_OPT_DT_ORDER) {
+        *value = ctx->dt_order;
         return NSX_OK;
     }
     return NSX_ERR_INVALID;

@@ -218,7 +218,7 @@ __global__ void dt_out_kernel(const uint32_t* buf0, const uint32_t* buf1, const 
                               int64_t po, int rows, int cols, int far_y, int calc_dist) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= cols) return;
-    const uint32_t* lab = idx[(y / DT) * tiles_x + x / DT] ? buf1 : buf0;
+    const uint32_t* lab = (idx && idx[(y / DT) * tiles_x + x / DT]) ? buf1 : buf0;
     const uint32_t l = lab[(int64_t)y * cols + x];
     char* row = out + (int64_t)y * po;
     if (calc_dist) {
@@ -229,6 +229,514 @@ __global__ void dt_out_kernel(const uint32_t* buf0, const uint32_t* buf1, const 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Queue order (the default).
+//
+// The reference pops its FIFO queue in generations: generation 0 is the
+// initial front in raster order (:205-222), generation g+1 what generation g
+// pushed, in push order: entry by entry, each entry's neighbours in the order
+// up-left, up-right, up, down-left, down-right, down, left, right (:122-193).
+// Within a generation the sequential run is fixed by two facts:
+//  * a pixel's label after a sequence of offers is the earliest offer with the
+//    smallest float distance (every offer taken is strictly closer than the
+//    label it replaces), the generation-start label counting as the first;
+//  * the label entry k offers, c_k, is its pixel's label when it is popped:
+//    by the first fact, from the pixel's generation-start label and the labels
+//    c_j that earlier entries j < k of the generation at its 8 neighbours
+//    offered it.
+// So the c_k solve a triangular system (c_k depends on c_j, j < k), whose one
+// solution is reached by relaxation passes: a pass in which nothing changes
+// has found it.  Entry k pushes neighbour n when c_k is strictly closer to n
+// than n's generation-start label and than every offer of an entry j < k at a
+// neighbour of n; after the generation n holds the offer of the last entry that
+// pushed it.  Each generation is then: link (the entries at each pixel, in a
+// tagged list), solve passes, push masks, an exclusive scan of their counts,
+// and the scatter of the next generation in push order.
+
+constexpr int QCH = 1024;   // entries per chunk: 256 threads x 4
+constexpr int QNB = 1024;   // blocks of the per-generation kernels, looping over chunks
+constexpr int QMAXSOLVE = 16;
+// control words
+enum { QC_LEN0 = 0, QC_LEN1 = 1, QC_STALL = 2, QC_ERR = 3, QC_CHG = 8 };   // QC_CHG + solve pass
+
+struct Dtq {
+    uint32_t* S;       // label of every pixel
+    uint64_t* head;    // per pixel: (tag << 32) | newest entry of the generation at the pixel
+    uint64_t* win;     // per pixel: (tag << 32) | last entry that pushed it
+    uint32_t* E[2];    // entry positions (y << 16 | x), by generation parity
+    uint32_t* c[2];    // entry labels, by generation parity
+    uint64_t* nxt;     // per entry: the head it replaced
+    uint8_t* mask;     // per entry: the neighbours it pushes (bit o = offer o)
+    uint32_t* part;    // per chunk: count, then exclusive offset
+    int32_t* ctl;
+    int rows, cols, far_y;
+    uint32_t tag0;     // generation g is tagged tag0 + g
+    uint32_t cap;      // entries a generation buffer holds
+};
+
+__constant__ int QDX[8] = {-1, 1, 0, -1, 1, 0, -1, 1};
+__constant__ int QDY[8] = {-1, -1, -1, 1, 1, 1, 0, 0};
+
+__device__ __forceinline__ int64_t qlin(const Dtq& Q, int x, int y) { return (int64_t)y * Q.cols + x; }
+__device__ __forceinline__ uint32_t qpack(int x, int y) { return ((uint32_t)y << 16) | (uint32_t)x; }
+
+// c_k: the label of entry k's pixel (x, y) when it is popped.  Ties in float
+// distance go to the earlier offer; the generation-start label is earliest.
+template <typename Val>
+__device__ __forceinline__ uint32_t dtq_popped(const Dtq& Q, uint32_t tag, int x, int y, int k, Val val) {
+    uint32_t best = Q.S[qlin(Q, x, y)];
+    uint64_t qb = dt_d2(x, y, best, Q.far_y);
+    if (qb == 0) return best;
+    int jb = -1;
+    for (int o = 0; o < 8; ++o) {
+        const int nx = x + QDX[o], ny = y + QDY[o];
+        if (nx < 0 || ny < 0 || nx >= Q.cols || ny >= Q.rows) continue;
+        uint64_t h = Q.head[qlin(Q, nx, ny)];
+        while ((uint32_t)(h >> 32) == tag) {
+            const int j = (int)(uint32_t)h;
+            if (j < k) {
+                const uint32_t v = val(j);
+                const uint64_t q = dt_d2(x, y, v, Q.far_y);
+                if (dt_closer(q, qb) || (jb >= 0 && j < jb && !dt_closer(qb, q))) {
+                    best = v;
+                    qb = q;
+                    jb = j;
+                }
+            }
+            h = Q.nxt[j];
+        }
+    }
+    return best;
+}
+
+// Whether entry k (label ck) pushes pixel (nx, ny): ck is strictly closer to
+// it than its generation-start label and than every offer of an earlier entry
+// of the generation at one of its neighbours.
+__device__ __forceinline__ bool dtq_pushes(const Dtq& Q, const uint32_t* cv, uint32_t tag, int nx, int ny, int k,
+                                           uint32_t ck) {
+    const uint64_t qk = dt_d2(nx, ny, ck, Q.far_y);
+    if (!dt_closer(qk, dt_d2(nx, ny, Q.S[qlin(Q, nx, ny)], Q.far_y))) return false;
+    for (int o = 0; o < 8; ++o) {
+        const int mx = nx + QDX[o], my = ny + QDY[o];
+        if (mx < 0 || my < 0 || mx >= Q.cols || my >= Q.rows) continue;
+        uint64_t h = Q.head[qlin(Q, mx, my)];
+        while ((uint32_t)(h >> 32) == tag) {
+            const int j = (int)(uint32_t)h;
+            if (j < k && !dt_closer(qk, dt_d2(nx, ny, cv[j], Q.far_y))) return false;
+            h = Q.nxt[j];
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Exclusive prefix of a 256-thread block's four counts in the order
+// (i, thread): the order of entries base + i * 256 + thread.  sh holds 17 words.
+__device__ __forceinline__ uint32_t block_scan4(const uint32_t (&cnt)[4], uint32_t (&ex)[4], uint32_t* sh) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        inc[i] = wave_incl_scan(cnt[i]);
+        if (lane == 63) sh[i * 4 + w] = inc[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t r = 0;
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t t = sh[q];
+            sh[q] = r;
+            r += t;
+        }
+        sh[16] = r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ex[i] = sh[i * 4 + w] + inc[i] - cnt[i];
+    const uint32_t total = sh[16];
+    __syncthreads();
+    return total;
+}
+
+// checkDistNeighbors8 (:33-88): a background pixel with a nonzero 8-neighbour.
+__device__ __forceinline__ bool dtq_front(const char* mask, int64_t pm, int rows, int cols, int x, int y) {
+    if (((const uint8_t*)(mask + (int64_t)y * pm))[x] != 0) return false;
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int yy = y + dy;
+        if (yy < 0 || yy >= rows) continue;
+        const uint8_t* r = (const uint8_t*)(mask + (int64_t)yy * pm);
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int xx = x + dx;
+            if ((dx || dy) && xx >= 0 && xx < cols && r[xx] != 0) return true;
+        }
+    }
+    return false;
+}
+
+// Generation 0, count pass: every label initialised (a background pixel holds
+// itself, the others the far point) and per chunk (1024 columns of a row,
+// chunks in raster order) the number of front pixels.
+__global__ __launch_bounds__(256) void dtq_front_count_kernel(Dtq Q, const char* mask, int64_t pm, int nseg) {
+    const int ci = blockIdx.x, y = ci / nseg, seg = ci - y * nseg;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int x = seg * QCH + i * 256 + (int)threadIdx.x;
+        if (x >= Q.cols) continue;
+        const bool bg = ((const uint8_t*)(mask + (int64_t)y * pm))[x] == 0;
+        Q.S[qlin(Q, x, y)] = bg ? qpack(x, y) : DT_FAR;
+        cnt += dtq_front(mask, pm, Q.rows, Q.cols, x, y) ? 1u : 0u;
+    }
+    __shared__ uint32_t red[4];
+    cnt = wave_sum(cnt);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) Q.part[ci] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Generation 0, scatter pass: the front pixels in raster order.
+__global__ __launch_bounds__(256) void dtq_front_scatter_kernel(Dtq Q, const char* mask, int64_t pm, int nseg) {
+    if (Q.ctl[QC_STALL]) return;
+    const int ci = blockIdx.x, y = ci / nseg, seg = ci - y * nseg;
+    __shared__ uint32_t sh[17];
+    uint32_t cnt[4], ex[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int x = seg * QCH + i * 256 + (int)threadIdx.x;
+        cnt[i] = (x < Q.cols && dtq_front(mask, pm, Q.rows, Q.cols, x, y)) ? 1u : 0u;
+    }
+    block_scan4(cnt, ex, sh);
+    const uint32_t base = Q.part[ci];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (cnt[i]) Q.E[0][base + ex[i]] = qpack(seg * QCH + i * 256 + (int)threadIdx.x, y);
+}
+
+// Exclusive prefix of part[0..n) in place, one block of 1024 threads; n is the
+// chunk count of the generation length at ctl[len_slot] (or nfix when
+// len_slot < 0).  The total, the next generation's length, goes to
+// ctl[out_slot]; above the buffers' capacity it sets the error and stall words.
+__global__ __launch_bounds__(1024) void dtq_scan_kernel(Dtq Q, int len_slot, int nfix, int out_slot) {
+    if (Q.ctl[QC_STALL]) return;
+    const int n = len_slot >= 0 ? (Q.ctl[len_slot] + QCH - 1) / QCH : nfix;
+    const int per = (n + 1023) / 1024;
+    const int b = (int)threadIdx.x * per, e = min(b + per, n);
+    uint64_t s = 0;
+    for (int i = b; i < e; ++i) s += Q.part[i];
+    // block scan of the per-thread sums (16 waves)
+    __shared__ uint64_t wt[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t inc = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t t = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += t;
+    }
+    if (lane == 63) wt[w] = inc;
+    __syncthreads();
+    uint64_t before = 0;
+    for (int q = 0; q < w; ++q) before += wt[q];
+    uint64_t run = before + inc - s;
+    for (int i = b; i < e; ++i) {
+        const uint32_t v = Q.part[i];
+        Q.part[i] = (uint32_t)run;
+        run += v;
+    }
+    if (threadIdx.x == 1023) {
+        if (run > Q.cap) {
+            Q.ctl[QC_ERR] = 1;
+            Q.ctl[QC_STALL] = -1;
+        } else {
+            Q.ctl[out_slot] = (int32_t)run;
+        }
+    }
+}
+
+// Generation g's entries: each takes its pixel's label (for g > 0 the offer of
+// the last entry of generation g - 1 that pushed it, which becomes the pixel's
+// label) and is linked into its pixel's list.
+__global__ __launch_bounds__(256) void dtq_link_kernel(Dtq Q, int g) {
+    if (Q.ctl[QC_STALL]) return;
+    if (blockIdx.x == 0 && threadIdx.x < QMAXSOLVE) Q.ctl[QC_CHG + threadIdx.x] = 0;
+    const int L = Q.ctl[g & 1];
+    const uint32_t tag = Q.tag0 + (uint32_t)g;
+    const uint32_t* E = Q.E[g & 1];
+    uint32_t* cv = Q.c[g & 1];
+    const uint32_t* cprev = Q.c[(g & 1) ^ 1];
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < L; k += gridDim.x * 256) {
+        const uint32_t p = E[k];
+        const int64_t li = qlin(Q, (int)(p & 0xFFFFu), (int)(p >> 16));
+        uint32_t v = p;   // a front pixel is background: it holds itself
+        if (g > 0) {
+            const uint64_t w = Q.win[li];
+            if ((uint32_t)(w >> 32) != tag - 1u) {
+                Q.ctl[QC_ERR] = 2;   // not pushed by the previous generation: an internal error
+                Q.ctl[QC_STALL] = -1;
+                return;
+            }
+            v = cprev[(uint32_t)w];
+            Q.S[li] = v;
+        }
+        cv[k] = v;
+        Q.nxt[k] = atomicExch((unsigned long long*)&Q.head[li], ((unsigned long long)tag << 32) | (uint32_t)k);
+    }
+}
+
+// One relaxation pass of generation g's labels, in place.  A chunk's labels
+// are relaxed in LDS until they stop changing (the offers of other chunks read
+// as they are); pass `pass` runs only if pass - 1 changed something.
+__global__ __launch_bounds__(256) void dtq_solve_kernel(Dtq Q, int g, int pass) {
+    if (Q.ctl[QC_STALL]) return;
+    if (pass > 0 && Q.ctl[QC_CHG + pass - 1] == 0) return;
+    const int L = Q.ctl[g & 1];
+    const uint32_t tag = Q.tag0 + (uint32_t)g;
+    const uint32_t* E = Q.E[g & 1];
+    uint32_t* cv = Q.c[g & 1];
+    __shared__ uint32_t lc[QCH];
+    for (int ci = blockIdx.x; ci * QCH < L; ci += gridDim.x) {
+        const int base = ci * QCH;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = base + i * 256 + (int)threadIdx.x;
+            lc[i * 256 + threadIdx.x] = k < L ? cv[k] : 0u;
+        }
+        __syncthreads();
+        bool any = false;
+        auto val = [&](int j) { return (j >= base && j < base + QCH) ? lc[j - base] : cv[j]; };
+        for (int sweep = 0; sweep < 64; ++sweep) {
+            bool ch = false;
+#pragma unroll 1
+            for (int i = 0; i < 4; ++i) {
+                const int k = base + i * 256 + (int)threadIdx.x;
+                if (k >= L) break;
+                const uint32_t p = E[k];
+                const uint32_t v = dtq_popped(Q, tag, (int)(p & 0xFFFFu), (int)(p >> 16), k, val);
+                if (v != lc[k - base]) {
+                    lc[k - base] = v;
+                    ch = true;
+                }
+            }
+            if (!__syncthreads_or(ch)) break;
+            any = true;
+        }
+        if (any) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int k = base + i * 256 + (int)threadIdx.x;
+                if (k < L) cv[k] = lc[k - base];
+            }
+            if (threadIdx.x == 0) Q.ctl[QC_CHG + pass] = 1;
+        }
+        __syncthreads();
+    }
+}
+
+// Generation g's push masks and per-chunk push counts.  For g > 0 each entry
+// first checks that its label solves its equation; if one does not, the
+// solve passes were too few: the stall word names the generation and every
+// later kernel returns at once, so the host can run more passes and go on.
+__global__ __launch_bounds__(256) void dtq_push_kernel(Dtq Q, int g) {
+    if (Q.ctl[QC_STALL]) return;
+    const int L = Q.ctl[g & 1];
+    const uint32_t tag = Q.tag0 + (uint32_t)g;
+    const uint32_t* E = Q.E[g & 1];
+    const uint32_t* cv = Q.c[g & 1];
+    __shared__ uint32_t red[4];
+    for (int ci = blockIdx.x; ci * QCH < L; ci += gridDim.x) {
+        const int base = ci * QCH;
+        uint32_t cnt = 0;
+#pragma unroll 1
+        for (int i = 0; i < 4; ++i) {
+            const int k = base + i * 256 + (int)threadIdx.x;
+            if (k >= L) break;
+            const uint32_t p = E[k];
+            const int x = (int)(p & 0xFFFFu), y = (int)(p >> 16);
+            const uint32_t ck = cv[k];
+            if (g > 0 && dtq_popped(Q, tag, x, y, k, [&](int j) { return cv[j]; }) != ck) Q.ctl[QC_STALL] = g + 1;
+            uint32_t m = 0;
+            for (int o = 0; o < 8; ++o) {
+                const int nx = x + QDX[o], ny = y + QDY[o];
+                if (nx < 0 || ny < 0 || nx >= Q.cols || ny >= Q.rows) continue;
+                if (dtq_pushes(Q, cv, tag, nx, ny, k, ck)) m |= 1u << o;
+            }
+            Q.mask[k] = (uint8_t)m;
+            cnt += (uint32_t)__builtin_popcount(m);
+        }
+        cnt = wave_sum(cnt);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) Q.part[ci] = red[0] + red[1] + red[2] + red[3];
+        __syncthreads();
+    }
+}
+
+// Generation g + 1 in push order, and each pushed pixel's last pusher.
+__global__ __launch_bounds__(256) void dtq_scatter_kernel(Dtq Q, int g) {
+    if (Q.ctl[QC_STALL]) return;
+    const int L = Q.ctl[g & 1];
+    const uint64_t tagw = (uint64_t)(Q.tag0 + (uint32_t)g) << 32;
+    const uint32_t* E = Q.E[g & 1];
+    uint32_t* En = Q.E[(g & 1) ^ 1];
+    __shared__ uint32_t sh[17];
+    for (int ci = blockIdx.x; ci * QCH < L; ci += gridDim.x) {
+        const int base = ci * QCH;
+        uint32_t m[4], cnt[4], ex[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = base + i * 256 + (int)threadIdx.x;
+            m[i] = k < L ? Q.mask[k] : 0u;
+            cnt[i] = (uint32_t)__builtin_popcount(m[i]);
+        }
+        block_scan4(cnt, ex, sh);
+        const uint32_t off = Q.part[ci];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!m[i]) continue;
+            const int k = base + i * 256 + (int)threadIdx.x;
+            const uint32_t p = E[k];
+            const int x = (int)(p & 0xFFFFu), y = (int)(p >> 16);
+            uint32_t pos = off + ex[i];
+            for (int o = 0; o < 8; ++o) {
+                if (!((m[i] >> o) & 1u)) continue;
+                const int nx = x + QDX[o], ny = y + QDY[o];
+                En[pos++] = qpack(nx, ny);
+                atomicMax((unsigned long long*)&Q.win[qlin(Q, nx, ny)], (unsigned long long)(tagw | (uint32_t)k));
+            }
+        }
+    }
+}
+
+// Queue order on the host: generation 0, then batches of generations with one
+// host check each (the next length, the stall and error words).
+nsx_status dist_queue_run(nsx_ctx* ctx, const View& mask, const View& out, bool calc_dist, hipStream_t s) {
+    const int rows = mask.rows, cols = mask.cols;
+    const int64_t npix = (int64_t)rows * cols;
+    const int nseg = (cols + QCH - 1) / QCH;
+    const int64_t nfront = (int64_t)rows * nseg;
+    // Solve passes launched per generation, and per resumed generation.
+    const int kSolve = std::min(std::max(ctx->dt_solve, 0), 8);
+    constexpr int kResumeSolve = 8;
+    static_assert(kResumeSolve <= QMAXSOLVE, "change words");
+    // Entries a generation buffer holds; a longer generation (a dense front)
+    // starts the call over with twice the room.
+    int64_t cap = npix / 4 + 4096;
+    for (;;) {
+        const int64_t nchunk = std::max<int64_t>(nfront, (cap + QCH - 1) / QCH);
+        void *lab_p, *tags_p, *ent_p, *part_p, *misc_p;
+        NSX_TRY(ctx->ws(WS_CCL_LABEL, sizeof(uint32_t) * (size_t)npix, &lab_p));
+        NSX_TRY(ctx->ws(WS_DT_TAGS, 2 * sizeof(uint64_t) * (size_t)npix, &tags_p));
+        NSX_TRY(ctx->ws(WS_DT_ENT, (size_t)cap * (4 * sizeof(uint32_t) + sizeof(uint64_t) + 1), &ent_p));
+        NSX_TRY(ctx->ws(WS_DT_PART, sizeof(uint32_t) * (size_t)nchunk, &part_p));
+        NSX_TRY(ctx->ws(WS_CCL_MISC, sizeof(int32_t) * 64, &misc_p));
+        // Tags only ever grow, so list heads and pushers left by earlier calls
+        // never match; a fresh allocation (or a tag near wrapping) is cleared.
+        const nsx::DevBuf& tb = ctx->bufs[WS_DT_TAGS];
+        if (tb.serial != ctx->dtq_tags_serial || ctx->dtq_tag == 0 || ctx->dtq_tag > 0xF0000000u) {
+            NSX_HIP_TRY(ctx, hipMemsetAsync(tb.ptr, 0, tb.bytes, s));
+            ctx->dtq_tags_serial = tb.serial;
+            ctx->dtq_tag = 1;
+        }
+        Dtq Q;
+        Q.S = (uint32_t*)lab_p;
+        Q.head = (uint64_t*)tags_p;
+        Q.win = Q.head + npix;
+        Q.nxt = (uint64_t*)ent_p;
+        Q.E[0] = (uint32_t*)(Q.nxt + cap);
+        Q.E[1] = Q.E[0] + cap;
+        Q.c[0] = Q.E[1] + cap;
+        Q.c[1] = Q.c[0] + cap;
+        Q.mask = (uint8_t*)(Q.c[1] + cap);
+        Q.part = (uint32_t*)part_p;
+        Q.ctl = (int32_t*)misc_p;
+        Q.rows = rows;
+        Q.cols = cols;
+        Q.far_y = 3 * rows;
+        Q.tag0 = ctx->dtq_tag;
+        Q.cap = (uint32_t)cap;
+        NSX_HIP_TRY(ctx, hipMemsetAsync(Q.ctl, 0, sizeof(int32_t) * 64, s));
+        hipLaunchKernelGGL(dtq_front_count_kernel, dim3((unsigned)nfront), dim3(256), 0, s, Q, mask.data, mask.pitch,
+                           nseg);
+        hipLaunchKernelGGL(dtq_scan_kernel, dim3(1), dim3(1024), 0, s, Q, -1, (int)nfront, (int)QC_LEN0);
+        hipLaunchKernelGGL(dtq_front_scatter_kernel, dim3((unsigned)nfront), dim3(256), 0, s, Q, mask.data,
+                           mask.pitch, nseg);
+        NSX_HIP_TRY(ctx, hipGetLastError());
+        auto generation = [&](int g, int solve, bool resumed) {
+            if (!resumed) hipLaunchKernelGGL(dtq_link_kernel, dim3(QNB), dim3(256), 0, s, Q, g);
+            if (g > 0)
+                for (int k = 0; k < solve; ++k) hipLaunchKernelGGL(dtq_solve_kernel, dim3(QNB), dim3(256), 0, s, Q, g, k);
+            hipLaunchKernelGGL(dtq_push_kernel, dim3(QNB), dim3(256), 0, s, Q, g);
+            hipLaunchKernelGGL(dtq_scan_kernel, dim3(1), dim3(1024), 0, s, Q, g & 1, 0, (g & 1) ^ 1);
+            hipLaunchKernelGGL(dtq_scatter_kernel, dim3(QNB), dim3(256), 0, s, Q, g);
+        };
+        // Every generation moves some label strictly closer, so the number of
+        // generations is bounded; the tag space bounds the loop here.
+        int g = 0, batch = 4;
+        bool grow = false;
+        for (;;) {
+            for (int b = 0; b < batch; ++b, ++g) generation(g, kSolve, false);
+            NSX_HIP_TRY(ctx, hipGetLastError());
+            NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, Q.ctl, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
+            NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+            int32_t len = ctx->h_pinned[40 + (g & 1)], stall = ctx->h_pinned[42], err = ctx->h_pinned[43];
+            while (stall > 0 && err == 0) {
+                // generation stall - 1 needs more solve passes; the kernels
+                // after it returned at once, so it resumes where it stopped
+                ++ctx->dtq_resumes;
+                g = stall - 1;
+                NSX_HIP_TRY(ctx, hipMemsetAsync(Q.ctl + QC_STALL, 0, sizeof(int32_t), s));
+                NSX_HIP_TRY(ctx, hipMemsetAsync(Q.ctl + QC_CHG, 0, sizeof(int32_t) * QMAXSOLVE, s));
+                generation(g, kResumeSolve, true);
+                ++g;
+                NSX_HIP_TRY(ctx, hipGetLastError());
+                NSX_HIP_TRY(ctx,
+                            hipMemcpyAsync(ctx->h_pinned + 40, Q.ctl, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
+                NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
+                len = ctx->h_pinned[40 + (g & 1)];
+                stall = ctx->h_pinned[42];
+                err = ctx->h_pinned[43];
+            }
+            if (err == 1) {
+                grow = true;
+                break;
+            }
+            if (err != 0) return ctx->fail(NSX_ERR_NOT_CONVERGED, "distanceTransform: queue generations inconsistent");
+            if (len == 0) break;
+            if ((uint64_t)ctx->dtq_tag + (uint64_t)g + 64 > 0xFFFFFFF0ull)
+                return ctx->fail(NSX_ERR_NOT_CONVERGED, "distanceTransform: generation tags exhausted");
+            batch = std::min(2 * batch, 32);
+        }
+        ctx->dtq_tag += (uint32_t)g + 1;
+        if (grow) {
+            // a generation longer than the entry buffers: start over with room
+            // for twice as many entries
+            if (cap > (int64_t)1 << 30)
+                return ctx->fail(NSX_ERR_NOMEM, "distanceTransform: queue generation beyond 2^30 entries");
+            cap *= 2;
+            continue;
+        }
+        const dim3 grid((cols + 255) / 256, rows);
+        hipLaunchKernelGGL(dt_out_kernel, grid, dim3(256), 0, s, (const uint32_t*)Q.S, (const uint32_t*)Q.S,
+                           (const uint8_t*)nullptr, 1, out.data, out.pitch, rows, cols, 3 * rows, calc_dist ? 1 : 0);
+        NSX_HIP_TRY(ctx, hipGetLastError());
+        return NSX_OK;
+    }
+}
+
 }  // namespace
 
 nsx_status dist_run(nsx_ctx* ctx, const View& mask, const View& out, bool calc_dist, hipStream_t s) {
@@ -236,6 +744,7 @@ nsx_status dist_run(nsx_ctx* ctx, const View& mask, const View& out, bool calc_d
     if (rows == 0 || cols == 0) return NSX_OK;
     if (rows >= 65535 || cols >= 65535 || (int64_t)rows * cols * 3 > INT32_MAX)
         return ctx->fail(NSX_ERR_INVALID, "distanceTransform: rows, cols < 65535 and rows*cols*3 within int32");
+    if (ctx->dt_order == NSX_DT_ORDER_QUEUE) return dist_queue_run(ctx, mask, out, calc_dist, s);
     const int64_t npix = (int64_t)rows * cols;
     const int tiles_x = (cols + DT - 1) / DT, tiles_y = (rows + DT - 1) / DT;
     const int ntiles = tiles_x * tiles_y;

@@ -41,6 +41,7 @@ struct Error {
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    uint64_t serial = 0;   // changes whenever the slot is (re)allocated
 };
 
 }  // namespace nsx
@@ -87,10 +88,20 @@ struct nsx_ctx {
     int engine = NSX_ENGINE_AUTO;     // NSX_TUNE_ENGINE
     int fill_scans = 0;               // NSX_TUNE_FILL_SCANS
     int drain = 0;                    // NSX_TUNE_DRAIN
+    int dt_solve = 1;                 // NSX_TUNE_DT_SOLVE
     int64_t drain_failures = 0;       // drains that gave up (restarted with per-round launches)
 
     // Options (nsx_ctx_set_option).
     int i32_invert = NSX_I32_INVERT_WRAP;
+    int dt_order = NSX_DT_ORDER_QUEUE;
+
+    // distanceTransform in queue order: the next generation tag, the
+    // allocation of WS_DT_TAGS the tags were last cleared for, and how many
+    // generations needed more solve passes than launched (a host round trip).
+    uint32_t dtq_tag = 0;
+    uint64_t dtq_tags_serial = 0;
+    int64_t dtq_resumes = 0;
+    uint64_t ws_serial = 0;
 
     // Propagation rounds the last reconstruction of a given size/mode needed.
     nsx::RoundHints round_hint;
@@ -167,6 +178,9 @@ enum WsSlot {
     WS_CCL2_SCAN,
     WS_CCL2_MISC,
     WS_CCL2_LROOT,
+    WS_DT_TAGS,          // distanceTransform queue order: per pixel two tagged words (list head, last offer)
+    WS_DT_ENT,           //   per queue entry: positions and labels (two generations), list links, push masks
+    WS_DT_PART,          //   per chunk of entries: counts, then offsets
     WS_NSLOTS
 };
 

@@ -28,9 +28,13 @@ from ._lib import NsxImg
 from ._lib import NSX_OPT_I32_INVERT as OPT_I32_INVERT
 from ._lib import NSX_I32_INVERT_WRAP as I32_INVERT_WRAP
 from ._lib import NSX_I32_INVERT_SATURATE as I32_INVERT_SATURATE
+from ._lib import NSX_OPT_DT_ORDER as OPT_DT_ORDER
+from ._lib import NSX_DT_ORDER_QUEUE as DT_ORDER_QUEUE
+from ._lib import NSX_DT_ORDER_TILES as DT_ORDER_TILES
 
 __all__ = [
     "NsxError", "Context", "default_context", "OPT_I32_INVERT", "I32_INVERT_WRAP", "I32_INVERT_SATURATE",
+    "OPT_DT_ORDER", "DT_ORDER_QUEUE", "DT_ORDER_TILES",
     "imreconstruct", "imreconstructBinary", "imfillHoles", "fillHoles", "imfill", "bwselect", "imhmin", "bounding_box", "localMaxima", "localMinima", "distanceTransform",
     "bwlabel2", "bwlabel", "bwareaopen2", "relabel", "threshold_gt", "segment_chain",
     "gen_disks_u8", "gen_marker_u8", "gen_maze_u8", "scale_i32", "scale_f32",
@@ -177,6 +181,13 @@ class Context:
         """Persistent drains (TUNE_DRAIN) that gave up and restarted with per-round launches."""
         v = C.c_int64(0)
         self.check(self.lib.nsx_ctx_drain_failures(self.h, C.byref(v)))
+        return int(v.value)
+
+    def dt_resumes(self) -> int:
+        """distanceTransform queue generations finished after a host round trip
+        (more solve passes than TUNE_DT_SOLVE launched)."""
+        v = C.c_int64(0)
+        self.check(self.lib.nsx_ctx_dt_resumes(self.h, C.byref(v)))
         return int(v.value)
 
     def set_option(self, option: int, value: int) -> None:

@@ -21,6 +21,7 @@ NSX_TUNE_LEVELSET_LEVELS = 1
 NSX_TUNE_ENGINE = 2
 NSX_TUNE_FILL_SCANS = 3
 NSX_TUNE_DRAIN = 4
+NSX_TUNE_DT_SOLVE = 5
 NSX_ENGINE_ROUNDS = 0
 NSX_ENGINE_TREE = 1
 NSX_ENGINE_AUTO = 2
@@ -28,6 +29,9 @@ NSX_ENGINE_PLANES = 3
 NSX_OPT_I32_INVERT = 0
 NSX_I32_INVERT_WRAP = 0
 NSX_I32_INVERT_SATURATE = 1
+NSX_OPT_DT_ORDER = 1
+NSX_DT_ORDER_QUEUE = 0
+NSX_DT_ORDER_TILES = 1
 NSX_MEM_DEVICE, NSX_MEM_HOST = 0, 1
 
 
@@ -62,6 +66,7 @@ SIGNATURES = {
     "nsx_ctx_set_tuning": (C.c_int, [_CTX, C.c_int, C.c_int]),
     "nsx_ctx_get_tuning": (C.c_int, [_CTX, C.c_int, P(C.c_int)]),
     "nsx_ctx_drain_failures": (C.c_int, [_CTX, P(C.c_int64)]),
+    "nsx_ctx_dt_resumes": (C.c_int, [_CTX, P(C.c_int64)]),
     "nsx_image_check": (C.c_int, [P(NsxImg), P(C.c_char_p)]),
     "nsx_ctx_set_option": (C.c_int, [_CTX, C.c_int, C.c_int]),
     "nsx_ctx_get_option": (C.c_int, [_CTX, C.c_int, P(C.c_int)]),

@@ -145,8 +145,13 @@ nsx_status nsx_ctx_read_tile_counts(nsx_ctx* ctx, int64_t* first_pass, int64_t* 
  * per-round launches.  The serving loop's slots always use per-round
  * launches; nsx_batch_autotune picks per tile group.  Counterpart of the
  * reference's persistent queue kernel (global_queue.cu:746-857). */
+/* NSX_TUNE_DT_SOLVE: relaxation passes nsx_distance_transform launches per
+ * queue generation (NSX_DT_ORDER_QUEUE) before checking, 0..8, default 1.  A
+ * generation whose labels the passes did not settle is found by the check and
+ * finished with more passes after a host round trip; results do not depend on
+ * this value. */
 enum { NSX_TUNE_VISIT_SWEEPS = 0, NSX_TUNE_LEVELSET_LEVELS = 1, NSX_TUNE_ENGINE = 2, NSX_TUNE_FILL_SCANS = 3,
-       NSX_TUNE_DRAIN = 4, NSX_TUNE_NPARAMS = 5 };
+       NSX_TUNE_DRAIN = 4, NSX_TUNE_DT_SOLVE = 5, NSX_TUNE_NPARAMS = 6 };
 enum { NSX_ENGINE_ROUNDS = 0, NSX_ENGINE_TREE = 1, NSX_ENGINE_AUTO = 2, NSX_ENGINE_PLANES = 3 };
 nsx_status nsx_ctx_set_tuning(nsx_ctx* ctx, int param, int value);
 nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value);
@@ -154,6 +159,11 @@ nsx_status nsx_ctx_get_tuning(const nsx_ctx* ctx, int param, int* value);
  * restarted with per-round launches, since the context was created.  No
  * reference counterpart. */
 nsx_status nsx_ctx_drain_failures(const nsx_ctx* ctx, int64_t* n);
+
+/* Queue generations of nsx_distance_transform (NSX_DT_ORDER_QUEUE) on this
+ * context that the launched solve passes (NSX_TUNE_DT_SOLVE) did not settle
+ * and that were finished after a host round trip.  No reference counterpart. */
+nsx_status nsx_ctx_dt_resumes(const nsx_ctx* ctx, int64_t* n);
 
 /* Semantic options of a context (these change results).
  * NSX_OPT_I32_INVERT: how PixelOperations::invert<int> (PixelOperations.cpp:
@@ -174,8 +184,19 @@ nsx_status nsx_ctx_drain_failures(const nsx_ctx* ctx, int64_t* n);
  * Applies to nsx_fill_holes and nsx_segment_chain on int32 images.  No
  * reference counterpart (OpenCV 2.x is not available to settle the
  * reference's CPU behaviour, so the default follows its GPU code). */
-enum { NSX_OPT_I32_INVERT = 0 };
+enum { NSX_OPT_I32_INVERT = 0, NSX_OPT_DT_ORDER = 1 };
 enum { NSX_I32_INVERT_WRAP = 0, NSX_I32_INVERT_SATURATE = 1 };
+/* NSX_OPT_DT_ORDER: the order in which nsx_distance_transform applies the
+ * reference's propagation rule (MorphologicOperations.cpp:90-119).
+ *   NSX_DT_ORDER_QUEUE (default): the reference's FIFO queue order
+ *     (:224-245), reproduced generation by generation, so labels and
+ *     distances equal the reference's bit for bit.
+ *   NSX_DT_ORDER_TILES: synchronous passes in 64x64 tiles to a fixed point of
+ *     the same rule.  Where the rule has several fixed points (pixels the
+ *     propagation leaves short of the nearest background, and equally near
+ *     backgrounds) the result can differ from the reference's; it is never
+ *     nearer than the exact Euclidean distance.  No reference counterpart. */
+enum { NSX_DT_ORDER_QUEUE = 0, NSX_DT_ORDER_TILES = 1 };
 nsx_status nsx_ctx_set_option(nsx_ctx* ctx, int option, int value);
 nsx_status nsx_ctx_get_option(const nsx_ctx* ctx, int option, int* value);
 

@@ -487,6 +487,9 @@ def _dt_gpu_masks(rng):
     wide = np.full((2, 400), 255, np.uint8)
     wide[0, 399] = 0
     out.append(wide)
+    # a front of more than a quarter of the pixels: the queue's entry buffers
+    # start smaller and the call starts over with more room
+    out.append(np.where(rng.random((200, 300)) < 0.6, 0, 255).astype(np.uint8))
     # distances above 1024, where float rounding makes ties between squares
     corner = np.full((1300, 1100), 255, np.uint8)
     corner[0, 0] = 0
@@ -496,18 +499,21 @@ def _dt_gpu_masks(rng):
 
 
 def test_distance_transform(nh, orc, gpu):
-    """distanceTransform: float distances equal the oracle's bit for bit; every
-    int32 label is a background pixel (or rows*cols*3) at that distance, and no
-    neighbour's label is strictly closer; host and device inputs."""
+    """distanceTransform in the reference's queue order (the default): float
+    distances and int32 labels equal the oracle's bit for bit, ties between
+    equally near backgrounds included; every label is a background pixel (or
+    rows*cols*3) at that distance, and no neighbour's label is strictly closer;
+    host and device inputs."""
     import torch
     rng = np.random.default_rng(151)
     for m in _dt_gpu_masks(rng):
-        _, want = orc.distance_transform(m)
+        want_nn, want = orc.distance_transform(m)
         got = nh.distanceTransform(m)
         assert got.dtype == np.float32
         assert np.array_equal(got, want), m.shape
         nn = nh.distanceTransform(m, False)
         assert nn.dtype == np.int32
+        assert np.array_equal(nn, want_nn.reshape(m.shape)), m.shape
         real = nn != 3 * m.size
         assert (m.reshape(-1)[nn[real]] == 0).all()
         assert np.array_equal(brute.label_distance(nn, m.shape), want)
@@ -526,37 +532,121 @@ def test_distance_transform(nh, orc, gpu):
     assert nh.distanceTransform(e).shape == e.shape
 
 
-def test_distance_transform_stuck_pixel(nh, orc, gpu):
-    """A sparse 1000x1000 background where the oracle's FIFO propagation stops
-    short of the exact distance at one pixel.  Which fixed point such a pixel
-    reaches depends on the order of the offers, so the GPU result is held to
-    what the rule fixes: the same distances elsewhere, never nearer than the
-    exact distance, and no neighbour's label strictly closer."""
+def _dt_tiles_ctx(nh):
+    c = nh.Context(0)
+    c.set_option(nh.OPT_DT_ORDER, nh.DT_ORDER_TILES)
+    assert c.get_option(nh.OPT_DT_ORDER) == nh.DT_ORDER_TILES
+    return c
+
+
+def test_distance_transform_tiles(nh, orc, gpu):
+    """The tile order (DT_ORDER_TILES) reaches a fixed point of the same rule:
+    on these fixtures its distances equal the oracle's, every label is a
+    background pixel at that distance and no neighbour's label is closer."""
+    c = _dt_tiles_ctx(nh)
+    try:
+        for m in _dt_gpu_masks(np.random.default_rng(151)):
+            _, want = orc.distance_transform(m)
+            assert np.array_equal(nh.distanceTransform(m, ctx=c), want), m.shape
+            nn = nh.distanceTransform(m, False, ctx=c)
+            assert np.array_equal(brute.label_distance(nn, m.shape), want)
+            if m.size <= 200_000:
+                assert brute.distance_fixed_point_violations(nn, m.shape) == 0
+    finally:
+        c.close()
+
+
+def _stuck_mask():
     rng = np.random.default_rng(2)
     m = None
     for p in (0.00002, 0.0001, 0.0005):
         m = np.where(rng.random((1000, 1000)) < p, 0, 255).astype(np.uint8)
+    return m
+
+
+def test_distance_transform_stuck_pixel(nh, orc, gpu):
+    """A sparse 1000x1000 background where the oracle's FIFO propagation stops
+    short of the exact distance at one pixel.  The queue order stops there too:
+    labels and distances equal the oracle's everywhere, on repeated calls."""
+    m = _stuck_mask()
+    want_nn, want = orc.distance_transform(m)
+    exact, _ = brute.exact_distance_with_far(m)
+    assert int((want != exact).sum()) >= 1
+    for _ in range(2):
+        assert np.array_equal(nh.distanceTransform(m), want)
+        assert np.array_equal(nh.distanceTransform(m, False), want_nn.reshape(m.shape))
+
+
+def test_distance_transform_stuck_pixel_tiles(nh, orc, gpu):
+    """The same image in tile order.  Which fixed point a stuck pixel reaches
+    depends on the order of the offers, so this result is held to what the rule
+    fixes: the same distances elsewhere, never nearer than the exact distance,
+    and no neighbour's label strictly closer."""
+    ctx = _dt_tiles_ctx(nh)
+    try:
+        _check_stuck_tiles(nh, orc, ctx)
+    finally:
+        ctx.close()
+
+
+def _check_stuck_tiles(nh, orc, ctx):
+    m = _stuck_mask()
     _, want = orc.distance_transform(m)
     exact, _ = brute.exact_distance_with_far(m)
-    got = nh.distanceTransform(m)
+    got = nh.distanceTransform(m, ctx=ctx)
     differ = got != want
     assert int(differ.sum()) <= 10
     assert (got >= exact).all()
     assert (want[differ] > exact[differ]).all() or (got[differ] > exact[differ]).all()
-    nn = nh.distanceTransform(m, False)
+    nn = nh.distanceTransform(m, False, ctx=ctx)
     assert np.array_equal(brute.label_distance(nn, m.shape), got)
     assert brute.distance_fixed_point_violations(nn, m.shape) == 0
     # the same fixed point on every call
-    assert np.array_equal(nh.distanceTransform(m, False), nn)
-    assert np.array_equal(nh.distanceTransform(m), got)
+    assert np.array_equal(nh.distanceTransform(m, False, ctx=ctx), nn)
+    assert np.array_equal(nh.distanceTransform(m, ctx=ctx), got)
 
 
-def test_distance_transform_4096_bounds(nh, orc, gpu):
+@pytest.mark.parametrize("solve", [0, 8])
+def test_distance_transform_solve_passes(nh, orc, gpu, solve):
+    """The queue order with no solve pass launched per generation (every
+    generation whose labels change is finished after a host round trip) and
+    with eight: the same labels as the oracle."""
+    ctx = nh.Context(0)
+    try:
+        ctx.set_tuning(nh._lib.NSX_TUNE_DT_SOLVE, solve)
+        assert ctx.get_tuning(nh._lib.NSX_TUNE_DT_SOLVE) == solve
+        rng = np.random.default_rng(152)
+        for m in _dt_gpu_masks(rng)[4:7] + [_stuck_mask()[:400, :600]]:
+            want_nn, want = orc.distance_transform(m)
+            assert np.array_equal(nh.distanceTransform(m, False, ctx=ctx), want_nn.reshape(m.shape)), m.shape
+            assert np.array_equal(nh.distanceTransform(m, ctx=ctx), want), m.shape
+        if solve == 0:
+            assert ctx.dt_resumes() > 0
+    finally:
+        ctx.close()
+    with pytest.raises(nh.NsxError):
+        nh.default_context().set_tuning(nh._lib.NSX_TUNE_DT_SOLVE, 9)
+
+
+def test_distance_transform_4096(nh, orc, gpu):
     """At 4096x4096 (the disks image of the chain workload thresholded at 128)
-    both propagation orders stop short of the exact distance at a few stuck
-    pixels, at different ones.  Pinned: the GPU is never nearer than the exact
-    Euclidean distance, is a fixed point of the rule, is exact at least as often
-    as the reference's FIFO order, and differs from it at fewer than 1 pixel in
+    the reference's queue order stops short of the exact distance at 1812 stuck
+    pixels; the queue order here gives the same labels and distances."""
+    import torch
+    from nscale_hip.workloads import chain_images
+    _, mask = chain_images()
+    binm = np.where(mask > 128, 255, 0).astype(np.uint8)
+    want_nn, want = orc.distance_transform(binm)
+    d = torch.from_numpy(binm).cuda()
+    assert np.array_equal(nh.distanceTransform(d).cpu().numpy(), want)
+    assert np.array_equal(nh.distanceTransform(d, False).cpu().numpy(), want_nn.reshape(binm.shape))
+
+
+def test_distance_transform_4096_bounds_tiles(nh, orc, gpu):
+    """The same image in tile order, which stops short at a few stuck pixels
+    too, at different ones.  Pinned: never nearer than the exact Euclidean
+    distance, a fixed point of the rule, exact at least as often as the
+    reference's FIFO order, and different from it at fewer than 1 pixel in
     10 000 (1397 of 16.7 M when measured)."""
     import torch
     from nscale_hip.workloads import chain_images
@@ -565,8 +655,12 @@ def test_distance_transform_4096_bounds(nh, orc, gpu):
     _, want = orc.distance_transform(binm)
     exact, _ = brute.exact_distance_with_far(binm)
     d = torch.from_numpy(binm).cuda()
-    got = nh.distanceTransform(d).cpu().numpy()
-    nn = nh.distanceTransform(d, False).cpu().numpy()
+    ctx = _dt_tiles_ctx(nh)
+    try:
+        got = nh.distanceTransform(d, ctx=ctx).cpu().numpy()
+        nn = nh.distanceTransform(d, False, ctx=ctx).cpu().numpy()
+    finally:
+        ctx.close()
     assert (got >= exact).all()
     assert int((got != want).sum()) < binm.size // 10_000
     assert int((got != exact).sum()) <= int((want != exact).sum())
