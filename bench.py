@@ -905,6 +905,22 @@ def bench_ops(args):
         total_ms += ms
         r = {"ms": round(ms, 4), "mpix_s": round(npx / ms / 1e3, 1),
              "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        tile_got = None
+        if name == "distanceTransform":
+            # the same rule in tile order (a fixed point of the rule, not the
+            # reference's queue order), for comparison
+            tctx = nh.Context(0)
+            tctx.set_option(nh.OPT_DT_ORDER, nh.DT_ORDER_TILES)
+            tile_got = nh.distanceTransform(d_bin, ctx=tctx)
+            torch.cuda.synchronize()
+            tts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                tile_got = nh.distanceTransform(d_bin, ctx=tctx)
+                torch.cuda.synchronize()
+                tts.append((time.perf_counter() - t0) * 1e3)
+            tctx.close()
+            r["tile_order_ms"] = round(statistics.median(tts), 4)
         if orc is not None:
             t0 = time.perf_counter()
             want = cpu()
@@ -917,16 +933,20 @@ def bench_ops(args):
                 if name == "distanceTransform":
                     # the propagation can stop short of the exact distance at
                     # a few pixels, at places that depend on the order of the
-                    # offers: count them on both sides against scipy's EDT
+                    # offers: count them on both sides against scipy's EDT,
+                    # for the queue order and the tile order
                     from scipy import ndimage
                     yy, xx = np.mgrid[0:n, 0:n]
                     d2 = np.rint(ndimage.distance_transform_edt(binm != 0) ** 2).astype(np.int64)
                     d2 = np.minimum(d2, xx.astype(np.int64) ** 2 + (3 * n - yy.astype(np.int64)) ** 2)
                     exact = np.sqrt(d2.astype(np.float64)).astype(np.float32)
+                    tg = tile_got.cpu().numpy()
                     r.update({"pixels_differing": int((g != want).sum()),
                               "oracle_not_exact": int((want != exact).sum()),
                               "gpu_not_exact": int((g != exact).sum()),
-                              "gpu_below_exact": int((g < exact).sum())})
+                              "gpu_below_exact": int((g < exact).sum()),
+                              "tile_order_pixels_differing": int((tg != want).sum()),
+                              "tile_order_not_exact": int((tg != exact).sum())})
             r.update({"cpu_ms": round(cpu_ms, 1), "speedup": round(cpu_ms / ms, 1), "bitexact": bool(same)})
         res[name] = r
         log(name, r)

@@ -8,35 +8,37 @@
 // ends at a fixed point of that rule: no pixel is strictly closer to a
 // neighbour's label than to its own.
 //
-// Here the same rule runs to a fixed point in 64x64 tiles.  A tile loads its
-// labels and a one-pixel halo into LDS, applies the rule in synchronous passes
-// (read one LDS buffer, write the other) until no label in it changes, and
-// writes the tile to its other global buffer; rounds of tile launches repeat
-// until a round changes nothing.  A tile runs in a round only when it or one of
-// its 8 neighbour tiles changed in the previous round.  Each tile's labels live
-// in one of two global buffers, named by a per-tile index (one byte per tile
-// and round parity): a tile that changes writes the buffer its index does not
-// name and flips the index for the next round; the others keep theirs, so
-// nothing is copied.  Every read is of the round-start state (the neighbours'
-// indices of this round), so the result is deterministic.  Rounds are
-// launched in batches with one host check of the last round's change word.
+// The rule has many fixed points: a pixel can be left with a label that no
+// neighbour can improve although a nearer background exists (the "stuck"
+// pixels of propagation distance transforms), and where several backgrounds
+// are equally far, which one a pixel keeps depends on the order of the offers.
+// Two orders are built here.
+//
+// Queue order (NSX_DT_ORDER_QUEUE, the default) reproduces the reference's
+// FIFO order generation by generation (see the section below) and so its
+// labels and distances, bit for bit.
+//
+// Tile order (NSX_DT_ORDER_TILES) runs the same rule to a fixed point in 64x64
+// tiles.  A tile loads its labels and a one-pixel halo into LDS, applies the
+// rule in synchronous passes (read one LDS buffer, write the other) until no
+// label in it changes, and writes the tile to its other global buffer; rounds
+// of tile launches repeat until a round changes nothing.  A tile runs in a
+// round only when it or one of its 8 neighbour tiles changed in the previous
+// round.  Each tile's labels live in one of two global buffers, named by a
+// per-tile index (one byte per tile and round parity): a tile that changes
+// writes the buffer its index does not name and flips the index for the next
+// round; the others keep theirs, so nothing is copied.  Every read is of the
+// round-start state (the neighbours' indices of this round), so the result is
+// deterministic.  Rounds are launched in batches with one host check of the
+// last round's change word.  At stuck pixels and ties its result can differ
+// from the reference's: on the 4096x4096 disks image (3000 disks) 1397 of
+// 16.7 M distances differ; the reference's order is off the exact Euclidean
+// distance at 1812 pixels and this one at 577, never below it.
 //
 // Labels are packed (y << 16) | x inside the kernels, with DT_FAR for the
 // starting point, so no integer division is needed per offer; the output turns
 // them into the reference's raster index (far = rows*cols*3) or its float
 // distance (:254-266).
-//
-// Parity is partial.  The rule has many fixed points: a pixel can be left with
-// a label that no neighbour can improve although a nearer background exists
-// (the "stuck" pixels of propagation distance transforms), and which fixed
-// point is reached depends on the order of the offers.  The reference's
-// sequential FIFO order cannot be reproduced by a parallel schedule, so at
-// stuck pixels the distances of the two may differ; on the 4096x4096 disks
-// image (3000 disks) 1397 of 16.7 M pixels differ, the reference's order is
-// off the exact Euclidean distance at 1812 pixels and this one at 577, never
-// below it.  Where several backgrounds are equally far the two also keep
-// different ones (first offered by the queue vs by the sweep), so index maps
-// differ there with equal distances.
 #include "nsx_internal.h"
 
 #include <cmath>
@@ -63,13 +65,16 @@ __device__ __forceinline__ uint64_t dt_d2(int x, int y, uint32_t lab, int far_y)
     return (uint64_t)(dx * dx + dy * dy);
 }
 
-// The reference's test: float(sqrt(double(a))) < float(sqrt(double(b))).  Below
-// 2^20 distinct squares stay distinct after rounding to float (their square
-// roots differ by more than 1/2048, a float ulp there is 2^-14), so the integer
-// test decides; above, ties from rounding are possible and the roots are taken.
+// The reference's test: float(sqrt(double(a))) < float(sqrt(double(b))).  For
+// a < b with a <= 2^20 the rounded roots stay apart: the roots differ by more
+// than 1/2048 and a float ulp up to 1024 is at most 2^-13 (and
+// float(sqrt(2^20 + 1)) > 1024), so the integer test decides.  Above, ties
+// from rounding are possible and the roots are taken.  (The far starting
+// point's squared distance is above 2^20, so the common test of an offer
+// against it stays integer.)
 __device__ __forceinline__ bool dt_closer(uint64_t a, uint64_t b) {
     if (a >= b) return false;
-    if (b <= (1ull << 20)) return true;
+    if (a <= (1ull << 20)) return true;
     return (float)sqrt((double)a) < (float)sqrt((double)b);
 }
 
@@ -253,80 +258,130 @@ __global__ void dt_out_kernel(const uint32_t* buf0, const uint32_t* buf1, const 
 // tagged list), solve passes, push masks, an exclusive scan of their counts,
 // and the scatter of the next generation in push order.
 
-constexpr int QCH = 1024;   // entries per chunk: 256 threads x 4
-constexpr int QNB = 1024;   // blocks of the per-generation kernels, looping over chunks
+constexpr int QCH = 1024;   // entries per solve chunk, one per thread of a block
+constexpr int QPC = 32;     // entries per push chunk (push counts, scan): 8 lanes per entry
+constexpr int QNB = 512;    // blocks of the solve kernel, looping over chunks
+constexpr int QNP = 2048;   // blocks of the push and scatter kernels, looping over chunks
 constexpr int QMAXSOLVE = 16;
 // control words
 enum { QC_LEN0 = 0, QC_LEN1 = 1, QC_STALL = 2, QC_ERR = 3, QC_CHG = 8 };   // QC_CHG + solve pass
 
 struct Dtq {
     uint32_t* S;       // label of every pixel
-    uint64_t* head;    // per pixel: (tag << 32) | newest entry of the generation at the pixel
+    // The entries of a generation at one pixel: head holds (tag << 32) | the
+    // first to arrive; head2 (tag << 32) | the latest of the others, with bit
+    // 31 set when there is a third, the ones before it linked through nxt.
+    uint64_t* head;
+    uint64_t* head2;
     uint64_t* win;     // per pixel: (tag << 32) | last entry that pushed it
     uint32_t* E[2];    // entry positions (y << 16 | x), by generation parity
     uint32_t* c[2];    // entry labels, by generation parity
-    uint64_t* nxt;     // per entry: the head it replaced
+    uint64_t* nxt;     // per later entry: the head2 it replaced
     uint8_t* mask;     // per entry: the neighbours it pushes (bit o = offer o)
-    uint32_t* part;    // per chunk: count, then exclusive offset
+    uint8_t* ext;      // per entry: 1 when an offer to it comes from an earlier chunk
+    uint32_t* part;    // per push chunk (or front chunk): count, then exclusive offset
     int32_t* ctl;
     int rows, cols, far_y;
     uint32_t tag0;     // generation g is tagged tag0 + g
     uint32_t cap;      // entries a generation buffer holds
 };
 
-__constant__ int QDX[8] = {-1, 1, 0, -1, 1, 0, -1, 1};
-__constant__ int QDY[8] = {-1, -1, -1, 1, 1, 1, 0, 0};
+// the reference's offer order: up-left, up-right, up, down-left, down-right,
+// down, left, right
+constexpr int kQdx[8] = {-1, 1, 0, -1, 1, 0, -1, 1};
+constexpr int kQdy[8] = {-1, -1, -1, 1, 1, 1, 0, 0};
 
 __device__ __forceinline__ int64_t qlin(const Dtq& Q, int x, int y) { return (int64_t)y * Q.cols + x; }
 __device__ __forceinline__ uint32_t qpack(int x, int y) { return ((uint32_t)y << 16) | (uint32_t)x; }
+__device__ __forceinline__ bool qin(const Dtq& Q, int x, int y) {
+    return x >= 0 && y >= 0 && x < Q.cols && y < Q.rows;
+}
 
-// c_k: the label of entry k's pixel (x, y) when it is popped.  Ties in float
-// distance go to the earlier offer; the generation-start label is earliest.
-template <typename Val>
-__device__ __forceinline__ uint32_t dtq_popped(const Dtq& Q, uint32_t tag, int x, int y, int k, Val val) {
-    uint32_t best = Q.S[qlin(Q, x, y)];
+// Offer j (label v, squared distance q) against the best so far (label best,
+// squared distance qb, entry jb; jb = -1 for the generation-start label, which
+// comes first): the smaller float distance wins, a tie goes to the earlier.
+__device__ __forceinline__ void dtq_take(uint32_t v, uint64_t q, int j, uint32_t& best, uint64_t& qb, int& jb) {
+    if (dt_closer(q, qb) || (jb >= 0 && j < jb && !dt_closer(qb, q))) {
+        best = v;
+        qb = q;
+        jb = j;
+    }
+}
+
+constexpr uint32_t QMORE = 0x80000000u;   // head2 flag: a third entry exists
+
+// Clamped position (always in the image), so that loads need no branch.
+__device__ __forceinline__ int64_t qclamp(const Dtq& Q, int x, int y) {
+    return qlin(Q, min(max(x, 0), Q.cols - 1), min(max(y, 0), Q.rows - 1));
+}
+
+// The entries of this generation at the 8 neighbours of (x, y): the first
+// (j, -1 for none), the latest later one (j2) and whether more exist (more
+// bit o), all loads independent.
+__device__ __forceinline__ void dtq_near(const Dtq& Q, uint32_t tag, int x, int y, int (&j)[8], int (&j2)[8],
+                                         uint32_t& more) {
+    uint64_t h[8], h2[8];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+        const int64_t li = qclamp(Q, x + kQdx[o], y + kQdy[o]);
+        h[o] = Q.head[li];
+        h2[o] = Q.head2[li];
+    }
+    more = 0;
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+        const bool in = qin(Q, x + kQdx[o], y + kQdy[o]);
+        j[o] = (in && (uint32_t)(h[o] >> 32) == tag) ? (int)(uint32_t)h[o] : -1;
+        const bool in2 = in && (uint32_t)(h2[o] >> 32) == tag;
+        j2[o] = in2 ? (int)((uint32_t)h2[o] & ~QMORE) : -1;
+        more |= (in2 && ((uint32_t)h2[o] & QMORE)) ? 1u << o : 0u;
+    }
+}
+
+// Calls f(j) for the entries of a list past its latest later one j2.
+template <typename F>
+__device__ __forceinline__ void dtq_rest(const Dtq& Q, uint32_t tag, int j2, F f) {
+    uint64_t h = Q.nxt[j2];
+    while ((uint32_t)(h >> 32) == tag) {
+        const int j = (int)((uint32_t)h & ~QMORE);
+        f(j);
+        h = Q.nxt[j];
+    }
+}
+
+// c_k: the best of the generation-start label s0 and the offers of entries
+// j < k at the 8 neighbours (val1(o, j) / val2(o, j): the label of a first /
+// latest later entry of list o, val(j): any entry's).
+template <typename V1, typename V2, typename Val>
+__device__ __forceinline__ uint32_t dtq_popped_near(const Dtq& Q, uint32_t tag, int x, int y, int k, uint32_t s0,
+                                                    const int (&j)[8], const int (&j2)[8], uint32_t more, V1 val1,
+                                                    V2 val2, Val val) {
+    uint32_t best = s0;
     uint64_t qb = dt_d2(x, y, best, Q.far_y);
     if (qb == 0) return best;
     int jb = -1;
+#pragma unroll
     for (int o = 0; o < 8; ++o) {
-        const int nx = x + QDX[o], ny = y + QDY[o];
-        if (nx < 0 || ny < 0 || nx >= Q.cols || ny >= Q.rows) continue;
-        uint64_t h = Q.head[qlin(Q, nx, ny)];
-        while ((uint32_t)(h >> 32) == tag) {
-            const int j = (int)(uint32_t)h;
-            if (j < k) {
-                const uint32_t v = val(j);
-                const uint64_t q = dt_d2(x, y, v, Q.far_y);
-                if (dt_closer(q, qb) || (jb >= 0 && j < jb && !dt_closer(qb, q))) {
-                    best = v;
-                    qb = q;
-                    jb = j;
-                }
-            }
-            h = Q.nxt[j];
+        if (j[o] >= 0 && j[o] < k) {
+            const uint32_t v = val1(o, j[o]);
+            dtq_take(v, dt_d2(x, y, v, Q.far_y), j[o], best, qb, jb);
+        }
+        if (j2[o] >= 0 && j2[o] < k) {
+            const uint32_t v = val2(o, j2[o]);
+            dtq_take(v, dt_d2(x, y, v, Q.far_y), j2[o], best, qb, jb);
         }
     }
+    if (more)
+#pragma unroll
+        for (int o = 0; o < 8; ++o)
+            if ((more >> o) & 1u)
+                dtq_rest(Q, tag, j2[o], [&](int jj) {
+                    if (jj < k) {
+                        const uint32_t v = val(jj);
+                        dtq_take(v, dt_d2(x, y, v, Q.far_y), jj, best, qb, jb);
+                    }
+                });
     return best;
-}
-
-// Whether entry k (label ck) pushes pixel (nx, ny): ck is strictly closer to
-// it than its generation-start label and than every offer of an earlier entry
-// of the generation at one of its neighbours.
-__device__ __forceinline__ bool dtq_pushes(const Dtq& Q, const uint32_t* cv, uint32_t tag, int nx, int ny, int k,
-                                           uint32_t ck) {
-    const uint64_t qk = dt_d2(nx, ny, ck, Q.far_y);
-    if (!dt_closer(qk, dt_d2(nx, ny, Q.S[qlin(Q, nx, ny)], Q.far_y))) return false;
-    for (int o = 0; o < 8; ++o) {
-        const int mx = nx + QDX[o], my = ny + QDY[o];
-        if (mx < 0 || my < 0 || mx >= Q.cols || my >= Q.rows) continue;
-        uint64_t h = Q.head[qlin(Q, mx, my)];
-        while ((uint32_t)(h >> 32) == tag) {
-            const int j = (int)(uint32_t)h;
-            if (j < k && !dt_closer(qk, dt_d2(nx, ny, cv[j], Q.far_y))) return false;
-            h = Q.nxt[j];
-        }
-    }
-    return true;
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
@@ -345,86 +400,80 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
-// Exclusive prefix of a 256-thread block's four counts in the order
-// (i, thread): the order of entries base + i * 256 + thread.  sh holds 17 words.
-__device__ __forceinline__ uint32_t block_scan4(const uint32_t (&cnt)[4], uint32_t (&ex)[4], uint32_t* sh) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t inc[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        inc[i] = wave_incl_scan(cnt[i]);
-        if (lane == 63) sh[i * 4 + w] = inc[i];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t r = 0;
-        for (int q = 0; q < 16; ++q) {
-            const uint32_t t = sh[q];
-            sh[q] = r;
-            r += t;
-        }
-        sh[16] = r;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 4; ++i) ex[i] = sh[i * 4 + w] + inc[i] - cnt[i];
-    const uint32_t total = sh[16];
-    __syncthreads();
-    return total;
-}
-
 // checkDistNeighbors8 (:33-88): a background pixel with a nonzero 8-neighbour.
+// All nine loads at clamped positions, then the tests.
 __device__ __forceinline__ bool dtq_front(const char* mask, int64_t pm, int rows, int cols, int x, int y) {
-    if (((const uint8_t*)(mask + (int64_t)y * pm))[x] != 0) return false;
-    for (int dy = -1; dy <= 1; ++dy) {
-        const int yy = y + dy;
-        if (yy < 0 || yy >= rows) continue;
-        const uint8_t* r = (const uint8_t*)(mask + (int64_t)yy * pm);
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int xx = x + dx;
-            if ((dx || dy) && xx >= 0 && xx < cols && r[xx] != 0) return true;
-        }
-    }
-    return false;
+    const uint8_t* r0 = (const uint8_t*)(mask + (int64_t)max(y - 1, 0) * pm);
+    const uint8_t* r1 = (const uint8_t*)(mask + (int64_t)y * pm);
+    const uint8_t* r2 = (const uint8_t*)(mask + (int64_t)min(y + 1, rows - 1) * pm);
+    const int xl = max(x - 1, 0), xr = min(x + 1, cols - 1);
+    const uint8_t a = r0[xl], b = r0[x], c = r0[xr], d = r1[xl], e = r1[x], f = r1[xr], g = r2[xl], h = r2[x],
+                  i = r2[xr];
+    const bool up = y > 0, dn = y < rows - 1, lf = x > 0, rt = x < cols - 1;
+    return e == 0 && ((up && (b || (lf && a) || (rt && c))) || (dn && (h || (lf && g) || (rt && i))) ||
+                      (lf && d) || (rt && f));
 }
 
 // Generation 0, count pass: every label initialised (a background pixel holds
 // itself, the others the far point) and per chunk (1024 columns of a row,
-// chunks in raster order) the number of front pixels.
-__global__ __launch_bounds__(256) void dtq_front_count_kernel(Dtq Q, const char* mask, int64_t pm, int nseg) {
-    const int ci = blockIdx.x, y = ci / nseg, seg = ci - y * nseg;
-    uint32_t cnt = 0;
+// chunks in raster order) the number of front pixels.  A block takes 4
+// consecutive chunks, their loads issued together.
+__global__ __launch_bounds__(1024) void dtq_front_count_kernel(Dtq Q, const char* mask, int64_t pm, int nseg,
+                                                               int nfront) {
+    uint32_t cnt[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const int x = seg * QCH + i * 256 + (int)threadIdx.x;
-        if (x >= Q.cols) continue;
-        const bool bg = ((const uint8_t*)(mask + (int64_t)y * pm))[x] == 0;
-        Q.S[qlin(Q, x, y)] = bg ? qpack(x, y) : DT_FAR;
-        cnt += dtq_front(mask, pm, Q.rows, Q.cols, x, y) ? 1u : 0u;
+        const int ci = blockIdx.x * 4 + i, y = ci / nseg, seg = ci - y * nseg;
+        const int x = seg * QCH + (int)threadIdx.x;
+        cnt[i] = 0;
+        if (ci < nfront && x < Q.cols) {
+            const bool bg = ((const uint8_t*)(mask + (int64_t)y * pm))[x] == 0;
+            Q.S[qlin(Q, x, y)] = bg ? qpack(x, y) : DT_FAR;
+            cnt[i] = dtq_front(mask, pm, Q.rows, Q.cols, x, y) ? 1u : 0u;
+        }
     }
-    __shared__ uint32_t red[4];
-    cnt = wave_sum(cnt);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+    __shared__ uint32_t sh[4][16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t v = wave_sum(cnt[i]);
+        if ((threadIdx.x & 63) == 0) sh[i][threadIdx.x >> 6] = v;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) Q.part[ci] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x < 4 && blockIdx.x * 4 + (int)threadIdx.x < nfront) {
+        uint32_t t = 0;
+        for (int q = 0; q < 16; ++q) t += sh[threadIdx.x][q];
+        Q.part[blockIdx.x * 4 + threadIdx.x] = t;
+    }
 }
 
 // Generation 0, scatter pass: the front pixels in raster order.
-__global__ __launch_bounds__(256) void dtq_front_scatter_kernel(Dtq Q, const char* mask, int64_t pm, int nseg) {
+__global__ __launch_bounds__(1024) void dtq_front_scatter_kernel(Dtq Q, const char* mask, int64_t pm, int nseg,
+                                                                 int nfront) {
     if (Q.ctl[QC_STALL]) return;
-    const int ci = blockIdx.x, y = ci / nseg, seg = ci - y * nseg;
-    __shared__ uint32_t sh[17];
-    uint32_t cnt[4], ex[4];
+    bool f[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const int x = seg * QCH + i * 256 + (int)threadIdx.x;
-        cnt[i] = (x < Q.cols && dtq_front(mask, pm, Q.rows, Q.cols, x, y)) ? 1u : 0u;
+        const int ci = blockIdx.x * 4 + i, y = ci / nseg, seg = ci - y * nseg;
+        const int x = seg * QCH + (int)threadIdx.x;
+        f[i] = ci < nfront && x < Q.cols && dtq_front(mask, pm, Q.rows, Q.cols, x, y);
     }
-    block_scan4(cnt, ex, sh);
-    const uint32_t base = Q.part[ci];
+    __shared__ uint32_t sh[4][16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-        if (cnt[i]) Q.E[0][base + ex[i]] = qpack(seg * QCH + i * 256 + (int)threadIdx.x, y);
+    for (int i = 0; i < 4; ++i) {
+        inc[i] = wave_incl_scan(f[i] ? 1u : 0u);
+        if (lane == 63) sh[i][w] = inc[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (!f[i]) continue;
+        uint32_t before = 0;
+        for (int q = 0; q < w; ++q) before += sh[i][q];
+        const int ci = blockIdx.x * 4 + i, y = ci / nseg, seg = ci - y * nseg;
+        Q.E[0][Q.part[ci] + before + inc[i] - 1u] = qpack(seg * QCH + (int)threadIdx.x, y);
+    }
 }
 
 // Exclusive prefix of part[0..n) in place, one block of 1024 threads; n is the
@@ -433,7 +482,7 @@ __global__ __launch_bounds__(256) void dtq_front_scatter_kernel(Dtq Q, const cha
 // ctl[out_slot]; above the buffers' capacity it sets the error and stall words.
 __global__ __launch_bounds__(1024) void dtq_scan_kernel(Dtq Q, int len_slot, int nfix, int out_slot) {
     if (Q.ctl[QC_STALL]) return;
-    const int n = len_slot >= 0 ? (Q.ctl[len_slot] + QCH - 1) / QCH : nfix;
+    const int n = len_slot >= 0 ? (Q.ctl[len_slot] + QPC - 1) / QPC : nfix;
     const int per = (n + 1023) / 1024;
     const int b = (int)threadIdx.x * per, e = min(b + per, n);
     uint64_t s = 0;
@@ -493,14 +542,39 @@ __global__ __launch_bounds__(256) void dtq_link_kernel(Dtq Q, int g) {
             Q.S[li] = v;
         }
         cv[k] = v;
-        Q.nxt[k] = atomicExch((unsigned long long*)&Q.head[li], ((unsigned long long)tag << 32) | (uint32_t)k);
+        Q.ext[k] = 1;   // checked by the push kernel unless a solve pass clears it
+        // the first entry at the pixel takes head; the others go to head2
+        const unsigned long long mine = ((unsigned long long)tag << 32) | (uint32_t)k;
+        unsigned long long cur = Q.head[li];
+        bool first = false;
+        while ((uint32_t)(cur >> 32) != tag) {   // at most twice: old tags only turn into this one
+            const unsigned long long got = atomicCAS((unsigned long long*)&Q.head[li], cur, mine);
+            if (got == cur) {
+                first = true;
+                break;
+            }
+            cur = got;
+        }
+        if (!first) {
+            unsigned long long old = Q.head2[li];
+            for (;;) {
+                const unsigned long long want = mine | ((uint32_t)(old >> 32) == tag ? QMORE : 0u);
+                const unsigned long long got = atomicCAS((unsigned long long*)&Q.head2[li], old, want);
+                if (got == old) break;
+                old = got;
+            }
+            Q.nxt[k] = old;
+        }
     }
 }
 
 // One relaxation pass of generation g's labels, in place.  A chunk's labels
-// are relaxed in LDS until they stop changing (the offers of other chunks read
-// as they are); pass `pass` runs only if pass - 1 changed something.
-__global__ __launch_bounds__(256) void dtq_solve_kernel(Dtq Q, int g, int pass) {
+// are relaxed in LDS until they stop changing (offers from earlier chunks read
+// as they are in global memory).  Pass 0 relaxes every entry and records which
+// entries take offers from earlier chunks; a later pass (run only if the pass
+// before it changed something) recomputes those first and relaxes the whole
+// chunk again only if one of them changed.
+__global__ __launch_bounds__(QCH) void dtq_solve_kernel(Dtq Q, int g, int pass) {
     if (Q.ctl[QC_STALL]) return;
     if (pass > 0 && Q.ctl[QC_CHG + pass - 1] == 0) return;
     const int L = Q.ctl[g & 1];
@@ -510,113 +584,233 @@ __global__ __launch_bounds__(256) void dtq_solve_kernel(Dtq Q, int g, int pass) 
     __shared__ uint32_t lc[QCH];
     for (int ci = blockIdx.x; ci * QCH < L; ci += gridDim.x) {
         const int base = ci * QCH;
+        const int k = base + (int)threadIdx.x;
+        const bool live = k < L;
+        int x = 0, y = 0, jn[8], jn2[8];
+        uint32_t more = 0;
+        uint32_t ev[8], ev2[8];   // labels of entries in earlier chunks, read once per pass
+        bool ext = false;
+        uint32_t s0 = 0;
+        bool load = live && (pass == 0 || Q.ext[k]);
+        auto fetch = [&]() {
+            const uint32_t p = E[k];
+            x = (int)(p & 0xFFFFu);
+            y = (int)(p >> 16);
+            s0 = Q.S[qlin(Q, x, y)];
+            dtq_near(Q, tag, x, y, jn, jn2, more);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int k = base + i * 256 + (int)threadIdx.x;
-            lc[i * 256 + threadIdx.x] = k < L ? cv[k] : 0u;
-        }
+            for (int o = 0; o < 8; ++o) {
+                ev[o] = cv[(jn[o] >= 0 && jn[o] < base) ? jn[o] : k];
+                ev2[o] = cv[(jn2[o] >= 0 && jn2[o] < base) ? jn2[o] : k];
+            }
+        };
+        if (load) fetch();
+        lc[threadIdx.x] = live ? cv[k] : 0u;
         __syncthreads();
-        bool any = false;
-        auto val = [&](int j) { return (j >= base && j < base + QCH) ? lc[j - base] : cv[j]; };
+        auto val = [&](int j) { return j >= base ? lc[j - base] : cv[j]; };
+        auto val1 = [&](int o, int j) { return j >= base ? lc[j - base] : ev[o]; };
+        auto val2 = [&](int o, int j) { return j >= base ? lc[j - base] : ev2[o]; };
+        bool any = false, settled = false;
         for (int sweep = 0; sweep < 64; ++sweep) {
+            if (sweep == 1 && live && !load) {
+                load = true;
+                fetch();
+            }
             bool ch = false;
-#pragma unroll 1
-            for (int i = 0; i < 4; ++i) {
-                const int k = base + i * 256 + (int)threadIdx.x;
-                if (k >= L) break;
-                const uint32_t p = E[k];
-                const uint32_t v = dtq_popped(Q, tag, (int)(p & 0xFFFFu), (int)(p >> 16), k, val);
-                if (v != lc[k - base]) {
-                    lc[k - base] = v;
+            if (load) {
+                const uint32_t v = dtq_popped_near(Q, tag, x, y, k, s0, jn, jn2, more, val1, val2, val);
+                if (v != lc[threadIdx.x]) {
+                    lc[threadIdx.x] = v;
                     ch = true;
                 }
             }
-            if (!__syncthreads_or(ch)) break;
+            if (!__syncthreads_or(ch)) {
+                settled = true;
+                break;
+            }
             any = true;
         }
-        if (any) {
+        if (live && (pass == 0 || !settled)) {
+            // offers from earlier chunks (a list with a second entry counts as
+            // one); a chunk that did not settle within the sweeps marks all
+            // its entries, so that the push kernel checks every one
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int k = base + i * 256 + (int)threadIdx.x;
-                if (k < L) cv[k] = lc[k - base];
-            }
+            for (int o = 0; o < 8; ++o) ext |= (jn[o] >= 0 && jn[o] < base) || (jn2[o] >= 0 && jn2[o] < base);
+            Q.ext[k] = (uint8_t)(ext || more || !settled);
+        }
+        if (any) {
+            if (live) cv[k] = lc[threadIdx.x];
             if (threadIdx.x == 0) Q.ctl[QC_CHG + pass] = 1;
         }
         __syncthreads();
     }
 }
 
-// Generation g's push masks and per-chunk push counts.  For g > 0 each entry
-// first checks that its label solves its equation; if one does not, the
-// solve passes were too few: the stall word names the generation and every
-// later kernel returns at once, so the host can run more passes and go on.
+// Generation g's push masks and per-chunk push counts, 8 lanes per entry.
+// The offers that decide an entry's pushes come from the entries at its 5x5
+// neighbourhood: the 8 lanes load its 25 pixels' first and latest later
+// entries and their labels together (3 or 4 pixels each) into LDS.  Then lane o
+// decides whether the entry pushes its neighbour o (the reference's offer
+// order): when its label is strictly closer to it than the neighbour's
+// generation-start label and than every offer of an earlier entry at one of
+// the neighbour's own 8 neighbours.  For g > 0 an entry that takes offers
+// from earlier chunks first checks that its label solves its equation, lane o
+// contributing the offers of the entries at neighbour o, reduced over the 8
+// lanes (the other entries were relaxed to it within their chunk).  If one
+// does not, the solve passes were too few: the stall word names the
+// generation and every later kernel returns at once, so the host can run more
+// passes and go on.
 __global__ __launch_bounds__(256) void dtq_push_kernel(Dtq Q, int g) {
     if (Q.ctl[QC_STALL]) return;
     const int L = Q.ctl[g & 1];
     const uint32_t tag = Q.tag0 + (uint32_t)g;
     const uint32_t* E = Q.E[g & 1];
     const uint32_t* cv = Q.c[g & 1];
+    const int o = threadIdx.x & 7, lane = threadIdx.x & 63, e = threadIdx.x >> 3;
+    // offer o's direction: up-left, up-right, up, down-left, down-right, down, left, right
+    const int odx = o < 6 ? (o % 3 == 0 ? -1 : o % 3 == 1 ? 1 : 0) : (o == 6 ? -1 : 1);
+    const int ody = o < 3 ? -1 : (o < 6 ? 1 : 0);
+    // per entry of the block and pixel t = (dy + 2) * 5 + dx + 2 of its 5x5
+    // neighbourhood: first entry, latest later entry (-1: none; bit 30 of the
+    // later one: more entries follow it), and their labels
+    __shared__ int sj1[QPC][25], sj2[QPC][25];
+    __shared__ uint32_t sc1[QPC][25], sc2[QPC][25];
     __shared__ uint32_t red[4];
-    for (int ci = blockIdx.x; ci * QCH < L; ci += gridDim.x) {
-        const int base = ci * QCH;
-        uint32_t cnt = 0;
-#pragma unroll 1
-        for (int i = 0; i < 4; ++i) {
-            const int k = base + i * 256 + (int)threadIdx.x;
-            if (k >= L) break;
+    constexpr int kMore = 1 << 30;
+    for (int ci = blockIdx.x; ci * QPC < L; ci += gridDim.x) {
+        const int k = ci * QPC + e;
+        const bool live = k < L;
+        int x = 0, y = 0;
+        uint32_t ck = 0;
+        if (live) {
             const uint32_t p = E[k];
-            const int x = (int)(p & 0xFFFFu), y = (int)(p >> 16);
-            const uint32_t ck = cv[k];
-            if (g > 0 && dtq_popped(Q, tag, x, y, k, [&](int j) { return cv[j]; }) != ck) Q.ctl[QC_STALL] = g + 1;
-            uint32_t m = 0;
-            for (int o = 0; o < 8; ++o) {
-                const int nx = x + QDX[o], ny = y + QDY[o];
-                if (nx < 0 || ny < 0 || nx >= Q.cols || ny >= Q.rows) continue;
-                if (dtq_pushes(Q, cv, tag, nx, ny, k, ck)) m |= 1u << o;
+            x = (int)(p & 0xFFFFu);
+            y = (int)(p >> 16);
+            ck = cv[k];
+            uint64_t h1[4], h2[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int t = o + 8 * i;
+                const int64_t li = qclamp(Q, x + t % 5 - 2, y + t / 5 - 2);
+                h1[i] = t < 25 ? Q.head[li] : 0ull;
+                h2[i] = t < 25 ? Q.head2[li] : 0ull;
             }
-            Q.mask[k] = (uint8_t)m;
-            cnt += (uint32_t)__builtin_popcount(m);
+            int j1[4], j2[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int t = o + 8 * i;
+                const bool in = t < 25 && qin(Q, x + t % 5 - 2, y + t / 5 - 2);
+                j1[i] = (in && (uint32_t)(h1[i] >> 32) == tag) ? (int)(uint32_t)h1[i] : -1;
+                j2[i] = (in && (uint32_t)(h2[i] >> 32) == tag) ? (int)((uint32_t)h2[i] & ~QMORE) : -1;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int t = o + 8 * i;
+                const uint32_t c1 = cv[j1[i] >= 0 ? j1[i] : k], c2 = cv[j2[i] >= 0 ? j2[i] : k];
+                if (t < 25) {
+                    sj1[e][t] = j1[i];
+                    sj2[e][t] = j2[i] >= 0 && ((uint32_t)h2[i] & QMORE) ? (j2[i] | kMore) : j2[i];
+                    sc1[e][t] = c1;
+                    sc2[e][t] = c2;
+                }
+            }
         }
-        cnt = wave_sum(cnt);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+        __syncthreads();
+        bool ok = false;
+        if (live) {
+            const int nx = x + odx, ny = y + ody;
+            const uint32_t sn = Q.S[qclamp(Q, nx, ny)];
+            const uint64_t qk = dt_d2(nx, ny, ck, Q.far_y);
+            ok = qin(Q, nx, ny) && dt_closer(qk, dt_d2(nx, ny, sn, Q.far_y));
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int t = (ody + kQdy[q] + 2) * 5 + (odx + kQdx[q] + 2);
+                const int a = sj1[e][t], b2 = sj2[e][t];
+                const int b = b2 & ~kMore;
+                if (a >= 0 && a < k && !dt_closer(qk, dt_d2(nx, ny, sc1[e][t], Q.far_y))) ok = false;
+                if (b2 >= 0 && b < k && !dt_closer(qk, dt_d2(nx, ny, sc2[e][t], Q.far_y))) ok = false;
+                if (ok && b2 >= 0 && (b2 & kMore))
+                    dtq_rest(Q, tag, b, [&](int jj) {
+                        if (jj < k && !dt_closer(qk, dt_d2(nx, ny, cv[jj], Q.far_y))) ok = false;
+                    });
+            }
+            if (g > 0 && Q.ext[k]) {
+                // the offers to (x, y) of the entries at neighbour o
+                uint64_t qb = ~0ull;
+                int jb = INT_MAX;
+                uint32_t vb = 0;
+                auto offer = [&](int jj, uint32_t v) {
+                    if (jj >= k) return;
+                    const uint64_t qv = dt_d2(x, y, v, Q.far_y);
+                    if (dt_closer(qv, qb) || (jj < jb && !dt_closer(qb, qv))) {
+                        qb = qv;
+                        jb = jj;
+                        vb = v;
+                    }
+                };
+                const int t = (ody + 2) * 5 + (odx + 2);
+                const int a = sj1[e][t], b2 = sj2[e][t];
+                if (a >= 0) offer(a, sc1[e][t]);
+                if (b2 >= 0) {
+                    offer(b2 & ~kMore, sc2[e][t]);
+                    if (b2 & kMore) dtq_rest(Q, tag, b2 & ~kMore, [&](int jj) { offer(jj, cv[jj]); });
+                }
+#pragma unroll
+                for (int d = 1; d < 8; d <<= 1) {
+                    const uint64_t q2 = __shfl_xor(qb, d, 64);
+                    const int jo = __shfl_xor(jb, d, 64);
+                    const uint32_t v2 = __shfl_xor(vb, d, 64);
+                    if (dt_closer(q2, qb) || (jo < jb && !dt_closer(qb, q2))) {
+                        qb = q2;
+                        jb = jo;
+                        vb = v2;
+                    }
+                }
+                if (o == 0) {
+                    // the generation-start label comes before every offer
+                    const uint32_t s0 = Q.S[qlin(Q, x, y)];
+                    const uint64_t q0 = dt_d2(x, y, s0, Q.far_y);
+                    const uint32_t want = (q0 != 0 && jb != INT_MAX && dt_closer(qb, q0)) ? vb : s0;
+                    if (want != ck) Q.ctl[QC_STALL] = g + 1;
+                }
+            }
+        }
+        const uint64_t bal = __ballot(ok);
+        const uint32_t m = (uint32_t)(bal >> (lane & ~7)) & 0xFFu;
+        if (o == 0 && live) Q.mask[k] = (uint8_t)m;
+        uint32_t c = o == 0 ? (uint32_t)__builtin_popcount(m) : 0u;
+        c = wave_sum(c);
+        if (lane == 0) red[threadIdx.x >> 6] = c;
         __syncthreads();
         if (threadIdx.x == 0) Q.part[ci] = red[0] + red[1] + red[2] + red[3];
-        __syncthreads();
     }
 }
 
-// Generation g + 1 in push order, and each pushed pixel's last pusher.
+// Generation g + 1 in push order, and each pushed pixel's last pusher.  A
+// block takes 256 entries (8 push chunks): a wave's two halves are two chunks.
 __global__ __launch_bounds__(256) void dtq_scatter_kernel(Dtq Q, int g) {
     if (Q.ctl[QC_STALL]) return;
     const int L = Q.ctl[g & 1];
     const uint64_t tagw = (uint64_t)(Q.tag0 + (uint32_t)g) << 32;
     const uint32_t* E = Q.E[g & 1];
     uint32_t* En = Q.E[(g & 1) ^ 1];
-    __shared__ uint32_t sh[17];
-    for (int ci = blockIdx.x; ci * QCH < L; ci += gridDim.x) {
-        const int base = ci * QCH;
-        uint32_t m[4], cnt[4], ex[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int k = base + i * 256 + (int)threadIdx.x;
-            m[i] = k < L ? Q.mask[k] : 0u;
-            cnt[i] = (uint32_t)__builtin_popcount(m[i]);
-        }
-        block_scan4(cnt, ex, sh);
-        const uint32_t off = Q.part[ci];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (!m[i]) continue;
-            const int k = base + i * 256 + (int)threadIdx.x;
-            const uint32_t p = E[k];
-            const int x = (int)(p & 0xFFFFu), y = (int)(p >> 16);
-            uint32_t pos = off + ex[i];
-            for (int o = 0; o < 8; ++o) {
-                if (!((m[i] >> o) & 1u)) continue;
-                const int nx = x + QDX[o], ny = y + QDY[o];
-                En[pos++] = qpack(nx, ny);
-                atomicMax((unsigned long long*)&Q.win[qlin(Q, nx, ny)], (unsigned long long)(tagw | (uint32_t)k));
-            }
+    const int lane = threadIdx.x & 63;
+    for (int b = blockIdx.x; b * 256 < L; b += gridDim.x) {
+        const int k = b * 256 + (int)threadIdx.x;
+        const uint32_t m = k < L ? Q.mask[k] : 0u;
+        const uint32_t cnt = (uint32_t)__builtin_popcount(m);
+        const uint32_t inc = wave_incl_scan(cnt);
+        const uint32_t half = __shfl(inc, 31, 64);
+        const uint32_t ex = inc - cnt - (lane >= 32 ? half : 0u);
+        if (!m) continue;
+        const uint32_t p = E[k];
+        const int x = (int)(p & 0xFFFFu), y = (int)(p >> 16);
+        uint32_t pos = Q.part[k / QPC] + ex;
+        for (int o = 0; o < 8; ++o) {
+            if (!((m >> o) & 1u)) continue;
+            const int nx = x + kQdx[o], ny = y + kQdy[o];
+            En[pos++] = qpack(nx, ny);
+            atomicMax((unsigned long long*)&Q.win[qlin(Q, nx, ny)], (unsigned long long)(tagw | (uint32_t)k));
         }
     }
 }
@@ -636,11 +830,11 @@ nsx_status dist_queue_run(nsx_ctx* ctx, const View& mask, const View& out, bool 
     // starts the call over with twice the room.
     int64_t cap = npix / 4 + 4096;
     for (;;) {
-        const int64_t nchunk = std::max<int64_t>(nfront, (cap + QCH - 1) / QCH);
+        const int64_t nchunk = std::max<int64_t>(nfront, (cap + QPC - 1) / QPC);
         void *lab_p, *tags_p, *ent_p, *part_p, *misc_p;
         NSX_TRY(ctx->ws(WS_CCL_LABEL, sizeof(uint32_t) * (size_t)npix, &lab_p));
-        NSX_TRY(ctx->ws(WS_DT_TAGS, 2 * sizeof(uint64_t) * (size_t)npix, &tags_p));
-        NSX_TRY(ctx->ws(WS_DT_ENT, (size_t)cap * (4 * sizeof(uint32_t) + sizeof(uint64_t) + 1), &ent_p));
+        NSX_TRY(ctx->ws(WS_DT_TAGS, 3 * sizeof(uint64_t) * (size_t)npix, &tags_p));
+        NSX_TRY(ctx->ws(WS_DT_ENT, (size_t)cap * (4 * sizeof(uint32_t) + sizeof(uint64_t) + 2), &ent_p));
         NSX_TRY(ctx->ws(WS_DT_PART, sizeof(uint32_t) * (size_t)nchunk, &part_p));
         NSX_TRY(ctx->ws(WS_CCL_MISC, sizeof(int32_t) * 64, &misc_p));
         // Tags only ever grow, so list heads and pushers left by earlier calls
@@ -654,13 +848,15 @@ nsx_status dist_queue_run(nsx_ctx* ctx, const View& mask, const View& out, bool 
         Dtq Q;
         Q.S = (uint32_t*)lab_p;
         Q.head = (uint64_t*)tags_p;
-        Q.win = Q.head + npix;
+        Q.head2 = Q.head + npix;
+        Q.win = Q.head2 + npix;
         Q.nxt = (uint64_t*)ent_p;
         Q.E[0] = (uint32_t*)(Q.nxt + cap);
         Q.E[1] = Q.E[0] + cap;
         Q.c[0] = Q.E[1] + cap;
         Q.c[1] = Q.c[0] + cap;
         Q.mask = (uint8_t*)(Q.c[1] + cap);
+        Q.ext = Q.mask + cap;
         Q.part = (uint32_t*)part_p;
         Q.ctl = (int32_t*)misc_p;
         Q.rows = rows;
@@ -669,19 +865,21 @@ nsx_status dist_queue_run(nsx_ctx* ctx, const View& mask, const View& out, bool 
         Q.tag0 = ctx->dtq_tag;
         Q.cap = (uint32_t)cap;
         NSX_HIP_TRY(ctx, hipMemsetAsync(Q.ctl, 0, sizeof(int32_t) * 64, s));
-        hipLaunchKernelGGL(dtq_front_count_kernel, dim3((unsigned)nfront), dim3(256), 0, s, Q, mask.data, mask.pitch,
-                           nseg);
+        const unsigned fblocks = (unsigned)((nfront + 3) / 4);
+        hipLaunchKernelGGL(dtq_front_count_kernel, dim3(fblocks), dim3(1024), 0, s, Q, mask.data, mask.pitch, nseg,
+                           (int)nfront);
         hipLaunchKernelGGL(dtq_scan_kernel, dim3(1), dim3(1024), 0, s, Q, -1, (int)nfront, (int)QC_LEN0);
-        hipLaunchKernelGGL(dtq_front_scatter_kernel, dim3((unsigned)nfront), dim3(256), 0, s, Q, mask.data,
-                           mask.pitch, nseg);
+        hipLaunchKernelGGL(dtq_front_scatter_kernel, dim3(fblocks), dim3(1024), 0, s, Q, mask.data, mask.pitch,
+                           nseg, (int)nfront);
         NSX_HIP_TRY(ctx, hipGetLastError());
         auto generation = [&](int g, int solve, bool resumed) {
             if (!resumed) hipLaunchKernelGGL(dtq_link_kernel, dim3(QNB), dim3(256), 0, s, Q, g);
             if (g > 0)
-                for (int k = 0; k < solve; ++k) hipLaunchKernelGGL(dtq_solve_kernel, dim3(QNB), dim3(256), 0, s, Q, g, k);
-            hipLaunchKernelGGL(dtq_push_kernel, dim3(QNB), dim3(256), 0, s, Q, g);
+                for (int k = 0; k < solve; ++k)
+                    hipLaunchKernelGGL(dtq_solve_kernel, dim3(QNB), dim3(QCH), 0, s, Q, g, k);
+            hipLaunchKernelGGL(dtq_push_kernel, dim3(QNP), dim3(256), 0, s, Q, g);
             hipLaunchKernelGGL(dtq_scan_kernel, dim3(1), dim3(1024), 0, s, Q, g & 1, 0, (g & 1) ^ 1);
-            hipLaunchKernelGGL(dtq_scatter_kernel, dim3(QNB), dim3(256), 0, s, Q, g);
+            hipLaunchKernelGGL(dtq_scatter_kernel, dim3(QNP), dim3(256), 0, s, Q, g);
         };
         // Every generation moves some label strictly closer, so the number of
         // generations is bounded; the tag space bounds the loop here.
@@ -718,7 +916,7 @@ nsx_status dist_queue_run(nsx_ctx* ctx, const View& mask, const View& out, bool 
             if (len == 0) break;
             if ((uint64_t)ctx->dtq_tag + (uint64_t)g + 64 > 0xFFFFFFF0ull)
                 return ctx->fail(NSX_ERR_NOT_CONVERGED, "distanceTransform: generation tags exhausted");
-            batch = std::min(2 * batch, 32);
+            batch = 8;
         }
         ctx->dtq_tag += (uint32_t)g + 1;
         if (grow) {

@@ -88,7 +88,7 @@ struct nsx_ctx {
     int engine = NSX_ENGINE_AUTO;     // NSX_TUNE_ENGINE
     int fill_scans = 0;               // NSX_TUNE_FILL_SCANS
     int drain = 0;                    // NSX_TUNE_DRAIN
-    int dt_solve = 1;                 // NSX_TUNE_DT_SOLVE
+    int dt_solve = 2;                 // NSX_TUNE_DT_SOLVE
     int64_t drain_failures = 0;       // drains that gave up (restarted with per-round launches)
 
     // Options (nsx_ctx_set_option).

@@ -146,7 +146,8 @@ nsx_status nsx_ctx_read_tile_counts(nsx_ctx* ctx, int64_t* first_pass, int64_t* 
  * launches; nsx_batch_autotune picks per tile group.  Counterpart of the
  * reference's persistent queue kernel (global_queue.cu:746-857). */
 /* NSX_TUNE_DT_SOLVE: relaxation passes nsx_distance_transform launches per
- * queue generation (NSX_DT_ORDER_QUEUE) before checking, 0..8, default 1.  A
+ * queue generation (NSX_DT_ORDER_QUEUE) before checking, 0..8, default 2 (the
+ * second pass only revisits entries that read labels of other chunks).  A
  * generation whose labels the passes did not settle is found by the check and
  * finished with more passes after a host round trip; results do not depend on
  * this value. */
@@ -276,9 +277,10 @@ nsx_status nsx_local_minima(nsx_ctx* ctx, const nsx_img* image, nsx_img* out, in
  * labels (a neighbour's label is taken when strictly closer after rounding the
  * distance to float); calc_dist == 0: out is int32, those labels as raster
  * indices (rows*cols*3 where no background was found, the reference's initial
- * value).  Where backgrounds tie the index may differ from the reference's,
- * whose choice follows its queue order; the distances do not.  rows and cols
- * must be below 65535 and rows*cols*3 must fit int32. */
+ * value).  In the default queue order (NSX_OPT_DT_ORDER) labels and
+ * distances are the reference's, ties and pixels the propagation leaves short
+ * of the nearest background included.  rows and cols must be below 65535 and
+ * rows*cols*3 must fit int32. */
 nsx_status nsx_distance_transform(nsx_ctx* ctx, const nsx_img* mask, nsx_img* out, int calc_dist, void* stream);
 
 /* ConnComponents::boundingBox (ConnComponents.cpp:241-284): for the labels of
