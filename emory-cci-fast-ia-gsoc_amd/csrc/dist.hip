@@ -264,7 +264,8 @@ constexpr int QNB = 512;    // blocks of the solve kernel, looping over chunks
 constexpr int QNP = 2048;   // blocks of the push and scatter kernels, looping over chunks
 constexpr int QMAXSOLVE = 16;
 // control words
-enum { QC_LEN0 = 0, QC_LEN1 = 1, QC_STALL = 2, QC_ERR = 3, QC_CHG = 8 };   // QC_CHG + solve pass
+// QC_NEED: entries a generation that did not fit needs
+enum { QC_LEN0 = 0, QC_LEN1 = 1, QC_STALL = 2, QC_ERR = 3, QC_NEED = 4, QC_CHG = 8 };   // QC_CHG + solve pass
 
 struct Dtq {
     uint32_t* S;       // label of every pixel
@@ -508,6 +509,7 @@ __global__ __launch_bounds__(1024) void dtq_scan_kernel(Dtq Q, int len_slot, int
     }
     if (threadIdx.x == 1023) {
         if (run > Q.cap) {
+            Q.ctl[QC_NEED] = (int32_t)min(run, (uint64_t)INT32_MAX);
             Q.ctl[QC_ERR] = 1;
             Q.ctl[QC_STALL] = -1;
         } else {
@@ -826,15 +828,20 @@ nsx_status dist_queue_run(nsx_ctx* ctx, const View& mask, const View& out, bool 
     const int kSolve = std::min(std::max(ctx->dt_solve, 0), 8);
     constexpr int kResumeSolve = 8;
     static_assert(kResumeSolve <= QMAXSOLVE, "change words");
-    // Entries a generation buffer holds; a longer generation (a dense front)
-    // starts the call over with twice the room.
+    // Entries a generation buffer holds (at least what the workspace already
+    // has room for); a longer generation (a dense front) starts the call over
+    // with room for it.
+    constexpr size_t kEntryBytes = 4 * sizeof(uint32_t) + sizeof(uint64_t) + 2;
     int64_t cap = npix / 4 + 4096;
+    if ((int)ctx->bufs.size() > WS_DT_ENT)
+        cap = std::min<int64_t>(std::max<int64_t>(cap, (int64_t)(ctx->bufs[WS_DT_ENT].bytes / kEntryBytes)),
+                                (int64_t)1 << 30);
     for (;;) {
         const int64_t nchunk = std::max<int64_t>(nfront, (cap + QPC - 1) / QPC);
         void *lab_p, *tags_p, *ent_p, *part_p, *misc_p;
         NSX_TRY(ctx->ws(WS_CCL_LABEL, sizeof(uint32_t) * (size_t)npix, &lab_p));
         NSX_TRY(ctx->ws(WS_DT_TAGS, 3 * sizeof(uint64_t) * (size_t)npix, &tags_p));
-        NSX_TRY(ctx->ws(WS_DT_ENT, (size_t)cap * (4 * sizeof(uint32_t) + sizeof(uint64_t) + 2), &ent_p));
+        NSX_TRY(ctx->ws(WS_DT_ENT, (size_t)cap * kEntryBytes, &ent_p));
         NSX_TRY(ctx->ws(WS_DT_PART, sizeof(uint32_t) * (size_t)nchunk, &part_p));
         NSX_TRY(ctx->ws(WS_CCL_MISC, sizeof(int32_t) * 64, &misc_p));
         // Tags only ever grow, so list heads and pushers left by earlier calls
@@ -885,10 +892,11 @@ nsx_status dist_queue_run(nsx_ctx* ctx, const View& mask, const View& out, bool 
         // generations is bounded; the tag space bounds the loop here.
         int g = 0, batch = 4;
         bool grow = false;
+        int64_t need = 0;
         for (;;) {
             for (int b = 0; b < batch; ++b, ++g) generation(g, kSolve, false);
             NSX_HIP_TRY(ctx, hipGetLastError());
-            NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, Q.ctl, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
+            NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, Q.ctl, sizeof(int32_t) * 5, hipMemcpyDeviceToHost, s));
             NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
             int32_t len = ctx->h_pinned[40 + (g & 1)], stall = ctx->h_pinned[42], err = ctx->h_pinned[43];
             while (stall > 0 && err == 0) {
@@ -902,7 +910,7 @@ nsx_status dist_queue_run(nsx_ctx* ctx, const View& mask, const View& out, bool 
                 ++g;
                 NSX_HIP_TRY(ctx, hipGetLastError());
                 NSX_HIP_TRY(ctx,
-                            hipMemcpyAsync(ctx->h_pinned + 40, Q.ctl, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
+                            hipMemcpyAsync(ctx->h_pinned + 40, Q.ctl, sizeof(int32_t) * 5, hipMemcpyDeviceToHost, s));
                 NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
                 len = ctx->h_pinned[40 + (g & 1)];
                 stall = ctx->h_pinned[42];
@@ -910,6 +918,7 @@ nsx_status dist_queue_run(nsx_ctx* ctx, const View& mask, const View& out, bool 
             }
             if (err == 1) {
                 grow = true;
+                need = ctx->h_pinned[40 + QC_NEED];
                 break;
             }
             if (err != 0) return ctx->fail(NSX_ERR_NOT_CONVERGED, "distanceTransform: queue generations inconsistent");
@@ -921,10 +930,11 @@ nsx_status dist_queue_run(nsx_ctx* ctx, const View& mask, const View& out, bool 
         ctx->dtq_tag += (uint32_t)g + 1;
         if (grow) {
             // a generation longer than the entry buffers: start over with room
-            // for twice as many entries
-            if (cap > (int64_t)1 << 30)
+            // for it
+            const int64_t want = std::max<int64_t>(2 * cap, need + need / 4);
+            if (want > (int64_t)1 << 30)
                 return ctx->fail(NSX_ERR_NOMEM, "distanceTransform: queue generation beyond 2^30 entries");
-            cap *= 2;
+            cap = want;
             continue;
         }
         const dim3 grid((cols + 255) / 256, rows);
