@@ -960,7 +960,8 @@ def bench_ops(args):
         "roofline": None,
         "cpu_baseline": None if orc is None else {
             "value": round(len(ops) * npx / (sum(r["cpu_ms"] for r in res.values()) * 1e-3) / 1e6, 3),
-            "unit": "Mpix/s", "cores": 1, "kind": "port", "sample": "each operator once on the same image"},
+            "unit": "Mpix/s", "cores": 1, "kind": "port", "sample": "each operator once on the same image",
+            "host_cpus": os.cpu_count(), "cpu_model": cpu_model()},
     }))
 
 
