@@ -997,8 +997,8 @@ TILE_PX = 62 * 62   # interior pixels of one reconstruction tile visit
 # launch of the kind.  FETCH_SIZE is scaled by 2.0 and WRITE_SIZE by 1.0: the
 # factors that make `--workload copy` (256 MiB read + 256 MiB written per
 # launch) read true.  Commands and raw figures: profiles/traffic_chain4096.md.
-TRAFFIC_CHAIN4096 = {"recon_round": 13130567, "recon_first": 27317364, "ccl_local": 50975703,
-                     "ccl_final": 101438946, "ccl_merge": 36879925}
+TRAFFIC_CHAIN4096 = {"recon_round": 13146098, "recon_first": 27326407, "ccl_local": 50972672,
+                     "ccl_final": 101438818, "ccl_merge": 36901740}
 
 
 def border_tiles(rows: int, cols: int) -> int:
