@@ -41,6 +41,7 @@ int main(int argc, char** argv) {
         save(dir + "/lmax8.bin", nscale::localMaxima<unsigned char>(mask, 8), 1);
         save(dir + "/lmin4.bin", nscale::localMinima<unsigned char>(mask, 4), 1);
         save(dir + "/dist.bin", nscale::distanceTransform(bin), 4);
+        save(dir + "/distnn.bin", nscale::distanceTransform(bin, false), 4);
         save(dir + "/label8.bin", nscale::bwlabel2(bin, 8, true), 4);
         int count = 0;
         save(dir + "/area8.bin", nscale::bwareaopen2(bin, false, false, 3, 40, 8, count), 4);

@@ -65,7 +65,9 @@ def test_dropin_matches_oracle(orc, tmp_path):
     assert np.array_equal(load("hmin8.bin", np.uint8), orc.imhmin(mask, 20, 8))
     assert np.array_equal(load("lmax8.bin", np.uint8), orc.local_extrema(mask, 8, False))
     assert np.array_equal(load("lmin4.bin", np.uint8), orc.local_extrema(mask, 4, True))
-    assert np.array_equal(load("dist.bin", np.float32), orc.distance_transform(binimg)[1])
+    want_nn, want_dist = orc.distance_transform(binimg)
+    assert np.array_equal(load("dist.bin", np.float32), want_dist)
+    assert np.array_equal(load("distnn.bin", np.int32), want_nn)
     assert np.array_equal(load("label8.bin", np.int32), orc.bwlabel2(binimg, 8, True)[0])
     want_area, want_n = orc.bwareaopen2(binimg, False, False, 3, 40, 8)
     assert np.array_equal(load("area8.bin", np.int32), want_area)
