@@ -243,12 +243,18 @@ __device__ __forceinline__ uint32_t med3_chg(uint32_t a, uint32_t b, uint32_t c,
     return r;
 }
 
-// Forward sweep over registers 1..TS-1 then backward over TS-2..0.  Returns
-// whether any of the lane's registers changed.  Values only rise, so for u8
-// keys a lane changed iff the sum of its registers rose (old values summed on
-// the forward sweep, new ones on the backward sweep; 64 x 255 cannot wrap);
-// 32-bit keys OR the differences instead.  The lane's mask column (row in
-// orientation B) is read from LDS once, before both sweeps.
+// A downward and an upward sweep as two independent chains of TS/2 steps each
+// (a visit's time is its dependent chains, not VALU issue): first the top half
+// downwards with the bottom half upwards, then each on through the other half.
+// Every row is revisited with its neighbour's newer value, so a pair that
+// changes nothing still means the tile is a fixed point along this axis; a
+// pair moves a value across the tile as before, but a path that turns back
+// inside the bottom half waits for the next pair.  Returns whether any of the
+// lane's registers changed.  Values only rise, so for u8 keys a lane changed
+// iff the sum of its registers rose (old values summed in the first half-step
+// of each row, new ones in the second; 64 x 255 cannot wrap); 32-bit keys OR
+// the differences instead.  The lane's mask column (row in orientation B) is
+// read from LDS once, before both sweeps.
 template <int CONN, bool B, typename LT>
 __device__ __forceinline__ bool sweep_pair(uint32_t (&c)[TS], const LT* mb) {
     uint32_t m[TS];
@@ -257,19 +263,30 @@ __device__ __forceinline__ bool sweep_pair(uint32_t (&c)[TS], const LT* mb) {
 #pragma unroll
     for (int q = 0; q < TS / LOAD_CH; ++q) arrived(*reinterpret_cast<uint32_t(*)[LOAD_CH]>(&m[q * LOAD_CH]));
     constexpr bool kSum = sizeof(LT) == 1;
-    uint32_t before = kSum ? c[0] : 0u, after = 0;
-    uint32_t p = c[0];
+    constexpr int H = TS / 2;
+    // Two independent chains at a time: first the top half downwards and the
+    // bottom half upwards, then each on through the other half (down through
+    // the bottom half from the top half's last row, up through the top half
+    // from the bottom half's first row as the first phase left it).
+    uint32_t before = kSum ? c[0] + c[TS - 1] : 0u, after = 0;
+    uint32_t pd = c[0], pu = c[TS - 1];
 #pragma unroll
-    for (int k = 1; k < TS; ++k) {
-        c[k] = kSum ? med3_sum_old(nbr<CONN>(p), c[k], m[k], before) : med3_chg(nbr<CONN>(p), c[k], m[k], before);
-        p = c[k];
+    for (int s = 1; s < H; ++s) {
+        const int kd = s, ku = TS - 1 - s;
+        c[kd] = kSum ? med3_sum_old(nbr<CONN>(pd), c[kd], m[kd], before) : med3_chg(nbr<CONN>(pd), c[kd], m[kd], before);
+        c[ku] = kSum ? med3_sum_old(nbr<CONN>(pu), c[ku], m[ku], before) : med3_chg(nbr<CONN>(pu), c[ku], m[ku], before);
+        pd = c[kd];
+        pu = c[ku];
     }
-    if (kSum) after += c[TS - 1];
-    p = c[TS - 1];
+    pd = c[H - 1];
+    pu = c[H];
 #pragma unroll
-    for (int k = TS - 2; k >= 0; --k) {
-        c[k] = kSum ? med3_sum_new(nbr<CONN>(p), c[k], m[k], after) : med3_chg(nbr<CONN>(p), c[k], m[k], after);
-        p = c[k];
+    for (int s = 0; s < H; ++s) {
+        const int kd = H + s, ku = H - 1 - s;
+        c[kd] = kSum ? med3_sum_new(nbr<CONN>(pd), c[kd], m[kd], after) : med3_chg(nbr<CONN>(pd), c[kd], m[kd], after);
+        c[ku] = kSum ? med3_sum_new(nbr<CONN>(pu), c[ku], m[ku], after) : med3_chg(nbr<CONN>(pu), c[ku], m[ku], after);
+        pd = c[kd];
+        pu = c[ku];
     }
     return kSum ? after != before : (before | after) != 0u;
 }
@@ -278,18 +295,26 @@ __device__ __forceinline__ bool sweep_pair(uint32_t (&c)[TS], const LT* mb) {
 // registers for the whole visit (no LDS mask reads).
 template <int CONN>
 __device__ __forceinline__ bool sweep_pair_reg(uint32_t (&c)[TS], const uint32_t (&m)[TS]) {
+    constexpr int H = TS / 2;
     uint32_t chg = 0;
-    uint32_t p = c[0];
+    uint32_t pd = c[0], pu = c[TS - 1];
 #pragma unroll
-    for (int k = 1; k < TS; ++k) {
-        c[k] = med3_chg(nbr<CONN>(p), c[k], m[k], chg);
-        p = c[k];
+    for (int s = 1; s < H; ++s) {
+        const int kd = s, ku = TS - 1 - s;
+        c[kd] = med3_chg(nbr<CONN>(pd), c[kd], m[kd], chg);
+        c[ku] = med3_chg(nbr<CONN>(pu), c[ku], m[ku], chg);
+        pd = c[kd];
+        pu = c[ku];
     }
-    p = c[TS - 1];
+    pd = c[H - 1];
+    pu = c[H];
 #pragma unroll
-    for (int k = TS - 2; k >= 0; --k) {
-        c[k] = med3_chg(nbr<CONN>(p), c[k], m[k], chg);
-        p = c[k];
+    for (int s = 0; s < H; ++s) {
+        const int kd = H + s, ku = H - 1 - s;
+        c[kd] = med3_chg(nbr<CONN>(pd), c[kd], m[kd], chg);
+        c[ku] = med3_chg(nbr<CONN>(pu), c[ku], m[ku], chg);
+        pd = c[kd];
+        pu = c[ku];
     }
     return chg != 0u;
 }
