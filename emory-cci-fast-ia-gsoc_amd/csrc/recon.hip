@@ -1283,8 +1283,9 @@ __global__ void chain_status_kernel(int32_t* out, const int32_t* ncomp, const in
 // independent chains); after one barrier each of them clamps its entry value
 // through the composed blocks on the way and reruns its registers.  A 63-step
 // dependent chain becomes 16 + up to 3 + 16 steps on four SIMDs.  Hole filling
-// with 4-connectivity uses this: its rounds hold few tiles each, so a round
-// lasts as long as one visit's dependent chain.
+// with 4-connectivity uses this, and 4-connected reconstruction for its rounds
+// after the first pass: such rounds hold few tiles each, so a round lasts as
+// long as one visit's dependent chain.
 constexpr int QW = 4;
 constexpr int QR = TS / QW;
 
@@ -1659,6 +1660,13 @@ bool pick(int conn, RoundFns* f) {
         f->first = launch_round<T, MODE, 8, true>;
         f->later = launch_round<T, MODE, 8, false>;
         f->drain = launch_drain<T, MODE, 8>;
+    } else if constexpr (!Codec<T, MODE>::kMarkerLater) {
+        // 4-connected reconstruction: the first pass (every tile) on one-wave
+        // tiles, the later rounds (few tiles, a visit's chain sets the time)
+        // on four-wave tiles, as the hole filling's
+        f->first = launch_round<T, MODE, 4, true>;
+        f->later = launch_quad<T, MODE, false>;
+        f->drain = launch_quad_drain<T, MODE>;
     } else {
         f->first = launch_round<T, MODE, 4, true>;
         f->later = launch_round<T, MODE, 4, false>;
