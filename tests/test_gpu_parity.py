@@ -606,6 +606,44 @@ def _check_stuck_tiles(nh, orc, ctx):
     assert np.array_equal(nh.distanceTransform(m, ctx=ctx), got)
 
 
+def test_distance_transform_random(nh, orc, gpu):
+    """Queue order against the oracle on 30 random masks: shapes from 1 to 320
+    pixels a side, background densities from 1e-4 to 0.9, blobs, diagonal
+    stripes, rings (many equally near backgrounds) and a few isolated
+    backgrounds; labels and distances bit for bit.  The tile order, a fixed
+    point of the same rule, is never nearer than the exact distance."""
+    from scipy import ndimage
+    rng = np.random.default_rng(0x5EED0D7)
+    tiles = _dt_tiles_ctx(nh)
+    try:
+        for case in range(30):
+            h, w = int(rng.integers(1, 321)), int(rng.integers(1, 321))
+            kind = case % 5
+            if kind == 0:
+                m = np.where(rng.random((h, w)) < 10 ** rng.uniform(-4, -0.05), 0, 255)
+            elif kind == 1:
+                m = np.where(ndimage.gaussian_filter(rng.random((h, w)), 2.5) > 0.5, 255, 0)
+            elif kind == 2:
+                yy, xx = np.mgrid[0:h, 0:w]
+                m = np.where(((xx + 2 * yy) // int(rng.integers(2, 9))) % 3 == 0, 0, 255)
+            elif kind == 3:
+                yy, xx = np.mgrid[0:h, 0:w]
+                r = np.hypot(yy - h / 2, xx - w / 2)
+                m = np.where((r.astype(np.int64) % int(rng.integers(3, 12))) == 0, 0, 255)
+            else:
+                m = np.full((h, w), 255)
+                m.reshape(-1)[rng.integers(0, h * w, size=int(rng.integers(1, 4)))] = 0
+            m = m.astype(np.uint8)
+            want_nn, want = orc.distance_transform(m)
+            assert np.array_equal(nh.distanceTransform(m, False), want_nn), (case, m.shape)
+            assert np.array_equal(nh.distanceTransform(m), want), (case, m.shape)
+            got_t = nh.distanceTransform(m, ctx=tiles)
+            exact, _ = brute.exact_distance_with_far(m)
+            assert (got_t >= exact).all(), (case, m.shape)
+    finally:
+        tiles.close()
+
+
 @pytest.mark.parametrize("solve", [0, 8])
 def test_distance_transform_solve_passes(nh, orc, gpu, solve):
     """The queue order with no solve pass launched per generation (every
