@@ -108,15 +108,17 @@ template <typename T> struct Codec<T, RM_RECON> {
 
 // imreconstructBinary: key = reached (0/1) in the set mask != 0, seeded by
 // marker != 0 && mask != 0; nonzero marker pixels keep their value
-// (MorphologicOperations.cpp:1665-1672,1712-1756).
+// (MorphologicOperations.cpp:1665-1672,1712-1756).  The first pass stores
+// the marker's value at its nonzero pixels and max<T> at the others it
+// reached; a stored nonzero value inside the mask is then a reached pixel, so
+// later rounds decode the key from state and mask alone, and write max<T> at
+// the pixels a visit newly reached and nothing elsewhere (marker values stay).
 template <typename T> struct Codec<T, RM_BINARY> {
-    static constexpr bool kMarkerFirst = true, kMarkerLater = true;
+    static constexpr bool kMarkerFirst = true, kMarkerLater = false;
     __device__ static uint32_t frame() { return 0u; }
     __device__ static uint32_t mkey(T, T m) { return m != T(0) ? 1u : 0u; }
     __device__ static uint32_t first(T a, T m) { return (a != T(0) && m != T(0)) ? 1u : 0u; }
-    __device__ static uint32_t dec(T s, T a, T m) {
-        return a != T(0) ? (m != T(0) ? 1u : 0u) : (s != T(0) ? 1u : 0u);
-    }
+    __device__ static uint32_t dec(T s, T, T m) { return (s != T(0) && m != T(0)) ? 1u : 0u; }
     __device__ static T enc(uint32_t k, T a) { return a != T(0) ? a : (k ? tmax<T>() : T(0)); }
 };
 
@@ -510,8 +512,9 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
     const int gx = x0 + lane;
     const bool col_in = (unsigned)gx < (unsigned)D.cols;
     const int cx = min(max(gx, 0), D.cols - 1);
-    // u8 images other than imreconstructBinary (whose later rounds read three
-    // images) load in 16-byte blocks through the staging buffer tb.
+    // u8 images other than imreconstructBinary load in 16-byte blocks through
+    // the staging buffer tb.  (Binary u8 through the blocks measured slower:
+    // its later rounds spill 82 registers at 3 waves per SIMD.)
     constexpr bool kWide = sizeof(T) == 1 && MODE != RM_BINARY;
     if constexpr (kWide) {
         const bool kSec = FIRST ? C::kMarkerFirst : true;   // marker (first pass) or state
@@ -627,6 +630,12 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
         }
     }
     }
+    // imreconstructBinary, later rounds: the pixels reached before the visit
+    uint64_t oldb = 0;
+    if constexpr (MODE == RM_BINARY && !FIRST) {
+#pragma unroll
+        for (int k = 1; k <= TI; ++k) oldb |= (uint64_t)(c[k] != 0u) << k;
+    }
     if constexpr (kRegMask) {
 #pragma unroll
         for (int k = 0; k < TS; ++k) mB[k] = mA[k];
@@ -695,8 +704,12 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
             for (int k = 1; k <= TI; ++k) {
                 const int gy = y0 + k;
                 if (gy < D.rows) {
-                    const T av = (MODE == RM_BINARY) ? ld<T>(D.marker, D.pm, gy, gx) : T(0);
-                    st<T>(D.state, D.ps, gy, gx, C::enc(c[k], av));
+                    if constexpr (MODE == RM_BINARY && !FIRST) {
+                        if (c[k] != 0u && !((oldb >> k) & 1u)) st<T>(D.state, D.ps, gy, gx, tmax<T>());
+                    } else {
+                        const T av = (MODE == RM_BINARY) ? ld<T>(D.marker, D.pm, gy, gx) : T(0);
+                        st<T>(D.state, D.ps, gy, gx, C::enc(c[k], av));
+                    }
                 }
             }
         }
@@ -718,9 +731,10 @@ __device__ __forceinline__ void process_tile(const ReconArgs& A, const ImgDesc& 
 // 10.3 KB of LDS and run 3 waves per SIMD (at most 168 VGPRs); 32-bit tiles
 // run 2 (at most 256 VGPRs).  f32 later rounds run 1 wave per SIMD: at 2 they
 // spilled 43 registers, at 1 they take 296 with none (batch 29.37 -> 28.91 ms
-// in a same-session A/B).  Binary reconstruction's later rounds read three
-// images per tile; they run 2 (u8: 14 spills instead of 49) or 1 (int32: none
-// instead of 99): bwselect 0.354 -> 0.340 ms at 4096^2.
+// in a same-session A/B).  Binary reconstruction's later rounds also keep a
+// bit mask of the pixels reached before the visit; they run 2 (u8: no spills
+// instead of 7) or 1 (int32: none instead of 7): a 2048^2 maze, 8-connected,
+// 0.92 ms against 0.95 at 3 and 2.
 template <typename T, int MODE, int CONN, bool FIRST>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(MODE == RM_BINARY && !FIRST ? (sizeof(T) == 1 ? 2 : 1)
@@ -1361,8 +1375,8 @@ __device__ __forceinline__ void qtranspose(uint32_t (&c)[QR], LT* tb, int w) {
     if (TAIL) __syncthreads();
 }
 
-// A visit of one tile by four waves (4-connectivity; modes without a marker
-// in later rounds, i.e. the hole-filling modes).
+// A visit of one tile by four waves (4-connectivity; every mode reads mask
+// and state only after the first pass).
 template <typename T, int MODE, bool FIRST>
 __device__ __forceinline__ void process_tile_q(const ReconArgs& A, const ImgDesc& D, int lt, int round,
                                                typename LdsT<T>::type* tb, QShared& Q) {
@@ -1393,6 +1407,12 @@ __device__ __forceinline__ void process_tile_q(const ReconArgs& A, const ImgDesc
         const uint32_t key = FIRST ? C::first(av, mv) : C::dec(unbits<T>(rs[k]), av, mv);
         c[k] = in ? key : F;
         mA[k] = in ? C::mkey(av, mv) : F;
+    }
+    // imreconstructBinary, later rounds: the pixels reached before the visit
+    uint32_t oldq = 0;
+    if constexpr (MODE == RM_BINARY && !FIRST) {
+#pragma unroll
+        for (int k = 0; k < QR; ++k) oldq |= (uint32_t)(c[k] != 0u) << k;
     }
 #pragma unroll
     for (int k = 0; k < QR; ++k) mB[k] = mA[k];
@@ -1487,7 +1507,13 @@ __device__ __forceinline__ void process_tile_q(const ReconArgs& A, const ImgDesc
 #pragma unroll
             for (int k = 0; k < QR; ++k) {
                 const int r = QR * w + k, gy = y0 + r;
-                if (r >= 1 && r <= TI && gy < D.rows) st<T>(D.state, D.ps, gy, gx, C::enc(c[k], T(0)));
+                if (r >= 1 && r <= TI && gy < D.rows) {
+                    if constexpr (MODE == RM_BINARY && !FIRST) {
+                        if (c[k] != 0u && !((oldq >> k) & 1u)) st<T>(D.state, D.ps, gy, gx, tmax<T>());
+                    } else {
+                        st<T>(D.state, D.ps, gy, gx, C::enc(c[k], T(0)));
+                    }
+                }
             }
         }
     }
@@ -1660,17 +1686,13 @@ bool pick(int conn, RoundFns* f) {
         f->first = launch_round<T, MODE, 8, true>;
         f->later = launch_round<T, MODE, 8, false>;
         f->drain = launch_drain<T, MODE, 8>;
-    } else if constexpr (!Codec<T, MODE>::kMarkerLater) {
+    } else {
         // 4-connected reconstruction: the first pass (every tile) on one-wave
         // tiles, the later rounds (few tiles, a visit's chain sets the time)
         // on four-wave tiles, as the hole filling's
         f->first = launch_round<T, MODE, 4, true>;
         f->later = launch_quad<T, MODE, false>;
         f->drain = launch_quad_drain<T, MODE>;
-    } else {
-        f->first = launch_round<T, MODE, 4, true>;
-        f->later = launch_round<T, MODE, 4, false>;
-        f->drain = launch_drain<T, MODE, 4>;
     }
     return true;
 }
