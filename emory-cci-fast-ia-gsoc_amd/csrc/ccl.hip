@@ -1421,6 +1421,114 @@ __global__ void lx_zero_kernel(int32_t* w) {
     if (threadIdx.x == 0) *w = 0;
 }
 
+// u8 images.  A candidate of value v >= 1 (the reconstruction of work - 1
+// stays at v - 1) has no strictly higher neighbour q: the reconstruction would
+// reach min(v, v(q) - 1) = v there.  Marks therefore only ever flood value-0
+// candidates, and every 0 pixel is a candidate (its marker saturates at 0).
+// A conn-connected set of 0 pixels that is not the whole image has a nonzero
+// neighbour, so it is dropped whole.  The result: 255 at the candidates of
+// value >= 1, and at the 0 pixels exactly when every pixel is 0.
+// LX_V pixels per thread along a row; `vec`: every row of every image starts
+// on a 16-byte boundary (one 16-byte access per thread).
+constexpr int LX_V = 16;
+
+struct LxBytes {
+    uint32_t w[4];
+};
+__device__ __forceinline__ LxBytes lx_load16(const char* row, int x, int cols, bool vec) {
+    LxBytes b;
+    if (vec && x + LX_V <= cols) {
+        const uint4 v = *reinterpret_cast<const uint4*>(row + x);
+        b.w[0] = v.x;
+        b.w[1] = v.y;
+        b.w[2] = v.z;
+        b.w[3] = v.w;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            uint32_t w = 0u;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int xx = x + 4 * i + j;
+                if (xx < cols) w |= (uint32_t)((const uint8_t*)row)[xx] << (8 * j);
+            }
+            b.w[i] = w;
+        }
+    }
+    return b;
+}
+__device__ __forceinline__ void lx_store16(char* row, int x, int cols, bool vec, const LxBytes& b) {
+    if (vec && x + LX_V <= cols) {
+        *reinterpret_cast<uint4*>(row + x) = make_uint4(b.w[0], b.w[1], b.w[2], b.w[3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int xx = x + 4 * i + j;
+                if (xx < cols) ((uint8_t*)row)[xx] = (uint8_t)(b.w[i] >> (8 * j));
+            }
+    }
+}
+// per byte: 0x80 where the byte is nonzero
+__device__ __forceinline__ uint32_t lx_nz_bytes(uint32_t w) {
+    return (w | ((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu)) & 0x80808080u;
+}
+
+// marker = work - 1 (saturating); work = 255 - image for minima (written then,
+// otherwise the image is the work image); *nonzero |= 1 if a work pixel is not 0.
+__global__ __launch_bounds__(256) void lx_prep_u8_kernel(const char* img, int64_t pi, char* work, int64_t pw,
+                                                         char* marker, int64_t pm, int rows, int cols, int minima,
+                                                         int vec, uint32_t* nonzero) {
+    const int x = (blockIdx.x * 256 + (int)threadIdx.x) * LX_V, y = blockIdx.y;
+    bool any = false;
+    if (x < cols) {
+        LxBytes b = lx_load16(img + (int64_t)y * pi, x, cols, vec != 0);
+        LxBytes m;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (minima) b.w[i] = ~b.w[i];
+            const uint32_t nz = lx_nz_bytes(b.w[i]);
+            m.w[i] = b.w[i] - (nz >> 7);   // no borrow: only nonzero bytes drop by 1
+            // bytes past the row end load as 0 (minima: 255): not counted
+            const int left = cols - x - 4 * i;
+            const uint32_t valid = left >= 4 ? 0xFFFFFFFFu : left <= 0 ? 0u : (1u << (8 * left)) - 1u;
+            any = any || (nz & valid) != 0u;
+        }
+        if (minima) lx_store16(work + (int64_t)y * pw, x, cols, vec != 0, b);
+        lx_store16(marker + (int64_t)y * pm, x, cols, vec != 0, m);
+    }
+    if (__syncthreads_or(any) && threadIdx.x == 0 && __hip_atomic_load(nonzero, __ATOMIC_RELAXED,
+                                                                        __HIP_MEMORY_SCOPE_AGENT) == 0u)
+        atomicOr(nonzero, 1u);
+}
+
+// out = 255 at candidates of value >= 1 (rec == work - 1), and at 0 pixels when
+// no work pixel is nonzero; 0 elsewhere.
+__global__ __launch_bounds__(256) void lx_out_u8_kernel(const char* img, int64_t pi, const char* rec, int64_t pr,
+                                                        char* out, int64_t po, int rows, int cols, int minima,
+                                                        int vec, const uint32_t* nonzero) {
+    const int x = (blockIdx.x * 256 + (int)threadIdx.x) * LX_V, y = blockIdx.y;
+    if (x >= cols) return;
+    const uint32_t zero_out = *nonzero ? 0u : 0xFFu;
+    const LxBytes b = lx_load16(img + (int64_t)y * pi, x, cols, vec != 0);
+    const LxBytes r = lx_load16(rec + (int64_t)y * pr, x, cols, vec != 0);
+    LxBytes o;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t w = minima ? ~b.w[i] : b.w[i];
+        uint32_t ow = 0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t v = (w >> (8 * j)) & 0xFFu, rv = (r.w[i] >> (8 * j)) & 0xFFu;
+            const uint32_t ob = v != 0u ? (rv == v - 1u ? 0xFFu : 0u) : zero_out;
+            ow |= ob << (8 * j);
+        }
+        o.w[i] = ow;
+    }
+    lx_store16(out + (int64_t)y * po, x, cols, vec != 0, o);
+}
+
 // ---- bounding boxes (ConnComponents::boundingBox) ------------------------------
 // misc: [0] raster index of the first label -1 or 0 that is not the background
 // (the reference stops there), [1] first raster index of a label outside
@@ -1718,6 +1826,27 @@ nsx_status local_extrema_run(nsx_ctx* ctx, const View& img, const View& out, int
     const View work{(char*)work_p, pitch, img.rows, img.cols, img.dtype};
     const View marker{(char*)marker_p, pitch, img.rows, img.cols, img.dtype};
     const View rec{(char*)rec_p, pitch, img.rows, img.cols, img.dtype};
+    if (img.dtype == NSX_U8) {
+        // u8: the marks' flood reduces to a test of the image (see
+        // lx_prep_u8_kernel); d_stats int32 word 56 holds "some work pixel
+        // is nonzero"
+        uint32_t* nonzero = (uint32_t*)ctx->d_stats + 56;
+        const View& wv = minima ? work : img;
+        const bool vec = ((((uintptr_t)img.data | (uintptr_t)img.pitch | (uintptr_t)work.data | (uintptr_t)pitch |
+                            (uintptr_t)marker.data | (uintptr_t)rec.data | (uintptr_t)out.data |
+                            (uintptr_t)out.pitch) & 15u) == 0u);
+        const dim3 vgrid((img.cols + 256 * LX_V - 1) / (256 * LX_V), img.rows);
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, nonzero, 1);
+        hipLaunchKernelGGL(lx_prep_u8_kernel, vgrid, dim3(256), 0, s, img.data, img.pitch, work.data, work.pitch,
+                           marker.data, marker.pitch, img.rows, img.cols, minima ? 1 : 0, vec ? 1 : 0, nonzero);
+        NSX_HIP_TRY(ctx, hipGetLastError());
+        NSX_TRY(recon_run(ctx, RM_RECON, &marker, wv, rec, conn, s));
+        hipLaunchKernelGGL(lx_out_u8_kernel, vgrid, dim3(256), 0, s, img.data, img.pitch, rec.data, rec.pitch,
+                           out.data, out.pitch, img.rows, img.cols, minima ? 1 : 0, vec ? 1 : 0,
+                           (const uint32_t*)nonzero);
+        NSX_HIP_TRY(ctx, hipGetLastError());
+        return NSX_OK;
+    }
     const dim3 grid((img.cols + 255) / 256, img.rows);
     hipLaunchKernelGGL(lx_prep_kernel, grid, dim3(256), 0, s, img.data, img.pitch, work.data, work.pitch, marker.data,
                        marker.pitch, img.rows, img.cols, img.dtype, minima ? 1 : 0);

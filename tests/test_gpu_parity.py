@@ -466,6 +466,30 @@ def test_local_extrema(nh, orc, gpu, conn):
             want = orc.local_extrema(img, conn, minima)
             fn = nh.localMinima if minima else nh.localMaxima
             assert np.array_equal(fn(img, conn), want), (end, minima)
+    # u8 images whose rows are 16-byte aligned (whole 16-pixel groups per
+    # thread), strided device views with a partial last group, and the 0 / 255
+    # plateaus that the u8 path settles without a flood
+    for shape in [(40, 512), (3, 4096), (17, 1040)]:
+        u = rng.integers(0, 256, size=shape).astype(np.uint8)
+        z = np.where(rng.random(shape) < 0.8, 0, rng.integers(1, 4, size=shape)).astype(np.uint8)
+        one = np.zeros(shape, np.uint8)
+        one[shape[0] // 2, shape[1] // 3] = 200
+        for img in (u, z, 255 - z, one, 255 - one, np.zeros(shape, np.uint8), np.full(shape, 255, np.uint8)):
+            for minima in (False, True):
+                fn = nh.localMinima if minima else nh.localMaxima
+                want = orc.local_extrema(img, conn, minima)
+                assert np.array_equal(fn(img, conn), want), (shape, minima)
+                assert np.array_equal(fn(torch.from_numpy(img).cuda(), conn).cpu().numpy(), want), (shape, minima)
+    base = torch.from_numpy(rng.integers(0, 3, size=(21, 1024)).astype(np.uint8) * 50).cuda()
+    obase = torch.zeros((21, 1024), dtype=torch.uint8, device="cuda")
+    for w in (1000, 1017, 1024):
+        img = base[:, :w]
+        for minima in (False, True):
+            fn = nh.localMinima if minima else nh.localMaxima
+            o = obase[:, :w]
+            fn(img, conn, out=o)
+            want = orc.local_extrema(np.ascontiguousarray(img.cpu().numpy()), conn, minima)
+            assert np.array_equal(o.cpu().numpy(), want), (w, minima)
     with pytest.raises(nh.NsxError):
         nh.localMaxima(np.zeros((3, 3), np.int32))
     with pytest.raises(nh.NsxError):

@@ -403,6 +403,43 @@ def test_local_extrema_closed_form(orc, conn):
                                   brute.local_maxima_closed_form(inv, conn))
 
 
+def _u8_extrema_no_flood(img, conn):
+    """u8 localMaxima without the marks' flood: 255 at pixels of value v >= 1
+    whose reconstruction of image - 1 stays at v - 1, and at the 0 pixels
+    exactly when the whole image is 0."""
+    v = img.astype(np.int32)
+    marker = np.maximum(v - 1, 0).astype(np.uint8)
+    rec = brute.recon_fixpoint(marker, img, conn).astype(np.int32)
+    keep = (v >= 1) & (rec == v - 1)
+    if not v.any():
+        keep = np.ones(img.shape, bool)
+    return np.where(keep, 255, 0).astype(np.uint8)
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_local_extrema_u8_without_flood(orc, conn):
+    """For u8 images the flood of marks over equal-valued candidates only ever
+    drops 0-valued pixels: a candidate of value v >= 1 has no strictly higher
+    neighbour, and a connected set of 0 pixels other than the whole image
+    touches a nonzero pixel.  The GPU's u8 path relies on this; here it is
+    checked against the oracle's flood on random, zero-heavy, plateau and
+    constant images, for maxima and minima."""
+    rng = np.random.default_rng(29 + conn)
+    for shape in [(1, 1), (1, 7), (7, 1), (5, 6), (12, 31), (33, 40)]:
+        imgs = [rng.integers(0, 256, size=shape).astype(np.uint8),
+                (rng.integers(0, 3, size=shape) * 60).astype(np.uint8),
+                np.where(rng.random(shape) < 0.9, 0, rng.integers(1, 4, size=shape)).astype(np.uint8),
+                np.where(rng.random(shape) < 0.9, 255, rng.integers(250, 255, size=shape)).astype(np.uint8),
+                np.zeros(shape, np.uint8), np.full(shape, 255, np.uint8), np.full(shape, 1, np.uint8)]
+        one = np.zeros(shape, np.uint8)
+        one[shape[0] // 2, shape[1] // 2] = 9
+        imgs.append(one)
+        for img in imgs:
+            assert np.array_equal(orc.local_extrema(img, conn), _u8_extrema_no_flood(img, conn)), (shape, img)
+            inv = (255 - img.astype(np.int32)).astype(np.uint8)
+            assert np.array_equal(orc.local_extrema(img, conn, minima=True), _u8_extrema_no_flood(inv, conn))
+
+
 def _dt_masks(rng):
     from scipy import ndimage
     out = []
