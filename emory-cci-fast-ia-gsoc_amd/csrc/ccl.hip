@@ -1427,7 +1427,12 @@ __global__ void lx_zero_kernel(int32_t* w) {
 // candidates, and every 0 pixel is a candidate (its marker saturates at 0).
 // A conn-connected set of 0 pixels that is not the whole image has a nonzero
 // neighbour, so it is dropped whole.  The result: 255 at the candidates of
-// value >= 1, and at the 0 pixels exactly when every pixel is 0.
+// value >= 1, and at the 0 pixels exactly when every pixel is 0.  A pixel of
+// value v >= 1 is a candidate exactly when no strictly higher pixel is reached
+// through pixels >= v, i.e. when its plateau (the conn-connected pixels of
+// value v around it) has no strictly higher neighbour: one labelling of equal
+// nonzero values, each plateau folding the largest of its pixels' "has a
+// higher neighbour" weights, replaces the reconstruction.
 // LX_V pixels per thread along a row; `vec`: every row of every image starts
 // on a 16-byte boundary (one 16-byte access per thread).
 constexpr int LX_V = 16;
@@ -1475,36 +1480,89 @@ __device__ __forceinline__ uint32_t lx_nz_bytes(uint32_t w) {
     return (w | ((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu)) & 0x80808080u;
 }
 
-// marker = work - 1 (saturating); work = 255 - image for minima (written then,
-// otherwise the image is the work image); *nonzero |= 1 if a work pixel is not 0.
-__global__ __launch_bounds__(256) void lx_prep_u8_kernel(const char* img, int64_t pi, char* work, int64_t pw,
-                                                         char* marker, int64_t pm, int rows, int cols, int minima,
-                                                         int vec, uint32_t* nonzero) {
+// Work value of byte i of a 16-pixel group, i = -1 and 16 the bytes beside it
+// (0 outside the row: pixels outside the image are never higher).
+struct LxRow {
+    uint32_t w[4];
+    uint32_t l, r;
+    __device__ __forceinline__ uint32_t at(int i) const {
+        return i < 0 ? l : i >= LX_V ? r : (w[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+    }
+};
+__device__ __forceinline__ LxRow lx_row(const char* img, int64_t pi, int y, int x, int rows, int cols, int minima,
+                                        bool vec) {
+    LxRow R;
+    if (y < 0 || y >= rows) {
+        R.w[0] = R.w[1] = R.w[2] = R.w[3] = R.l = R.r = 0u;
+        return R;
+    }
+    const char* row = img + (int64_t)y * pi;
+    const LxBytes b = lx_load16(row, x, cols, vec);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int left = cols - x - 4 * i;
+        const uint32_t valid = left >= 4 ? 0xFFFFFFFFu : left <= 0 ? 0u : (1u << (8 * left)) - 1u;
+        R.w[i] = (minima ? ~b.w[i] : b.w[i]) & valid;
+    }
+    const uint32_t fl = minima ? 0xFFu : 0u;
+    R.l = x > 0 ? (((const uint8_t*)row)[x - 1] ^ fl) : 0u;
+    R.r = x + LX_V < cols ? (((const uint8_t*)row)[x + LX_V] ^ fl) : 0u;
+    return R;
+}
+
+// Per group of 16 pixels of row y: weight = 1 where a work pixel of value >= 1
+// has a strictly higher conn-neighbour; the classes to label (the work value;
+// 0 for 0 and 255, since a 255 plateau never has a higher neighbour, and for a
+// pixel with no equal-valued neighbour, a plateau of its own); the fold plane
+// (0 where the labelling folds the plateau's weight in, the weight itself for
+// the single-pixel plateaus); *nonzero |= 1 if some work pixel is not 0.
+__global__ __launch_bounds__(256) void lx_prep_u8_kernel(const char* img, int64_t pi, char* cls, int64_t pc,
+                                                         char* weight, int64_t pwt, char* fold, int64_t pf, int rows,
+                                                         int cols, int minima, int conn, int vec, uint32_t* nonzero) {
     const int x = (blockIdx.x * 256 + (int)threadIdx.x) * LX_V, y = blockIdx.y;
     bool any = false;
     if (x < cols) {
-        LxBytes b = lx_load16(img + (int64_t)y * pi, x, cols, vec != 0);
-        LxBytes m;
+        const bool vc = vec != 0;
+        const LxRow up = lx_row(img, pi, y - 1, x, rows, cols, minima, vc);
+        const LxRow mid = lx_row(img, pi, y, x, rows, cols, minima, vc);
+        const LxRow dn = lx_row(img, pi, y + 1, x, rows, cols, minima, vc);
+        LxBytes wt, cb, fb;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            if (minima) b.w[i] = ~b.w[i];
-            const uint32_t nz = lx_nz_bytes(b.w[i]);
-            m.w[i] = b.w[i] - (nz >> 7);   // no borrow: only nonzero bytes drop by 1
-            // bytes past the row end load as 0 (minima: 255): not counted
-            const int left = cols - x - 4 * i;
-            const uint32_t valid = left >= 4 ? 0xFFFFFFFFu : left <= 0 ? 0u : (1u << (8 * left)) - 1u;
-            any = any || (nz & valid) != 0u;
+            uint32_t ww = 0u, cw = 0u, fw = 0u;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = 4 * i + j;
+                const uint32_t v = mid.at(k);
+                uint32_t h = max(max(up.at(k), dn.at(k)), max(mid.at(k - 1), mid.at(k + 1)));
+                bool eq = up.at(k) == v || dn.at(k) == v || mid.at(k - 1) == v || mid.at(k + 1) == v;
+                if (conn == 8) {
+                    h = max(h, max(max(up.at(k - 1), up.at(k + 1)), max(dn.at(k - 1), dn.at(k + 1))));
+                    eq = eq || up.at(k - 1) == v || up.at(k + 1) == v || dn.at(k - 1) == v || dn.at(k + 1) == v;
+                }
+                const uint32_t wgt = (v != 0u && h > v) ? 1u : 0u;
+                const bool lab = eq && v != 0u && v != 0xFFu;
+                ww |= wgt << (8 * j);
+                cw |= (lab ? v : 0u) << (8 * j);
+                fw |= (lab ? 0u : wgt) << (8 * j);
+            }
+            wt.w[i] = ww;
+            cb.w[i] = cw;
+            fb.w[i] = fw;
+            any = any || mid.w[i] != 0u;
         }
-        if (minima) lx_store16(work + (int64_t)y * pw, x, cols, vec != 0, b);
-        lx_store16(marker + (int64_t)y * pm, x, cols, vec != 0, m);
+        lx_store16(weight + (int64_t)y * pwt, x, cols, vc, wt);
+        lx_store16(cls + (int64_t)y * pc, x, cols, vc, cb);
+        lx_store16(fold + (int64_t)y * pf, x, cols, vc, fb);
     }
     if (__syncthreads_or(any) && threadIdx.x == 0 && __hip_atomic_load(nonzero, __ATOMIC_RELAXED,
                                                                         __HIP_MEMORY_SCOPE_AGENT) == 0u)
         atomicOr(nonzero, 1u);
 }
 
-// out = 255 at candidates of value >= 1 (rec == work - 1), and at 0 pixels when
-// no work pixel is nonzero; 0 elsewhere.
+// out = 255 at work pixels of value 255, at those of value 1..254 whose plateau
+// took no weight (fold == 0), and at 0 pixels when no work pixel is nonzero;
+// 0 elsewhere.
 __global__ __launch_bounds__(256) void lx_out_u8_kernel(const char* img, int64_t pi, const char* rec, int64_t pr,
                                                         char* out, int64_t po, int rows, int cols, int minima,
                                                         int vec, const uint32_t* nonzero) {
@@ -1521,7 +1579,7 @@ __global__ __launch_bounds__(256) void lx_out_u8_kernel(const char* img, int64_t
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const uint32_t v = (w >> (8 * j)) & 0xFFu, rv = (r.w[i] >> (8 * j)) & 0xFFu;
-            const uint32_t ob = v != 0u ? (rv == v - 1u ? 0xFFu : 0u) : zero_out;
+            const uint32_t ob = v == 0xFFu ? 0xFFu : v != 0u ? (rv == 0u ? 0xFFu : 0u) : zero_out;
             ow |= ob << (8 * j);
         }
         o.w[i] = ow;
@@ -1827,21 +1885,23 @@ nsx_status local_extrema_run(nsx_ctx* ctx, const View& img, const View& out, int
     const View marker{(char*)marker_p, pitch, img.rows, img.cols, img.dtype};
     const View rec{(char*)rec_p, pitch, img.rows, img.cols, img.dtype};
     if (img.dtype == NSX_U8) {
-        // u8: the marks' flood reduces to a test of the image (see
-        // lx_prep_u8_kernel); d_stats int32 word 56 holds "some work pixel
-        // is nonzero"
+        // u8: plateaus without a higher neighbour (see lx_prep_u8_kernel);
+        // d_stats int32 word 56 holds "some work pixel is nonzero"
         uint32_t* nonzero = (uint32_t*)ctx->d_stats + 56;
-        const View& wv = minima ? work : img;
+        const View& cls = work;        // plateau classes
+        const View& weight = marker;   // 0/1 per pixel
+        const View& fold = rec;        // per plateau: largest weight
         const bool vec = ((((uintptr_t)img.data | (uintptr_t)img.pitch | (uintptr_t)work.data | (uintptr_t)pitch |
-                            (uintptr_t)marker.data | (uintptr_t)rec.data | (uintptr_t)out.data |
+                            (uintptr_t)weight.data | (uintptr_t)fold.data | (uintptr_t)out.data |
                             (uintptr_t)out.pitch) & 15u) == 0u);
         const dim3 vgrid((img.cols + 256 * LX_V - 1) / (256 * LX_V), img.rows);
         hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, nonzero, 1);
         hipLaunchKernelGGL(lx_prep_u8_kernel, vgrid, dim3(256), 0, s, img.data, img.pitch, work.data, work.pitch,
-                           marker.data, marker.pitch, img.rows, img.cols, minima ? 1 : 0, vec ? 1 : 0, nonzero);
+                           weight.data, weight.pitch, fold.data, fold.pitch, img.rows, img.cols, minima ? 1 : 0, conn,
+                           vec ? 1 : 0, nonzero);
         NSX_HIP_TRY(ctx, hipGetLastError());
-        NSX_TRY(recon_run(ctx, RM_RECON, &marker, wv, rec, conn, s));
-        hipLaunchKernelGGL(lx_out_u8_kernel, vgrid, dim3(256), 0, s, img.data, img.pitch, rec.data, rec.pitch,
+        NSX_TRY(ccl_class_fold_run(ctx, cls, weight, fold, conn, s));
+        hipLaunchKernelGGL(lx_out_u8_kernel, vgrid, dim3(256), 0, s, img.data, img.pitch, fold.data, fold.pitch,
                            out.data, out.pitch, img.rows, img.cols, minima ? 1 : 0, vec ? 1 : 0,
                            (const uint32_t*)nonzero);
         NSX_HIP_TRY(ctx, hipGetLastError());

@@ -416,14 +416,39 @@ def _u8_extrema_no_flood(img, conn):
     return np.where(keep, 255, 0).astype(np.uint8)
 
 
+def _u8_extrema_plateaus(img, conn):
+    """The same through plateaus, without a reconstruction: a pixel of value
+    v >= 1 reaches a strictly higher pixel through pixels >= v exactly when its
+    plateau (the conn-connected pixels of value v around it) has a strictly
+    higher neighbour.  255 on the nonzero plateaus with no such neighbour, and
+    on the 0 pixels when the whole image is 0."""
+    from scipy import ndimage
+    v = img.astype(np.int32)
+    pad = np.pad(v, 1, constant_values=-1)
+    H, W = v.shape
+    offs = [(-1, 0), (1, 0), (0, -1), (0, 1)] + ([(-1, -1), (-1, 1), (1, -1), (1, 1)] if conn == 8 else [])
+    higher = np.zeros(v.shape, bool)
+    for dy, dx in offs:
+        higher |= pad[1 + dy:1 + dy + H, 1 + dx:1 + dx + W] > v
+    keep = np.zeros(v.shape, bool)
+    for val in np.unique(v[v >= 1]):
+        lab, _ = ndimage.label(v == val, structure=brute.structure(conn))
+        bad = np.unique(lab[higher & (lab > 0)])
+        keep |= (lab > 0) & ~np.isin(lab, bad)
+    if not v.any():
+        keep = np.ones(v.shape, bool)
+    return np.where(keep, 255, 0).astype(np.uint8)
+
+
 @pytest.mark.parametrize("conn", [4, 8])
 def test_local_extrema_u8_without_flood(orc, conn):
     """For u8 images the flood of marks over equal-valued candidates only ever
     drops 0-valued pixels: a candidate of value v >= 1 has no strictly higher
     neighbour, and a connected set of 0 pixels other than the whole image
-    touches a nonzero pixel.  The GPU's u8 path relies on this; here it is
-    checked against the oracle's flood on random, zero-heavy, plateau and
-    constant images, for maxima and minima."""
+    touches a nonzero pixel.  The GPU's u8 path relies on this, in the plateau
+    form (one labelling of equal nonzero values); here both forms are checked
+    against the oracle's flood on random, zero-heavy, plateau and constant
+    images, for maxima and minima."""
     rng = np.random.default_rng(29 + conn)
     for shape in [(1, 1), (1, 7), (7, 1), (5, 6), (12, 31), (33, 40)]:
         imgs = [rng.integers(0, 256, size=shape).astype(np.uint8),
@@ -435,9 +460,13 @@ def test_local_extrema_u8_without_flood(orc, conn):
         one[shape[0] // 2, shape[1] // 2] = 9
         imgs.append(one)
         for img in imgs:
-            assert np.array_equal(orc.local_extrema(img, conn), _u8_extrema_no_flood(img, conn)), (shape, img)
+            want = orc.local_extrema(img, conn)
+            assert np.array_equal(want, _u8_extrema_no_flood(img, conn)), (shape, img)
+            assert np.array_equal(want, _u8_extrema_plateaus(img, conn)), (shape, img)
             inv = (255 - img.astype(np.int32)).astype(np.uint8)
-            assert np.array_equal(orc.local_extrema(img, conn, minima=True), _u8_extrema_no_flood(inv, conn))
+            want = orc.local_extrema(img, conn, minima=True)
+            assert np.array_equal(want, _u8_extrema_no_flood(inv, conn))
+            assert np.array_equal(want, _u8_extrema_plateaus(inv, conn))
 
 
 def _dt_masks(rng):
