@@ -1593,19 +1593,30 @@ __global__ __launch_bounds__(256) void lx_out_u8_kernel(const char* img, int64_t
 // [-1, rows*cols], [2] the largest label.  Blocks stride over rows; one
 // atomic per block and word.
 __global__ __launch_bounds__(256) void bbox_scan_kernel(const char* lab, int64_t pl, int rows, int cols, int bgval,
-                                                        int32_t* misc) {
+                                                        int vec, int32_t* misc) {
     __shared__ int32_t red[3][4];
     int32_t stop = INT_MAX, bad = INT_MAX, mx = 0;
     const int64_t lim = (int64_t)rows * cols;
+    auto see = [&](int32_t v, int32_t i) {
+        if (v == bgval) return;
+        if (v == -1 || v == 0) stop = min(stop, i);
+        else if (v < -1 || (int64_t)v > lim) bad = min(bad, i);
+        else mx = max(mx, v);
+    };
     for (int y = blockIdx.x; y < rows; y += gridDim.x) {
         const int32_t* row = (const int32_t*)(lab + (int64_t)y * pl);
-        for (int x = threadIdx.x; x < cols; x += 256) {
-            const int32_t v = row[x];
-            if (v == bgval) continue;
+        // four labels per thread (one 16-byte load when rows are 16-byte aligned)
+        for (int x = 4 * (int)threadIdx.x; x < cols; x += 1024) {
             const int32_t i = y * cols + x;
-            if (v == -1 || v == 0) stop = min(stop, i);
-            else if (v < -1 || (int64_t)v > lim) bad = min(bad, i);
-            else mx = max(mx, v);
+            if (vec && x + 4 <= cols) {
+                const int4 q = *reinterpret_cast<const int4*>(row + x);
+                see(q.x, i);
+                see(q.y, i + 1);
+                see(q.z, i + 2);
+                see(q.w, i + 3);
+            } else {
+                for (int k = 0; k < 4 && x + k < cols; ++k) see(row[x + k], i + k);
+            }
         }
     }
 #pragma unroll
@@ -1645,28 +1656,79 @@ __global__ void bbox_init_kernel(int32_t* tab, int64_t cap) {
     }
 }
 
-// Runs of one label along a row, within each wave's 64 columns: the run's
-// first lane updates the label's box once.  Pixels from the stop index on
-// count as background.
-__global__ void bbox_runs_kernel(const char* lab, int64_t pl, int rows, int cols, int bgval, const int32_t* misc,
-                                 int32_t* tab, int64_t cap) {
-    const int lane = threadIdx.x & 63;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    const int32_t i = y * cols + x;
-    int32_t v = bgval;
-    if (x < cols && i < misc[0]) v = ((const int32_t*)(lab + (int64_t)y * pl))[x];
-    const int32_t left = __shfl_up(v, 1, 64), right = __shfl_down(v, 1, 64);
-    const bool fg = v != bgval;
-    const bool start = fg && (lane == 0 || left != v);
-    const bool end = fg && (lane == 63 || right != v);
-    const uint64_t ends = __ballot(end);
-    if (start) {
-        const int xe = x + __builtin_ctzll(ends >> lane);
-        atomicMin(&tab[v], i);
-        atomicMin(&tab[cap + v], x);
-        atomicMax(&tab[2 * cap + v], xe);
-        atomicMin(&tab[3 * cap + v], y);
-        atomicMax(&tab[4 * cap + v], y);
+// Runs of one label along a row, within each wave's 64 columns; pixels from
+// the stop index on count as background.  A block takes a tile of 64 columns
+// and BB_ROWS rows: the run's first lane folds the run into the tile's LDS
+// table (open addressing on the label), and the table's entries update the
+// global boxes once per label and tile.  A run whose label finds no free slot
+// within BB_PROBE probes updates the global boxes itself.
+constexpr int BB_ROWS = 64;
+constexpr int BB_SLOTS = 256;
+constexpr int BB_PROBE = 16;
+__global__ __launch_bounds__(256) void bbox_runs_kernel(const char* lab, int64_t pl, int rows, int cols, int bgval,
+                                                        const int32_t* misc, int32_t* tab, int64_t cap) {
+    __shared__ int32_t key[BB_SLOTS];
+    __shared__ int32_t sb[5][BB_SLOTS];   // first raster index, min x, max x, min y, max y
+    constexpr int32_t kEmpty = -1;        // labels folded here are >= 1
+    for (int k = threadIdx.x; k < BB_SLOTS; k += 256) {
+        key[k] = kEmpty;
+        sb[0][k] = INT_MAX;
+        sb[1][k] = INT_MAX;
+        sb[2][k] = -1;
+        sb[3][k] = INT_MAX;
+        sb[4][k] = -1;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int x = blockIdx.x * 64 + lane;
+    const int32_t stop = misc[0];
+    for (int r = w; r < BB_ROWS; r += 4) {
+        const int y = blockIdx.y * BB_ROWS + r;
+        if (y >= rows) break;
+        const int32_t i = y * cols + x;
+        int32_t v = bgval;
+        if (x < cols && i < stop) v = ((const int32_t*)(lab + (int64_t)y * pl))[x];
+        const int32_t left = __shfl_up(v, 1, 64), right = __shfl_down(v, 1, 64);
+        const bool fg = v != bgval;
+        const bool start = fg && (lane == 0 || left != v);
+        const bool end = fg && (lane == 63 || right != v);
+        const uint64_t ends = __ballot(end);
+        if (start) {
+            const int xe = x + __builtin_ctzll(ends >> lane);
+            int slot = -1;
+            uint32_t h = ((uint32_t)v * 2654435761u) >> 24;   // 8 bits: BB_SLOTS
+            for (int p = 0; p < BB_PROBE; ++p, h = (h + 1) & (BB_SLOTS - 1)) {
+                int32_t k = key[h];
+                if (k == kEmpty) k = atomicCAS(&key[h], kEmpty, v);
+                if (k == kEmpty || k == v) {
+                    slot = (int)h;
+                    break;
+                }
+            }
+            if (slot >= 0) {
+                atomicMin(&sb[0][slot], i);
+                atomicMin(&sb[1][slot], x);
+                atomicMax(&sb[2][slot], xe);
+                atomicMin(&sb[3][slot], y);
+                atomicMax(&sb[4][slot], y);
+            } else {
+                atomicMin(&tab[v], i);
+                atomicMin(&tab[cap + v], x);
+                atomicMax(&tab[2 * cap + v], xe);
+                atomicMin(&tab[3 * cap + v], y);
+                atomicMax(&tab[4 * cap + v], y);
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < BB_SLOTS; k += 256) {
+        const int32_t v = key[k];
+        if (v == kEmpty) continue;
+        atomicMin(&tab[v], sb[0][k]);
+        atomicMin(&tab[cap + v], sb[1][k]);
+        atomicMax(&tab[2 * cap + v], sb[2][k]);
+        atomicMin(&tab[3 * cap + v], sb[3][k]);
+        atomicMax(&tab[4 * cap + v], sb[4][k]);
     }
 }
 
@@ -1829,10 +1891,11 @@ nsx_status bbox_run(nsx_ctx* ctx, const View& lab, int bgval, int32_t* boxes, in
     void *misc_p, *tab_p, *bits_p, *scan_p, *out_p;
     NSX_TRY(ctx->ws(WS_CCL_MISC, 64, &misc_p));
     int32_t* misc = (int32_t*)misc_p;
-    const dim3 grid((lab.cols + 255) / 256, lab.rows);
+    const dim3 grid((lab.cols + 63) / 64, (lab.rows + BB_ROWS - 1) / BB_ROWS);
+    const int vec = ((((uintptr_t)lab.data | (uintptr_t)lab.pitch) & 15u) == 0u) ? 1 : 0;
     hipLaunchKernelGGL(bbox_misc_init_kernel, dim3(1), dim3(64), 0, s, misc);
     hipLaunchKernelGGL(bbox_scan_kernel, dim3(std::min(lab.rows, 8 * ctx->cu_count)), dim3(256), 0, s, lab.data,
-                       lab.pitch, lab.rows, lab.cols, bgval, misc);
+                       lab.pitch, lab.rows, lab.cols, bgval, vec, misc);
     NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, misc, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
     const int32_t stop = ctx->h_pinned[40], bad = ctx->h_pinned[41], maxv = ctx->h_pinned[42];

@@ -417,6 +417,14 @@ def test_bounding_box(nh, orc, gpu):
         wn, wb = orc.bounding_box(stray, -1)
         n, b = nh.bounding_box(stray, -1)
         assert n == wn and np.array_equal(b, wb)
+    # isolated pixels: 1024 labels per 64x64 tile, more than a tile's table holds
+    dots = np.zeros((130, 200), np.uint8)
+    dots[::2, ::2] = 255
+    canon, _ = orc.bwlabel2(dots, 8, True)
+    for labels in (canon, torch.from_numpy(canon).cuda()):
+        wn, wb = orc.bounding_box(canon, 0)
+        n, b = nh.bounding_box(labels, 0)
+        assert n == wn and np.array_equal(b, wb)
     bad = np.full((4, 4), 7, np.int32)
     bad[1, 1] = 100000
     with pytest.raises(nh.NsxError):
