@@ -32,6 +32,7 @@
 #include "nsx_internal.h"
 
 #include <algorithm>
+#include <initializer_list>
 #include <cmath>
 
 namespace nsx {
@@ -1277,19 +1278,78 @@ __global__ void threshold_kernel(const char* in, int64_t pin, int dtype, int row
     ((uint8_t*)(out + (int64_t)y * pout))[x] = v > k ? 255 : 0;
 }
 
+// ---- u8 elementwise kernels: 16 pixels per thread along a row ------------------
+// `vec`: every row of every image involved starts on a 16-byte boundary (one
+// 16-byte access per row group); otherwise, and for a row's last partial
+// group, byte accesses.
+constexpr int U8V = 16;
+struct U8x16 {
+    uint32_t w[4];
+};
+__device__ __forceinline__ U8x16 u8x16_load(const char* row, int x, int cols, bool vec) {
+    U8x16 b;
+    if (vec && x + U8V <= cols) {
+        const uint4 v = *reinterpret_cast<const uint4*>(row + x);
+        b.w[0] = v.x;
+        b.w[1] = v.y;
+        b.w[2] = v.z;
+        b.w[3] = v.w;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            uint32_t w = 0u;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int xx = x + 4 * i + j;
+                if (xx < cols) w |= (uint32_t)((const uint8_t*)row)[xx] << (8 * j);
+            }
+            b.w[i] = w;
+        }
+    }
+    return b;
+}
+__device__ __forceinline__ void u8x16_store(char* row, int x, int cols, bool vec, const U8x16& b) {
+    if (vec && x + U8V <= cols) {
+        *reinterpret_cast<uint4*>(row + x) = make_uint4(b.w[0], b.w[1], b.w[2], b.w[3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int xx = x + 4 * i + j;
+                if (xx < cols) ((uint8_t*)row)[xx] = (uint8_t)(b.w[i] >> (8 * j));
+            }
+    }
+}
+inline bool u8_vec(std::initializer_list<uintptr_t> v) {
+    uintptr_t o = 0;
+    for (uintptr_t x : v) o |= x;
+    return (o & 15u) == 0u;
+}
+inline dim3 u8x16_grid(int rows, int cols) {
+    return dim3((unsigned)((cols + 256 * U8V - 1) / (256 * U8V)), (unsigned)rows);
+}
+
 // Seeded markers of imfill / bwselect (u8): marker = image where seeds != 0,
 // else 0; with `invert`, the image is first complemented (255 - x) and the
 // complement is also written to `mask` (imfill's reconstruction mask).
-__global__ void seeded_marker_kernel(const char* img, int64_t pi, const char* seeds, int64_t ps, char* marker,
-                                     int64_t pm, char* mask, int64_t pk, int rows, int cols, int invert) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+__global__ __launch_bounds__(256) void seeded_marker16_kernel(const char* img, int64_t pi, const char* seeds, int64_t ps,
+                                                              char* marker, int64_t pm, char* mask, int64_t pk,
+                                                              int cols, int invert, int vec) {
+    const int x = (blockIdx.x * 256 + (int)threadIdx.x) * U8V, y = blockIdx.y;
     if (x >= cols) return;
-    uint8_t v = ((const uint8_t*)(img + (int64_t)y * pi))[x];
-    if (invert) {
-        v = (uint8_t)(255 - v);
-        ((uint8_t*)(mask + (int64_t)y * pk))[x] = v;
+    const bool vc = vec != 0;
+    U8x16 v = u8x16_load(img + (int64_t)y * pi, x, cols, vc);
+    const U8x16 sd = u8x16_load(seeds + (int64_t)y * ps, x, cols, vc);
+    U8x16 m;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (invert) v.w[i] = ~v.w[i];
+        const uint32_t nz = (sd.w[i] | ((sd.w[i] & 0x7F7F7F7Fu) + 0x7F7F7F7Fu)) & 0x80808080u;
+        m.w[i] = v.w[i] & ((nz >> 7) * 0xFFu);
     }
-    ((uint8_t*)(marker + (int64_t)y * pm))[x] = ((const uint8_t*)(seeds + (int64_t)y * ps))[x] != 0 ? v : 0;
+    if (invert) u8x16_store(mask + (int64_t)y * pk, x, cols, vc, v);
+    u8x16_store(marker + (int64_t)y * pm, x, cols, vc, m);
 }
 
 // ---- local maxima / minima (localMaxima, localMinima) ---------------------------
@@ -1433,48 +1493,8 @@ __global__ void lx_zero_kernel(int32_t* w) {
 // value v around it) has no strictly higher neighbour: one labelling of equal
 // nonzero values, each plateau folding the largest of its pixels' "has a
 // higher neighbour" weights, replaces the reconstruction.
-// LX_V pixels per thread along a row; `vec`: every row of every image starts
-// on a 16-byte boundary (one 16-byte access per thread).
-constexpr int LX_V = 16;
+constexpr int LX_V = U8V;
 
-struct LxBytes {
-    uint32_t w[4];
-};
-__device__ __forceinline__ LxBytes lx_load16(const char* row, int x, int cols, bool vec) {
-    LxBytes b;
-    if (vec && x + LX_V <= cols) {
-        const uint4 v = *reinterpret_cast<const uint4*>(row + x);
-        b.w[0] = v.x;
-        b.w[1] = v.y;
-        b.w[2] = v.z;
-        b.w[3] = v.w;
-    } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            uint32_t w = 0u;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int xx = x + 4 * i + j;
-                if (xx < cols) w |= (uint32_t)((const uint8_t*)row)[xx] << (8 * j);
-            }
-            b.w[i] = w;
-        }
-    }
-    return b;
-}
-__device__ __forceinline__ void lx_store16(char* row, int x, int cols, bool vec, const LxBytes& b) {
-    if (vec && x + LX_V <= cols) {
-        *reinterpret_cast<uint4*>(row + x) = make_uint4(b.w[0], b.w[1], b.w[2], b.w[3]);
-    } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int xx = x + 4 * i + j;
-                if (xx < cols) ((uint8_t*)row)[xx] = (uint8_t)(b.w[i] >> (8 * j));
-            }
-    }
-}
 // per byte: 0x80 where the byte is nonzero
 __device__ __forceinline__ uint32_t lx_nz_bytes(uint32_t w) {
     return (w | ((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu)) & 0x80808080u;
@@ -1497,7 +1517,7 @@ __device__ __forceinline__ LxRow lx_row(const char* img, int64_t pi, int y, int 
         return R;
     }
     const char* row = img + (int64_t)y * pi;
-    const LxBytes b = lx_load16(row, x, cols, vec);
+    const U8x16 b = u8x16_load(row, x, cols, vec);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int left = cols - x - 4 * i;
@@ -1526,7 +1546,7 @@ __global__ __launch_bounds__(256) void lx_prep_u8_kernel(const char* img, int64_
         const LxRow up = lx_row(img, pi, y - 1, x, rows, cols, minima, vc);
         const LxRow mid = lx_row(img, pi, y, x, rows, cols, minima, vc);
         const LxRow dn = lx_row(img, pi, y + 1, x, rows, cols, minima, vc);
-        LxBytes wt, cb, fb;
+        U8x16 wt, cb, fb;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             uint32_t ww = 0u, cw = 0u, fw = 0u;
@@ -1551,9 +1571,9 @@ __global__ __launch_bounds__(256) void lx_prep_u8_kernel(const char* img, int64_
             fb.w[i] = fw;
             any = any || mid.w[i] != 0u;
         }
-        lx_store16(weight + (int64_t)y * pwt, x, cols, vc, wt);
-        lx_store16(cls + (int64_t)y * pc, x, cols, vc, cb);
-        lx_store16(fold + (int64_t)y * pf, x, cols, vc, fb);
+        u8x16_store(weight + (int64_t)y * pwt, x, cols, vc, wt);
+        u8x16_store(cls + (int64_t)y * pc, x, cols, vc, cb);
+        u8x16_store(fold + (int64_t)y * pf, x, cols, vc, fb);
     }
     if (__syncthreads_or(any) && threadIdx.x == 0 && __hip_atomic_load(nonzero, __ATOMIC_RELAXED,
                                                                         __HIP_MEMORY_SCOPE_AGENT) == 0u)
@@ -1569,9 +1589,9 @@ __global__ __launch_bounds__(256) void lx_out_u8_kernel(const char* img, int64_t
     const int x = (blockIdx.x * 256 + (int)threadIdx.x) * LX_V, y = blockIdx.y;
     if (x >= cols) return;
     const uint32_t zero_out = *nonzero ? 0u : 0xFFu;
-    const LxBytes b = lx_load16(img + (int64_t)y * pi, x, cols, vec != 0);
-    const LxBytes r = lx_load16(rec + (int64_t)y * pr, x, cols, vec != 0);
-    LxBytes o;
+    const U8x16 b = u8x16_load(img + (int64_t)y * pi, x, cols, vec != 0);
+    const U8x16 r = u8x16_load(rec + (int64_t)y * pr, x, cols, vec != 0);
+    U8x16 o;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const uint32_t w = minima ? ~b.w[i] : b.w[i];
@@ -1584,7 +1604,7 @@ __global__ __launch_bounds__(256) void lx_out_u8_kernel(const char* img, int64_t
         }
         o.w[i] = ow;
     }
-    lx_store16(out + (int64_t)y * po, x, cols, vec != 0, o);
+    u8x16_store(out + (int64_t)y * po, x, cols, vec != 0, o);
 }
 
 // ---- bounding boxes (ConnComponents::boundingBox) ------------------------------
@@ -1774,6 +1794,49 @@ __global__ void hmin_prep_kernel(const char* img, int64_t pi, char* mask, int64_
     }
 }
 
+// imhmin's u8 inputs, 16 pixels per thread.
+__global__ __launch_bounds__(256) void hmin_prep16_kernel(const char* img, int64_t pi, char* mask, int64_t pk,
+                                                          char* marker, int64_t pm, int cols, int hu, int vec) {
+    const int x = (blockIdx.x * 256 + (int)threadIdx.x) * U8V, y = blockIdx.y;
+    if (x >= cols) return;
+    const bool vc = vec != 0;
+    U8x16 v = u8x16_load(img + (int64_t)y * pi, x, cols, vc);
+    U8x16 m;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        v.w[i] = ~v.w[i];
+        uint32_t mw = 0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int b = (int)((v.w[i] >> (8 * j)) & 0xFFu);
+            mw |= (uint32_t)(b > hu ? b - hu : 0) << (8 * j);
+        }
+        m.w[i] = mw;
+    }
+    u8x16_store(mask + (int64_t)y * pk, x, cols, vc, v);
+    u8x16_store(marker + (int64_t)y * pm, x, cols, vc, m);
+}
+
+// u8 out = 255 - out in place (op 2), out = image | out (op 0) or image & out
+// (op 1); 16 pixels per thread.
+__global__ __launch_bounds__(256) void u8_op16_kernel(const char* img, int64_t pi, char* out, int64_t po, int cols,
+                                                      int op, int vec) {
+    const int x = (blockIdx.x * 256 + (int)threadIdx.x) * U8V, y = blockIdx.y;
+    if (x >= cols) return;
+    const bool vc = vec != 0;
+    char* orow = out + (int64_t)y * po;
+    U8x16 o = u8x16_load(orow, x, cols, vc);
+    if (op == 2) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o.w[i] = ~o.w[i];
+    } else {
+        const U8x16 a = u8x16_load(img + (int64_t)y * pi, x, cols, vc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o.w[i] = op == 0 ? (a.w[i] | o.w[i]) : (a.w[i] & o.w[i]);
+    }
+    u8x16_store(orow, x, cols, vc, o);
+}
+
 // out = invert(out) in place (u8 255 - x, f32 0 - x).
 __global__ void invert_kernel(char* out, int64_t po, int rows, int cols, int dtype) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
@@ -1781,15 +1844,6 @@ __global__ void invert_kernel(char* out, int64_t po, int rows, int cols, int dty
     char* row = out + (int64_t)y * po;
     if (dtype == NSX_U8) ((uint8_t*)row)[x] = (uint8_t)(255 - ((uint8_t*)row)[x]);
     else ((float*)row)[x] = 0.0f - ((float*)row)[x];
-}
-
-// out = image | out (op 0) or image & out (op 1), u8.
-__global__ void bitwise_u8_kernel(const char* img, int64_t pi, char* out, int64_t po, int rows, int cols, int op) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= cols) return;
-    const uint8_t a = ((const uint8_t*)(img + (int64_t)y * pi))[x];
-    uint8_t* o = (uint8_t*)(out + (int64_t)y * po) + x;
-    *o = op == 0 ? (uint8_t)(a | *o) : (uint8_t)(a & *o);
 }
 
 __global__ void fill_i32_kernel(char* out, int64_t pout, int rows, int cols, int32_t v) {
@@ -1865,10 +1919,12 @@ nsx_status threshold_run(nsx_ctx* ctx, const View& in, double k, const View& out
 nsx_status seeded_marker_run(nsx_ctx* ctx, const View& img, const View& seeds, const View& marker, const View* mask,
                              hipStream_t s) {
     if (img.rows == 0 || img.cols == 0) return NSX_OK;
-    const dim3 grid((img.cols + 255) / 256, img.rows);
-    hipLaunchKernelGGL(seeded_marker_kernel, grid, dim3(256), 0, s, img.data, img.pitch, seeds.data, seeds.pitch,
-                       marker.data, marker.pitch, mask ? mask->data : nullptr, mask ? mask->pitch : 0, img.rows,
-                       img.cols, mask ? 1 : 0);
+    const bool vec = u8_vec({(uintptr_t)img.data, (uintptr_t)img.pitch, (uintptr_t)seeds.data, (uintptr_t)seeds.pitch,
+                             (uintptr_t)marker.data, (uintptr_t)marker.pitch, mask ? (uintptr_t)mask->data : 0u,
+                             mask ? (uintptr_t)mask->pitch : 0u});
+    hipLaunchKernelGGL(seeded_marker16_kernel, u8x16_grid(img.rows, img.cols), dim3(256), 0, s, img.data, img.pitch,
+                       seeds.data, seeds.pitch, marker.data, marker.pitch, mask ? mask->data : nullptr,
+                       mask ? mask->pitch : 0, img.cols, mask ? 1 : 0, vec ? 1 : 0);
     NSX_HIP_TRY(ctx, hipGetLastError());
     return NSX_OK;
 }
@@ -1876,6 +1932,14 @@ nsx_status seeded_marker_run(nsx_ctx* ctx, const View& img, const View& seeds, c
 nsx_status hmin_prep_run(nsx_ctx* ctx, const View& img, double h, const View& mask, const View& marker,
                          hipStream_t s) {
     if (img.rows == 0 || img.cols == 0) return NSX_OK;
+    if (img.dtype == NSX_U8) {
+        const bool vec = u8_vec({(uintptr_t)img.data, (uintptr_t)img.pitch, (uintptr_t)mask.data,
+                                 (uintptr_t)mask.pitch, (uintptr_t)marker.data, (uintptr_t)marker.pitch});
+        hipLaunchKernelGGL(hmin_prep16_kernel, u8x16_grid(img.rows, img.cols), dim3(256), 0, s, img.data, img.pitch,
+                           mask.data, mask.pitch, marker.data, marker.pitch, img.cols, (int)h, vec ? 1 : 0);
+        NSX_HIP_TRY(ctx, hipGetLastError());
+        return NSX_OK;
+    }
     const dim3 grid((img.cols + 255) / 256, img.rows);
     hipLaunchKernelGGL(hmin_prep_kernel, grid, dim3(256), 0, s, img.data, img.pitch, mask.data, mask.pitch,
                        marker.data, marker.pitch, img.rows, img.cols, img.dtype, (int)h, (float)h);
@@ -1957,7 +2021,7 @@ nsx_status local_extrema_run(nsx_ctx* ctx, const View& img, const View& out, int
         const bool vec = ((((uintptr_t)img.data | (uintptr_t)img.pitch | (uintptr_t)work.data | (uintptr_t)pitch |
                             (uintptr_t)weight.data | (uintptr_t)fold.data | (uintptr_t)out.data |
                             (uintptr_t)out.pitch) & 15u) == 0u);
-        const dim3 vgrid((img.cols + 256 * LX_V - 1) / (256 * LX_V), img.rows);
+        const dim3 vgrid = u8x16_grid(img.rows, img.cols);
         hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, nonzero, 1);
         hipLaunchKernelGGL(lx_prep_u8_kernel, vgrid, dim3(256), 0, s, img.data, img.pitch, work.data, work.pitch,
                            weight.data, weight.pitch, fold.data, fold.pitch, img.rows, img.cols, minima ? 1 : 0, conn,
@@ -2013,6 +2077,13 @@ nsx_status local_extrema_run(nsx_ctx* ctx, const View& img, const View& out, int
 
 nsx_status invert_run(nsx_ctx* ctx, const View& out, hipStream_t s) {
     if (out.rows == 0 || out.cols == 0) return NSX_OK;
+    if (out.dtype == NSX_U8) {
+        const bool vec = u8_vec({(uintptr_t)out.data, (uintptr_t)out.pitch});
+        hipLaunchKernelGGL(u8_op16_kernel, u8x16_grid(out.rows, out.cols), dim3(256), 0, s, (const char*)out.data,
+                           out.pitch, out.data, out.pitch, out.cols, 2, vec ? 1 : 0);
+        NSX_HIP_TRY(ctx, hipGetLastError());
+        return NSX_OK;
+    }
     const dim3 grid((out.cols + 255) / 256, out.rows);
     hipLaunchKernelGGL(invert_kernel, grid, dim3(256), 0, s, out.data, out.pitch, out.rows, out.cols, out.dtype);
     NSX_HIP_TRY(ctx, hipGetLastError());
@@ -2021,9 +2092,9 @@ nsx_status invert_run(nsx_ctx* ctx, const View& out, hipStream_t s) {
 
 nsx_status bitwise_u8_run(nsx_ctx* ctx, const View& img, const View& out, bool and_op, hipStream_t s) {
     if (img.rows == 0 || img.cols == 0) return NSX_OK;
-    const dim3 grid((img.cols + 255) / 256, img.rows);
-    hipLaunchKernelGGL(bitwise_u8_kernel, grid, dim3(256), 0, s, img.data, img.pitch, out.data, out.pitch, img.rows,
-                       img.cols, and_op ? 1 : 0);
+    const bool vec = u8_vec({(uintptr_t)img.data, (uintptr_t)img.pitch, (uintptr_t)out.data, (uintptr_t)out.pitch});
+    hipLaunchKernelGGL(u8_op16_kernel, u8x16_grid(img.rows, img.cols), dim3(256), 0, s, img.data, img.pitch, out.data,
+                       out.pitch, img.cols, and_op ? 1 : 0, vec ? 1 : 0);
     NSX_HIP_TRY(ctx, hipGetLastError());
     return NSX_OK;
 }

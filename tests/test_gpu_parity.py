@@ -344,7 +344,8 @@ def test_imfill_seeded_and_bwselect(nh, orc, gpu, conn):
     arrays and device tensors, at reconstruction tile edges."""
     import torch
     rng = np.random.default_rng(71 + conn)
-    for shape in [(1, 1), (5, 70), (62, 63), (127, 130), (300, 257)]:
+    # (40, 512), (3, 1040): rows 16-byte aligned, 16 pixels per thread
+    for shape in [(1, 1), (5, 70), (62, 63), (127, 130), (300, 257), (40, 512), (3, 1040)]:
         bw = np.where(rng.random(shape) < 0.55, 255, 0).astype(np.uint8)
         g = rng.integers(0, 256, size=shape).astype(np.uint8)
         seeds = np.where(rng.random(shape) < 0.01, rng.integers(1, 256, shape), 0).astype(np.uint8)
@@ -379,7 +380,7 @@ def test_imhmin(nh, orc, gpu, conn):
     host arrays and device tensors; uchar h must be an integer in 0..255."""
     import torch
     rng = np.random.default_rng(91 + conn)
-    for shape in [(1, 1), (61, 62), (130, 257)]:
+    for shape in [(1, 1), (61, 62), (130, 257), (40, 512)]:
         u = rng.integers(0, 256, size=shape).astype(np.uint8)
         for h in (0, 9, 255):
             want = orc.imhmin(u, h, conn)
