@@ -401,79 +401,88 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
-// checkDistNeighbors8 (:33-88): a background pixel with a nonzero 8-neighbour.
-// All nine loads at clamped positions, then the tests.
-__device__ __forceinline__ bool dtq_front(const char* mask, int64_t pm, int rows, int cols, int x, int y) {
-    const uint8_t* r0 = (const uint8_t*)(mask + (int64_t)max(y - 1, 0) * pm);
-    const uint8_t* r1 = (const uint8_t*)(mask + (int64_t)y * pm);
-    const uint8_t* r2 = (const uint8_t*)(mask + (int64_t)min(y + 1, rows - 1) * pm);
-    const int xl = max(x - 1, 0), xr = min(x + 1, cols - 1);
-    const uint8_t a = r0[xl], b = r0[x], c = r0[xr], d = r1[xl], e = r1[x], f = r1[xr], g = r2[xl], h = r2[x],
-                  i = r2[xr];
-    const bool up = y > 0, dn = y < rows - 1, lf = x > 0, rt = x < cols - 1;
-    return e == 0 && ((up && (b || (lf && a) || (rt && c))) || (dn && (h || (lf && g) || (rt && i))) ||
-                      (lf && d) || (rt && f));
+// Mask bytes x-1 .. x+16 of row y (0 outside the image), for the front test of
+// a group of 16 pixels.
+struct DtqRow {
+    uint32_t w[4];
+    uint32_t l, r;
+    __device__ __forceinline__ uint32_t at(int i) const {
+        return i < 0 ? l : i >= 16 ? r : (w[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+    }
+};
+__device__ __forceinline__ DtqRow dtq_row(const char* mask, int64_t pm, int rows, int cols, int x, int y, bool vec) {
+    DtqRow R;
+    R.w[0] = R.w[1] = R.w[2] = R.w[3] = R.l = R.r = 0u;
+    if (y < 0 || y >= rows || x >= cols) return R;
+    const uint8_t* row = (const uint8_t*)(mask + (int64_t)y * pm);
+    if (vec && x + 16 <= cols) {
+        const uint4 v = *reinterpret_cast<const uint4*>(row + x);
+        R.w[0] = v.x;
+        R.w[1] = v.y;
+        R.w[2] = v.z;
+        R.w[3] = v.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (x + k < cols) R.w[k >> 2] |= (uint32_t)row[x + k] << (8 * (k & 3));
+    }
+    R.l = x > 0 ? row[x - 1] : 0u;
+    R.r = x + 16 < cols ? row[x + 16] : 0u;
+    return R;
+}
+// checkDistNeighbors8 (:33-88) for pixels x .. x+15 of row y: bit k set for a
+// background pixel with a nonzero 8-neighbour in the image.
+__device__ __forceinline__ uint32_t dtq_front16(const char* mask, int64_t pm, int rows, int cols, int x, int y,
+                                                bool vec) {
+    const DtqRow up = dtq_row(mask, pm, rows, cols, x, y - 1, vec), mid = dtq_row(mask, pm, rows, cols, x, y, vec),
+                 dn = dtq_row(mask, pm, rows, cols, x, y + 1, vec);
+    uint32_t fm = 0u;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t nb = up.at(k - 1) | up.at(k) | up.at(k + 1) | mid.at(k - 1) | mid.at(k + 1) | dn.at(k - 1) |
+                            dn.at(k) | dn.at(k + 1);
+        if (x + k < cols && mid.at(k) == 0u && nb != 0u) fm |= 1u << k;
+    }
+    return fm;
 }
 
 // Generation 0, count pass: every label initialised (a background pixel holds
 // itself, the others the far point) and per chunk (1024 columns of a row,
-// chunks in raster order) the number of front pixels.  A block takes 4
-// consecutive chunks, their loads issued together.
-__global__ __launch_bounds__(1024) void dtq_front_count_kernel(Dtq Q, const char* mask, int64_t pm, int nseg,
-                                                               int nfront) {
-    uint32_t cnt[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int ci = blockIdx.x * 4 + i, y = ci / nseg, seg = ci - y * nseg;
-        const int x = seg * QCH + (int)threadIdx.x;
-        cnt[i] = 0;
-        if (ci < nfront && x < Q.cols) {
-            const bool bg = ((const uint8_t*)(mask + (int64_t)y * pm))[x] == 0;
-            Q.S[qlin(Q, x, y)] = bg ? qpack(x, y) : DT_FAR;
-            cnt[i] = dtq_front(mask, pm, Q.rows, Q.cols, x, y) ? 1u : 0u;
-        }
-    }
-    __shared__ uint32_t sh[4][16];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const uint32_t v = wave_sum(cnt[i]);
-        if ((threadIdx.x & 63) == 0) sh[i][threadIdx.x >> 6] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 4 && blockIdx.x * 4 + (int)threadIdx.x < nfront) {
-        uint32_t t = 0;
-        for (int q = 0; q < 16; ++q) t += sh[threadIdx.x][q];
-        Q.part[blockIdx.x * 4 + threadIdx.x] = t;
+// chunks in raster order) the number of front pixels.  A wave takes a chunk,
+// 16 pixels per lane; a block takes 4 consecutive chunks.
+__global__ __launch_bounds__(256) void dtq_front_count_kernel(Dtq Q, const char* mask, int64_t pm, int nseg,
+                                                              int nfront, int vec) {
+    const int lane = threadIdx.x & 63;
+    const int ci = blockIdx.x * 4 + ((int)threadIdx.x >> 6);
+    if (ci >= nfront) return;
+    const int y = ci / nseg, x0 = (ci - y * nseg) * QCH;
+    const uint32_t fm = dtq_front16(mask, pm, Q.rows, Q.cols, x0 + 16 * lane, y, vec != 0);
+    const uint32_t t = wave_sum((uint32_t)__builtin_popcount(fm));
+    if (lane == 0) Q.part[ci] = t;
+    // labels, lane-contiguous
+    const uint8_t* row = (const uint8_t*)(mask + (int64_t)y * pm);
+#pragma unroll 4
+    for (int k = 0; k < QCH / 64; ++k) {
+        const int x = x0 + 64 * k + lane;
+        if (x < Q.cols) Q.S[qlin(Q, x, y)] = row[x] == 0 ? qpack(x, y) : DT_FAR;
     }
 }
 
 // Generation 0, scatter pass: the front pixels in raster order.
-__global__ __launch_bounds__(1024) void dtq_front_scatter_kernel(Dtq Q, const char* mask, int64_t pm, int nseg,
-                                                                 int nfront) {
+__global__ __launch_bounds__(256) void dtq_front_scatter_kernel(Dtq Q, const char* mask, int64_t pm, int nseg,
+                                                                int nfront, int vec) {
     if (Q.ctl[QC_STALL]) return;
-    bool f[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int ci = blockIdx.x * 4 + i, y = ci / nseg, seg = ci - y * nseg;
-        const int x = seg * QCH + (int)threadIdx.x;
-        f[i] = ci < nfront && x < Q.cols && dtq_front(mask, pm, Q.rows, Q.cols, x, y);
-    }
-    __shared__ uint32_t sh[4][16];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t inc[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        inc[i] = wave_incl_scan(f[i] ? 1u : 0u);
-        if (lane == 63) sh[i][w] = inc[i];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        if (!f[i]) continue;
-        uint32_t before = 0;
-        for (int q = 0; q < w; ++q) before += sh[i][q];
-        const int ci = blockIdx.x * 4 + i, y = ci / nseg, seg = ci - y * nseg;
-        Q.E[0][Q.part[ci] + before + inc[i] - 1u] = qpack(seg * QCH + (int)threadIdx.x, y);
+    const int lane = threadIdx.x & 63;
+    const int ci = blockIdx.x * 4 + ((int)threadIdx.x >> 6);
+    if (ci >= nfront) return;
+    const int y = ci / nseg, x = (ci - y * nseg) * QCH + 16 * lane;
+    uint32_t fm = dtq_front16(mask, pm, Q.rows, Q.cols, x, y, vec != 0);
+    const uint32_t cnt = (uint32_t)__builtin_popcount(fm);
+    uint32_t pos = Q.part[ci] + wave_incl_scan(cnt) - cnt;
+    while (fm) {
+        const int k = __builtin_ctz(fm);
+        fm &= fm - 1u;
+        Q.E[0][pos++] = qpack(x + k, y);
     }
 }
 
@@ -873,11 +882,12 @@ nsx_status dist_queue_run(nsx_ctx* ctx, const View& mask, const View& out, bool 
         Q.cap = (uint32_t)cap;
         NSX_HIP_TRY(ctx, hipMemsetAsync(Q.ctl, 0, sizeof(int32_t) * 64, s));
         const unsigned fblocks = (unsigned)((nfront + 3) / 4);
-        hipLaunchKernelGGL(dtq_front_count_kernel, dim3(fblocks), dim3(1024), 0, s, Q, mask.data, mask.pitch, nseg,
-                           (int)nfront);
+        const int fvec = ((((uintptr_t)mask.data | (uintptr_t)mask.pitch) & 15u) == 0u) ? 1 : 0;
+        hipLaunchKernelGGL(dtq_front_count_kernel, dim3(fblocks), dim3(256), 0, s, Q, mask.data, mask.pitch, nseg,
+                           (int)nfront, fvec);
         hipLaunchKernelGGL(dtq_scan_kernel, dim3(1), dim3(1024), 0, s, Q, -1, (int)nfront, (int)QC_LEN0);
-        hipLaunchKernelGGL(dtq_front_scatter_kernel, dim3(fblocks), dim3(1024), 0, s, Q, mask.data, mask.pitch,
-                           nseg, (int)nfront);
+        hipLaunchKernelGGL(dtq_front_scatter_kernel, dim3(fblocks), dim3(256), 0, s, Q, mask.data, mask.pitch,
+                           nseg, (int)nfront, fvec);
         NSX_HIP_TRY(ctx, hipGetLastError());
         auto generation = [&](int g, int solve, bool resumed) {
             if (!resumed) hipLaunchKernelGGL(dtq_link_kernel, dim3(QNB), dim3(256), 0, s, Q, g);
