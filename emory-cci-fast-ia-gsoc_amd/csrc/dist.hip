@@ -219,19 +219,32 @@ __global__ void dt_idx_init_kernel(uint8_t* idx, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) idx[i] = 0;
 }
 
-__global__ void dt_out_kernel(const uint32_t* buf0, const uint32_t* buf1, const uint8_t* idx, int tiles_x, char* out,
-                              int64_t po, int rows, int cols, int far_y, int calc_dist) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+// Four pixels per thread (x .. x+3, one tile since DT is a multiple of 4); with
+// `vec` (rows of labels and output 16-byte aligned) one 16-byte load and store.
+__global__ __launch_bounds__(256) void dt_out_kernel(const uint32_t* buf0, const uint32_t* buf1, const uint8_t* idx,
+                                                     int tiles_x, char* out, int64_t po, int rows, int cols, int far_y,
+                                                     int calc_dist, int vec) {
+    const int x = 4 * (blockIdx.x * blockDim.x + threadIdx.x), y = blockIdx.y;
     if (x >= cols) return;
     const uint32_t* lab = (idx && idx[(y / DT) * tiles_x + x / DT]) ? buf1 : buf0;
-    const uint32_t l = lab[(int64_t)y * cols + x];
+    const uint32_t* lrow = lab + (int64_t)y * cols;
     char* row = out + (int64_t)y * po;
-    if (calc_dist) {
-        ((float*)row)[x] = (float)sqrt((double)dt_d2(x, y, l, far_y));
+    auto value = [&](int xx, uint32_t l) -> uint32_t {
+        if (calc_dist) return __float_as_uint((float)sqrt((double)dt_d2(xx, y, l, far_y)));
+        return l == DT_FAR ? (uint32_t)(int32_t)((int64_t)rows * cols * 3)
+                           : (l >> 16) * (uint32_t)cols + (l & 0xFFFFu);
+    };
+    if (vec && x + 4 <= cols) {
+        const uint4 l = *reinterpret_cast<const uint4*>(lrow + x);
+        *reinterpret_cast<uint4*>(row + 4 * (int64_t)x) =
+            make_uint4(value(x, l.x), value(x + 1, l.y), value(x + 2, l.z), value(x + 3, l.w));
     } else {
-        ((int32_t*)row)[x] =
-            l == DT_FAR ? (int32_t)((int64_t)rows * cols * 3) : (int32_t)((l >> 16) * (uint32_t)cols + (l & 0xFFFFu));
+        for (int k = 0; k < 4 && x + k < cols; ++k) ((uint32_t*)row)[x + k] = value(x + k, lrow[x + k]);
     }
+}
+inline dim3 dt_out_grid(int rows, int cols) { return dim3((unsigned)((cols + 1023) / 1024), (unsigned)rows); }
+inline int dt_out_vec(const void* lab, int cols, const View& out) {
+    return ((((uintptr_t)lab | (uintptr_t)cols * 4u | (uintptr_t)out.data | (uintptr_t)out.pitch) & 15u) == 0u) ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -947,9 +960,9 @@ nsx_status dist_queue_run(nsx_ctx* ctx, const View& mask, const View& out, bool 
             cap = want;
             continue;
         }
-        const dim3 grid((cols + 255) / 256, rows);
-        hipLaunchKernelGGL(dt_out_kernel, grid, dim3(256), 0, s, (const uint32_t*)Q.S, (const uint32_t*)Q.S,
-                           (const uint8_t*)nullptr, 1, out.data, out.pitch, rows, cols, 3 * rows, calc_dist ? 1 : 0);
+        hipLaunchKernelGGL(dt_out_kernel, dt_out_grid(rows, cols), dim3(256), 0, s, (const uint32_t*)Q.S,
+                           (const uint32_t*)Q.S, (const uint8_t*)nullptr, 1, out.data, out.pitch, rows, cols, 3 * rows,
+                           calc_dist ? 1 : 0, dt_out_vec(Q.S, cols, out));
         NSX_HIP_TRY(ctx, hipGetLastError());
         return NSX_OK;
     }
@@ -1007,9 +1020,9 @@ nsx_status dist_run(nsx_ctx* ctx, const View& mask, const View& out, bool calc_d
         batch = std::min(2 * batch, kBatch);
     }
     // after `round` rounds the indices of parity round & 1 are current
-    hipLaunchKernelGGL(dt_out_kernel, grid, dim3(256), 0, s, (const uint32_t*)buf0, (const uint32_t*)buf1,
-                       (const uint8_t*)ix[round & 1], tiles_x, out.data, out.pitch, rows, cols, 3 * rows,
-                       calc_dist ? 1 : 0);
+    hipLaunchKernelGGL(dt_out_kernel, dt_out_grid(rows, cols), dim3(256), 0, s, (const uint32_t*)buf0,
+                       (const uint32_t*)buf1, (const uint8_t*)ix[round & 1], tiles_x, out.data, out.pitch, rows, cols,
+                       3 * rows, calc_dist ? 1 : 0, dt_out_vec(buf0, cols, out) & dt_out_vec(buf1, cols, out));
     NSX_HIP_TRY(ctx, hipGetLastError());
     return NSX_OK;
 }
