@@ -1035,11 +1035,11 @@ __global__ __launch_bounds__(256) void levels_few_u8_kernel(const char* mask, in
 #pragma unroll
     for (int j = 0; j < LV_KCAP; ++j) sv[j] = first;
     int cnt = 1;
-    bool over = false;
+    bool over = false, seen = false;   // seen: another block (or pass) set the overflow word
     lv_u16x2 ml = {0, 0}, mh = {0, 0};
     for (int y = blockIdx.x; y < rows; y += gridDim.x) {
         if (__builtin_amdgcn_readfirstlane(*(volatile uint32_t*)overflow)) {
-            over = true;
+            over = seen = true;
             break;
         }
         const uintptr_t ms = (uintptr_t)(mask + (int64_t)y * pk), me = ms + (uintptr_t)cols;
@@ -1105,7 +1105,8 @@ __global__ __launch_bounds__(256) void levels_few_u8_kernel(const char* mask, in
     for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
     if ((threadIdx.x & 63) == 0) {
         if (over) {
-            atomicOr(overflow, 1u);
+            // one atomic per block that found the overflow itself
+            if (!seen && threadIdx.x == 0) atomicOr(overflow, 1u);
         } else {
             atomicMax(&lm, m);
             for (int j = 0; j < cnt; ++j) atomicOr(&lp[sv[j] >> 5], 1u << (sv[j] & 31u));
@@ -1140,6 +1141,7 @@ __global__ void level_floor_kernel(char* out, int64_t po, int rows, int cols, in
 }
 }  // namespace
 
+constexpr int kLvSample = 32;
 nsx_status recon_levelset_u8(nsx_ctx* ctx, const View& marker, const View& mask, const View& out, int conn,
                              int max_levels, bool* used, hipStream_t s, LevelLabelReq* req) {
     *used = false;
@@ -1148,7 +1150,18 @@ nsx_status recon_levelset_u8(nsx_ctx* ctx, const View& marker, const View& mask,
     // 49: more distinct values than the level-set path takes
     uint32_t* d = (uint32_t*)ctx->d_stats + 40;
     const int grid = std::max(1, std::min(mask.rows, 4 * ctx->cu_count));
+    const uint32_t* h = (const uint32_t*)ctx->h_pinned + 40;
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, d, 10);
+    if (max_levels + 1 <= LV_KCAP && mask.rows >= 8 * kLvSample) {
+        // every kLvSample-th row first: on an image with many values (most)
+        // it sets the overflow word after a short pass, and the full pass then
+        // stops at its first row.  Otherwise the full pass adds to the same
+        // words (the sampled values are among the image's).
+        const int rs = (mask.rows - 1) / kLvSample + 1;
+        hipLaunchKernelGGL(levels_few_u8_kernel, dim3(std::min(rs, 4 * ctx->cu_count)), dim3(256), 0, s, mask.data,
+                           mask.pitch * kLvSample, marker.data, marker.pitch * kLvSample, rs, mask.cols,
+                           max_levels + 1, d, d + 8, d + 9);
+    }
     if (max_levels + 1 <= LV_KCAP)
         hipLaunchKernelGGL(levels_few_u8_kernel, dim3(grid), dim3(256), 0, s, mask.data, mask.pitch, marker.data,
                            marker.pitch, mask.rows, mask.cols, max_levels + 1, d, d + 8, d + 9);
@@ -1157,7 +1170,6 @@ nsx_status recon_levelset_u8(nsx_ctx* ctx, const View& marker, const View& mask,
                            marker.pitch, mask.rows, mask.cols, d, d + 8);
     NSX_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned + 40, d, 10 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     NSX_HIP_TRY(ctx, hipStreamSynchronize(s));
-    const uint32_t* h = (const uint32_t*)ctx->h_pinned + 40;
     if (h[9] != 0u) return NSX_OK;
     std::vector<int> levels;   // descending
     for (int v = 255; v >= 0; --v)
