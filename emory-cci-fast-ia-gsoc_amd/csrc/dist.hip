@@ -812,30 +812,77 @@ __global__ __launch_bounds__(256) void dtq_push_kernel(Dtq Q, int g) {
 
 // Generation g + 1 in push order, and each pushed pixel's last pusher.  A
 // block takes 256 entries (8 push chunks): a wave's two halves are two chunks.
+// The chunks' offsets come from their push counts directly: each block sums
+// the counts of the chunks before its own and of all chunks (the latter is the
+// next generation's length; above the buffers' capacity every block returns
+// before writing and the error and stall words are set).
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
 __global__ __launch_bounds__(256) void dtq_scatter_kernel(Dtq Q, int g) {
     if (Q.ctl[QC_STALL]) return;
     const int L = Q.ctl[g & 1];
+    if (L == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) Q.ctl[(g & 1) ^ 1] = 0;
+        return;
+    }
+    const int n = (L + QPC - 1) / QPC;
     const uint64_t tagw = (uint64_t)(Q.tag0 + (uint32_t)g) << 32;
     const uint32_t* E = Q.E[g & 1];
     uint32_t* En = Q.E[(g & 1) ^ 1];
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ uint64_t red[2][4];
+    __shared__ uint32_t coff[8];
     for (int b = blockIdx.x; b * 256 < L; b += gridDim.x) {
+        uint64_t sb = 0, st = 0;
+        for (int i = (int)threadIdx.x; i < n; i += 256) {
+            const uint32_t v = Q.part[i];
+            st += v;
+            if (i < 8 * b) sb += v;
+        }
+        sb = wave_sum64(sb);
+        st = wave_sum64(st);
+        if (lane == 0) {
+            red[0][w] = sb;
+            red[1][w] = st;
+        }
+        __syncthreads();
+        const uint64_t before = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        const uint64_t total = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        if (total > Q.cap) {
+            if (threadIdx.x == 0) {
+                Q.ctl[QC_NEED] = (int32_t)min(total, (uint64_t)INT32_MAX);
+                Q.ctl[QC_ERR] = 1;
+                Q.ctl[QC_STALL] = -1;
+            }
+            return;
+        }
+        if (b == 0 && threadIdx.x == 0) Q.ctl[(g & 1) ^ 1] = (int32_t)total;
+        if (threadIdx.x < 8) {
+            uint32_t o = (uint32_t)before;
+            for (int j = 0; j < (int)threadIdx.x && 8 * b + j < n; ++j) o += Q.part[8 * b + j];
+            coff[threadIdx.x] = o;
+        }
+        __syncthreads();
         const int k = b * 256 + (int)threadIdx.x;
         const uint32_t m = k < L ? Q.mask[k] : 0u;
         const uint32_t cnt = (uint32_t)__builtin_popcount(m);
         const uint32_t inc = wave_incl_scan(cnt);
         const uint32_t half = __shfl(inc, 31, 64);
         const uint32_t ex = inc - cnt - (lane >= 32 ? half : 0u);
-        if (!m) continue;
-        const uint32_t p = E[k];
-        const int x = (int)(p & 0xFFFFu), y = (int)(p >> 16);
-        uint32_t pos = Q.part[k / QPC] + ex;
-        for (int o = 0; o < 8; ++o) {
-            if (!((m >> o) & 1u)) continue;
-            const int nx = x + kQdx[o], ny = y + kQdy[o];
-            En[pos++] = qpack(nx, ny);
-            atomicMax((unsigned long long*)&Q.win[qlin(Q, nx, ny)], (unsigned long long)(tagw | (uint32_t)k));
+        if (m) {
+            const uint32_t p = E[k];
+            const int x = (int)(p & 0xFFFFu), y = (int)(p >> 16);
+            uint32_t pos = coff[threadIdx.x >> 5] + ex;
+            for (int o = 0; o < 8; ++o) {
+                if (!((m >> o) & 1u)) continue;
+                const int nx = x + kQdx[o], ny = y + kQdy[o];
+                En[pos++] = qpack(nx, ny);
+                atomicMax((unsigned long long*)&Q.win[qlin(Q, nx, ny)], (unsigned long long)(tagw | (uint32_t)k));
+            }
         }
+        __syncthreads();   // red and coff are reused by the next 256 entries
     }
 }
 
@@ -908,7 +955,6 @@ nsx_status dist_queue_run(nsx_ctx* ctx, const View& mask, const View& out, bool 
                 for (int k = 0; k < solve; ++k)
                     hipLaunchKernelGGL(dtq_solve_kernel, dim3(QNB), dim3(QCH), 0, s, Q, g, k);
             hipLaunchKernelGGL(dtq_push_kernel, dim3(QNP), dim3(256), 0, s, Q, g);
-            hipLaunchKernelGGL(dtq_scan_kernel, dim3(1), dim3(1024), 0, s, Q, g & 1, 0, (g & 1) ^ 1);
             hipLaunchKernelGGL(dtq_scatter_kernel, dim3(QNP), dim3(256), 0, s, Q, g);
         };
         // Every generation moves some label strictly closer, so the number of
