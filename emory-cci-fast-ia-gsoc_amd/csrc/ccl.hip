@@ -1670,9 +1670,12 @@ __global__ __launch_bounds__(256) void bbox_scan_kernel(const char* lab, int64_t
             bad = min(bad, red[1][k]);
             mx = max(mx, red[2][k]);
         }
-        if (stop != INT_MAX) atomicMin(&misc[0], stop);
-        if (bad != INT_MAX) atomicMin(&misc[1], bad);
-        if (mx > 0) atomicMax(&misc[2], mx);
+        // the words only move one way: a block that cannot move one skips its
+        // atomic (every block has a largest label, so most would contend)
+        volatile int32_t* vm = misc;
+        if (stop != INT_MAX && stop < vm[0]) atomicMin(&misc[0], stop);
+        if (bad != INT_MAX && bad < vm[1]) atomicMin(&misc[1], bad);
+        if (mx > 0 && mx > vm[2]) atomicMax(&misc[2], mx);
     }
 }
 
