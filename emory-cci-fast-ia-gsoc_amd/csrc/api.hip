@@ -640,14 +640,13 @@ nsx_status nsx_bwselect(nsx_ctx* ctx, const nsx_img* binary_image, const nsx_img
     NSX_TRY(seeded_setup(ctx, binary_image, seeds, out, "bwselect"));
     if (is_empty(binary_image)) return NSX_OK;
     hipStream_t s = pick_stream(ctx, stream);
-    View vi, vs, vo, vm;
+    View vi, vs, vo;
     NSX_TRY(to_device(ctx, binary_image, WS_STAGE_B, true, s, &vi));
     NSX_TRY(to_device(ctx, seeds, WS_STAGE_A, true, s, &vs));
     NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
-    NSX_TRY(scratch_like(ctx, WS_CHAIN_RECON, vi, &vm));
-    NSX_TRY(seeded_marker_run(ctx, vi, vs, vm, nullptr, s));
-    NSX_TRY(recon_run(ctx, RM_BINARY, &vm, vi, vo, conn == 8 ? 8 : 4, s));
-    NSX_TRY(bitwise_u8_run(ctx, vi, vo, true, s));
+    // the reconstruction of the seeded marker reaches exactly the components
+    // of the nonzero pixels that hold a seed: one labelling finds them
+    NSX_TRY(bwselect_run(ctx, vi, vs, vo, conn == 8 ? 8 : 4, s));
     return finish_recon(ctx, vo, out, s);
 }
 

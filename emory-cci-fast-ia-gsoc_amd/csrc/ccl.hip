@@ -1832,8 +1832,34 @@ __global__ __launch_bounds__(256) void hmin_prep16_kernel(const char* img, int64
     u8x16_store(marker + (int64_t)y * pm, x, cols, vc, m);
 }
 
-// u8 out = 255 - out in place (op 2), out = image | out (op 0) or image & out
-// (op 1); 16 pixels per thread.
+// bwselect through a labelling: classes = 1 on the nonzero pixels, weight = 1
+// where a nonzero pixel holds a nonzero seed, the fold plane zeroed; 16 pixels
+// per thread.
+__global__ __launch_bounds__(256) void bwsel_prep16_kernel(const char* img, int64_t pi, const char* seeds, int64_t ps,
+                                                           char* cls, int64_t pc, char* wgt, int64_t pw, char* fold,
+                                                           int64_t pf, int cols, int vec) {
+    const int x = (blockIdx.x * 256 + (int)threadIdx.x) * U8V, y = blockIdx.y;
+    if (x >= cols) return;
+    const bool vc = vec != 0;
+    const U8x16 a = u8x16_load(img + (int64_t)y * pi, x, cols, vc);
+    const U8x16 sd = u8x16_load(seeds + (int64_t)y * ps, x, cols, vc);
+    U8x16 c, w;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t na = ((a.w[i] | ((a.w[i] & 0x7F7F7F7Fu) + 0x7F7F7F7Fu)) & 0x80808080u) >> 7;
+        const uint32_t ns = ((sd.w[i] | ((sd.w[i] & 0x7F7F7F7Fu) + 0x7F7F7F7Fu)) & 0x80808080u) >> 7;
+        c.w[i] = na;
+        w.w[i] = na & ns;
+    }
+    u8x16_store(cls + (int64_t)y * pc, x, cols, vc, c);
+    u8x16_store(wgt + (int64_t)y * pw, x, cols, vc, w);
+    const U8x16 z{{0u, 0u, 0u, 0u}};
+    u8x16_store(fold + (int64_t)y * pf, x, cols, vc, z);
+}
+
+// u8 out = 255 - out in place (op 2), out = image | out (op 0), image & out
+// (op 1), or the image where out is nonzero and 0 elsewhere (op 3); 16 pixels
+// per thread.
 __global__ __launch_bounds__(256) void u8_op16_kernel(const char* img, int64_t pi, char* out, int64_t po, int cols,
                                                       int op, int vec) {
     const int x = (blockIdx.x * 256 + (int)threadIdx.x) * U8V, y = blockIdx.y;
@@ -1847,7 +1873,10 @@ __global__ __launch_bounds__(256) void u8_op16_kernel(const char* img, int64_t p
     } else {
         const U8x16 a = u8x16_load(img + (int64_t)y * pi, x, cols, vc);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) o.w[i] = op == 0 ? (a.w[i] | o.w[i]) : (a.w[i] & o.w[i]);
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t nz = ((o.w[i] | ((o.w[i] & 0x7F7F7F7Fu) + 0x7F7F7F7Fu)) & 0x80808080u) >> 7;
+            o.w[i] = op == 0 ? (a.w[i] | o.w[i]) : op == 1 ? (a.w[i] & o.w[i]) : (a.w[i] & (nz * 0xFFu));
+        }
     }
     u8x16_store(orow, x, cols, vc, o);
 }
@@ -2110,6 +2139,28 @@ nsx_status bitwise_u8_run(nsx_ctx* ctx, const View& img, const View& out, bool a
     const bool vec = u8_vec({(uintptr_t)img.data, (uintptr_t)img.pitch, (uintptr_t)out.data, (uintptr_t)out.pitch});
     hipLaunchKernelGGL(u8_op16_kernel, u8x16_grid(img.rows, img.cols), dim3(256), 0, s, img.data, img.pitch, out.data,
                        out.pitch, img.cols, and_op ? 1 : 0, vec ? 1 : 0);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    return NSX_OK;
+}
+
+nsx_status bwselect_run(nsx_ctx* ctx, const View& img, const View& seeds, const View& out, int conn, hipStream_t s) {
+    if (img.rows == 0 || img.cols == 0) return NSX_OK;
+    const size_t plane = (size_t)img.rows * img.cols;
+    void *cls_p, *wgt_p;
+    NSX_TRY(ctx->ws(WS_CHAIN_RECON, plane, &cls_p));
+    NSX_TRY(ctx->ws(WS_CHAIN_FILL, plane, &wgt_p));
+    const View cls{(char*)cls_p, (int64_t)img.cols, img.rows, img.cols, NSX_U8};
+    const View wgt{(char*)wgt_p, (int64_t)img.cols, img.rows, img.cols, NSX_U8};
+    const bool vec = u8_vec({(uintptr_t)img.data, (uintptr_t)img.pitch, (uintptr_t)seeds.data, (uintptr_t)seeds.pitch,
+                             (uintptr_t)cls.data, (uintptr_t)wgt.data, (uintptr_t)img.cols, (uintptr_t)out.data,
+                             (uintptr_t)out.pitch});
+    const dim3 grid = u8x16_grid(img.rows, img.cols);
+    hipLaunchKernelGGL(bwsel_prep16_kernel, grid, dim3(256), 0, s, img.data, img.pitch, seeds.data, seeds.pitch,
+                       cls.data, cls.pitch, wgt.data, wgt.pitch, out.data, out.pitch, img.cols, vec ? 1 : 0);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    NSX_TRY(ccl_class_fold_run(ctx, cls, wgt, out, conn, s));
+    hipLaunchKernelGGL(u8_op16_kernel, grid, dim3(256), 0, s, img.data, img.pitch, out.data, out.pitch, img.cols, 3,
+                       vec ? 1 : 0);
     NSX_HIP_TRY(ctx, hipGetLastError());
     return NSX_OK;
 }

@@ -384,6 +384,11 @@ nsx_status ccl_launch(nsx_ctx* ctx, const CclBatch& B, const CclParams& p, hipSt
 nsx_status ccl_class_fold_run(nsx_ctx* ctx, const View& classes, const View& weight, const View& out, int conn,
                               hipStream_t s);
 
+// bwselect (u8): `out` = the image on the conn-connected components of its
+// nonzero pixels that hold a nonzero seed, 0 elsewhere; one labelling.
+// Enqueues work only.
+nsx_status bwselect_run(nsx_ctx* ctx, const View& img, const View& seeds, const View& out, int conn, hipStream_t s);
+
 // Area filter on int32 root labels (background -1), out int32 or u8.
 nsx_status ccl_area_labeled(nsx_ctx* ctx, const View& in, const View& out, int min_size, int max_size,
                             bool flatten, int32_t* count_dev, hipStream_t s);

@@ -469,6 +469,26 @@ def test_local_extrema_u8_without_flood(orc, conn):
             assert np.array_equal(want, _u8_extrema_plateaus(inv, conn))
 
 
+@pytest.mark.parametrize("conn", [4, 8])
+def test_bwselect_as_components(orc, conn):
+    """bwselect keeps, with their values, the conn-connected components of the
+    nonzero pixels that hold a pixel with a nonzero seed: the reconstruction of
+    the seeded marker reaches exactly those.  The GPU's bwselect labels the
+    components once on this ground; here it is checked against the oracle's
+    reconstruction, on 0/255 and many-valued images."""
+    from scipy import ndimage
+    rng = np.random.default_rng(47 + conn)
+    for shape in [(1, 1), (1, 9), (9, 1), (17, 23), (40, 61)]:
+        for p in (0.3, 0.6):
+            for img in (np.where(rng.random(shape) < p, 255, 0).astype(np.uint8),
+                        np.where(rng.random(shape) < p, rng.integers(1, 256, size=shape), 0).astype(np.uint8)):
+                seeds = np.where(rng.random(shape) < 0.05, rng.integers(1, 256, size=shape), 0).astype(np.uint8)
+                lab, _ = ndimage.label(img != 0, structure=brute.structure(conn))
+                hit = np.unique(lab[(seeds != 0) & (img != 0)])
+                want = np.where(np.isin(lab, hit[hit > 0]), img, 0).astype(np.uint8)
+                assert np.array_equal(orc.bwselect(img, seeds, conn), want), (shape, p)
+
+
 def _dt_masks(rng):
     from scipy import ndimage
     out = []
