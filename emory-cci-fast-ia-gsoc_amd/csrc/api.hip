@@ -504,7 +504,9 @@ nsx_status nsx_imreconstruct_binary(nsx_ctx* ctx, const nsx_img* marker, const n
     NSX_TRY(to_device(ctx, marker, WS_STAGE_A, true, s, &vm));
     NSX_TRY(to_device(ctx, mask, WS_STAGE_B, true, s, &vk));
     NSX_TRY(to_device(ctx, out, WS_STAGE_C, false, s, &vo));
-    NSX_TRY(recon_run(ctx, RM_BINARY, &vm, vk, vo, conn == 8 ? 8 : 4, s));
+    // the reached pixels are the components of the nonzero mask pixels that
+    // hold a nonzero marker pixel: one labelling finds them
+    NSX_TRY(binrecon_run(ctx, vm, vk, vo, conn == 8 ? 8 : 4, s));
     return finish_recon(ctx, vo, out, s);
 }
 

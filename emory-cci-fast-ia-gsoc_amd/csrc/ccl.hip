@@ -2143,6 +2143,73 @@ nsx_status bitwise_u8_run(nsx_ctx* ctx, const View& img, const View& out, bool a
     return NSX_OK;
 }
 
+// imreconstructBinary through a labelling: classes = 1 on the nonzero mask
+// pixels, weight = 1 where such a pixel has a nonzero marker, the fold plane
+// zeroed; then out = the marker where it is nonzero, the type's maximum at the
+// other pixels whose component took a weight, 0 elsewhere.  Four pixels per
+// thread along a row.
+template <typename T>
+__global__ __launch_bounds__(256) void binrec_prep_kernel(const char* marker, int64_t pm, const char* mask, int64_t pk,
+                                                          uint8_t* cls, uint8_t* wgt, uint8_t* fold, int cols) {
+    const int x0 = 4 * (blockIdx.x * 256 + (int)threadIdx.x), y = blockIdx.y;
+    const T* a = (const T*)(marker + (int64_t)y * pm);
+    const T* m = (const T*)(mask + (int64_t)y * pk);
+    const int64_t row = (int64_t)y * cols;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int x = x0 + k;
+        if (x >= cols) break;
+        const bool in = m[x] != T(0);
+        cls[row + x] = in ? 1 : 0;
+        wgt[row + x] = (in && a[x] != T(0)) ? 1 : 0;
+        fold[row + x] = 0;
+    }
+}
+template <typename T>
+__global__ __launch_bounds__(256) void binrec_out_kernel(const char* marker, int64_t pm, const uint8_t* fold, char* out,
+                                                         int64_t po, int cols, T top) {
+    const int x0 = 4 * (blockIdx.x * 256 + (int)threadIdx.x), y = blockIdx.y;
+    const T* a = (const T*)(marker + (int64_t)y * pm);
+    T* o = (T*)(out + (int64_t)y * po);
+    const int64_t row = (int64_t)y * cols;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int x = x0 + k;
+        if (x >= cols) break;
+        const T av = a[x];
+        o[x] = av != T(0) ? av : (fold[row + x] ? top : T(0));
+    }
+}
+
+nsx_status binrecon_run(nsx_ctx* ctx, const View& marker, const View& mask, const View& out, int conn, hipStream_t s) {
+    if (mask.rows == 0 || mask.cols == 0) return NSX_OK;
+    const size_t plane = (size_t)mask.rows * mask.cols;
+    void *cls_p, *wgt_p, *fold_p;
+    NSX_TRY(ctx->ws(WS_CHAIN_RECON, plane, &cls_p));
+    NSX_TRY(ctx->ws(WS_CHAIN_FILL, plane, &wgt_p));
+    NSX_TRY(ctx->ws(WS_BP_L, plane, &fold_p));
+    const View cls{(char*)cls_p, (int64_t)mask.cols, mask.rows, mask.cols, NSX_U8};
+    const View wgt{(char*)wgt_p, (int64_t)mask.cols, mask.rows, mask.cols, NSX_U8};
+    const View fold{(char*)fold_p, (int64_t)mask.cols, mask.rows, mask.cols, NSX_U8};
+    const dim3 grid((unsigned)((mask.cols + 1023) / 1024), (unsigned)mask.rows);
+    if (mask.dtype == NSX_U8)
+        hipLaunchKernelGGL(binrec_prep_kernel<uint8_t>, grid, dim3(256), 0, s, marker.data, marker.pitch, mask.data,
+                           mask.pitch, (uint8_t*)cls_p, (uint8_t*)wgt_p, (uint8_t*)fold_p, mask.cols);
+    else
+        hipLaunchKernelGGL(binrec_prep_kernel<int32_t>, grid, dim3(256), 0, s, marker.data, marker.pitch, mask.data,
+                           mask.pitch, (uint8_t*)cls_p, (uint8_t*)wgt_p, (uint8_t*)fold_p, mask.cols);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    NSX_TRY(ccl_class_fold_run(ctx, cls, wgt, fold, conn, s));
+    if (mask.dtype == NSX_U8)
+        hipLaunchKernelGGL(binrec_out_kernel<uint8_t>, grid, dim3(256), 0, s, marker.data, marker.pitch,
+                           (const uint8_t*)fold_p, out.data, out.pitch, mask.cols, (uint8_t)255);
+    else
+        hipLaunchKernelGGL(binrec_out_kernel<int32_t>, grid, dim3(256), 0, s, marker.data, marker.pitch,
+                           (const uint8_t*)fold_p, out.data, out.pitch, mask.cols, (int32_t)INT32_MAX);
+    NSX_HIP_TRY(ctx, hipGetLastError());
+    return NSX_OK;
+}
+
 nsx_status bwselect_run(nsx_ctx* ctx, const View& img, const View& seeds, const View& out, int conn, hipStream_t s) {
     if (img.rows == 0 || img.cols == 0) return NSX_OK;
     const size_t plane = (size_t)img.rows * img.cols;

@@ -384,6 +384,12 @@ nsx_status ccl_launch(nsx_ctx* ctx, const CclBatch& B, const CclParams& p, hipSt
 nsx_status ccl_class_fold_run(nsx_ctx* ctx, const View& classes, const View& weight, const View& out, int conn,
                               hipStream_t s);
 
+// imreconstructBinary (u8, int32): the marker where it is nonzero, the type's
+// maximum at the other pixels of the conn-connected components of the nonzero
+// mask pixels that hold a nonzero marker pixel, 0 elsewhere; one labelling.
+// Enqueues work only.
+nsx_status binrecon_run(nsx_ctx* ctx, const View& marker, const View& mask, const View& out, int conn, hipStream_t s);
+
 // bwselect (u8): `out` = the image on the conn-connected components of its
 // nonzero pixels that hold a nonzero seed, 0 elsewhere; one labelling.
 // Enqueues work only.

@@ -75,16 +75,21 @@ def test_drain_recon_i32_f32(nh, orc, dctx, conn):
         assert np.array_equal(got, orc.imreconstruct(kf, mf, conn))
 
 
-@pytest.mark.parametrize("dtype", [np.uint8, np.int32])
 @pytest.mark.parametrize("conn", [4, 8])
-def test_drain_binary(nh, orc, dctx, dtype, conn):
+def test_drain_binary(nh, orc, dctx, conn):
+    """Binary reconstruction rounds under the drain: the binary seeded imfill
+    (imreconstructBinary itself is one labelling and runs no rounds)."""
     rng = np.random.default_rng(50 + conn)
     for shape in [(62, 63), (150, 131), (400, 90)]:
-        mask = ((rng.random(shape) < 0.55) * rng.integers(1, 4, size=shape)).astype(dtype)
-        marker = ((rng.random(shape) < 0.03) * rng.integers(1, 9, size=shape)).astype(dtype)
-        got, n = drained(nh, dctx, lambda: nh.imreconstructBinary(marker, mask, conn, ctx=dctx))
+        img = np.where(rng.random(shape) < 0.45, 255, 0).astype(np.uint8)
+        seeds = np.where(rng.random(shape) < 0.03, rng.integers(1, 9, size=shape), 0).astype(np.uint8)
+        got, n = drained(nh, dctx, lambda: nh.imfill(img, seeds, True, conn, ctx=dctx))
         assert n == 1
-        assert np.array_equal(got, orc.imreconstruct_binary(marker, mask, conn))
+        assert np.array_equal(got, orc.imfill(img, seeds, True, conn))
+        mask = ((rng.random(shape) < 0.55) * rng.integers(1, 4, size=shape)).astype(np.int32)
+        marker = ((rng.random(shape) < 0.03) * rng.integers(1, 9, size=shape)).astype(np.int32)
+        assert np.array_equal(nh.imreconstructBinary(marker, mask, conn, ctx=dctx),
+                              orc.imreconstruct_binary(marker, mask, conn))
 
 
 @pytest.mark.parametrize("conn", [4, 8])
